@@ -1,0 +1,148 @@
+/*
+ * oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C FP64 restatement of the reference's CPU path for the finite-volume pressure
+ * assembly and solve. It is the checker the parity tests, smoke() and bench.py's CPU-baseline
+ * leg compare the HIP library against; nothing in the product links, loads or calls it.
+ * Every function names the reference file:line it restates (paths relative to the
+ * reference root).
+ *
+ * Conventions: int32 labels, double scalars, structure-of-arrays, vectors as xyz triples
+ * (3*n doubles). Internal faces are in OpenFOAM's upper-triangular order (sorted by owner,
+ * owner < neighbour). A "folded" system has the boundary's internalCoeffs in diag and its
+ * boundaryCoeffs in the source.
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   geometry + laplacian ........ validation/matrixDump/matrix.dat (OpenFOAM dump, 1e-12)
+ *   amul ........................ validation/matrixDump/amul.dat (1e-12) + dense brute force
+ *   pcg_dic ..................... validation/matrixDump/solution.dat (224 iterations exactly)
+ *   of agglomeration ............ demo/amgpcg/case20/trace/of_hierarchy.csv (bit-exact maps)
+ *   gamg_solve (OpenFOAM GAMG) .. validation/matrixDumpGAMG/solution.dat (101 iterations)
+ *   brae agglomeration .......... demo/amgpcg/case20/trace/brae_hierarchy.csv (bit-exact maps)
+ *   brae galerkin ............... demo/amgpcg/case20/trace/brae_levels.csv
+ *   brae amg-pcg ................ demo/amgpcg/case20/trace/brae_cycles.csv (per-iteration psi)
+ *   div upwind .................. validation/matrixDump/matrix_div.dat
+ *   fold / setReference / flux .. closed forms and definitions (tests state them)
+ */
+#ifndef MFV_ORACLE_H
+#define MFV_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- geometry: src/finiteVolume/fvMesh/fv_geometry.cu:13-134 (OpenFOAM primitiveMeshTools) ---- */
+int orc_geometry(int nPoints, const double* points, int nFaces, const int* faceOffsets, const int* faceVerts,
+                 int nCells, int nInternal, const int* owner, const int* neighbour,
+                 double* Cf, double* Sf, double* magSf, double* C, double* V,
+                 double* weights, double* deltaCoeffs, double* nonOrthDeltaCoeffs, double* corrVec);
+
+/* boundary-face deltaCoeffs 1/|nHat.(Cf - C)|: src/finiteVolume/fvMesh/fv_patch.cu:144-166 */
+void orc_patch_delta(int nBFaces, const int* bFaceCell, const double* bCf, const double* bSf,
+                     const double* bMagSf, const double* C, double* bDeltaCoeffs);
+
+/* ---- lduAddressing: src/matrices/lduMatrix/lduMatrix/device_ldu.cuh:109-146 ---- */
+int orc_ldu_addressing(int nCells, int nFaces, const int* owner, const int* neighbour,
+                       int* ownerStart, int* losortStart, int* losort);
+
+/* ---- fvm::laplacian interior: src/finiteVolume/finiteVolume/fvm.cuh:113-151 ---- */
+void orc_laplacian(int nCells, int nFaces, const int* owner, const int* neighbour,
+                   const double* dc, const double* gammaf, const double* magSf,
+                   double* diag, double* upper, double* lower);
+
+/* Boundary coefficients of a patch field, per boundary face (bcType: 0 zeroGradient, 1 fixedValue,
+ * 2 empty -> no contribution). fvm.cuh:43-103 (laplacian) and fvm.cuh:556-619 (div). */
+enum { ORC_BC_ZERO_GRADIENT = 0, ORC_BC_FIXED_VALUE = 1, ORC_BC_EMPTY = 2 };
+void orc_laplacian_bcoeffs(int nBFaces, const int* bcType, const double* bcValue, const double* gammab,
+                           const double* bMagSf, const double* bDeltaCoeffs, double* iC, double* bC);
+
+/* ---- fvm::div(phi, psi) upwind interior + boundary: fvm.cuh:476-554, 556-619 ---- */
+void orc_div_upwind(int nCells, int nFaces, const int* owner, const int* neighbour, const double* phi,
+                    double* diag, double* upper, double* lower);
+void orc_div_bcoeffs(int nBFaces, const int* bcType, const double* bcValue, const double* phib,
+                     double* iC, double* bC);
+
+/* ---- fold + setReference: gamg_solver_cpp.cu:769-797; device_simple.cuh:104-106 ---- */
+void orc_fold(int nCells, const double* diag, const double* source, int nBFaces, const int* bFaceCell,
+              const int* bcType, const double* iC, const double* bC, double* diagC, double* b);
+void orc_set_reference(double* diag, double* b, int refCell, double refValue);
+
+/* ---- LDU products and reductions: ldu_spmv.cu:5-24; gamg_solver_cpp.cu:62-159 ---- */
+void orc_amul(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+              const double* upper, const double* lower, const double* psi, double* Apsi);
+void orc_residual(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                  const double* upper, const double* lower, const double* psi, const double* source, double* r);
+double orc_norm_factor(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                       const double* upper, const double* lower, const double* psi, const double* source);
+double orc_sum_mag(int n, const double* x);
+double orc_sum_prod(int n, const double* a, const double* b);
+
+/* flux = upper*psi[nei] - lower*psi[own]: pEqn_cpp.cuh:134-139, device_simple.cuh:51-52 */
+void orc_face_flux(int nFaces, const int* owner, const int* neighbour, const double* upper,
+                   const double* lower, const double* psi, double* flux);
+
+typedef struct { double initialResidual, finalResidual; int nIterations; } orc_perf;
+
+/* ---- PCG with DIC on a folded symmetric system: src/OpenFOAM/matrices/pcg.cu:6-149 ---- */
+orc_perf orc_pcg_dic(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                     const double* upper, const double* b, double* psi, double tol, double relTol,
+                     int maxIter, int minIter);
+
+/* ---- agglomeration hierarchies ---- */
+typedef struct orc_aggl orc_aggl;
+/* OpenFOAM faceAreaPair weights |Sf/sqrt(magSf) * (1,1.01,1.02)|: pair_gamg_agglomeration_cpp.cu:301-311 */
+void orc_faceareapair_weights(int nFaces, const double* Sf, const double* magSf, double* w);
+/* OpenFOAM faceAreaPair hierarchy: pair_gamg_agglomeration_cpp.cu:28-345. *forward is the static flag. */
+orc_aggl* orc_of_agglomeration(int nCells, int nFaces, const int* owner, const int* neighbour,
+                               const double* weights, int nCellsInCoarsestLevel, int* forward);
+/* brae's device-path hierarchy: device_amg.cu:444-685 (agglomerate, coarseFacesByBucket), :1346-1431
+ * (buildAMG, injection branch, MERGE 1). */
+orc_aggl* orc_brae_agglomeration(int nCells, int nFaces, const int* owner, const int* neighbour,
+                                 const double* weights, int target);
+int orc_aggl_nlevels(const orc_aggl* a);            /* number of coarse levels */
+/* level i describes the step from grid i to grid i+1 (grid 0 = the mesh) */
+void orc_aggl_sizes(const orc_aggl* a, int i, int* nFineCells, int* nFineFaces, int* nCoarseCells, int* nCoarseFaces);
+void orc_aggl_level(const orc_aggl* a, int i, int* restrictMap, int* coarseOwner, int* coarseNeighbour,
+                    int* faceRestrict, signed char* faceFlip);
+void orc_aggl_free(orc_aggl* a);
+
+/* ---- Galerkin coarse matrices (symmetric): gamg_solver_cpp.cu:474-526 (OpenFOAM order);
+ *      device_amg.cu:156-223 (brae gather order). Levels are written back to back. ---- */
+void orc_galerkin_of(const orc_aggl* a, const double* fineDiag, const double* fineUpper,
+                     double* coarseDiag, double* coarseUpper);
+void orc_galerkin_brae(const orc_aggl* a, const double* fineDiag, const double* fineUpper,
+                       double* coarseDiag, double* coarseUpper);
+
+/* ---- OpenFOAM GAMG: gamg_solver_cpp.cu:627-735 (vCycle), :979-1023 (solve), :1027-1143 (PCG+GAMG) ---- */
+typedef struct {
+    int smoother;             /* 0 GaussSeidel, 1 symGaussSeidel, 2 DIC, 3 DICGaussSeidel */
+    double tolerance, relTol; /* the GAMG's own (coarsest PCG uses them) */
+    int maxIter, minIter;
+    int nPreSweeps, preSweepsLevelMultiplier, maxPreSweeps;
+    int nPostSweeps, postSweepsLevelMultiplier, maxPostSweeps;
+    int nFinestSweeps;
+    int scaleCorrection;
+    int nVcycles;             /* GAMG as a preconditioner */
+} orc_gamg_controls;
+void orc_gamg_default_controls(orc_gamg_controls* c);
+orc_perf orc_gamg_solve(const orc_aggl* a, const double* diag, const double* upper, const double* b,
+                        double* psi, const orc_gamg_controls* c);
+orc_perf orc_pcg_gamg_solve(const orc_aggl* a, const double* diag, const double* upper, const double* b,
+                            double* psi, double tol, double relTol, int maxIter, int minIter,
+                            const orc_gamg_controls* c);
+
+/* ---- brae AMG-PCG (device path semantics, FP64 arithmetic here):
+ *      V(nPre,nPost) weighted Jacobi (omega 0.8) from zero, injection, direct coarsest:
+ *      device_amg_vcycle.cu:982-1143 / 1658-1766, amg_kernels.cuh:27-53,257-282;
+ *      PCG recurrence device_amg_pcg.cu:60-170 (Fletcher-Reeves) or flexBetaK :37-49 (flexible).
+ *      normFactorIn > 0 replaces OpenFOAM's normFactor (the demo trace passes sum|b| + 1e-20).
+ *      psiTrace (nullable, (maxIter+1)*nCells) receives psi after each iteration. ---- */
+orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* upper, const double* b,
+                         double* psi, double tol, double relTol, int maxIter, int minIter,
+                         int nPre, int nPost, int flexible, int single, double normFactorIn,
+                         double* psiTrace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,286 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) for the tests, smoke() and bench.py's CPU
+baseline. Test infrastructure: the product never imports this module."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_LIB = None
+
+
+class Perf(ctypes.Structure):
+    _fields_ = [("initialResidual", ctypes.c_double), ("finalResidual", ctypes.c_double),
+                ("nIterations", ctypes.c_int)]
+
+
+class GamgControls(ctypes.Structure):
+    _fields_ = [("smoother", ctypes.c_int), ("tolerance", ctypes.c_double), ("relTol", ctypes.c_double),
+                ("maxIter", ctypes.c_int), ("minIter", ctypes.c_int),
+                ("nPreSweeps", ctypes.c_int), ("preSweepsLevelMultiplier", ctypes.c_int),
+                ("maxPreSweeps", ctypes.c_int), ("nPostSweeps", ctypes.c_int),
+                ("postSweepsLevelMultiplier", ctypes.c_int), ("maxPostSweeps", ctypes.c_int),
+                ("nFinestSweeps", ctypes.c_int), ("scaleCorrection", ctypes.c_int), ("nVcycles", ctypes.c_int)]
+
+
+SMOOTHERS = {"GaussSeidel": 0, "symGaussSeidel": 1, "DIC": 2, "DICGaussSeidel": 3}
+BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_EMPTY = 0, 1, 2
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        I, D = ctypes.c_int, ctypes.c_double
+        sig = {
+            "orc_geometry": (I, [I, P, I, P, P, I, I, P, P] + [P] * 9),
+            "orc_patch_delta": (None, [I, P, P, P, P, P, P]),
+            "orc_ldu_addressing": (I, [I, I, P, P, P, P, P]),
+            "orc_laplacian": (None, [I, I, P, P, P, P, P, P, P, P]),
+            "orc_laplacian_bcoeffs": (None, [I, P, P, P, P, P, P, P]),
+            "orc_div_upwind": (None, [I, I, P, P, P, P, P, P]),
+            "orc_div_bcoeffs": (None, [I, P, P, P, P, P]),
+            "orc_fold": (None, [I, P, P, I, P, P, P, P, P, P]),
+            "orc_set_reference": (None, [P, P, I, D]),
+            "orc_amul": (None, [I, I, P, P, P, P, P, P, P]),
+            "orc_residual": (None, [I, I, P, P, P, P, P, P, P, P]),
+            "orc_norm_factor": (D, [I, I, P, P, P, P, P, P, P]),
+            "orc_sum_mag": (D, [I, P]),
+            "orc_sum_prod": (D, [I, P, P]),
+            "orc_face_flux": (None, [I, P, P, P, P, P, P]),
+            "orc_pcg_dic": (Perf, [I, I, P, P, P, P, P, P, D, D, I, I]),
+            "orc_faceareapair_weights": (None, [I, P, P, P]),
+            "orc_of_agglomeration": (P, [I, I, P, P, P, I, P]),
+            "orc_brae_agglomeration": (P, [I, I, P, P, P, I]),
+            "orc_aggl_nlevels": (I, [P]),
+            "orc_aggl_sizes": (None, [P, I, P, P, P, P]),
+            "orc_aggl_level": (None, [P, I, P, P, P, P, P]),
+            "orc_aggl_free": (None, [P]),
+            "orc_galerkin_of": (None, [P, P, P, P, P]),
+            "orc_galerkin_brae": (None, [P, P, P, P, P]),
+            "orc_gamg_default_controls": (None, [P]),
+            "orc_gamg_solve": (Perf, [P, P, P, P, P, P]),
+            "orc_pcg_gamg_solve": (Perf, [P, P, P, P, P, D, D, I, I, P]),
+            "orc_brae_amgpcg": (Perf, [P, P, P, P, P, D, D, I, I, I, I, I, I, D, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def p(a):
+    """data pointer of a C-contiguous numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Mesh:
+    """A polyMesh (points, faces, owner, neighbour, patches) with the oracle's geometry."""
+
+    def __init__(self, points, faceOffsets, faceVerts, owner, neighbour, patchNames=(), patchStart=(),
+                 patchSize=(), patchTypes=()):
+        self.points = f64(points)
+        self.faceOffsets = i32(faceOffsets)
+        self.faceVerts = i32(faceVerts)
+        self.owner = i32(owner)
+        self.neighbour = i32(neighbour)
+        self.nFaces = self.owner.size
+        self.nInternal = self.neighbour.size
+        self.nCells = int(max(self.owner.max(), self.neighbour.max() if self.nInternal else 0)) + 1
+        self.patchNames = [str(x) for x in patchNames]
+        self.patchStart = [int(x) for x in patchStart]
+        self.patchSize = [int(x) for x in patchSize]
+        self.patchTypes = [str(x) for x in patchTypes]
+        self.geometry()
+
+    @classmethod
+    def from_npz(cls, d):
+        return cls(d["points"], d["faceOffsets"], d["faceVerts"], d["owner"], d["neighbour"],
+                   d["patchNames"], d["patchStart"], d["patchSize"], d["patchTypes"])
+
+    def geometry(self):
+        nF, nI, nC = self.nFaces, self.nInternal, self.nCells
+        self.Cf = np.zeros((nF, 3)); self.Sf = np.zeros((nF, 3)); self.magSf = np.zeros(nF)
+        self.C = np.zeros((nC, 3)); self.V = np.zeros(nC)
+        self.weights = np.zeros(nI); self.deltaCoeffs = np.zeros(nI)
+        self.nonOrthDeltaCoeffs = np.zeros(nI); self.corrVec = np.zeros((nI, 3))
+        lib().orc_geometry(self.points.shape[0], p(self.points), nF, p(self.faceOffsets), p(self.faceVerts),
+                           nC, nI, p(self.owner), p(self.neighbour), p(self.Cf), p(self.Sf), p(self.magSf),
+                           p(self.C), p(self.V), p(self.weights), p(self.deltaCoeffs),
+                           p(self.nonOrthDeltaCoeffs), p(self.corrVec))
+        nB = nF - nI
+        self.bFaceCell = i32(self.owner[nI:])
+        self.bDeltaCoeffs = np.zeros(nB)
+        lib().orc_patch_delta(nB, p(self.bFaceCell), p(f64(self.Cf[nI:])), p(f64(self.Sf[nI:])),
+                              p(f64(self.magSf[nI:])), p(self.C), p(self.bDeltaCoeffs))
+
+    @property
+    def nBFaces(self):
+        return self.nFaces - self.nInternal
+
+    def patch_slice(self, name):
+        k = self.patchNames.index(name)
+        s = self.patchStart[k] - self.nInternal
+        return slice(s, s + self.patchSize[k])
+
+    def own(self):
+        return i32(self.owner[:self.nInternal])
+
+    def addressing(self):
+        os_ = np.zeros(self.nCells + 1, np.int32)
+        ls = np.zeros(self.nCells + 1, np.int32)
+        lo = np.zeros(self.nInternal, np.int32)
+        rc = lib().orc_ldu_addressing(self.nCells, self.nInternal, p(self.own()), p(self.neighbour),
+                                      p(os_), p(ls), p(lo))
+        assert rc == 0
+        return os_, ls, lo
+
+
+def laplacian(mesh, dc, gammaf):
+    nC, nI = mesh.nCells, mesh.nInternal
+    diag, upper, lower = np.zeros(nC), np.zeros(nI), np.zeros(nI)
+    lib().orc_laplacian(nC, nI, p(mesh.own()), p(mesh.neighbour), p(f64(dc)), p(f64(gammaf)),
+                        p(f64(mesh.magSf[:nI])), p(diag), p(upper), p(lower))
+    return diag, upper, lower
+
+
+def laplacian_bcoeffs(mesh, bcType, bcValue, gammab):
+    nB, nI = mesh.nBFaces, mesh.nInternal
+    iC, bC = np.zeros(nB), np.zeros(nB)
+    lib().orc_laplacian_bcoeffs(nB, p(i32(bcType)), p(f64(bcValue)), p(f64(gammab)), p(f64(mesh.magSf[nI:])),
+                                p(mesh.bDeltaCoeffs), p(iC), p(bC))
+    return iC, bC
+
+
+def fold(mesh, diag, source, bcType, iC, bC):
+    diagC, b = np.zeros(mesh.nCells), np.zeros(mesh.nCells)
+    lib().orc_fold(mesh.nCells, p(f64(diag)), p(f64(source)), mesh.nBFaces, p(mesh.bFaceCell), p(i32(bcType)),
+                   p(f64(iC)), p(f64(bC)), p(diagC), p(b))
+    return diagC, b
+
+
+def amul(nC, own, nei, diag, upper, lower, psi):
+    out = np.zeros(nC)
+    lib().orc_amul(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(diag)), p(f64(upper)),
+                   p(None if lower is None else f64(lower)), p(f64(psi)), p(out))
+    return out
+
+
+def norm_factor(nC, own, nei, diag, upper, psi, b):
+    return lib().orc_norm_factor(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(diag)), p(f64(upper)), None,
+                                 p(f64(psi)), p(f64(b)))
+
+
+def pcg_dic(nC, own, nei, diag, upper, b, psi, tol, relTol=0.0, maxIter=1000, minIter=0):
+    psi = f64(psi).copy()
+    perf = lib().orc_pcg_dic(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(diag)), p(f64(upper)), p(f64(b)),
+                             p(psi), tol, relTol, maxIter, minIter)
+    return psi, perf
+
+
+class Agglomeration:
+    def __init__(self, handle):
+        self.h = handle
+        L = lib()
+        self.nLevels = L.orc_aggl_nlevels(self.h)
+        self.levels = []
+        for i in range(self.nLevels):
+            s = [ctypes.c_int() for _ in range(4)]
+            L.orc_aggl_sizes(self.h, i, *[ctypes.addressof(x) for x in s])
+            nF, nFF, nCo, nCF = (x.value for x in s)
+            m, co, cn, fr = (np.zeros(nF, np.int32), np.zeros(nCF, np.int32), np.zeros(nCF, np.int32),
+                             np.zeros(nFF, np.int32))
+            ff = np.zeros(nFF, np.int8)
+            L.orc_aggl_level(self.h, i, p(m), p(co), p(cn), p(fr), p(ff))
+            self.levels.append(dict(nFine=nF, nFineFaces=nFF, nCoarse=nCo, nCoarseFaces=nCF, map=m,
+                                    cOwn=co, cNei=cn, faceRestrict=fr, faceFlip=ff))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_aggl_free(self.h)
+            self.h = None
+
+    def total_coarse(self):
+        return (sum(l["nCoarse"] for l in self.levels), sum(l["nCoarseFaces"] for l in self.levels))
+
+    def galerkin(self, diag, upper, order="brae"):
+        tc, tf = self.total_coarse()
+        cd, cu = np.zeros(tc), np.zeros(tf)
+        fn = lib().orc_galerkin_brae if order == "brae" else lib().orc_galerkin_of
+        fn(self.h, p(f64(diag)), p(f64(upper)), p(cd), p(cu))
+        out, oc, of = [], 0, 0
+        for l in self.levels:
+            out.append((cd[oc:oc + l["nCoarse"]], cu[of:of + l["nCoarseFaces"]]))
+            oc += l["nCoarse"]; of += l["nCoarseFaces"]
+        return out
+
+
+def of_agglomeration(nC, own, nei, weights, nCellsInCoarsestLevel=10, forward=True):
+    fwd = ctypes.c_int(1 if forward else 0)
+    h = lib().orc_of_agglomeration(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(weights)),
+                                   nCellsInCoarsestLevel, ctypes.addressof(fwd))
+    return Agglomeration(h)
+
+
+def brae_agglomeration(nC, own, nei, weights, target=64):
+    h = lib().orc_brae_agglomeration(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(weights)), target)
+    return Agglomeration(h)
+
+
+def faceareapair_weights(Sf, magSf):
+    w = np.zeros(len(magSf))
+    lib().orc_faceareapair_weights(len(magSf), p(f64(Sf)), p(f64(magSf)), p(w))
+    return w
+
+
+def gamg_controls(**kw):
+    c = GamgControls()
+    lib().orc_gamg_default_controls(ctypes.addressof(c))
+    for k, v in kw.items():
+        if k == "smoother" and isinstance(v, str):
+            v = SMOOTHERS[v]
+        setattr(c, k, v)
+    return c
+
+
+def gamg_solve(aggl, diag, upper, b, psi, **kw):
+    c = gamg_controls(**kw)
+    psi = f64(psi).copy()
+    perf = lib().orc_gamg_solve(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(b)), p(psi), ctypes.addressof(c))
+    return psi, perf
+
+
+def pcg_gamg_solve(aggl, diag, upper, b, psi, tol, relTol=0.0, maxIter=1000, minIter=0, **kw):
+    c = gamg_controls(**kw)
+    psi = f64(psi).copy()
+    perf = lib().orc_pcg_gamg_solve(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(b)), p(psi), tol, relTol,
+                                    maxIter, minIter, ctypes.addressof(c))
+    return psi, perf
+
+
+def brae_amgpcg(aggl, diag, upper, b, psi, tol, relTol=0.0, maxIter=1000, minIter=0, nPre=1, nPost=1,
+                flexible=False, single=False, normFactor=0.0, trace=False):
+    psi = f64(psi).copy()
+    tr = np.zeros((maxIter + 1, psi.size)) if trace else None
+    perf = lib().orc_brae_amgpcg(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(b)), p(psi), tol, relTol, maxIter,
+                                 minIter, nPre, nPost, int(flexible), int(single), normFactor, p(tr))
+    return (psi, perf, tr) if trace else (psi, perf)
