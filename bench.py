@@ -1,0 +1,318 @@
+"""Benchmark: the SIMPLE pressure corrector on one MI355X.
+
+Workload (one step = one SIMPLE iteration's pressure corrector on the 500x250x160 graded, jittered
+channel, 20,000,000 cells): rAUf = interpolate(rAU); assemble -laplacian(rAUf, p) (face coefficients,
+diagonal, fixedValue/zeroGradient boundary coefficients); source = -div(phiHbyA)*V; fold; normFactor;
+Galerkin re-coarsening; AMG-preconditioned CG to a normalised residual of 1e-6 from p = 0;
+phi = phiHbyA - pEqn.flux(). The mixed-precision path runs the V-cycle in FP32 (or FP16) inside FP64
+flexible CG; the FP64 path runs the same solve with an FP64 cycle. Both are timed; `value` is the
+mixed path's seconds per step (lower is better), and the FP64 time, the ratio, iterations, residuals
+and the mixed/FP64 solution difference are reported beside it.
+
+Inputs are synthetic and seeded: perturbed plug flow U = (10,0,0) + uniform(-1,1) per component
+(inlet fixedValue (10,0,0), outlet zeroGradient, walls no-slip), phiHbyA = interpolate(U).Sf,
+rAU = 0.05*V^(1/3)/|U|; p: fixedValue 0 at the outlet, zeroGradient elsewhere.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "s/SIMPLE-iter @20M cells, FP64 vs mixed AMG at equal resid; solves/s + p99"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", default="500x250x160")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16"])
+    ap.add_argument("--tolerance", type=float, default=1e-6)
+    ap.add_argument("--cpu-sample", default="250x125x80", help="channel the CPU oracle is timed on")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--check-every", type=int, default=4)
+    return ap.parse_args()
+
+
+class Channel:
+    """The graded, jittered channel and its synthetic pressure-corrector inputs on the device."""
+
+    def __init__(self, nx, ny, nz, torch, mfv, hexmesh, ctx):
+        t0 = time.time()
+        self.shape = (nx, ny, nz)
+        pts = hexmesh.box_points(nx, ny, nz, (4.0, 1.0, 0.5), (1.0, 20.0, 20.0), 0.1, seed=2024)
+        faces, owner, nei, patches = hexmesh.box_topology(nx, ny, nz)
+        self.nC, self.nI = nx * ny * nz, nei.size
+        self.nB = owner.size - self.nI
+        self.patches = patches
+        log("mesh %dx%dx%d: %d cells, %d internal faces (%.1fs)" % (nx, ny, nz, self.nC, self.nI, time.time() - t0))
+        self.mesh = mfv.Mesh(ctx, self.nC, np.ascontiguousarray(owner[:self.nI]), nei,
+                             np.ascontiguousarray(owner[self.nI:]))
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+        z = lambda n: torch.zeros(int(n), dtype=torch.float64, device="cuda:0")
+        nF = owner.size
+        dpts = dev(pts)
+        dfo = dev(np.arange(0, 4 * nF + 1, 4, dtype=np.int32))
+        dfv = dev(faces.reshape(-1))
+        self.Cf, self.Sf, self.magSf = z(3 * nF), z(3 * nF), z(nF)
+        self.C, self.V = z(3 * self.nC), z(self.nC)
+        self.w, self.dc, self.ndc, self.corr, self.bDelta = z(self.nI), z(self.nI), z(self.nI), z(3 * self.nI), z(self.nB)
+        torch.cuda.synchronize()
+        ctx.call("mfv_mesh_geometry", self.mesh.h, *[mfv.ptr(t) for t in (
+            dpts, dfo, dfv, self.Cf, self.Sf, self.magSf, self.C, self.V, self.w, self.dc, self.ndc, self.corr,
+            self.bDelta)])
+        ctx.sync()
+        del dpts, dfo, dfv
+        log("geometry on device (%.1fs)" % (time.time() - t0))
+        # synthetic seeded fields
+        g = torch.Generator(device="cuda:0")
+        g.manual_seed(7)
+        U = torch.rand((self.nC, 3), dtype=torch.float64, device="cuda:0", generator=g) * 2.0 - 1.0
+        U[:, 0] += 10.0
+        own_d, nei_d = dev(owner[:self.nI]).long(), dev(nei).long()
+        Sf = self.Sf.view(nF, 3)
+        wv = self.w.view(-1, 1)
+        Uf = wv * U[own_d] + (1.0 - wv) * U[nei_d]
+        self.phiHbyA = (Uf * Sf[:self.nI]).sum(1).contiguous()
+        phiB = torch.zeros(self.nB, dtype=torch.float64, device="cuda:0")
+        bcell = dev(owner[self.nI:]).long()
+        self.types = torch.zeros(self.nB, dtype=torch.int32, device="cuda:0")
+        for name, start, size in patches:
+            s = slice(start - self.nI, start - self.nI + size)
+            if name == "inlet":
+                phiB[s] = 10.0 * Sf[start:start + size, 0]
+            elif name == "outlet":
+                phiB[s] = (U[bcell[s]] * Sf[start:start + size]).sum(1)
+                self.types[s] = 1
+        self.phiB = phiB.contiguous()
+        self.bMagSf = self.magSf[self.nI:].contiguous()
+        self.magSfI = self.magSf[:self.nI].contiguous()
+        Umag = U.norm(dim=1)
+        self.negRAU = (-0.05 * self.V.pow(1.0 / 3.0) / Umag).contiguous()
+        self.bcValue = z(self.nB)
+        del U, Uf, own_d, nei_d, bcell
+        # work arrays
+        self.gf, self.gb = z(self.nI), z(self.nB)
+        self.diag, self.upper = z(self.nC), z(self.nI)
+        self.iC, self.bC = z(self.nB), z(self.nB)
+        self.source, self.diagC, self.b = z(self.nC), z(self.nC), z(self.nC)
+        self.p, self.phi = z(self.nC), z(self.nI)
+        torch.cuda.synchronize()
+        log("fields ready (%.1fs)" % (time.time() - t0))
+
+
+def gpu_run(args):
+    import torch
+    from brae_amd import hexmesh, mfv
+
+    nx, ny, nz = (int(v) for v in args.size.split("x"))
+    stream = torch.cuda.current_stream()
+    ctx = mfv.Context(0, stream=stream.cuda_stream)
+    ch = Channel(nx, ny, nz, torch, mfv, hexmesh, ctx)
+    t0 = time.time()
+    w_host = ch.magSfI.cpu().numpy()
+    amg = mfv.Amg(ch.mesh, w_host, mfv.AGG_BRAE, 64)
+    sizes = [amg.grid_size(g) for g in range(amg.nGrids)]
+    log("AMG hierarchy: %d grids, coarsest %d cells (%.1fs host build)" % (amg.nGrids, sizes[-1][0], time.time() - t0))
+    P = mfv.ptr
+    A = mfv.ldu(ch.diagC, ch.upper)
+    prec_mixed = mfv.PREC_FP32 if args.precision == "fp32" else mfv.PREC_FP16
+
+    def step(prec):
+        ch.p.zero_()
+        ch.source.zero_()
+        ctx.call("mfv_interpolate", ch.mesh.h, P(ch.w), P(ch.negRAU), P(ch.gf), P(ch.gb))
+        ctx.call("mfv_laplacian_coeffs", ch.mesh.h, P(ch.ndc), P(ch.gf), P(ch.magSfI), P(ch.diag), P(ch.upper), None)
+        ctx.call("mfv_laplacian_bcoeffs", ch.mesh.h, P(ch.types), P(ch.bcValue), P(ch.gb), P(ch.bMagSf),
+                 P(ch.bDelta), P(ch.iC), P(ch.bC))
+        ctx.call("mfv_div_source", ch.mesh.h, P(ch.phiHbyA), P(ch.phiB), P(ch.V), -1.0, P(ch.source))
+        ctx.call("mfv_fold", ch.mesh.h, P(ch.diag), P(ch.source), P(ch.iC), P(ch.bC), P(ch.diagC), P(ch.b))
+        amg.galerkin(ch.diagC, ch.upper, prec)
+        perf = amg.solve(ch.diagC, ch.upper, ch.b, ch.p, tolerance=args.tolerance, precision=prec,
+                         flexible=int(prec != mfv.PREC_FP64), checkEvery=args.check_every)
+        ctx.call("mfv_flux_correct", ch.mesh.h, ctypes.byref(A), P(ch.p), P(ch.phiHbyA), -1.0, P(ch.phi))
+        return perf
+
+    def timed(prec, profile):
+        for _ in range(args.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        if profile:
+            ctx.profile(True)
+            for tag in range(7):
+                ctx.profile_read(tag)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        perfs = [step(prec) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        prof = None
+        if profile:
+            prof = {tag: ctx.profile_read(tag) for tag in range(7)}
+            ctx.profile(False)
+        return dt, perfs, prof
+
+    res = {}
+    for name, prec in (("mixed", prec_mixed), ("fp64", mfv.PREC_FP64)):
+        dt, perfs, prof = timed(prec, profile=(name == "mixed"))
+        res[name] = dict(dt=dt, perfs=perfs, prof=prof, p=ch.p.clone())
+        log("%s: %.4f s/step, iterations %s, final residual %.3e" % (
+            name, dt, [q.nIterations for q in perfs], perfs[-1].finalResidual))
+    pm, p64 = res["mixed"]["p"], res["fp64"]["p"]
+    rel_l2 = float((pm - p64).norm() / p64.norm())
+
+    # roofline of the dominant kernel of the mixed step (largest summed event time)
+    N, F, Nc = ch.nC, ch.nI, sizes[1][0]
+    zb = 4  # the cycle's vector width (FP32 vectors for the FP32 and FP16 cycles)
+    vb = 4 if prec_mixed == mfv.PREC_FP32 else 2
+    iters = sum(q.nIterations for q in res["mixed"]["perfs"])
+    names = {
+        mfv.PROF_PCG_SPMV64: ("pcgDirAmulK: p = z + beta*p, q = A.p (FP64), p.q", 8 * (N + F) + 8 * F + zb * N + 24 * N,
+                              iters),
+        mfv.PROF_PCG_UPDATE: ("pcgUpdateK: psi, r update, sum|r|, next cycle entry", 8 * (2 * N + 4 * N + N) + 3 * zb * N,
+                              iters),
+        mfv.PROF_PCG_ZR: ("pcgZrK: z.r, z.r_old", zb * N + 16 * N, iters),
+        mfv.PROF_AMG_RESIDUAL0: ("residualK grid 0 (cycle precision)", zb * N + vb * F + 3 * zb * N + 8 * F, iters),
+        mfv.PROF_AMG_PROLONG_SMOOTH0: ("prolongSmoothK grid 0", zb * N + vb * F + 4 * zb * N + 4 * N + zb * Nc + 8 * F,
+                                       iters),
+        mfv.PROF_AMG_RESTRICT0: ("restrictK grid 0 -> 1", zb * N + 4 * N + 4 * Nc + 2 * zb * Nc, iters),
+    }
+    prof = res["mixed"]["prof"]
+    dom = max(names, key=lambda t: prof[t][1])
+    label, alg_bytes, launches = names[dom]
+    avg_ms = prof[dom][1] / max(1, launches)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    per_kernel = {names[t][0]: {"launches": launches, "avg_us": round(prof[t][1] / max(1, launches) * 1e3, 2),
+                                "alg_bytes": int(names[t][1]),
+                                "alg_GBps": round(names[t][1] / (prof[t][1] / max(1, launches) * 1e-3) / 1e9, 1)}
+                  for t in names if prof[t][1] > 0}
+    vc_n, vc_ms = prof[mfv.PROF_VCYCLE]
+    out = dict(
+        dt_mixed=res["mixed"]["dt"], dt_fp64=res["fp64"]["dt"],
+        iters_mixed=[q.nIterations for q in res["mixed"]["perfs"]],
+        iters_fp64=[q.nIterations for q in res["fp64"]["perfs"]],
+        final_mixed=res["mixed"]["perfs"][-1].finalResidual, final_fp64=res["fp64"]["perfs"][-1].finalResidual,
+        p_rel_l2_mixed_vs_fp64=rel_l2, n_grids=amg.nGrids, coarsest=sizes[-1][0],
+        roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                      frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None, kernel=label,
+                      alg_bytes_per_launch=int(alg_bytes), avg_launch_us=round(avg_ms * 1e3, 2)),
+        per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1),
+    )
+    amg.close()
+    ch.mesh.close()
+    ctx.close()
+    return out
+
+
+def cpu_baseline(args, n_full):
+    """The oracle's OpenFOAM GAMG-preconditioned PCG (GaussSeidel, V(0,2,2), nVcycles 2, faceAreaPair)
+    on a reduced channel of the same construction, single-threaded, scaled to the full cell count."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from brae_amd import hexmesh
+    nx, ny, nz = (int(v) for v in args.cpu_sample.split("x"))
+    m = hexmesh.box_mesh(nx, ny, nz, (4.0, 1.0, 0.5), (1.0, 20.0, 20.0), 0.1, seed=2024)
+    nC, nI = m["nCells"], m["nInternal"]
+    nB = m["faces"].shape[0] - nI
+    rng = np.random.default_rng(7)
+    U = rng.uniform(-1.0, 1.0, (nC, 3))
+    U[:, 0] += 10.0
+    own, nei = np.ascontiguousarray(m["owner"][:nI]), m["neighbour"]
+    w = m["weights"][:, None]
+    phi = ((w * U[own] + (1 - w) * U[nei]) * m["Sf"][:nI]).sum(1)
+    phiB = np.zeros(nB)
+    types = np.zeros(nB, np.int32)
+    bcell = np.ascontiguousarray(m["owner"][nI:])
+    for name, start, size in m["patches"]:
+        s = slice(start - nI, start - nI + size)
+        if name == "inlet":
+            phiB[s] = 10.0 * m["Sf"][start:start + size, 0]
+        elif name == "outlet":
+            phiB[s] = (U[bcell[s]] * m["Sf"][start:start + size]).sum(1)
+            types[s] = 1
+    negRAU = -0.05 * np.cbrt(m["V"]) / np.linalg.norm(U, axis=1)
+    t = time.perf_counter()
+    gf = m["weights"] * negRAU[own] + (1 - m["weights"]) * negRAU[nei]
+    diag, up, _ = np.zeros(nC), np.zeros(nI), None
+    O.lib().orc_laplacian(nC, nI, O.p(own), O.p(nei), O.p(m["nonOrthDeltaCoeffs"]), O.p(gf),
+                          O.p(O.f64(m["magSf"][:nI])), O.p(diag), O.p(up), None)
+    iC, bC = np.zeros(nB), np.zeros(nB)
+    O.lib().orc_laplacian_bcoeffs(nB, O.p(types), O.p(np.zeros(nB)), O.p(O.f64(negRAU[bcell])),
+                                  O.p(O.f64(m["magSf"][nI:])), O.p(O.f64(m["bDeltaCoeffs"])), O.p(iC), O.p(bC))
+    div = np.zeros(nC)
+    np.add.at(div, own, phi)
+    np.add.at(div, nei, -phi)
+    np.add.at(div, bcell, phiB)
+    src = -((div / m["V"]) * m["V"])
+    diagC, b = np.zeros(nC), np.zeros(nC)
+    O.lib().orc_fold(nC, O.p(diag), O.p(src), nB, O.p(bcell), O.p(types), O.p(iC), O.p(bC), O.p(diagC), O.p(b))
+    wts = O.faceareapair_weights(m["Sf"][:nI], m["magSf"][:nI])
+    t_setup = time.perf_counter() - t
+    ag = O.of_agglomeration(nC, own, nei, wts, nCellsInCoarsestLevel=10)
+    t = time.perf_counter()
+    psi, perf = O.pcg_gamg_solve(ag, diagC, up, b, np.zeros(nC), args.tolerance, smoother="GaussSeidel")
+    t_solve = time.perf_counter() - t
+    flux = np.zeros(nI)
+    O.lib().orc_face_flux(nI, O.p(own), O.p(nei), O.p(up), None, O.p(psi), O.p(flux))
+    sec = t_setup + t_solve
+    scaled = sec * n_full / nC
+    return dict(value=round(scaled, 3), unit="s/SIMPLE-iter (pressure corrector) @ full size, linear in cells",
+                cores=1, kind="port",
+                sample="oracle FP64 pressure corrector (assembly, fold, OpenFOAM GAMG-PCG GaussSeidel V(0,2,2) "
+                       "nVcycles 2, tol %g) on the %s channel (%d cells): %.2f s, %d PCG iterations; scaled x%.1f"
+                       % (args.tolerance, args.cpu_sample, nC, sec, perf.nIterations, n_full / nC))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (each box has one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    nx, ny, nz = (int(v) for v in args.size.split("x"))
+    r = gpu_run(args)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, nx * ny * nz)
+    line = {
+        "metric": METRIC,
+        "value": round(r["dt_mixed"], 5),
+        "unit": "s/SIMPLE-iter (pressure corrector)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["dt_mixed"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 outer / %s V-cycle" % args.precision,
+        "data": "synthetic (seeded perturbed plug flow on the graded jittered channel; p solved from 0)",
+        "config": {"workload": "SIMPLE pressure corrector, %s graded (ratio 20) jittered (10%%) channel 4x1x0.5, "
+                               "%d cells, AMG-PCG to %g" % (args.size, nx * ny * nz, args.tolerance),
+                   "cells": nx * ny * nz, "tolerance": args.tolerance, "v_cycle": args.precision},
+        "fp64_s_per_iter": round(r["dt_fp64"], 5),
+        "speedup_mixed_vs_fp64": round(r["dt_fp64"] / r["dt_mixed"], 3),
+        "pcg_iterations": {"mixed": r["iters_mixed"], "fp64": r["iters_fp64"]},
+        "final_residual": {"mixed": r["final_mixed"], "fp64": r["final_fp64"]},
+        "p_rel_l2_mixed_vs_fp64": r["p_rel_l2_mixed_vs_fp64"],
+        "amg": {"grids": r["n_grids"], "coarsest_cells": r["coarsest"], "vcycle_avg_us": r["vcycle_avg_us"]},
+        "roofline": r["roofline"],
+        "per_kernel": r["per_kernel"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

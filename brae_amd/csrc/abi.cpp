@@ -1,0 +1,687 @@
+// The C ABI (include/mfv.h): argument checks, exception-to-status translation, the context and
+// the mesh addressing. Numerical work is in the kernels_*.hip launchers and the solver drivers.
+#include "amg.hpp"
+#include "mfv.h"
+#include "mfv_internal.hpp"
+
+#include <cstring>
+#include <new>
+
+struct mfv_ctx_s
+{
+    mfv::Ctx c;
+};
+
+struct mfv_mesh_s
+{
+    mfv_ctx ctx;
+    mfv::Mesh m;
+};
+
+struct mfv_amg_s
+{
+    mfv_ctx ctx;
+    mfv_mesh mesh;
+    mfv::Amg a;
+};
+
+namespace mfv {
+
+namespace {
+
+thread_local std::string tlsError;
+
+template <typename F>
+int guarded(mfv_ctx ctx, F&& f)
+{
+    try
+    {
+        f();
+        if (ctx) ctx->c.lastError.clear();
+        return MFV_OK;
+    }
+    catch (const Error& e)
+    {
+        if (ctx) ctx->c.lastError = e.what();
+        tlsError = e.what();
+        return e.code;
+    }
+    catch (const std::bad_alloc&)
+    {
+        if (ctx) ctx->c.lastError = "mfv: host allocation failed";
+        tlsError = "mfv: host allocation failed";
+        return MFV_ERR_ALLOC;
+    }
+    catch (const std::exception& e)
+    {
+        if (ctx) ctx->c.lastError = e.what();
+        tlsError = e.what();
+        return MFV_ERR_INVALID;
+    }
+}
+
+void need(bool ok, const char* what)
+{
+    if (!ok) invalid(what);
+}
+
+} // namespace
+
+// ownerStart, stable losort by neighbour, boundary gather (device_ldu.cuh:109-146)
+void buildMesh(Ctx& c, Mesh& m, int nCells, int nFaces, const int* owner, const int* neighbour, int nB,
+               const int* bFaceCell)
+{
+    if (nCells <= 0) invalid("mesh: nCells must be positive");
+    if (nFaces < 0 || nB < 0) invalid("mesh: negative face count");
+    if (nFaces && (!owner || !neighbour)) invalid("mesh: owner/neighbour missing");
+    if (nB && !bFaceCell) invalid("mesh: bFaceCell missing");
+    m.nCells = nCells;
+    m.nFaces = nFaces;
+    m.nBFaces = nB;
+    m.hOwner.assign(owner, owner + nFaces);
+    m.hNeighbour.assign(neighbour, neighbour + nFaces);
+    m.hBFaceCell.assign(bFaceCell, bFaceCell + nB);
+    std::vector<label> os(nCells + 1, 0), ls(nCells + 1, 0), lo(nFaces), loOwn(nFaces);
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const int o = owner[f], n = neighbour[f];
+        if (o < 0 || n >= nCells || o >= n)
+            invalid("mesh: internal face " + std::to_string(f) + " is not owner < neighbour within [0, nCells)");
+        if (f > 0 && o < owner[f - 1])
+            invalid("mesh: internal face " + std::to_string(f) +
+                    " breaks OpenFOAM's upper-triangular order (faces sorted by owner)");
+        ++os[o + 1];
+        ++ls[n + 1];
+    }
+    for (int i = 0; i < nCells; ++i)
+    {
+        os[i + 1] += os[i];
+        ls[i + 1] += ls[i];
+    }
+    {
+        std::vector<label> at(ls.begin(), ls.end() - 1);
+        for (int f = 0; f < nFaces; ++f) lo[at[neighbour[f]]++] = f;
+    }
+    for (int k = 0; k < nFaces; ++k) loOwn[k] = owner[lo[k]];
+    std::vector<label> bs(nCells + 1, 0), bf(nB);
+    for (int i = 0; i < nB; ++i)
+    {
+        if (bFaceCell[i] < 0 || bFaceCell[i] >= nCells) invalid("mesh: boundary face cell out of range");
+        ++bs[bFaceCell[i] + 1];
+    }
+    for (int i = 0; i < nCells; ++i) bs[i + 1] += bs[i];
+    {
+        std::vector<label> at(bs.begin(), bs.end() - 1);
+        for (int i = 0; i < nB; ++i) bf[at[bFaceCell[i]]++] = i;
+    }
+    m.owner.upload(owner, nFaces, c.stream);
+    m.neighbour.upload(neighbour, nFaces, c.stream);
+    m.ownerStart.upload(os, c.stream);
+    m.losortStart.upload(ls, c.stream);
+    m.losort.upload(lo, c.stream);
+    m.losortOwner.upload(loOwn, c.stream);
+    m.bFaceCell.upload(bFaceCell, nB, c.stream);
+    m.bndStart.upload(bs, c.stream);
+    m.bndFaces.upload(bf, c.stream);
+    hipCheck(hipStreamSynchronize(c.stream), "mesh upload");
+}
+
+} // namespace mfv
+
+using namespace mfv;
+
+namespace {
+
+LduView view(const mfv_mesh mesh, const mfv_ldu* A)
+{
+    need(A && A->dDiag && A->dUpper, "matrix: diag/upper missing");
+    return LduView{mesh->m.nCells, mesh->m.nFaces, A->dDiag, A->dUpper, A->dLower ? A->dLower : A->dUpper};
+}
+
+void needMesh(mfv_ctx ctx, mfv_mesh mesh)
+{
+    need(ctx != nullptr, "null context");
+    need(mesh != nullptr, "null mesh");
+    need(mesh->ctx == ctx, "mesh belongs to another context");
+}
+
+} // namespace
+
+extern "C" {
+
+int mfv_ctx_create(int device, void* hStream, mfv_ctx* out)
+{
+    if (!out) return MFV_ERR_INVALID;
+    *out = nullptr;
+    mfv_ctx ctx = new (std::nothrow) mfv_ctx_s();
+    if (!ctx) return MFV_ERR_ALLOC;
+    const int rc = guarded(nullptr, [&] {
+        hipCheck(hipSetDevice(device), "set device");
+        ctx->c.device = device;
+        if (hStream)
+            ctx->c.stream = static_cast<hipStream_t>(hStream);
+        else
+        {
+            hipCheck(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking), "stream create");
+            ctx->c.ownsStream = true;
+        }
+        ctx->c.partials.resize((size_t)8 * kMaxReduceBlocks);
+        ctx->c.tickets.resize(8);
+        hipCheck(hipMemsetAsync(ctx->c.tickets.p, 0, 8 * sizeof(unsigned int), ctx->c.stream), "ticket clear");
+        ctx->c.scalars.resize(64);
+        hipCheck(hipMemsetAsync(ctx->c.scalars.p, 0, 64 * sizeof(double), ctx->c.stream), "scalar clear");
+        hipCheck(hipHostMalloc((void**)&ctx->c.mailbox, 64 * sizeof(double), hipHostMallocDefault), "mailbox");
+        int cu = 0;
+        hipCheck(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device), "cu count");
+        ctx->c.cuCount = cu > 0 ? cu : 256;
+        hipCheck(hipStreamSynchronize(ctx->c.stream), "ctx sync");
+    });
+    if (rc != MFV_OK)
+    {
+        if (ctx->c.mailbox) (void)hipHostFree(ctx->c.mailbox);
+        if (ctx->c.ownsStream && ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return MFV_OK;
+}
+
+int mfv_ctx_destroy(mfv_ctx ctx)
+{
+    if (!ctx) return MFV_OK;
+    (void)hipStreamSynchronize(ctx->c.stream);
+    for (auto& v : ctx->c.prof)
+        for (auto& e : v)
+        {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    for (hipEvent_t e : ctx->c.eventPool) (void)hipEventDestroy(e);
+    if (ctx->c.mailbox) (void)hipHostFree(ctx->c.mailbox);
+    if (ctx->c.ownsStream) (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+    return MFV_OK;
+}
+
+const char* mfv_last_error(mfv_ctx ctx)
+{
+    return ctx ? ctx->c.lastError.c_str() : tlsError.c_str();
+}
+
+void* mfv_ctx_stream(mfv_ctx ctx)
+{
+    return ctx ? (void*)ctx->c.stream : nullptr;
+}
+
+int mfv_ctx_sync(mfv_ctx ctx)
+{
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        hipCheck(hipStreamSynchronize(ctx->c.stream), "sync");
+    });
+}
+
+int mfv_ctx_set_sequential_reduce(mfv_ctx ctx, int sequential)
+{
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        ctx->c.sequentialReduce = sequential != 0;
+    });
+}
+
+int mfv_mesh_create(mfv_ctx ctx, int nCells, int nFaces, const int* hOwner, const int* hNeighbour, int nBFaces,
+                    const int* hBFaceCell, mfv_mesh* out)
+{
+    if (!out) return MFV_ERR_INVALID;
+    *out = nullptr;
+    mfv_mesh mesh = nullptr;
+    const int rc = guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        mesh = new mfv_mesh_s();
+        mesh->ctx = ctx;
+        buildMesh(ctx->c, mesh->m, nCells, nFaces, hOwner, hNeighbour, nBFaces, hBFaceCell);
+    });
+    if (rc != MFV_OK)
+    {
+        delete mesh;
+        return rc;
+    }
+    *out = mesh;
+    return MFV_OK;
+}
+
+int mfv_mesh_destroy(mfv_mesh mesh)
+{
+    if (mesh) (void)hipStreamSynchronize(mesh->ctx->c.stream);
+    delete mesh;
+    return MFV_OK;
+}
+
+int mfv_mesh_addressing(mfv_mesh mesh, const int** dOwner, const int** dNeighbour, const int** dOwnerStart,
+                        const int** dLosortStart, const int** dLosort)
+{
+    if (!mesh) return MFV_ERR_INVALID;
+    if (dOwner) *dOwner = mesh->m.owner.p;
+    if (dNeighbour) *dNeighbour = mesh->m.neighbour.p;
+    if (dOwnerStart) *dOwnerStart = mesh->m.ownerStart.p;
+    if (dLosortStart) *dLosortStart = mesh->m.losortStart.p;
+    if (dLosort) *dLosort = mesh->m.losort.p;
+    return MFV_OK;
+}
+
+int mfv_laplacian_coeffs(mfv_ctx ctx, mfv_mesh mesh, const double* dDeltaCoeffs, const double* dGammaf,
+                         const double* dMagSf, double* dDiag, double* dUpper, double* dLower)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dDeltaCoeffs && dGammaf && dMagSf && dDiag && dUpper, "laplacian: null argument");
+        launchLaplacian(ctx->c, mesh->m, dDeltaCoeffs, dGammaf, dMagSf, dDiag, dUpper, dLower ? dLower : dUpper);
+    });
+}
+
+int mfv_laplacian_bcoeffs(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue,
+                          const double* dGammab, const double* dBMagSf, const double* dBDeltaCoeffs, double* dIC,
+                          double* dBC)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(!mesh->m.nBFaces || (dBcType && dBcValue && dGammab && dBMagSf && dBDeltaCoeffs && dIC && dBC),
+             "laplacian boundary coefficients: null argument");
+        launchLaplacianBCoeffs(ctx->c, mesh->m.nBFaces, dBcType, dBcValue, dGammab, dBMagSf, dBDeltaCoeffs, dIC, dBC);
+    });
+}
+
+int mfv_div_upwind_coeffs(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, double* dDiag, double* dUpper,
+                          double* dLower)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPhi && dDiag && dUpper && dLower, "div: null argument");
+        need(dUpper != dLower, "div: an upwind matrix is asymmetric; upper and lower must be separate buffers");
+        launchDivUpwind(ctx->c, mesh->m, dPhi, dDiag, dUpper, dLower);
+    });
+}
+
+int mfv_div_bcoeffs(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue, const double* dPhib,
+                    double* dIC, double* dBC)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(!mesh->m.nBFaces || (dBcType && dBcValue && dPhib && dIC && dBC), "div boundary: null argument");
+        launchDivBCoeffs(ctx->c, mesh->m.nBFaces, dBcType, dBcValue, dPhib, dIC, dBC);
+    });
+}
+
+int mfv_fold(mfv_ctx ctx, mfv_mesh mesh, const double* dDiag, const double* dSource, const double* dIC,
+             const double* dBC, double* dDiagC, double* dB)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dDiag && dSource && dDiagC && dB, "fold: null argument");
+        need(!mesh->m.nBFaces || (dIC && dBC), "fold: boundary coefficients missing");
+        launchFold(ctx->c, mesh->m, dDiag, dSource, dIC, dBC, dDiagC, dB);
+    });
+}
+
+int mfv_set_reference(mfv_ctx ctx, double* dDiag, double* dB, int refCell, double refValue)
+{
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        need(dDiag && dB, "setReference: null argument");
+        if (refCell < 0) return;
+        launchSetReference(ctx->c, dDiag, dB, refCell, refValue);
+    });
+}
+
+int mfv_face_flux(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, double* dFlux)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPsi && dFlux, "face flux: null argument");
+        launchFaceFlux(ctx->c, mesh->m, view(mesh, A), dPsi, dFlux);
+    });
+}
+
+int mfv_amul(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, double* dApsi)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPsi && dApsi, "amul: null argument");
+        launchAmul(ctx->c, mesh->m, view(mesh, A), dPsi, dApsi);
+    });
+}
+
+int mfv_residual(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dB, double* dR)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPsi && dB && dR, "residual: null argument");
+        launchResidual(ctx->c, mesh->m, view(mesh, A), dPsi, dB, dR);
+    });
+}
+
+int mfv_dot(mfv_ctx ctx, int n, const double* dA, const double* dB, double* dResult)
+{
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        need(n >= 0 && (n == 0 || (dA && dB)) && dResult, "dot: bad argument");
+        launchDot(ctx->c, n, dA, dB, dResult, 0);
+    });
+}
+
+int mfv_sum_mag(mfv_ctx ctx, int n, const double* dA, double* dResult)
+{
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        need(n >= 0 && (n == 0 || dA) && dResult, "sum_mag: bad argument");
+        launchSumMag(ctx->c, n, dA, dResult, 0);
+    });
+}
+
+int mfv_norm_factor(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dB,
+                    double* dNormFactor)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPsi && dB && dNormFactor, "normFactor: null argument");
+        launchNormFactor(ctx->c, mesh->m, view(mesh, A), dPsi, dB, dNormFactor);
+    });
+}
+
+
+int mfv_amg_build(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const mfv_amg_controls* controls,
+                  mfv_amg* out)
+{
+    if (!out) return MFV_ERR_INVALID;
+    *out = nullptr;
+    mfv_amg amg = nullptr;
+    const int rc = guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(hFaceWeights != nullptr || mesh->m.nFaces == 0, "amg build: face weights missing");
+        AmgBuildControls bc;
+        if (controls)
+        {
+            if (controls->agglomeration != MFV_AGG_BRAE && controls->agglomeration != MFV_AGG_FACEAREAPAIR)
+                throw Error(kErrUnsupported, "mfv: amg build: unknown agglomeration " +
+                                                 std::to_string(controls->agglomeration));
+            if (controls->mergeLevels > 1)
+                throw Error(kErrUnsupported, "mfv: amg build: mergeLevels above 1 is not carried");
+            bc.kind = controls->agglomeration == MFV_AGG_BRAE ? AggKind::Brae : AggKind::OpenFOAM;
+            bc.target = controls->nCellsInCoarsestLevel > 0 ? controls->nCellsInCoarsestLevel
+                                                            : (bc.kind == AggKind::Brae ? 64 : 10);
+        }
+        amg = new mfv_amg_s();
+        amg->ctx = ctx;
+        amg->mesh = mesh;
+        std::vector<double> w(hFaceWeights, hFaceWeights + mesh->m.nFaces);
+        amgBuild(ctx->c, mesh->m, amg->a, w, bc);
+    });
+    if (rc != MFV_OK)
+    {
+        delete amg;
+        return rc;
+    }
+    *out = amg;
+    return MFV_OK;
+}
+
+int mfv_amg_destroy(mfv_amg amg)
+{
+    if (amg) (void)hipStreamSynchronize(amg->ctx->c.stream);
+    delete amg;
+    return MFV_OK;
+}
+
+int mfv_amg_num_grids(mfv_amg amg, int* nGrids)
+{
+    if (!amg || !nGrids) return MFV_ERR_INVALID;
+    *nGrids = (int)amg->a.grids.size();
+    return MFV_OK;
+}
+
+int mfv_amg_grid_size(mfv_amg amg, int grid, int* nCells, int* nFaces)
+{
+    if (!amg || grid < 0 || grid >= (int)amg->a.grids.size()) return MFV_ERR_INVALID;
+    if (nCells) *nCells = amg->a.grids[grid].n;
+    if (nFaces) *nFaces = amg->a.grids[grid].nFaces;
+    return MFV_OK;
+}
+
+int mfv_amg_level_addressing(mfv_amg amg, int level, int* hMap, int* hCoarseOwner, int* hCoarseNeighbour,
+                             int* hFaceRestrict, signed char* hFaceFlip)
+{
+    if (!amg || level < 0 || level >= (int)amg->a.agg.size()) return MFV_ERR_INVALID;
+    const AggLevel& L = amg->a.agg[level];
+    if (hMap) std::copy(L.map.begin(), L.map.end(), hMap);
+    if (hCoarseOwner) std::copy(L.cOwn.begin(), L.cOwn.end(), hCoarseOwner);
+    if (hCoarseNeighbour) std::copy(L.cNei.begin(), L.cNei.end(), hCoarseNeighbour);
+    if (hFaceRestrict) std::copy(L.faceRestrict.begin(), L.faceRestrict.end(), hFaceRestrict);
+    if (hFaceFlip)
+        for (size_t i = 0; i < L.faceFlip.size(); ++i) hFaceFlip[i] = (signed char)L.faceFlip[i];
+    return MFV_OK;
+}
+
+namespace {
+
+Prec precOf(int p)
+{
+    switch (p)
+    {
+    case MFV_PREC_FP64: return Prec::F64;
+    case MFV_PREC_FP32: return Prec::F32;
+    case MFV_PREC_FP16: return Prec::F16;
+    }
+    throw Error(kErrUnsupported, "mfv: precision " + std::to_string(p) + " is not one of 64, 32, 16");
+}
+
+} // namespace
+
+int mfv_amg_galerkin(mfv_ctx ctx, mfv_amg amg, const double* dFineDiag, const double* dFineUpper,
+                     const double* dFineLower, int precision)
+{
+    return guarded(ctx, [&] {
+        need(ctx && amg && amg->ctx == ctx, "amg galerkin: bad context or hierarchy");
+        need(dFineDiag && dFineUpper, "amg galerkin: fine matrix missing");
+        if (dFineLower && dFineLower != dFineUpper)
+            throw Error(kErrUnsupported, "mfv: amg galerkin: the hierarchy carries symmetric matrices only (one "
+                                         "off-diagonal); pass lower == upper or NULL");
+        amgGalerkin(ctx->c, amg->a, dFineDiag, dFineUpper, precOf(precision));
+    });
+}
+
+int mfv_amg_grid_values(mfv_ctx ctx, mfv_amg amg, int grid, double* hDiag, double* hUpper)
+{
+    return guarded(ctx, [&] {
+        need(ctx && amg && amg->ctx == ctx, "amg grid values: bad context or hierarchy");
+        need(grid >= 1 && grid < (int)amg->a.grids.size(), "amg grid values: grid out of range");
+        need(amg->a.galerkinDone, "amg grid values: no Galerkin yet");
+        Grid& g = amg->a.grids[grid];
+        hipCheck(hipStreamSynchronize(ctx->c.stream), "sync");
+        if (hDiag) hipCheck(hipMemcpy(hDiag, g.diag64.p, sizeof(double) * g.n, hipMemcpyDeviceToHost), "diag");
+        if (hUpper && g.nFaces)
+            hipCheck(hipMemcpy(hUpper, g.upper64.p, sizeof(double) * g.nFaces, hipMemcpyDeviceToHost), "upper");
+    });
+}
+
+int mfv_amg_vcycle(mfv_ctx ctx, mfv_amg amg, const double* dR, double* dZ, int nPreSweeps, int nPostSweeps)
+{
+    return guarded(ctx, [&] {
+        need(ctx && amg && amg->ctx == ctx, "amg vcycle: bad context or hierarchy");
+        need(dR && dZ, "amg vcycle: null vector");
+        need(nPreSweeps >= 0 && nPostSweeps >= 0, "amg vcycle: negative sweeps");
+        const VcycleControls vc{nPreSweeps, nPostSweeps};
+        amgStartFrom(ctx->c, amg->a, dR, vc, nullptr);
+        const void* res = amgVcycle(ctx->c, amg->a, vc, nullptr);
+        amgResultToDouble(ctx->c, amg->a, res, dZ);
+    });
+}
+
+int mfv_pcg_default_controls(mfv_pcg_controls* c)
+{
+    if (!c) return MFV_ERR_INVALID;
+    c->tolerance = 1e-6;
+    c->relTol = 0.0;
+    c->maxIter = 1000;
+    c->minIter = 0;
+    c->precision = MFV_PREC_FP32;
+    c->nPreSweeps = 1;
+    c->nPostSweeps = 1;
+    c->flexible = 0;
+    c->smoother = 0;
+    c->checkEvery = 4;
+    c->nVcycles = 1;
+    c->scaleCorrection = 0;
+    return MFV_OK;
+}
+
+int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
+                      const double* dNormFactor, const mfv_pcg_controls* controls, mfv_perf* perf)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(amg && amg->ctx == ctx && amg->mesh == mesh, "pcg: hierarchy of another mesh or context");
+        need(dB && dPsi && controls, "pcg: null argument");
+        const LduView v = view(mesh, A);
+        if (v.lower != v.upper)
+            throw Error(kErrUnsupported, "mfv: AMG-PCG solves symmetric matrices (lower == upper)");
+        if (controls->smoother != 0)
+            throw Error(kErrUnsupported, "mfv: pcg: smoother " + std::to_string(controls->smoother) +
+                                             " is not carried (0 = weighted Jacobi)");
+        if (controls->nVcycles > 1) throw Error(kErrUnsupported, "mfv: pcg: nVcycles above 1 is not carried");
+        if (controls->scaleCorrection) throw Error(kErrUnsupported, "mfv: pcg: correction scaling is not carried");
+        PcgControls pc;
+        pc.tolerance = controls->tolerance;
+        pc.relTol = controls->relTol;
+        pc.maxIter = controls->maxIter;
+        pc.minIter = controls->minIter;
+        pc.prec = precOf(controls->precision);
+        pc.nPre = controls->nPreSweeps;
+        pc.nPost = controls->nPostSweeps;
+        pc.flexible = controls->flexible != 0;
+        pc.checkEvery = controls->checkEvery;
+        const PcgPerf r = amgPcgSolve(ctx->c, mesh->m, amg->a, v.diag, v.upper, dB, dPsi, dNormFactor, pc);
+        if (perf)
+        {
+            perf->initialResidual = r.initialResidual;
+            perf->finalResidual = r.finalResidual;
+            perf->nIterations = r.nIterations;
+            perf->converged = r.converged ? 1 : 0;
+        }
+    });
+}
+
+int mfv_interpolate(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dVol, double* dFace,
+                    double* dBFace)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dWeights && dVol && dFace, "interpolate: null argument");
+        launchInterpolate(ctx->c, mesh->m, dWeights, dVol, dFace, dBFace);
+    });
+}
+
+int mfv_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double* dPhiB, const double* dV,
+                   double sign, double* dSource)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPhi && dV && dSource, "div source: null argument");
+        launchDivSource(ctx->c, mesh->m, dPhi, dPhiB, dV, sign, dSource);
+    });
+}
+
+int mfv_flux_correct(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dPhiHbyA,
+                     double sign, double* dPhi)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPsi && dPhiHbyA && dPhi, "flux correct: null argument");
+        launchFluxCorrect(ctx->c, mesh->m, view(mesh, A), dPsi, dPhiHbyA, sign, dPhi);
+    });
+}
+
+int mfv_mesh_geometry(mfv_ctx ctx, mfv_mesh mesh, const double* dPoints, const int* dFaceOffsets,
+                      const int* dFaceVerts, double* dCf, double* dSf, double* dMagSf, double* dC, double* dV,
+                      double* dWeights, double* dDeltaCoeffs, double* dNonOrthDeltaCoeffs, double* dCorrVec,
+                      double* dBDeltaCoeffs)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPoints && dFaceOffsets && dFaceVerts && dCf && dSf && dMagSf && dC && dV, "geometry: null argument");
+        need(!mesh->m.nFaces || (dWeights && dDeltaCoeffs && dNonOrthDeltaCoeffs && dCorrVec),
+             "geometry: interpolation outputs missing");
+        need(!mesh->m.nBFaces || dBDeltaCoeffs, "geometry: boundary deltaCoeffs output missing");
+        launchGeometry(ctx->c, mesh->m, dPoints, dFaceOffsets, dFaceVerts, dCf, dSf, dMagSf, dC, dV, dWeights,
+                       dDeltaCoeffs, dNonOrthDeltaCoeffs, dCorrVec, dBDeltaCoeffs);
+    });
+}
+
+int mfv_ctx_profile(mfv_ctx ctx, int enable)
+{
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        ctx->c.profiling = enable != 0;
+        if ((int)ctx->c.prof.size() < kProfCount) ctx->c.prof.resize(kProfCount);
+    });
+}
+
+int mfv_ctx_profile_read(mfv_ctx ctx, int tag, int* count, double* totalMs)
+{
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        need(tag >= 0 && tag < kProfCount, "profile: unknown tag");
+        if ((int)ctx->c.prof.size() < kProfCount) ctx->c.prof.resize(kProfCount);
+        hipCheck(hipStreamSynchronize(ctx->c.stream), "profile sync");
+        double tot = 0.0;
+        auto& v = ctx->c.prof[tag];
+        for (auto& e : v)
+        {
+            float ms = 0.0f;
+            hipCheck(hipEventElapsedTime(&ms, e.first, e.second), "event elapsed");
+            tot += ms;
+            ctx->c.eventPool.push_back(e.first);
+            ctx->c.eventPool.push_back(e.second);
+        }
+        if (count) *count = (int)v.size();
+        if (totalMs) *totalMs = tot;
+        v.clear();
+    });
+}
+
+} // extern "C"
+
+namespace mfv {
+
+namespace {
+hipEvent_t takeEvent(Ctx& c)
+{
+    if (!c.eventPool.empty())
+    {
+        hipEvent_t e = c.eventPool.back();
+        c.eventPool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipCheck(hipEventCreate(&e), "event create");
+    return e;
+}
+} // namespace
+
+ProfScope::ProfScope(Ctx& ctx, int t) : c(ctx), tag(t)
+{
+    if (!c.profiling || tag < 0 || tag >= kProfCount) return;
+    e0 = takeEvent(c);
+    e1 = takeEvent(c);
+    hipCheck(hipEventRecord(e0, c.stream), "event record");
+}
+
+ProfScope::~ProfScope()
+{
+    if (!e0) return;
+    if (hipEventRecord(e1, c.stream) != hipSuccess) return;
+    if ((int)c.prof.size() < kProfCount) c.prof.resize(kProfCount);
+    c.prof[tag].emplace_back(e0, e1);
+}
+
+} // namespace mfv

@@ -1,0 +1,772 @@
+// AMG hierarchy: sliced-row layout build, Galerkin coarse matrices, and the V-cycle kernels in FP64,
+// FP32 and FP16 (scaled) precision.
+//
+// Reference semantics (brae's device path): Galerkin gathers in ascending fine index
+// (device_amg.cu:156-223, 238-284); V(nPre, nPost) weighted-Jacobi cycle with omega 0.8 and the
+// safeDiag floor, first pre-sweep from zero, injection restriction/prolongation, exact coarsest solve
+// (device_amg_vcycle.cu:982-1143 FP64, :1658-1766 FP32; amg_kernels.cuh:17-64, 257-282); the
+// coarsest grid solved with a precomputed dense inverse (the reference's default coarse kind,
+// device_amg_coarse_default.cuh:13) computed here by Gauss-Jordan with partial pivoting and the
+// reference's pivot floor 1e-14*max|diag| (device_amg_detail.cuh:93-94).
+#include "amg.hpp"
+#include "reduce.hpp"
+#include "sell_device.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+
+namespace mfv {
+
+namespace {
+
+constexpr int kTpb = 256;
+inline int blocks(long long n) { return (int)((n + kTpb - 1) / kTpb); }
+
+// ---------------- V-cycle kernels ----------------
+
+// r = b - A x  (amgSpmvF then residualT)
+template <typename Tm, typename Tv>
+__global__ __launch_bounds__(kTpb) void residualK(SellView S, const Tm* __restrict__ val, const Tv* __restrict__ diag,
+                                                  const double* __restrict__ scale, const Tv* __restrict__ x,
+                                                  const Tv* __restrict__ b, Tv* __restrict__ r,
+                                                  const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int c = blockIdx.x * kTpb + threadIdx.x;
+    if (c >= S.n) return;
+    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, PlainX<Tv>{x});
+    r[c] = b[c] - ax;
+}
+
+// xOut = x + omega*(b - A x)/D  (amgSpmvF then smoothT)
+template <typename Tm, typename Tv>
+__global__ __launch_bounds__(kTpb) void smoothK(SellView S, const Tm* __restrict__ val, const Tv* __restrict__ diag,
+                                                const double* __restrict__ scale, const Tv* __restrict__ x,
+                                                const Tv* __restrict__ b, Tv* __restrict__ xOut,
+                                                const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int c = blockIdx.x * kTpb + threadIdx.x;
+    if (c >= S.n) return;
+    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, PlainX<Tv>{x});
+    xOut[c] = x[c] + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+}
+
+// prolongation fused into the first post-sweep: xin = x + xc[map] (prolongT), then
+// xOut = xin + omega*(b - A xin)/D
+template <typename Tm, typename Tv>
+__global__ __launch_bounds__(kTpb) void prolongSmoothK(SellView S, const Tm* __restrict__ val,
+                                                       const Tv* __restrict__ diag, const double* __restrict__ scale,
+                                                       const int* __restrict__ map, const Tv* __restrict__ xc,
+                                                       const Tv* __restrict__ x, const Tv* __restrict__ b,
+                                                       Tv* __restrict__ xOut, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int c = blockIdx.x * kTpb + threadIdx.x;
+    if (c >= S.n) return;
+    ProlongedX<Tv> X{x, map, xc};
+    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X);
+    xOut[c] = X(c) + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+}
+
+// prolongation alone (no post-sweep): x += xc[map]
+template <typename Tv>
+__global__ __launch_bounds__(kTpb) void prolongK(int n, const int* __restrict__ map, const Tv* __restrict__ xc,
+                                                 Tv* __restrict__ x, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int c = blockIdx.x * kTpb + threadIdx.x;
+    if (c < n) x[c] += xc[map[c]];
+}
+
+// coarse rhs = sum of the fine residual over each agglomerate, ascending fine cell (restrictGatherT);
+// with fromZero the coarse grid's first sweep from zero x = 0 + omega*(b - 0)/D (smoothFromZeroT)
+template <typename Tv>
+__global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict__ cellStart,
+                                                  const int* __restrict__ cellList, const Tv* __restrict__ r,
+                                                  Tv* __restrict__ bc, const Tv* __restrict__ diagC,
+                                                  Tv* __restrict__ xc, int fromZero, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int C = blockIdx.x * kTpb + threadIdx.x;
+    if (C >= nc) return;
+    Tv s = Tv(0);
+    for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += r[cellList[k]];
+    bc[C] = s;
+    if (fromZero) xc[C] = Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(diagC[C]);
+}
+
+template <typename Tv>
+__global__ __launch_bounds__(kTpb) void zeroK(int n, Tv* __restrict__ x, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int c = blockIdx.x * kTpb + threadIdx.x;
+    if (c < n) x[c] = Tv(0);
+}
+
+// grid 0 entry from the FP64 residual: b = (Tv) r, x = omega*b/D when pre-smoothing from zero
+template <typename Tv>
+__global__ __launch_bounds__(kTpb) void startK(int n, const double* __restrict__ r, const Tv* __restrict__ diag,
+                                               Tv* __restrict__ b, Tv* __restrict__ x, int fromZero,
+                                               const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int c = blockIdx.x * kTpb + threadIdx.x;
+    if (c >= n) return;
+    const Tv bv = (Tv)r[c];
+    b[c] = bv;
+    x[c] = fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(diag[c]) : Tv(0);
+}
+
+// coarsest grid: x = Ainv * b in FP64, ascending column per row (one workgroup)
+template <typename Tv>
+__global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __restrict__ inv, const Tv* __restrict__ b,
+                                                     Tv* __restrict__ x, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    __shared__ double sb[1024];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = (double)b[j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+    {
+        double s = 0.0;
+        const double* row = inv + (size_t)i * n;
+        for (int j = 0; j < n; ++j) s += row[j] * sb[j];
+        x[i] = (Tv)s;
+    }
+}
+
+// ---------------- Galerkin and fill kernels ----------------
+
+// coarse diag = sum(fine diag, ascending fine cell) + sum(upper + lower of the fine faces inside the
+// agglomerate, ascending) (galDiagGatherK)
+__global__ __launch_bounds__(kTpb) void galDiagK(int nc, const int* __restrict__ cellStart,
+                                                 const int* __restrict__ cellList,
+                                                 const int* __restrict__ dfaceStart,
+                                                 const int* __restrict__ dfaceList, const double* __restrict__ fd,
+                                                 const double* __restrict__ fu, double* __restrict__ cd)
+{
+    const int C = blockIdx.x * kTpb + threadIdx.x;
+    if (C >= nc) return;
+    double s = 0.0;
+    for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += fd[cellList[k]];
+    for (int k = dfaceStart[C]; k < dfaceStart[C + 1]; ++k)
+    {
+        const int f = dfaceList[k];
+        s += fu[f] + fu[f];
+    }
+    cd[C] = s;
+}
+
+// coarse face = sum of its fine faces, ascending (galFaceGatherSymK)
+__global__ __launch_bounds__(kTpb) void galFaceK(int ncf, const int* __restrict__ faceStart,
+                                                 const int* __restrict__ faceList, const double* __restrict__ fu,
+                                                 double* __restrict__ cu)
+{
+    const int F = blockIdx.x * kTpb + threadIdx.x;
+    if (F >= ncf) return;
+    double u = 0.0;
+    for (int k = faceStart[F]; k < faceStart[F + 1]; ++k) u += fu[faceList[k]];
+    cu[F] = u;
+}
+
+// sliced-row copy of a grid's face values in precision Tm, scaled by 1/s (s from the device)
+template <typename Tm>
+__global__ __launch_bounds__(kTpb) void fillK(int nf, const int* __restrict__ slotLo, const int* __restrict__ slotHi,
+                                              const double* __restrict__ up, const double* __restrict__ lo,
+                                              const double* __restrict__ scale, Tm* __restrict__ val)
+{
+    const int f = blockIdx.x * kTpb + threadIdx.x;
+    if (f >= nf) return;
+    const double inv = scale ? 1.0 / *scale : 1.0;
+    if constexpr (sizeof(Tm) == 2)
+    {
+        val[slotLo[f]] = __float2half((float)(up[f] * inv));
+        val[slotHi[f]] = __float2half((float)(lo[f] * inv));
+    }
+    else
+    {
+        val[slotLo[f]] = (Tm)up[f];
+        val[slotHi[f]] = (Tm)lo[f];
+    }
+}
+
+template <typename Tv>
+__global__ __launch_bounds__(kTpb) void castK(int n, const double* __restrict__ a, Tv* __restrict__ b)
+{
+    const int i = blockIdx.x * kTpb + threadIdx.x;
+    if (i < n) b[i] = (Tv)a[i];
+}
+
+__global__ __launch_bounds__(kTpb) void widenK(int n, const float* __restrict__ a, double* __restrict__ b)
+{
+    const int i = blockIdx.x * kTpb + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+// per-grid FP16 scale: the smallest power of two >= max|upper| (exact scaling)
+__global__ __launch_bounds__(kReduceBlock) void maxAbsPow2K(int n, const double* __restrict__ a,
+                                                           double* __restrict__ partials,
+                                                           unsigned int* __restrict__ ticket, double* __restrict__ out)
+{
+    __shared__ double lds[kReduceBlock / 64];
+    double v = 0.0;
+    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock) v = fmax(v, fabs(a[i]));
+    // max combine: butterfly max, then fixed wave order
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double m = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kReduceBlock / 64; ++w) m = fmax(m, lds[w]);
+    __syncthreads();
+    __shared__ bool last;
+    if (threadIdx.x == 0)
+    {
+        partials[blockIdx.x] = m;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0)
+    {
+        double mm = 0.0;
+        for (int i = 0; i < (int)gridDim.x; ++i) mm = fmax(mm, partials[i]);
+        *out = mm > 0.0 ? exp2(ceil(log2(mm))) : 1.0;
+        *ticket = 0u;
+    }
+}
+
+// Gauss-Jordan inverse with partial pivoting (first largest |a_ik|, i >= k) on [A | I], the
+// reference's pivot floor 1e-14*max|diag|. One workgroup; every element update is the sequential
+// algorithm's own, so the result is bit-identical to a one-thread loop.
+__global__ __launch_bounds__(kTpb) void coarseInverseK(int n, int nf, const int* __restrict__ own,
+                                                       const int* __restrict__ nei, const double* __restrict__ diag,
+                                                       const double* __restrict__ upper, double* __restrict__ M,
+                                                       double* __restrict__ inv)
+{
+    const int tid = threadIdx.x;
+    const int w = 2 * n;
+    __shared__ double fcol[1024];
+    __shared__ double shScale, shPiv;
+    __shared__ int shRow;
+    for (int i = tid; i < n * w; i += kTpb) M[i] = 0.0;
+    __syncthreads();
+    for (int c = tid; c < n; c += kTpb)
+    {
+        M[c * w + c] = diag[c];
+        M[c * w + n + c] = 1.0;
+    }
+    __syncthreads();
+    if (tid == 0)
+    {
+        for (int f = 0; f < nf; ++f)
+        {
+            M[own[f] * w + nei[f]] += upper[f];
+            M[nei[f] * w + own[f]] += upper[f];
+        }
+        double m = 0.0;
+        for (int c = 0; c < n; ++c) m = fmax(m, fabs(diag[c]));
+        shScale = m > 0.0 ? m : 1.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k)
+    {
+        if (tid == 0)
+        {
+            double bv = -1.0;
+            int bi = k;
+            for (int i = k; i < n; ++i)
+            {
+                const double v = fabs(M[i * w + k]);
+                if (v > bv)
+                {
+                    bv = v;
+                    bi = i;
+                }
+            }
+            shRow = bi;
+        }
+        __syncthreads();
+        const int p = shRow;
+        if (p != k)
+            for (int j = tid; j < w; j += kTpb)
+            {
+                const double t = M[k * w + j];
+                M[k * w + j] = M[p * w + j];
+                M[p * w + j] = t;
+            }
+        __syncthreads();
+        if (tid == 0)
+        {
+            double d = M[k * w + k];
+            const double fl = 1e-14 * shScale;
+            if (!(fabs(d) > fl)) d = d < 0.0 ? -fl : fl;
+            M[k * w + k] = d;
+            shPiv = d;
+        }
+        __syncthreads();
+        const double rp = 1.0 / shPiv;
+        for (int j = tid; j < w; j += kTpb) M[k * w + j] *= rp;
+        __syncthreads();
+        for (int i = tid; i < n; i += kTpb) fcol[i] = M[i * w + k];
+        __syncthreads();
+        for (int idx = tid; idx < n * w; idx += kTpb)
+        {
+            const int i = idx / w, j = idx - i * w;
+            if (i == k) continue;
+            M[i * w + j] -= fcol[i] * M[k * w + j];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < n * n; idx += kTpb)
+    {
+        const int i = idx / n, j = idx - i * n;
+        inv[idx] = M[i * w + n + j];
+    }
+}
+
+// y = A x in FP64 on the sliced rows (the fine grid's FP64 copy)
+__global__ __launch_bounds__(kTpb) void sellAmul64K(SellView S, const double* __restrict__ val,
+                                                    const double* __restrict__ diag, const double* __restrict__ x,
+                                                    double* __restrict__ y)
+{
+    const int c = blockIdx.x * kTpb + threadIdx.x;
+    if (c >= S.n) return;
+    y[c] = rowOffSum<double, double>(S, c, val, PlainX<double>{x}, diag[c] * x[c]);
+}
+
+// ---------------- host side ----------------
+
+void buildSell(Ctx& c, Sell& S, int n, const std::vector<int>& own, const std::vector<int>& nei)
+{
+    const int nF = (int)own.size();
+    std::vector<int> os(n + 1, 0), ls(n + 1, 0), lo(nF);
+    for (int f = 0; f < nF; ++f)
+    {
+        ++os[own[f] + 1];
+        ++ls[nei[f] + 1];
+    }
+    for (int i = 0; i < n; ++i)
+    {
+        os[i + 1] += os[i];
+        ls[i + 1] += ls[i];
+    }
+    {
+        std::vector<int> at(ls.begin(), ls.end() - 1);
+        for (int f = 0; f < nF; ++f) lo[at[nei[f]]++] = f;
+    }
+    const int nSl = (n + kSlice - 1) / kSlice;
+    std::vector<int> slicePtr(nSl + 1, 0);
+    int width = 0;
+    for (int s = 0; s < nSl; ++s)
+    {
+        int wmax = 0;
+        for (int r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r)
+            wmax = std::max(wmax, (ls[r + 1] - ls[r]) + (os[r + 1] - os[r]));
+        width = std::max(width, wmax);
+        const long long next = (long long)slicePtr[s] + (long long)wmax * kSlice;
+        if (next > 0x7fffffffLL) invalid("sliced rows exceed 2^31 entries");
+        slicePtr[s + 1] = (int)next;
+    }
+    const long long nnz = slicePtr[nSl];
+    std::vector<int> cols(nnz), slotLo(nF), slotHi(nF);
+    std::vector<uint8_t> pat(n, 0);
+    std::map<std::vector<int>, int> patterns;
+    bool patterned = true;
+    std::vector<int> offs;
+    for (int r = 0; r < n; ++r)
+    {
+        const int s = r / kSlice, lane = r % kSlice;
+        const int w = (slicePtr[s + 1] - slicePtr[s]) / kSlice;
+        int k = 0;
+        offs.clear();
+        for (int q = ls[r]; q < ls[r + 1]; ++q, ++k)
+        {
+            const int f = lo[q];
+            const int e = slicePtr[s] + k * kSlice + lane;
+            cols[e] = own[f];
+            slotHi[f] = e;
+            offs.push_back(own[f] - r);
+        }
+        for (int f = os[r]; f < os[r + 1]; ++f, ++k)
+        {
+            const int e = slicePtr[s] + k * kSlice + lane;
+            cols[e] = nei[f];
+            slotLo[f] = e;
+            offs.push_back(nei[f] - r);
+        }
+        for (; k < w; ++k) cols[slicePtr[s] + k * kSlice + lane] = r;
+        if (patterned)
+        {
+            auto it = patterns.find(offs);
+            if (it == patterns.end())
+            {
+                if (patterns.size() >= 255)
+                    patterned = false;
+                else
+                    it = patterns.emplace(offs, (int)patterns.size()).first;
+            }
+            if (patterned) pat[r] = (uint8_t)it->second;
+        }
+    }
+    // rows of the last slice beyond n: padding points at row 0 (never read: threads exit)
+    for (int r = n; r < nSl * kSlice; ++r)
+    {
+        const int s = r / kSlice, lane = r % kSlice;
+        const int w = (slicePtr[s + 1] - slicePtr[s]) / kSlice;
+        for (int k = 0; k < w; ++k) cols[slicePtr[s] + k * kSlice + lane] = 0;
+    }
+    S.n = n;
+    S.nSlices = nSl;
+    S.width = width;
+    S.nnz = nnz;
+    S.patterned = patterned && width > 0;
+    S.slicePtr.upload(slicePtr, c.stream);
+    if (S.patterned)
+    {
+        std::vector<int> table(patterns.size() * (size_t)width, 0);
+        for (const auto& kv : patterns)
+            for (size_t k = 0; k < kv.first.size(); ++k) table[(size_t)kv.second * width + k] = kv.first[k];
+        S.pattern.upload(pat, c.stream);
+        S.patOff.upload(table, c.stream);
+    }
+    else
+    {
+        S.cols.upload(cols, c.stream);
+    }
+    S.slotLo.upload(slotLo, c.stream);
+    S.slotHi.upload(slotHi, c.stream);
+    hipCheck(hipStreamSynchronize(c.stream), "sell upload");
+}
+
+template <typename T>
+void zeroBuf(Ctx& c, DBuf<T>& b, size_t n)
+{
+    b.resize(n);
+    if (n) hipCheck(hipMemsetAsync(b.p, 0, n * sizeof(T), c.stream), "zero");
+}
+
+void restrictionLists(Ctx& c, Grid& G, const AggLevel& L)
+{
+    const int nC = L.nCoarse, nCF = L.nCoarseFaces, nF = L.nFineFaces;
+    std::vector<int> cs(nC + 1, 0), cl(L.nFine), ds(nC + 1, 0), fs(nCF + 1, 0);
+    for (int i = 0; i < L.nFine; ++i) ++cs[L.map[i] + 1];
+    for (int i = 0; i < nC; ++i) cs[i + 1] += cs[i];
+    {
+        std::vector<int> at(cs.begin(), cs.end() - 1);
+        for (int i = 0; i < L.nFine; ++i) cl[at[L.map[i]]++] = i;
+    }
+    for (int f = 0; f < nF; ++f)
+    {
+        const int r = L.faceRestrict[f];
+        if (r >= 0)
+            ++fs[r + 1];
+        else
+            ++ds[-1 - r + 1];
+    }
+    for (int i = 0; i < nC; ++i) ds[i + 1] += ds[i];
+    for (int i = 0; i < nCF; ++i) fs[i + 1] += fs[i];
+    std::vector<int> dl(ds[nC]), fl(fs[nCF]);
+    {
+        std::vector<int> dat(ds.begin(), ds.end() - 1), fat(fs.begin(), fs.end() - 1);
+        for (int f = 0; f < nF; ++f)
+        {
+            const int r = L.faceRestrict[f];
+            if (r >= 0)
+                fl[fat[r]++] = f;
+            else
+                dl[dat[-1 - r]++] = f;
+        }
+    }
+    G.map.upload(L.map, c.stream);
+    G.cellStart.upload(cs, c.stream);
+    G.cellList.upload(cl, c.stream);
+    G.dfaceStart.upload(ds, c.stream);
+    G.dfaceList.upload(dl, c.stream);
+    G.faceStart.upload(fs, c.stream);
+    G.faceList.upload(fl, c.stream);
+}
+
+template <typename Tm>
+DBuf<Tm>& offOf(Grid& g);
+template <>
+DBuf<double>& offOf<double>(Grid& g) { return g.off64; }
+template <>
+DBuf<float>& offOf<float>(Grid& g) { return g.off32; }
+template <>
+DBuf<__half>& offOf<__half>(Grid& g) { return g.off16; }
+
+template <typename Tv>
+struct VecSet
+{
+    Tv *b, *x, *y, *r;
+    const Tv* diag;
+};
+
+VecSet<double> vecs64(Grid& g) { return {g.b64.p, g.x64.p, g.y64.p, g.r64.p, g.diagV64.p}; }
+VecSet<float> vecs32(Grid& g) { return {g.b32.p, g.x32.p, g.y32.p, g.r32.p, g.diagV32.p}; }
+
+// Runs the cycle from grid gi down and back; returns the buffer holding grid gi's result (x or y:
+// the ping-pong sweeps leave it in either, and the caller reads it where it is).
+template <typename Tm, typename Tv>
+Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, VecSet<Tv> (*vs)(Grid&))
+{
+    Grid& g = a.grids[gi];
+    VecSet<Tv> v = vs(g);
+    const int n = g.n;
+    const int last = (int)a.grids.size() - 1;
+    if (gi == last)
+    {
+        coarseApplyK<Tv><<<1, kTpb, 0, c.stream>>>(n, a.coarseInv.p, v.b, v.x, stop);
+        return v.x;
+    }
+    const SellView S = viewOf(g.sell);
+    const Tm* val = offOf<Tm>(g).p;
+    const double* sc = a.scales.p + gi;
+    // pre-sweeps after the first (the first ran from zero where b was made)
+    for (int s = 1; s < vc.nPre; ++s)
+    {
+        smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, stop);
+        std::swap(v.x, v.y);
+    }
+    if (vc.nPre == 0) zeroK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, v.x, stop);
+    {
+        ProfScope ps(c, gi == 0 ? kProfAmgResidual0 : kProfCount);
+        residualK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.r, stop);
+    }
+    Grid& gc = a.grids[gi + 1];
+    VecSet<Tv> w = vs(gc);
+    const int coarseIsLast = (gi + 1 == last);
+    {
+        ProfScope ps(c, gi == 0 ? kProfAmgRestrict0 : kProfCount);
+        restrictK<Tv><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, v.r, w.b, w.diag, w.x,
+                                                            (!coarseIsLast && vc.nPre > 0) ? 1 : 0, stop);
+    }
+    const Tv* xc = vcycleAt<Tm, Tv>(c, a, gi + 1, vc, stop, vs);
+    if (vc.nPost > 0)
+    {
+        {
+            ProfScope ps(c, gi == 0 ? kProfAmgProlongSmooth0 : kProfCount);
+            prolongSmoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, g.map.p, xc, v.x, v.b, v.y,
+                                                                      stop);
+        }
+        std::swap(v.x, v.y);
+        for (int s = 1; s < vc.nPost; ++s)
+        {
+            smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, stop);
+            std::swap(v.x, v.y);
+        }
+    }
+    else
+    {
+        prolongK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, g.map.p, xc, v.x, stop);
+    }
+    hipCheck(hipGetLastError(), "vcycle");
+    return v.x;
+}
+
+template <typename Tv>
+void allocVectors(Ctx& c, Grid& g);
+template <>
+void allocVectors<double>(Ctx& c, Grid& g)
+{
+    zeroBuf(c, g.b64, g.n);
+    zeroBuf(c, g.x64, g.n);
+    zeroBuf(c, g.y64, g.n);
+    zeroBuf(c, g.r64, g.n);
+}
+template <>
+void allocVectors<float>(Ctx& c, Grid& g)
+{
+    zeroBuf(c, g.b32, g.n);
+    zeroBuf(c, g.x32, g.n);
+    zeroBuf(c, g.y32, g.n);
+    zeroBuf(c, g.r32, g.n);
+}
+
+} // namespace
+
+void amgBuild(Ctx& c, const Mesh& m, Amg& a, const std::vector<double>& faceWeights, const AmgBuildControls& ctl)
+{
+    if ((int)faceWeights.size() != m.nFaces) invalid("amg build: one face weight per internal face is required");
+    if (ctl.target < 1) invalid("amg build: the coarsest-level target must be positive");
+    bool forward = true;
+    a.nFine = m.nCells;
+    a.agg = agglomerate(ctl.kind, m.nCells, m.hOwner, m.hNeighbour, faceWeights, ctl.target, forward);
+    const int nG = (int)a.agg.size() + 1;
+    a.grids.clear();
+    a.grids.resize(nG);
+    for (int gi = 0; gi < nG; ++gi)
+    {
+        Grid& g = a.grids[gi];
+        const std::vector<int>& own = gi == 0 ? m.hOwner : a.agg[gi - 1].cOwn;
+        const std::vector<int>& nei = gi == 0 ? m.hNeighbour : a.agg[gi - 1].cNei;
+        g.n = gi == 0 ? m.nCells : a.agg[gi - 1].nCoarse;
+        g.nFaces = (int)own.size();
+        g.hOwn = own;
+        g.hNei = nei;
+        if (gi < nG - 1) restrictionLists(c, g, a.agg[gi]);
+        if (gi > 0)
+        {
+            g.diag64.resize(g.n);
+            g.upper64.resize(g.nFaces);
+        }
+        if (gi < nG - 1 || gi == 0) buildSell(c, g.sell, g.n, own, nei);
+    }
+    a.nCoarsest = a.grids.back().n;
+    if (a.nCoarsest > 1024) invalid("amg build: the coarsest grid has " + std::to_string(a.nCoarsest) +
+                                    " cells; the dense coarsest solve takes at most 1024");
+    Grid& gl = a.grids.back();
+    gl.diag64.resize(gl.n);
+    gl.upper64.resize(gl.nFaces);
+    gl.denseOwn.upload(gl.hOwn, c.stream);
+    gl.denseNei.upload(gl.hNei, c.stream);
+    a.coarseInv.resize((size_t)a.nCoarsest * a.nCoarsest);
+    a.coarseWork.resize((size_t)a.nCoarsest * 2 * a.nCoarsest);
+    a.scales.resize(nG);
+    std::vector<double> ones(nG, 1.0);
+    a.scales.upload(ones, c.stream);
+    a.galerkinDone = false;
+    a.allocatedVectors = false;
+    hipCheck(hipStreamSynchronize(c.stream), "amg build");
+}
+
+void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper, Prec prec)
+{
+    const int nG = (int)a.grids.size();
+    if (nG < 2) invalid("amg: the mesh is too small for a coarse grid");
+    for (int gi = 0; gi + 1 < nG; ++gi)
+    {
+        Grid& g = a.grids[gi];
+        Grid& gc = a.grids[gi + 1];
+        const double* fd = gi == 0 ? fineDiag : g.diag64.p;
+        const double* fu = gi == 0 ? fineUpper : g.upper64.p;
+        galDiagK<<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, g.dfaceStart.p,
+                                                       g.dfaceList.p, fd, fu, gc.diag64.p);
+        if (gc.nFaces)
+            galFaceK<<<blocks(gc.nFaces), kTpb, 0, c.stream>>>(gc.nFaces, g.faceStart.p, g.faceList.p, fu,
+                                                                gc.upper64.p);
+    }
+    hipCheck(hipGetLastError(), "galerkin");
+    // V-cycle copies of every non-coarsest grid in the working precision
+    for (int gi = 0; gi + 1 < nG; ++gi)
+    {
+        Grid& g = a.grids[gi];
+        const double* d = gi == 0 ? fineDiag : g.diag64.p;
+        const double* u = gi == 0 ? fineUpper : g.upper64.p;
+        const long long nnz = g.sell.nnz;
+        if (prec == Prec::F64 || gi == 0)
+        {
+            if (g.off64.n != (size_t)nnz) zeroBuf(c, g.off64, nnz);
+            if (g.nFaces)
+                fillK<double><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.sell.slotLo.p, g.sell.slotHi.p, u, u,
+                                                                       nullptr, g.off64.p);
+        }
+        if (prec == Prec::F64)
+        {
+            g.diagV64.resize(g.n);
+            hipCheck(hipMemcpyAsync(g.diagV64.p, d, sizeof(double) * g.n, hipMemcpyDeviceToDevice, c.stream), "diag");
+        }
+        else
+        {
+            g.diagV32.resize(g.n);
+            castK<float><<<blocks(g.n), kTpb, 0, c.stream>>>(g.n, d, g.diagV32.p);
+            if (prec == Prec::F32)
+            {
+                if (g.off32.n != (size_t)nnz) zeroBuf(c, g.off32, nnz);
+                if (g.nFaces)
+                    fillK<float><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.sell.slotLo.p, g.sell.slotHi.p, u,
+                                                                          u, nullptr, g.off32.p);
+            }
+            else
+            {
+                if (g.off16.n != (size_t)nnz) zeroBuf(c, g.off16, nnz);
+                if (g.nFaces)
+                {
+                    maxAbsPow2K<<<reduceGrid(g.nFaces), kReduceBlock, 0, c.stream>>>(
+                        g.nFaces, u, c.partials.p + 4 * (size_t)kMaxReduceBlocks, c.tickets.p + 4, a.scales.p + gi);
+                    fillK<__half><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.sell.slotLo.p, g.sell.slotHi.p,
+                                                                           u, u, a.scales.p + gi, g.off16.p);
+                }
+            }
+        }
+    }
+    // coarsest: vector-precision diag (the dense solve reads FP64 face values), dense inverse
+    Grid& gl = a.grids.back();
+    if (prec == Prec::F64)
+    {
+        gl.diagV64.resize(gl.n);
+        hipCheck(hipMemcpyAsync(gl.diagV64.p, gl.diag64.p, sizeof(double) * gl.n, hipMemcpyDeviceToDevice, c.stream),
+                 "coarse diag");
+    }
+    else
+    {
+        gl.diagV32.resize(gl.n);
+        castK<float><<<blocks(gl.n), kTpb, 0, c.stream>>>(gl.n, gl.diag64.p, gl.diagV32.p);
+    }
+    coarseInverseK<<<1, kTpb, 0, c.stream>>>(gl.n, gl.nFaces, gl.denseOwn.p, gl.denseNei.p, gl.diag64.p, gl.upper64.p,
+                                             a.coarseWork.p, a.coarseInv.p);
+    hipCheck(hipGetLastError(), "coarsest inverse");
+    if (!a.allocatedVectors || a.vectorsFor != prec)
+    {
+        for (Grid& g : a.grids)
+        {
+            if (prec == Prec::F64)
+                allocVectors<double>(c, g);
+            else
+                allocVectors<float>(c, g);
+        }
+        a.allocatedVectors = true;
+        a.vectorsFor = prec;
+    }
+    a.prec = prec;
+    a.galerkinDone = true;
+}
+
+void amgStartFrom(Ctx& c, Amg& a, const double* r, const VcycleControls& vc, const int* stop)
+{
+    Grid& g = a.grids[0];
+    const int fromZero = vc.nPre > 0 ? 1 : 0;
+    if (a.prec == Prec::F64)
+        startK<double><<<blocks(g.n), kTpb, 0, c.stream>>>(g.n, r, g.diagV64.p, g.b64.p, g.x64.p, fromZero, stop);
+    else
+        startK<float><<<blocks(g.n), kTpb, 0, c.stream>>>(g.n, r, g.diagV32.p, g.b32.p, g.x32.p, fromZero, stop);
+    hipCheck(hipGetLastError(), "vcycle start");
+}
+
+void* amgVcycle(Ctx& c, Amg& a, const VcycleControls& vc, const int* stop)
+{
+    if (!a.galerkinDone) throw Error(kErrState, "mfv: amg_galerkin must run before a V-cycle");
+    switch (a.prec)
+    {
+    case Prec::F64: return vcycleAt<double, double>(c, a, 0, vc, stop, vecs64);
+    case Prec::F32: return vcycleAt<float, float>(c, a, 0, vc, stop, vecs32);
+    case Prec::F16: return vcycleAt<__half, float>(c, a, 0, vc, stop, vecs32);
+    }
+    return nullptr;
+}
+
+void amgResultToDouble(Ctx& c, Amg& a, const void* res, double* z)
+{
+    const int n = a.grids[0].n;
+    if (a.prec == Prec::F64)
+        hipCheck(hipMemcpyAsync(z, res, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream), "result copy");
+    else
+        widenK<<<blocks(n), kTpb, 0, c.stream>>>(n, static_cast<const float*>(res), z);
+    hipCheck(hipGetLastError(), "result");
+}
+
+void sellAmul64(Ctx& c, const Grid& g, const double* diag, const double* x, double* y)
+{
+    sellAmul64K<<<blocks(g.n), kTpb, 0, c.stream>>>(viewOf(g.sell), g.off64.p, diag, x, y);
+    hipCheck(hipGetLastError(), "sell amul");
+}
+
+} // namespace mfv

@@ -1,0 +1,151 @@
+// The AMG hierarchy on the device and the preconditioned-CG solver built on it.
+//
+// Layout. Every grid stores its off-diagonal coefficients ROW-WISE in a sliced-ELL layout: slices
+// of 64 rows (one wavefront), each slice padded to its longest row, entries column-major inside the
+// slice so lane l of a wavefront reads row 64*s + l and a wave's loads are contiguous. A row's entries
+// are in ascending FACE order (its neighbour-side faces, then the faces it owns), so a row product
+// sums in the order the reference's face loop does. Column indices are either explicit (int32 per
+// entry) or, when the grid has at most 255 distinct row patterns (a structured hex mesh has 27), a
+// one-byte pattern id per row that selects a table of column offsets -- index traffic drops from
+// 4 bytes per entry to 1 byte per row.
+//
+// The FP64 face-form values of every coarse grid (upper per coarse face, diag per coarse cell) are
+// kept as the Galerkin masters (brae's amgGalerkin, device_amg.cu:1693-1846); the sliced rows hold a
+// copy in the V-cycle's precision (FP64, FP32, or FP16 scaled by a per-grid power of two).
+#pragma once
+
+#include "agglom.hpp"
+#include "mfv_internal.hpp"
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace mfv {
+
+constexpr int kSlice = 64;
+constexpr double kOmega = 0.8;   // weighted-Jacobi relaxation (device_amg_detail.cuh:39)
+
+enum class Prec : int { F64 = 64, F32 = 32, F16 = 16 };
+
+// Sliced-ELL structure of one grid.
+struct Sell
+{
+    int n = 0;
+    int nSlices = 0;
+    int width = 0;               // widest row
+    long long nnz = 0;           // padded entries
+    bool patterned = false;
+    DBuf<int> slicePtr;          // [nSlices+1] first entry of each slice
+    DBuf<int> cols;              // [nnz] explicit columns (padding: the row itself), when !patterned
+    DBuf<uint8_t> pattern;       // [n] row pattern id, when patterned
+    DBuf<int> patOff;            // [nPatterns*width] column offsets (padding 0)
+    DBuf<int> slotLo;            // [nFaces] entry of face f in its owner's row (value upper[f])
+    DBuf<int> slotHi;            // [nFaces] entry of face f in its neighbour's row (value lower[f])
+};
+
+struct Grid
+{
+    int n = 0;
+    int nFaces = 0;
+    Sell sell;
+    // FP64 face-form values (coarse grids only; grid 0 reads the caller's matrix)
+    DBuf<double> diag64, upper64;
+    // V-cycle copies
+    DBuf<double> off64;
+    DBuf<float> off32;
+    DBuf<__half> off16;
+    DBuf<double> diagV64;
+    DBuf<float> diagV32;
+    // V-cycle vectors in the vector precision (double for the FP64 cycle, float otherwise)
+    DBuf<double> b64, x64, y64, r64;
+    DBuf<float> b32, x32, y32, r32;
+    // restriction/prolongation to the next grid (absent on the coarsest)
+    DBuf<int> map;                       // [n] fine -> coarse
+    DBuf<int> cellStart, cellList;       // coarse cell <- fine cells, ascending
+    DBuf<int> dfaceStart, dfaceList;     // coarse cell <- fine faces inside it, ascending
+    DBuf<int> faceStart, faceList;       // coarse face <- fine faces, ascending
+    DBuf<int> denseOwn, denseNei;        // coarsest grid: addressing for the dense assembly
+    // host copies for inspection
+    std::vector<int> hOwn, hNei;
+};
+
+struct AmgBuildControls
+{
+    AggKind kind = AggKind::Brae;
+    int target = 64;
+};
+
+// Work vectors and device state of the PCG solve on a hierarchy (FP64, one per cell).
+struct PcgWork
+{
+    DBuf<double> r, p0, p1, q, rOld, st;
+    DBuf<int> fl;
+    DBuf<double> seqA, seqB;   // per-element terms for sequential (OpenFOAM-order) reductions
+};
+
+struct Amg
+{
+    int nFine = 0;
+    std::vector<AggLevel> agg;     // host addressing, agg[i]: grid i -> grid i+1
+    std::vector<Grid> grids;       // grids[0] = the mesh
+    int nCoarsest = 0;
+    DBuf<double> coarseInv;        // dense inverse of the coarsest grid, row-major, FP64
+    DBuf<double> coarseWork;       // Gauss-Jordan augmented matrix
+    DBuf<double> scales;           // per grid: FP16 scale (1 otherwise)
+    Prec prec = Prec::F32;         // precision the V-cycle copies are filled in
+    bool galerkinDone = false;
+    bool allocatedVectors = false;
+    Prec vectorsFor = Prec::F32;
+    std::unique_ptr<PcgWork> work;
+};
+
+// Build the hierarchy (host agglomeration + device structures).
+void amgBuild(Ctx& c, const Mesh& m, Amg& a, const std::vector<double>& faceWeights, const AmgBuildControls& ctl);
+// Coarse matrices from the fine FP64 matrix (symmetric: lower == upper), V-cycle copies in `prec`,
+// the coarsest inverse.
+void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper, Prec prec);
+
+struct VcycleControls
+{
+    int nPre = 1, nPost = 1;
+};
+
+// One V-cycle on grid 0's b (set by amgStartFrom, with x = omega*b/D when nPre > 0). Returns the
+// device buffer holding the result (double* for the FP64 cycle, float* otherwise). `stop` (device
+// int, may be null) makes every kernel return at once when nonzero.
+void* amgVcycle(Ctx& c, Amg& a, const VcycleControls& vc, const int* stop);
+
+// z (FP64) = the V-cycle result `res` (as amgVcycle returned it)
+void amgResultToDouble(Ctx& c, Amg& a, const void* res, double* z);
+
+// Fill grid 0's V-cycle vectors from an FP64 residual: b0 = r, x0 = omega*b0/D0 (pre-smooth from zero).
+void amgStartFrom(Ctx& c, Amg& a, const double* r, const VcycleControls& vc, const int* stop);
+
+// Row product on a grid's sliced rows in FP64 (the fine grid's FP64 copy): y = A x.
+void sellAmul64(Ctx& c, const Grid& g, const double* diag, const double* x, double* y);
+
+// ---- PCG ----
+struct PcgControls
+{
+    double tolerance = 1e-6, relTol = 0.0;
+    int maxIter = 1000, minIter = 0;
+    Prec prec = Prec::F32;
+    int nPre = 1, nPost = 1;
+    bool flexible = false;
+    int checkEvery = 4;
+};
+
+struct PcgPerf
+{
+    double initialResidual = 0, finalResidual = 0;
+    int nIterations = 0;
+    bool converged = false;
+};
+
+// AMG-preconditioned CG on the folded fine matrix (diag, upper symmetric). psi updated in place.
+// normFactor: device scalar, or null to compute OpenFOAM's normFactor here.
+PcgPerf amgPcgSolve(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* upper, const double* b,
+                    double* psi, const double* normFactor, const PcgControls& pc);
+
+} // namespace mfv

@@ -1,0 +1,205 @@
+// Internal declarations shared by the mfv library's translation units: error handling, device
+// buffers, the context (one HIP stream + scratch), the mesh's LDU addressing and the kernel
+// launchers each .hip file provides. Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mfv {
+
+using label = int32_t;
+using scalar = double;
+
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum : int
+{
+    kOk = 0,
+    kErrInvalid = 1,
+    kErrHip = 2,
+    kErrUnsupported = 3,
+    kErrAlloc = 4,
+    kErrState = 5,
+};
+
+inline void hipCheck(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw Error(kErrHip, std::string("mfv hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+[[noreturn]] inline void invalid(const std::string& m) { throw Error(kErrInvalid, "mfv: " + m); }
+
+// Device allocation owned by the library. Moves, never copies.
+template <typename T>
+struct DBuf
+{
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { resize(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void resize(size_t count)
+    {
+        if (count == n) return;
+        release();
+        if (count)
+        {
+            if (hipMalloc(&p, count * sizeof(T)) != hipSuccess)
+            {
+                p = nullptr;
+                throw Error(kErrAlloc, "mfv: device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+            }
+        }
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s)
+    {
+        resize(count);
+        if (count) hipCheck(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    std::vector<T> download(hipStream_t s) const
+    {
+        std::vector<T> h(n);
+        if (n)
+        {
+            hipCheck(hipMemcpyAsync(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, s), "download");
+            hipCheck(hipStreamSynchronize(s), "download sync");
+        }
+        return h;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Fixed partition of a reduction: a grid that depends on n only, so every call on the same length
+// sums in the same order.
+constexpr int kReduceBlock = 256;
+constexpr int kMaxReduceBlocks = 2048;
+
+// Per-context scratch for two-stage reductions: partial sums per block, the last-block ticket and
+// the results (device scalars), plus a pinned host mailbox for read-backs.
+struct Ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownsStream = false;
+    std::string lastError;
+    bool sequentialReduce = false;   // OpenFOAM order: one thread, ascending index
+    DBuf<double> partials;           // kMaxReduceBlocks * 4 slots
+    DBuf<unsigned int> tickets;      // one ticket per concurrent reduction slot
+    DBuf<double> scalars;            // 64 device scalars for solver state
+    double* mailbox = nullptr;       // pinned host, 64 doubles
+    int cuCount = 256;
+    // kernel timing (off unless enabled): per tag, the event pairs recorded around its launches
+    bool profiling = false;
+    std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof;
+    std::vector<hipEvent_t> eventPool;
+};
+
+// kernel classes the profiler can time
+enum ProfTag : int
+{
+    kProfPcgSpmv64 = 0,   // p update + FP64 A.p + p.q (the outer product)
+    kProfPcgUpdate,       // psi, r update + sum|r| + next cycle entry
+    kProfPcgZr,           // z.r
+    kProfVcycle,          // one whole V-cycle
+    kProfAmgResidual0,    // grid-0 residual in the cycle precision
+    kProfAmgProlongSmooth0,  // grid-0 prolongation + post-sweep
+    kProfAmgRestrict0,    // grid 0 -> 1 restriction
+    kProfCount
+};
+
+// The mesh's internal-face LDU addressing on the device (lduAddressing), plus the boundary
+// gather (cells' boundary faces in ascending face order).
+struct Mesh
+{
+    int nCells = 0;
+    int nFaces = 0;    // internal faces
+    int nBFaces = 0;
+    DBuf<label> owner, neighbour;
+    DBuf<label> ownerStart;    // [nCells+1]
+    DBuf<label> losortStart;   // [nCells+1]
+    DBuf<label> losort;        // [nFaces] faces by neighbour, stable
+    DBuf<label> losortOwner;   // [nFaces] owner[losort[k]]
+    DBuf<label> bFaceCell;     // [nBFaces]
+    DBuf<label> bndStart;      // [nCells+1] into bndFaces
+    DBuf<label> bndFaces;      // boundary face indices (0-based within the boundary) per cell, ascending
+    std::vector<label> hOwner, hNeighbour, hBFaceCell;
+};
+
+// ---- reductions / BLAS (kernels_blas.hip) ----
+void launchDot(Ctx& c, int n, const double* a, const double* b, double* out, int slot);
+void launchSumMag(Ctx& c, int n, const double* a, double* out, int slot);
+void launchCopy(Ctx& c, int n, const double* src, double* dst);
+
+// ---- LDU kernels (kernels_ldu.hip) ----
+struct LduView
+{
+    int nCells;
+    int nFaces;
+    const double* diag;
+    const double* upper;
+    const double* lower;   // == upper for a symmetric matrix
+};
+void launchAmul(Ctx& c, const Mesh& m, const LduView& A, const double* x, double* y);
+void launchResidual(Ctx& c, const Mesh& m, const LduView& A, const double* x, const double* b, double* r);
+void launchNormFactor(Ctx& c, const Mesh& m, const LduView& A, const double* psi, const double* b, double* out);
+void launchLaplacian(Ctx& c, const Mesh& m, const double* dc, const double* gammaf, const double* magSf,
+                     double* diag, double* upper, double* lower);
+void launchLaplacianBCoeffs(Ctx& c, int nB, const int* bcType, const double* bcValue, const double* gammab,
+                            const double* bMagSf, const double* bDelta, double* iC, double* bC);
+void launchDivUpwind(Ctx& c, const Mesh& m, const double* phi, double* diag, double* upper, double* lower);
+void launchDivBCoeffs(Ctx& c, int nB, const int* bcType, const double* bcValue, const double* phib, double* iC,
+                      double* bC);
+void launchFold(Ctx& c, const Mesh& m, const double* diag, const double* source, const double* iC, const double* bC,
+                double* diagC, double* b);
+void launchSetReference(Ctx& c, double* diag, double* b, int refCell, double refValue);
+void launchFaceFlux(Ctx& c, const Mesh& m, const LduView& A, const double* psi, double* flux);
+void launchInterpolate(Ctx& c, const Mesh& m, const double* w, const double* vol, double* face, double* bface);
+void launchDivSource(Ctx& c, const Mesh& m, const double* phi, const double* phib, const double* V, double sign,
+                     double* source);
+void launchFluxCorrect(Ctx& c, const Mesh& m, const LduView& A, const double* psi, const double* phiHbyA, double sign,
+                       double* phi);
+
+// ---- geometry (kernels_geom.hip) ----
+void launchGeometry(Ctx& c, const Mesh& m, const double* points, const int* faceOffsets, const int* faceVerts,
+                    double* Cf, double* Sf, double* magSf, double* C, double* V, double* weights, double* dc,
+                    double* ndc, double* corr, double* bDelta);
+
+// ---- kernel timing: HIP events around tagged launches on the context's stream ----
+struct ProfScope
+{
+    Ctx& c;
+    int tag;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ProfScope(Ctx& ctx, int t);
+    ~ProfScope();
+};
+
+} // namespace mfv

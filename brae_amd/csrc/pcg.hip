@@ -1,0 +1,421 @@
+// AMG-preconditioned conjugate gradients, device-resident.
+//
+// Reference: brae's deviceAMGPCG (src/matrices/lduMatrix/solvers/PCG/device_amg_pcg.cu:183-411,
+// 638-848): OpenFOAM's residual norm (sum|r| / normFactor), Fletcher-Reeves beta = wArA/wArAold
+// with the 1e-300 guard (pcgSearchDirK :60-77), or the flexible Polak-Ribiere+ beta
+// max(0, (z.r - z.r_old)/rho_old) (flexBetaK :37-49) for a preconditioner that is not a fixed linear
+// operator (a reduced-precision cycle); stop when res < tol, or res < relTol*init, or at maxIter, never
+// before minIter (pcgSetCondK :164-180).
+//
+// HIP has no conditional graph nodes, so the loop is host-driven but never waits on the device per
+// iteration: the stop decision is taken ON THE DEVICE by the last workgroup of the update reduction,
+// and every kernel of an iteration returns at once when the stop flag is set. The host enqueues
+// iterations back to back and reads the flag through a pinned mailbox every `checkEvery` iterations;
+// iterations enqueued past convergence do no work, so the result does not depend on checkEvery.
+#include "amg.hpp"
+#include "reduce.hpp"
+#include "sell_device.hpp"
+
+namespace mfv {
+
+namespace {
+
+constexpr int kTpb = 256;
+inline int blocks(long long n) { return (int)((n + kTpb - 1) / kTpb); }
+
+// device state (doubles)
+enum : int { S_RHO = 0, S_PQ, S_RES, S_INIT, S_NF, S_BETA, S_SUMPSI, S_FINAL, S_TOL, S_RELTOL, S_COUNT = 16 };
+// device flags (ints)
+enum : int { F_ITER = 0, F_STOP, F_CONV, F_MAXIT, F_MINIT, F_COUNT = 8 };
+
+struct PcgDev
+{
+    double* st;
+    int* fl;
+};
+
+__device__ __forceinline__ bool convergedAt(const double* st, double res)
+{
+    return res < st[S_TOL] || (st[S_RELTOL] > 0.0 && res < st[S_RELTOL] * st[S_INIT]);
+}
+
+__global__ __launch_bounds__(kReduceBlock) void sumPsiK(int n, const double* __restrict__ a, double* partials,
+                                                        unsigned int* ticket, double* out)
+{
+    __shared__ double lds[kReduceBlock / 64];
+    double v = 0.0;
+    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock) v += a[i];
+    finishReduction(blockSum(v, lds), partials, ticket, lds, StoreTo{out});
+}
+
+// r = b - A psi; sum|r| and (without a caller normFactor) the normFactor summand
+// |A psi - sumA*xRef| + |b - sumA*xRef|, sumA = A.1 in face order (device_pcg.cu:118-170).
+struct InitPost
+{
+    PcgDev d;
+    int haveNf;
+    const double* nfIn;
+    __device__ void operator()(double sumR, double sumNf) const
+    {
+        const double nf = haveNf ? *nfIn : sumNf + 1e-20;
+        d.st[S_NF] = nf;
+        const double init = sumR / nf;
+        d.st[S_INIT] = init;
+        d.st[S_FINAL] = init;
+        d.st[S_RHO] = 1e300;
+        d.fl[F_ITER] = 0;
+        const bool conv = convergedAt(d.st, init);
+        d.fl[F_CONV] = conv ? 1 : 0;
+        d.fl[F_STOP] = (d.fl[F_MINIT] <= 0 && conv) ? 1 : 0;
+    }
+};
+
+__global__ __launch_bounds__(kReduceBlock) void pcgInitK(SellView S, const double* __restrict__ val,
+                                                         const double* __restrict__ diag,
+                                                         const double* __restrict__ psi, const double* __restrict__ b,
+                                                         double* __restrict__ r, PcgDev d, int haveNf,
+                                                         const double* nfIn, double* pA, double* pB,
+                                                         unsigned int* ticket, double* seqA, double* seqB)
+{
+    __shared__ double lds[kReduceBlock / 64];
+    const double xRef = haveNf ? 0.0 : d.st[S_SUMPSI] / (double)S.n;
+    double sr = 0.0, snf = 0.0;
+    for (int c = blockIdx.x * kReduceBlock + threadIdx.x; c < S.n; c += gridDim.x * kReduceBlock)
+    {
+        const double ax = rowOffSum<double, double>(S, c, val, PlainX<double>{psi}, diag[c] * psi[c]);
+        const double rc = b[c] - ax;
+        r[c] = rc;
+        double e = 0.0;
+        if (!haveNf)
+        {
+            const double sa = rowOffSum<double, double>(S, c, val, [](int) { return 1.0; }, diag[c]);
+            const double t = sa * xRef;
+            e = fabs(ax - t) + fabs(b[c] - t);
+        }
+        if (seqA)
+        {
+            seqA[c] = fabs(rc);
+            seqB[c] = e;
+        }
+        else
+        {
+            sr += fabs(rc);
+            snf += e;
+        }
+    }
+    if (seqA) return;
+    const double a = blockSum(sr, lds);
+    const double bsum = blockSum(snf, lds);
+    finishReduction2(a, bsum, pA, pB, ticket, lds, InitPost{d, haveNf, nfIn});
+}
+
+// Sequential (OpenFOAM-order) completion of a reduction whose per-element terms were stored: one
+// thread sums them in ascending index, exactly the host loop's order, and applies the same post step.
+template <typename Post>
+__global__ void seqFinish2K(int n, const double* __restrict__ a, const double* __restrict__ b, Post post,
+                            const int* __restrict__ stop)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    if (stop && *stop) return;
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < n; ++i)
+    {
+        sa += a[i];
+        sb += b[i];
+    }
+    post(sa, sb);
+}
+
+template <typename Post>
+__global__ void seqFinishK(int n, const double* __restrict__ a, Post post, const int* __restrict__ stop)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    if (stop && *stop) return;
+    double sa = 0.0;
+    for (int i = 0; i < n; ++i) sa += a[i];
+    post(sa);
+}
+
+__global__ void seqSumK(int n, const double* __restrict__ a, double* __restrict__ out)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i];
+    *out = s;
+}
+
+// rho = z.r (and z.r_old for the flexible beta); the last workgroup forms beta
+struct ZrPost
+{
+    PcgDev d;
+    int flexible;
+    __device__ void operator()(double zr, double zrOld) const
+    {
+        const double prev = d.st[S_RHO];
+        double beta = 0.0;
+        if (d.fl[F_ITER] > 0)
+        {
+            if (flexible)
+                beta = (prev > 1e-300 || prev < -1e-300) ? fmax(0.0, (zr - zrOld) / prev) : 0.0;
+            else
+                beta = fabs(prev) > 1e-300 ? zr / prev : 0.0;
+        }
+        d.st[S_BETA] = beta;
+        d.st[S_RHO] = zr;
+    }
+};
+
+template <typename Tv>
+__global__ __launch_bounds__(kReduceBlock) void pcgZrK(int n, const Tv* __restrict__ z, const double* __restrict__ r,
+                                                       const double* __restrict__ rOld, int flexible, PcgDev d,
+                                                       double* pA, double* pB, unsigned int* ticket, double* seqA,
+                                                       double* seqB)
+{
+    if (d.fl[F_STOP]) return;
+    __shared__ double lds[kReduceBlock / 64];
+    double a = 0.0, o = 0.0;
+    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock)
+    {
+        const double zi = (double)z[i];
+        const double ea = zi * r[i];
+        const double eo = flexible ? zi * rOld[i] : 0.0;
+        if (seqA)
+        {
+            seqA[i] = ea;
+            seqB[i] = eo;
+        }
+        else
+        {
+            a += ea;
+            o += eo;
+        }
+    }
+    if (seqA) return;
+    const double sa = blockSum(a, lds);
+    const double so = blockSum(o, lds);
+    finishReduction2(sa, so, pA, pB, ticket, lds, ZrPost{d, flexible});
+}
+
+template <typename Tv>
+struct DirX
+{
+    const Tv* __restrict__ z;
+    const double* __restrict__ pOld;
+    double beta;
+    int first;
+    __device__ double operator()(int j) const
+    {
+        const double zj = (double)z[j];
+        return first ? zj : beta * pOld[j] + zj;
+    }
+};
+
+// p = z (first iteration) or beta*p + z; q = A p; p.q
+struct PqPost
+{
+    PcgDev d;
+    __device__ void operator()(double s) const { d.st[S_PQ] = s; }
+};
+
+template <typename Tv>
+__global__ __launch_bounds__(kReduceBlock) void pcgDirAmulK(SellView S, const double* __restrict__ val,
+                                                            const double* __restrict__ diag, const Tv* __restrict__ z,
+                                                            const double* __restrict__ pOld,
+                                                            double* __restrict__ pNew, double* __restrict__ q, PcgDev d,
+                                                            double* partials, unsigned int* ticket, double* seqA)
+{
+    if (d.fl[F_STOP]) return;
+    __shared__ double lds[kReduceBlock / 64];
+    const DirX<Tv> P{z, pOld, d.st[S_BETA], d.fl[F_ITER] == 0 ? 1 : 0};
+    double s = 0.0;
+    for (int c = blockIdx.x * kReduceBlock + threadIdx.x; c < S.n; c += gridDim.x * kReduceBlock)
+    {
+        const double pc = P(c);
+        const double ax = rowOffSum<double, double>(S, c, val, P, diag[c] * pc);
+        pNew[c] = pc;
+        q[c] = ax;
+        if (seqA)
+            seqA[c] = ax * pc;
+        else
+            s += ax * pc;
+    }
+    if (seqA) return;
+    finishReduction(blockSum(s, lds), partials, ticket, lds, PqPost{d});
+}
+
+// psi += alpha p, r -= alpha q, sum|r|; the stop decision; the next cycle's grid-0 entry
+struct UpdPost
+{
+    PcgDev d;
+    __device__ void operator()(double sumR) const
+    {
+        const double res = sumR / d.st[S_NF];
+        d.st[S_FINAL] = res;
+        const int it = d.fl[F_ITER] + 1;
+        d.fl[F_ITER] = it;
+        const bool conv = convergedAt(d.st, res);
+        d.fl[F_CONV] = conv ? 1 : 0;
+        d.fl[F_STOP] = ((conv || it >= d.fl[F_MAXIT]) && it >= d.fl[F_MINIT]) ? 1 : 0;
+    }
+};
+
+template <typename Tv>
+__global__ __launch_bounds__(kReduceBlock) void pcgUpdateK(int n, const double* __restrict__ p,
+                                                           const double* __restrict__ q, double* __restrict__ psi,
+                                                           double* __restrict__ r, double* __restrict__ rOld,
+                                                           int flexible, const Tv* __restrict__ diag0,
+                                                           Tv* __restrict__ b0, Tv* __restrict__ x0, int fromZero,
+                                                           PcgDev d, double* partials, unsigned int* ticket,
+                                                           double* seqA)
+{
+    if (d.fl[F_STOP]) return;
+    __shared__ double lds[kReduceBlock / 64];
+    const double pq = d.st[S_PQ];
+    const double alpha = fabs(pq) > 1e-300 ? d.st[S_RHO] / pq : 0.0;
+    double s = 0.0;
+    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock)
+    {
+        psi[i] += alpha * p[i];
+        const double ro = r[i];
+        if (flexible) rOld[i] = ro;
+        const double rn = ro - alpha * q[i];
+        r[i] = rn;
+        if (seqA)
+            seqA[i] = fabs(rn);
+        else
+            s += fabs(rn);
+        const Tv bv = (Tv)rn;
+        b0[i] = bv;
+        x0[i] = fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(diag0[i]) : Tv(0);
+    }
+    if (seqA) return;
+    finishReduction(blockSum(s, lds), partials, ticket, lds, UpdPost{d});
+}
+
+__global__ void pcgSetupK(PcgDev d, double tol, double relTol, int maxIter, int minIter)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    d.st[S_TOL] = tol;
+    d.st[S_RELTOL] = relTol;
+    d.fl[F_MAXIT] = maxIter;
+    d.fl[F_MINIT] = minIter;
+    d.fl[F_STOP] = 0;
+    d.fl[F_ITER] = 0;
+}
+
+template <typename Tv>
+PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* b, double* psi,
+               const double* normFactor, const PcgControls& pc, Tv* b0, Tv* x0, const Tv* diag0)
+{
+    const int n = m.nCells;
+    PcgWork& w = *a.work;
+    PcgDev d{w.st.p, w.fl.p};
+    const Grid& g0 = a.grids[0];
+    const SellView S = viewOf(g0.sell);
+    const int gR = reduceGrid(n);
+    double* pa = c.partials.p + 5 * (size_t)kMaxReduceBlocks;
+    double* pb = c.partials.p + 6 * (size_t)kMaxReduceBlocks;
+    unsigned int* tk = c.tickets.p + 5;
+    const bool seq = c.sequentialReduce;
+    if (seq && w.seqA.n != (size_t)n)
+    {
+        w.seqA.resize(n);
+        w.seqB.resize(n);
+    }
+    double* sA = seq ? w.seqA.p : nullptr;
+    double* sB = seq ? w.seqB.p : nullptr;
+    const int* stopF = w.fl.p + F_STOP;
+    pcgSetupK<<<1, 64, 0, c.stream>>>(d, pc.tolerance, pc.relTol, pc.maxIter, pc.minIter);
+    if (!normFactor)
+    {
+        if (seq)
+            seqSumK<<<1, 64, 0, c.stream>>>(n, psi, w.st.p + S_SUMPSI);
+        else
+            sumPsiK<<<gR, kReduceBlock, 0, c.stream>>>(n, psi, pa, tk, w.st.p + S_SUMPSI);
+    }
+    pcgInitK<<<gR, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, psi, b, w.r.p, d, normFactor ? 1 : 0,
+                                               normFactor, pa, pb, tk, sA, sB);
+    if (seq) seqFinish2K<<<1, 64, 0, c.stream>>>(n, sA, sB, InitPost{d, normFactor ? 1 : 0, normFactor}, nullptr);
+    hipCheck(hipGetLastError(), "pcg init");
+    const VcycleControls vc{pc.nPre, pc.nPost};
+    amgStartFrom(c, a, w.r.p, vc, w.fl.p + F_STOP);
+    double* p[2] = {w.p0.p, w.p1.p};
+    int it = 0;
+    for (; it < pc.maxIter; ++it)
+    {
+        const Tv* z;
+        {
+            ProfScope ps(c, kProfVcycle);
+            z = static_cast<const Tv*>(amgVcycle(c, a, vc, w.fl.p + F_STOP));
+        }
+        {
+            ProfScope ps(c, kProfPcgZr);
+            pcgZrK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(n, z, w.r.p, w.rOld.p, pc.flexible ? 1 : 0, d, pa, pb, tk,
+                                                         sA, sB);
+            if (seq) seqFinish2K<<<1, 64, 0, c.stream>>>(n, sA, sB, ZrPost{d, pc.flexible ? 1 : 0}, stopF);
+        }
+        {
+            ProfScope ps(c, kProfPcgSpmv64);
+            pcgDirAmulK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, z, p[it & 1], p[(it + 1) & 1],
+                                                              w.q.p, d, pa, tk, sA);
+            if (seq) seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, PqPost{d}, stopF);
+        }
+        {
+            ProfScope ps(c, kProfPcgUpdate);
+            pcgUpdateK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(n, p[(it + 1) & 1], w.q.p, psi, w.r.p, w.rOld.p,
+                                                             pc.flexible ? 1 : 0, diag0, b0, x0, pc.nPre > 0 ? 1 : 0,
+                                                             d, pa, tk, sA);
+            if (seq) seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, UpdPost{d}, stopF);
+        }
+        hipCheck(hipGetLastError(), "pcg iteration");
+        if ((it + 1) % std::max(1, pc.checkEvery) == 0 || it + 1 == pc.maxIter)
+        {
+            hipCheck(hipMemcpyAsync(c.mailbox, w.fl.p, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost, c.stream), "poll");
+            hipCheck(hipStreamSynchronize(c.stream), "poll sync");
+            if (reinterpret_cast<const int*>(c.mailbox)[F_STOP]) break;
+        }
+    }
+    hipCheck(hipMemcpyAsync(c.mailbox, w.st.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, c.stream), "report");
+    hipCheck(hipMemcpyAsync(c.mailbox + S_COUNT, w.fl.p, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost, c.stream),
+             "report flags");
+    hipCheck(hipStreamSynchronize(c.stream), "report sync");
+    const double* st = c.mailbox;
+    const int* fl = reinterpret_cast<const int*>(c.mailbox + S_COUNT);
+    PcgPerf perf;
+    perf.initialResidual = st[S_INIT];
+    perf.finalResidual = st[S_FINAL];
+    perf.nIterations = fl[F_ITER];
+    perf.converged = fl[F_CONV] != 0;
+    return perf;
+}
+
+} // namespace
+
+PcgPerf amgPcgSolve(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* upper, const double* b,
+                    double* psi, const double* normFactor, const PcgControls& pc)
+{
+    if (!a.galerkinDone) throw Error(kErrState, "mfv: amg_galerkin must run before the solve");
+    if (a.prec != pc.prec) throw Error(kErrState, "mfv: the hierarchy was prepared in another precision");
+    if (a.nFine != m.nCells) invalid("pcg: the hierarchy was built for another mesh");
+    if (pc.maxIter < 0 || pc.minIter < 0) invalid("pcg: negative iteration limits");
+    (void)upper;
+    if (!a.work) a.work.reset(new PcgWork());
+    PcgWork& w = *a.work;
+    const size_t n = (size_t)m.nCells;
+    if (w.r.n != n)
+    {
+        w.r.resize(n);
+        w.p0.resize(n);
+        w.p1.resize(n);
+        w.q.resize(n);
+        w.rOld.resize(n);
+        w.st.resize(S_COUNT);
+        w.fl.resize(F_COUNT);
+        hipCheck(hipMemsetAsync(w.rOld.p, 0, n * sizeof(double), c.stream), "pcg work");
+    }
+    Grid& g0 = a.grids[0];
+    if (pc.prec == Prec::F64) return solveT<double>(c, m, a, diag, b, psi, normFactor, pc, g0.b64.p, g0.x64.p, g0.diagV64.p);
+    return solveT<float>(c, m, a, diag, b, psi, normFactor, pc, g0.b32.p, g0.x32.p, g0.diagV32.p);
+}
+
+} // namespace mfv

@@ -1,0 +1,305 @@
+"""ctypes binding of libmfv.so (the C ABI in include/mfv.h).
+
+The library is the product; this module only loads it and declares the signatures so Python
+callers (tests, bench.py) can drive it with device memory they own (torch tensors on cuda:0 are
+plain HBM allocations here). It never falls back to anything: if libmfv.so is missing or does not
+load, every call raises.
+"""
+import atexit
+import collections
+import ctypes
+import os
+import re
+import weakref
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "libmfv.so")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("mfv.h",)]
+
+MFV_OK = 0
+_lib = None
+
+P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+
+
+class MfvError(RuntimeError):
+    pass
+
+
+class Ldu(ctypes.Structure):
+    _fields_ = [("dDiag", P), ("dUpper", P), ("dLower", P)]
+
+
+class AmgControls(ctypes.Structure):
+    _fields_ = [("agglomeration", I), ("nCellsInCoarsestLevel", I), ("mergeLevels", I)]
+
+
+class PcgControls(ctypes.Structure):
+    _fields_ = [("tolerance", D), ("relTol", D), ("maxIter", I), ("minIter", I), ("precision", I),
+                ("nPreSweeps", I), ("nPostSweeps", I), ("flexible", I), ("smoother", I), ("checkEvery", I),
+                ("nVcycles", I), ("scaleCorrection", I)]
+
+
+AGG_BRAE, AGG_FACEAREAPAIR = 0, 1
+PROF_PCG_SPMV64, PROF_PCG_UPDATE, PROF_PCG_ZR, PROF_VCYCLE, PROF_AMG_RESIDUAL0, PROF_AMG_PROLONG_SMOOTH0, \
+    PROF_AMG_RESTRICT0 = range(7)
+PREC_FP64, PREC_FP32, PREC_FP16 = 64, 32, 16
+
+
+class Perf(ctypes.Structure):
+    _fields_ = [("initialResidual", D), ("finalResidual", D), ("nIterations", I), ("converged", I)]
+
+
+_SIGS = {
+    "mfv_ctx_create": [I, P, P],
+    "mfv_ctx_destroy": [P],
+    "mfv_last_error": [P],
+    "mfv_ctx_stream": [P],
+    "mfv_ctx_sync": [P],
+    "mfv_ctx_set_sequential_reduce": [P, I],
+    "mfv_mesh_create": [P, I, I, P, P, I, P, P],
+    "mfv_mesh_destroy": [P],
+    "mfv_mesh_addressing": [P, P, P, P, P, P],
+    "mfv_laplacian_coeffs": [P, P, P, P, P, P, P, P],
+    "mfv_laplacian_bcoeffs": [P, P, P, P, P, P, P, P, P],
+    "mfv_div_upwind_coeffs": [P, P, P, P, P, P],
+    "mfv_div_bcoeffs": [P, P, P, P, P, P, P],
+    "mfv_fold": [P, P, P, P, P, P, P, P],
+    "mfv_set_reference": [P, P, P, I, D],
+    "mfv_face_flux": [P, P, P, P, P],
+    "mfv_amul": [P, P, P, P, P],
+    "mfv_residual": [P, P, P, P, P, P],
+    "mfv_dot": [P, I, P, P, P],
+    "mfv_sum_mag": [P, I, P, P],
+    "mfv_norm_factor": [P, P, P, P, P, P],
+    "mfv_interpolate": [P, P, P, P, P, P],
+    "mfv_div_source": [P, P, P, P, P, D, P],
+    "mfv_flux_correct": [P, P, P, P, P, D, P],
+    "mfv_mesh_geometry": [P, P, P, P, P] + [P] * 10,
+    "mfv_ctx_profile": [P, I],
+    "mfv_ctx_profile_read": [P, I, P, P],
+    "mfv_amg_build": [P, P, P, P, P],
+    "mfv_amg_destroy": [P],
+    "mfv_amg_num_grids": [P, P],
+    "mfv_amg_grid_size": [P, I, P, P],
+    "mfv_amg_level_addressing": [P, I, P, P, P, P, P],
+    "mfv_amg_galerkin": [P, P, P, P, P, I],
+    "mfv_amg_grid_values": [P, P, I, P, P],
+    "mfv_amg_vcycle": [P, P, P, P, I, I],
+    "mfv_pcg_default_controls": [P],
+    "mfv_amg_pcg_solve": [P, P, P, P, P, P, P, P, P],
+}
+_RESTYPES = {"mfv_last_error": ctypes.c_char_p, "mfv_ctx_stream": P}
+
+
+def declared_symbols():
+    """Every mfv_* function declared in the public headers."""
+    names = []
+    for h in HEADERS:
+        txt = open(h).read()
+        names += re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(mfv_\w+)\s*\(", txt, flags=re.M)
+    return sorted(set(names))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MfvError("libmfv.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, I)
+        _lib = L
+    return _lib
+
+
+def declare(name, args, restype=I):
+    """Register a signature for an entry point added after this module was written."""
+    _SIGS[name] = args
+    if _lib is not None:
+        fn = getattr(_lib, name)
+        fn.argtypes = args
+        fn.restype = restype
+
+
+_KEEP = collections.deque(maxlen=256)
+
+
+def ptr(t):
+    """Device/host address of a torch tensor or numpy array (None -> NULL). The object is kept
+    alive for the next calls so a temporary in an argument list outlives the call using it."""
+    if t is None:
+        return None
+    _KEEP.append(t)
+    if hasattr(t, "data_ptr"):
+        assert t.is_contiguous()
+        return t.data_ptr()
+    assert t.flags["C_CONTIGUOUS"]
+    return t.ctypes.data
+
+
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    """Release library objects before the HIP runtime tears down at process exit: meshes and
+    hierarchies first, contexts last."""
+    objs = list(_LIVE)
+    for o in objs:
+        if not isinstance(o, Context):
+            o.close()
+    for o in objs:
+        if isinstance(o, Context):
+            o.close()
+    _KEEP.clear()
+
+
+class Context:
+    def __init__(self, device=0, stream=None):
+        self.h = P()
+        self._check(lib().mfv_ctx_create(device, stream, ctypes.byref(self.h)), "ctx_create")
+        _LIVE.add(self)
+
+    def _check(self, rc, what):
+        if rc != MFV_OK:
+            msg = lib().mfv_last_error(self.h if isinstance(self, Context) and self.h else None)
+            raise MfvError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def call(self, name, *args):
+        rc = getattr(lib(), name)(self.h, *args)
+        self._check(rc, name)
+
+    def sync(self):
+        self.call("mfv_ctx_sync")
+
+    def sequential_reduce(self, on=True):
+        self.call("mfv_ctx_set_sequential_reduce", 1 if on else 0)
+
+    def profile(self, on=True):
+        self.call("mfv_ctx_profile", 1 if on else 0)
+
+    def profile_read(self, tag):
+        n, ms = ctypes.c_int(), ctypes.c_double()
+        self.call("mfv_ctx_profile_read", tag, ctypes.byref(n), ctypes.byref(ms))
+        return n.value, ms.value
+
+    @property
+    def stream(self):
+        return lib().mfv_ctx_stream(self.h)
+
+    def close(self):
+        if self.h:
+            lib().mfv_ctx_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mesh:
+    """Device LDU addressing of a mesh (host numpy int32 arrays in)."""
+
+    def __init__(self, ctx, nCells, owner, neighbour, bFaceCell):
+        import numpy as np
+        self.ctx = ctx
+        self.nCells = int(nCells)
+        self.owner = np.ascontiguousarray(owner, np.int32)
+        self.neighbour = np.ascontiguousarray(neighbour, np.int32)
+        self.bFaceCell = np.ascontiguousarray(bFaceCell, np.int32)
+        self.nFaces = self.owner.size
+        self.nBFaces = self.bFaceCell.size
+        self.h = P()
+        ctx._check(lib().mfv_mesh_create(ctx.h, self.nCells, self.nFaces, ptr(self.owner), ptr(self.neighbour),
+                                         self.nBFaces, ptr(self.bFaceCell), ctypes.byref(self.h)), "mesh_create")
+        _LIVE.add(self)
+
+    def close(self):
+        if self.h:
+            lib().mfv_mesh_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ldu(diag, upper, lower=None):
+    return Ldu(ptr(diag), ptr(upper), ptr(lower if lower is not None else upper))
+
+
+class Amg:
+    """AMG hierarchy of a mesh (host agglomeration, device levels)."""
+
+    def __init__(self, mesh, weights, agglomeration=AGG_BRAE, nCellsInCoarsestLevel=0):
+        import numpy as np
+        self.ctx = mesh.ctx
+        self.mesh = mesh
+        w = np.ascontiguousarray(weights, np.float64)
+        assert w.size == mesh.nFaces
+        c = AmgControls(agglomeration, nCellsInCoarsestLevel, 1)
+        self.h = P()
+        self.ctx._check(lib().mfv_amg_build(self.ctx.h, mesh.h, ptr(w), ctypes.byref(c), ctypes.byref(self.h)),
+                        "amg_build")
+        _LIVE.add(self)
+        n = ctypes.c_int()
+        lib().mfv_amg_num_grids(self.h, ctypes.byref(n))
+        self.nGrids = n.value
+
+    def grid_size(self, g):
+        nc, nf = ctypes.c_int(), ctypes.c_int()
+        lib().mfv_amg_grid_size(self.h, g, ctypes.byref(nc), ctypes.byref(nf))
+        return nc.value, nf.value
+
+    def level(self, i):
+        import numpy as np
+        nF, fF = self.grid_size(i)
+        nC, cF = self.grid_size(i + 1)
+        m, co, cn = np.zeros(nF, np.int32), np.zeros(cF, np.int32), np.zeros(cF, np.int32)
+        fr, ff = np.zeros(fF, np.int32), np.zeros(fF, np.int8)
+        rc = lib().mfv_amg_level_addressing(self.h, i, ptr(m), ptr(co), ptr(cn), ptr(fr), ptr(ff))
+        assert rc == 0
+        return dict(map=m, cOwn=co, cNei=cn, faceRestrict=fr, faceFlip=ff, nCoarse=nC, nCoarseFaces=cF)
+
+    def galerkin(self, diag, upper, precision=PREC_FP32):
+        self.ctx.call("mfv_amg_galerkin", self.h, ptr(diag), ptr(upper), None, precision)
+
+    def grid_values(self, g):
+        import numpy as np
+        n, f = self.grid_size(g)
+        d, u = np.zeros(n), np.zeros(f)
+        self.ctx.call("mfv_amg_grid_values", self.h, g, ptr(d), ptr(u))
+        return d, u
+
+    def vcycle(self, r, z, nPre=1, nPost=1):
+        self.ctx.call("mfv_amg_vcycle", self.h, ptr(r), ptr(z), nPre, nPost)
+
+    def solve(self, diag, upper, b, psi, normFactor=None, **kw):
+        c = PcgControls()
+        lib().mfv_pcg_default_controls(ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        A = ldu(diag, upper)
+        perf = Perf()
+        self.ctx.call("mfv_amg_pcg_solve", self.mesh.h, self.h, ctypes.byref(A), ptr(b), ptr(psi),
+                      ptr(normFactor), ctypes.byref(c), ctypes.byref(perf))
+        return perf
+
+    def close(self):
+        if self.h:
+            lib().mfv_amg_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

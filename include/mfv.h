@@ -1,0 +1,227 @@
+/*
+ * mfv.h -- C ABI of the MI355X-native finite-volume assembly and pressure-solve library (libmfv.so).
+ *
+ * It replaces the reference's device lduMatrix / assembly / AMG-PCG entry points for the pressure
+ * path. Every entry point says which reference interface it stands for (file:line, relative to the
+ * reference root). Conventions:
+ *   - int32 labels, double scalars, structure-of-arrays; internal faces in OpenFOAM's
+ *     upper-triangular order (sorted by owner, owner < neighbour);
+ *   - pointers named d* are DEVICE pointers (HBM), h* are host pointers;
+ *   - every call returns an int status (MFV_OK or an MFV_ERR_*), never throws, and records a
+ *     message mfv_last_error(ctx) returns; work is enqueued on the context's HIP stream and is
+ *     asynchronous unless the entry point says it synchronises;
+ *   - a context is used by one host thread at a time; different contexts are independent.
+ *   - a symmetric matrix passes dLower == dUpper (one off-diagonal, device_amg.cu:1703-1710).
+ * Non-convergence of a solve is not an error: it returns MFV_OK with nIterations == maxIter.
+ */
+#ifndef MFV_H
+#define MFV_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFV_OK 0
+#define MFV_ERR_INVALID 1     /* bad argument or mesh (names it) */
+#define MFV_ERR_HIP 2         /* a HIP runtime call failed */
+#define MFV_ERR_UNSUPPORTED 3 /* an input the path does not carry (refused by name) */
+#define MFV_ERR_ALLOC 4       /* device allocation failed */
+#define MFV_ERR_STATE 5       /* call order (e.g. a solve before amg_galerkin) */
+
+typedef struct mfv_ctx_s* mfv_ctx;
+typedef struct mfv_mesh_s* mfv_mesh;
+typedef struct mfv_amg_s* mfv_amg;
+
+/* Values of an LDU matrix on a mesh's (or an AMG level's) addressing; device pointers.
+ * Stands for DeviceLduView's value half (src/matrices/lduMatrix/lduMatrix/device_ldu.cuh:20-66). */
+typedef struct
+{
+    const double* dDiag;
+    const double* dUpper;
+    const double* dLower;
+} mfv_ldu;
+
+/* ---------------------------------------------------------------------------------------------
+ * Context: one HIP stream, the reduction scratch and a pinned read-back mailbox. Replaces the
+ * process-wide singletons of reductions.cu:89-111 so concurrent contexts do not share state.
+ * hStream may be NULL (the context creates its own default-priority stream).
+ * ------------------------------------------------------------------------------------------- */
+int mfv_ctx_create(int device, void* hStream, mfv_ctx* out);
+int mfv_ctx_destroy(mfv_ctx ctx);
+const char* mfv_last_error(mfv_ctx ctx);
+/* hipStream_t of the context */
+void* mfv_ctx_stream(mfv_ctx ctx);
+/* waits for the context's stream */
+int mfv_ctx_sync(mfv_ctx ctx);
+/* 1: every dot and sum|x| runs in OpenFOAM's sequential order (one thread; the reference's
+ * BRAE_DEVICE_OF_REDUCE, reductions.cu:36-56); 0 (default): the deterministic wave64 tree. */
+int mfv_ctx_set_sequential_reduce(mfv_ctx ctx, int sequential);
+/* Kernel timing with HIP events recorded around the launches of each kernel class on the context's
+ * stream (no counterpart in the reference, whose timing is BRAE_PHASE_TIME/BRAE_AMG_PCG_SPLIT by
+ * draining the device). Tags: 0 PCG p-update + FP64 product, 1 PCG update, 2 PCG z.r, 3 one V-cycle,
+ * 4 grid-0 residual, 5 grid-0 prolongation + smoothing, 6 grid 0 -> 1 restriction. */
+int mfv_ctx_profile(mfv_ctx ctx, int enable);
+/* launches recorded for `tag` since the last read and their summed duration; synchronises, resets */
+int mfv_ctx_profile_read(mfv_ctx ctx, int tag, int* count, double* totalMs);
+
+/* ---------------------------------------------------------------------------------------------
+ * Mesh addressing: lduAddressing of the internal faces (owner, neighbour, ownerStart, losort,
+ * losortStart) and the boundary gather (bFaceCell). Host arrays in; uploaded and kept on the
+ * device. Refuses faces not in upper-triangular order.
+ * Replaces buildDeviceLdu (src/matrices/lduMatrix/lduMatrix/device_ldu.cuh:109-146) and the
+ * DeviceMesh addressing upload (src/cuda/device_mesh.cuh:31-60).
+ * ------------------------------------------------------------------------------------------- */
+int mfv_mesh_create(mfv_ctx ctx, int nCells, int nFaces, const int* hOwner, const int* hNeighbour, int nBFaces,
+                    const int* hBFaceCell, mfv_mesh* out);
+int mfv_mesh_destroy(mfv_mesh mesh);
+/* device pointers owned by the mesh: owner, neighbour [nFaces]; ownerStart, losortStart [nCells+1];
+ * losort [nFaces] (any may be NULL) */
+int mfv_mesh_addressing(mfv_mesh mesh, const int** dOwner, const int** dNeighbour, const int** dOwnerStart,
+                        const int** dLosortStart, const int** dLosort);
+
+/* ---------------------------------------------------------------------------------------------
+ * Assembly. All device pointers. Boundary arrays are per boundary face (nBFaces), bcType
+ * 0 zeroGradient, 1 fixedValue, 2 empty.
+ * ------------------------------------------------------------------------------------------- */
+/* deviceLaplacianCoeffs / deviceLaplacianCoeffsSymmetric (src/cuda/device_fvm.cu:304-367):
+ * upper = dc*gammaf*magSf per internal face (dLower may equal dUpper or be NULL), diag = -sum. */
+int mfv_laplacian_coeffs(mfv_ctx ctx, mfv_mesh mesh, const double* dDeltaCoeffs, const double* dGammaf,
+                         const double* dMagSf, double* dDiag, double* dUpper, double* dLower);
+/* laplacianBoundaryCoeffs (src/finiteVolume/finiteVolume/fvm.cuh:43-103), fixedValue/zeroGradient */
+int mfv_laplacian_bcoeffs(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue,
+                          const double* dGammab, const double* dBMagSf, const double* dBDeltaCoeffs, double* dIC,
+                          double* dBC);
+/* deviceDivUpwindCoeffs (src/cuda/device_fvm.cu:646-662): lower = -pos0(phi)*phi, upper = lower + phi */
+int mfv_div_upwind_coeffs(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, double* dDiag, double* dUpper,
+                          double* dLower);
+/* fvm::div boundary coefficients (fvm.cuh:545-552) */
+int mfv_div_bcoeffs(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue, const double* dPhib,
+                    double* dIC, double* dBC);
+/* deviceFold (src/cuda/device_fvm.cu:895-912): diagC = diag + sum iC, b = source + sum bC */
+int mfv_fold(mfv_ctx ctx, mfv_mesh mesh, const double* dDiag, const double* dSource, const double* dIC,
+             const double* dBC, double* dDiagC, double* dB);
+/* deviceSetReference (src/applications/solvers/simpleFoam/device_simple.cuh:104-106) */
+int mfv_set_reference(mfv_ctx ctx, double* dDiag, double* dB, int refCell, double refValue);
+/* deviceMatrixFluxInternal (device_simple.cuh:51-52): flux = upper*psi[nei] - lower*psi[own] */
+int mfv_face_flux(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, double* dFlux);
+
+/* fvc::interpolate, linear (src/finiteVolume/finiteVolume/fvc.cu:626-656): face = w*vol[own] +
+ * (1-w)*vol[nei]; dBFace (may be NULL) takes the owner cell's value (an extrapolated field such as rAU,
+ * src/applications/solvers/simpleFoam/pEqn_cpp.cu:217-222) */
+int mfv_interpolate(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dVol, double* dFace,
+                    double* dBFace);
+/* the pressure equation's right-hand side (pEqn_cpp.cu:238-242): source += sign * div(phi)*V, with
+ * div(phi) = (sum_own phi - sum_nei phi + sum_boundary phiB)/V (fvc.cu:706-730); dPhiB may be NULL */
+int mfv_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double* dPhiB, const double* dV,
+                   double sign, double* dSource);
+/* correctFlux (pEqn_cpp.cuh:134-139): phi = phiHbyA - sign*(upper*psi[nei] - lower*psi[own]); sign -1
+ * when A is the negated (positive-definite) pressure matrix */
+int mfv_flux_correct(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dPhiHbyA,
+                     double sign, double* dPhi);
+
+/* FvGeometry::build (src/finiteVolume/fvMesh/fv_geometry.cu:13-158) and the patches' deltaCoeffs
+ * (fv_patch.cu:160-165) on the device. Faces are polygons given by dFaceOffsets [nFaces+1] into
+ * dFaceVerts; all nInternal + nBFaces faces, boundary faces after the internal ones. Outputs: Cf, Sf
+ * [3*nFaces], magSf [nFaces], C [3*nCells], V [nCells], weights, deltaCoeffs, nonOrthDeltaCoeffs
+ * [nInternal], corrVec [3*nInternal], bDeltaCoeffs [nBFaces]. */
+int mfv_mesh_geometry(mfv_ctx ctx, mfv_mesh mesh, const double* dPoints, const int* dFaceOffsets,
+                      const int* dFaceVerts, double* dCf, double* dSf, double* dMagSf, double* dC, double* dV,
+                      double* dWeights, double* dDeltaCoeffs, double* dNonOrthDeltaCoeffs, double* dCorrVec,
+                      double* dBDeltaCoeffs);
+
+/* ---------------------------------------------------------------------------------------------
+ * Products and reductions.
+ * ------------------------------------------------------------------------------------------- */
+/* deviceAmul (src/matrices/lduMatrix/lduMatrix/device_ldu.cuh:241, device_spmv.cu:302) */
+int mfv_amul(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, double* dApsi);
+/* residualKernel (device_spmv.cu:116): r = b - A psi */
+int mfv_residual(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dB, double* dR);
+/* deviceDotInto / deviceSumMagInto (src/matrices/lduMatrix/lduMatrix/reductions.cu:315,387):
+ * result to a device scalar */
+int mfv_dot(mfv_ctx ctx, int n, const double* dA, const double* dB, double* dResult);
+int mfv_sum_mag(mfv_ctx ctx, int n, const double* dA, double* dResult);
+/* deviceNormFactorInto (src/matrices/lduMatrix/solvers/PCG/device_pcg.cu:133-170) */
+int mfv_norm_factor(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dB,
+                    double* dNormFactor);
+
+/* ---------------------------------------------------------------------------------------------
+ * AMG hierarchy (pairwise agglomeration; the addressing is built on the host once per mesh).
+ * ------------------------------------------------------------------------------------------- */
+#define MFV_AGG_BRAE 0          /* brae's device-path pairing: forward greedy, sorted coarse faces */
+#define MFV_AGG_FACEAREAPAIR 1  /* OpenFOAM faceAreaPair: alternating direction, first-appearance faces */
+
+#define MFV_PREC_FP64 64
+#define MFV_PREC_FP32 32
+#define MFV_PREC_FP16 16        /* FP16 coefficients scaled per grid by a power of two, FP32 vectors */
+
+typedef struct
+{
+    int agglomeration;          /* MFV_AGG_* */
+    int nCellsInCoarsestLevel;  /* brae: coarsen while n > this (its TARGET, default 64);
+                                   faceAreaPair: nCellsInCoarsestLevel (default 10) */
+    int mergeLevels;            /* 1 (only value carried) */
+} mfv_amg_controls;
+
+/* buildAMG (src/matrices/lduMatrix/preconditioners/GAMGPreconditioner/device_amg.cuh:393,
+ * device_amg.cu:1346-1431), and for MFV_AGG_FACEAREAPAIR faceAreaPairGamgAgglomeration
+ * (src/matrices/lduMatrix/solvers/GAMG/GAMGAgglomerations/pair_gamg_agglomeration_cpp.cu:286-345).
+ * hFaceWeights: one per internal face (brae passes |Sf|; faceAreaPair its perturbed weights). */
+int mfv_amg_build(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const mfv_amg_controls* controls,
+                  mfv_amg* out);
+int mfv_amg_destroy(mfv_amg amg);
+/* number of grids (the mesh plus its coarse levels) */
+int mfv_amg_num_grids(mfv_amg amg, int* nGrids);
+int mfv_amg_grid_size(mfv_amg amg, int grid, int* nCells, int* nFaces);
+/* host copies of the step from grid `level` to grid `level+1` (any pointer may be NULL):
+ * restrict map [nCells(level)], coarse owner/neighbour [nFaces(level+1)], faceRestrict and faceFlip
+ * [nFaces(level)] -- AMGLevel's map/cOwn/cNei/faceRestrict/faceFlip (device_amg.cuh:119-182) */
+int mfv_amg_level_addressing(mfv_amg amg, int level, int* hMap, int* hCoarseOwner, int* hCoarseNeighbour,
+                             int* hFaceRestrict, signed char* hFaceFlip);
+/* amgGalerkin (device_amg.cuh:454, device_amg.cu:1693-1846): coarse matrices from the folded fine
+ * matrix (symmetric only: dFineLower must be NULL or equal dFineUpper), the V-cycle copies in
+ * `precision`, the coarsest dense inverse. */
+int mfv_amg_galerkin(mfv_ctx ctx, mfv_amg amg, const double* dFineDiag, const double* dFineUpper,
+                     const double* dFineLower, int precision);
+/* host copies of a grid's FP64 Galerkin values (grid >= 1): diag [nCells], upper [nFaces]; synchronises */
+int mfv_amg_grid_values(mfv_ctx ctx, mfv_amg amg, int grid, double* hDiag, double* hUpper);
+/* amgVCycleApply (device_amg_vcycle.cu:1793-1875): dZ = one V(nPre,nPost) cycle applied to dR */
+int mfv_amg_vcycle(mfv_ctx ctx, mfv_amg amg, const double* dR, double* dZ, int nPreSweeps, int nPostSweeps);
+
+/* ---------------------------------------------------------------------------------------------
+ * AMG-preconditioned CG (deviceAMGPCG, device_amg.cuh:550-553; device_amg_pcg.cu:638-848).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct
+{
+    double tolerance;      /* on OpenFOAM's normalised residual sum|r|/normFactor */
+    double relTol;
+    int maxIter, minIter;
+    int precision;         /* MFV_PREC_*: the V-cycle's precision; must match mfv_amg_galerkin's */
+    int nPreSweeps, nPostSweeps;
+    int flexible;          /* 1: Polak-Ribiere+ beta (flexBetaK); 0: Fletcher-Reeves */
+    int smoother;          /* 0: weighted Jacobi (omega 0.8), the only value carried */
+    int checkEvery;        /* host polls the device stop flag every k iterations (result independent of k) */
+    int nVcycles;          /* 1 (only value carried) */
+    int scaleCorrection;   /* 0 (only value carried) */
+} mfv_pcg_controls;
+
+/* SolverPerformance (src/OpenFOAM/matrices/pcg.cuh:16-21) */
+typedef struct
+{
+    double initialResidual, finalResidual;
+    int nIterations;
+    int converged;
+} mfv_perf;
+
+int mfv_pcg_default_controls(mfv_pcg_controls* controls);
+/* A: the folded symmetric fine matrix the hierarchy's last mfv_amg_galerkin saw; dNormFactor: a device
+ * scalar (deviceNormFactorInto's) or NULL to compute OpenFOAM's normFactor; psi updated in place.
+ * Synchronises. */
+int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
+                      const double* dNormFactor, const mfv_pcg_controls* controls, mfv_perf* perf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

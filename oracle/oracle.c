@@ -1478,45 +1478,62 @@ static void br_amul_f(const br_mat* A, const float* x, float* y)
     }
 }
 
-static void dense_lu_solve(int n, const double* Ain, const double* b, double* x)
+/* Explicit inverse of the coarsest grid: the reference's default coarse solve kind is Inverse
+ * (device_amg_coarse_default.cuh:13); formed here by Gauss-Jordan with partial pivoting (first
+ * largest |a_ik|, i >= k) and the pivot floor 1e-14*max|diag| (device_amg_detail.cuh:93-94). */
+static void gauss_jordan_inverse(int n, const double* A, const double* diag, double* inv)
 {
-    double* A = (double*)xcalloc((size_t)n * n, sizeof(double));
-    double* rhs = (double*)xcalloc((size_t)n, sizeof(double));
-    memcpy(A, Ain, (size_t)n * n * sizeof(double));
-    memcpy(rhs, b, (size_t)n * sizeof(double));
+    const int w = 2 * n;
+    double* M = (double*)xcalloc((size_t)n * w, sizeof(double));
+    double* fcol = (double*)xcalloc((size_t)n, sizeof(double));
+    for (int i = 0; i < n; ++i)
+    {
+        for (int j = 0; j < n; ++j) M[(size_t)i * w + j] = A[(size_t)i * n + j];
+        M[(size_t)i * w + n + i] = 1.0;
+    }
+    double scale = 0.0;
+    for (int c = 0; c < n; ++c) scale = fmax(scale, fabs(diag[c]));
+    if (!(scale > 0.0)) scale = 1.0;
     for (int k = 0; k < n; ++k)
     {
-        int piv = k;
-        double best = fabs(A[(size_t)k * n + k]);
-        for (int i = k + 1; i < n; ++i)
-            if (fabs(A[(size_t)i * n + k]) > best) { best = fabs(A[(size_t)i * n + k]); piv = i; }
-        if (piv != k)
-        {
-            for (int j = 0; j < n; ++j)
+        double bv = -1.0;
+        int p = k;
+        for (int i = k; i < n; ++i)
+            if (fabs(M[(size_t)i * w + k]) > bv) { bv = fabs(M[(size_t)i * w + k]); p = i; }
+        if (p != k)
+            for (int j = 0; j < w; ++j)
             {
-                double t = A[(size_t)k * n + j];
-                A[(size_t)k * n + j] = A[(size_t)piv * n + j];
-                A[(size_t)piv * n + j] = t;
+                double t = M[(size_t)k * w + j];
+                M[(size_t)k * w + j] = M[(size_t)p * w + j];
+                M[(size_t)p * w + j] = t;
             }
-            double t = rhs[k]; rhs[k] = rhs[piv]; rhs[piv] = t;
-        }
-        const double dkk = A[(size_t)k * n + k];
-        for (int i = k + 1; i < n; ++i)
+        double d = M[(size_t)k * w + k];
+        const double fl = 1e-14 * scale;
+        if (!(fabs(d) > fl)) d = d < 0.0 ? -fl : fl;
+        M[(size_t)k * w + k] = d;
+        const double rp = 1.0 / d;
+        for (int j = 0; j < w; ++j) M[(size_t)k * w + j] *= rp;
+        for (int i = 0; i < n; ++i) fcol[i] = M[(size_t)i * w + k];
+        for (int i = 0; i < n; ++i)
         {
-            const double fct = A[(size_t)i * n + k] / dkk;
-            if (fct == 0.0) continue;
-            for (int j = k; j < n; ++j) A[(size_t)i * n + j] -= fct * A[(size_t)k * n + j];
-            rhs[i] -= fct * rhs[k];
+            if (i == k) continue;
+            for (int j = 0; j < w; ++j) M[(size_t)i * w + j] -= fcol[i] * M[(size_t)k * w + j];
         }
     }
-    for (int i = n - 1; i >= 0; --i)
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) inv[(size_t)i * n + j] = M[(size_t)i * w + n + j];
+    free(M);
+    free(fcol);
+}
+
+static void dense_apply(int n, const double* inv, const double* b, double* x)
+{
+    for (int i = 0; i < n; ++i)
     {
-        double s = rhs[i];
-        for (int j = i + 1; j < n; ++j) s -= A[(size_t)i * n + j] * x[j];
-        x[i] = s / A[(size_t)i * n + i];
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += inv[(size_t)i * n + j] * b[j];
+        x[i] = s;
     }
-    free(A);
-    free(rhs);
 }
 
 typedef struct
@@ -1524,7 +1541,7 @@ typedef struct
     const orc_aggl* a;
     int nG;            /* grids = levels + 1 */
     br_mat* g;         /* g[0] fine */
-    double* dense;     /* coarsest dense matrix */
+    double* dense;     /* coarsest dense inverse */
     int nPre, nPost, single;
     double **b, **x, **r, **Ax;
     float **bf, **xf, **rf, **Axf;
@@ -1537,7 +1554,7 @@ static void br_vcycle_d(br_hier* H, int g)
     double *b = H->b[g], *x = H->x[g], *r = H->r[g], *Ax = H->Ax[g];
     if (g == H->nG - 1)
     {
-        dense_lu_solve(n, H->dense, b, x);
+        dense_apply(n, H->dense, b, x);
         return;
     }
     for (int s = 0; s < H->nPre; ++s)
@@ -1579,7 +1596,7 @@ static void br_vcycle_f(br_hier* H, int g)
         double* bd = H->b[g];
         double* xd = H->x[g];
         for (int i = 0; i < n; ++i) bd[i] = (double)b[i];
-        dense_lu_solve(n, H->dense, bd, xd);
+        dense_apply(n, H->dense, bd, xd);
         for (int i = 0; i < n; ++i) x[i] = (float)xd[i];
         return;
     }
@@ -1612,9 +1629,9 @@ static void br_vcycle_f(br_hier* H, int g)
     }
 }
 
-orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* upper, const double* b,
-                         double* psi, double tol, double relTol, int maxIter, int minIter,
-                         int nPre, int nPost, int flexible, int single, double normFactorIn, double* psiTrace)
+/* hierarchy values (brae Galerkin), float copies and work vectors for the cycles below */
+static void br_hier_build(br_hier* Hp, const orc_aggl* a, const double* diag, const double* upper, int nPre,
+                          int nPost, int single, double** cDp, double** cUp)
 {
     const int *own, *nei;
     aggl_fine_addressing(a, &own, &nei);
@@ -1671,14 +1688,68 @@ orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* up
     {
         br_mat* Ac = &H.g[H.nG - 1];
         const int n = Ac->n;
-        H.dense = (double*)xcalloc((size_t)n * n, sizeof(double));
-        for (int c = 0; c < n; ++c) H.dense[(size_t)c * n + c] = Ac->diag[c];
+        double* dense = (double*)xcalloc((size_t)n * n, sizeof(double));
+        for (int c = 0; c < n; ++c) dense[(size_t)c * n + c] = Ac->diag[c];
         for (int f = 0; f < Ac->nf; ++f)
         {
-            H.dense[(size_t)Ac->own[f] * n + Ac->nei[f]] += Ac->upper[f];
-            H.dense[(size_t)Ac->nei[f] * n + Ac->own[f]] += Ac->upper[f];
+            dense[(size_t)Ac->own[f] * n + Ac->nei[f]] += Ac->upper[f];
+            dense[(size_t)Ac->nei[f] * n + Ac->own[f]] += Ac->upper[f];
         }
+        H.dense = (double*)xcalloc((size_t)n * n, sizeof(double));
+        gauss_jordan_inverse(n, dense, Ac->diag, H.dense);
+        free(dense);
     }
+    *Hp = H;
+    *cDp = cD;
+    *cUp = cU;
+}
+
+static void br_hier_free(br_hier* Hp, double* cD, double* cU)
+{
+    br_hier H = *Hp;
+    for (int g = 0; g < H.nG; ++g)
+    {
+        free(H.g[g].fdiag); free(H.g[g].fupper);
+        free(H.b[g]); free(H.x[g]); free(H.r[g]); free(H.Ax[g]);
+        free(H.bf[g]); free(H.xf[g]); free(H.rf[g]); free(H.Axf[g]);
+    }
+    free(H.g); free(H.b); free(H.x); free(H.r); free(H.Ax); free(H.bf); free(H.xf); free(H.rf); free(H.Axf);
+    free(H.dense);
+    free(cD);
+    free(cU);
+}
+
+void orc_brae_vcycle(const orc_aggl* a, const double* diag, const double* upper, const double* r, double* z,
+                     int nPre, int nPost, int single)
+{
+    br_hier H;
+    double *cD, *cU;
+    br_hier_build(&H, a, diag, upper, nPre, nPost, single, &cD, &cU);
+    const int nC = a->fineCells;
+    if (single)
+    {
+        for (int i = 0; i < nC; ++i) H.bf[0][i] = (float)r[i];
+        br_vcycle_f(&H, 0);
+        for (int i = 0; i < nC; ++i) z[i] = (double)H.xf[0][i];
+    }
+    else
+    {
+        memcpy(H.b[0], r, (size_t)nC * sizeof(double));
+        br_vcycle_d(&H, 0);
+        memcpy(z, H.x[0], (size_t)nC * sizeof(double));
+    }
+    br_hier_free(&H, cD, cU);
+}
+
+orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* upper, const double* b,
+                         double* psi, double tol, double relTol, int maxIter, int minIter,
+                         int nPre, int nPost, int flexible, int single, double normFactorIn, double* psiTrace)
+{
+    const int *own, *nei;
+    aggl_fine_addressing(a, &own, &nei);
+    br_hier H;
+    double *cD, *cU;
+    br_hier_build(&H, a, diag, upper, nPre, nPost, single, &cD, &cU);
     const int nC = a->fineCells;
     br_mat* A0 = &H.g[0];
     double* pA = (double*)xcalloc((size_t)nC, sizeof(double));
@@ -1753,13 +1824,7 @@ orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* up
         perf.nIterations = it;
     }
 #undef BR_CONV
-    for (int g = 0; g < H.nG; ++g)
-    {
-        free(H.g[g].fdiag); free(H.g[g].fupper);
-        free(H.b[g]); free(H.x[g]); free(H.r[g]); free(H.Ax[g]);
-        free(H.bf[g]); free(H.xf[g]); free(H.rf[g]); free(H.Axf[g]);
-    }
-    free(H.g); free(H.b); free(H.x); free(H.r); free(H.Ax); free(H.bf); free(H.xf); free(H.rf); free(H.Axf);
-    free(H.dense); free(cD); free(cU); free(pA); free(wA); free(rA); free(rOld);
+    br_hier_free(&H, cD, cU);
+    free(pA); free(wA); free(rA); free(rOld);
     return perf;
 }

@@ -132,7 +132,8 @@ orc_perf orc_pcg_gamg_solve(const orc_aggl* a, const double* diag, const double*
                             const orc_gamg_controls* c);
 
 /* ---- brae AMG-PCG (device path semantics, FP64 arithmetic here):
- *      V(nPre,nPost) weighted Jacobi (omega 0.8) from zero, injection, direct coarsest:
+ *      V(nPre,nPost) weighted Jacobi (omega 0.8) from zero, injection, coarsest by the explicit
+ *      inverse (Gauss-Jordan, partial pivoting; the reference's default coarse kind):
  *      device_amg_vcycle.cu:982-1143 / 1658-1766, amg_kernels.cuh:27-53,257-282;
  *      PCG recurrence device_amg_pcg.cu:60-170 (Fletcher-Reeves) or flexBetaK :37-49 (flexible).
  *      normFactorIn > 0 replaces OpenFOAM's normFactor (the demo trace passes sum|b| + 1e-20).
@@ -141,6 +142,10 @@ orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* up
                          double* psi, double tol, double relTol, int maxIter, int minIter,
                          int nPre, int nPost, int flexible, int single, double normFactorIn,
                          double* psiTrace);
+
+/* one brae V-cycle z = M^-1 r on the hierarchy (same cycle as orc_brae_amgpcg applies) */
+void orc_brae_vcycle(const orc_aggl* a, const double* diag, const double* upper, const double* r, double* z,
+                     int nPre, int nPost, int single);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,54 @@
+"""Helpers for the GPU parity tests: device tensors and a mesh + context pair."""
+import numpy as np
+
+from brae_amd import hexmesh, mfv
+
+
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need an MI355X"
+    return torch
+
+
+def dev(a, dtype=None):
+    torch = torch_cuda()
+    a = np.ascontiguousarray(a)
+    t = torch.from_numpy(a.copy())
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda:0").contiguous()
+
+
+def zeros(n, dtype="float64"):
+    torch = torch_cuda()
+    return torch.zeros(int(n), dtype=getattr(torch, dtype), device="cuda:0")
+
+
+def host(t):
+    torch_cuda().cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+class Case:
+    """Context + device mesh for a polyMesh given as numpy arrays."""
+
+    def __init__(self, nCells, owner_internal, neighbour, bFaceCell, ctx=None):
+        torch_cuda().cuda.synchronize()
+        self.ctx = ctx or mfv.Context(0)
+        self.mesh = mfv.Mesh(self.ctx, nCells, owner_internal, neighbour, bFaceCell)
+        self.nCells = nCells
+        self.nFaces = len(neighbour)
+        self.nBFaces = len(bFaceCell)
+
+    def call(self, name, *args):
+        self.ctx.call(name, *args)
+
+    def sync(self):
+        self.ctx.sync()
+
+
+def hex_case(nx, ny, nz, **kw):
+    m = hexmesh.box_mesh(nx, ny, nz, **kw)
+    nI = m["nInternal"]
+    c = Case(m["nCells"], m["owner"][:nI], m["neighbour"], m["owner"][nI:])
+    return m, c

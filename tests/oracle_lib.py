@@ -1,5 +1,6 @@
 """ctypes binding of the oracle (oracle/liboracle.so) for the tests, smoke() and bench.py's CPU
 baseline. Test infrastructure: the product never imports this module."""
+import collections
 import ctypes
 import os
 import subprocess
@@ -68,6 +69,7 @@ def lib():
             "orc_gamg_solve": (Perf, [P, P, P, P, P, P]),
             "orc_pcg_gamg_solve": (Perf, [P, P, P, P, P, D, D, I, I, P]),
             "orc_brae_amgpcg": (Perf, [P, P, P, P, P, D, D, I, I, I, I, I, I, D, P]),
+            "orc_brae_vcycle": (None, [P, P, P, P, P, I, I, I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -77,11 +79,16 @@ def lib():
     return _LIB
 
 
+_KEEP = collections.deque(maxlen=256)
+
+
 def p(a):
-    """data pointer of a C-contiguous numpy array (None passes NULL)."""
+    """data pointer of a C-contiguous numpy array (None passes NULL). The array is kept alive for
+    the next calls, so a temporary built in an argument list outlives the call that uses it."""
     if a is None:
         return None
     assert a.flags["C_CONTIGUOUS"]
+    _KEEP.append(a)
     return a.ctypes.data
 
 
@@ -284,3 +291,9 @@ def brae_amgpcg(aggl, diag, upper, b, psi, tol, relTol=0.0, maxIter=1000, minIte
     perf = lib().orc_brae_amgpcg(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(b)), p(psi), tol, relTol, maxIter,
                                  minIter, nPre, nPost, int(flexible), int(single), normFactor, p(tr))
     return (psi, perf, tr) if trace else (psi, perf)
+
+
+def brae_vcycle(aggl, diag, upper, r, nPre=1, nPost=1, single=False):
+    z = np.zeros(len(r))
+    lib().orc_brae_vcycle(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(r)), p(z), nPre, nPost, int(single))
+    return z

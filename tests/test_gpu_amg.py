@@ -1,0 +1,221 @@
+"""GPU parity of the AMG hierarchy and AMG-PCG against the oracle.
+
+Addressing (agglomeration maps, coarse faces, face restriction/flip) is integer work: bit-exact.
+Galerkin values, one V-cycle and the whole PCG with sequential (OpenFOAM-order) reductions follow
+the oracle's operation order term by term and are held bit-exact too, in FP64 and FP32. With the
+default tree reductions the dots round differently, so the PCG is held to the oracle's iteration
+count within one and to the solution within the solve tolerance's reach."""
+import os
+
+import numpy as np
+import pytest
+
+import gpu_util as G
+import oracle_lib as O
+from brae_amd import hexmesh, mfv
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+class System:
+    """A folded symmetric pressure-like system on a mesh, with oracle and device copies."""
+
+    def __init__(self, nC, own, nei, bFaceCell, magSf, Sf, diag, upper, b, label):
+        self.label = label
+        self.nC, self.own, self.nei = nC, O.i32(own), O.i32(nei)
+        self.magSf, self.Sf = O.f64(magSf), O.f64(Sf)
+        self.diag, self.upper, self.b = O.f64(diag), O.f64(upper), O.f64(b)
+        self.case = G.Case(nC, self.own, self.nei, O.i32(bFaceCell))
+        self.dd, self.du, self.db = G.dev(self.diag), G.dev(self.upper), G.dev(self.b)
+
+
+def pitz_system():
+    d = np.load(os.path.join(GOLD, "pitzdaily.npz"))
+    m = O.Mesh.from_npz(d)
+    nI = m.nInternal
+    diag, up, _ = O.laplacian(m, m.deltaCoeffs, np.ones(nI))
+    types = np.zeros(m.nBFaces, np.int32)
+    for name, t in zip(m.patchNames, d["T_types"]):
+        types[m.patch_slice(name)] = {"fixedValue": 1, "zeroGradient": 0, "empty": 2}[str(t)]
+    iC, bC = O.laplacian_bcoeffs(m, types, d["T_values"], np.ones(m.nBFaces))
+    diagC, b = O.fold(m, -diag, np.zeros(m.nCells), types, -iC, -bC)
+    return System(m.nCells, m.own(), m.neighbour, m.bFaceCell, m.magSf[:nI], m.Sf[:nI], diagC, -up, b, "pitzDaily")
+
+
+def channel_system(nx=40, ny=20, nz=12, seed=5):
+    """-laplacian(gamma, p) on a graded jittered channel, outlet fixedValue 0, random source."""
+    m = hexmesh.box_mesh(nx, ny, nz, length=(4.0, 1.0, 0.5), grading=(1.0, 20.0, 20.0), jitter=0.1, seed=seed)
+    nI, nC = m["nInternal"], m["nCells"]
+    rng = np.random.default_rng(seed)
+    gamma = rng.uniform(0.5, 2.0, nI)
+    own = np.ascontiguousarray(m["owner"][:nI])
+    diag, up, lo = np.zeros(nC), np.zeros(nI), np.zeros(nI)
+    O.lib().orc_laplacian(nC, nI, O.p(own), O.p(m["neighbour"]), O.p(m["nonOrthDeltaCoeffs"]), O.p(gamma),
+                          O.p(O.f64(m["magSf"][:nI])), O.p(diag), O.p(up), O.p(lo))
+    nB = m["faces"].shape[0] - nI
+    types = np.zeros(nB, np.int32)
+    types[hexmesh.patch_slice(m, "outlet")] = O.BC_FIXED_VALUE
+    iC, bC = np.zeros(nB), np.zeros(nB)
+    O.lib().orc_laplacian_bcoeffs(nB, O.p(types), O.p(np.zeros(nB)), O.p(np.ones(nB)), O.p(O.f64(m["magSf"][nI:])),
+                                  O.p(O.f64(m["bDeltaCoeffs"])), O.p(iC), O.p(bC))
+    bfc = np.ascontiguousarray(m["owner"][nI:])
+    diagC, src = np.zeros(nC), np.zeros(nC)
+    O.lib().orc_fold(nC, O.p(-diag), O.p(rng.normal(size=nC) * m["V"]), nB, O.p(bfc), O.p(types), O.p(-iC),
+                     O.p(-bC), O.p(diagC), O.p(src))
+    return System(nC, own, m["neighbour"], bfc, m["magSf"][:nI], m["Sf"][:nI], diagC, -up, src, "channel")
+
+
+@pytest.fixture(scope="module")
+def systems():
+    return [pitz_system(), channel_system()]
+
+
+def build_pair(S, kind):
+    if kind == "brae":
+        w = S.magSf
+        ag = O.brae_agglomeration(S.nC, S.own, S.nei, w, target=64)
+        amg = mfv.Amg(S.case.mesh, w, mfv.AGG_BRAE, 64)
+    else:
+        w = O.faceareapair_weights(S.Sf, S.magSf)
+        ag = O.of_agglomeration(S.nC, S.own, S.nei, w, nCellsInCoarsestLevel=10)
+        amg = mfv.Amg(S.case.mesh, w, mfv.AGG_FACEAREAPAIR, 10)
+    return ag, amg
+
+
+@pytest.mark.parametrize("kind", ["brae", "faceAreaPair"])
+def test_hierarchy_bit_exact(systems, kind):
+    for S in systems:
+        print("case:", S.label, kind, flush=True)
+        ag, amg = build_pair(S, kind)
+        assert amg.nGrids == ag.nLevels + 1
+        for i, L in enumerate(ag.levels):
+            got = amg.level(i)
+            for k in ("map", "cOwn", "cNei", "faceRestrict"):
+                assert np.array_equal(got[k], L[k]), (S.label, i, k)
+            assert np.array_equal(got["faceFlip"], L["faceFlip"]), (S.label, i)
+        amg.close()
+
+
+def test_case20_hierarchies_match_dumps():
+    d = np.load(os.path.join(GOLD, "case20.npz"))
+    m = O.Mesh.from_npz(d)
+    case = G.Case(m.nCells, m.own(), m.neighbour, m.bFaceCell)
+    amg = mfv.Amg(case.mesh, m.magSf[:m.nInternal], mfv.AGG_BRAE, 4)
+    ref = d["brae_hierarchy"]
+    assert amg.nGrids - 1 == ref[:, 0].max() + 1
+    for i in range(amg.nGrids - 1):
+        rows = ref[ref[:, 0] == i]
+        assert np.array_equal(amg.level(i)["map"][rows[:, 1]], rows[:, 2])
+    w = O.faceareapair_weights(m.Sf[:m.nInternal], m.magSf[:m.nInternal])
+    amg2 = mfv.Amg(case.mesh, w, mfv.AGG_FACEAREAPAIR, 4)
+    ref = d["of_hierarchy"]
+    assert amg2.nGrids - 1 == ref[:, 0].max() + 1
+    for i in range(amg2.nGrids - 1):
+        rows = ref[ref[:, 0] == i]
+        assert np.array_equal(amg2.level(i)["map"][rows[:, 1]], rows[:, 2])
+
+
+def test_galerkin_bit_exact(systems):
+    for S in systems:
+        ag, amg = build_pair(S, "brae")
+        amg.galerkin(S.dd, S.du, mfv.PREC_FP64)
+        ref = ag.galerkin(S.diag, S.upper, order="brae")
+        for g, (cd, cu) in enumerate(ref, start=1):
+            d, u = amg.grid_values(g)
+            assert np.array_equal(d, cd), (S.label, g)
+            assert np.array_equal(u, cu), (S.label, g)
+        amg.close()
+
+
+@pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32])
+@pytest.mark.parametrize("sweeps", [(1, 1), (2, 2), (0, 1)])
+def test_vcycle_bit_exact(systems, prec, sweeps):
+    for S in systems:
+        print("case:", S.label, prec, sweeps, flush=True)
+        ag, amg = build_pair(S, "brae")
+        amg.galerkin(S.dd, S.du, prec)
+        r = np.random.default_rng(4).normal(size=S.nC)
+        dr, dz = G.dev(r), G.zeros(S.nC)
+        amg.vcycle(dr, dz, *sweeps)
+        z = G.host(dz)
+        ref = O.brae_vcycle(ag, S.diag, S.upper, r, *sweeps, single=(prec == mfv.PREC_FP32))
+        assert np.array_equal(z, ref), (S.label, np.abs(z - ref).max())
+        amg.close()
+
+
+@pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32])
+@pytest.mark.parametrize("flexible", [False, True])
+def test_pcg_bit_exact_with_sequential_reductions(systems, prec, flexible):
+    for S in systems:
+        print("case:", S.label, prec, flexible, flush=True)
+        ag, amg = build_pair(S, "brae")
+        amg.galerkin(S.dd, S.du, prec)
+        S.case.ctx.sequential_reduce(True)
+        psi = G.zeros(S.nC)
+        perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-9, precision=prec, flexible=int(flexible),
+                         checkEvery=3)
+        S.case.ctx.sequential_reduce(False)
+        ref_psi, ref = O.brae_amgpcg(ag, S.diag, S.upper, S.b, np.zeros(S.nC), 1e-9, flexible=flexible,
+                                     single=(prec == mfv.PREC_FP32))
+        assert perf.nIterations == ref.nIterations, (S.label, perf.nIterations, ref.nIterations)
+        assert perf.initialResidual == ref.initialResidual
+        assert perf.finalResidual == ref.finalResidual
+        assert np.array_equal(G.host(psi), ref_psi)
+        amg.close()
+
+
+@pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32, mfv.PREC_FP16])
+def test_pcg_tree_reductions_converge_like_oracle(systems, prec):
+    for S in systems:
+        ag, amg = build_pair(S, "brae")
+        amg.galerkin(S.dd, S.du, prec)
+        psi = G.zeros(S.nC)
+        perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-10, precision=prec, flexible=int(prec != 64))
+        ref_psi, ref = O.brae_amgpcg(ag, S.diag, S.upper, S.b, np.zeros(S.nC), 1e-10, flexible=prec != 64,
+                                     single=(prec != mfv.PREC_FP64))
+        assert perf.converged and perf.finalResidual < 1e-10
+        slack = 1 if prec != mfv.PREC_FP16 else max(3, ref.nIterations // 5)
+        assert abs(perf.nIterations - ref.nIterations) <= slack, (S.label, perf.nIterations, ref.nIterations)
+        x = G.host(psi)
+        # both converged to 1e-10 of the normalised residual: the solutions agree to that order
+        assert np.abs(x - ref_psi).max() <= 1e-7 * np.abs(ref_psi).max(), S.label
+        amg.close()
+
+
+def test_pcg_stop_rules(systems):
+    S = systems[1]
+    _, amg = build_pair(S, "brae")
+    amg.galerkin(S.dd, S.du, mfv.PREC_FP32)
+    psi = G.zeros(S.nC)
+    perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-30, maxIter=5, checkEvery=2)
+    assert perf.nIterations == 5 and not perf.converged
+    # minIter keeps iterating past a converged start
+    psi2 = G.zeros(S.nC)
+    perf = amg.solve(S.dd, S.du, S.db, psi2, tolerance=1e30, minIter=3)
+    assert perf.nIterations == 3
+    # the host polling interval does not change the result
+    a, b = G.zeros(S.nC), G.zeros(S.nC)
+    p1 = amg.solve(S.dd, S.du, S.db, a, tolerance=1e-8, checkEvery=1)
+    p7 = amg.solve(S.dd, S.du, S.db, b, tolerance=1e-8, checkEvery=7)
+    assert p1.nIterations == p7.nIterations and np.array_equal(G.host(a), G.host(b))
+    amg.close()
+
+
+def test_case20_trace_fp32():
+    """brae_cycles.csv (the reference's own deviceAMGPCG, FP32 cycle, normFactor sum|b|+1e-20)."""
+    import torch
+    d = np.load(os.path.join(GOLD, "case20.npz"))
+    m = O.Mesh.from_npz(d)
+    case = G.Case(m.nCells, m.own(), m.neighbour, m.bFaceCell)
+    amg = mfv.Amg(case.mesh, m.magSf[:m.nInternal], mfv.AGG_BRAE, 4)
+    dd, du, db = G.dev(d["m_diag"]), G.dev(d["m_upper"]), G.dev(d["rhs"])
+    amg.galerkin(dd, du, mfv.PREC_FP32)
+    nf = torch.tensor([np.abs(d["rhs"]).sum() + 1e-20], dtype=torch.float64, device="cuda:0")
+    cyc = d["brae_cycles"]
+    for k in range(1, 8):
+        psi = G.zeros(m.nCells)
+        perf = amg.solve(dd, du, db, psi, normFactor=nf, tolerance=0.0, maxIter=k)
+        assert perf.nIterations == k
+        ref = cyc[cyc[:, 0] == k][:, 2]
+        assert np.abs(G.host(psi) - ref).max() < 2e-6, k

@@ -1,0 +1,289 @@
+"""GPU parity of the LDU kernels against the oracle: assembly, fold, products, reductions.
+
+Integer/index work and the fixed-order FP64 face loops are held BIT-EXACT to the oracle (the
+kernels walk each cell's faces in the reference's ascending face order and the library is built
+without FMA contraction); tree reductions are held to a stated relative tolerance and to
+run-to-run bit equality, and in sequential mode to bit equality with the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import gpu_util as G
+import oracle_lib as O
+from brae_amd import mfv
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def pitz():
+    d = np.load(os.path.join(GOLD, "pitzdaily.npz"))
+    mesh = O.Mesh.from_npz(d)
+    case = G.Case(mesh.nCells, mesh.own(), mesh.neighbour, mesh.bFaceCell)
+    return d, mesh, case
+
+
+def t_types(d, mesh):
+    types = np.zeros(mesh.nBFaces, np.int32)
+    for name, t in zip(mesh.patchNames, d["T_types"]):
+        types[mesh.patch_slice(name)] = {"fixedValue": 1, "zeroGradient": 0, "empty": 2}[str(t)]
+    return types
+
+
+def test_laplacian_assembly_bit_exact(pitz):
+    d, mesh, case = pitz
+    print("case: pitzDaily laplacian, nCells", mesh.nCells, flush=True)
+    nC, nI, nB = mesh.nCells, mesh.nInternal, mesh.nBFaces
+    rng = np.random.default_rng(11)
+    gamma = rng.uniform(0.5, 2.0, nI)
+    dc = mesh.nonOrthDeltaCoeffs
+    o_diag, o_up, o_lo = O.laplacian(mesh, dc, gamma)
+    ddc, dg, dm = G.dev(dc), G.dev(gamma), G.dev(mesh.magSf[:nI])
+    diag, up = G.zeros(nC), G.zeros(nI)
+    case.call("mfv_laplacian_coeffs", case.mesh.h, mfv.ptr(ddc), mfv.ptr(dg), mfv.ptr(dm), mfv.ptr(diag),
+              mfv.ptr(up), mfv.ptr(up))
+    case.sync()
+    assert np.array_equal(G.host(diag), o_diag)
+    assert np.array_equal(G.host(up), o_up)
+    # boundary coefficients for T's patch fields
+    types = t_types(d, mesh)
+    gb = rng.uniform(0.5, 2.0, nB)
+    o_iC, o_bC = O.laplacian_bcoeffs(mesh, types, d["T_values"], gb)
+    iC, bC = G.zeros(nB), G.zeros(nB)
+    dtypes, dvals, dgb = G.dev(types), G.dev(d["T_values"]), G.dev(gb)
+    dbm, dbd = G.dev(mesh.magSf[nI:]), G.dev(mesh.bDeltaCoeffs)
+    case.call("mfv_laplacian_bcoeffs", case.mesh.h, mfv.ptr(dtypes), mfv.ptr(dvals), mfv.ptr(dgb), mfv.ptr(dbm),
+              mfv.ptr(dbd), mfv.ptr(iC), mfv.ptr(bC))
+    case.sync()
+    assert np.array_equal(G.host(iC), o_iC)
+    assert np.array_equal(G.host(bC), o_bC)
+    # fold against the oracle, and against OpenFOAM's dump for gamma = 1 through the same kernels
+    src = rng.normal(size=nC)
+    o_dC, o_b = O.fold(mesh, o_diag, src, types, o_iC, o_bC)
+    dC, b = G.zeros(nC), G.zeros(nC)
+    dsrc = G.dev(src)
+    case.call("mfv_fold", case.mesh.h, mfv.ptr(diag), mfv.ptr(dsrc), mfv.ptr(iC), mfv.ptr(bC), mfv.ptr(dC), mfv.ptr(b))
+    case.sync()
+    assert np.array_equal(G.host(dC), o_dC)
+    assert np.array_equal(G.host(b), o_b)
+
+
+def test_laplacian_matches_openfoam_dump_on_gpu(pitz):
+    d, mesh, case = pitz
+    nC, nI = mesh.nCells, mesh.nInternal
+    diag, up = G.zeros(nC), G.zeros(nI)
+    ddc, dg, dm = G.dev(mesh.deltaCoeffs), G.dev(np.ones(nI)), G.dev(mesh.magSf[:nI])
+    case.call("mfv_laplacian_coeffs", case.mesh.h, mfv.ptr(ddc), mfv.ptr(dg), mfv.ptr(dm), mfv.ptr(diag),
+              mfv.ptr(up), None)
+    case.sync()
+    for a, ref in ((G.host(diag), d["lap_diag"]), (G.host(up), d["lap_upper"])):
+        assert np.abs(a - ref).max() / np.abs(ref).max() <= 1e-12
+
+
+def test_div_upwind_bit_exact(pitz):
+    import ctypes
+    d, mesh, case = pitz
+    nC, nI, nB = mesh.nCells, mesh.nInternal, mesh.nBFaces
+    phi = d["phi_internal"]
+    od, ou, ol = np.zeros(nC), np.zeros(nI), np.zeros(nI)
+    O.lib().orc_div_upwind(nC, nI, O.p(mesh.own()), O.p(mesh.neighbour), O.p(O.f64(phi)), O.p(od), O.p(ou), O.p(ol))
+    diag, up, lo = G.zeros(nC), G.zeros(nI), G.zeros(nI)
+    dphi = G.dev(phi)
+    case.call("mfv_div_upwind_coeffs", case.mesh.h, mfv.ptr(dphi), mfv.ptr(diag), mfv.ptr(up), mfv.ptr(lo))
+    case.sync()
+    assert np.array_equal(G.host(diag), od) and np.array_equal(G.host(up), ou) and np.array_equal(G.host(lo), ol)
+    types = t_types(d, mesh)
+    oi, ob = np.zeros(nB), np.zeros(nB)
+    O.lib().orc_div_bcoeffs(nB, O.p(types), O.p(O.f64(d["T_values"])), O.p(O.f64(d["phi_boundary"])), O.p(oi), O.p(ob))
+    iC, bC = G.zeros(nB), G.zeros(nB)
+    dt, dv, dp = G.dev(types), G.dev(d["T_values"]), G.dev(d["phi_boundary"])
+    case.call("mfv_div_bcoeffs", case.mesh.h, mfv.ptr(dt), mfv.ptr(dv), mfv.ptr(dp), mfv.ptr(iC), mfv.ptr(bC))
+    case.sync()
+    assert np.array_equal(G.host(iC), oi) and np.array_equal(G.host(bC), ob)
+    # asymmetric product through both off-diagonals
+    x = np.random.default_rng(5).normal(size=nC)
+    y = G.zeros(nC)
+    dx = G.dev(x)
+    A = mfv.ldu(diag, up, lo)
+    case.call("mfv_amul", case.mesh.h, ctypes.byref(A), mfv.ptr(dx), mfv.ptr(y))
+    case.sync()
+    assert np.array_equal(G.host(y), O.amul(nC, mesh.own(), mesh.neighbour, od, ou, ol, x))
+
+
+def test_amul_residual_flux_bit_exact(pitz):
+    import ctypes
+    d, mesh, case = pitz
+    nC, nI = mesh.nCells, mesh.nInternal
+    rng = np.random.default_rng(2)
+    diag, up = rng.normal(size=nC) + 8, rng.normal(size=nI)
+    x, b = rng.normal(size=nC), rng.normal(size=nC)
+    dd, du, dx, db = G.dev(diag), G.dev(up), G.dev(x), G.dev(b)
+    y, r, fl = G.zeros(nC), G.zeros(nC), G.zeros(nI)
+    A = mfv.ldu(dd, du)
+    case.call("mfv_amul", case.mesh.h, ctypes.byref(A), mfv.ptr(dx), mfv.ptr(y))
+    case.call("mfv_residual", case.mesh.h, ctypes.byref(A), mfv.ptr(dx), mfv.ptr(db), mfv.ptr(r))
+    case.call("mfv_face_flux", case.mesh.h, ctypes.byref(A), mfv.ptr(dx), mfv.ptr(fl))
+    case.sync()
+    assert np.array_equal(G.host(y), O.amul(nC, mesh.own(), mesh.neighbour, diag, up, None, x))
+    orr = np.zeros(nC)
+    O.lib().orc_residual(nC, nI, O.p(mesh.own()), O.p(mesh.neighbour), O.p(diag), O.p(up), None, O.p(x), O.p(b), O.p(orr))
+    assert np.array_equal(G.host(r), orr)
+    of = np.zeros(nI)
+    O.lib().orc_face_flux(nI, O.p(mesh.own()), O.p(mesh.neighbour), O.p(up), None, O.p(x), O.p(of))
+    assert np.array_equal(G.host(fl), of)
+    # amul.dat (OpenFOAM) through the GPU product
+    dl, ul, ll = O.laplacian(mesh, mesh.deltaCoeffs, np.ones(nI))
+    dd2, du2, dpsi = G.dev(dl), G.dev(ul), G.dev(d["amul_psi"])
+    A2 = mfv.ldu(dd2, du2)
+    case.call("mfv_amul", case.mesh.h, ctypes.byref(A2), mfv.ptr(dpsi), mfv.ptr(y))
+    case.sync()
+    ref = d["amul_Apsi"]
+    assert np.abs(G.host(y) - ref).max() / np.abs(ref).max() <= 1e-12
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1000, 262144, 3_000_001])
+def test_reductions(n):
+    import ctypes
+    rng = np.random.default_rng(n + 1)
+    a, b = rng.normal(size=n), rng.normal(size=n)
+    da, db = G.dev(a), G.dev(b)
+    out = G.zeros(4)
+    ctx = mfv.Context(0)
+    ctx.call("mfv_dot", n, mfv.ptr(da), mfv.ptr(db), out.data_ptr())
+    ctx.call("mfv_sum_mag", n, mfv.ptr(da), out.data_ptr() + 8)
+    ctx.sync()
+    o = G.host(out)
+    ref_dot, ref_mag = float(np.dot(a, b)) if n else 0.0, float(np.abs(a).sum())
+    scale = max(1.0, float(np.abs(a * b).sum()))
+    assert abs(o[0] - ref_dot) <= 1e-14 * scale * max(1.0, np.log2(max(n, 2)))
+    assert abs(o[1] - ref_mag) <= 1e-14 * max(1.0, ref_mag)
+    # run-to-run bit equality of the tree
+    ctx.call("mfv_dot", n, mfv.ptr(da), mfv.ptr(db), out.data_ptr() + 16)
+    ctx.sync()
+    o2 = G.host(out)
+    assert o2[2] == o[0]
+    # sequential mode equals the oracle's ascending loop to the bit
+    if n <= 262144:
+        ctx.sequential_reduce(True)
+        ctx.call("mfv_dot", n, mfv.ptr(da), mfv.ptr(db), out.data_ptr())
+        ctx.call("mfv_sum_mag", n, mfv.ptr(da), out.data_ptr() + 8)
+        ctx.sync()
+        o = G.host(out)
+        assert o[0] == O.lib().orc_sum_prod(n, O.p(a), O.p(b))
+        assert o[1] == O.lib().orc_sum_mag(n, O.p(a))
+    ctx.close()
+    del ctypes
+
+
+def test_norm_factor(pitz):
+    import ctypes
+    d, mesh, case = pitz
+    nC, nI = mesh.nCells, mesh.nInternal
+    rng = np.random.default_rng(9)
+    diag, up = rng.normal(size=nC) + 8, rng.normal(size=nI)
+    psi, b = rng.normal(size=nC), rng.normal(size=nC)
+    ref = O.norm_factor(nC, mesh.own(), mesh.neighbour, diag, up, psi, b)
+    dd, du, dp, db = G.dev(diag), G.dev(up), G.dev(psi), G.dev(b)
+    out = G.zeros(2)
+    A = mfv.ldu(dd, du)
+    case.call("mfv_norm_factor", case.mesh.h, ctypes.byref(A), mfv.ptr(dp), mfv.ptr(db), out.data_ptr())
+    case.ctx.sequential_reduce(True)
+    case.call("mfv_norm_factor", case.mesh.h, ctypes.byref(A), mfv.ptr(dp), mfv.ptr(db), out.data_ptr() + 8)
+    case.ctx.sequential_reduce(False)
+    case.sync()
+    o = G.host(out)
+    assert o[1] == ref
+    assert abs(o[0] - ref) <= 1e-13 * ref
+
+
+def test_hex_channel_addressing_and_products():
+    """A graded, jittered channel: the product and assembly on a mesh like the benchmark's."""
+    import ctypes
+    m, case = G.hex_case(24, 12, 8, length=(4.0, 1.0, 0.5), grading=(1.0, 20.0, 20.0), jitter=0.1, seed=3)
+    nC, nI = m["nCells"], m["nInternal"]
+    own = m["owner"][:nI]
+    gamma = np.random.default_rng(1).uniform(0.1, 1.0, nI)
+    od, ou, _ = (np.zeros(nC), np.zeros(nI), None)
+    O.lib().orc_laplacian(nC, nI, O.p(O.i32(own)), O.p(m["neighbour"]), O.p(m["nonOrthDeltaCoeffs"]), O.p(gamma),
+                          O.p(O.f64(m["magSf"][:nI])), O.p(od), O.p(ou), None)
+    diag, up = G.zeros(nC), G.zeros(nI)
+    a1, a2, a3 = G.dev(m["nonOrthDeltaCoeffs"]), G.dev(gamma), G.dev(m["magSf"][:nI])
+    case.call("mfv_laplacian_coeffs", case.mesh.h, mfv.ptr(a1), mfv.ptr(a2), mfv.ptr(a3), mfv.ptr(diag), mfv.ptr(up), None)
+    x = np.random.default_rng(2).normal(size=nC)
+    dx, y = G.dev(x), G.zeros(nC)
+    A = mfv.ldu(diag, up)
+    case.call("mfv_amul", case.mesh.h, ctypes.byref(A), mfv.ptr(dx), mfv.ptr(y))
+    case.sync()
+    assert np.array_equal(G.host(diag), od) and np.array_equal(G.host(up), ou)
+    assert np.array_equal(G.host(y), O.amul(nC, own, m["neighbour"], od, ou, None, x))
+
+
+def test_geometry_kernel_bit_exact():
+    """Device FvGeometry on a graded jittered channel and on pitzDaily's polygon faces, against the
+    oracle's restatement of fv_geometry.cu, to the bit."""
+    d = np.load(os.path.join(GOLD, "pitzdaily.npz"))
+    pz = O.Mesh.from_npz(d)
+    from brae_amd import hexmesh
+    hm = hexmesh.box_mesh(20, 10, 6, length=(4.0, 1.0, 0.5), grading=(1.0, 20.0, 20.0), jitter=0.1, seed=9)
+    nFh = hm["faces"].shape[0]
+    hx = O.Mesh(hm["points"], np.arange(0, 4 * nFh + 1, 4), hm["faces"].ravel(), hm["owner"], hm["neighbour"])
+    for om in (pz, hx):
+        case = G.Case(om.nCells, om.own(), om.neighbour, om.bFaceCell)
+        nF, nI, nC, nB = om.nFaces, om.nInternal, om.nCells, om.nBFaces
+        dp, dfo, dfv = G.dev(om.points), G.dev(om.faceOffsets), G.dev(om.faceVerts)
+        Cf, Sf, mS = G.zeros(3 * nF), G.zeros(3 * nF), G.zeros(nF)
+        C, V = G.zeros(3 * nC), G.zeros(nC)
+        w, dc, ndc, cv, bd = G.zeros(nI), G.zeros(nI), G.zeros(nI), G.zeros(3 * nI), G.zeros(nB)
+        case.call("mfv_mesh_geometry", case.mesh.h, *[mfv.ptr(t) for t in (dp, dfo, dfv, Cf, Sf, mS, C, V, w, dc, ndc,
+                                                                           cv, bd)])
+        case.sync()
+        for got, ref in ((Cf, om.Cf), (Sf, om.Sf), (mS, om.magSf), (C, om.C), (V, om.V), (w, om.weights),
+                         (dc, om.deltaCoeffs), (ndc, om.nonOrthDeltaCoeffs), (cv, om.corrVec), (bd, om.bDeltaCoeffs)):
+            assert np.array_equal(G.host(got).reshape(ref.shape), ref)
+
+
+def test_interpolate_div_source_flux_correct(pitz):
+    import ctypes
+    d, mesh, case = pitz
+    nC, nI, nB = mesh.nCells, mesh.nInternal, mesh.nBFaces
+    rng = np.random.default_rng(21)
+    vol = rng.uniform(0.5, 2.0, nC)
+    w = mesh.weights
+    ref_face = w * vol[mesh.own()] + (1.0 - w) * vol[mesh.neighbour]
+    ref_b = vol[mesh.bFaceCell]
+    dw, dv = G.dev(w), G.dev(vol)
+    f, bf = G.zeros(nI), G.zeros(nB)
+    case.call("mfv_interpolate", case.mesh.h, mfv.ptr(dw), mfv.ptr(dv), mfv.ptr(f), mfv.ptr(bf))
+    case.sync()
+    assert np.array_equal(G.host(f), ref_face) and np.array_equal(G.host(bf), ref_b)
+    # fvc::div(phi)*V added with a sign, in the reference's face order (fvc.cu:706-730)
+    phi, phib = d["phi_internal"], d["phi_boundary"]
+    ref = np.zeros(nC)
+    for fi in range(nI):
+        ref[mesh.own()[fi]] += phi[fi]
+        ref[mesh.neighbour[fi]] -= phi[fi]
+    types = t_types(d, mesh)
+    for i in range(nB):
+        if types[i] != 2:
+            ref[mesh.bFaceCell[i]] += phib[i]
+    ref = ref / mesh.V
+    base = rng.normal(size=nC)
+    expect = base + (-1.0) * (ref * mesh.V)
+    src = G.dev(base)
+    phib_used = np.where(types == 2, 0.0, phib)
+    dphi, dphib, dV = G.dev(phi), G.dev(phib_used), G.dev(mesh.V)
+    case.call("mfv_div_source", case.mesh.h, mfv.ptr(dphi), mfv.ptr(dphib), mfv.ptr(dV), -1.0, mfv.ptr(src))
+    case.sync()
+    assert np.abs(G.host(src) - expect).max() <= 1e-15 * np.abs(expect).max()
+    # flux correction against the oracle's face flux
+    diag, up = rng.normal(size=nC), rng.normal(size=nI)
+    psi = rng.normal(size=nC)
+    of = np.zeros(nI)
+    O.lib().orc_face_flux(nI, O.p(mesh.own()), O.p(mesh.neighbour), O.p(up), None, O.p(psi), O.p(of))
+    dd, du, dps, dphiH, out = G.dev(diag), G.dev(up), G.dev(psi), G.dev(phi), G.zeros(nI)
+    A = mfv.ldu(dd, du)
+    case.call("mfv_flux_correct", case.mesh.h, ctypes.byref(A), mfv.ptr(dps), mfv.ptr(dphiH), -1.0, mfv.ptr(out))
+    case.sync()
+    assert np.array_equal(G.host(out), phi - (-1.0) * of)
