@@ -27,6 +27,8 @@ import numpy as np  # noqa: E402
 
 METRIC = "s/SIMPLE-iter @20M cells, FP64 vs mixed AMG at equal resid; solves/s + p99"
 PEAK_HBM_GBS = 8000.0
+NTAGS = 144
+GRID_DOWN, GRID_UP = 16, 80
 
 
 def log(*a):
@@ -44,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-sample", default="250x125x80", help="channel the CPU oracle is timed on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-every", type=int, default=4)
+    ap.add_argument("--no-scale", action="store_true", help="plain injection instead of the scaled correction")
+    ap.add_argument("--npre", type=int, default=1)
+    ap.add_argument("--npost", type=int, default=1)
     return ap.parse_args()
 
 
@@ -142,7 +147,8 @@ def gpu_run(args):
         ctx.call("mfv_fold", ch.mesh.h, P(ch.diag), P(ch.source), P(ch.iC), P(ch.bC), P(ch.diagC), P(ch.b))
         amg.galerkin(ch.diagC, ch.upper, prec)
         perf = amg.solve(ch.diagC, ch.upper, ch.b, ch.p, tolerance=args.tolerance, precision=prec,
-                         flexible=int(prec != mfv.PREC_FP64), checkEvery=args.check_every)
+                         flexible=int(prec != mfv.PREC_FP64 or not args.no_scale), checkEvery=args.check_every,
+                         scaleCorrection=int(not args.no_scale), nPreSweeps=args.npre, nPostSweeps=args.npost)
         ctx.call("mfv_flux_correct", ch.mesh.h, ctypes.byref(A), P(ch.p), P(ch.phiHbyA), -1.0, P(ch.phi))
         return perf
 
@@ -152,7 +158,7 @@ def gpu_run(args):
         torch.cuda.synchronize()
         if profile:
             ctx.profile(True)
-            for tag in range(7):
+            for tag in range(NTAGS):
                 ctx.profile_read(tag)
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -161,7 +167,7 @@ def gpu_run(args):
         dt = (time.perf_counter() - t) / args.steps
         prof = None
         if profile:
-            prof = {tag: ctx.profile_read(tag) for tag in range(7)}
+            prof = {tag: ctx.profile_read(tag) for tag in range(NTAGS)}
             ctx.profile(False)
         return dt, perfs, prof
 
@@ -200,6 +206,13 @@ def gpu_run(args):
                                 "alg_GBps": round(names[t][1] / (prof[t][1] / max(1, launches) * 1e-3) / 1e9, 1)}
                   for t in names if prof[t][1] > 0}
     vc_n, vc_ms = prof[mfv.PROF_VCYCLE]
+    grids = []
+    for gi in range(amg.nGrids):
+        dn, up = prof[GRID_DOWN + gi][1], prof[GRID_UP + gi][1]
+        grids.append({"grid": gi, "cells": sizes[gi][0], "down_us": round(dn / max(1, iters) * 1e3, 1),
+                      "up_us": round(up / max(1, iters) * 1e3, 1)})
+        log("  grid %2d %10d cells: down %8.1f us  up %8.1f us" % (gi, sizes[gi][0], dn / max(1, iters) * 1e3,
+                                                                   up / max(1, iters) * 1e3))
     out = dict(
         dt_mixed=res["mixed"]["dt"], dt_fp64=res["fp64"]["dt"],
         iters_mixed=[q.nIterations for q in res["mixed"]["perfs"]],
@@ -209,7 +222,7 @@ def gpu_run(args):
         roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                       frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None, kernel=label,
                       alg_bytes_per_launch=int(alg_bytes), avg_launch_us=round(avg_ms * 1e3, 2)),
-        per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1),
+        per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1), grids=grids,
     )
     amg.close()
     ch.mesh.close()
@@ -306,7 +319,10 @@ def main():
         "pcg_iterations": {"mixed": r["iters_mixed"], "fp64": r["iters_fp64"]},
         "final_residual": {"mixed": r["final_mixed"], "fp64": r["final_fp64"]},
         "p_rel_l2_mixed_vs_fp64": r["p_rel_l2_mixed_vs_fp64"],
-        "amg": {"grids": r["n_grids"], "coarsest_cells": r["coarsest"], "vcycle_avg_us": r["vcycle_avg_us"]},
+        "amg": {"grids": r["n_grids"], "coarsest_cells": r["coarsest"], "vcycle_avg_us": r["vcycle_avg_us"],
+                "cycle": "V(%d,%d) weighted Jacobi%s" % (args.npre, args.npost,
+                                                        "" if args.no_scale else ", scaled correction"),
+                "per_grid_us": r["grids"]},
         "roofline": r["roofline"],
         "per_kernel": r["per_kernel"],
         "cpu_baseline": cpu,

@@ -166,8 +166,6 @@ int mfv_ctx_create(int device, void* hStream, mfv_ctx* out)
             ctx->c.ownsStream = true;
         }
         ctx->c.partials.resize((size_t)8 * kMaxReduceBlocks);
-        ctx->c.tickets.resize(8);
-        hipCheck(hipMemsetAsync(ctx->c.tickets.p, 0, 8 * sizeof(unsigned int), ctx->c.stream), "ticket clear");
         ctx->c.scalars.resize(64);
         hipCheck(hipMemsetAsync(ctx->c.scalars.p, 0, 64 * sizeof(double), ctx->c.stream), "scalar clear");
         hipCheck(hipHostMalloc((void**)&ctx->c.mailbox, 64 * sizeof(double), hipHostMallocDefault), "mailbox");
@@ -504,13 +502,17 @@ int mfv_amg_grid_values(mfv_ctx ctx, mfv_amg amg, int grid, double* hDiag, doubl
     });
 }
 
-int mfv_amg_vcycle(mfv_ctx ctx, mfv_amg amg, const double* dR, double* dZ, int nPreSweeps, int nPostSweeps)
+int mfv_amg_vcycle(mfv_ctx ctx, mfv_amg amg, const double* dR, double* dZ, int nPreSweeps, int nPostSweeps,
+                   int scaleCorrection)
 {
     return guarded(ctx, [&] {
         need(ctx && amg && amg->ctx == ctx, "amg vcycle: bad context or hierarchy");
         need(dR && dZ, "amg vcycle: null vector");
         need(nPreSweeps >= 0 && nPostSweeps >= 0, "amg vcycle: negative sweeps");
-        const VcycleControls vc{nPreSweeps, nPostSweeps};
+        VcycleControls vc;
+        vc.nPre = nPreSweeps;
+        vc.nPost = nPostSweeps;
+        vc.scale = scaleCorrection != 0;
         amgStartFrom(ctx->c, amg->a, dR, vc, nullptr);
         const void* res = amgVcycle(ctx->c, amg->a, vc, nullptr);
         amgResultToDouble(ctx->c, amg->a, res, dZ);
@@ -549,7 +551,6 @@ int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A,
             throw Error(kErrUnsupported, "mfv: pcg: smoother " + std::to_string(controls->smoother) +
                                              " is not carried (0 = weighted Jacobi)");
         if (controls->nVcycles > 1) throw Error(kErrUnsupported, "mfv: pcg: nVcycles above 1 is not carried");
-        if (controls->scaleCorrection) throw Error(kErrUnsupported, "mfv: pcg: correction scaling is not carried");
         PcgControls pc;
         pc.tolerance = controls->tolerance;
         pc.relTol = controls->relTol;
@@ -559,6 +560,7 @@ int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A,
         pc.nPre = controls->nPreSweeps;
         pc.nPost = controls->nPostSweeps;
         pc.flexible = controls->flexible != 0;
+        pc.scaleCorrection = controls->scaleCorrection != 0;
         pc.checkEvery = controls->checkEvery;
         const PcgPerf r = amgPcgSolve(ctx->c, mesh->m, amg->a, v.diag, v.upper, dB, dPsi, dNormFactor, pc);
         if (perf)
@@ -676,12 +678,14 @@ ProfScope::ProfScope(Ctx& ctx, int t) : c(ctx), tag(t)
     hipCheck(hipEventRecord(e0, c.stream), "event record");
 }
 
-ProfScope::~ProfScope()
+void ProfScope::close()
 {
     if (!e0) return;
-    if (hipEventRecord(e1, c.stream) != hipSuccess) return;
+    hipEvent_t a = e0, b = e1;
+    e0 = e1 = nullptr;
+    if (hipEventRecord(b, c.stream) != hipSuccess) return;
     if ((int)c.prof.size() < kProfCount) c.prof.resize(kProfCount);
-    c.prof[tag].emplace_back(e0, e1);
+    c.prof[tag].emplace_back(a, b);
 }
 
 } // namespace mfv

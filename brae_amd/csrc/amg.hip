@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kTpb) void residualK(SellView S, const Tm* __restri
                                                   const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
     const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, PlainX<Tv>{x});
     r[c] = b[c] - ax;
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kTpb) void smoothK(SellView S, const Tm* __restrict
                                                 const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
     const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, PlainX<Tv>{x});
     xOut[c] = x[c] + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
@@ -64,11 +64,85 @@ __global__ __launch_bounds__(kTpb) void prolongSmoothK(SellView S, const Tm* __r
                                                        Tv* __restrict__ xOut, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
     ProlongedX<Tv> X{x, map, xc};
     const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X);
     xOut[c] = X(c) + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+}
+
+// Scaled coarse correction, first half: e = xc[map] (prolonged), Ae = A e on the grid (e gathered at the
+// neighbours), and the two sums r.e, e.Ae accumulated in FP64; the last workgroup forms
+// alpha = (r.e)/(e.Ae) with the reference's guard (scaleFactorK: 1 when e.Ae <= 1e-30).
+struct AlphaPost
+{
+    double* alpha;
+    __device__ void operator()(double num, double den) const { *alpha = den > 1e-30 ? num / den : 1.0; }
+};
+
+template <typename Tv>
+struct CorrX
+{
+    const int* __restrict__ map;
+    const Tv* __restrict__ xc;
+    __device__ Tv operator()(int j) const { return xc[map[j]]; }
+};
+
+template <typename Tm, typename Tv>
+__global__ __launch_bounds__(kReduceBlock) void corrAmulK(SellView S, const Tm* __restrict__ val,
+                                                          const Tv* __restrict__ diag, const double* __restrict__ scale,
+                                                          const int* __restrict__ map, const Tv* __restrict__ xc,
+                                                          const Tv* __restrict__ r, Tv* __restrict__ Ae,
+                                                          double* pa, double* pb, double* seqA, double* seqB,
+                                                          const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    __shared__ double lds[kReduceBlock / 64];
+    const CorrX<Tv> E{map, xc};
+    double num = 0.0, den = 0.0;
+    MFV_ROWS(S.n, c)
+    {
+        const Tv ae = rowAx<Tm, Tv>(S, c, val, diag, scale, E);
+        Ae[c] = ae;
+        const double e = (double)E(c);
+        const double tn = e * (double)r[c], td = e * (double)ae;
+        if (seqA)
+        {
+            seqA[c] = tn;
+            seqB[c] = td;
+        }
+        else
+        {
+            num += tn;
+            den += td;
+        }
+    }
+    if (seqA) return;
+    const double sn = blockSum(num, lds);
+    const double sd = blockSum(den, lds);
+    if (threadIdx.x == 0)
+    {
+        pa[blockIdx.x] = sn;
+        pb[blockIdx.x] = sd;
+    }
+}
+
+// second half, in place: x = x + alpha*e + omega*(r - alpha*Ae)/D (smooth) or x + alpha*e (no sweep)
+template <typename Tv>
+__global__ __launch_bounds__(kTpb) void corrSmoothK(int n, const int* __restrict__ map, const Tv* __restrict__ xc,
+                                                    const Tv* __restrict__ r, const Tv* __restrict__ Ae,
+                                                    const Tv* __restrict__ diag, const double* __restrict__ alpha,
+                                                    int smooth, Tv* __restrict__ x, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    if (c >= n) return;
+    const Tv al = (Tv)(*alpha);
+    const Tv e = xc[map[c]];
+    if (smooth)
+        x[c] = x[c] + al * e + Tv(kOmega) * (r[c] - al * Ae[c]) / safeDiag(diag[c]);
+    else
+        x[c] = x[c] + al * e;
 }
 
 // prolongation alone (no post-sweep): x += xc[map]
@@ -77,7 +151,7 @@ __global__ __launch_bounds__(kTpb) void prolongK(int n, const int* __restrict__ 
                                                  Tv* __restrict__ x, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c < n) x[c] += xc[map[c]];
 }
 
@@ -90,7 +164,7 @@ __global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict_
                                                   Tv* __restrict__ xc, int fromZero, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    const int C = blockIdx.x * kTpb + threadIdx.x;
+    const int C = xcdBlock() * kTpb + threadIdx.x;
     if (C >= nc) return;
     Tv s = Tv(0);
     for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += r[cellList[k]];
@@ -102,7 +176,7 @@ template <typename Tv>
 __global__ __launch_bounds__(kTpb) void zeroK(int n, Tv* __restrict__ x, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c < n) x[c] = Tv(0);
 }
 
@@ -113,7 +187,7 @@ __global__ __launch_bounds__(kTpb) void startK(int n, const double* __restrict__
                                                const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     const Tv bv = (Tv)r[c];
     b[c] = bv;
@@ -148,7 +222,7 @@ __global__ __launch_bounds__(kTpb) void galDiagK(int nc, const int* __restrict__
                                                  const int* __restrict__ dfaceList, const double* __restrict__ fd,
                                                  const double* __restrict__ fu, double* __restrict__ cd)
 {
-    const int C = blockIdx.x * kTpb + threadIdx.x;
+    const int C = xcdBlock() * kTpb + threadIdx.x;
     if (C >= nc) return;
     double s = 0.0;
     for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += fd[cellList[k]];
@@ -165,7 +239,7 @@ __global__ __launch_bounds__(kTpb) void galFaceK(int ncf, const int* __restrict_
                                                  const int* __restrict__ faceList, const double* __restrict__ fu,
                                                  double* __restrict__ cu)
 {
-    const int F = blockIdx.x * kTpb + threadIdx.x;
+    const int F = xcdBlock() * kTpb + threadIdx.x;
     if (F >= ncf) return;
     double u = 0.0;
     for (int k = faceStart[F]; k < faceStart[F + 1]; ++k) u += fu[faceList[k]];
@@ -178,7 +252,7 @@ __global__ __launch_bounds__(kTpb) void fillK(int nf, const int* __restrict__ sl
                                               const double* __restrict__ up, const double* __restrict__ lo,
                                               const double* __restrict__ scale, Tm* __restrict__ val)
 {
-    const int f = blockIdx.x * kTpb + threadIdx.x;
+    const int f = xcdBlock() * kTpb + threadIdx.x;
     if (f >= nf) return;
     const double inv = scale ? 1.0 / *scale : 1.0;
     if constexpr (sizeof(Tm) == 2)
@@ -196,51 +270,43 @@ __global__ __launch_bounds__(kTpb) void fillK(int nf, const int* __restrict__ sl
 template <typename Tv>
 __global__ __launch_bounds__(kTpb) void castK(int n, const double* __restrict__ a, Tv* __restrict__ b)
 {
-    const int i = blockIdx.x * kTpb + threadIdx.x;
+    const int i = xcdBlock() * kTpb + threadIdx.x;
     if (i < n) b[i] = (Tv)a[i];
 }
 
 __global__ __launch_bounds__(kTpb) void widenK(int n, const float* __restrict__ a, double* __restrict__ b)
 {
-    const int i = blockIdx.x * kTpb + threadIdx.x;
+    const int i = xcdBlock() * kTpb + threadIdx.x;
     if (i < n) b[i] = (double)a[i];
 }
 
-// per-grid FP16 scale: the smallest power of two >= max|upper| (exact scaling)
-__global__ __launch_bounds__(kReduceBlock) void maxAbsPow2K(int n, const double* __restrict__ a,
-                                                           double* __restrict__ partials,
-                                                           unsigned int* __restrict__ ticket, double* __restrict__ out)
+// per-grid FP16 scale: the smallest power of two >= max|upper| (exact scaling); max is exact, so the
+// combine order does not matter
+__global__ __launch_bounds__(kReduceBlock) void maxAbsK(int n, const double* __restrict__ a,
+                                                       double* __restrict__ partials)
 {
     __shared__ double lds[kReduceBlock / 64];
     double v = 0.0;
-    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock) v = fmax(v, fabs(a[i]));
-    // max combine: butterfly max, then fixed wave order
+    MFV_ROWS(n, i) v = fmax(v, fabs(a[i]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) lds[wave] = v;
     __syncthreads();
-    double m = 0.0;
     if (threadIdx.x == 0)
+    {
+        double m = 0.0;
         for (int w = 0; w < kReduceBlock / 64; ++w) m = fmax(m, lds[w]);
-    __syncthreads();
-    __shared__ bool last;
-    if (threadIdx.x == 0)
-    {
         partials[blockIdx.x] = m;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x == 0)
-    {
-        double mm = 0.0;
-        for (int i = 0; i < (int)gridDim.x; ++i) mm = fmax(mm, partials[i]);
-        *out = mm > 0.0 ? exp2(ceil(log2(mm))) : 1.0;
-        *ticket = 0u;
-    }
+}
+
+__global__ void maxPow2FinishK(int nParts, const double* __restrict__ partials, double* __restrict__ out)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    double mm = 0.0;
+    for (int i = 0; i < nParts; ++i) mm = fmax(mm, partials[i]);
+    *out = mm > 0.0 ? exp2(ceil(log2(mm))) : 1.0;
 }
 
 // Gauss-Jordan inverse with partial pivoting (first largest |a_ik|, i >= k) on [A | I], the
@@ -337,7 +403,7 @@ __global__ __launch_bounds__(kTpb) void sellAmul64K(SellView S, const double* __
                                                     const double* __restrict__ diag, const double* __restrict__ x,
                                                     double* __restrict__ y)
 {
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
     y[c] = rowOffSum<double, double>(S, c, val, PlainX<double>{x}, diag[c] * x[c]);
 }
@@ -524,9 +590,11 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
     const int last = (int)a.grids.size() - 1;
     if (gi == last)
     {
+        ProfScope ps(c, kProfGridDown + gi);
         coarseApplyK<Tv><<<1, kTpb, 0, c.stream>>>(n, a.coarseInv.p, v.b, v.x, stop);
         return v.x;
     }
+    ProfScope psDown(c, kProfGridDown + gi);
     const SellView S = viewOf(g.sell);
     const Tm* val = offOf<Tm>(g).p;
     const double* sc = a.scales.p + gi;
@@ -549,8 +617,34 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
         restrictK<Tv><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, v.r, w.b, w.diag, w.x,
                                                             (!coarseIsLast && vc.nPre > 0) ? 1 : 0, stop);
     }
+    psDown.close();
     const Tv* xc = vcycleAt<Tm, Tv>(c, a, gi + 1, vc, stop, vs);
-    if (vc.nPost > 0)
+    ProfScope psUp(c, kProfGridUp + gi);
+    if (vc.scale)
+    {
+        const bool seq = c.sequentialReduce;
+        {
+            ProfScope ps(c, gi == 0 ? kProfAmgProlongSmooth0 : kProfCount);
+            const int gr = reduceGrid(n);
+            double* p1 = c.partials.p + 1 * (size_t)kMaxReduceBlocks;
+            double* p2 = c.partials.p + 2 * (size_t)kMaxReduceBlocks;
+            corrAmulK<Tm, Tv><<<gr, kReduceBlock, 0, c.stream>>>(S, val, v.diag, sc, g.map.p, xc, v.r, v.y, p1, p2,
+                                                                 seq ? a.seqA.p : nullptr, seq ? a.seqB.p : nullptr,
+                                                                 stop);
+            if (seq)
+                seqFinish2K<<<1, 64, 0, c.stream>>>(n, a.seqA.p, a.seqB.p, AlphaPost{a.alphas.p + gi}, stop);
+            else
+                finish2K<<<1, kReduceBlock, 0, c.stream>>>(gr, p1, p2, AlphaPost{a.alphas.p + gi}, stop);
+            corrSmoothK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, g.map.p, xc, v.r, v.y, v.diag, a.alphas.p + gi,
+                                                              vc.nPost > 0 ? 1 : 0, v.x, stop);
+        }
+        for (int s = 1; s < vc.nPost; ++s)
+        {
+            smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, stop);
+            std::swap(v.x, v.y);
+        }
+    }
+    else if (vc.nPost > 0)
     {
         {
             ProfScope ps(c, gi == 0 ? kProfAmgProlongSmooth0 : kProfCount);
@@ -633,6 +727,9 @@ void amgBuild(Ctx& c, const Mesh& m, Amg& a, const std::vector<double>& faceWeig
     a.scales.resize(nG);
     std::vector<double> ones(nG, 1.0);
     a.scales.upload(ones, c.stream);
+    a.alphas.resize(nG);
+    a.seqA.resize(m.nCells);
+    a.seqB.resize(m.nCells);
     a.galerkinDone = false;
     a.allocatedVectors = false;
     hipCheck(hipStreamSynchronize(c.stream), "amg build");
@@ -690,8 +787,10 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
                 if (g.off16.n != (size_t)nnz) zeroBuf(c, g.off16, nnz);
                 if (g.nFaces)
                 {
-                    maxAbsPow2K<<<reduceGrid(g.nFaces), kReduceBlock, 0, c.stream>>>(
-                        g.nFaces, u, c.partials.p + 4 * (size_t)kMaxReduceBlocks, c.tickets.p + 4, a.scales.p + gi);
+                    const int gm = reduceGrid(g.nFaces);
+                    double* pm = c.partials.p + 4 * (size_t)kMaxReduceBlocks;
+                    maxAbsK<<<gm, kReduceBlock, 0, c.stream>>>(g.nFaces, u, pm);
+                    maxPow2FinishK<<<1, 64, 0, c.stream>>>(gm, pm, a.scales.p + gi);
                     fillK<__half><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.sell.slotLo.p, g.sell.slotHi.p,
                                                                            u, u, a.scales.p + gi, g.off16.p);
                 }

@@ -93,6 +93,8 @@ struct Amg
     DBuf<double> coarseInv;        // dense inverse of the coarsest grid, row-major, FP64
     DBuf<double> coarseWork;       // Gauss-Jordan augmented matrix
     DBuf<double> scales;           // per grid: FP16 scale (1 otherwise)
+    DBuf<double> alphas;           // per grid: the scaled correction's alpha
+    DBuf<double> seqA, seqB;       // per-cell terms for sequential (OpenFOAM-order) reductions
     Prec prec = Prec::F32;         // precision the V-cycle copies are filled in
     bool galerkinDone = false;
     bool allocatedVectors = false;
@@ -109,6 +111,11 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
 struct VcycleControls
 {
     int nPre = 1, nPost = 1;
+    // Scale each grid's prolonged coarse correction e by the energy-minimising
+    // alpha = (r.e)/(e.Ae) (brae's corrScaling, device_amg_vcycle.cu:1113-1126; OpenFOAM GAMG's scale),
+    // fused with the first post-sweep: x = x + alpha*e + omega*(r - alpha*Ae)/D, r the residual the
+    // grid restricted. Makes the preconditioner nonlinear: use the flexible CG.
+    bool scale = false;
 };
 
 // One V-cycle on grid 0's b (set by amgStartFrom, with x = omega*b/D when nPre > 0). Returns the
@@ -133,6 +140,7 @@ struct PcgControls
     Prec prec = Prec::F32;
     int nPre = 1, nPost = 1;
     bool flexible = false;
+    bool scaleCorrection = false;
     int checkEvery = 4;
 };
 

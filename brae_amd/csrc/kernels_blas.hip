@@ -8,25 +8,21 @@ namespace mfv {
 namespace {
 
 __global__ __launch_bounds__(kReduceBlock) void dotK(int n, const double* __restrict__ a, const double* __restrict__ b,
-                                                     double* __restrict__ partials, unsigned int* __restrict__ ticket,
-                                                     double* __restrict__ out)
+                                                     double* __restrict__ partials)
 {
     __shared__ double lds[kReduceBlock / 64];
     double v = 0.0;
-    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock) v += a[i] * b[i];
-    const double s = blockSum(v, lds);
-    finishReduction(s, partials, ticket, lds, StoreTo{out});
+    MFV_ROWS(n, i) v += a[i] * b[i];
+    storePartial(blockSum(v, lds), partials);
 }
 
 __global__ __launch_bounds__(kReduceBlock) void sumMagK(int n, const double* __restrict__ a,
-                                                        double* __restrict__ partials,
-                                                        unsigned int* __restrict__ ticket, double* __restrict__ out)
+                                                        double* __restrict__ partials)
 {
     __shared__ double lds[kReduceBlock / 64];
     double v = 0.0;
-    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock) v += fabs(a[i]);
-    const double s = blockSum(v, lds);
-    finishReduction(s, partials, ticket, lds, StoreTo{out});
+    MFV_ROWS(n, i) v += fabs(a[i]);
+    storePartial(blockSum(v, lds), partials);
 }
 
 // OpenFOAM's own order: one thread, ascending index (sumProd / sumMag in lduMatrix).
@@ -63,8 +59,9 @@ void launchDot(Ctx& c, int n, const double* a, const double* b, double* out, int
     else
     {
         const int g = reduceGrid(n);
-        dotK<<<g, kReduceBlock, 0, c.stream>>>(n, a, b, c.partials.p + (size_t)slot * kMaxReduceBlocks,
-                                               c.tickets.p + slot, out);
+        double* pa = c.partials.p + (size_t)slot * kMaxReduceBlocks;
+        dotK<<<g, kReduceBlock, 0, c.stream>>>(n, a, b, pa);
+        finishK<<<1, kReduceBlock, 0, c.stream>>>(g, pa, StoreTo{out}, nullptr);
     }
     hipCheck(hipGetLastError(), "dot");
 }
@@ -78,8 +75,9 @@ void launchSumMag(Ctx& c, int n, const double* a, double* out, int slot)
     else
     {
         const int g = reduceGrid(n);
-        sumMagK<<<g, kReduceBlock, 0, c.stream>>>(n, a, c.partials.p + (size_t)slot * kMaxReduceBlocks,
-                                                  c.tickets.p + slot, out);
+        double* pa = c.partials.p + (size_t)slot * kMaxReduceBlocks;
+        sumMagK<<<g, kReduceBlock, 0, c.stream>>>(n, a, pa);
+        finishK<<<1, kReduceBlock, 0, c.stream>>>(g, pa, StoreTo{out}, nullptr);
     }
     hipCheck(hipGetLastError(), "sumMag");
 }

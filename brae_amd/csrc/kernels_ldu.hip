@@ -26,7 +26,7 @@ __global__ __launch_bounds__(kTpb) void amulK(int n, const label* __restrict__ o
                                               const double* __restrict__ upper, const double* __restrict__ lower,
                                               const double* __restrict__ x, double* __restrict__ y)
 {
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     double s = diag[c] * x[c];
     const int l0 = losortStart[c], l1 = losortStart[c + 1];
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kTpb) void residualK(int n, const label* __restrict
                                                   const double* __restrict__ lower, const double* __restrict__ x,
                                                   const double* __restrict__ b, double* __restrict__ r)
 {
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     double s = b[c] - diag[c] * x[c];
     const int l0 = losortStart[c], l1 = losortStart[c + 1];
@@ -57,14 +57,12 @@ __global__ __launch_bounds__(kTpb) void residualK(int n, const label* __restrict
 }
 
 __global__ __launch_bounds__(kReduceBlock) void sumK(int n, const double* __restrict__ a,
-                                                     double* __restrict__ partials, unsigned int* __restrict__ ticket,
-                                                     double* __restrict__ out)
+                                                     double* __restrict__ partials)
 {
     __shared__ double lds[kReduceBlock / 64];
     double v = 0.0;
-    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock) v += a[i];
-    const double s = blockSum(v, lds);
-    finishReduction(s, partials, ticket, lds, StoreTo{out});
+    MFV_ROWS(n, i) v += a[i];
+    storePartial(blockSum(v, lds), partials);
 }
 
 __global__ void sumSeqK(int n, const double* __restrict__ a, double* __restrict__ out)
@@ -115,15 +113,14 @@ __global__ __launch_bounds__(kReduceBlock) void normFactorK(
     const label* __restrict__ losort, const label* __restrict__ losortOwner, const label* __restrict__ neighbour,
     const double* __restrict__ diag, const double* __restrict__ upper, const double* __restrict__ lower,
     const double* __restrict__ psi, const double* __restrict__ b, const double* __restrict__ sumPsi,
-    double* __restrict__ partials, unsigned int* __restrict__ ticket, double* __restrict__ out)
+    double* __restrict__ partials)
 {
     __shared__ double lds[kReduceBlock / 64];
     const double xRef = *sumPsi / (double)n;
     double v = 0.0;
-    for (int c = blockIdx.x * kReduceBlock + threadIdx.x; c < n; c += gridDim.x * kReduceBlock)
-        v += normSummand(c, ownerStart, losortStart, losort, losortOwner, neighbour, diag, upper, lower, psi, b, xRef);
-    const double s = blockSum(v, lds);
-    finishReduction(s, partials, ticket, lds, NormPost{out});
+    MFV_ROWS(n, c)
+    v += normSummand(c, ownerStart, losortStart, losort, losortOwner, neighbour, diag, upper, lower, psi, b, xRef);
+    storePartial(blockSum(v, lds), partials);
 }
 
 __global__ void normFactorSeqK(int n, const label* __restrict__ ownerStart, const label* __restrict__ losortStart,
@@ -146,7 +143,7 @@ __global__ __launch_bounds__(kTpb) void lapFaceK(int nF, const double* __restric
                                                  const double* __restrict__ gammaf, const double* __restrict__ magSf,
                                                  double* __restrict__ upper, double* __restrict__ lower)
 {
-    const int f = blockIdx.x * kTpb + threadIdx.x;
+    const int f = xcdBlock() * kTpb + threadIdx.x;
     if (f >= nF) return;
     const double coeff = dc[f] * gammaf[f] * magSf[f];
     upper[f] = coeff;
@@ -159,7 +156,7 @@ __global__ __launch_bounds__(kTpb) void lapDiagK(int n, const label* __restrict_
                                                  const label* __restrict__ losort, const double* __restrict__ upper,
                                                  double* __restrict__ diag)
 {
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     double d = 0.0;
     const int l0 = losortStart[c], l1 = losortStart[c + 1];
@@ -177,7 +174,7 @@ __global__ __launch_bounds__(kTpb) void lapBCoeffK(int nB, const int* __restrict
                                                    const double* __restrict__ bDelta, double* __restrict__ iC,
                                                    double* __restrict__ bC)
 {
-    const int i = blockIdx.x * kTpb + threadIdx.x;
+    const int i = xcdBlock() * kTpb + threadIdx.x;
     if (i >= nB) return;
     const int t = bcType[i];
     if (t == 2)
@@ -197,7 +194,7 @@ __global__ __launch_bounds__(kTpb) void lapBCoeffK(int nB, const int* __restrict
 __global__ __launch_bounds__(kTpb) void divFaceK(int nF, const double* __restrict__ phi, double* __restrict__ upper,
                                                  double* __restrict__ lower)
 {
-    const int f = blockIdx.x * kTpb + threadIdx.x;
+    const int f = xcdBlock() * kTpb + threadIdx.x;
     if (f >= nF) return;
     const double ph = phi[f];
     const double w = ph >= 0.0 ? 1.0 : 0.0;
@@ -212,7 +209,7 @@ __global__ __launch_bounds__(kTpb) void divDiagK(int n, const label* __restrict_
                                                  const label* __restrict__ losort, const double* __restrict__ upper,
                                                  const double* __restrict__ lower, double* __restrict__ diag)
 {
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     double d = 0.0;
     const int l0 = losortStart[c], l1 = losortStart[c + 1];
@@ -228,7 +225,7 @@ __global__ __launch_bounds__(kTpb) void divBCoeffK(int nB, const int* __restrict
                                                    const double* __restrict__ phib, double* __restrict__ iC,
                                                    double* __restrict__ bC)
 {
-    const int i = blockIdx.x * kTpb + threadIdx.x;
+    const int i = xcdBlock() * kTpb + threadIdx.x;
     if (i >= nB) return;
     const int t = bcType[i];
     if (t == 2)
@@ -251,7 +248,7 @@ __global__ __launch_bounds__(kTpb) void foldK(int n, const label* __restrict__ b
                                               const double* __restrict__ bC, double* __restrict__ diagC,
                                               double* __restrict__ b)
 {
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     double d = diag[c], s = source[c];
     const int k0 = bndStart[c], k1 = bndStart[c + 1];
@@ -279,7 +276,7 @@ __global__ __launch_bounds__(kTpb) void faceFluxK(int nF, const label* __restric
                                                   const double* __restrict__ upper, const double* __restrict__ lower,
                                                   const double* __restrict__ psi, double* __restrict__ flux)
 {
-    const int f = blockIdx.x * kTpb + threadIdx.x;
+    const int f = xcdBlock() * kTpb + threadIdx.x;
     if (f >= nF) return;
     flux[f] = upper[f] * psi[neighbour[f]] - lower[f] * psi[owner[f]];
 }
@@ -292,7 +289,7 @@ __global__ __launch_bounds__(kTpb) void interpK(int nF, int nB, const label* __r
                                                 const double* __restrict__ vol, double* __restrict__ face,
                                                 double* __restrict__ bface)
 {
-    const int f = blockIdx.x * kTpb + threadIdx.x;
+    const int f = xcdBlock() * kTpb + threadIdx.x;
     if (f < nF) face[f] = w[f] * vol[owner[f]] + (1.0 - w[f]) * vol[neighbour[f]];
     else if (f < nF + nB && bface) bface[f - nF] = vol[bFaceCell[f - nF]];
 }
@@ -307,7 +304,7 @@ __global__ __launch_bounds__(kTpb) void divSourceK(int n, const label* __restric
                                                    const double* __restrict__ phib, const double* __restrict__ V,
                                                    double sign, double* __restrict__ source)
 {
-    const int c = blockIdx.x * kTpb + threadIdx.x;
+    const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     double d = 0.0;
     const int l0 = losortStart[c], l1 = losortStart[c + 1];
@@ -328,7 +325,7 @@ __global__ __launch_bounds__(kTpb) void fluxCorrectK(int nF, const label* __rest
                                                      const double* __restrict__ phiHbyA, double sign,
                                                      double* __restrict__ phi)
 {
-    const int f = blockIdx.x * kTpb + threadIdx.x;
+    const int f = xcdBlock() * kTpb + threadIdx.x;
     if (f >= nF) return;
     const double flux = upper[f] * psi[neighbour[f]] - lower[f] * psi[owner[f]];
     phi[f] = phiHbyA[f] - sign * flux;
@@ -393,12 +390,13 @@ void launchNormFactor(Ctx& c, const Mesh& m, const LduView& A, const double* psi
     else
     {
         const int g = reduceGrid(n);
-        sumK<<<g, kReduceBlock, 0, c.stream>>>(n, psi, c.partials.p + 3 * (size_t)kMaxReduceBlocks, c.tickets.p + 3,
-                                               sumPsi);
+        double* pa = c.partials.p + 3 * (size_t)kMaxReduceBlocks;
+        sumK<<<g, kReduceBlock, 0, c.stream>>>(n, psi, pa);
+        finishK<<<1, kReduceBlock, 0, c.stream>>>(g, pa, StoreTo{sumPsi}, nullptr);
         normFactorK<<<g, kReduceBlock, 0, c.stream>>>(n, m.ownerStart.p, m.losortStart.p, m.losort.p,
                                                       m.losortOwner.p, m.neighbour.p, A.diag, A.upper, A.lower, psi,
-                                                      b, sumPsi, c.partials.p + 3 * (size_t)kMaxReduceBlocks,
-                                                      c.tickets.p + 3, out);
+                                                      b, sumPsi, pa);
+        finishK<<<1, kReduceBlock, 0, c.stream>>>(g, pa, NormPost{out}, nullptr);
     }
     hipCheck(hipGetLastError(), "normFactor");
 }

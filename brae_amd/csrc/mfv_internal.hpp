@@ -100,10 +100,10 @@ struct DBuf
 // Fixed partition of a reduction: a grid that depends on n only, so every call on the same length
 // sums in the same order.
 constexpr int kReduceBlock = 256;
-constexpr int kMaxReduceBlocks = 2048;
+constexpr int kMaxReduceBlocks = 65536;
 
-// Per-context scratch for two-stage reductions: partial sums per block, the last-block ticket and
-// the results (device scalars), plus a pinned host mailbox for read-backs.
+// Per-context scratch for two-stage reductions: partial sums per workgroup and the results (device
+// scalars), plus a pinned host mailbox for read-backs.
 struct Ctx
 {
     int device = 0;
@@ -111,8 +111,7 @@ struct Ctx
     bool ownsStream = false;
     std::string lastError;
     bool sequentialReduce = false;   // OpenFOAM order: one thread, ascending index
-    DBuf<double> partials;           // kMaxReduceBlocks * 4 slots
-    DBuf<unsigned int> tickets;      // one ticket per concurrent reduction slot
+    DBuf<double> partials;           // kMaxReduceBlocks per reduction slot, 8 slots
     DBuf<double> scalars;            // 64 device scalars for solver state
     double* mailbox = nullptr;       // pinned host, 64 doubles
     int cuCount = 256;
@@ -132,7 +131,9 @@ enum ProfTag : int
     kProfAmgResidual0,    // grid-0 residual in the cycle precision
     kProfAmgProlongSmooth0,  // grid-0 prolongation + post-sweep
     kProfAmgRestrict0,    // grid 0 -> 1 restriction
-    kProfCount
+    kProfGridDown = 16,   // + grid: that grid's way down (sweeps, residual, restriction; the coarsest solve)
+    kProfGridUp = 80,     // + grid: that grid's way up (prolongation, scaling, sweeps)
+    kProfCount = 144
 };
 
 // The mesh's internal-face LDU addressing on the device (lduAddressing), plus the boundary
@@ -199,7 +200,8 @@ struct ProfScope
     int tag;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ProfScope(Ctx& ctx, int t);
-    ~ProfScope();
+    ~ProfScope() { close(); }
+    void close();   // record the end event now (idempotent)
 };
 
 } // namespace mfv

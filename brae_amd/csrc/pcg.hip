@@ -39,13 +39,21 @@ __device__ __forceinline__ bool convergedAt(const double* st, double res)
     return res < st[S_TOL] || (st[S_RELTOL] > 0.0 && res < st[S_RELTOL] * st[S_INIT]);
 }
 
-__global__ __launch_bounds__(kReduceBlock) void sumPsiK(int n, const double* __restrict__ a, double* partials,
-                                                        unsigned int* ticket, double* out)
+__global__ __launch_bounds__(kReduceBlock) void sumPsiK(int n, const double* __restrict__ a, double* partials)
 {
     __shared__ double lds[kReduceBlock / 64];
     double v = 0.0;
-    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock) v += a[i];
-    finishReduction(blockSum(v, lds), partials, ticket, lds, StoreTo{out});
+    MFV_ROWS(n, i) v += a[i];
+    storePartial(blockSum(v, lds), partials);
+}
+
+__device__ __forceinline__ void storePartials2(double a, double b, double* pA, double* pB)
+{
+    if (threadIdx.x == 0)
+    {
+        pA[blockIdx.x] = a;
+        pB[blockIdx.x] = b;
+    }
 }
 
 // r = b - A psi; sum|r| and (without a caller normFactor) the normFactor summand
@@ -74,13 +82,12 @@ __global__ __launch_bounds__(kReduceBlock) void pcgInitK(SellView S, const doubl
                                                          const double* __restrict__ diag,
                                                          const double* __restrict__ psi, const double* __restrict__ b,
                                                          double* __restrict__ r, PcgDev d, int haveNf,
-                                                         const double* nfIn, double* pA, double* pB,
-                                                         unsigned int* ticket, double* seqA, double* seqB)
+                                                         double* pA, double* pB, double* seqA, double* seqB)
 {
     __shared__ double lds[kReduceBlock / 64];
     const double xRef = haveNf ? 0.0 : d.st[S_SUMPSI] / (double)S.n;
     double sr = 0.0, snf = 0.0;
-    for (int c = blockIdx.x * kReduceBlock + threadIdx.x; c < S.n; c += gridDim.x * kReduceBlock)
+    MFV_ROWS(S.n, c)
     {
         const double ax = rowOffSum<double, double>(S, c, val, PlainX<double>{psi}, diag[c] * psi[c]);
         const double rc = b[c] - ax;
@@ -106,34 +113,7 @@ __global__ __launch_bounds__(kReduceBlock) void pcgInitK(SellView S, const doubl
     if (seqA) return;
     const double a = blockSum(sr, lds);
     const double bsum = blockSum(snf, lds);
-    finishReduction2(a, bsum, pA, pB, ticket, lds, InitPost{d, haveNf, nfIn});
-}
-
-// Sequential (OpenFOAM-order) completion of a reduction whose per-element terms were stored: one
-// thread sums them in ascending index, exactly the host loop's order, and applies the same post step.
-template <typename Post>
-__global__ void seqFinish2K(int n, const double* __restrict__ a, const double* __restrict__ b, Post post,
-                            const int* __restrict__ stop)
-{
-    if (threadIdx.x || blockIdx.x) return;
-    if (stop && *stop) return;
-    double sa = 0.0, sb = 0.0;
-    for (int i = 0; i < n; ++i)
-    {
-        sa += a[i];
-        sb += b[i];
-    }
-    post(sa, sb);
-}
-
-template <typename Post>
-__global__ void seqFinishK(int n, const double* __restrict__ a, Post post, const int* __restrict__ stop)
-{
-    if (threadIdx.x || blockIdx.x) return;
-    if (stop && *stop) return;
-    double sa = 0.0;
-    for (int i = 0; i < n; ++i) sa += a[i];
-    post(sa);
+    storePartials2(a, bsum, pA, pB);
 }
 
 __global__ void seqSumK(int n, const double* __restrict__ a, double* __restrict__ out)
@@ -168,13 +148,12 @@ struct ZrPost
 template <typename Tv>
 __global__ __launch_bounds__(kReduceBlock) void pcgZrK(int n, const Tv* __restrict__ z, const double* __restrict__ r,
                                                        const double* __restrict__ rOld, int flexible, PcgDev d,
-                                                       double* pA, double* pB, unsigned int* ticket, double* seqA,
-                                                       double* seqB)
+                                                       double* pA, double* pB, double* seqA, double* seqB)
 {
     if (d.fl[F_STOP]) return;
     __shared__ double lds[kReduceBlock / 64];
     double a = 0.0, o = 0.0;
-    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock)
+    MFV_ROWS(n, i)
     {
         const double zi = (double)z[i];
         const double ea = zi * r[i];
@@ -193,7 +172,7 @@ __global__ __launch_bounds__(kReduceBlock) void pcgZrK(int n, const Tv* __restri
     if (seqA) return;
     const double sa = blockSum(a, lds);
     const double so = blockSum(o, lds);
-    finishReduction2(sa, so, pA, pB, ticket, lds, ZrPost{d, flexible});
+    storePartials2(sa, so, pA, pB);
 }
 
 template <typename Tv>
@@ -222,13 +201,13 @@ __global__ __launch_bounds__(kReduceBlock) void pcgDirAmulK(SellView S, const do
                                                             const double* __restrict__ diag, const Tv* __restrict__ z,
                                                             const double* __restrict__ pOld,
                                                             double* __restrict__ pNew, double* __restrict__ q, PcgDev d,
-                                                            double* partials, unsigned int* ticket, double* seqA)
+                                                            double* partials, double* seqA)
 {
     if (d.fl[F_STOP]) return;
     __shared__ double lds[kReduceBlock / 64];
     const DirX<Tv> P{z, pOld, d.st[S_BETA], d.fl[F_ITER] == 0 ? 1 : 0};
     double s = 0.0;
-    for (int c = blockIdx.x * kReduceBlock + threadIdx.x; c < S.n; c += gridDim.x * kReduceBlock)
+    MFV_ROWS(S.n, c)
     {
         const double pc = P(c);
         const double ax = rowOffSum<double, double>(S, c, val, P, diag[c] * pc);
@@ -240,7 +219,7 @@ __global__ __launch_bounds__(kReduceBlock) void pcgDirAmulK(SellView S, const do
             s += ax * pc;
     }
     if (seqA) return;
-    finishReduction(blockSum(s, lds), partials, ticket, lds, PqPost{d});
+    storePartial(blockSum(s, lds), partials);
 }
 
 // psi += alpha p, r -= alpha q, sum|r|; the stop decision; the next cycle's grid-0 entry
@@ -265,15 +244,14 @@ __global__ __launch_bounds__(kReduceBlock) void pcgUpdateK(int n, const double* 
                                                            double* __restrict__ r, double* __restrict__ rOld,
                                                            int flexible, const Tv* __restrict__ diag0,
                                                            Tv* __restrict__ b0, Tv* __restrict__ x0, int fromZero,
-                                                           PcgDev d, double* partials, unsigned int* ticket,
-                                                           double* seqA)
+                                                           PcgDev d, double* partials, double* seqA)
 {
     if (d.fl[F_STOP]) return;
     __shared__ double lds[kReduceBlock / 64];
     const double pq = d.st[S_PQ];
     const double alpha = fabs(pq) > 1e-300 ? d.st[S_RHO] / pq : 0.0;
     double s = 0.0;
-    for (int i = blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += gridDim.x * kReduceBlock)
+    MFV_ROWS(n, i)
     {
         psi[i] += alpha * p[i];
         const double ro = r[i];
@@ -289,7 +267,7 @@ __global__ __launch_bounds__(kReduceBlock) void pcgUpdateK(int n, const double* 
         x0[i] = fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(diag0[i]) : Tv(0);
     }
     if (seqA) return;
-    finishReduction(blockSum(s, lds), partials, ticket, lds, UpdPost{d});
+    storePartial(blockSum(s, lds), partials);
 }
 
 __global__ void pcgSetupK(PcgDev d, double tol, double relTol, int maxIter, int minIter)
@@ -315,7 +293,6 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     const int gR = reduceGrid(n);
     double* pa = c.partials.p + 5 * (size_t)kMaxReduceBlocks;
     double* pb = c.partials.p + 6 * (size_t)kMaxReduceBlocks;
-    unsigned int* tk = c.tickets.p + 5;
     const bool seq = c.sequentialReduce;
     if (seq && w.seqA.n != (size_t)n)
     {
@@ -331,13 +308,23 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         if (seq)
             seqSumK<<<1, 64, 0, c.stream>>>(n, psi, w.st.p + S_SUMPSI);
         else
-            sumPsiK<<<gR, kReduceBlock, 0, c.stream>>>(n, psi, pa, tk, w.st.p + S_SUMPSI);
+        {
+            sumPsiK<<<gR, kReduceBlock, 0, c.stream>>>(n, psi, pa);
+            finishK<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, StoreTo{w.st.p + S_SUMPSI}, nullptr);
+        }
     }
-    pcgInitK<<<gR, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, psi, b, w.r.p, d, normFactor ? 1 : 0,
-                                               normFactor, pa, pb, tk, sA, sB);
-    if (seq) seqFinish2K<<<1, 64, 0, c.stream>>>(n, sA, sB, InitPost{d, normFactor ? 1 : 0, normFactor}, nullptr);
+    pcgInitK<<<gR, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, psi, b, w.r.p, d, normFactor ? 1 : 0, pa, pb,
+                                               sA, sB);
+    const InitPost initPost{d, normFactor ? 1 : 0, normFactor};
+    if (seq)
+        seqFinish2K<<<1, 64, 0, c.stream>>>(n, sA, sB, initPost, nullptr);
+    else
+        finish2K<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, pb, initPost, nullptr);
     hipCheck(hipGetLastError(), "pcg init");
-    const VcycleControls vc{pc.nPre, pc.nPost};
+    VcycleControls vc;
+    vc.nPre = pc.nPre;
+    vc.nPost = pc.nPost;
+    vc.scale = pc.scaleCorrection;
     amgStartFrom(c, a, w.r.p, vc, w.fl.p + F_STOP);
     double* p[2] = {w.p0.p, w.p1.p};
     int it = 0;
@@ -350,22 +337,32 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         }
         {
             ProfScope ps(c, kProfPcgZr);
-            pcgZrK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(n, z, w.r.p, w.rOld.p, pc.flexible ? 1 : 0, d, pa, pb, tk,
-                                                         sA, sB);
-            if (seq) seqFinish2K<<<1, 64, 0, c.stream>>>(n, sA, sB, ZrPost{d, pc.flexible ? 1 : 0}, stopF);
+            pcgZrK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(n, z, w.r.p, w.rOld.p, pc.flexible ? 1 : 0, d, pa, pb, sA,
+                                                         sB);
+            const ZrPost post{d, pc.flexible ? 1 : 0};
+            if (seq)
+                seqFinish2K<<<1, 64, 0, c.stream>>>(n, sA, sB, post, stopF);
+            else
+                finish2K<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, pb, post, stopF);
         }
         {
             ProfScope ps(c, kProfPcgSpmv64);
             pcgDirAmulK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, z, p[it & 1], p[(it + 1) & 1],
-                                                              w.q.p, d, pa, tk, sA);
-            if (seq) seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, PqPost{d}, stopF);
+                                                              w.q.p, d, pa, sA);
+            if (seq)
+                seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, PqPost{d}, stopF);
+            else
+                finishK<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, PqPost{d}, stopF);
         }
         {
             ProfScope ps(c, kProfPcgUpdate);
             pcgUpdateK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(n, p[(it + 1) & 1], w.q.p, psi, w.r.p, w.rOld.p,
                                                              pc.flexible ? 1 : 0, diag0, b0, x0, pc.nPre > 0 ? 1 : 0,
-                                                             d, pa, tk, sA);
-            if (seq) seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, UpdPost{d}, stopF);
+                                                             d, pa, sA);
+            if (seq)
+                seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, UpdPost{d}, stopF);
+            else
+                finishK<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, UpdPost{d}, stopF);
         }
         hipCheck(hipGetLastError(), "pcg iteration");
         if ((it + 1) % std::max(1, pc.checkEvery) == 0 || it + 1 == pc.maxIter)

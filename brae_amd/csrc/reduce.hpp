@@ -1,12 +1,16 @@
-// Deterministic wave64 reductions for gfx950.
+// Deterministic wave64 reductions and the XCD-aware row partition for gfx950.
 //
-// Every reduction in the library uses ONE partition of [0, n) that depends on n only: reduceGrid(n)
-// workgroups of kReduceBlock threads, each thread summing its grid-stride elements in ascending
-// order, each workgroup combining its four wavefronts by a fixed xor-butterfly and a fixed
-// wave order, and the last workgroup to finish (a ticket counter) summing the per-workgroup
-// partials in ascending workgroup order. The bits of a result therefore depend on n and the
-// data, never on scheduling -- the same guarantee the reference's two-stage reductions give
-// (reductions.cu:1-21), with the second stage folded into the first launch.
+// Partition. A kernel over n rows runs reduceGrid(n) workgroups of kReduceBlock threads; workgroup w
+// owns the contiguous chunk of rows [w*chunk, (w+1)*chunk) (chunk a multiple of kReduceBlock, a
+// function of n only). The hardware hands workgroups to the eight XCDs round-robin, so xcdBlock()
+// renumbers them such that XCD k works through one contiguous eighth of the rows in order: a row's
+// neighbours (c +- 1, c +- nx, ...) were just touched by the same XCD and sit in its L2.
+//
+// Reductions. Each workgroup sums its rows in ascending order per thread, combines its four
+// wavefronts by a fixed xor butterfly and a fixed wave order, and stores one partial; a second
+// one-workgroup kernel sums the partials in a fixed order and applies the caller's post step. The
+// result depends on n and the data only -- the reference's two-stage reductions (reductions.cu:1-21)
+// -- and no kernel needs a device-scope fence (which on MI355X writes back the L2).
 #pragma once
 
 #include "mfv_internal.hpp"
@@ -15,10 +19,25 @@ namespace mfv {
 
 inline int reduceGrid(int n)
 {
-    int b = (n + kReduceBlock * 4 - 1) / (kReduceBlock * 4);
+    int b = (n + kReduceBlock * 2 - 1) / (kReduceBlock * 2);
     if (b < 1) b = 1;
     if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
     return b;
+}
+
+// rows per workgroup for a reduceGrid partition (a multiple of kReduceBlock)
+__host__ __device__ __forceinline__ int chunkRows(int n, int grid)
+{
+    const int per = (n + grid - 1) / grid;
+    return (per + kReduceBlock - 1) / kReduceBlock * kReduceBlock;
+}
+
+// workgroup index renumbered so each XCD processes a contiguous run of workgroups
+__device__ __forceinline__ int xcdBlock()
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    const int q = G >> 3, r = G & 7, x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
 }
 
 __device__ __forceinline__ double waveSum(double v)
@@ -45,66 +64,68 @@ __device__ __forceinline__ double blockSum(double v, double* lds)
     return s;
 }
 
-// Stage two, run by every workgroup after its partial is stored: the last one to arrive sums the
-// partials in ascending order and writes *out (through `post` so callers can scale or combine).
-// Returns true in the workgroup that finished the reduction.
-template <typename Post>
-__device__ __forceinline__ bool finishReduction(double partial, double* partials, unsigned int* ticket, double* lds,
-                                                Post post)
+// Store this workgroup's partial (thread 0 holds the block sum).
+__device__ __forceinline__ void storePartial(double blockTotal, double* partials)
 {
-    __shared__ bool last;
-    if (threadIdx.x == 0)
-    {
-        partials[blockIdx.x] = partial;
-        __threadfence();
-        const unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double v = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kReduceBlock) v += partials[i];
-    const double s = blockSum(v, lds);
-    if (threadIdx.x == 0)
-    {
-        post(s);
-        *ticket = 0u;
-    }
-    return true;
+    if (threadIdx.x == 0) partials[blockIdx.x] = blockTotal;
 }
 
-// Two sums reduced together (one ticket): the last workgroup hands both totals to `post`.
+// Stage two: one workgroup sums nParts partials in a fixed order and hands the total to `post`.
 template <typename Post>
-__device__ __forceinline__ bool finishReduction2(double pa, double pb, double* partialsA, double* partialsB,
-                                                 unsigned int* ticket, double* lds, Post post)
+__global__ __launch_bounds__(kReduceBlock) void finishK(int nParts, const double* __restrict__ pa, Post post,
+                                                        const int* __restrict__ stop)
 {
-    __shared__ bool last;
-    if (threadIdx.x == 0)
-    {
-        partialsA[blockIdx.x] = pa;
-        partialsB[blockIdx.x] = pb;
-        __threadfence();
-        const unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+    if (stop && *stop) return;
+    __shared__ double lds[kReduceBlock / 64];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nParts; i += kReduceBlock) v += pa[i];
+    const double s = blockSum(v, lds);
+    if (threadIdx.x == 0) post(s);
+}
+
+template <typename Post>
+__global__ __launch_bounds__(kReduceBlock) void finish2K(int nParts, const double* __restrict__ pa,
+                                                         const double* __restrict__ pb, Post post,
+                                                         const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    __shared__ double lds[kReduceBlock / 64];
     double va = 0.0, vb = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kReduceBlock)
+    for (int i = threadIdx.x; i < nParts; i += kReduceBlock)
     {
-        va += partialsA[i];
-        vb += partialsB[i];
+        va += pa[i];
+        vb += pb[i];
     }
     const double sa = blockSum(va, lds);
     const double sb = blockSum(vb, lds);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) post(sa, sb);
+}
+
+// Sequential (OpenFOAM-order) completion of a reduction whose per-element terms were stored: one
+// thread sums them in ascending index, the host loop's order, and applies the same post step.
+template <typename Post>
+__global__ void seqFinishK(int n, const double* __restrict__ a, Post post, const int* __restrict__ stop)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    if (stop && *stop) return;
+    double sa = 0.0;
+    for (int i = 0; i < n; ++i) sa += a[i];
+    post(sa);
+}
+
+template <typename Post>
+__global__ void seqFinish2K(int n, const double* __restrict__ a, const double* __restrict__ b, Post post,
+                            const int* __restrict__ stop)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    if (stop && *stop) return;
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < n; ++i)
     {
-        post(sa, sb);
-        *ticket = 0u;
+        sa += a[i];
+        sb += b[i];
     }
-    return true;
+    post(sa, sb);
 }
 
 struct StoreTo
@@ -112,5 +133,12 @@ struct StoreTo
     double* out;
     __device__ void operator()(double s) const { *out = s; }
 };
+
+// Row loop over this workgroup's chunk of a reduceGrid partition, XCD-ordered.
+#define MFV_ROWS(n, c)                                                                                            \
+    const int mfvChunk_ = chunkRows((n), gridDim.x);                                                             \
+    const int mfvBegin_ = xcdBlock() * mfvChunk_;                                                                 \
+    const int mfvEnd_ = (mfvBegin_ + mfvChunk_ < (n)) ? mfvBegin_ + mfvChunk_ : (n);                              \
+    for (int c = mfvBegin_ + (int)threadIdx.x; c < mfvEnd_; c += kReduceBlock)
 
 } // namespace mfv

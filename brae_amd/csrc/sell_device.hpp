@@ -38,13 +38,52 @@ inline SellView viewOf(const Sell& s)
 }
 
 // sum_k A'_ck * X(col_k) over row c's entries in face order, in the vector precision. X is a functor
-// returning the operand at a column (plain x, or x + prolonged coarse correction).
+// returning the operand at a column (plain x, or x + prolonged coarse correction). Rows of up to
+// kRowRegs entries load every coefficient and column first and then gather, so a row's loads are all in
+// flight together; the sum is still taken in entry order.
+constexpr int kRowRegs = 8;
+
 template <typename Tm, typename Tv, typename XF>
 __device__ __forceinline__ Tv rowOffSum(const SellView& S, int c, const Tm* __restrict__ val, XF X, Tv acc)
 {
     const int s = c >> 6, lane = c & 63;
     const int p0 = S.slicePtr[s];
     const int w = (S.slicePtr[s + 1] - p0) >> 6;
+    if (w <= kRowRegs)
+    {
+        Tv a[kRowRegs];
+        int col[kRowRegs];
+        if (S.pattern)
+        {
+            const int* off = S.patOff + (int)S.pattern[c] * S.width;
+#pragma unroll
+            for (int k = 0; k < kRowRegs; ++k)
+                if (k < w)
+                {
+                    a[k] = cvt<Tv>(val[p0 + (k << 6) + lane]);
+                    col[k] = c + off[k];
+                }
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < kRowRegs; ++k)
+                if (k < w)
+                {
+                    const int e = p0 + (k << 6) + lane;
+                    a[k] = cvt<Tv>(val[e]);
+                    col[k] = S.cols[e];
+                }
+        }
+        Tv xv[kRowRegs];
+#pragma unroll
+        for (int k = 0; k < kRowRegs; ++k)
+            if (k < w) xv[k] = X(col[k]);
+#pragma unroll
+        for (int k = 0; k < kRowRegs; ++k)
+            if (k < w) acc += a[k] * xv[k];
+        return acc;
+    }
     if (S.pattern)
     {
         const int* off = S.patOff + (int)S.pattern[c] * S.width;

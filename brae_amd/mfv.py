@@ -86,7 +86,7 @@ _SIGS = {
     "mfv_amg_level_addressing": [P, I, P, P, P, P, P],
     "mfv_amg_galerkin": [P, P, P, P, P, I],
     "mfv_amg_grid_values": [P, P, I, P, P],
-    "mfv_amg_vcycle": [P, P, P, P, I, I],
+    "mfv_amg_vcycle": [P, P, P, P, I, I, I],
     "mfv_pcg_default_controls": [P],
     "mfv_amg_pcg_solve": [P, P, P, P, P, P, P, P, P],
 }
@@ -279,8 +279,8 @@ class Amg:
         self.ctx.call("mfv_amg_grid_values", self.h, g, ptr(d), ptr(u))
         return d, u
 
-    def vcycle(self, r, z, nPre=1, nPost=1):
-        self.ctx.call("mfv_amg_vcycle", self.h, ptr(r), ptr(z), nPre, nPost)
+    def vcycle(self, r, z, nPre=1, nPost=1, scale=False):
+        self.ctx.call("mfv_amg_vcycle", self.h, ptr(r), ptr(z), nPre, nPost, 1 if scale else 0)
 
     def solve(self, diag, upper, b, psi, normFactor=None, **kw):
         c = PcgControls()

@@ -186,8 +186,10 @@ int mfv_amg_galerkin(mfv_ctx ctx, mfv_amg amg, const double* dFineDiag, const do
                      const double* dFineLower, int precision);
 /* host copies of a grid's FP64 Galerkin values (grid >= 1): diag [nCells], upper [nFaces]; synchronises */
 int mfv_amg_grid_values(mfv_ctx ctx, mfv_amg amg, int grid, double* hDiag, double* hUpper);
-/* amgVCycleApply (device_amg_vcycle.cu:1793-1875): dZ = one V(nPre,nPost) cycle applied to dR */
-int mfv_amg_vcycle(mfv_ctx ctx, mfv_amg amg, const double* dR, double* dZ, int nPreSweeps, int nPostSweeps);
+/* amgVCycleApply (device_amg_vcycle.cu:1793-1875): dZ = one V(nPre,nPost) cycle applied to dR;
+ * scaleCorrection as in mfv_pcg_controls */
+int mfv_amg_vcycle(mfv_ctx ctx, mfv_amg amg, const double* dR, double* dZ, int nPreSweeps, int nPostSweeps,
+                   int scaleCorrection);
 
 /* ---------------------------------------------------------------------------------------------
  * AMG-preconditioned CG (deviceAMGPCG, device_amg.cuh:550-553; device_amg_pcg.cu:638-848).
@@ -203,7 +205,9 @@ typedef struct
     int smoother;          /* 0: weighted Jacobi (omega 0.8), the only value carried */
     int checkEvery;        /* host polls the device stop flag every k iterations (result independent of k) */
     int nVcycles;          /* 1 (only value carried) */
-    int scaleCorrection;   /* 0 (only value carried) */
+    int scaleCorrection;   /* 1: each grid's coarse correction scaled by (r.e)/(e.Ae) (the reference's
+                              corrScaling, device_amg_vcycle.cu:1113-1126), fused with the first
+                              post-sweep; use with flexible = 1 */
 } mfv_pcg_controls;
 
 /* SolverPerformance (src/OpenFOAM/matrices/pcg.cuh:16-21) */

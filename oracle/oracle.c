@@ -1542,7 +1542,7 @@ typedef struct
     int nG;            /* grids = levels + 1 */
     br_mat* g;         /* g[0] fine */
     double* dense;     /* coarsest dense inverse */
-    int nPre, nPost, single;
+    int nPre, nPost, single, scale;
     double **b, **x, **r, **Ax;
     float **bf, **xf, **rf, **Axf;
 } br_hier;
@@ -1578,6 +1578,29 @@ static void br_vcycle_d(br_hier* H, int g)
     for (int i = 0; i < n; ++i) bc[L->map[i]] += r[i];
     br_vcycle_d(H, g + 1);
     double* xc = H->x[g + 1];
+    if (H->scale)
+    {
+        /* scaled correction fused with the first post-sweep: e = P xc, alpha = (r.e)/(e.Ae)
+         * (corrScaling, device_amg_vcycle.cu:1113-1126; guard scaleFactorK :69-79), then
+         * x = x + alpha*e + omega*(r - alpha*Ae)/D with r the residual restricted above */
+        double* e = (double*)xcalloc((size_t)n, sizeof(double));
+        for (int i = 0; i < n; ++i) e[i] = xc[L->map[i]];
+        br_amul_d(A, e, Ax);
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < n; ++i) num += e[i] * r[i];
+        for (int i = 0; i < n; ++i) den += e[i] * Ax[i];
+        const double al = den > 1e-30 ? num / den : 1.0;
+        for (int i = 0; i < n; ++i)
+            x[i] = H->nPost > 0 ? x[i] + al * e[i] + 0.8 * (r[i] - al * Ax[i]) / safe_diag_d(A->diag[i])
+                                : x[i] + al * e[i];
+        free(e);
+        for (int s = 1; s < H->nPost; ++s)
+        {
+            br_amul_d(A, x, Ax);
+            for (int i = 0; i < n; ++i) x[i] += 0.8 * (b[i] - Ax[i]) / safe_diag_d(A->diag[i]);
+        }
+        return;
+    }
     for (int i = 0; i < n; ++i) x[i] += xc[L->map[i]];
     for (int s = 0; s < H->nPost; ++s)
     {
@@ -1621,6 +1644,26 @@ static void br_vcycle_f(br_hier* H, int g)
     for (int i = 0; i < n; ++i) bc[L->map[i]] += r[i];
     br_vcycle_f(H, g + 1);
     float* xc = H->xf[g + 1];
+    if (H->scale)
+    {
+        float* e = (float*)xcalloc((size_t)n, sizeof(float));
+        for (int i = 0; i < n; ++i) e[i] = xc[L->map[i]];
+        br_amul_f(A, e, Ax);
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < n; ++i) num += (double)e[i] * (double)r[i];
+        for (int i = 0; i < n; ++i) den += (double)e[i] * (double)Ax[i];
+        const float al = (float)(den > 1e-30 ? num / den : 1.0);
+        for (int i = 0; i < n; ++i)
+            x[i] = H->nPost > 0 ? x[i] + al * e[i] + om * (r[i] - al * Ax[i]) / safe_diag_f(A->fdiag[i])
+                                : x[i] + al * e[i];
+        free(e);
+        for (int s = 1; s < H->nPost; ++s)
+        {
+            br_amul_f(A, x, Ax);
+            for (int i = 0; i < n; ++i) x[i] += om * (b[i] - Ax[i]) / safe_diag_f(A->fdiag[i]);
+        }
+        return;
+    }
     for (int i = 0; i < n; ++i) x[i] += xc[L->map[i]];
     for (int s = 0; s < H->nPost; ++s)
     {
@@ -1631,7 +1674,7 @@ static void br_vcycle_f(br_hier* H, int g)
 
 /* hierarchy values (brae Galerkin), float copies and work vectors for the cycles below */
 static void br_hier_build(br_hier* Hp, const orc_aggl* a, const double* diag, const double* upper, int nPre,
-                          int nPost, int single, double** cDp, double** cUp)
+                          int nPost, int single, int scale, double** cDp, double** cUp)
 {
     const int *own, *nei;
     aggl_fine_addressing(a, &own, &nei);
@@ -1642,6 +1685,7 @@ static void br_hier_build(br_hier* Hp, const orc_aggl* a, const double* diag, co
     H.nPre = nPre;
     H.nPost = nPost;
     H.single = single;
+    H.scale = scale;
     H.g = (br_mat*)xcalloc((size_t)H.nG, sizeof(br_mat));
     H.b = (double**)xcalloc((size_t)H.nG, sizeof(double*));
     H.x = (double**)xcalloc((size_t)H.nG, sizeof(double*));
@@ -1720,11 +1764,11 @@ static void br_hier_free(br_hier* Hp, double* cD, double* cU)
 }
 
 void orc_brae_vcycle(const orc_aggl* a, const double* diag, const double* upper, const double* r, double* z,
-                     int nPre, int nPost, int single)
+                     int nPre, int nPost, int single, int scale)
 {
     br_hier H;
     double *cD, *cU;
-    br_hier_build(&H, a, diag, upper, nPre, nPost, single, &cD, &cU);
+    br_hier_build(&H, a, diag, upper, nPre, nPost, single, scale, &cD, &cU);
     const int nC = a->fineCells;
     if (single)
     {
@@ -1743,13 +1787,14 @@ void orc_brae_vcycle(const orc_aggl* a, const double* diag, const double* upper,
 
 orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* upper, const double* b,
                          double* psi, double tol, double relTol, int maxIter, int minIter,
-                         int nPre, int nPost, int flexible, int single, double normFactorIn, double* psiTrace)
+                         int nPre, int nPost, int flexible, int single, int scale, double normFactorIn,
+                         double* psiTrace)
 {
     const int *own, *nei;
     aggl_fine_addressing(a, &own, &nei);
     br_hier H;
     double *cD, *cU;
-    br_hier_build(&H, a, diag, upper, nPre, nPost, single, &cD, &cU);
+    br_hier_build(&H, a, diag, upper, nPre, nPost, single, scale, &cD, &cU);
     const int nC = a->fineCells;
     br_mat* A0 = &H.g[0];
     double* pA = (double*)xcalloc((size_t)nC, sizeof(double));

@@ -136,16 +136,18 @@ orc_perf orc_pcg_gamg_solve(const orc_aggl* a, const double* diag, const double*
  *      inverse (Gauss-Jordan, partial pivoting; the reference's default coarse kind):
  *      device_amg_vcycle.cu:982-1143 / 1658-1766, amg_kernels.cuh:27-53,257-282;
  *      PCG recurrence device_amg_pcg.cu:60-170 (Fletcher-Reeves) or flexBetaK :37-49 (flexible).
+ *      scale: each grid's coarse correction scaled by alpha = (r.e)/(e.Ae) (corrScaling,
+ *      device_amg_vcycle.cu:1113-1126), fused with the first post-sweep.
  *      normFactorIn > 0 replaces OpenFOAM's normFactor (the demo trace passes sum|b| + 1e-20).
  *      psiTrace (nullable, (maxIter+1)*nCells) receives psi after each iteration. ---- */
 orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* upper, const double* b,
                          double* psi, double tol, double relTol, int maxIter, int minIter,
-                         int nPre, int nPost, int flexible, int single, double normFactorIn,
+                         int nPre, int nPost, int flexible, int single, int scale, double normFactorIn,
                          double* psiTrace);
 
 /* one brae V-cycle z = M^-1 r on the hierarchy (same cycle as orc_brae_amgpcg applies) */
 void orc_brae_vcycle(const orc_aggl* a, const double* diag, const double* upper, const double* r, double* z,
-                     int nPre, int nPost, int single);
+                     int nPre, int nPost, int single, int scale);
 
 #ifdef __cplusplus
 }

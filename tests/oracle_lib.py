@@ -68,8 +68,8 @@ def lib():
             "orc_gamg_default_controls": (None, [P]),
             "orc_gamg_solve": (Perf, [P, P, P, P, P, P]),
             "orc_pcg_gamg_solve": (Perf, [P, P, P, P, P, D, D, I, I, P]),
-            "orc_brae_amgpcg": (Perf, [P, P, P, P, P, D, D, I, I, I, I, I, I, D, P]),
-            "orc_brae_vcycle": (None, [P, P, P, P, P, I, I, I]),
+            "orc_brae_amgpcg": (Perf, [P, P, P, P, P, D, D, I, I, I, I, I, I, I, D, P]),
+            "orc_brae_vcycle": (None, [P, P, P, P, P, I, I, I, I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -285,15 +285,15 @@ def pcg_gamg_solve(aggl, diag, upper, b, psi, tol, relTol=0.0, maxIter=1000, min
 
 
 def brae_amgpcg(aggl, diag, upper, b, psi, tol, relTol=0.0, maxIter=1000, minIter=0, nPre=1, nPost=1,
-                flexible=False, single=False, normFactor=0.0, trace=False):
+                flexible=False, single=False, normFactor=0.0, trace=False, scale=False):
     psi = f64(psi).copy()
     tr = np.zeros((maxIter + 1, psi.size)) if trace else None
     perf = lib().orc_brae_amgpcg(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(b)), p(psi), tol, relTol, maxIter,
-                                 minIter, nPre, nPost, int(flexible), int(single), normFactor, p(tr))
+                                 minIter, nPre, nPost, int(flexible), int(single), int(scale), normFactor, p(tr))
     return (psi, perf, tr) if trace else (psi, perf)
 
 
-def brae_vcycle(aggl, diag, upper, r, nPre=1, nPost=1, single=False):
+def brae_vcycle(aggl, diag, upper, r, nPre=1, nPost=1, single=False, scale=False):
     z = np.zeros(len(r))
-    lib().orc_brae_vcycle(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(r)), p(z), nPre, nPost, int(single))
+    lib().orc_brae_vcycle(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(r)), p(z), nPre, nPost, int(single), int(scale))
     return z

@@ -130,6 +130,24 @@ def test_galerkin_bit_exact(systems):
 
 @pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32])
 @pytest.mark.parametrize("sweeps", [(1, 1), (2, 2), (0, 1)])
+def test_vcycle_scaled_bit_exact_sequential(systems, prec, sweeps):
+    """The scaled-correction cycle: its alphas are sums, so the sequential mode is the bit-exact one."""
+    for S in systems:
+        ag, amg = build_pair(S, "brae")
+        amg.galerkin(S.dd, S.du, prec)
+        r = np.random.default_rng(8).normal(size=S.nC)
+        dr, dz = G.dev(r), G.zeros(S.nC)
+        S.case.ctx.sequential_reduce(True)
+        amg.vcycle(dr, dz, *sweeps, scale=True)
+        S.case.ctx.sequential_reduce(False)
+        ref = O.brae_vcycle(ag, S.diag, S.upper, r, *sweeps, single=(prec == mfv.PREC_FP32), scale=True)
+        z = G.host(dz)
+        assert np.array_equal(z, ref), (S.label, np.abs(z - ref).max())
+        amg.close()
+
+
+@pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32])
+@pytest.mark.parametrize("sweeps", [(1, 1), (2, 2), (0, 1)])
 def test_vcycle_bit_exact(systems, prec, sweeps):
     for S in systems:
         print("case:", S.label, prec, sweeps, flush=True)
@@ -145,19 +163,19 @@ def test_vcycle_bit_exact(systems, prec, sweeps):
 
 
 @pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32])
-@pytest.mark.parametrize("flexible", [False, True])
-def test_pcg_bit_exact_with_sequential_reductions(systems, prec, flexible):
+@pytest.mark.parametrize("flexible,scale", [(False, False), (True, False), (True, True)])
+def test_pcg_bit_exact_with_sequential_reductions(systems, prec, flexible, scale):
     for S in systems:
-        print("case:", S.label, prec, flexible, flush=True)
+        print("case:", S.label, prec, flexible, scale, flush=True)
         ag, amg = build_pair(S, "brae")
         amg.galerkin(S.dd, S.du, prec)
         S.case.ctx.sequential_reduce(True)
         psi = G.zeros(S.nC)
         perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-9, precision=prec, flexible=int(flexible),
-                         checkEvery=3)
+                         checkEvery=3, scaleCorrection=int(scale))
         S.case.ctx.sequential_reduce(False)
         ref_psi, ref = O.brae_amgpcg(ag, S.diag, S.upper, S.b, np.zeros(S.nC), 1e-9, flexible=flexible,
-                                     single=(prec == mfv.PREC_FP32))
+                                     single=(prec == mfv.PREC_FP32), scale=scale)
         assert perf.nIterations == ref.nIterations, (S.label, perf.nIterations, ref.nIterations)
         assert perf.initialResidual == ref.initialResidual
         assert perf.finalResidual == ref.finalResidual
@@ -166,14 +184,17 @@ def test_pcg_bit_exact_with_sequential_reductions(systems, prec, flexible):
 
 
 @pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32, mfv.PREC_FP16])
-def test_pcg_tree_reductions_converge_like_oracle(systems, prec):
+@pytest.mark.parametrize("scale", [False, True])
+def test_pcg_tree_reductions_converge_like_oracle(systems, prec, scale):
     for S in systems:
         ag, amg = build_pair(S, "brae")
         amg.galerkin(S.dd, S.du, prec)
         psi = G.zeros(S.nC)
-        perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-10, precision=prec, flexible=int(prec != 64))
-        ref_psi, ref = O.brae_amgpcg(ag, S.diag, S.upper, S.b, np.zeros(S.nC), 1e-10, flexible=prec != 64,
-                                     single=(prec != mfv.PREC_FP64))
+        flex = prec != 64 or scale
+        perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-10, precision=prec, flexible=int(flex),
+                         scaleCorrection=int(scale))
+        ref_psi, ref = O.brae_amgpcg(ag, S.diag, S.upper, S.b, np.zeros(S.nC), 1e-10, flexible=flex,
+                                     single=(prec != mfv.PREC_FP64), scale=scale)
         assert perf.converged and perf.finalResidual < 1e-10
         slack = 1 if prec != mfv.PREC_FP16 else max(3, ref.nIterations // 5)
         assert abs(perf.nIterations - ref.nIterations) <= slack, (S.label, perf.nIterations, ref.nIterations)
