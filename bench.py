@@ -205,6 +205,7 @@ def gpu_run(args):
                                 "alg_bytes": int(names[t][1]),
                                 "alg_GBps": round(names[t][1] / (prof[t][1] / max(1, launches) * 1e-3) / 1e9, 1)}
                   for t in names if prof[t][1] > 0}
+    copy_gbs = copy_peak(torch, N)
     vc_n, vc_ms = prof[mfv.PROF_VCYCLE]
     grids = []
     for gi in range(amg.nGrids):
@@ -220,14 +221,51 @@ def gpu_run(args):
         final_mixed=res["mixed"]["perfs"][-1].finalResidual, final_fp64=res["fp64"]["perfs"][-1].finalResidual,
         p_rel_l2_mixed_vs_fp64=rel_l2, n_grids=amg.nGrids, coarsest=sizes[-1][0],
         roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                      frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None, kernel=label,
-                      alg_bytes_per_launch=int(alg_bytes), avg_launch_us=round(avg_ms * 1e3, 2)),
+                      frac=round(achieved / PEAK_HBM_GBS, 4), traffic=measured_traffic(label), kernel=label,
+                      alg_bytes_per_launch=int(alg_bytes), avg_launch_us=round(avg_ms * 1e3, 2),
+                      copy_peak_GBps=round(copy_gbs, 1)),
         per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1), grids=grids,
     )
     amg.close()
     ch.mesh.close()
     ctx.close()
     return out
+
+
+def copy_peak(torch, n):
+    """Measured device-to-device copy rate (read + write bytes / time) of n doubles: the plain copy kernel
+    the HBM byte counters are calibrated against, and the box's practical bandwidth ceiling."""
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 16.0 * n / (ms * 1e-3) / 1e9
+
+
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_pressure_corrector_20M.md")
+
+
+def measured_traffic(label):
+    """HBM bytes per launch of `label` from the committed FETCH_SIZE/WRITE_SIZE counter table (rows
+    '| label | ... | traffic_bytes |'), or None when the table does not list the kernel."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 2 and cells[0] == label:
+                    return int(float(cells[-1]))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(args, n_full):
