@@ -1,10 +1,12 @@
 """Benchmark: the SIMPLE pressure corrector on one MI355X.
 
 Workload (one step = one SIMPLE iteration's pressure corrector on the 500x250x160 graded, jittered
-channel, 20,000,000 cells): rAUf = interpolate(rAU); assemble -laplacian(rAUf, p) (face coefficients,
-diagonal, fixedValue/zeroGradient boundary coefficients); source = -div(phiHbyA)*V; fold; normFactor;
-Galerkin re-coarsening; AMG-preconditioned CG to a normalised residual of 1e-6 from p = 0;
-phi = phiHbyA - pEqn.flux(). The mixed-precision path runs the V-cycle in FP32 (or FP16) inside FP64
+channel, 20,000,000 cells, nNonOrthogonalCorrectors 1): rAUf = interpolate(rAU); assemble
+-laplacian(rAUf, p) (face coefficients on nonOrthDeltaCoeffs, diagonal, fixedValue/zeroGradient boundary
+coefficients); Galerkin re-coarsening; then twice: grad(p), the explicit non-orthogonal correction flux
+and its divergence into the source, source -= div(phiHbyA)*V, fold, normFactor, AMG-preconditioned CG to
+a normalised residual of 1e-6 (from p = 0, then from the first solution); finally
+phi = phiHbyA - pEqn.flux() with the stored correction flux. The mixed-precision path runs the V-cycle in FP32 (or FP16) inside FP64
 flexible CG; the FP64 path runs the same solve with an FP64 cycle. Both are timed; `value` is the
 mixed path's seconds per step (lower is better), and the FP64 time, the ratio, iterations, residuals
 and the mixed/FP64 solution difference are reported beside it.
@@ -49,6 +51,8 @@ def parse():
     ap.add_argument("--no-scale", action="store_true", help="plain injection instead of the scaled correction")
     ap.add_argument("--npre", type=int, default=1)
     ap.add_argument("--npost", type=int, default=1)
+    ap.add_argument("--merge-levels", type=int, default=1, help="pairwise passes per AMG level")
+    ap.add_argument("--non-orth", type=int, default=1, help="nNonOrthogonalCorrectors (solves per step - 1)")
     return ap.parse_args()
 
 
@@ -115,6 +119,7 @@ class Channel:
         self.iC, self.bC = z(self.nB), z(self.nB)
         self.source, self.diagC, self.b = z(self.nC), z(self.nC), z(self.nC)
         self.p, self.phi = z(self.nC), z(self.nI)
+        self.gradP, self.ffc = z(3 * self.nC), z(self.nI)
         torch.cuda.synchronize()
         log("fields ready (%.1fs)" % (time.time() - t0))
 
@@ -129,7 +134,7 @@ def gpu_run(args):
     ch = Channel(nx, ny, nz, torch, mfv, hexmesh, ctx)
     t0 = time.time()
     w_host = ch.magSfI.cpu().numpy()
-    amg = mfv.Amg(ch.mesh, w_host, mfv.AGG_BRAE, 64)
+    amg = mfv.Amg(ch.mesh, w_host, mfv.AGG_BRAE, 64, mergeLevels=args.merge_levels)
     sizes = [amg.grid_size(g) for g in range(amg.nGrids)]
     log("AMG hierarchy: %d grids, coarsest %d cells (%.1fs host build)" % (amg.nGrids, sizes[-1][0], time.time() - t0))
     P = mfv.ptr
@@ -137,20 +142,31 @@ def gpu_run(args):
     prec_mixed = mfv.PREC_FP32 if args.precision == "fp32" else mfv.PREC_FP16
 
     def step(prec):
+        """one SIMPLE pressure corrector: nNonOrthogonalCorrectors + 1 assemblies and solves of the same
+        matrix, the explicit non-orthogonal correction rebuilt from the latest p before each, and the flux
+        (with the stored correction flux) after the last"""
         ch.p.zero_()
-        ch.source.zero_()
         ctx.call("mfv_interpolate", ch.mesh.h, P(ch.w), P(ch.negRAU), P(ch.gf), P(ch.gb))
         ctx.call("mfv_laplacian_coeffs", ch.mesh.h, P(ch.ndc), P(ch.gf), P(ch.magSfI), P(ch.diag), P(ch.upper), None)
         ctx.call("mfv_laplacian_bcoeffs", ch.mesh.h, P(ch.types), P(ch.bcValue), P(ch.gb), P(ch.bMagSf),
                  P(ch.bDelta), P(ch.iC), P(ch.bC))
-        ctx.call("mfv_div_source", ch.mesh.h, P(ch.phiHbyA), P(ch.phiB), P(ch.V), -1.0, P(ch.source))
-        ctx.call("mfv_fold", ch.mesh.h, P(ch.diag), P(ch.source), P(ch.iC), P(ch.bC), P(ch.diagC), P(ch.b))
-        amg.galerkin(ch.diagC, ch.upper, prec)
-        perf = amg.solve(ch.diagC, ch.upper, ch.b, ch.p, tolerance=args.tolerance, precision=prec,
-                         flexible=int(prec != mfv.PREC_FP64 or not args.no_scale), checkEvery=args.check_every,
-                         scaleCorrection=int(not args.no_scale), nPreSweeps=args.npre, nPostSweeps=args.npost)
-        ctx.call("mfv_flux_correct", ch.mesh.h, ctypes.byref(A), P(ch.p), P(ch.phiHbyA), -1.0, P(ch.phi))
-        return perf
+        perfs = []
+        for corr in range(args.non_orth + 1):
+            ch.source.zero_()
+            ctx.call("mfv_grad_gauss", ch.mesh.h, P(ch.types), P(ch.bcValue), P(ch.Sf), P(ch.w), P(ch.V), P(ch.p),
+                     P(ch.gradP))
+            ctx.call("mfv_laplacian_corr", ch.mesh.h, P(ch.gradP), P(ch.w), P(ch.corr), P(ch.gf), P(ch.magSfI),
+                     P(ch.ffc), P(ch.source))
+            ctx.call("mfv_div_source", ch.mesh.h, P(ch.phiHbyA), P(ch.phiB), P(ch.V), -1.0, P(ch.source))
+            ctx.call("mfv_fold", ch.mesh.h, P(ch.diag), P(ch.source), P(ch.iC), P(ch.bC), P(ch.diagC), P(ch.b))
+            if corr == 0:
+                amg.galerkin(ch.diagC, ch.upper, prec)
+            perfs.append(amg.solve(ch.diagC, ch.upper, ch.b, ch.p, tolerance=args.tolerance, precision=prec,
+                                   flexible=int(prec != mfv.PREC_FP64 or not args.no_scale),
+                                   checkEvery=args.check_every, scaleCorrection=int(not args.no_scale),
+                                   nPreSweeps=args.npre, nPostSweeps=args.npost))
+        ctx.call("mfv_flux_correct", ch.mesh.h, ctypes.byref(A), P(ch.p), P(ch.ffc), P(ch.phiHbyA), -1.0, P(ch.phi))
+        return perfs
 
     def timed(prec, profile):
         for _ in range(args.warmup):
@@ -162,7 +178,7 @@ def gpu_run(args):
                 ctx.profile_read(tag)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        perfs = [step(prec) for _ in range(args.steps)]
+        perfs = [q for _ in range(args.steps) for q in step(prec)]
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
         prof = None
@@ -192,9 +208,9 @@ def gpu_run(args):
                               iters),
         mfv.PROF_PCG_ZR: ("pcgZrK: z.r, z.r_old", zb * N + 16 * N, iters),
         mfv.PROF_AMG_RESIDUAL0: ("residualK grid 0 (cycle precision)", zb * N + vb * F + 3 * zb * N + 8 * F, iters),
-        mfv.PROF_AMG_PROLONG_SMOOTH0: ("prolongSmoothK grid 0", zb * N + vb * F + 4 * zb * N + 4 * N + zb * Nc + 8 * F,
-                                       iters),
-        mfv.PROF_AMG_RESTRICT0: ("restrictK grid 0 -> 1", zb * N + 4 * N + 4 * Nc + 2 * zb * Nc, iters),
+        mfv.PROF_AMG_CORR_AMUL0: ("corrAmulK grid 0: A e, r.e, e.Ae", zb * N + vb * F + 3 * zb * N + 8 * F, iters),
+        mfv.PROF_AMG_CORR_SMOOTH0: ("corrSmoothK grid 0", 6 * zb * N, iters),
+        mfv.PROF_AMG_RESTRICT0: ("restrictK grid 0 -> 1", zb * N + 4 * N + 4 * Nc + 3 * zb * Nc, iters),
     }
     prof = res["mixed"]["prof"]
     dom = max(names, key=lambda t: prof[t][1])
@@ -269,8 +285,10 @@ def measured_traffic(label):
 
 
 def cpu_baseline(args, n_full):
-    """The oracle's OpenFOAM GAMG-preconditioned PCG (GaussSeidel, V(0,2,2), nVcycles 2, faceAreaPair)
-    on a reduced channel of the same construction, single-threaded, scaled to the full cell count."""
+    """The oracle's FP64 pressure corrector on a reduced channel of the same construction,
+    single-threaded, scaled to the full cell count: the same passes as the GPU step (assembly, then per
+    non-orthogonal pass grad(p), correction flux and source, div(phiHbyA), fold and OpenFOAM's
+    GAMG-preconditioned PCG (faceAreaPair, GaussSeidel, V(0,2,2), nVcycles 2), then the flux)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from brae_amd import hexmesh
@@ -295,9 +313,11 @@ def cpu_baseline(args, n_full):
             phiB[s] = (U[bcell[s]] * m["Sf"][start:start + size]).sum(1)
             types[s] = 1
     negRAU = -0.05 * np.cbrt(m["V"]) / np.linalg.norm(U, axis=1)
+    wts = O.faceareapair_weights(m["Sf"][:nI], m["magSf"][:nI])
+    ag = O.of_agglomeration(nC, own, nei, wts, nCellsInCoarsestLevel=10)
     t = time.perf_counter()
     gf = m["weights"] * negRAU[own] + (1 - m["weights"]) * negRAU[nei]
-    diag, up, _ = np.zeros(nC), np.zeros(nI), None
+    diag, up = np.zeros(nC), np.zeros(nI)
     O.lib().orc_laplacian(nC, nI, O.p(own), O.p(nei), O.p(m["nonOrthDeltaCoeffs"]), O.p(gf),
                           O.p(O.f64(m["magSf"][:nI])), O.p(diag), O.p(up), None)
     iC, bC = np.zeros(nB), np.zeros(nB)
@@ -307,24 +327,26 @@ def cpu_baseline(args, n_full):
     np.add.at(div, own, phi)
     np.add.at(div, nei, -phi)
     np.add.at(div, bcell, phiB)
-    src = -((div / m["V"]) * m["V"])
-    diagC, b = np.zeros(nC), np.zeros(nC)
-    O.lib().orc_fold(nC, O.p(diag), O.p(src), nB, O.p(bcell), O.p(types), O.p(iC), O.p(bC), O.p(diagC), O.p(b))
-    wts = O.faceareapair_weights(m["Sf"][:nI], m["magSf"][:nI])
-    t_setup = time.perf_counter() - t
-    ag = O.of_agglomeration(nC, own, nei, wts, nCellsInCoarsestLevel=10)
-    t = time.perf_counter()
-    psi, perf = O.pcg_gamg_solve(ag, diagC, up, b, np.zeros(nC), args.tolerance, smoother="GaussSeidel")
-    t_solve = time.perf_counter() - t
-    flux = np.zeros(nI)
-    O.lib().orc_face_flux(nI, O.p(own), O.p(nei), O.p(up), None, O.p(psi), O.p(flux))
-    sec = t_setup + t_solve
+    psi = np.zeros(nC)
+    iters = []
+    for _ in range(args.non_orth + 1):
+        g = O.grad_gauss(nC, own, nei, bcell, types, np.zeros(nB), m["Sf"], m["weights"], m["V"], psi)
+        ffc, src = O.laplacian_corr(nC, own, nei, g, m["weights"], m["corrVec"], gf, m["magSf"][:nI],
+                                    source=np.zeros(nC))
+        src = src - (div / m["V"]) * m["V"]
+        diagC, b = np.zeros(nC), np.zeros(nC)
+        O.lib().orc_fold(nC, O.p(diag), O.p(src), nB, O.p(bcell), O.p(types), O.p(iC), O.p(bC), O.p(diagC), O.p(b))
+        psi, perf = O.pcg_gamg_solve(ag, diagC, up, b, psi, args.tolerance, smoother="GaussSeidel")
+        iters.append(perf.nIterations)
+    O.flux_correct(own, nei, up, None, psi, ffc, phi)
+    sec = time.perf_counter() - t
     scaled = sec * n_full / nC
     return dict(value=round(scaled, 3), unit="s/SIMPLE-iter (pressure corrector) @ full size, linear in cells",
                 cores=1, kind="port",
-                sample="oracle FP64 pressure corrector (assembly, fold, OpenFOAM GAMG-PCG GaussSeidel V(0,2,2) "
-                       "nVcycles 2, tol %g) on the %s channel (%d cells): %.2f s, %d PCG iterations; scaled x%.1f"
-                       % (args.tolerance, args.cpu_sample, nC, sec, perf.nIterations, n_full / nC))
+                sample="oracle FP64 pressure corrector (assembly, %d non-orthogonal passes of grad/correction/"
+                       "fold/OpenFOAM GAMG-PCG GaussSeidel V(0,2,2) nVcycles 2 to %g, flux) on the %s channel "
+                       "(%d cells): %.2f s, PCG iterations %s; scaled x%.1f (hierarchy build excluded)"
+                       % (args.non_orth + 1, args.tolerance, args.cpu_sample, nC, sec, iters, n_full / nC))
 
 
 def main():
@@ -349,8 +371,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 outer / %s V-cycle" % args.precision,
         "data": "synthetic (seeded perturbed plug flow on the graded jittered channel; p solved from 0)",
-        "config": {"workload": "SIMPLE pressure corrector, %s graded (ratio 20) jittered (10%%) channel 4x1x0.5, "
-                               "%d cells, AMG-PCG to %g" % (args.size, nx * ny * nz, args.tolerance),
+        "config": {"workload": "SIMPLE pressure corrector (nNonOrthogonalCorrectors %d), %s graded (ratio 20) "
+                               "jittered (10%%) channel 4x1x0.5, %d cells, AMG-PCG to %g"
+                               % (args.non_orth, args.size, nx * ny * nz, args.tolerance),
                    "cells": nx * ny * nz, "tolerance": args.tolerance, "v_cycle": args.precision},
         "fp64_s_per_iter": round(r["dt_fp64"], 5),
         "speedup_mixed_vs_fp64": round(r["dt_fp64"] / r["dt_mixed"], 3),
@@ -358,6 +381,7 @@ def main():
         "final_residual": {"mixed": r["final_mixed"], "fp64": r["final_fp64"]},
         "p_rel_l2_mixed_vs_fp64": r["p_rel_l2_mixed_vs_fp64"],
         "amg": {"grids": r["n_grids"], "coarsest_cells": r["coarsest"], "vcycle_avg_us": r["vcycle_avg_us"],
+                "mergeLevels": args.merge_levels,
                 "cycle": "V(%d,%d) weighted Jacobi%s" % (args.npre, args.npost,
                                                         "" if args.no_scale else ", scaled correction"),
                 "per_grid_us": r["grids"]},

@@ -403,8 +403,9 @@ int mfv_amg_build(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const 
             if (controls->agglomeration != MFV_AGG_BRAE && controls->agglomeration != MFV_AGG_FACEAREAPAIR)
                 throw Error(kErrUnsupported, "mfv: amg build: unknown agglomeration " +
                                                  std::to_string(controls->agglomeration));
-            if (controls->mergeLevels > 1)
-                throw Error(kErrUnsupported, "mfv: amg build: mergeLevels above 1 is not carried");
+            if (controls->mergeLevels > 4)
+                throw Error(kErrUnsupported, "mfv: amg build: mergeLevels above 4 is not carried");
+            bc.mergeLevels = std::max(1, controls->mergeLevels);
             bc.kind = controls->agglomeration == MFV_AGG_BRAE ? AggKind::Brae : AggKind::OpenFOAM;
             bc.target = controls->nCellsInCoarsestLevel > 0 ? controls->nCellsInCoarsestLevel
                                                             : (bc.kind == AggKind::Brae ? 64 : 10);
@@ -593,13 +594,37 @@ int mfv_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double*
     });
 }
 
-int mfv_flux_correct(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dPhiHbyA,
-                     double sign, double* dPhi)
+int mfv_flux_correct(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dFfc,
+                     const double* dPhiHbyA, double sign, double* dPhi)
 {
     return guarded(ctx, [&] {
         needMesh(ctx, mesh);
         need(dPsi && dPhiHbyA && dPhi, "flux correct: null argument");
-        launchFluxCorrect(ctx->c, mesh->m, view(mesh, A), dPsi, dPhiHbyA, sign, dPhi);
+        launchFluxCorrect(ctx->c, mesh->m, view(mesh, A), dPsi, dFfc, dPhiHbyA, sign, dPhi);
+    });
+}
+
+int mfv_grad_gauss(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue, const double* dSf,
+                   const double* dWeights, const double* dV, const double* dPsi, double* dGrad)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dSf && dV && dPsi && dGrad, "grad: null argument");
+        need(!mesh->m.nFaces || dWeights, "grad: weights missing");
+        need(!mesh->m.nBFaces || (dBcType && dBcValue), "grad: boundary types/values missing");
+        launchGradGauss(ctx->c, mesh->m, dBcType, dBcValue, dSf, dWeights, dV, dPsi, dGrad);
+    });
+}
+
+int mfv_laplacian_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dGrad, const double* dWeights,
+                       const double* dCorrVec, const double* dGammaf, const double* dMagSf, double* dFfc,
+                       double* dSource)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(!mesh->m.nFaces || (dGrad && dWeights && dCorrVec && dGammaf && dMagSf && dFfc),
+             "laplacian correction: null argument");
+        launchLaplacianCorr(ctx->c, mesh->m, dGrad, dWeights, dCorrVec, dGammaf, dMagSf, dFfc, dSource);
     });
 }
 

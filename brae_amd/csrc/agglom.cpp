@@ -326,4 +326,51 @@ std::vector<AggLevel> agglomerate(AggKind kind, int nCells, const std::vector<in
     return levels;
 }
 
+// OpenFOAM's mergeLevels (pairGAMGAgglomeration: combineLevels while nPairLevels % mergeLevels != 0):
+// each run of `merge` consecutive passes becomes one level. The composed level maps a fine cell through
+// both passes; a fine face lands on the second pass's coarse face, or inside a coarse cell when either
+// pass swallowed it; its flip is the parity of the two flips.
+static AggLevel compose(const AggLevel& A, const AggLevel& B)
+{
+    AggLevel C;
+    C.nFine = A.nFine;
+    C.nFineFaces = A.nFineFaces;
+    C.nCoarse = B.nCoarse;
+    C.nCoarseFaces = B.nCoarseFaces;
+    C.map.resize(A.nFine);
+    for (int i = 0; i < A.nFine; ++i) C.map[i] = B.map[A.map[i]];
+    C.cOwn = B.cOwn;
+    C.cNei = B.cNei;
+    C.faceRestrict.resize(A.nFineFaces);
+    C.faceFlip.assign(A.nFineFaces, 0);
+    for (int f = 0; f < A.nFineFaces; ++f)
+    {
+        const int a = A.faceRestrict[f];
+        if (a >= 0)
+        {
+            const int b = B.faceRestrict[a];
+            C.faceRestrict[f] = b;
+            if (b >= 0) C.faceFlip[f] = (int8_t)((A.faceFlip[f] != 0) != (B.faceFlip[a] != 0));
+        }
+        else
+        {
+            C.faceRestrict[f] = -(B.map[-1 - a] + 1);
+        }
+    }
+    return C;
+}
+
+std::vector<AggLevel> mergeLevels(std::vector<AggLevel> passes, int merge)
+{
+    if (merge <= 1) return passes;
+    std::vector<AggLevel> out;
+    for (size_t i = 0; i < passes.size(); i += (size_t)merge)
+    {
+        AggLevel L = std::move(passes[i]);
+        for (size_t k = i + 1; k < passes.size() && k < i + (size_t)merge; ++k) L = compose(L, passes[k]);
+        out.push_back(std::move(L));
+    }
+    return out;
+}
+
 } // namespace mfv

@@ -32,4 +32,7 @@ std::vector<AggLevel> agglomerate(AggKind kind, int nCells, const std::vector<in
                                   const std::vector<int>& fineNeighbour, const std::vector<double>& weights,
                                   int target, bool& forward);
 
+// Composes each run of `merge` consecutive levels into one (OpenFOAM's mergeLevels); 1 leaves them.
+std::vector<AggLevel> mergeLevels(std::vector<AggLevel> levels, int merge);
+
 } // namespace mfv

@@ -40,35 +40,64 @@ __global__ __launch_bounds__(kTpb) void residualK(SellView S, const Tm* __restri
     r[c] = b[c] - ax;
 }
 
+// The coarse-to-fine injection, fused into whichever kernel finishes a coarse grid's cycle: coarse cell
+// C's final value goes to every fine cell of its agglomerate (the fine grid's cellStart/cellList), so
+// the fine grid reads its correction e = xc[map] as a plain vector (prolongT's values, written once).
+template <typename Tv>
+struct Scatter
+{
+    const int* __restrict__ start;
+    const int* __restrict__ list;
+    Tv* __restrict__ e;   // null: no fine grid waits for this one (grid 0)
+    __device__ void operator()(int C, Tv v) const
+    {
+        if (!e) return;
+        for (int k = start[C]; k < start[C + 1]; ++k) e[list[k]] = v;
+    }
+};
+
 // xOut = x + omega*(b - A x)/D  (amgSpmvF then smoothT)
 template <typename Tm, typename Tv>
 __global__ __launch_bounds__(kTpb) void smoothK(SellView S, const Tm* __restrict__ val, const Tv* __restrict__ diag,
                                                 const double* __restrict__ scale, const Tv* __restrict__ x,
-                                                const Tv* __restrict__ b, Tv* __restrict__ xOut,
+                                                const Tv* __restrict__ b, Tv* __restrict__ xOut, Scatter<Tv> sc,
                                                 const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
     const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, PlainX<Tv>{x});
-    xOut[c] = x[c] + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+    const Tv xo = x[c] + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+    xOut[c] = xo;
+    sc(c, xo);
 }
 
-// prolongation fused into the first post-sweep: xin = x + xc[map] (prolongT), then
+// x + e at any cell: the prolonged iterate a fused first post-sweep works on
+template <typename Tv>
+struct SumX
+{
+    const Tv* __restrict__ x;
+    const Tv* __restrict__ e;
+    __device__ Tv operator()(int j) const { return x[j] + e[j]; }
+};
+
+// prolongation fused into the first post-sweep: xin = x + e (prolongT), then
 // xOut = xin + omega*(b - A xin)/D
 template <typename Tm, typename Tv>
 __global__ __launch_bounds__(kTpb) void prolongSmoothK(SellView S, const Tm* __restrict__ val,
                                                        const Tv* __restrict__ diag, const double* __restrict__ scale,
-                                                       const int* __restrict__ map, const Tv* __restrict__ xc,
-                                                       const Tv* __restrict__ x, const Tv* __restrict__ b,
-                                                       Tv* __restrict__ xOut, const int* __restrict__ stop)
+                                                       const Tv* __restrict__ e, const Tv* __restrict__ x,
+                                                       const Tv* __restrict__ b, Tv* __restrict__ xOut,
+                                                       Scatter<Tv> sc, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
-    ProlongedX<Tv> X{x, map, xc};
+    const SumX<Tv> X{x, e};
     const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X);
-    xOut[c] = X(c) + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+    const Tv xo = X(c) + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+    xOut[c] = xo;
+    sc(c, xo);
 }
 
 // Scaled coarse correction, first half: e = xc[map] (prolonged), Ae = A e on the grid (e gathered at the
@@ -80,32 +109,23 @@ struct AlphaPost
     __device__ void operator()(double num, double den) const { *alpha = den > 1e-30 ? num / den : 1.0; }
 };
 
-template <typename Tv>
-struct CorrX
-{
-    const int* __restrict__ map;
-    const Tv* __restrict__ xc;
-    __device__ Tv operator()(int j) const { return xc[map[j]]; }
-};
-
 template <typename Tm, typename Tv>
 __global__ __launch_bounds__(kReduceBlock) void corrAmulK(SellView S, const Tm* __restrict__ val,
                                                           const Tv* __restrict__ diag, const double* __restrict__ scale,
-                                                          const int* __restrict__ map, const Tv* __restrict__ xc,
-                                                          const Tv* __restrict__ r, Tv* __restrict__ Ae,
-                                                          double* pa, double* pb, double* seqA, double* seqB,
-                                                          const int* __restrict__ stop)
+                                                          const Tv* __restrict__ e, const Tv* __restrict__ r,
+                                                          Tv* __restrict__ Ae, double* pa, double* pb, double* seqA,
+                                                          double* seqB, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     __shared__ double lds[kReduceBlock / 64];
-    const CorrX<Tv> E{map, xc};
+    const PlainX<Tv> E{e};
     double num = 0.0, den = 0.0;
     MFV_ROWS(S.n, c)
     {
         const Tv ae = rowAx<Tm, Tv>(S, c, val, diag, scale, E);
         Ae[c] = ae;
-        const double e = (double)E(c);
-        const double tn = e * (double)r[c], td = e * (double)ae;
+        const double ev = (double)E(c);
+        const double tn = ev * (double)r[c], td = ev * (double)ae;
         if (seqA)
         {
             seqA[c] = tn;
@@ -129,30 +149,32 @@ __global__ __launch_bounds__(kReduceBlock) void corrAmulK(SellView S, const Tm* 
 
 // second half, in place: x = x + alpha*e + omega*(r - alpha*Ae)/D (smooth) or x + alpha*e (no sweep)
 template <typename Tv>
-__global__ __launch_bounds__(kTpb) void corrSmoothK(int n, const int* __restrict__ map, const Tv* __restrict__ xc,
-                                                    const Tv* __restrict__ r, const Tv* __restrict__ Ae,
-                                                    const Tv* __restrict__ diag, const double* __restrict__ alpha,
-                                                    int smooth, Tv* __restrict__ x, const int* __restrict__ stop)
+__global__ __launch_bounds__(kTpb) void corrSmoothK(int n, const Tv* __restrict__ e, const Tv* __restrict__ r,
+                                                    const Tv* __restrict__ Ae, const Tv* __restrict__ diag,
+                                                    const double* __restrict__ alpha, int smooth,
+                                                    Tv* __restrict__ x, Scatter<Tv> sc, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     const Tv al = (Tv)(*alpha);
-    const Tv e = xc[map[c]];
-    if (smooth)
-        x[c] = x[c] + al * e + Tv(kOmega) * (r[c] - al * Ae[c]) / safeDiag(diag[c]);
-    else
-        x[c] = x[c] + al * e;
+    const Tv ev = e[c];
+    const Tv xo = smooth ? x[c] + al * ev + Tv(kOmega) * (r[c] - al * Ae[c]) / safeDiag(diag[c]) : x[c] + al * ev;
+    x[c] = xo;
+    sc(c, xo);
 }
 
-// prolongation alone (no post-sweep): x += xc[map]
+// prolongation alone (no post-sweep): x += e
 template <typename Tv>
-__global__ __launch_bounds__(kTpb) void prolongK(int n, const int* __restrict__ map, const Tv* __restrict__ xc,
-                                                 Tv* __restrict__ x, const int* __restrict__ stop)
+__global__ __launch_bounds__(kTpb) void prolongK(int n, const Tv* __restrict__ e, Tv* __restrict__ x, Scatter<Tv> sc,
+                                                 const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
-    if (c < n) x[c] += xc[map[c]];
+    if (c >= n) return;
+    const Tv xo = x[c] + e[c];
+    x[c] = xo;
+    sc(c, xo);
 }
 
 // coarse rhs = sum of the fine residual over each agglomerate, ascending fine cell (restrictGatherT);
@@ -197,7 +219,7 @@ __global__ __launch_bounds__(kTpb) void startK(int n, const double* __restrict__
 // coarsest grid: x = Ainv * b in FP64, ascending column per row (one workgroup)
 template <typename Tv>
 __global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __restrict__ inv, const Tv* __restrict__ b,
-                                                     Tv* __restrict__ x, const int* __restrict__ stop)
+                                                     Tv* __restrict__ x, Scatter<Tv> sc, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     __shared__ double sb[1024];
@@ -209,6 +231,7 @@ __global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __rest
         const double* row = inv + (size_t)i * n;
         for (int j = 0; j < n; ++j) s += row[j] * sb[j];
         x[i] = (Tv)s;
+        sc(i, (Tv)s);
     }
 }
 
@@ -572,26 +595,29 @@ DBuf<__half>& offOf<__half>(Grid& g) { return g.off16; }
 template <typename Tv>
 struct VecSet
 {
-    Tv *b, *x, *y, *r;
+    Tv *b, *x, *y, *r, *e;
     const Tv* diag;
 };
 
-VecSet<double> vecs64(Grid& g) { return {g.b64.p, g.x64.p, g.y64.p, g.r64.p, g.diagV64.p}; }
-VecSet<float> vecs32(Grid& g) { return {g.b32.p, g.x32.p, g.y32.p, g.r32.p, g.diagV32.p}; }
+VecSet<double> vecs64(Grid& g) { return {g.b64.p, g.x64.p, g.y64.p, g.r64.p, g.e64.p, g.diagV64.p}; }
+VecSet<float> vecs32(Grid& g) { return {g.b32.p, g.x32.p, g.y32.p, g.r32.p, g.e32.p, g.diagV32.p}; }
 
-// Runs the cycle from grid gi down and back; returns the buffer holding grid gi's result (x or y:
-// the ping-pong sweeps leave it in either, and the caller reads it where it is).
+// Runs the cycle from grid gi down and back. Grid gi's final values go to up (the finer grid's e, or
+// nowhere for grid 0); the return value is the buffer that holds them (x or y: the ping-pong sweeps
+// leave it in either).
 template <typename Tm, typename Tv>
-Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, VecSet<Tv> (*vs)(Grid&))
+Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, VecSet<Tv> (*vs)(Grid&),
+             Scatter<Tv> up)
 {
     Grid& g = a.grids[gi];
     VecSet<Tv> v = vs(g);
     const int n = g.n;
     const int last = (int)a.grids.size() - 1;
+    const Scatter<Tv> none{nullptr, nullptr, nullptr};
     if (gi == last)
     {
         ProfScope ps(c, kProfGridDown + gi);
-        coarseApplyK<Tv><<<1, kTpb, 0, c.stream>>>(n, a.coarseInv.p, v.b, v.x, stop);
+        coarseApplyK<Tv><<<1, kTpb, 0, c.stream>>>(n, a.coarseInv.p, v.b, v.x, up, stop);
         return v.x;
     }
     ProfScope psDown(c, kProfGridDown + gi);
@@ -601,7 +627,7 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
     // pre-sweeps after the first (the first ran from zero where b was made)
     for (int s = 1; s < vc.nPre; ++s)
     {
-        smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, stop);
+        smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, none, stop);
         std::swap(v.x, v.y);
     }
     if (vc.nPre == 0) zeroK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, v.x, stop);
@@ -618,49 +644,57 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
                                                             (!coarseIsLast && vc.nPre > 0) ? 1 : 0, stop);
     }
     psDown.close();
-    const Tv* xc = vcycleAt<Tm, Tv>(c, a, gi + 1, vc, stop, vs);
+    // the coarse grid leaves its result, injected, in this grid's e
+    vcycleAt<Tm, Tv>(c, a, gi + 1, vc, stop, vs, Scatter<Tv>{g.cellStart.p, g.cellList.p, v.e});
     ProfScope psUp(c, kProfGridUp + gi);
+    // the kernel that finishes this grid hands its values on to the finer grid
+    const int nExtra = std::max(0, vc.nPost - 1);
     if (vc.scale)
     {
         const bool seq = c.sequentialReduce;
+        const int gr = reduceGrid(n);
+        double* p1 = c.partials.p + 1 * (size_t)kMaxReduceBlocks;
+        double* p2 = c.partials.p + 2 * (size_t)kMaxReduceBlocks;
         {
-            ProfScope ps(c, gi == 0 ? kProfAmgProlongSmooth0 : kProfCount);
-            const int gr = reduceGrid(n);
-            double* p1 = c.partials.p + 1 * (size_t)kMaxReduceBlocks;
-            double* p2 = c.partials.p + 2 * (size_t)kMaxReduceBlocks;
-            corrAmulK<Tm, Tv><<<gr, kReduceBlock, 0, c.stream>>>(S, val, v.diag, sc, g.map.p, xc, v.r, v.y, p1, p2,
+            ProfScope ps(c, gi == 0 ? kProfAmgCorrAmul0 : kProfCount);
+            corrAmulK<Tm, Tv><<<gr, kReduceBlock, 0, c.stream>>>(S, val, v.diag, sc, v.e, v.r, v.y, p1, p2,
                                                                  seq ? a.seqA.p : nullptr, seq ? a.seqB.p : nullptr,
                                                                  stop);
-            if (seq)
-                seqFinish2K<<<1, 64, 0, c.stream>>>(n, a.seqA.p, a.seqB.p, AlphaPost{a.alphas.p + gi}, stop);
-            else
-                finish2K<<<1, kReduceBlock, 0, c.stream>>>(gr, p1, p2, AlphaPost{a.alphas.p + gi}, stop);
-            corrSmoothK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, g.map.p, xc, v.r, v.y, v.diag, a.alphas.p + gi,
-                                                              vc.nPost > 0 ? 1 : 0, v.x, stop);
         }
-        for (int s = 1; s < vc.nPost; ++s)
+        if (seq)
+            seqFinish2K<<<1, 64, 0, c.stream>>>(n, a.seqA.p, a.seqB.p, AlphaPost{a.alphas.p + gi}, stop);
+        else
+            finish2K<<<1, kFinishBlock, 0, c.stream>>>(gr, p1, p2, AlphaPost{a.alphas.p + gi}, stop);
         {
-            smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, stop);
+            ProfScope ps(c, gi == 0 ? kProfAmgCorrSmooth0 : kProfCount);
+            corrSmoothK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, v.e, v.r, v.y, v.diag, a.alphas.p + gi,
+                                                              vc.nPost > 0 ? 1 : 0, v.x, nExtra ? none : up, stop);
+        }
+        for (int s = 0; s < nExtra; ++s)
+        {
+            smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y,
+                                                              s + 1 == nExtra ? up : none, stop);
             std::swap(v.x, v.y);
         }
     }
     else if (vc.nPost > 0)
     {
         {
-            ProfScope ps(c, gi == 0 ? kProfAmgProlongSmooth0 : kProfCount);
-            prolongSmoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, g.map.p, xc, v.x, v.b, v.y,
-                                                                      stop);
+            ProfScope ps(c, gi == 0 ? kProfAmgCorrSmooth0 : kProfCount);
+            prolongSmoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.e, v.x, v.b, v.y,
+                                                                      nExtra ? none : up, stop);
         }
         std::swap(v.x, v.y);
-        for (int s = 1; s < vc.nPost; ++s)
+        for (int s = 0; s < nExtra; ++s)
         {
-            smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, stop);
+            smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y,
+                                                              s + 1 == nExtra ? up : none, stop);
             std::swap(v.x, v.y);
         }
     }
     else
     {
-        prolongK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, g.map.p, xc, v.x, stop);
+        prolongK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, v.e, v.x, up, stop);
     }
     hipCheck(hipGetLastError(), "vcycle");
     return v.x;
@@ -675,6 +709,7 @@ void allocVectors<double>(Ctx& c, Grid& g)
     zeroBuf(c, g.x64, g.n);
     zeroBuf(c, g.y64, g.n);
     zeroBuf(c, g.r64, g.n);
+    zeroBuf(c, g.e64, g.n);
 }
 template <>
 void allocVectors<float>(Ctx& c, Grid& g)
@@ -683,6 +718,7 @@ void allocVectors<float>(Ctx& c, Grid& g)
     zeroBuf(c, g.x32, g.n);
     zeroBuf(c, g.y32, g.n);
     zeroBuf(c, g.r32, g.n);
+    zeroBuf(c, g.e32, g.n);
 }
 
 } // namespace
@@ -693,7 +729,8 @@ void amgBuild(Ctx& c, const Mesh& m, Amg& a, const std::vector<double>& faceWeig
     if (ctl.target < 1) invalid("amg build: the coarsest-level target must be positive");
     bool forward = true;
     a.nFine = m.nCells;
-    a.agg = agglomerate(ctl.kind, m.nCells, m.hOwner, m.hNeighbour, faceWeights, ctl.target, forward);
+    a.agg = mergeLevels(agglomerate(ctl.kind, m.nCells, m.hOwner, m.hNeighbour, faceWeights, ctl.target, forward),
+                        ctl.mergeLevels);
     const int nG = (int)a.agg.size() + 1;
     a.grids.clear();
     a.grids.resize(nG);
@@ -845,9 +882,9 @@ void* amgVcycle(Ctx& c, Amg& a, const VcycleControls& vc, const int* stop)
     if (!a.galerkinDone) throw Error(kErrState, "mfv: amg_galerkin must run before a V-cycle");
     switch (a.prec)
     {
-    case Prec::F64: return vcycleAt<double, double>(c, a, 0, vc, stop, vecs64);
-    case Prec::F32: return vcycleAt<float, float>(c, a, 0, vc, stop, vecs32);
-    case Prec::F16: return vcycleAt<__half, float>(c, a, 0, vc, stop, vecs32);
+    case Prec::F64: return vcycleAt<double, double>(c, a, 0, vc, stop, vecs64, Scatter<double>{});
+    case Prec::F32: return vcycleAt<float, float>(c, a, 0, vc, stop, vecs32, Scatter<float>{});
+    case Prec::F16: return vcycleAt<__half, float>(c, a, 0, vc, stop, vecs32, Scatter<float>{});
     }
     return nullptr;
 }

@@ -58,8 +58,8 @@ struct Grid
     DBuf<double> diagV64;
     DBuf<float> diagV32;
     // V-cycle vectors in the vector precision (double for the FP64 cycle, float otherwise)
-    DBuf<double> b64, x64, y64, r64;
-    DBuf<float> b32, x32, y32, r32;
+    DBuf<double> b64, x64, y64, r64, e64;   // e: the coarse correction prolonged onto this grid
+    DBuf<float> b32, x32, y32, r32, e32;
     // restriction/prolongation to the next grid (absent on the coarsest)
     DBuf<int> map;                       // [n] fine -> coarse
     DBuf<int> cellStart, cellList;       // coarse cell <- fine cells, ascending
@@ -74,6 +74,7 @@ struct AmgBuildControls
 {
     AggKind kind = AggKind::Brae;
     int target = 64;
+    int mergeLevels = 1;   // pairwise passes composed into each level
 };
 
 // Work vectors and device state of the PCG solve on a hierarchy (FP64, one per cell).

@@ -61,7 +61,7 @@ void launchDot(Ctx& c, int n, const double* a, const double* b, double* out, int
         const int g = reduceGrid(n);
         double* pa = c.partials.p + (size_t)slot * kMaxReduceBlocks;
         dotK<<<g, kReduceBlock, 0, c.stream>>>(n, a, b, pa);
-        finishK<<<1, kReduceBlock, 0, c.stream>>>(g, pa, StoreTo{out}, nullptr);
+        finishK<<<1, kFinishBlock, 0, c.stream>>>(g, pa, StoreTo{out}, nullptr);
     }
     hipCheck(hipGetLastError(), "dot");
 }
@@ -77,7 +77,7 @@ void launchSumMag(Ctx& c, int n, const double* a, double* out, int slot)
         const int g = reduceGrid(n);
         double* pa = c.partials.p + (size_t)slot * kMaxReduceBlocks;
         sumMagK<<<g, kReduceBlock, 0, c.stream>>>(n, a, pa);
-        finishK<<<1, kReduceBlock, 0, c.stream>>>(g, pa, StoreTo{out}, nullptr);
+        finishK<<<1, kFinishBlock, 0, c.stream>>>(g, pa, StoreTo{out}, nullptr);
     }
     hipCheck(hipGetLastError(), "sumMag");
 }

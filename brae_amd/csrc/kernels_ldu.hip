@@ -317,18 +317,107 @@ __global__ __launch_bounds__(kTpb) void divSourceK(int n, const label* __restric
     source[c] += sign * (d * V[c]);
 }
 
-// phi = phiHbyA - sign*(upper*psi[nei] - lower*psi[own]) (pEqn_cpp.cuh:134-139 with the matrix's sign)
+// phi = phiHbyA - sign*(upper*psi[nei] - lower*psi[own] + ffc) (pEqn_cpp.cuh:134-139 with the
+// matrix's sign; ffc the stored non-orthogonal correction flux, fv_matrix_ops.cuh:46-48, or none)
 __global__ __launch_bounds__(kTpb) void fluxCorrectK(int nF, const label* __restrict__ owner,
                                                      const label* __restrict__ neighbour,
                                                      const double* __restrict__ upper,
                                                      const double* __restrict__ lower, const double* __restrict__ psi,
+                                                     const double* __restrict__ ffc,
                                                      const double* __restrict__ phiHbyA, double sign,
                                                      double* __restrict__ phi)
 {
     const int f = xcdBlock() * kTpb + threadIdx.x;
     if (f >= nF) return;
-    const double flux = upper[f] * psi[neighbour[f]] - lower[f] * psi[owner[f]];
+    double flux = upper[f] * psi[neighbour[f]] - lower[f] * psi[owner[f]];
+    if (ffc) flux += ffc[f];
     phi[f] = phiHbyA[f] - sign * flux;
+}
+
+// ---------------- non-orthogonal correction ----------------
+
+// Gauss linear gradient of a cell field with its patch values (fvc.cu:67-121): per cell, its faces in
+// ascending face index (neighbour-side run, then owned run: -Sf*pf and +Sf*pf with the face value
+// w*(P - N) + N), then its boundary faces in order (empty: none; fixedValue: the bc value;
+// zeroGradient: the cell value), divided by V. Sf: xyz per face, internal then boundary; grad: xyz.
+__global__ __launch_bounds__(kTpb) void gradGaussK(int n, int nF, const label* __restrict__ ownerStart,
+                                                   const label* __restrict__ losortStart,
+                                                   const label* __restrict__ losort,
+                                                   const label* __restrict__ losortOwner,
+                                                   const label* __restrict__ neighbour,
+                                                   const label* __restrict__ bndStart,
+                                                   const label* __restrict__ bndFaces, const int* __restrict__ bcType,
+                                                   const double* __restrict__ bcValue, const double* __restrict__ Sf,
+                                                   const double* __restrict__ w, const double* __restrict__ V,
+                                                   const double* __restrict__ psi, double* __restrict__ grad)
+{
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    if (c >= n) return;
+    const double pc = psi[c];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int k = losortStart[c]; k < losortStart[c + 1]; ++k)
+    {
+        const int f = losort[k];
+        const double pf = w[f] * (psi[losortOwner[k]] - pc) + pc;
+        gx = gx - Sf[3 * f] * pf;
+        gy = gy - Sf[3 * f + 1] * pf;
+        gz = gz - Sf[3 * f + 2] * pf;
+    }
+    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f)
+    {
+        const double pf = w[f] * (pc - psi[neighbour[f]]) + psi[neighbour[f]];
+        gx = gx + Sf[3 * f] * pf;
+        gy = gy + Sf[3 * f + 1] * pf;
+        gz = gz + Sf[3 * f + 2] * pf;
+    }
+    for (int k = bndStart[c]; k < bndStart[c + 1]; ++k)
+    {
+        const int i = bndFaces[k];
+        const int t = bcType[i];
+        if (t == 2) continue;
+        const double pv = t == 1 ? bcValue[i] : pc;
+        const long long fb = 3LL * (nF + i);
+        gx = gx + Sf[fb] * pv;
+        gy = gy + Sf[fb + 1] * pv;
+        gz = gz + Sf[fb + 2] * pv;
+    }
+    const double v = V[c];
+    grad[3 * c] = gx / v;
+    grad[3 * c + 1] = gy / v;
+    grad[3 * c + 2] = gz / v;
+}
+
+// ffc = (gamma*magSf) * (corrVec . (w*grad[own] + (1 - w)*grad[nei])) per internal face (fvm.cuh:170-222)
+__global__ __launch_bounds__(kTpb) void corrFluxK(int nF, const label* __restrict__ owner,
+                                                  const label* __restrict__ neighbour, const double* __restrict__ grad,
+                                                  const double* __restrict__ w, const double* __restrict__ cv,
+                                                  const double* __restrict__ gammaf, const double* __restrict__ magSf,
+                                                  double* __restrict__ ffc)
+{
+    const int f = xcdBlock() * kTpb + threadIdx.x;
+    if (f >= nF) return;
+    const int o = owner[f], nb = neighbour[f];
+    const double wf = w[f], wn = 1.0 - wf;
+    const double fx = wf * grad[3 * o] + wn * grad[3 * nb];
+    const double fy = wf * grad[3 * o + 1] + wn * grad[3 * nb + 1];
+    const double fz = wf * grad[3 * o + 2] + wn * grad[3 * nb + 2];
+    const double corr = cv[3 * f] * fx + cv[3 * f + 1] * fy + cv[3 * f + 2] * fz;
+    ffc[f] = (gammaf[f] * magSf[f]) * corr;
+}
+
+// source -= sum over the cell's faces of (+ffc owned, -ffc neighbour-side), ascending face index
+// (fvm.cuh:293-299, pEqn_cpp.cu:228)
+__global__ __launch_bounds__(kTpb) void corrSourceK(int n, const label* __restrict__ ownerStart,
+                                                    const label* __restrict__ losortStart,
+                                                    const label* __restrict__ losort, const double* __restrict__ ffc,
+                                                    double* __restrict__ source)
+{
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    if (c >= n) return;
+    double s = 0.0;
+    for (int k = losortStart[c]; k < losortStart[c + 1]; ++k) s -= ffc[losort[k]];
+    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f) s += ffc[f];
+    source[c] -= s;
 }
 
 } // namespace
@@ -351,13 +440,35 @@ void launchDivSource(Ctx& c, const Mesh& m, const double* phi, const double* phi
     hipCheck(hipGetLastError(), "div source");
 }
 
-void launchFluxCorrect(Ctx& c, const Mesh& m, const LduView& A, const double* psi, const double* phiHbyA, double sign,
-                       double* phi)
+void launchFluxCorrect(Ctx& c, const Mesh& m, const LduView& A, const double* psi, const double* ffc,
+                       const double* phiHbyA, double sign, double* phi)
 {
     if (!m.nFaces) return;
     fluxCorrectK<<<blocks(m.nFaces), kTpb, 0, c.stream>>>(m.nFaces, m.owner.p, m.neighbour.p, A.upper, A.lower, psi,
-                                                          phiHbyA, sign, phi);
+                                                          ffc, phiHbyA, sign, phi);
     hipCheck(hipGetLastError(), "flux correct");
+}
+
+void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcValue, const double* Sf,
+                     const double* w, const double* V, const double* psi, double* grad)
+{
+    if (!m.nCells) return;
+    gradGaussK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.nFaces, m.ownerStart.p, m.losortStart.p,
+                                                        m.losort.p, m.losortOwner.p, m.neighbour.p, m.bndStart.p,
+                                                        m.bndFaces.p, bcType, bcValue, Sf, w, V, psi, grad);
+    hipCheck(hipGetLastError(), "grad gauss");
+}
+
+void launchLaplacianCorr(Ctx& c, const Mesh& m, const double* grad, const double* w, const double* corrVec,
+                         const double* gammaf, const double* magSf, double* ffc, double* source)
+{
+    if (m.nFaces)
+        corrFluxK<<<blocks(m.nFaces), kTpb, 0, c.stream>>>(m.nFaces, m.owner.p, m.neighbour.p, grad, w, corrVec, gammaf,
+                                                           magSf, ffc);
+    if (source && m.nCells)
+        corrSourceK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.ownerStart.p, m.losortStart.p, m.losort.p,
+                                                             ffc, source);
+    hipCheck(hipGetLastError(), "laplacian correction");
 }
 
 void launchAmul(Ctx& c, const Mesh& m, const LduView& A, const double* x, double* y)
@@ -392,11 +503,11 @@ void launchNormFactor(Ctx& c, const Mesh& m, const LduView& A, const double* psi
         const int g = reduceGrid(n);
         double* pa = c.partials.p + 3 * (size_t)kMaxReduceBlocks;
         sumK<<<g, kReduceBlock, 0, c.stream>>>(n, psi, pa);
-        finishK<<<1, kReduceBlock, 0, c.stream>>>(g, pa, StoreTo{sumPsi}, nullptr);
+        finishK<<<1, kFinishBlock, 0, c.stream>>>(g, pa, StoreTo{sumPsi}, nullptr);
         normFactorK<<<g, kReduceBlock, 0, c.stream>>>(n, m.ownerStart.p, m.losortStart.p, m.losort.p,
                                                       m.losortOwner.p, m.neighbour.p, A.diag, A.upper, A.lower, psi,
                                                       b, sumPsi, pa);
-        finishK<<<1, kReduceBlock, 0, c.stream>>>(g, pa, NormPost{out}, nullptr);
+        finishK<<<1, kFinishBlock, 0, c.stream>>>(g, pa, NormPost{out}, nullptr);
     }
     hipCheck(hipGetLastError(), "normFactor");
 }

@@ -97,10 +97,12 @@ struct DBuf
     size_t bytes() const { return n * sizeof(T); }
 };
 
-// Fixed partition of a reduction: a grid that depends on n only, so every call on the same length
-// sums in the same order.
+// Fixed partition of a reduction: a grid that depends on n only (two rows per thread), so every call
+// on the same length sums in the same order. kMaxReduceBlocks partials per slot cover 134M rows; the
+// finishing workgroup (kFinishBlock threads) reads them with several loads in flight per thread.
 constexpr int kReduceBlock = 256;
-constexpr int kMaxReduceBlocks = 65536;
+constexpr int kMaxReduceBlocks = 1 << 18;
+constexpr int kFinishBlock = 1024;
 
 // Per-context scratch for two-stage reductions: partial sums per workgroup and the results (device
 // scalars), plus a pinned host mailbox for read-backs.
@@ -129,8 +131,9 @@ enum ProfTag : int
     kProfPcgZr,           // z.r
     kProfVcycle,          // one whole V-cycle
     kProfAmgResidual0,    // grid-0 residual in the cycle precision
-    kProfAmgProlongSmooth0,  // grid-0 prolongation + post-sweep
+    kProfAmgCorrSmooth0,  // grid-0 correction update + first post-sweep (corrSmoothK or prolongSmoothK)
     kProfAmgRestrict0,    // grid 0 -> 1 restriction
+    kProfAmgCorrAmul0,    // grid-0 scaled correction's A e and its two sums (corrAmulK)
     kProfGridDown = 16,   // + grid: that grid's way down (sweeps, residual, restriction; the coarsest solve)
     kProfGridUp = 80,     // + grid: that grid's way up (prolongation, scaling, sweeps)
     kProfCount = 144
@@ -185,8 +188,12 @@ void launchFaceFlux(Ctx& c, const Mesh& m, const LduView& A, const double* psi, 
 void launchInterpolate(Ctx& c, const Mesh& m, const double* w, const double* vol, double* face, double* bface);
 void launchDivSource(Ctx& c, const Mesh& m, const double* phi, const double* phib, const double* V, double sign,
                      double* source);
-void launchFluxCorrect(Ctx& c, const Mesh& m, const LduView& A, const double* psi, const double* phiHbyA, double sign,
-                       double* phi);
+void launchFluxCorrect(Ctx& c, const Mesh& m, const LduView& A, const double* psi, const double* ffc,
+                       const double* phiHbyA, double sign, double* phi);
+void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcValue, const double* Sf,
+                     const double* w, const double* V, const double* psi, double* grad);
+void launchLaplacianCorr(Ctx& c, const Mesh& m, const double* grad, const double* w, const double* corrVec,
+                         const double* gammaf, const double* magSf, double* ffc, double* source);
 
 // ---- geometry (kernels_geom.hip) ----
 void launchGeometry(Ctx& c, const Mesh& m, const double* points, const int* faceOffsets, const int* faceVerts,

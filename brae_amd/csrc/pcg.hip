@@ -310,7 +310,7 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         else
         {
             sumPsiK<<<gR, kReduceBlock, 0, c.stream>>>(n, psi, pa);
-            finishK<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, StoreTo{w.st.p + S_SUMPSI}, nullptr);
+            finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, StoreTo{w.st.p + S_SUMPSI}, nullptr);
         }
     }
     pcgInitK<<<gR, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, psi, b, w.r.p, d, normFactor ? 1 : 0, pa, pb,
@@ -319,7 +319,7 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     if (seq)
         seqFinish2K<<<1, 64, 0, c.stream>>>(n, sA, sB, initPost, nullptr);
     else
-        finish2K<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, pb, initPost, nullptr);
+        finish2K<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, pb, initPost, nullptr);
     hipCheck(hipGetLastError(), "pcg init");
     VcycleControls vc;
     vc.nPre = pc.nPre;
@@ -339,31 +339,33 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
             ProfScope ps(c, kProfPcgZr);
             pcgZrK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(n, z, w.r.p, w.rOld.p, pc.flexible ? 1 : 0, d, pa, pb, sA,
                                                          sB);
+        }
+        {
             const ZrPost post{d, pc.flexible ? 1 : 0};
             if (seq)
                 seqFinish2K<<<1, 64, 0, c.stream>>>(n, sA, sB, post, stopF);
             else
-                finish2K<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, pb, post, stopF);
+                finish2K<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, pb, post, stopF);
         }
         {
             ProfScope ps(c, kProfPcgSpmv64);
             pcgDirAmulK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, z, p[it & 1], p[(it + 1) & 1],
                                                               w.q.p, d, pa, sA);
-            if (seq)
-                seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, PqPost{d}, stopF);
-            else
-                finishK<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, PqPost{d}, stopF);
         }
+        if (seq)
+            seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, PqPost{d}, stopF);
+        else
+            finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, PqPost{d}, stopF);
         {
             ProfScope ps(c, kProfPcgUpdate);
             pcgUpdateK<Tv><<<gR, kReduceBlock, 0, c.stream>>>(n, p[(it + 1) & 1], w.q.p, psi, w.r.p, w.rOld.p,
                                                              pc.flexible ? 1 : 0, diag0, b0, x0, pc.nPre > 0 ? 1 : 0,
                                                              d, pa, sA);
-            if (seq)
-                seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, UpdPost{d}, stopF);
-            else
-                finishK<<<1, kReduceBlock, 0, c.stream>>>(gR, pa, UpdPost{d}, stopF);
         }
+        if (seq)
+            seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, UpdPost{d}, stopF);
+        else
+            finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, UpdPost{d}, stopF);
         hipCheck(hipGetLastError(), "pcg iteration");
         if ((it + 1) % std::max(1, pc.checkEvery) == 0 || it + 1 == pc.maxIter)
         {

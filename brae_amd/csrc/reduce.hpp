@@ -47,7 +47,8 @@ __device__ __forceinline__ double waveSum(double v)
     return v;
 }
 
-// Sum over the workgroup (kReduceBlock threads); the result is valid in thread 0.
+// Sum over a workgroup of N threads; the result is valid in thread 0.
+template <int N = kReduceBlock>
 __device__ __forceinline__ double blockSum(double v, double* lds)
 {
     v = waveSum(v);
@@ -58,7 +59,7 @@ __device__ __forceinline__ double blockSum(double v, double* lds)
     if (threadIdx.x == 0)
     {
 #pragma unroll
-        for (int w = 0; w < kReduceBlock / 64; ++w) s += lds[w];
+        for (int w = 0; w < N / 64; ++w) s += lds[w];
     }
     __syncthreads();
     return s;
@@ -70,34 +71,43 @@ __device__ __forceinline__ void storePartial(double blockTotal, double* partials
     if (threadIdx.x == 0) partials[blockIdx.x] = blockTotal;
 }
 
+// One finishing thread's share of nParts partials (i = tid, tid + kFinishBlock, ...), with four loads in
+// flight: four running sums in a fixed rotation, combined in a fixed order.
+__device__ __forceinline__ double finishShare(int nParts, const double* __restrict__ pa)
+{
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * kFinishBlock < nParts; i += 4 * kFinishBlock)
+    {
+        a0 += pa[i];
+        a1 += pa[i + kFinishBlock];
+        a2 += pa[i + 2 * kFinishBlock];
+        a3 += pa[i + 3 * kFinishBlock];
+    }
+    for (; i < nParts; i += kFinishBlock) a0 += pa[i];
+    return (a0 + a1) + (a2 + a3);
+}
+
 // Stage two: one workgroup sums nParts partials in a fixed order and hands the total to `post`.
 template <typename Post>
-__global__ __launch_bounds__(kReduceBlock) void finishK(int nParts, const double* __restrict__ pa, Post post,
+__global__ __launch_bounds__(kFinishBlock) void finishK(int nParts, const double* __restrict__ pa, Post post,
                                                         const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    __shared__ double lds[kReduceBlock / 64];
-    double v = 0.0;
-    for (int i = threadIdx.x; i < nParts; i += kReduceBlock) v += pa[i];
-    const double s = blockSum(v, lds);
+    __shared__ double lds[kFinishBlock / 64];
+    const double s = blockSum<kFinishBlock>(finishShare(nParts, pa), lds);
     if (threadIdx.x == 0) post(s);
 }
 
 template <typename Post>
-__global__ __launch_bounds__(kReduceBlock) void finish2K(int nParts, const double* __restrict__ pa,
+__global__ __launch_bounds__(kFinishBlock) void finish2K(int nParts, const double* __restrict__ pa,
                                                          const double* __restrict__ pb, Post post,
                                                          const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    __shared__ double lds[kReduceBlock / 64];
-    double va = 0.0, vb = 0.0;
-    for (int i = threadIdx.x; i < nParts; i += kReduceBlock)
-    {
-        va += pa[i];
-        vb += pb[i];
-    }
-    const double sa = blockSum(va, lds);
-    const double sb = blockSum(vb, lds);
+    __shared__ double lds[kFinishBlock / 64];
+    const double sa = blockSum<kFinishBlock>(finishShare(nParts, pa), lds);
+    const double sb = blockSum<kFinishBlock>(finishShare(nParts, pb), lds);
     if (threadIdx.x == 0) post(sa, sb);
 }
 

@@ -38,71 +38,64 @@ inline SellView viewOf(const Sell& s)
 }
 
 // sum_k A'_ck * X(col_k) over row c's entries in face order, in the vector precision. X is a functor
-// returning the operand at a column (plain x, or x + prolonged coarse correction). Rows of up to
-// kRowRegs entries load every coefficient and column first and then gather, so a row's loads are all in
-// flight together; the sum is still taken in entry order.
-constexpr int kRowRegs = 8;
+// returning the operand at a column (plain x, x + prolonged correction, a direction vector built on the
+// fly). Rows are handed out with each wavefront's 64 lanes on one slice (row c in lane c & 63), so the
+// slice's start and width are wave-uniform: they are read once and kept in scalar registers, and the
+// width selects straight-line code that issues every coefficient (and column) load of a group of up to
+// eight entries before the gathers, then sums in entry order. Padded slots hold a zero coefficient and
+// the row's own column.
+template <int W, bool Pat, typename Tm, typename Tv, typename XF>
+__device__ __forceinline__ Tv rowGroup(const SellView& S, int c, int lane, int base, const int* off,
+                                       const Tm* __restrict__ val, XF X, Tv acc)
+{
+    Tv a[W];
+    int col[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k)
+    {
+        const int e = base + (k << 6) + lane;
+        a[k] = cvt<Tv>(val[e]);
+        col[k] = Pat ? c + off[k] : S.cols[e];
+    }
+    Tv xv[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) xv[k] = X(col[k]);
+#pragma unroll
+    for (int k = 0; k < W; ++k) acc += a[k] * xv[k];
+    return acc;
+}
+
+template <bool Pat, typename Tm, typename Tv, typename XF>
+__device__ __forceinline__ Tv rowEntries(const SellView& S, int c, int lane, int p0, int w, const int* off,
+                                         const Tm* __restrict__ val, XF X, Tv acc)
+{
+    int k0 = 0;
+    for (; k0 + 8 <= w; k0 += 8)
+        acc = rowGroup<8, Pat, Tm, Tv>(S, c, lane, p0 + (k0 << 6), Pat ? off + k0 : off, val, X, acc);
+    const int* o = Pat ? off + k0 : off;
+    const int base = p0 + (k0 << 6);
+    switch (w - k0)
+    {
+    case 1: return rowGroup<1, Pat, Tm, Tv>(S, c, lane, base, o, val, X, acc);
+    case 2: return rowGroup<2, Pat, Tm, Tv>(S, c, lane, base, o, val, X, acc);
+    case 3: return rowGroup<3, Pat, Tm, Tv>(S, c, lane, base, o, val, X, acc);
+    case 4: return rowGroup<4, Pat, Tm, Tv>(S, c, lane, base, o, val, X, acc);
+    case 5: return rowGroup<5, Pat, Tm, Tv>(S, c, lane, base, o, val, X, acc);
+    case 6: return rowGroup<6, Pat, Tm, Tv>(S, c, lane, base, o, val, X, acc);
+    case 7: return rowGroup<7, Pat, Tm, Tv>(S, c, lane, base, o, val, X, acc);
+    default: return acc;
+    }
+}
 
 template <typename Tm, typename Tv, typename XF>
 __device__ __forceinline__ Tv rowOffSum(const SellView& S, int c, const Tm* __restrict__ val, XF X, Tv acc)
 {
-    const int s = c >> 6, lane = c & 63;
-    const int p0 = S.slicePtr[s];
-    const int w = (S.slicePtr[s + 1] - p0) >> 6;
-    if (w <= kRowRegs)
-    {
-        Tv a[kRowRegs];
-        int col[kRowRegs];
-        if (S.pattern)
-        {
-            const int* off = S.patOff + (int)S.pattern[c] * S.width;
-#pragma unroll
-            for (int k = 0; k < kRowRegs; ++k)
-                if (k < w)
-                {
-                    a[k] = cvt<Tv>(val[p0 + (k << 6) + lane]);
-                    col[k] = c + off[k];
-                }
-        }
-        else
-        {
-#pragma unroll
-            for (int k = 0; k < kRowRegs; ++k)
-                if (k < w)
-                {
-                    const int e = p0 + (k << 6) + lane;
-                    a[k] = cvt<Tv>(val[e]);
-                    col[k] = S.cols[e];
-                }
-        }
-        Tv xv[kRowRegs];
-#pragma unroll
-        for (int k = 0; k < kRowRegs; ++k)
-            if (k < w) xv[k] = X(col[k]);
-#pragma unroll
-        for (int k = 0; k < kRowRegs; ++k)
-            if (k < w) acc += a[k] * xv[k];
-        return acc;
-    }
+    const int s = __builtin_amdgcn_readfirstlane(c >> 6), lane = c & 63;
+    const int p0 = __builtin_amdgcn_readfirstlane(S.slicePtr[s]);
+    const int w = __builtin_amdgcn_readfirstlane((S.slicePtr[s + 1] - p0) >> 6);
     if (S.pattern)
-    {
-        const int* off = S.patOff + (int)S.pattern[c] * S.width;
-        for (int k = 0; k < w; ++k)
-        {
-            const Tv a = cvt<Tv>(val[p0 + (k << 6) + lane]);
-            acc += a * X(c + off[k]);
-        }
-    }
-    else
-    {
-        for (int k = 0; k < w; ++k)
-        {
-            const int e = p0 + (k << 6) + lane;
-            const Tv a = cvt<Tv>(val[e]);
-            acc += a * X(S.cols[e]);
-        }
-    }
-    return acc;
+        return rowEntries<true, Tm, Tv>(S, c, lane, p0, w, S.patOff + (int)S.pattern[c] * S.width, val, X, acc);
+    return rowEntries<false, Tm, Tv>(S, c, lane, p0, w, nullptr, val, X, acc);
 }
 
 // A x at row c: D x + sum (FP64/FP32: accumulated term by term after D x, the reference's order);
@@ -128,15 +121,6 @@ struct PlainX
 {
     const Tv* __restrict__ x;
     __device__ Tv operator()(int j) const { return x[j]; }
-};
-
-template <typename Tv>
-struct ProlongedX
-{
-    const Tv* __restrict__ x;
-    const int* __restrict__ map;
-    const Tv* __restrict__ xc;
-    __device__ Tv operator()(int j) const { return x[j] + xc[map[j]]; }
 };
 
 

@@ -42,8 +42,8 @@ class PcgControls(ctypes.Structure):
 
 
 AGG_BRAE, AGG_FACEAREAPAIR = 0, 1
-PROF_PCG_SPMV64, PROF_PCG_UPDATE, PROF_PCG_ZR, PROF_VCYCLE, PROF_AMG_RESIDUAL0, PROF_AMG_PROLONG_SMOOTH0, \
-    PROF_AMG_RESTRICT0 = range(7)
+PROF_PCG_SPMV64, PROF_PCG_UPDATE, PROF_PCG_ZR, PROF_VCYCLE, PROF_AMG_RESIDUAL0, PROF_AMG_CORR_SMOOTH0, \
+    PROF_AMG_RESTRICT0, PROF_AMG_CORR_AMUL0 = range(8)
 PREC_FP64, PREC_FP32, PREC_FP16 = 64, 32, 16
 
 
@@ -75,7 +75,9 @@ _SIGS = {
     "mfv_norm_factor": [P, P, P, P, P, P],
     "mfv_interpolate": [P, P, P, P, P, P],
     "mfv_div_source": [P, P, P, P, P, D, P],
-    "mfv_flux_correct": [P, P, P, P, P, D, P],
+    "mfv_flux_correct": [P, P, P, P, P, P, D, P],
+    "mfv_grad_gauss": [P, P, P, P, P, P, P, P, P],
+    "mfv_laplacian_corr": [P, P, P, P, P, P, P, P, P],
     "mfv_mesh_geometry": [P, P, P, P, P] + [P] * 10,
     "mfv_ctx_profile": [P, I],
     "mfv_ctx_profile_read": [P, I, P, P],
@@ -239,13 +241,13 @@ def ldu(diag, upper, lower=None):
 class Amg:
     """AMG hierarchy of a mesh (host agglomeration, device levels)."""
 
-    def __init__(self, mesh, weights, agglomeration=AGG_BRAE, nCellsInCoarsestLevel=0):
+    def __init__(self, mesh, weights, agglomeration=AGG_BRAE, nCellsInCoarsestLevel=0, mergeLevels=1):
         import numpy as np
         self.ctx = mesh.ctx
         self.mesh = mesh
         w = np.ascontiguousarray(weights, np.float64)
         assert w.size == mesh.nFaces
-        c = AmgControls(agglomeration, nCellsInCoarsestLevel, 1)
+        c = AmgControls(agglomeration, nCellsInCoarsestLevel, mergeLevels)
         self.h = P()
         self.ctx._check(lib().mfv_amg_build(self.ctx.h, mesh.h, ptr(w), ctypes.byref(c), ctypes.byref(self.h)),
                         "amg_build")

@@ -60,8 +60,11 @@ int mfv_ctx_sync(mfv_ctx ctx);
 int mfv_ctx_set_sequential_reduce(mfv_ctx ctx, int sequential);
 /* Kernel timing with HIP events recorded around the launches of each kernel class on the context's
  * stream (no counterpart in the reference, whose timing is BRAE_PHASE_TIME/BRAE_AMG_PCG_SPLIT by
- * draining the device). Tags: 0 PCG p-update + FP64 product, 1 PCG update, 2 PCG z.r, 3 one V-cycle,
- * 4 grid-0 residual, 5 grid-0 prolongation + smoothing, 6 grid 0 -> 1 restriction. */
+ * draining the device). Each tag below 16 times one kernel: 0 PCG p-update + FP64 product
+ * (pcgDirAmulK), 1 PCG update (pcgUpdateK), 2 PCG z.r (pcgZrK), 3 one whole V-cycle, 4 grid-0
+ * residual (residualK), 5 grid-0 correction update + first post-sweep (corrSmoothK / prolongSmoothK),
+ * 6 grid 0 -> 1 restriction (restrictK), 7 grid-0 scaled correction product (corrAmulK);
+ * 16 + g: grid g's way down; 80 + g: grid g's way up. */
 int mfv_ctx_profile(mfv_ctx ctx, int enable);
 /* launches recorded for `tag` since the last read and their summed duration; synchronises, resets */
 int mfv_ctx_profile_read(mfv_ctx ctx, int tag, int* count, double* totalMs);
@@ -116,10 +119,24 @@ int mfv_interpolate(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const do
  * div(phi) = (sum_own phi - sum_nei phi + sum_boundary phiB)/V (fvc.cu:706-730); dPhiB may be NULL */
 int mfv_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double* dPhiB, const double* dV,
                    double sign, double* dSource);
-/* correctFlux (pEqn_cpp.cuh:134-139): phi = phiHbyA - sign*(upper*psi[nei] - lower*psi[own]); sign -1
- * when A is the negated (positive-definite) pressure matrix */
-int mfv_flux_correct(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dPhiHbyA,
-                     double sign, double* dPhi);
+/* correctFlux (pEqn_cpp.cuh:134-139) with fvMatrix::flux's stored correction (fv_matrix_ops.cuh:33-50):
+ * phi = phiHbyA - sign*(upper*psi[nei] - lower*psi[own] + ffc); dFfc may be NULL (no non-orthogonal
+ * correction); sign -1 when A is the negated (positive-definite) pressure matrix */
+int mfv_flux_correct(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dFfc,
+                     const double* dPhiHbyA, double sign, double* dPhi);
+/* fvc::gaussGrad of a cell field with its patch values (src/finiteVolume/finiteVolume/fvc.cu:67-121):
+ * dSf xyz per face (internal then boundary, as mfv_mesh_geometry writes it), dWeights per internal face,
+ * patch values from dBcType/dBcValue (fixedValue: the value; zeroGradient: the cell's; empty: none);
+ * dGrad xyz per cell */
+int mfv_grad_gauss(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue, const double* dSf,
+                   const double* dWeights, const double* dV, const double* dPsi, double* dGrad);
+/* The explicit non-orthogonal correction of fvm::laplacian (fvm::laplacianCorrFlux and
+ * laplacianNonOrthSource, src/finiteVolume/finiteVolume/fvm.cuh:170-222,276-310; pEqn_cpp.cu:223-235):
+ * dFfc = (gammaf*magSf) * (corrVec . interpolate(grad)) per internal face, and when dSource is not NULL
+ * dSource -= div(ffc) (face order per cell). Pass dFfc to mfv_flux_correct afterwards. */
+int mfv_laplacian_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dGrad, const double* dWeights,
+                       const double* dCorrVec, const double* dGammaf, const double* dMagSf, double* dFfc,
+                       double* dSource);
 
 /* FvGeometry::build (src/finiteVolume/fvMesh/fv_geometry.cu:13-158) and the patches' deltaCoeffs
  * (fv_patch.cu:160-165) on the device. Faces are polygons given by dFaceOffsets [nFaces+1] into
@@ -161,7 +178,8 @@ typedef struct
     int agglomeration;          /* MFV_AGG_* */
     int nCellsInCoarsestLevel;  /* brae: coarsen while n > this (its TARGET, default 64);
                                    faceAreaPair: nCellsInCoarsestLevel (default 10) */
-    int mergeLevels;            /* 1 (only value carried) */
+    int mergeLevels;            /* pairwise passes composed into one level (GAMG's mergeLevels, 1..4;
+                                   0 or 1: every pass is a level, the reference's only setting) */
 } mfv_amg_controls;
 
 /* buildAMG (src/matrices/lduMatrix/preconditioners/GAMGPreconditioner/device_amg.cuh:393,

@@ -370,6 +370,77 @@ void orc_face_flux(int nFaces, const int* owner, const int* neighbour, const dou
 }
 
 /* ------------------------------------------------------------------------------------------------
+ * Non-orthogonal correction. Gauss linear gradient of a cell field with its patch values
+ * (fvc.cu:67-121, gaussGrad of a GeometricField): internal faces in face order, face value
+ * w*(P - N) + N, +Sf*pf to the owner and -Sf*pf to the neighbour; then boundary faces in order (an
+ * empty patch contributes nothing; the patch value is the bc value for fixedValue, the cell value for
+ * zeroGradient); then divided by V. Sf is per face, internal then boundary, xyz triples.
+ * ---------------------------------------------------------------------------------------------- */
+void orc_grad_gauss(int nCells, int nFaces, const int* owner, const int* neighbour, int nBFaces,
+                    const int* bFaceCell, const int* bcType, const double* bcValue, const double* Sf,
+                    const double* weights, const double* V, const double* psi, double* grad)
+{
+    memset(grad, 0, sizeof(double) * 3 * (size_t)nCells);
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const int o = owner[f], n = neighbour[f];
+        const double pf = weights[f] * (psi[o] - psi[n]) + psi[n];
+        const vec3 s = v_scale(pf, v_get(Sf, f));
+        v_put(grad, o, v_add(v_get(grad, o), s));
+        v_put(grad, n, v_sub(v_get(grad, n), s));
+    }
+    for (int i = 0; i < nBFaces; ++i)
+    {
+        if (bcType[i] == ORC_BC_EMPTY) continue;
+        const int c = bFaceCell[i];
+        const double pv = bcType[i] == ORC_BC_FIXED_VALUE ? bcValue[i] : psi[c];
+        v_put(grad, c, v_add(v_get(grad, c), v_scale(pv, v_get(Sf, nFaces + i))));
+    }
+    for (int c = 0; c < nCells; ++c) v_put(grad, c, v_div(v_get(grad, c), V[c]));
+}
+
+/* The correction as a face flux (fvm.cuh:170-222, unlimited):
+ *   ffc_f = (gamma_f*magSf_f) * (cv_f . (w_f*grad[own] + (1 - w_f)*grad[nei]))
+ * and, when source is given, its divergence taken off the source in face order (fvm.cuh:276-299 and
+ * pEqn_cpp.cu:223-229): src[own] += ffc, src[nei] -= ffc, source -= src. */
+void orc_laplacian_corr(int nCells, int nFaces, const int* owner, const int* neighbour, const double* grad,
+                        const double* weights, const double* corrVec, const double* gammaf, const double* magSf,
+                        double* ffc, double* source)
+{
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const double w = weights[f];
+        const vec3 gf = v_add(v_scale(w, v_get(grad, owner[f])), v_scale(1.0 - w, v_get(grad, neighbour[f])));
+        const vec3 cv = v_get(corrVec, f);
+        const double corr = cv.x * gf.x + cv.y * gf.y + cv.z * gf.z;
+        ffc[f] = (gammaf[f] * magSf[f]) * corr;
+    }
+    if (!source) return;
+    double* src = (double*)xcalloc((size_t)nCells, sizeof(double));
+    for (int f = 0; f < nFaces; ++f)
+    {
+        src[owner[f]] += ffc[f];
+        src[neighbour[f]] -= ffc[f];
+    }
+    for (int c = 0; c < nCells; ++c) source[c] -= src[c];
+    free(src);
+}
+
+/* phi = phiHbyA - (upper*psi[nei] - lower*psi[own] + ffc) (fv_matrix_ops.cuh:33-50 with the stored
+ * faceFluxCorrection, then pEqn_cpp.cu:255-258); ffc may be NULL */
+void orc_flux_correct(int nFaces, const int* owner, const int* neighbour, const double* upper, const double* lower,
+                      const double* psi, const double* ffc, const double* phiHbyA, double* phi)
+{
+    if (!lower) lower = upper;
+    for (int f = 0; f < nFaces; ++f)
+    {
+        double fl = upper[f] * psi[neighbour[f]] - lower[f] * psi[owner[f]];
+        if (ffc) fl += ffc[f];
+        phi[f] = phiHbyA[f] - fl;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
  * PCG with DIC, pcg.cu:6-149 (OpenFOAM PCG + DICPreconditioner + normFactor). Folded system in.
  * ---------------------------------------------------------------------------------------------- */
 orc_perf orc_pcg_dic(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,

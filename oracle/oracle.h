@@ -81,6 +81,17 @@ double orc_sum_prod(int n, const double* a, const double* b);
 void orc_face_flux(int nFaces, const int* owner, const int* neighbour, const double* upper,
                    const double* lower, const double* psi, double* flux);
 
+/* ---- non-orthogonal correction: fvc.cu:67-121 (Gauss grad), fvm.cuh:170-222 + 276-299 (face flux and
+ *      its divergence off the source), fv_matrix_ops.cuh:33-50 + pEqn_cpp.cu:255-258 (phi) ---- */
+void orc_grad_gauss(int nCells, int nFaces, const int* owner, const int* neighbour, int nBFaces,
+                    const int* bFaceCell, const int* bcType, const double* bcValue, const double* Sf,
+                    const double* weights, const double* V, const double* psi, double* grad);
+void orc_laplacian_corr(int nCells, int nFaces, const int* owner, const int* neighbour, const double* grad,
+                        const double* weights, const double* corrVec, const double* gammaf, const double* magSf,
+                        double* ffc, double* source);
+void orc_flux_correct(int nFaces, const int* owner, const int* neighbour, const double* upper, const double* lower,
+                      const double* psi, const double* ffc, const double* phiHbyA, double* phi);
+
 typedef struct { double initialResidual, finalResidual; int nIterations; } orc_perf;
 
 /* ---- PCG with DIC on a folded symmetric system: src/OpenFOAM/matrices/pcg.cu:6-149 ---- */

@@ -55,6 +55,9 @@ def lib():
             "orc_sum_mag": (D, [I, P]),
             "orc_sum_prod": (D, [I, P, P]),
             "orc_face_flux": (None, [I, P, P, P, P, P, P]),
+            "orc_grad_gauss": (None, [I, I, P, P, I, P, P, P, P, P, P, P, P]),
+            "orc_laplacian_corr": (None, [I, I, P, P, P, P, P, P, P, P, P]),
+            "orc_flux_correct": (None, [I, P, P, P, P, P, P, P, P]),
             "orc_pcg_dic": (Perf, [I, I, P, P, P, P, P, P, D, D, I, I]),
             "orc_faceareapair_weights": (None, [I, P, P, P]),
             "orc_of_agglomeration": (P, [I, I, P, P, P, I, P]),
@@ -183,6 +186,30 @@ def fold(mesh, diag, source, bcType, iC, bC):
     lib().orc_fold(mesh.nCells, p(f64(diag)), p(f64(source)), mesh.nBFaces, p(mesh.bFaceCell), p(i32(bcType)),
                    p(f64(iC)), p(f64(bC)), p(diagC), p(b))
     return diagC, b
+
+
+def grad_gauss(nC, own, nei, bFaceCell, bcType, bcValue, Sf, weights, V, psi):
+    """Gauss linear gradient (nC x 3); Sf per face, internal then boundary"""
+    g = np.zeros((nC, 3))
+    lib().orc_grad_gauss(nC, len(own), p(i32(own)), p(i32(nei)), len(bFaceCell), p(i32(bFaceCell)), p(i32(bcType)),
+                         p(f64(bcValue)), p(f64(Sf)), p(f64(weights)), p(f64(V)), p(f64(psi)), p(g))
+    return g
+
+
+def laplacian_corr(nC, own, nei, grad, weights, corrVec, gammaf, magSf, source=None):
+    """non-orthogonal correction flux; with a source, also source -= div(ffc) (in place, returned)"""
+    ffc = np.zeros(len(own))
+    src = None if source is None else f64(source).copy()
+    lib().orc_laplacian_corr(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(grad)), p(f64(weights)), p(f64(corrVec)),
+                             p(f64(gammaf)), p(f64(magSf)), p(ffc), p(src))
+    return ffc, src
+
+
+def flux_correct(own, nei, upper, lower, psi, ffc, phiHbyA):
+    phi = np.zeros(len(own))
+    lib().orc_flux_correct(len(own), p(i32(own)), p(i32(nei)), p(f64(upper)), p(None if lower is None else f64(lower)),
+                           p(f64(psi)), p(None if ffc is None else f64(ffc)), p(f64(phiHbyA)), p(phi))
+    return phi
 
 
 def amul(nC, own, nei, diag, upper, lower, psi):

@@ -240,3 +240,42 @@ def test_case20_trace_fp32():
         assert perf.nIterations == k
         ref = cyc[cyc[:, 0] == k][:, 2]
         assert np.abs(G.host(psi) - ref).max() < 2e-6, k
+
+
+def compose_levels(L1, L2):
+    """two pairwise levels as one: cells through both maps; a face onto the second level's coarse
+    face, or inside a coarse cell (-(cell+1)) when either level swallowed it"""
+    a = L1["faceRestrict"]
+    fr = np.where(a >= 0, L2["faceRestrict"][np.maximum(a, 0)], -(L2["map"][np.maximum(-a - 1, 0)] + 1))
+    return dict(map=L2["map"][L1["map"]], cOwn=L2["cOwn"], cNei=L2["cNei"], faceRestrict=fr.astype(np.int32))
+
+
+def test_merged_levels(systems):
+    """mergeLevels 2: each level is two pairwise passes composed; the Galerkin values of a composed level
+    equal the two-step ones up to summation order; the PCG converges to the oracle's solution."""
+    for S in systems:
+        print("case:", S.label, flush=True)
+        ag = O.brae_agglomeration(S.nC, S.own, S.nei, S.magSf, target=64)
+        amg = mfv.Amg(S.case.mesh, S.magSf, mfv.AGG_BRAE, 64, mergeLevels=2)
+        assert amg.nGrids - 1 == (ag.nLevels + 1) // 2
+        for k in range(amg.nGrids - 1):
+            want = ag.levels[2 * k] if 2 * k + 1 >= ag.nLevels else compose_levels(ag.levels[2 * k],
+                                                                                     ag.levels[2 * k + 1])
+            got = amg.level(k)
+            for key in ("map", "cOwn", "cNei", "faceRestrict"):
+                assert np.array_equal(got[key], want[key]), (S.label, k, key)
+        amg.galerkin(S.dd, S.du, mfv.PREC_FP64)
+        ref = ag.galerkin(S.diag, S.upper, order="brae")
+        for k in range(1, amg.nGrids):
+            cd, cu = ref[min(2 * k, ag.nLevels) - 1]
+            d, u = amg.grid_values(k)
+            assert np.abs(d - cd).max() <= 1e-12 * np.abs(cd).max(), (S.label, k)
+            assert np.abs(u - cu).max() <= 1e-12 * np.abs(cu).max(), (S.label, k)
+        psi = G.zeros(S.nC)
+        perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-10, precision=mfv.PREC_FP64, flexible=1,
+                         scaleCorrection=1)
+        ref_psi, _ = O.brae_amgpcg(ag, S.diag, S.upper, S.b, np.zeros(S.nC), 1e-10, flexible=True, scale=True)
+        assert perf.converged and perf.finalResidual < 1e-10
+        x = G.host(psi)
+        assert np.abs(x - ref_psi).max() <= 1e-7 * np.abs(ref_psi).max(), S.label
+        amg.close()
