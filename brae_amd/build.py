@@ -46,6 +46,24 @@ def build(verbose=False):
     return LIB
 
 
+def build_cpp_callers(verbose=False):
+    """The C++ callers under tests/cpp (brae's call sites through include/brae_mfv.hpp), linked against
+    the in-tree libmfv.so with an $ORIGIN-relative rpath so they run wherever the tree is copied."""
+    out = []
+    for src in sorted(glob.glob(os.path.join(ROOT, "tests", "cpp", "*.cpp"))):
+        exe = src[:-4] + ".bin"
+        cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
+               "-L", PKG, "-lmfv", "-Wl,-rpath,$ORIGIN/../../brae_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+        out.append(exe)
+        if verbose:
+            print("built", exe)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_callers(verbose=True)
     sys.exit(0)

@@ -689,8 +689,10 @@ hipEvent_t takeEvent(Ctx& c)
         c.eventPool.pop_back();
         return e;
     }
+    // device-scope release: recording the event does not write the L2 back to memory, so the timed
+    // window is the kernels' own
     hipEvent_t e;
-    hipCheck(hipEventCreate(&e), "event create");
+    hipCheck(hipEventCreateWithFlags(&e, hipEventReleaseToDevice), "event create");
     return e;
 }
 } // namespace

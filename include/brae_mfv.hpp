@@ -1,0 +1,471 @@
+/*
+ * brae_mfv.hpp -- header-only C++ layer over the mfv C ABI (mfv.h) that re-creates the signatures
+ * brae's callers use for the pressure assembly and solve path, in namespace brae, so a call site such
+ * as the SIMPLE pressure block (src/applications/solvers/simpleFoam/simpleFoam.cu:343-381) compiles
+ * against it unchanged apart from the include. Host code only; link with libmfv.so.
+ *
+ * Conventions kept from the reference:
+ *   - DeviceBuffer<T> is RAII device memory; callees resize the caller's outputs
+ *     (device_fvm.cu:313 `upper.resize(nIf)`), views are borrowed (src/cuda/device_buffer.cuh:716-760);
+ *   - errors throw std::runtime_error("brae hip: ...") as cudaCheck does (device_buffer.cuh:25-28);
+ *     a solve that does not converge is not an error (nIterations == maxIter);
+ *   - one solve at a time per host thread, each host thread on its own stream (the reference runs on
+ *     cudaStreamPerThread): threadContext() gives every thread its own mfv context;
+ *   - a symmetric matrix passes lower == upper (device_amg.cu:1703-1710).
+ * Deliberate differences: vector fields (gradients, Sf) are xyz-interleaved arrays of 3*n doubles; the
+ * smoothed-aggregation switch of buildAMG and the V-cycle graph capture are refused / ignored by name.
+ */
+#pragma once
+
+#include "mfv.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace brae {
+
+using scalar = double;
+using label = int;
+
+inline void mfvCheck(int rc, mfv_ctx ctx, const char* what)
+{
+    if (rc != MFV_OK)
+        throw std::runtime_error(std::string("brae hip: ") + what + ": " + mfv_last_error(ctx));
+}
+
+inline void hipCheckOrThrow(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string("brae hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+// The calling thread's context: its own stream and reduction scratch, created on first use on the
+// current device and released when the thread ends.
+inline mfv_ctx threadContext()
+{
+    struct Holder
+    {
+        mfv_ctx c = nullptr;
+        ~Holder()
+        {
+            if (c) mfv_ctx_destroy(c);
+        }
+    };
+    thread_local Holder h;
+    if (!h.c)
+    {
+        int dev = 0;
+        hipCheckOrThrow(hipGetDevice(&dev), "hipGetDevice");
+        mfvCheck(mfv_ctx_create(dev, nullptr, &h.c), nullptr, "context");
+    }
+    return h.c;
+}
+
+inline hipStream_t threadStream() { return static_cast<hipStream_t>(mfv_ctx_stream(threadContext())); }
+
+// RAII device array (DeviceBuffer, src/cuda/device_buffer.cuh:716-760)
+template <typename T>
+class DeviceBuffer
+{
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(std::size_t n) { resize(n); }
+    explicit DeviceBuffer(const std::vector<T>& h) { upload(h); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        if (this != &o)
+        {
+            release();
+            p_ = o.p_;
+            n_ = o.n_;
+            o.p_ = nullptr;
+            o.n_ = 0;
+        }
+        return *this;
+    }
+    ~DeviceBuffer() { release(); }
+
+    void resize(std::size_t n)
+    {
+        if (n == n_) return;
+        release();
+        if (n) hipCheckOrThrow(hipMalloc(&p_, n * sizeof(T)), "hipMalloc");
+        n_ = n;
+    }
+    void upload(const std::vector<T>& h)
+    {
+        resize(h.size());
+        if (n_)
+        {
+            hipCheckOrThrow(hipMemcpyAsync(p_, h.data(), n_ * sizeof(T), hipMemcpyHostToDevice, threadStream()),
+                            "upload");
+            hipCheckOrThrow(hipStreamSynchronize(threadStream()), "upload sync");
+        }
+    }
+    std::vector<T> download() const
+    {
+        std::vector<T> h(n_);
+        if (n_)
+        {
+            hipCheckOrThrow(hipMemcpyAsync(h.data(), p_, n_ * sizeof(T), hipMemcpyDeviceToHost, threadStream()),
+                            "download");
+            hipCheckOrThrow(hipStreamSynchronize(threadStream()), "download sync");
+        }
+        return h;
+    }
+    void zero()
+    {
+        if (n_) hipCheckOrThrow(hipMemsetAsync(p_, 0, n_ * sizeof(T), threadStream()), "zero");
+    }
+    T* data() { return p_; }
+    const T* data() const { return p_; }
+    std::size_t size() const { return n_; }
+
+private:
+    void release()
+    {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+// The mesh as the path sees it (DeviceMesh, src/cuda/device_mesh.cuh): the LDU addressing handle and
+// the geometry the assembly reads. Built from host addressing and points+faces (geometry on the device).
+struct DeviceMesh
+{
+    mfv_mesh mesh = nullptr;
+    int nCells = 0, nInternalFaces = 0, nBoundaryFaces = 0;
+    DeviceBuffer<scalar> Cf, Sf, magSf, C, V;               // per face (internal then boundary) / per cell
+    DeviceBuffer<scalar> weights, deltaCoeffs, nonOrthDeltaCoeffs, corrVec;   // per internal face
+    DeviceBuffer<scalar> bDeltaCoeffs;                        // per boundary face
+    DeviceBuffer<scalar> magSfInternal, magSfBoundary;
+
+    DeviceMesh() = default;
+    DeviceMesh(const DeviceMesh&) = delete;
+    DeviceMesh& operator=(const DeviceMesh&) = delete;
+    ~DeviceMesh()
+    {
+        if (mesh) mfv_mesh_destroy(mesh);
+    }
+
+    // owner: every face (internal faces first, upper-triangular; then boundary faces by patch);
+    // neighbour: internal faces; faces: polygons as offsets [nFaces+1] into vertex lists; points xyz.
+    void build(int nC, const std::vector<label>& owner, const std::vector<label>& neighbour,
+               const std::vector<scalar>& points, const std::vector<label>& faceOffsets,
+               const std::vector<label>& faceVerts)
+    {
+        mfv_ctx ctx = threadContext();
+        nCells = nC;
+        nInternalFaces = static_cast<int>(neighbour.size());
+        nBoundaryFaces = static_cast<int>(owner.size()) - nInternalFaces;
+        const std::vector<label> bfc(owner.begin() + nInternalFaces, owner.end());
+        mfvCheck(mfv_mesh_create(ctx, nCells, nInternalFaces, owner.data(), neighbour.data(), nBoundaryFaces,
+                                 bfc.data(), &mesh),
+                 ctx, "mesh");
+        const std::size_t nF = owner.size();
+        DeviceBuffer<scalar> dPts(points);
+        DeviceBuffer<label> dOff(faceOffsets), dVerts(faceVerts);
+        Cf.resize(3 * nF);
+        Sf.resize(3 * nF);
+        magSf.resize(nF);
+        C.resize(3 * (std::size_t)nCells);
+        V.resize(nCells);
+        weights.resize(nInternalFaces);
+        deltaCoeffs.resize(nInternalFaces);
+        nonOrthDeltaCoeffs.resize(nInternalFaces);
+        corrVec.resize(3 * (std::size_t)nInternalFaces);
+        bDeltaCoeffs.resize(nBoundaryFaces);
+        mfvCheck(mfv_mesh_geometry(ctx, mesh, dPts.data(), dOff.data(), dVerts.data(), Cf.data(), Sf.data(),
+                                   magSf.data(), C.data(), V.data(), weights.data(), deltaCoeffs.data(),
+                                   nonOrthDeltaCoeffs.data(), corrVec.data(), bDeltaCoeffs.data()),
+                 ctx, "geometry");
+        magSfInternal.resize(nInternalFaces);
+        magSfBoundary.resize(nBoundaryFaces);
+        if (nInternalFaces)
+            hipCheckOrThrow(hipMemcpyAsync(magSfInternal.data(), magSf.data(), nInternalFaces * sizeof(scalar),
+                                           hipMemcpyDeviceToDevice, threadStream()), "magSf");
+        if (nBoundaryFaces)
+            hipCheckOrThrow(hipMemcpyAsync(magSfBoundary.data(), magSf.data() + nInternalFaces,
+                                           nBoundaryFaces * sizeof(scalar), hipMemcpyDeviceToDevice, threadStream()),
+                            "magSf");
+        mfvCheck(mfv_ctx_sync(ctx), ctx, "mesh sync");
+    }
+};
+
+// The value half of DeviceLduView (src/matrices/lduMatrix/lduMatrix/device_ldu.cuh:20-66) plus the
+// mesh it lives on.
+struct DeviceLduView
+{
+    int nCells = 0, nInternalFaces = 0;
+    const scalar* diag = nullptr;
+    const scalar* upper = nullptr;
+    const scalar* lower = nullptr;
+    mfv_mesh mesh = nullptr;
+    mfv_ldu ldu() const { return mfv_ldu{diag, upper, lower}; }
+};
+
+inline DeviceLduView lduView(const DeviceMesh& dm, const DeviceBuffer<scalar>& diag,
+                             const DeviceBuffer<scalar>& upper, const DeviceBuffer<scalar>* lower = nullptr)
+{
+    return DeviceLduView{dm.nCells, dm.nInternalFaces, diag.data(), upper.data(),
+                         lower ? lower->data() : upper.data(), dm.mesh};
+}
+
+struct DeviceSolverPerf
+{
+    scalar initialResidual = 0, finalResidual = 0;
+    int nIterations = 0;
+};
+
+// ---------------- assembly (src/cuda/device_mesh.cuh:495-612, device_fvm.cu) ----------------
+
+// deviceLaplacianCoeffs (device_fvm.cu:304): upper = lower = dc*gammaf*|Sf|, diag = -sum in face
+// order; nonOrth selects nonOrthDeltaCoeffs
+inline void deviceLaplacianCoeffs(const DeviceMesh& dm, const DeviceBuffer<scalar>& gammafInt,
+                                  DeviceBuffer<scalar>& diag, DeviceBuffer<scalar>& upper,
+                                  DeviceBuffer<scalar>& lower, bool nonOrth = false)
+{
+    mfv_ctx ctx = threadContext();
+    diag.resize(dm.nCells);
+    upper.resize(dm.nInternalFaces);
+    lower.resize(dm.nInternalFaces);
+    const DeviceBuffer<scalar>& dc = nonOrth ? dm.nonOrthDeltaCoeffs : dm.deltaCoeffs;
+    mfvCheck(mfv_laplacian_coeffs(ctx, dm.mesh, dc.data(), gammafInt.data(), dm.magSfInternal.data(), diag.data(),
+                                  upper.data(), lower.data()),
+             ctx, "deviceLaplacianCoeffs");
+}
+
+// deviceLaplacianCoeffsSymmetric (device_fvm.cu:349): the off-diagonal held once
+inline void deviceLaplacianCoeffsSymmetric(const DeviceMesh& dm, const DeviceBuffer<scalar>& gammafInt,
+                                           DeviceBuffer<scalar>& diag, DeviceBuffer<scalar>& upper,
+                                           bool nonOrth = false)
+{
+    mfv_ctx ctx = threadContext();
+    diag.resize(dm.nCells);
+    upper.resize(dm.nInternalFaces);
+    const DeviceBuffer<scalar>& dc = nonOrth ? dm.nonOrthDeltaCoeffs : dm.deltaCoeffs;
+    mfvCheck(mfv_laplacian_coeffs(ctx, dm.mesh, dc.data(), gammafInt.data(), dm.magSfInternal.data(), diag.data(),
+                                  upper.data(), nullptr),
+             ctx, "deviceLaplacianCoeffsSymmetric");
+}
+
+// deviceDivUpwindCoeffs (device_fvm.cu:646-662)
+inline void deviceDivUpwindCoeffs(const DeviceMesh& dm, const DeviceBuffer<scalar>& phiInt,
+                                  DeviceBuffer<scalar>& diag, DeviceBuffer<scalar>& upper,
+                                  DeviceBuffer<scalar>& lower)
+{
+    mfv_ctx ctx = threadContext();
+    diag.resize(dm.nCells);
+    upper.resize(dm.nInternalFaces);
+    lower.resize(dm.nInternalFaces);
+    mfvCheck(mfv_div_upwind_coeffs(ctx, dm.mesh, phiInt.data(), diag.data(), upper.data(), lower.data()), ctx,
+             "deviceDivUpwindCoeffs");
+}
+
+// deviceFold (device_fvm.cu:895-912): diagC = rawDiag + iC, b = source + bC (flattened boundary faces)
+inline void deviceFold(const DeviceMesh& dm, const DeviceBuffer<scalar>& rawDiag, const DeviceBuffer<scalar>& source,
+                       const DeviceBuffer<scalar>& iC, const DeviceBuffer<scalar>& bC, DeviceBuffer<scalar>& diagC,
+                       DeviceBuffer<scalar>& b)
+{
+    mfv_ctx ctx = threadContext();
+    diagC.resize(dm.nCells);
+    b.resize(dm.nCells);
+    mfvCheck(mfv_fold(ctx, dm.mesh, rawDiag.data(), source.data(), iC.data(), bC.data(), diagC.data(), b.data()),
+             ctx, "deviceFold");
+}
+
+// deviceSetReference (src/applications/solvers/simpleFoam/device_simple.cuh:104-106)
+inline void deviceSetReference(DeviceBuffer<scalar>& diag, DeviceBuffer<scalar>& b, label refCell, scalar refValue)
+{
+    mfv_ctx ctx = threadContext();
+    mfvCheck(mfv_set_reference(ctx, diag.data(), b.data(), refCell, refValue), ctx, "deviceSetReference");
+}
+
+// deviceMatrixFluxInternal (device_simple.cuh:51-52): flux = upper*p[nei] - lower*p[own]
+inline void deviceMatrixFluxInternal(const DeviceLduView& A, const DeviceBuffer<scalar>& p,
+                                     DeviceBuffer<scalar>& fluxInt)
+{
+    mfv_ctx ctx = threadContext();
+    fluxInt.resize(A.nInternalFaces);
+    const mfv_ldu l = A.ldu();
+    mfvCheck(mfv_face_flux(ctx, A.mesh, &l, p.data(), fluxInt.data()), ctx, "deviceMatrixFluxInternal");
+}
+
+// fvc::gaussGrad with the patch values of a field whose patches are fixedValue (bcType 1, value
+// bcValue) or zeroGradient (0) or empty (2); grad is xyz-interleaved (3*nCells)
+inline void deviceGaussGrad(const DeviceMesh& dm, const DeviceBuffer<scalar>& vol, const DeviceBuffer<int>& bcType,
+                            const DeviceBuffer<scalar>& bcValue, DeviceBuffer<scalar>& grad)
+{
+    mfv_ctx ctx = threadContext();
+    grad.resize(3 * (std::size_t)dm.nCells);
+    mfvCheck(mfv_grad_gauss(ctx, dm.mesh, bcType.data(), bcValue.data(), dm.Sf.data(), dm.weights.data(),
+                            dm.V.data(), vol.data(), grad.data()),
+             ctx, "deviceGaussGrad");
+}
+
+// deviceLaplacianCorrFlux + deviceFaceDivSource (device_mesh.cuh:536-552): ffc per internal face and,
+// when src is given, src -= div(ffc); grad xyz-interleaved
+inline void deviceLaplacianCorrFlux(const DeviceMesh& dm, const DeviceBuffer<scalar>& gammafInt,
+                                    const DeviceBuffer<scalar>& grad, DeviceBuffer<scalar>& ffc,
+                                    DeviceBuffer<scalar>* src = nullptr)
+{
+    mfv_ctx ctx = threadContext();
+    ffc.resize(dm.nInternalFaces);
+    mfvCheck(mfv_laplacian_corr(ctx, dm.mesh, grad.data(), dm.weights.data(), dm.corrVec.data(), gammafInt.data(),
+                                dm.magSfInternal.data(), ffc.data(), src ? src->data() : nullptr),
+             ctx, "deviceLaplacianCorrFlux");
+}
+
+// ---------------- products and norms ----------------
+
+// deviceAmul (device_ldu.cuh:241); onField has no effect without cyclic jumps, which the path refuses
+inline void deviceAmul(const DeviceLduView& A, const DeviceBuffer<scalar>& psi, DeviceBuffer<scalar>& Apsi,
+                       bool onField = false)
+{
+    (void)onField;
+    mfv_ctx ctx = threadContext();
+    Apsi.resize(A.nCells);
+    const mfv_ldu l = A.ldu();
+    mfvCheck(mfv_amul(ctx, A.mesh, &l, psi.data(), Apsi.data()), ctx, "deviceAmul");
+}
+
+// deviceNormFactorInto (device_pcg.cuh:44); `ones` is not needed (sumA is formed in place)
+inline void deviceNormFactorInto(const DeviceLduView& A, const DeviceBuffer<scalar>& psi,
+                                 const DeviceBuffer<scalar>& b, const DeviceBuffer<scalar>& ones,
+                                 DeviceBuffer<scalar>& dNf)
+{
+    (void)ones;
+    mfv_ctx ctx = threadContext();
+    dNf.resize(1);
+    const mfv_ldu l = A.ldu();
+    mfvCheck(mfv_norm_factor(ctx, A.mesh, &l, psi.data(), b.data(), dNf.data()), ctx, "deviceNormFactorInto");
+}
+
+// ---------------- AMG hierarchy and solve ----------------
+
+// AMGData (device_amg.cuh:308-388): the hierarchy, its work vectors, the V-cycle precision. Movable.
+struct AMGData
+{
+    mfv_amg amg = nullptr;
+    mfv_mesh addressing = nullptr;   // the fine addressing the hierarchy was built on (owned)
+    int nFine = 0;
+    int precision = MFV_PREC_FP32;   // the reference's FP32 V-cycle
+    int nPreSweeps = 1, nPostSweeps = 1;
+
+    AMGData() = default;
+    AMGData(const AMGData&) = delete;
+    AMGData& operator=(const AMGData&) = delete;
+    AMGData(AMGData&& o) noexcept { *this = std::move(o); }
+    AMGData& operator=(AMGData&& o) noexcept
+    {
+        if (this != &o)
+        {
+            reset();
+            amg = o.amg;
+            addressing = o.addressing;
+            nFine = o.nFine;
+            precision = o.precision;
+            nPreSweeps = o.nPreSweeps;
+            nPostSweeps = o.nPostSweeps;
+            o.amg = nullptr;
+            o.addressing = nullptr;
+        }
+        return *this;
+    }
+    ~AMGData() { reset(); }
+    int nLevels() const
+    {
+        int n = 0;
+        if (amg) mfv_amg_num_grids(amg, &n);
+        return n > 0 ? n - 1 : 0;
+    }
+
+private:
+    void reset()
+    {
+        if (amg) mfv_amg_destroy(amg);
+        if (addressing) mfv_mesh_destroy(addressing);
+        amg = nullptr;
+        addressing = nullptr;
+    }
+};
+
+// buildAMG (device_amg.cuh:393): brae's pairwise hierarchy from the fine internal addressing and face
+// weights (|Sf|); smoothed aggregation is not carried and is refused
+inline AMGData buildAMG(const std::vector<label>& fineOwner, const std::vector<label>& fineNei,
+                        const std::vector<scalar>& faceWeights, int nFine, const bool* smoothedAggregation = nullptr)
+{
+    if (smoothedAggregation && *smoothedAggregation)
+        throw std::runtime_error("brae hip: buildAMG: smoothed aggregation is not carried");
+    mfv_ctx ctx = threadContext();
+    AMGData a;
+    a.nFine = nFine;
+    mfvCheck(mfv_mesh_create(ctx, nFine, static_cast<int>(fineOwner.size()), fineOwner.data(), fineNei.data(), 0,
+                             nullptr, &a.addressing),
+             ctx, "buildAMG addressing");
+    mfv_amg_controls c{MFV_AGG_BRAE, 64, 1};
+    mfvCheck(mfv_amg_build(ctx, a.addressing, faceWeights.data(), &c, &a.amg), ctx, "buildAMG");
+    return a;
+}
+
+// amgGalerkin (device_amg.cuh:454): symmetric matrices only (amgRefuseAsymmetric)
+inline void amgGalerkin(AMGData& A, const DeviceBuffer<scalar>& fineDiag, const DeviceBuffer<scalar>& fineUpper,
+                        const DeviceBuffer<scalar>& fineLower)
+{
+    mfv_ctx ctx = threadContext();
+    mfvCheck(mfv_amg_galerkin(ctx, A.amg, fineDiag.data(), fineUpper.data(), fineLower.data(), A.precision), ctx,
+             "amgGalerkin");
+}
+
+// deviceAMGPCG (device_amg.cuh:550-553) with the normFactor on the device (NULL: computed here);
+// captureVcycle is accepted and ignored (kernels are enqueued ahead on the stream)
+inline DeviceSolverPerf deviceAMGPCG(const DeviceLduView& Afine, AMGData& amg, const DeviceBuffer<scalar>& b,
+                                     DeviceBuffer<scalar>& psi, const scalar* dNormFactor, scalar tol, scalar relTol,
+                                     int maxIter, bool captureVcycle = false, int checkEvery = 1,
+                                     bool corrScaling = false, int minIter = 0)
+{
+    (void)captureVcycle;
+    mfv_ctx ctx = threadContext();
+    mfv_pcg_controls c;
+    mfv_pcg_default_controls(&c);
+    c.tolerance = tol;
+    c.relTol = relTol;
+    c.maxIter = maxIter;
+    c.minIter = minIter;
+    c.precision = amg.precision;
+    c.nPreSweeps = amg.nPreSweeps;
+    c.nPostSweeps = amg.nPostSweeps;
+    c.checkEvery = checkEvery;
+    c.scaleCorrection = corrScaling ? 1 : 0;
+    c.flexible = corrScaling ? 1 : 0;   // a scaled cycle is nonlinear: flexible CG, as the reference switches
+    if (Afine.nCells != amg.nFine) throw std::runtime_error("brae hip: deviceAMGPCG: matrix and hierarchy sizes differ");
+    mfv_perf perf{};
+    const mfv_ldu l = Afine.ldu();
+    // the hierarchy's own copy of the fine addressing (the same internal faces as Afine's mesh)
+    mfvCheck(mfv_amg_pcg_solve(ctx, amg.addressing, amg.amg, &l, b.data(), psi.data(), dNormFactor, &c, &perf), ctx,
+             "deviceAMGPCG");
+    return DeviceSolverPerf{perf.initialResidual, perf.finalResidual, perf.nIterations};
+}
+
+// the same solve with the normFactor given on the host
+inline DeviceSolverPerf deviceAMGPCG(const DeviceLduView& Afine, AMGData& amg, const DeviceBuffer<scalar>& b,
+                                     DeviceBuffer<scalar>& psi, scalar normFactor, scalar tol, scalar relTol,
+                                     int maxIter, bool captureVcycle = false, int checkEvery = 1,
+                                     bool corrScaling = false, int minIter = 0)
+{
+    DeviceBuffer<scalar> nf(std::vector<scalar>{normFactor});
+    return deviceAMGPCG(Afine, amg, b, psi, nf.data(), tol, relTol, maxIter, captureVcycle, checkEvery, corrScaling,
+                        minIter);
+}
+
+} // namespace brae

@@ -1,0 +1,171 @@
+// A C++ caller of the pressure path through brae_mfv.hpp, written the way brae's own drivers call it
+// (simpleFoam.cu:343-381): assemble -laplacian on a box, fold, reference cell, buildAMG, amgGalerkin,
+// deviceAMGPCG, residual check with deviceAmul and deviceNormFactorInto, flux; and the error
+// behaviour (refusals throw std::runtime_error("brae hip: ...")). Prints one JSON line; exit status 0
+// when every check holds.
+#include "brae_mfv.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+
+using namespace brae;
+
+namespace {
+
+struct Box
+{
+    int nC = 0;
+    std::vector<label> owner, neighbour, faceOffsets, faceVerts;
+    std::vector<scalar> points;
+};
+
+// nx*ny*nz hex box of the given lengths: internal faces by owner (+x, +y, +z neighbours), then the six
+// boundary patches (x-min, x-max, y-min, y-max, z-min, z-max), every face a quad with its normal out of
+// the owner; interior points jittered by +-jit of the local spacing
+Box makeBox(int nx, int ny, int nz, double lx, double ly, double lz, double jit, unsigned seed)
+{
+    Box b;
+    b.nC = nx * ny * nz;
+    const int px = nx + 1, py = ny + 1, pz = nz + 1;
+    auto P = [&](int i, int j, int k) { return i + px * (j + py * k); };
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    b.points.resize(3 * (size_t)px * py * pz);
+    for (int k = 0; k < pz; ++k)
+        for (int j = 0; j < py; ++j)
+            for (int i = 0; i < px; ++i)
+            {
+                double x = lx * i / nx, y = ly * j / ny, z = lz * k / nz;
+                if (i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz)
+                {
+                    x += jit * lx / nx * u(rng);
+                    y += jit * ly / ny * u(rng);
+                    z += jit * lz / nz * u(rng);
+                }
+                const int p = P(i, j, k);
+                b.points[3 * p] = x;
+                b.points[3 * p + 1] = y;
+                b.points[3 * p + 2] = z;
+            }
+    auto quad = [&](int a, int bb, int c, int d) {
+        b.faceVerts.insert(b.faceVerts.end(), {a, bb, c, d});
+    };
+    for (int c = 0; c < b.nC; ++c)
+    {
+        const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        if (i < nx - 1)
+        {
+            b.owner.push_back(c);
+            b.neighbour.push_back(c + 1);
+            quad(P(i + 1, j, k), P(i + 1, j + 1, k), P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1));
+        }
+        if (j < ny - 1)
+        {
+            b.owner.push_back(c);
+            b.neighbour.push_back(c + nx);
+            quad(P(i, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k));
+        }
+        if (k < nz - 1)
+        {
+            b.owner.push_back(c);
+            b.neighbour.push_back(c + nx * ny);
+            quad(P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j + 1, k + 1), P(i, j + 1, k + 1));
+        }
+    }
+    for (int side = 0; side < 6; ++side)
+        for (int c = 0; c < b.nC; ++c)
+        {
+            const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+            switch (side)
+            {
+            case 0: if (i == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i, j, k + 1), P(i, j + 1, k + 1), P(i, j + 1, k)); } break;
+            case 1: if (i == nx - 1) { b.owner.push_back(c); quad(P(i + 1, j, k), P(i + 1, j + 1, k), P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1)); } break;
+            case 2: if (j == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i + 1, j, k), P(i + 1, j, k + 1), P(i, j, k + 1)); } break;
+            case 3: if (j == ny - 1) { b.owner.push_back(c); quad(P(i, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k)); } break;
+            case 4: if (k == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i, j + 1, k), P(i + 1, j + 1, k), P(i + 1, j, k)); } break;
+            case 5: if (k == nz - 1) { b.owner.push_back(c); quad(P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j + 1, k + 1), P(i, j + 1, k + 1)); } break;
+            }
+        }
+    for (size_t f = 0; f <= b.owner.size(); ++f) b.faceOffsets.push_back(static_cast<label>(4 * f));
+    return b;
+}
+
+template <typename F>
+bool throwsBrae(F f)
+{
+    try
+    {
+        f();
+    }
+    catch (const std::runtime_error& e)
+    {
+        return std::string(e.what()).rfind("brae hip: ", 0) == 0;
+    }
+    return false;
+}
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+    const int n = argc > 1 ? std::atoi(argv[1]) : 32;
+    const Box box = makeBox(n, n, n, 1.0, 1.0, 1.0, 0.1, 7);
+    DeviceMesh dm;
+    dm.build(box.nC, box.owner, box.neighbour, box.points, box.faceOffsets, box.faceVerts);
+    const int nC = dm.nCells, nI = dm.nInternalFaces, nB = dm.nBoundaryFaces;
+
+    // the pressure matrix the way brae holds it (negated laplacian, positive definite): gamma_f = -1
+    DeviceBuffer<scalar> gammaf(std::vector<scalar>(nI, -1.0));
+    DeviceBuffer<scalar> diag, upper;
+    deviceLaplacianCoeffsSymmetric(dm, gammaf, diag, upper, true);
+    // all-Neumann pressure with a zero-mean source and a reference cell
+    std::mt19937_64 rng(3);
+    std::normal_distribution<double> g(0.0, 1.0);
+    std::vector<scalar> src(nC);
+    double mean = 0.0;
+    for (auto& s : src) mean += (s = g(rng));
+    mean /= nC;
+    for (auto& s : src) s -= mean;
+    DeviceBuffer<scalar> source(src), iC(std::vector<scalar>(nB, 0.0)), bC(std::vector<scalar>(nB, 0.0));
+    DeviceBuffer<scalar> diagC, b;
+    deviceFold(dm, diag, source, iC, bC, diagC, b);
+    deviceSetReference(diagC, b, 0, 0.0);
+
+    std::vector<label> own(box.owner.begin(), box.owner.begin() + nI);
+    const std::vector<scalar> magSf = dm.magSfInternal.download();
+    AMGData amg = buildAMG(own, box.neighbour, magSf, nC);
+    amgGalerkin(amg, diagC, upper, upper);
+    const DeviceLduView A = lduView(dm, diagC, upper);
+    DeviceBuffer<scalar> psi(std::vector<scalar>(nC, 0.0));
+    const DeviceSolverPerf perf = deviceAMGPCG(A, amg, b, psi, static_cast<const scalar*>(nullptr), 1e-8, 0.0, 500,
+                                               false, 4, true, 0);
+
+    // independent residual check: sum|b - A psi| / normFactor(psi, b)
+    DeviceBuffer<scalar> Apsi, nf, ones;
+    deviceAmul(A, psi, Apsi);
+    deviceNormFactorInto(A, psi, b, ones, nf);
+    const std::vector<scalar> hA = Apsi.download(), hb = b.download(), hnf = nf.download();
+    double res = 0.0;
+    for (int c = 0; c < nC; ++c) res += std::fabs(hb[c] - hA[c]);
+    res /= hnf[0];
+    DeviceBuffer<scalar> flux;
+    deviceMatrixFluxInternal(A, psi, flux);
+
+    // refusals by name
+    const bool refusesSA = throwsBrae([&] {
+        const bool sa = true;
+        AMGData x = buildAMG(own, box.neighbour, magSf, nC, &sa);
+    });
+    DeviceBuffer<scalar> lower(std::vector<scalar>(nI, -0.5));
+    const bool refusesAsym = throwsBrae([&] { amgGalerkin(amg, diagC, upper, lower); });
+
+    const bool ok = perf.nIterations > 0 && perf.nIterations < 500 && perf.finalResidual < 1e-8 && res < 2e-8 &&
+                    refusesSA && refusesAsym && flux.size() == (size_t)nI;
+    std::printf("{\"cells\": %d, \"levels\": %d, \"iterations\": %d, \"initial\": %.6e, \"final\": %.6e, "
+                "\"recomputed\": %.6e, \"refuses_sa\": %s, \"refuses_asym\": %s, \"ok\": %s}\n",
+                nC, amg.nLevels(), perf.nIterations, perf.initialResidual, perf.finalResidual, res,
+                refusesSA ? "true" : "false", refusesAsym ? "true" : "false", ok ? "true" : "false");
+    return ok ? 0 : 1;
+}
