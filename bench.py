@@ -229,6 +229,8 @@ def gpu_run(args):
                                 "alg_bytes": int(names[t][1]),
                                 "alg_GBps": round(names[t][1] / (prof[t][1] / max(1, prof[t][0]) * 1e-3) / 1e9, 1)}
                   for t in names if prof[t][1] > 0}
+    free_b, total_b = torch.cuda.mem_get_info()
+    hbm_used_gib = (total_b - free_b) / 2 ** 30
     copy_gbs = copy_peak(torch, N)
     vc_n, vc_ms = prof[mfv.PROF_VCYCLE]
     grids = []
@@ -249,6 +251,7 @@ def gpu_run(args):
                       alg_bytes_per_launch=int(alg_bytes), avg_launch_us=round(avg_ms * 1e3, 2),
                       copy_peak_GBps=round(copy_gbs, 1)),
         per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1), grids=grids,
+        hbm_used_gib=round(hbm_used_gib, 2),
     )
     amg.close()
     ch.mesh.close()
@@ -393,6 +396,7 @@ def main():
                 "cycle": "V(%d,%d) weighted Jacobi%s" % (args.npre, args.npost,
                                                         "" if args.no_scale else ", scaled correction"),
                 "per_grid_us": r["grids"]},
+        "hbm_used_gib": r["hbm_used_gib"],
         "roofline": r["roofline"],
         "per_kernel": r["per_kernel"],
         "cpu_baseline": cpu,

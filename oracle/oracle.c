@@ -1649,6 +1649,27 @@ static void br_vcycle_d(br_hier* H, int g)
     for (int i = 0; i < n; ++i) bc[L->map[i]] += r[i];
     br_vcycle_d(H, g + 1);
     double* xc = H->x[g + 1];
+    if (H->scale == 2)
+    {
+        /* the reference's own order (device_amg_vcycle.cu:1113-1126): c = P xc (prolongToK), Ac,
+         * alpha = (r.c)/(c.Ac) (deviceDotInto x2, scaleFactorK), xg += alpha*c (deviceAxpyDev), then
+         * the post-sweeps each with a fresh product */
+        double* e = (double*)xcalloc((size_t)n, sizeof(double));
+        for (int i = 0; i < n; ++i) e[i] = xc[L->map[i]];
+        br_amul_d(A, e, Ax);
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < n; ++i) num += r[i] * e[i];
+        for (int i = 0; i < n; ++i) den += e[i] * Ax[i];
+        const double al = den > 1e-30 ? num / den : 1.0;
+        for (int i = 0; i < n; ++i) x[i] = x[i] + al * e[i];
+        free(e);
+        for (int s = 0; s < H->nPost; ++s)
+        {
+            br_amul_d(A, x, Ax);
+            for (int i = 0; i < n; ++i) x[i] += 0.8 * (b[i] - Ax[i]) / safe_diag_d(A->diag[i]);
+        }
+        return;
+    }
     if (H->scale)
     {
         /* scaled correction fused with the first post-sweep: e = P xc, alpha = (r.e)/(e.Ae)

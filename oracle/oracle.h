@@ -148,7 +148,9 @@ orc_perf orc_pcg_gamg_solve(const orc_aggl* a, const double* diag, const double*
  *      device_amg_vcycle.cu:982-1143 / 1658-1766, amg_kernels.cuh:27-53,257-282;
  *      PCG recurrence device_amg_pcg.cu:60-170 (Fletcher-Reeves) or flexBetaK :37-49 (flexible).
  *      scale: each grid's coarse correction scaled by alpha = (r.e)/(e.Ae) (corrScaling,
- *      device_amg_vcycle.cu:1113-1126), fused with the first post-sweep.
+ *      device_amg_vcycle.cu:1113-1126); 1: fused with the first post-sweep as the HIP library
+ *      computes it (x + alpha*e + omega*(r - alpha*Ae)/D); 2 (FP64 cycle only): the reference's own
+ *      order, x += alpha*e and then full post-sweeps.
  *      normFactorIn > 0 replaces OpenFOAM's normFactor (the demo trace passes sum|b| + 1e-20).
  *      psiTrace (nullable, (maxIter+1)*nCells) receives psi after each iteration. ---- */
 orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* upper, const double* b,
