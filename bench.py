@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--npost", type=int, default=1)
     ap.add_argument("--merge-levels", type=int, default=1, help="pairwise passes per AMG level")
     ap.add_argument("--non-orth", type=int, default=1, help="nNonOrthogonalCorrectors (solves per step - 1)")
+    ap.add_argument("--serving", type=int, default=512, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-slots", type=int, default=8)
     return ap.parse_args()
 
 
@@ -259,6 +262,42 @@ def gpu_run(args):
     return out
 
 
+def serving_run(args):
+    """The serving stream: Poisson arrivals at --serve-rate of pressure-Poisson requests on boxes of
+    log-uniform 32k-4M cells, x-aspect uniform 1-8, 30% with 10%-jittered points, 1-20 right-hand sides
+    each, admitted by a 200 GB HBM budget into --serve-slots concurrent slots; FP32 V-cycle AMG-PCG to
+    1e-6. Latency = finish - arrival."""
+    from brae_amd import mfv
+    rng = np.random.default_rng(2025)
+    t, reqs = 0.0, []
+    for k in range(args.serving):
+        t += rng.exponential(1.0 / args.serve_rate)
+        cells = float(np.exp(rng.uniform(np.log(32e3), np.log(4e6))))
+        a = rng.uniform(1.0, 8.0)
+        ny = max(2, int(round((cells / a) ** (1.0 / 3.0))))
+        nx = max(2, int(round(a * ny)))
+        jit = 0.1 if rng.uniform() < 0.3 else 0.0
+        reqs.append(dict(nx=nx, ny=ny, nz=ny, lx=a, ly=1.0, lz=1.0, jitter=jit, seed=1000 + k,
+                         nSteps=int(rng.integers(1, 21)), tolerance=1e-6, arrival=t))
+    t0 = time.perf_counter()
+    rc, res = mfv.serve(reqs, nSlots=args.serve_slots, hbmBudgetBytes=200e9, precision=32)
+    wall = time.perf_counter() - t0
+    if rc != mfv.MFV_OK:
+        raise RuntimeError("serving run failed (%d)" % rc)
+    lat = np.array([r.finished - r.arrival for r in res])
+    solves = sum(r.nSteps for r in res)
+    makespan = max(r.finished for r in res) - min(r.arrival for r in res)
+    log("serving: %d requests, %d solves in %.1f s; latency p50 %.3f s p99 %.3f s" % (
+        len(res), solves, makespan, np.percentile(lat, 50), np.percentile(lat, 99)))
+    return dict(requests=len(res), solves=solves, cells_total=int(sum(r.nCells for r in res)),
+                solves_per_s=round(solves / makespan, 2), p50_latency_s=round(float(np.percentile(lat, 50)), 4),
+                p99_latency_s=round(float(np.percentile(lat, 99)), 4), makespan_s=round(makespan, 2),
+                wall_s=round(wall, 2), arrival_rate_per_s=args.serve_rate, slots=args.serve_slots,
+                hbm_budget_gb=200, max_residual=max(r.finalResidual for r in res),
+                workload="pressure Poisson on hex boxes, log-uniform 32k-4M cells, x-aspect 1-8, 30% jittered, "
+                         "1-20 RHS per request, FP32 V-cycle AMG-PCG to 1e-6")
+
+
 def copy_peak(torch, n):
     """Measured device-to-device copy rate (read + write bytes / time) of n doubles: the plain copy kernel
     the HBM byte counters are calibrated against, and the box's practical bandwidth ceiling."""
@@ -368,6 +407,7 @@ def main():
         sys.exit(2)
     nx, ny, nz = (int(v) for v in args.size.split("x"))
     r = gpu_run(args)
+    serving = serving_run(args) if args.serving > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, nx * ny * nz)
     line = {
         "metric": METRIC,
@@ -400,6 +440,7 @@ def main():
         "roofline": r["roofline"],
         "per_kernel": r["per_kernel"],
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

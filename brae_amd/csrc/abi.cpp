@@ -388,33 +388,36 @@ int mfv_norm_factor(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* 
 }
 
 
-int mfv_amg_build(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const mfv_amg_controls* controls,
-                  mfv_amg* out)
+extern "C++" {
+namespace {
+
+AmgBuildControls buildControls(const mfv_amg_controls* controls)
+{
+    AmgBuildControls bc;
+    if (!controls) return bc;
+    if (controls->agglomeration != MFV_AGG_BRAE && controls->agglomeration != MFV_AGG_FACEAREAPAIR)
+        throw Error(kErrUnsupported, "mfv: amg build: unknown agglomeration " + std::to_string(controls->agglomeration));
+    if (controls->mergeLevels > 4) throw Error(kErrUnsupported, "mfv: amg build: mergeLevels above 4 is not carried");
+    bc.mergeLevels = std::max(1, controls->mergeLevels);
+    bc.kind = controls->agglomeration == MFV_AGG_BRAE ? AggKind::Brae : AggKind::OpenFOAM;
+    bc.target = controls->nCellsInCoarsestLevel > 0 ? controls->nCellsInCoarsestLevel
+                                                    : (bc.kind == AggKind::Brae ? 64 : 10);
+    return bc;
+}
+
+// runs `fill` on a new hierarchy of `mesh`; hands it out only when fill succeeded
+template <typename F>
+int newAmg(mfv_ctx ctx, mfv_mesh mesh, mfv_amg* out, F fill)
 {
     if (!out) return MFV_ERR_INVALID;
     *out = nullptr;
     mfv_amg amg = nullptr;
     const int rc = guarded(ctx, [&] {
         needMesh(ctx, mesh);
-        need(hFaceWeights != nullptr || mesh->m.nFaces == 0, "amg build: face weights missing");
-        AmgBuildControls bc;
-        if (controls)
-        {
-            if (controls->agglomeration != MFV_AGG_BRAE && controls->agglomeration != MFV_AGG_FACEAREAPAIR)
-                throw Error(kErrUnsupported, "mfv: amg build: unknown agglomeration " +
-                                                 std::to_string(controls->agglomeration));
-            if (controls->mergeLevels > 4)
-                throw Error(kErrUnsupported, "mfv: amg build: mergeLevels above 4 is not carried");
-            bc.mergeLevels = std::max(1, controls->mergeLevels);
-            bc.kind = controls->agglomeration == MFV_AGG_BRAE ? AggKind::Brae : AggKind::OpenFOAM;
-            bc.target = controls->nCellsInCoarsestLevel > 0 ? controls->nCellsInCoarsestLevel
-                                                            : (bc.kind == AggKind::Brae ? 64 : 10);
-        }
         amg = new mfv_amg_s();
         amg->ctx = ctx;
         amg->mesh = mesh;
-        std::vector<double> w(hFaceWeights, hFaceWeights + mesh->m.nFaces);
-        amgBuild(ctx->c, mesh->m, amg->a, w, bc);
+        fill(amg);
     });
     if (rc != MFV_OK)
     {
@@ -423,6 +426,102 @@ int mfv_amg_build(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const 
     }
     *out = amg;
     return MFV_OK;
+}
+
+} // namespace
+} // extern "C++"
+
+int mfv_amg_build(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const mfv_amg_controls* controls,
+                  mfv_amg* out)
+{
+    return newAmg(ctx, mesh, out, [&](mfv_amg amg) {
+        need(hFaceWeights != nullptr || mesh->m.nFaces == 0, "amg build: face weights missing");
+        const AmgBuildControls bc = buildControls(controls);
+        std::vector<double> w(hFaceWeights, hFaceWeights + mesh->m.nFaces);
+        amgBuild(ctx->c, mesh->m, amg->a, w, bc);
+        amg->a.signature = amgSignature(mesh->m.hOwner, mesh->m.hNeighbour, w, mesh->m.nCells, bc);
+    });
+}
+
+int mfv_amg_signature(int nCells, int nFaces, const int* hOwner, const int* hNeighbour, const double* hFaceWeights,
+                      const mfv_amg_controls* controls, unsigned long long* signature)
+{
+    if (!signature || nCells <= 0 || nFaces < 0 || (nFaces && (!hOwner || !hNeighbour || !hFaceWeights)))
+        return MFV_ERR_INVALID;
+    try
+    {
+        const AmgBuildControls bc = buildControls(controls);
+        *signature = amgSignature(std::vector<int>(hOwner, hOwner + nFaces),
+                                  std::vector<int>(hNeighbour, hNeighbour + nFaces),
+                                  std::vector<double>(hFaceWeights, hFaceWeights + nFaces), nCells, bc);
+        return MFV_OK;
+    }
+    catch (const Error& e)
+    {
+        return e.code;
+    }
+    catch (...)
+    {
+        return MFV_ERR_INVALID;
+    }
+}
+
+int mfv_amg_save(mfv_ctx ctx, mfv_amg amg, const char* path)
+{
+    return guarded(ctx, [&] {
+        need(amg && amg->ctx == ctx, "amg save: hierarchy of another context");
+        need(path && *path, "amg save: no path");
+        amgSave(amg->a, amg->a.signature, path);
+    });
+}
+
+int mfv_amg_build_or_load(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const mfv_amg_controls* controls,
+                          const char* cacheDir, int writeCache, mfv_amg* out, int* how)
+{
+    return newAmg(ctx, mesh, out, [&](mfv_amg amg) {
+        need(hFaceWeights != nullptr || mesh->m.nFaces == 0, "amg build: face weights missing");
+        need(cacheDir && *cacheDir, "amg build_or_load: no cache directory");
+        const AmgBuildControls bc = buildControls(controls);
+        std::vector<double> w(hFaceWeights, hFaceWeights + mesh->m.nFaces);
+        const uint64_t sig = amgSignature(mesh->m.hOwner, mesh->m.hNeighbour, w, mesh->m.nCells, bc);
+        const std::string path = std::string(cacheDir) + "/.mfv_amgcache";
+        std::vector<AggLevel> levels;
+        AmgBuildControls stored;
+        CacheRead r = amgReadLevels(path, sig, levels, stored);
+        if (r == CacheRead::Loaded)
+        {
+            try
+            {
+                amgBuildFrom(ctx->c, mesh->m, amg->a, std::move(levels), stored);
+            }
+            catch (const Error& e)
+            {
+                if (e.code != kErrInvalid) throw;
+                r = CacheRead::Unreadable;   // a file whose levels do not fit this mesh: build afresh
+            }
+        }
+        if (r != CacheRead::Loaded)
+        {
+            amgBuild(ctx->c, mesh->m, amg->a, w, bc);
+            if (writeCache)
+            {
+                amg->a.signature = sig;
+                amgSave(amg->a, sig, path);
+            }
+        }
+        amg->a.signature = sig;
+        if (how) *how = (int)r;
+    });
+}
+
+int mfv_amg_clone(mfv_ctx ctx, mfv_amg src, mfv_amg* out)
+{
+    if (!src) return MFV_ERR_INVALID;
+    return newAmg(ctx, src->mesh, out, [&](mfv_amg amg) {
+        need(src->ctx == ctx, "amg clone: hierarchy of another context");
+        amgBuildFrom(ctx->c, src->mesh->m, amg->a, src->a.agg, src->a.controls);
+        amg->a.signature = src->a.signature;
+    });
 }
 
 int mfv_amg_destroy(mfv_amg amg)
@@ -536,6 +635,50 @@ int mfv_pcg_default_controls(mfv_pcg_controls* c)
     c->nVcycles = 1;
     c->scaleCorrection = 0;
     return MFV_OK;
+}
+
+int mfv_dic_precondition(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dR, double* dW)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dR && dW && dR != dW, "dic: null or aliased argument");
+        DBuf<double> rD(mesh->m.nCells);
+        launchDicPrecondition(ctx->c, mesh->m, view(mesh, A), dR, dW, rD.p);
+        hipCheck(hipStreamSynchronize(ctx->c.stream), "dic sync");
+    });
+}
+
+int mfv_gauss_seidel(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dB, double* dPsi, int nSweeps)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dB && dPsi && dB != dPsi, "gauss-seidel: null or aliased argument");
+        need(nSweeps >= 0, "gauss-seidel: negative sweep count");
+        launchGaussSeidel(ctx->c, mesh->m, view(mesh, A), dB, dPsi, nSweeps);
+    });
+}
+
+int mfv_pcg_dic_solve(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dB, double* dPsi,
+                      const double* dNormFactor, const mfv_pcg_controls* controls, mfv_perf* perf)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dB && dPsi && controls, "pcg-dic: null argument");
+        PcgControls pc;
+        pc.tolerance = controls->tolerance;
+        pc.relTol = controls->relTol;
+        pc.maxIter = controls->maxIter;
+        pc.minIter = controls->minIter;
+        pc.checkEvery = controls->checkEvery;
+        const PcgPerf r = pcgDicSolve(ctx->c, mesh->m, view(mesh, A), dB, dPsi, dNormFactor, pc);
+        if (perf)
+        {
+            perf->initialResidual = r.initialResidual;
+            perf->finalResidual = r.finalResidual;
+            perf->nIterations = r.nIterations;
+            perf->converged = r.converged ? 1 : 0;
+        }
+    });
 }
 
 int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,

@@ -235,6 +235,195 @@ __global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __rest
     }
 }
 
+// ---------------- the small grids in one workgroup ----------------
+//
+// Below kTailCells cells a grid's kernels are launch-bound (a few microseconds each, whatever their
+// size). The tail kernel runs the whole cycle from grid tailStart down to the coarsest and back in one
+// workgroup of kTailThreads threads, with a barrier between steps: the same steps, in the same order
+// and arithmetic as the per-grid kernels, the scaled correction's two sums reduced over the workgroup
+// in a fixed order. It hands its first grid's values up through the same injection as the per-grid
+// path. (The sequential-reduction mode keeps the per-grid kernels, whose sums it defines.)
+constexpr int kTailThreads = 1024;
+constexpr int kTailCells = 4096;
+constexpr int kMaxTail = 32;
+
+struct TailGrid
+{
+    SellView S;
+    const void* val;
+    const void* diag;
+    const double* scale;
+    void *b, *x, *y, *r, *e;
+    const int* cellStart;   // this grid's cells per coarse cell of the next grid
+    const int* cellList;
+    int n;
+};
+
+template <typename Tm, typename Tv>
+__device__ __forceinline__ void tailSweep(const TailGrid& G, const Tv* __restrict__ xin, Tv* __restrict__ xout,
+                                          const Scatter<Tv>& sc)
+{
+    const Tm* val = static_cast<const Tm*>(G.val);
+    const Tv* diag = static_cast<const Tv*>(G.diag);
+    const Tv* b = static_cast<const Tv*>(G.b);
+    for (int c = threadIdx.x; c < G.n; c += kTailThreads)
+    {
+        const Tv ax = rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, PlainX<Tv>{xin});
+        const Tv xo = xin[c] + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+        xout[c] = xo;
+        sc(c, xo);
+    }
+}
+
+template <typename Tm, typename Tv>
+__global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict__ D, int T,
+                                                     const double* __restrict__ inv, int nPre, int nPost,
+                                                     int scaled, Scatter<Tv> up, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    __shared__ double lds[kTailThreads / 64];
+    __shared__ double sb[1024];
+    __shared__ double shAlpha;
+    const int tid = threadIdx.x;
+    const Scatter<Tv> none{nullptr, nullptr, nullptr};
+    // grid g's current iterate is its x buffer, or y after an odd number of ping-pong sweeps (bit g)
+    unsigned flip = 0;
+    auto X = [&](int g) { return static_cast<Tv*>((flip >> g) & 1u ? D[g].y : D[g].x); };
+    auto Y = [&](int g) { return static_cast<Tv*>((flip >> g) & 1u ? D[g].x : D[g].y); };
+    // the finishing step of grid g hands its values to grid g-1's e, or out of the tail for g = 0
+    auto upOf = [&](int g) {
+        if (g == 0) return up;
+        return Scatter<Tv>{D[g - 1].cellStart, D[g - 1].cellList, static_cast<Tv*>(D[g - 1].e)};
+    };
+    // ---- down ----
+    for (int g = 0; g + 1 < T; ++g)
+    {
+        const TailGrid G = D[g];
+        const Tm* val = static_cast<const Tm*>(G.val);
+        const Tv* diag = static_cast<const Tv*>(G.diag);
+        const Tv* b = static_cast<const Tv*>(G.b);
+        Tv* r = static_cast<Tv*>(G.r);
+        for (int s = 1; s < nPre; ++s)
+        {
+            tailSweep<Tm, Tv>(G, X(g), Y(g), none);
+            __syncthreads();
+            flip ^= 1u << g;
+        }
+        Tv* x = X(g);
+        if (nPre == 0)
+        {
+            for (int c = tid; c < G.n; c += kTailThreads) x[c] = Tv(0);
+            __syncthreads();
+        }
+        for (int c = tid; c < G.n; c += kTailThreads)
+            r[c] = b[c] - rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, PlainX<Tv>{x});
+        __syncthreads();
+        const TailGrid C = D[g + 1];
+        const Tv* dC = static_cast<const Tv*>(C.diag);
+        Tv* bC = static_cast<Tv*>(C.b);
+        Tv* xC = X(g + 1);
+        const int fromZero = (g + 2 < T && nPre > 0) ? 1 : 0;
+        for (int k = tid; k < C.n; k += kTailThreads)
+        {
+            Tv s = Tv(0);
+            for (int q = G.cellStart[k]; q < G.cellStart[k + 1]; ++q) s += r[G.cellList[q]];
+            bC[k] = s;
+            if (fromZero) xC[k] = Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(dC[k]);
+        }
+        __syncthreads();
+    }
+    // ---- coarsest: x = Ainv b ----
+    {
+        const TailGrid G = D[T - 1];
+        const Tv* b = static_cast<const Tv*>(G.b);
+        Tv* x = X(T - 1);
+        for (int j = tid; j < G.n; j += kTailThreads) sb[j] = (double)b[j];
+        __syncthreads();
+        const Scatter<Tv> sc = upOf(T - 1);
+        for (int i = tid; i < G.n; i += kTailThreads)
+        {
+            double s = 0.0;
+            const double* row = inv + (size_t)i * G.n;
+            for (int j = 0; j < G.n; ++j) s += row[j] * sb[j];
+            x[i] = (Tv)s;
+            sc(i, (Tv)s);
+        }
+        __syncthreads();
+    }
+    // ---- up ----
+    const int nExtra = nPost > 1 ? nPost - 1 : 0;
+    for (int g = T - 2; g >= 0; --g)
+    {
+        const TailGrid G = D[g];
+        const Tm* val = static_cast<const Tm*>(G.val);
+        const Tv* diag = static_cast<const Tv*>(G.diag);
+        const Tv* r = static_cast<const Tv*>(G.r);
+        const Tv* e = static_cast<const Tv*>(G.e);
+        const Scatter<Tv> sc = upOf(g);
+        if (scaled)
+        {
+            Tv* x = X(g);
+            Tv* Ae = Y(g);
+            double num = 0.0, den = 0.0;
+            for (int c = tid; c < G.n; c += kTailThreads)
+            {
+                const Tv ae = rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, PlainX<Tv>{e});
+                Ae[c] = ae;
+                const double ev = (double)e[c];
+                num += ev * (double)r[c];
+                den += ev * (double)ae;
+            }
+            num = blockSum<kTailThreads>(num, lds);
+            den = blockSum<kTailThreads>(den, lds);
+            if (tid == 0) shAlpha = den > 1e-30 ? num / den : 1.0;
+            __syncthreads();
+            const Tv al = (Tv)shAlpha;
+            for (int c = tid; c < G.n; c += kTailThreads)
+            {
+                const Tv ev = e[c];
+                const Tv xo = nPost > 0 ? x[c] + al * ev + Tv(kOmega) * (r[c] - al * Ae[c]) / safeDiag(diag[c])
+                                        : x[c] + al * ev;
+                x[c] = xo;
+                if (!nExtra) sc(c, xo);
+            }
+            __syncthreads();
+        }
+        else if (nPost > 0)
+        {
+            const Tv* b = static_cast<const Tv*>(G.b);
+            const SumX<Tv> XE{X(g), e};
+            Tv* y = Y(g);
+            for (int c = tid; c < G.n; c += kTailThreads)
+            {
+                const Tv ax = rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, XE);
+                const Tv xo = XE(c) + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
+                y[c] = xo;
+                if (!nExtra) sc(c, xo);
+            }
+            __syncthreads();
+            flip ^= 1u << g;
+        }
+        else
+        {
+            Tv* x = X(g);
+            for (int c = tid; c < G.n; c += kTailThreads)
+            {
+                const Tv xo = x[c] + e[c];
+                x[c] = xo;
+                sc(c, xo);
+            }
+            __syncthreads();
+        }
+        if (scaled || nPost > 0)
+            for (int s = 0; s < nExtra; ++s)
+            {
+                tailSweep<Tm, Tv>(G, X(g), Y(g), s + 1 == nExtra ? sc : none);
+                __syncthreads();
+                flip ^= 1u << g;
+            }
+    }
+}
+
 // ---------------- Galerkin and fill kernels ----------------
 
 // coarse diag = sum(fine diag, ascending fine cell) + sum(upper + lower of the fine faces inside the
@@ -620,6 +809,15 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
         coarseApplyK<Tv><<<1, kTpb, 0, c.stream>>>(n, a.coarseInv.p, v.b, v.x, up, stop);
         return v.x;
     }
+    if (gi > 0 && gi == a.tailStart && !c.sequentialReduce)
+    {
+        ProfScope ps(c, kProfGridDown + gi);
+        tailK<Tm, Tv><<<1, kTailThreads, 0, c.stream>>>(reinterpret_cast<const TailGrid*>(a.tailDesc.p),
+                                                         last - gi + 1, a.coarseInv.p, vc.nPre, vc.nPost,
+                                                         vc.scale ? 1 : 0, up, stop);
+        hipCheck(hipGetLastError(), "vcycle tail");
+        return nullptr;   // grid gi's values went up through `up`; no caller reads them here
+    }
     ProfScope psDown(c, kProfGridDown + gi);
     const SellView S = viewOf(g.sell);
     const Tm* val = offOf<Tm>(g).p;
@@ -728,9 +926,36 @@ void amgBuild(Ctx& c, const Mesh& m, Amg& a, const std::vector<double>& faceWeig
     if ((int)faceWeights.size() != m.nFaces) invalid("amg build: one face weight per internal face is required");
     if (ctl.target < 1) invalid("amg build: the coarsest-level target must be positive");
     bool forward = true;
+    amgBuildFrom(c, m, a,
+                 mergeLevels(agglomerate(ctl.kind, m.nCells, m.hOwner, m.hNeighbour, faceWeights, ctl.target,
+                                         forward),
+                             ctl.mergeLevels),
+                 ctl);
+}
+
+void amgBuildFrom(Ctx& c, const Mesh& m, Amg& a, std::vector<AggLevel> levels, const AmgBuildControls& ctl)
+{
     a.nFine = m.nCells;
-    a.agg = mergeLevels(agglomerate(ctl.kind, m.nCells, m.hOwner, m.hNeighbour, faceWeights, ctl.target, forward),
-                        ctl.mergeLevels);
+    a.controls = ctl;
+    a.agg = std::move(levels);
+    for (size_t i = 0; i < a.agg.size(); ++i)
+    {
+        const AggLevel& L = a.agg[i];
+        const int nF = i == 0 ? m.nCells : a.agg[i - 1].nCoarse;
+        const int nFF = i == 0 ? m.nFaces : a.agg[i - 1].nCoarseFaces;
+        if (L.nFine != nF || L.nFineFaces != nFF || (int)L.map.size() != nF || (int)L.faceRestrict.size() != nFF ||
+            (int)L.faceFlip.size() != nFF || (int)L.cOwn.size() != L.nCoarseFaces ||
+            (int)L.cNei.size() != L.nCoarseFaces)
+            invalid("amg: level " + std::to_string(i) + " does not fit the grid above it");
+        for (int v : L.map)
+            if (v < 0 || v >= L.nCoarse) invalid("amg: level " + std::to_string(i) + " maps outside its coarse grid");
+        for (int f = 0; f < L.nCoarseFaces; ++f)
+            if (L.cOwn[f] < 0 || L.cOwn[f] >= L.cNei[f] || L.cNei[f] >= L.nCoarse)
+                invalid("amg: level " + std::to_string(i) + " has a coarse face outside upper-triangular order");
+        for (int r : L.faceRestrict)
+            if (r >= L.nCoarseFaces || r < -L.nCoarse)
+                invalid("amg: level " + std::to_string(i) + " restricts a face outside its coarse grid");
+    }
     const int nG = (int)a.agg.size() + 1;
     a.grids.clear();
     a.grids.resize(nG);
@@ -771,6 +996,58 @@ void amgBuild(Ctx& c, const Mesh& m, Amg& a, const std::vector<double>& faceWeig
     a.allocatedVectors = false;
     hipCheck(hipStreamSynchronize(c.stream), "amg build");
 }
+
+namespace {
+
+// The tail's per-grid descriptors for the current precision (pointers into the grids' buffers, which
+// stay put until the next allocation): grids from the first one of at most kTailCells cells (never
+// grid 0) to the coarsest.
+void buildTail(Ctx& c, Amg& a)
+{
+    if (a.tailFor == (int)a.prec) return;
+    a.tailFor = (int)a.prec;
+    const int nG = (int)a.grids.size();
+    a.tailStart = 0;
+    for (int g = 1; g < nG; ++g)
+        if (a.grids[g].n <= kTailCells)
+        {
+            a.tailStart = g;
+            break;
+        }
+    if (!a.tailStart || nG - a.tailStart > kMaxTail)
+    {
+        a.tailStart = 0;
+        return;
+    }
+    std::vector<TailGrid> d;
+    for (int g = a.tailStart; g < nG; ++g)
+    {
+        Grid& G = a.grids[g];
+        TailGrid t{};
+        t.S = viewOf(G.sell);
+        t.n = G.n;
+        t.scale = a.scales.p + g;
+        t.cellStart = G.cellStart.p;
+        t.cellList = G.cellList.p;
+        if (a.prec == Prec::F64)
+        {
+            t.val = G.off64.p;
+            t.diag = G.diagV64.p;
+            t.b = G.b64.p; t.x = G.x64.p; t.y = G.y64.p; t.r = G.r64.p; t.e = G.e64.p;
+        }
+        else
+        {
+            t.val = a.prec == Prec::F32 ? (const void*)G.off32.p : (const void*)G.off16.p;
+            t.diag = G.diagV32.p;
+            t.b = G.b32.p; t.x = G.x32.p; t.y = G.y32.p; t.r = G.r32.p; t.e = G.e32.p;
+        }
+        d.push_back(t);
+    }
+    a.tailDesc.upload(reinterpret_cast<const uint8_t*>(d.data()), d.size() * sizeof(TailGrid), c.stream);
+    hipCheck(hipStreamSynchronize(c.stream), "tail descriptors");
+}
+
+} // namespace
 
 void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper, Prec prec)
 {
@@ -861,8 +1138,10 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
         }
         a.allocatedVectors = true;
         a.vectorsFor = prec;
+        a.tailFor = -1;
     }
     a.prec = prec;
+    buildTail(c, a);
     a.galerkinDone = true;
 }
 

@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 namespace mfv {
@@ -88,6 +89,8 @@ struct PcgWork
 struct Amg
 {
     int nFine = 0;
+    AmgBuildControls controls;
+    uint64_t signature = 0;        // fingerprint of the inputs the structure was built from
     std::vector<AggLevel> agg;     // host addressing, agg[i]: grid i -> grid i+1
     std::vector<Grid> grids;       // grids[0] = the mesh
     int nCoarsest = 0;
@@ -97,6 +100,9 @@ struct Amg
     DBuf<double> alphas;           // per grid: the scaled correction's alpha
     DBuf<double> seqA, seqB;       // per-cell terms for sequential (OpenFOAM-order) reductions
     Prec prec = Prec::F32;         // precision the V-cycle copies are filled in
+    int tailStart = 0;             // first grid the single-workgroup tail kernel runs (0: none)
+    DBuf<uint8_t> tailDesc;        // per tail grid: pointers and sizes for the tail kernel
+    int tailFor = -1;              // precision the descriptors were built for (-1: stale)
     bool galerkinDone = false;
     bool allocatedVectors = false;
     Prec vectorsFor = Prec::F32;
@@ -105,6 +111,19 @@ struct Amg
 
 // Build the hierarchy (host agglomeration + device structures).
 void amgBuild(Ctx& c, const Mesh& m, Amg& a, const std::vector<double>& faceWeights, const AmgBuildControls& ctl);
+// The device side of a build from given levels (a cache load or a clone); checks that the levels fit.
+void amgBuildFrom(Ctx& c, const Mesh& m, Amg& a, std::vector<AggLevel> levels, const AmgBuildControls& ctl);
+
+// Hierarchy persistence (amgHierarchySignature, device_amg.cu:1262-1285; writeAMGCache/readAMGCache,
+// device_amg_cache.cu:176-326): a 64-bit fingerprint of the addressing, the face weights and the build
+// controls; the structure (levels, no values) in one file written under a temporary name and renamed.
+enum class CacheRead : int { Loaded = 0, Absent = 1, OtherMeshOrBuild = 2, Unreadable = 3 };
+uint64_t amgSignature(const std::vector<int>& owner, const std::vector<int>& neighbour,
+                      const std::vector<double>& weights, int nCells, const AmgBuildControls& ctl);
+void amgSave(const Amg& a, uint64_t signature, const std::string& path);
+// reads the levels when the file holds this signature; the caller then builds the device side
+CacheRead amgReadLevels(const std::string& path, uint64_t signature, std::vector<AggLevel>& levels,
+                        AmgBuildControls& ctl);
 // Coarse matrices from the fine FP64 matrix (symmetric: lower == upper), V-cycle copies in `prec`,
 // the coarsest inverse.
 void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper, Prec prec);

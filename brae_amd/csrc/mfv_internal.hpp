@@ -8,6 +8,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -139,6 +140,15 @@ enum ProfTag : int
     kProfCount = 144
 };
 
+// Level schedule of the lower-triangular (forward) and upper-triangular (backward) dependency order of
+// a mesh's cells: a cell's forward level is one past its highest lower neighbour's, so every cell of a
+// level depends only on earlier levels. Cells are listed level by level, ascending within a level.
+struct LevelSched
+{
+    std::vector<int> fwdStart, bwdStart;   // [levels + 1] into the lists (host)
+    DBuf<int> fwdCells, bwdCells;          // [nCells] each
+};
+
 // The mesh's internal-face LDU addressing on the device (lduAddressing), plus the boundary
 // gather (cells' boundary faces in ascending face order).
 struct Mesh
@@ -155,6 +165,7 @@ struct Mesh
     DBuf<label> bndStart;      // [nCells+1] into bndFaces
     DBuf<label> bndFaces;      // boundary face indices (0-based within the boundary) per cell, ascending
     std::vector<label> hOwner, hNeighbour, hBFaceCell;
+    std::unique_ptr<LevelSched> sched;   // built on first use by the sequential sweeps (DIC, Gauss-Seidel)
 };
 
 // ---- reductions / BLAS (kernels_blas.hip) ----
@@ -194,6 +205,14 @@ void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcV
                      const double* w, const double* V, const double* psi, double* grad);
 void launchLaplacianCorr(Ctx& c, const Mesh& m, const double* grad, const double* w, const double* corrVec,
                          const double* gammaf, const double* magSf, double* ffc, double* source);
+
+// ---- sequential-order sweeps on the level schedule (dic.hip) ----
+struct PcgControls;
+struct PcgPerf;
+void launchDicPrecondition(Ctx& c, Mesh& m, const LduView& A, const double* r, double* w, double* rD);
+void launchGaussSeidel(Ctx& c, Mesh& m, const LduView& A, const double* b, double* psi, int nSweeps);
+PcgPerf pcgDicSolve(Ctx& c, Mesh& m, const LduView& A, const double* b, double* psi, const double* normFactor,
+                    const PcgControls& pc);
 
 // ---- geometry (kernels_geom.hip) ----
 void launchGeometry(Ctx& c, const Mesh& m, const double* points, const int* faceOffsets, const int* faceVerts,

@@ -1,0 +1,394 @@
+// The serving loop: a stream of pressure-Poisson requests on hex boxes of uneven sizes, admitted by an
+// HBM budget into a fixed number of slots, each slot a host thread with its own context (HIP stream),
+// so admitted requests run concurrently on the device. A request keeps its mesh, matrix and AMG
+// hierarchy resident across its sequence of right-hand sides and releases them at the end. There is
+// no counterpart in the reference (its solvers run one case per process); the request shape follows
+// its pressure corrector: 7-point FP64 laplacian on the box, zeroGradient walls with a reference cell,
+// right-hand side = -div of a seeded random velocity, mixed-precision AMG-PCG.
+//
+// Results do not depend on what else is running: every kernel's arithmetic order is fixed by its
+// inputs (no atomics, per-context scratch), so a request's solution is the same bits alone or in a mix.
+#include "mfv.h"
+#include "mfv_internal.hpp"
+#include "reduce.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace mfv {
+
+namespace {
+
+constexpr int kTpb = 256;
+inline int blocks(long long n) { return (int)((n + kTpb - 1) / kTpb); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// seeded uniform(-1, 1) velocity per cell component (53-bit mantissa mapping of a counter hash)
+__global__ void velocityK(int n, uint64_t seed, double* __restrict__ U)
+{
+    const int i = blockIdx.x * kTpb + threadIdx.x;
+    if (i >= 3 * n) return;
+    const uint64_t h = mix64(seed * 0x100000001b3ull + (uint64_t)i);
+    U[i] = 2.0 * ((double)(h >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+
+// phi = interpolate(U) . Sf on internal faces (linear weights, xyz-interleaved U and Sf)
+__global__ void faceFluxUK(int nF, const int* __restrict__ own, const int* __restrict__ nei,
+                           const double* __restrict__ w, const double* __restrict__ U, const double* __restrict__ Sf,
+                           double* __restrict__ phi)
+{
+    const int f = blockIdx.x * kTpb + threadIdx.x;
+    if (f >= nF) return;
+    const int o = own[f], n = nei[f];
+    const double wf = w[f], wn = 1.0 - wf;
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k) s += (wf * U[3 * o + k] + wn * U[3 * n + k]) * Sf[3 * f + k];
+    phi[f] = s;
+}
+
+// order-independent fingerprint of an array's bits: per workgroup the wrapping sum of mixed words
+__global__ __launch_bounds__(kTpb) void bitsSumK(int n, const double* __restrict__ x, uint64_t* __restrict__ parts)
+{
+    __shared__ uint64_t lds[kTpb];
+    uint64_t s = 0;
+    for (int i = blockIdx.x * kTpb + threadIdx.x; i < n; i += gridDim.x * kTpb)
+    {
+        uint64_t b;
+        memcpy(&b, &x[i], 8);
+        s += mix64(b ^ ((uint64_t)i << 1));
+    }
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint64_t t = 0;
+        for (int k = 0; k < kTpb; ++k) t += lds[k];
+        parts[blockIdx.x] = t;
+    }
+}
+
+struct Box
+{
+    int nC = 0;
+    std::vector<int> owner, neighbour, faceOffsets, faceVerts;
+    std::vector<double> points;
+};
+
+uint64_t splitmix(uint64_t& s)
+{
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// uniform box: internal faces by owner (+x, +y, +z neighbours), then the six walls; quads with their
+// normal out of the owner; interior points jittered by +-jitter of the spacing
+Box makeBox(int nx, int ny, int nz, double lx, double ly, double lz, double jitter, uint64_t seed)
+{
+    Box b;
+    b.nC = nx * ny * nz;
+    const int px = nx + 1, py = ny + 1, pz = nz + 1;
+    auto P = [&](int i, int j, int k) { return i + px * (j + py * k); };
+    b.points.resize(3 * (size_t)px * py * pz);
+    uint64_t s = seed;
+    for (int k = 0; k < pz; ++k)
+        for (int j = 0; j < py; ++j)
+            for (int i = 0; i < px; ++i)
+            {
+                double x = lx * i / nx, y = ly * j / ny, z = lz * k / nz;
+                if (jitter > 0 && i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz)
+                {
+                    auto u = [&] { return 2.0 * ((double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0)) - 1.0; };
+                    x += jitter * lx / nx * u();
+                    y += jitter * ly / ny * u();
+                    z += jitter * lz / nz * u();
+                }
+                const size_t p = (size_t)P(i, j, k);
+                b.points[3 * p] = x;
+                b.points[3 * p + 1] = y;
+                b.points[3 * p + 2] = z;
+            }
+    auto quad = [&](int a, int bb, int c, int d) { b.faceVerts.insert(b.faceVerts.end(), {a, bb, c, d}); };
+    const size_t nI = (size_t)(nx - 1) * ny * nz + (size_t)nx * (ny - 1) * nz + (size_t)nx * ny * (nz - 1);
+    b.owner.reserve(nI + 2 * ((size_t)nx * ny + (size_t)ny * nz + (size_t)nx * nz));
+    b.neighbour.reserve(nI);
+    b.faceVerts.reserve(4 * b.owner.capacity());
+    for (int c = 0; c < b.nC; ++c)
+    {
+        const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        if (i < nx - 1)
+        {
+            b.owner.push_back(c);
+            b.neighbour.push_back(c + 1);
+            quad(P(i + 1, j, k), P(i + 1, j + 1, k), P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1));
+        }
+        if (j < ny - 1)
+        {
+            b.owner.push_back(c);
+            b.neighbour.push_back(c + nx);
+            quad(P(i, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k));
+        }
+        if (k < nz - 1)
+        {
+            b.owner.push_back(c);
+            b.neighbour.push_back(c + nx * ny);
+            quad(P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j + 1, k + 1), P(i, j + 1, k + 1));
+        }
+    }
+    for (int side = 0; side < 6; ++side)
+        for (int c = 0; c < b.nC; ++c)
+        {
+            const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+            if (side == 0 && i == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i, j, k + 1), P(i, j + 1, k + 1), P(i, j + 1, k)); }
+            if (side == 1 && i == nx - 1) { b.owner.push_back(c); quad(P(i + 1, j, k), P(i + 1, j + 1, k), P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1)); }
+            if (side == 2 && j == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i + 1, j, k), P(i + 1, j, k + 1), P(i, j, k + 1)); }
+            if (side == 3 && j == ny - 1) { b.owner.push_back(c); quad(P(i, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k)); }
+            if (side == 4 && k == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i, j + 1, k), P(i + 1, j + 1, k), P(i + 1, j, k)); }
+            if (side == 5 && k == nz - 1) { b.owner.push_back(c); quad(P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j + 1, k + 1), P(i, j + 1, k + 1)); }
+        }
+    b.faceOffsets.resize(b.owner.size() + 1);
+    for (size_t f = 0; f <= b.owner.size(); ++f) b.faceOffsets[f] = (int)(4 * f);
+    return b;
+}
+
+double now(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void check(int rc, mfv_ctx ctx, const char* what)
+{
+    if (rc != MFV_OK) throw Error(rc, std::string("mfv: serve: ") + what + ": " + mfv_last_error(ctx));
+}
+
+} // namespace
+
+// one request, start to end, on the calling thread's context
+void serveOne(mfv_ctx ctx, const mfv_serve_request& q, int precision, mfv_serve_result& res)
+{
+    const Box box = makeBox(q.nx, q.ny, q.nz, q.lx, q.ly, q.lz, q.jitter, q.seed);
+    const int nC = box.nC, nI = (int)box.neighbour.size(), nB = (int)box.owner.size() - nI;
+    const int nF = (int)box.owner.size();
+    hipStream_t st = static_cast<hipStream_t>(mfv_ctx_stream(ctx));
+    mfv_mesh mesh = nullptr;
+    mfv_amg amg = nullptr;
+    try
+    {
+        const std::vector<int> bfc(box.owner.begin() + nI, box.owner.end());
+        check(mfv_mesh_create(ctx, nC, nI, box.owner.data(), box.neighbour.data(), nB, bfc.data(), &mesh), ctx, "mesh");
+        DBuf<double> pts, Cf(3 * (size_t)nF), Sf(3 * (size_t)nF), magSf(nF), C(3 * (size_t)nC), V(nC), w(nI), dc(nI),
+            ndc(nI), corr(3 * (size_t)nI), bDelta(nB), gammaf(nI), diag(nC), upper(nI), b(nC), p(nC), U(3 * (size_t)nC),
+            phi(nI), zeroB(nB);
+        DBuf<int> off, verts;
+        pts.upload(box.points, st);
+        off.upload(box.faceOffsets, st);
+        verts.upload(box.faceVerts, st);
+        check(mfv_mesh_geometry(ctx, mesh, pts.p, off.p, verts.p, Cf.p, Sf.p, magSf.p, C.p, V.p, w.p, dc.p, ndc.p,
+                                corr.p, bDelta.p),
+              ctx, "geometry");
+        // -laplacian(1, p) held positive (brae's sign), all walls zeroGradient, reference cell 0
+        std::vector<double> minusOne(nI, -1.0);
+        gammaf.upload(minusOne, st);
+        check(mfv_laplacian_coeffs(ctx, mesh, dc.p, gammaf.p, magSf.p, diag.p, upper.p, nullptr), ctx, "laplacian");
+        hipCheck(hipMemsetAsync(b.p, 0, sizeof(double) * nC, st), "rhs clear");
+        check(mfv_set_reference(ctx, diag.p, b.p, 0, 0.0), ctx, "reference");
+        std::vector<double> hw(nI);
+        hipCheck(hipMemcpyAsync(hw.data(), magSf.p, sizeof(double) * nI, hipMemcpyDeviceToHost, st), "weights");
+        hipCheck(hipStreamSynchronize(st), "weights sync");
+        mfv_amg_controls ac{MFV_AGG_BRAE, 64, 1};
+        check(mfv_amg_build(ctx, mesh, hw.data(), &ac, &amg), ctx, "amg");
+        check(mfv_amg_galerkin(ctx, amg, diag.p, upper.p, nullptr, precision), ctx, "galerkin");
+        mfv_pcg_controls pc;
+        mfv_pcg_default_controls(&pc);
+        pc.tolerance = q.tolerance;
+        pc.precision = precision;
+        pc.scaleCorrection = 1;   // scaled correction: a nonlinear cycle, so flexible CG
+        pc.flexible = 1;
+        const mfv_ldu A{diag.p, upper.p, upper.p};
+        hipCheck(hipMemsetAsync(p.p, 0, sizeof(double) * nC, st), "p clear");
+        const int *dOwn = nullptr, *dNei = nullptr;
+        check(mfv_mesh_addressing(mesh, &dOwn, &dNei, nullptr, nullptr, nullptr), ctx, "addressing");
+        hipCheck(hipMemsetAsync(zeroB.p, 0, sizeof(double) * nB, st), "phib clear");
+        const int nParts = std::min(1024, blocks(nC));
+        DBuf<uint64_t> parts(nParts);
+        std::vector<uint64_t> hp(nParts);
+        uint64_t fp = 0;
+        res.totalIterations = 0;
+        for (int s = 0; s < q.nSteps; ++s)
+        {
+            velocityK<<<blocks(3LL * nC), kTpb, 0, st>>>(nC, q.seed * 1000003ull + (uint64_t)s, U.p);
+            faceFluxUK<<<blocks(nI), kTpb, 0, st>>>(nI, dOwn, dNei, w.p, U.p, Sf.p, phi.p);
+            hipCheck(hipMemsetAsync(b.p, 0, sizeof(double) * nC, st), "rhs clear");
+            check(mfv_div_source(ctx, mesh, phi.p, zeroB.p, V.p, -1.0, b.p), ctx, "div");
+            mfv_perf perf{};
+            check(mfv_amg_pcg_solve(ctx, mesh, amg, &A, b.p, p.p, nullptr, &pc, &perf), ctx, "solve");
+            res.totalIterations += perf.nIterations;
+            res.finalResidual = perf.finalResidual;
+            bitsSumK<<<nParts, kTpb, 0, st>>>(nC, p.p, parts.p);
+            hipCheck(hipMemcpyAsync(hp.data(), parts.p, sizeof(uint64_t) * nParts, hipMemcpyDeviceToHost, st), "fp");
+            hipCheck(hipStreamSynchronize(st), "fp sync");
+            uint64_t t = 0;
+            for (uint64_t v : hp) t += v;
+            fp = fp * 0x100000001b3ull ^ t;
+        }
+        res.fingerprint = fp;
+        res.nCells = nC;
+        res.nSteps = q.nSteps;
+    }
+    catch (...)
+    {
+        if (amg) mfv_amg_destroy(amg);
+        if (mesh) mfv_mesh_destroy(mesh);
+        throw;
+    }
+    mfv_amg_destroy(amg);
+    mfv_mesh_destroy(mesh);
+}
+
+} // namespace mfv
+
+using namespace mfv;
+
+struct mfv_server_s
+{
+    int device = 0, nSlots = 1, precision = MFV_PREC_FP32;
+    double budget = 0;
+    double bytesPerCell = 600.0;
+};
+
+extern "C" {
+
+int mfv_server_create(int device, int nSlots, double hbmBudgetBytes, int precision, mfv_server* out)
+{
+    if (!out || nSlots < 1 || nSlots > 64 || hbmBudgetBytes <= 0) return MFV_ERR_INVALID;
+    if (precision != MFV_PREC_FP64 && precision != MFV_PREC_FP32 && precision != MFV_PREC_FP16)
+        return MFV_ERR_INVALID;
+    mfv_server s = new (std::nothrow) mfv_server_s();
+    if (!s) return MFV_ERR_ALLOC;
+    s->device = device;
+    s->nSlots = nSlots;
+    s->budget = hbmBudgetBytes;
+    s->precision = precision;
+    *out = s;
+    return MFV_OK;
+}
+
+int mfv_server_destroy(mfv_server s)
+{
+    delete s;
+    return MFV_OK;
+}
+
+double mfv_server_request_bytes(mfv_server s, const mfv_serve_request* q)
+{
+    return s && q ? s->bytesPerCell * (double)q->nx * q->ny * q->nz : 0.0;
+}
+
+int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_serve_result* results)
+{
+    if (!s || nReq < 0 || (nReq && (!reqs || !results))) return MFV_ERR_INVALID;
+    for (int i = 0; i < nReq; ++i)
+    {
+        const mfv_serve_request& q = reqs[i];
+        if (q.nx < 2 || q.ny < 2 || q.nz < 2 || q.nSteps < 1 || q.tolerance <= 0 ||
+            (double)q.nx * q.ny * q.nz > 1.0e8 || mfv_server_request_bytes(s, &q) > s->budget)
+            return MFV_ERR_INVALID;
+        if (i && q.arrival < reqs[i - 1].arrival) return MFV_ERR_INVALID;   // arrivals in order
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    std::mutex mu;
+    std::condition_variable cv;
+    int next = 0;            // first request not yet admitted (FIFO)
+    double inflight = 0;     // admitted bytes
+    int idle = s->nSlots;
+    int firstError = MFV_OK;
+    std::string errorText;
+    auto worker = [&]() {
+        mfv_ctx ctx = nullptr;
+        if (mfv_ctx_create(s->device, nullptr, &ctx) != MFV_OK)
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (firstError == MFV_OK) firstError = MFV_ERR_HIP;
+            next = nReq;
+            cv.notify_all();
+            return;
+        }
+        for (;;)
+        {
+            int i;
+            double bytes;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                // admit the queue's head when it has arrived and fits in what the budget has left
+                for (;;)
+                {
+                    if (next >= nReq) break;
+                    const double wait = reqs[next].arrival - now(t0);
+                    bytes = mfv_server_request_bytes(s, &reqs[next]);
+                    if (wait <= 0 && inflight + bytes <= s->budget) break;
+                    if (wait > 0)
+                        cv.wait_for(lk, std::chrono::duration<double>(wait));
+                    else
+                        cv.wait(lk);
+                }
+                if (next >= nReq) break;
+                i = next++;
+                inflight += bytes;
+                --idle;
+                results[i].admitted = now(t0);
+                results[i].arrival = reqs[i].arrival;
+            }
+            cv.notify_all();
+            int rc = MFV_OK;
+            std::string err;
+            try
+            {
+                serveOne(ctx, reqs[i], s->precision, results[i]);
+            }
+            catch (const Error& e)
+            {
+                rc = e.code;
+                err = e.what();
+            }
+            catch (const std::exception& e)
+            {
+                rc = MFV_ERR_INVALID;
+                err = e.what();
+            }
+            {
+                std::lock_guard<std::mutex> g(mu);
+                results[i].finished = now(t0);
+                results[i].status = rc;
+                inflight -= bytes;
+                ++idle;
+                if (rc != MFV_OK && firstError == MFV_OK)
+                {
+                    firstError = rc;
+                    errorText = err;
+                }
+            }
+            cv.notify_all();
+        }
+        mfv_ctx_destroy(ctx);
+    };
+    std::vector<std::thread> threads;
+    for (int k = 0; k < s->nSlots; ++k) threads.emplace_back(worker);
+    for (auto& t : threads) t.join();
+    (void)idle;
+    return firstError;
+}
+
+} // extern "C"

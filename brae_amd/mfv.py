@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PKG, "libmfv.so")
 HEADERS = [os.path.join(ROOT, "include", h) for h in ("mfv.h",)]
 
 MFV_OK = 0
+MFV_ERR_INVALID, MFV_ERR_HIP, MFV_ERR_UNSUPPORTED, MFV_ERR_ALLOC, MFV_ERR_STATE = 1, 2, 3, 4, 5
 _lib = None
 
 P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
@@ -42,9 +43,20 @@ class PcgControls(ctypes.Structure):
 
 
 AGG_BRAE, AGG_FACEAREAPAIR = 0, 1
+CACHE_LOADED, CACHE_ABSENT, CACHE_OTHER_MESH_OR_BUILD, CACHE_UNREADABLE = range(4)
 PROF_PCG_SPMV64, PROF_PCG_UPDATE, PROF_PCG_ZR, PROF_VCYCLE, PROF_AMG_RESIDUAL0, PROF_AMG_CORR_SMOOTH0, \
     PROF_AMG_RESTRICT0, PROF_AMG_CORR_AMUL0 = range(8)
 PREC_FP64, PREC_FP32, PREC_FP16 = 64, 32, 16
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("nx", I), ("ny", I), ("nz", I), ("lx", D), ("ly", D), ("lz", D), ("jitter", D),
+                ("seed", ctypes.c_ulonglong), ("nSteps", I), ("tolerance", D), ("arrival", D)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("arrival", D), ("admitted", D), ("finished", D), ("status", I), ("nCells", I), ("nSteps", I),
+                ("totalIterations", I), ("finalResidual", D), ("fingerprint", ctypes.c_ulonglong)]
 
 
 class Perf(ctypes.Structure):
@@ -83,6 +95,10 @@ _SIGS = {
     "mfv_ctx_profile_read": [P, I, P, P],
     "mfv_amg_build": [P, P, P, P, P],
     "mfv_amg_destroy": [P],
+    "mfv_amg_signature": [I, I, P, P, P, P, P],
+    "mfv_amg_save": [P, P, ctypes.c_char_p],
+    "mfv_amg_build_or_load": [P, P, P, P, ctypes.c_char_p, I, P, P],
+    "mfv_amg_clone": [P, P, P],
     "mfv_amg_num_grids": [P, P],
     "mfv_amg_grid_size": [P, I, P, P],
     "mfv_amg_level_addressing": [P, I, P, P, P, P, P],
@@ -91,8 +107,51 @@ _SIGS = {
     "mfv_amg_vcycle": [P, P, P, P, I, I, I],
     "mfv_pcg_default_controls": [P],
     "mfv_amg_pcg_solve": [P, P, P, P, P, P, P, P, P],
+    "mfv_dic_precondition": [P, P, P, P, P],
+    "mfv_gauss_seidel": [P, P, P, P, P, I],
+    "mfv_pcg_dic_solve": [P, P, P, P, P, P, P, P],
+    "mfv_server_create": [I, I, D, I, P],
+    "mfv_server_destroy": [P],
+    "mfv_server_request_bytes": [P, P],
+    "mfv_server_run": [P, P, I, P],
 }
-_RESTYPES = {"mfv_last_error": ctypes.c_char_p, "mfv_ctx_stream": P}
+_RESTYPES = {"mfv_last_error": ctypes.c_char_p, "mfv_ctx_stream": P, "mfv_server_request_bytes": D}
+
+
+def serve(requests, nSlots=8, hbmBudgetBytes=200e9, precision=32, device=0):
+    """run a request stream (list of ServeRequest or dicts of its fields) through the serving loop;
+    returns (status, [ServeResult])"""
+    reqs = (ServeRequest * len(requests))()
+    for i, r in enumerate(requests):
+        if isinstance(r, dict):
+            for k, v in r.items():
+                setattr(reqs[i], k, v)
+        else:
+            reqs[i] = r
+    res = (ServeResult * len(requests))()
+    srv = P()
+    rc = lib().mfv_server_create(device, nSlots, float(hbmBudgetBytes), precision, ctypes.byref(srv))
+    if rc != MFV_OK:
+        raise MfvError("server_create failed (%d)" % rc)
+    try:
+        rc = lib().mfv_server_run(srv, reqs, len(requests), res)
+    finally:
+        lib().mfv_server_destroy(srv)
+    return rc, list(res)
+
+
+def amg_signature(nCells, owner, neighbour, weights, agglomeration=AGG_BRAE, nCellsInCoarsestLevel=0, mergeLevels=1):
+    """the hierarchy fingerprint (host only; no GPU needed)"""
+    import numpy as np
+    own = np.ascontiguousarray(owner, np.int32)
+    nei = np.ascontiguousarray(neighbour, np.int32)
+    w = np.ascontiguousarray(weights, np.float64)
+    sig = ctypes.c_ulonglong()
+    c = AmgControls(agglomeration, nCellsInCoarsestLevel, mergeLevels)
+    rc = lib().mfv_amg_signature(int(nCells), own.size, ptr(own), ptr(nei), ptr(w), ctypes.byref(c), ctypes.byref(sig))
+    if rc != MFV_OK:
+        raise MfvError("amg_signature failed (%d)" % rc)
+    return sig.value
 
 
 def declared_symbols():
@@ -241,20 +300,43 @@ def ldu(diag, upper, lower=None):
 class Amg:
     """AMG hierarchy of a mesh (host agglomeration, device levels)."""
 
-    def __init__(self, mesh, weights, agglomeration=AGG_BRAE, nCellsInCoarsestLevel=0, mergeLevels=1):
+    def __init__(self, mesh, weights, agglomeration=AGG_BRAE, nCellsInCoarsestLevel=0, mergeLevels=1,
+                 cacheDir=None, writeCache=False, _handle=None):
+        """build the hierarchy; with cacheDir, load <cacheDir>/.mfv_amgcache when it is this mesh's
+        (self.how tells what the file held: CACHE_*)"""
         import numpy as np
         self.ctx = mesh.ctx
         self.mesh = mesh
-        w = np.ascontiguousarray(weights, np.float64)
-        assert w.size == mesh.nFaces
-        c = AmgControls(agglomeration, nCellsInCoarsestLevel, mergeLevels)
+        self.how = None
         self.h = P()
-        self.ctx._check(lib().mfv_amg_build(self.ctx.h, mesh.h, ptr(w), ctypes.byref(c), ctypes.byref(self.h)),
-                        "amg_build")
+        if _handle is not None:
+            self.h = _handle
+        else:
+            w = np.ascontiguousarray(weights, np.float64)
+            assert w.size == mesh.nFaces
+            c = AmgControls(agglomeration, nCellsInCoarsestLevel, mergeLevels)
+            if cacheDir is None:
+                self.ctx._check(lib().mfv_amg_build(self.ctx.h, mesh.h, ptr(w), ctypes.byref(c), ctypes.byref(self.h)),
+                                "amg_build")
+            else:
+                how = ctypes.c_int(-1)
+                self.ctx._check(lib().mfv_amg_build_or_load(self.ctx.h, mesh.h, ptr(w), ctypes.byref(c),
+                                                            str(cacheDir).encode(), int(writeCache),
+                                                            ctypes.byref(self.h), ctypes.byref(how)),
+                                "amg_build_or_load")
+                self.how = how.value
         _LIVE.add(self)
         n = ctypes.c_int()
         lib().mfv_amg_num_grids(self.h, ctypes.byref(n))
         self.nGrids = n.value
+
+    def clone(self):
+        h = P()
+        self.ctx._check(lib().mfv_amg_clone(self.ctx.h, self.h, ctypes.byref(h)), "amg_clone")
+        return Amg(self.mesh, None, _handle=h)
+
+    def save(self, path):
+        self.ctx._check(lib().mfv_amg_save(self.ctx.h, self.h, str(path).encode()), "amg_save")
 
     def grid_size(self, g):
         nc, nf = ctypes.c_int(), ctypes.c_int()

@@ -189,6 +189,26 @@ typedef struct
 int mfv_amg_build(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const mfv_amg_controls* controls,
                   mfv_amg* out);
 int mfv_amg_destroy(mfv_amg amg);
+
+/* Hierarchy persistence (amgHierarchySignature, device_amg.cu:1262-1285; writeAMGCache/readAMGCache,
+ * device_amg_cache.cu:176-326; buildOrLoadAMG, device_amg.cuh:448; cloneAMG, device_amg_cache.cu:70).
+ * The structure only (levels, coarse addressing, face maps), never matrix values. */
+#define MFV_CACHE_LOADED 0
+#define MFV_CACHE_ABSENT 1
+#define MFV_CACHE_OTHER_MESH_OR_BUILD 2
+#define MFV_CACHE_UNREADABLE 3
+/* 64-bit fingerprint of what a build depends on: the internal addressing, the face weights, the
+ * controls and this library's build version. Host only, no context. */
+int mfv_amg_signature(int nCells, int nFaces, const int* hOwner, const int* hNeighbour, const double* hFaceWeights,
+                      const mfv_amg_controls* controls, unsigned long long* signature);
+/* writes the hierarchy's structure to path (a temporary file renamed over it) */
+int mfv_amg_save(mfv_ctx ctx, mfv_amg amg, const char* path);
+/* loads <cacheDir>/.mfv_amgcache when it holds this mesh's signature, else builds (and with writeCache
+ * writes the file); *how (may be NULL) says what the file held: MFV_CACHE_* */
+int mfv_amg_build_or_load(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const mfv_amg_controls* controls,
+                          const char* cacheDir, int writeCache, mfv_amg* out, int* how);
+/* a second hierarchy of the same structure on the same mesh; its values start unset (amg_galerkin) */
+int mfv_amg_clone(mfv_ctx ctx, mfv_amg src, mfv_amg* out);
 /* number of grids (the mesh plus its coarse levels) */
 int mfv_amg_num_grids(mfv_amg amg, int* nGrids);
 int mfv_amg_grid_size(mfv_amg amg, int grid, int* nCells, int* nFaces);
@@ -242,6 +262,57 @@ int mfv_pcg_default_controls(mfv_pcg_controls* controls);
  * Synchronises. */
 int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
                       const double* dNormFactor, const mfv_pcg_controls* controls, mfv_perf* perf);
+
+/* ---------------------------------------------------------------------------------------------
+ * OpenFOAM's sequential sweeps, level-scheduled on the device and equal to the host loops to the bit.
+ * ------------------------------------------------------------------------------------------- */
+/* DICPreconditioner: rD (calcReciprocalD) then w = forward/backward substitution of r
+ * (src/matrices/lduMatrix/solvers/GAMG/gamg_solver_cpp.cu:194-231; pcg.cu:6-149); synchronises */
+int mfv_dic_precondition(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dR, double* dW);
+/* nSweeps forward Gauss-Seidel sweeps on A psi = b in place (GaussSeidelSmoother,
+ * src/OpenFOAM/matrices/smooth_solver_cpp.cu:87-155; gamg.cu:187-205) */
+int mfv_gauss_seidel(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dB, double* dPsi, int nSweeps);
+/* OpenFOAM's PCG with the DIC preconditioner (the reference's `pcg`, src/OpenFOAM/matrices/pcg.cu:6-149):
+ * controls' tolerance, relTol, maxIter, minIter and checkEvery are read; dNormFactor as in
+ * mfv_amg_pcg_solve. Synchronises. */
+int mfv_pcg_dic_solve(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dB, double* dPsi,
+                      const double* dNormFactor, const mfv_pcg_controls* controls, mfv_perf* perf);
+
+/* ---------------------------------------------------------------------------------------------
+ * Serving: a stream of pressure-Poisson requests (no counterpart in the reference, which runs one
+ * case per process). Each request is a box of nx*ny*nz hex cells (lengths lx, ly, lz; interior points
+ * jittered by +-jitter of the spacing), solved nSteps times for right-hand sides -div(U) of seeded
+ * random velocities, with the 7-point FP64 laplacian, zeroGradient walls and reference cell 0, by
+ * AMG-PCG to `tolerance` in the server's cycle precision. Requests are admitted in arrival order when
+ * they have arrived (seconds after mfv_server_run starts), a slot is free and their HBM estimate fits
+ * the budget; each slot is a host thread with its own stream, so admitted requests overlap on the
+ * device. A request's mesh, matrix and hierarchy stay resident across its steps.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct mfv_server_s* mfv_server;
+typedef struct
+{
+    int nx, ny, nz;
+    double lx, ly, lz;
+    double jitter;
+    unsigned long long seed;
+    int nSteps;
+    double tolerance;
+    double arrival;
+} mfv_serve_request;
+typedef struct
+{
+    double arrival, admitted, finished;   /* seconds after the run started */
+    int status;                           /* MFV_OK or the request's error */
+    int nCells, nSteps, totalIterations;
+    double finalResidual;                 /* of the last step */
+    unsigned long long fingerprint;       /* of every step's solution bits */
+} mfv_serve_result;
+int mfv_server_create(int device, int nSlots, double hbmBudgetBytes, int precision, mfv_server* out);
+int mfv_server_destroy(mfv_server server);
+/* the HBM estimate admission charges a request */
+double mfv_server_request_bytes(mfv_server server, const mfv_serve_request* request);
+/* runs the stream to the end; returns MFV_OK or the first request error (each result has its status) */
+int mfv_server_run(mfv_server server, const mfv_serve_request* requests, int nRequests, mfv_serve_result* results);
 
 #ifdef __cplusplus
 }

@@ -1298,6 +1298,45 @@ static void of_mat_free(of_mat* m)
     free(m->diag); free(m->upper); free(m->ownStart); free(m->rD);
 }
 
+/* The GAMG restatement's own Gauss-Seidel sweeps and DIC preconditioner, on a plain symmetric system
+ * (exported so the device's level-scheduled sweeps can be held to them). */
+static of_mat of_wrap(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                      const double* upper)
+{
+    of_mat m;
+    m.nCells = nCells;
+    m.nFaces = nFaces;
+    m.lo = owner;
+    m.up = neighbour;
+    m.diag = (double*)xcalloc((size_t)nCells, sizeof(double));
+    m.upper = (double*)xcalloc((size_t)(nFaces ? nFaces : 1), sizeof(double));
+    memcpy(m.diag, diag, (size_t)nCells * sizeof(double));
+    if (nFaces) memcpy(m.upper, upper, (size_t)nFaces * sizeof(double));
+    m.ownStart = own_start(nCells, nFaces, owner);
+    m.rD = NULL;
+    return m;
+}
+
+void orc_gauss_seidel(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                      const double* upper, const double* b, double* psi, int nSweeps)
+{
+    of_mat m = of_wrap(nCells, nFaces, owner, neighbour, diag, upper);
+    of_gauss_seidel(&m, psi, b, nSweeps, 0);
+    of_mat_free(&m);
+}
+
+void orc_dic_precondition(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                          const double* upper, const double* r, double* w)
+{
+    of_mat m = of_wrap(nCells, nFaces, owner, neighbour, diag, upper);
+    double* rD = of_dic_rd(&m);
+    for (int c = 0; c < nCells; ++c) w[c] = rD[c] * r[c];
+    of_dic_substitute(&m, rD, w);
+    free(rD);
+    of_mat_free(&m);
+}
+
+
 static void of_hier_free(of_hier* h)
 {
     of_mat_free(&h->fine);

@@ -158,6 +158,13 @@ orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* up
                          int nPre, int nPost, int flexible, int single, int scale, double normFactorIn,
                          double* psiTrace);
 
+/* OpenFOAM's GaussSeidel sweeps (smooth_solver_cpp.cu:87-155) and DIC preconditioner (calcReciprocalD +
+ * precondition, gamg_solver_cpp.cu:194-231) on a folded symmetric system, as the GAMG restatement runs them */
+void orc_gauss_seidel(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                      const double* upper, const double* b, double* psi, int nSweeps);
+void orc_dic_precondition(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                          const double* upper, const double* r, double* w);
+
 /* one brae V-cycle z = M^-1 r on the hierarchy (same cycle as orc_brae_amgpcg applies) */
 void orc_brae_vcycle(const orc_aggl* a, const double* diag, const double* upper, const double* r, double* z,
                      int nPre, int nPost, int single, int scale);

@@ -56,6 +56,8 @@ def lib():
             "orc_sum_prod": (D, [I, P, P]),
             "orc_face_flux": (None, [I, P, P, P, P, P, P]),
             "orc_grad_gauss": (None, [I, I, P, P, I, P, P, P, P, P, P, P, P]),
+            "orc_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
+            "orc_dic_precondition": (None, [I, I, P, P, P, P, P, P]),
             "orc_laplacian_corr": (None, [I, I, P, P, P, P, P, P, P, P, P]),
             "orc_flux_correct": (None, [I, P, P, P, P, P, P, P, P]),
             "orc_pcg_dic": (Perf, [I, I, P, P, P, P, P, P, D, D, I, I]),
@@ -210,6 +212,19 @@ def flux_correct(own, nei, upper, lower, psi, ffc, phiHbyA):
     lib().orc_flux_correct(len(own), p(i32(own)), p(i32(nei)), p(f64(upper)), p(None if lower is None else f64(lower)),
                            p(f64(psi)), p(None if ffc is None else f64(ffc)), p(f64(phiHbyA)), p(phi))
     return phi
+
+
+def gauss_seidel(nC, own, nei, diag, upper, b, psi, nSweeps):
+    psi = f64(psi).copy()
+    lib().orc_gauss_seidel(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(diag)), p(f64(upper)), p(f64(b)), p(psi),
+                           nSweeps)
+    return psi
+
+
+def dic_precondition(nC, own, nei, diag, upper, r):
+    w = np.zeros(nC)
+    lib().orc_dic_precondition(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(diag)), p(f64(upper)), p(f64(r)), p(w))
+    return w
 
 
 def amul(nC, own, nei, diag, upper, lower, psi):

@@ -35,3 +35,24 @@ def test_context_without_gpu_fails_loudly():
     rc = mfv.lib().mfv_ctx_create(0, None, ctypes.byref(h))
     assert rc != 0
     assert b"hip" in (mfv.lib().mfv_last_error(None) or b"")
+
+
+def test_amg_signature_fingerprints_inputs():
+    """the hierarchy fingerprint (host only): stable for the same inputs, different when the addressing,
+    a weight or a control changes"""
+    import numpy as np
+    from brae_amd import hexmesh
+    m = hexmesh.box_mesh(6, 5, 4, geometry=True)
+    nI = m["nInternal"]
+    own, nei, w = m["owner"][:nI], m["neighbour"], m["magSf"][:nI]
+    s0 = mfv.amg_signature(m["nCells"], own, nei, w)
+    assert s0 == mfv.amg_signature(m["nCells"], own, nei, w.copy())
+    w2 = w.copy()
+    w2[7] = np.nextafter(w2[7], 2.0)
+    assert mfv.amg_signature(m["nCells"], own, nei, w2) != s0
+    nei2 = nei.copy()
+    nei2[[3, 4]] = nei2[[4, 3]]
+    assert mfv.amg_signature(m["nCells"], own, nei2, w) != s0
+    assert mfv.amg_signature(m["nCells"], own, nei, w, mergeLevels=2) != s0
+    assert mfv.amg_signature(m["nCells"], own, nei, w, agglomeration=mfv.AGG_FACEAREAPAIR) != s0
+    assert mfv.amg_signature(m["nCells"], own, nei, w, nCellsInCoarsestLevel=8) != s0

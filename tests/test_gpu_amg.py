@@ -279,3 +279,51 @@ def test_merged_levels(systems):
         x = G.host(psi)
         assert np.abs(x - ref_psi).max() <= 1e-7 * np.abs(ref_psi).max(), S.label
         amg.close()
+
+
+def test_hierarchy_cache_and_clone(systems, tmp_path):
+    """build_or_load: absent -> built and written; loaded -> the same levels and the same solve to the bit;
+    other weights -> rebuilt; a damaged file -> unreadable and rebuilt; clone -> the same structure and
+    solve"""
+    S = systems[1]
+    d = str(tmp_path)
+
+    def solve(amg):
+        amg.galerkin(S.dd, S.du, mfv.PREC_FP32)
+        psi = G.zeros(S.nC)
+        perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-9, precision=mfv.PREC_FP32, flexible=1,
+                         scaleCorrection=1)
+        return perf.nIterations, G.host(psi)
+
+    def levels(amg):
+        return [amg.level(i) for i in range(amg.nGrids - 1)]
+
+    a = mfv.Amg(S.case.mesh, S.magSf, mfv.AGG_BRAE, 64, cacheDir=d, writeCache=True)
+    assert a.how == mfv.CACHE_ABSENT and os.path.exists(os.path.join(d, ".mfv_amgcache"))
+    b = mfv.Amg(S.case.mesh, S.magSf, mfv.AGG_BRAE, 64, cacheDir=d)
+    assert b.how == mfv.CACHE_LOADED and b.nGrids == a.nGrids
+    for la, lb in zip(levels(a), levels(b)):
+        for k in ("map", "cOwn", "cNei", "faceRestrict", "faceFlip"):
+            assert np.array_equal(la[k], lb[k])
+    ia, xa = solve(a)
+    ib, xb = solve(b)
+    assert ia == ib and np.array_equal(xa, xb)
+    w2 = S.magSf.copy()
+    w2[0] *= 1.5
+    c = mfv.Amg(S.case.mesh, w2, mfv.AGG_BRAE, 64, cacheDir=d)
+    assert c.how == mfv.CACHE_OTHER_MESH_OR_BUILD
+    m2 = mfv.Amg(S.case.mesh, S.magSf, mfv.AGG_BRAE, 64, mergeLevels=2, cacheDir=d)
+    assert m2.how == mfv.CACHE_OTHER_MESH_OR_BUILD and m2.nGrids < a.nGrids
+    path = os.path.join(d, ".mfv_amgcache")
+    raw = open(path, "rb").read()
+    open(path, "wb").write(raw[: len(raw) // 2])
+    e = mfv.Amg(S.case.mesh, S.magSf, mfv.AGG_BRAE, 64, cacheDir=d)
+    assert e.how == mfv.CACHE_UNREADABLE and e.nGrids == a.nGrids
+    a.save(path)
+    f = mfv.Amg(S.case.mesh, S.magSf, mfv.AGG_BRAE, 64, cacheDir=d)
+    assert f.how == mfv.CACHE_LOADED
+    k = a.clone()
+    ik, xk = solve(k)
+    assert ik == ia and np.array_equal(xk, xa)
+    for o in (a, b, c, m2, e, f, k):
+        o.close()

@@ -1,0 +1,52 @@
+"""The serving loop: requests of uneven sizes admitted by an HBM budget into concurrent slots.
+
+Each request's solution is the same bits alone and in a mix (fingerprint of every step's p), every
+request converges, admission follows arrival order and the budget, and the requests really overlap."""
+import pytest
+
+from brae_amd import mfv
+
+pytestmark = pytest.mark.gpu
+
+
+def req(n, aspect=(1.0, 1.0, 1.0), jitter=0.0, seed=1, steps=3, arrival=0.0):
+    nx, ny, nz = n
+    return dict(nx=nx, ny=ny, nz=nz, lx=aspect[0], ly=aspect[1], lz=aspect[2], jitter=jitter, seed=seed,
+                nSteps=steps, tolerance=1e-6, arrival=arrival)
+
+
+MIX = [req((40, 30, 20), (2.0, 1.0, 1.0), 0.1, seed=11, steps=3),
+       req((24, 24, 24), seed=12, steps=2, arrival=0.01),
+       req((64, 32, 16), (4.0, 1.0, 0.5), 0.1, seed=13, steps=4, arrival=0.02),
+       req((20, 20, 60), (1.0, 1.0, 3.0), seed=14, steps=2, arrival=0.02),
+       req((50, 25, 25), (8.0, 1.0, 1.0), 0.1, seed=15, steps=3, arrival=0.03),
+       req((16, 16, 16), seed=16, steps=5, arrival=0.03)]
+
+
+def test_results_identical_alone_and_in_a_mix():
+    print("case: serving mix of", len(MIX), flush=True)
+    rc, mixed = mfv.serve(MIX, nSlots=4, hbmBudgetBytes=50e9)
+    assert rc == mfv.MFV_OK
+    for r in mixed:
+        assert r.status == 0 and r.finalResidual < 1e-6 and r.totalIterations > 0
+    # overlap: some request was admitted before another finished
+    spans = sorted((r.admitted, r.finished) for r in mixed)
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    for i in (0, 2, 4):
+        rc, alone = mfv.serve([dict(MIX[i], arrival=0.0)], nSlots=1, hbmBudgetBytes=50e9)
+        assert rc == mfv.MFV_OK
+        assert alone[0].fingerprint == mixed[i].fingerprint, i
+        assert alone[0].totalIterations == mixed[i].totalIterations
+
+
+def test_admission_by_budget():
+    """a budget that holds one request at a time serialises them in arrival order"""
+    reqs = [req((30, 30, 30), seed=21 + k, steps=1, arrival=0.0) for k in range(3)]
+    budget = 1.5 * 600.0 * 27000
+    rc, res = mfv.serve(reqs, nSlots=3, hbmBudgetBytes=budget)
+    assert rc == mfv.MFV_OK
+    for a, b in zip(res, res[1:]):
+        assert b.admitted >= a.finished - 1e-6
+    # a request larger than the whole budget is refused up front
+    rc, _ = mfv.serve([req((40, 40, 40), seed=3)], nSlots=1, hbmBudgetBytes=budget)
+    assert rc == mfv.MFV_ERR_INVALID
