@@ -145,6 +145,50 @@ void needMesh(mfv_ctx ctx, mfv_mesh mesh)
     need(mesh->ctx == ctx, "mesh belongs to another context");
 }
 
+void putPerf(const PcgPerf& r, mfv_perf* perf)
+{
+    if (!perf) return;
+    perf->initialResidual = r.initialResidual;
+    perf->finalResidual = r.finalResidual;
+    perf->nIterations = r.nIterations;
+    perf->converged = r.converged ? 1 : 0;
+}
+
+GamgControls gamgControls(const mfv_gamg_controls& c)
+{
+    if (c.smoother < 0 || c.smoother > 3)
+        throw Error(kErrUnsupported, "mfv: gamg: smoother " + std::to_string(c.smoother) +
+                                         " is not carried (GaussSeidel, symGaussSeidel, DIC, DICGaussSeidel)");
+    GamgControls g;
+    g.smoother = (OfSmoother)c.smoother;
+    g.tolerance = c.tolerance;
+    g.relTol = c.relTol;
+    g.maxIter = c.maxIter;
+    g.minIter = c.minIter;
+    g.nPreSweeps = c.nPreSweeps;
+    g.preSweepsLevelMultiplier = c.preSweepsLevelMultiplier;
+    g.maxPreSweeps = c.maxPreSweeps;
+    g.nPostSweeps = c.nPostSweeps;
+    g.postSweepsLevelMultiplier = c.postSweepsLevelMultiplier;
+    g.maxPostSweeps = c.maxPostSweeps;
+    g.nFinestSweeps = c.nFinestSweeps;
+    g.scaleCorrection = c.scaleCorrection != 0;
+    g.nVcycles = c.nVcycles;
+    return g;
+}
+
+LduView gamgArgs(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, const double* dPsi,
+                 const mfv_gamg_controls* controls)
+{
+    needMesh(ctx, mesh);
+    need(amg && amg->ctx == ctx && amg->mesh == mesh, "gamg: hierarchy of another mesh or context");
+    need(dB && dPsi && controls && dB != dPsi, "gamg: null or aliased argument");
+    const LduView v = view(mesh, A);
+    // the reference refuses an asymmetric matrix (the smoothers and the coarsest solver differ)
+    if (v.lower != v.upper) throw Error(kErrUnsupported, "mfv: gamg: an asymmetric matrix is not carried");
+    return v;
+}
+
 } // namespace
 
 extern "C" {
@@ -655,6 +699,61 @@ int mfv_gauss_seidel(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double*
         need(dB && dPsi && dB != dPsi, "gauss-seidel: null or aliased argument");
         need(nSweeps >= 0, "gauss-seidel: negative sweep count");
         launchGaussSeidel(ctx->c, mesh->m, view(mesh, A), dB, dPsi, nSweeps);
+    });
+}
+
+int mfv_sym_gauss_seidel(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dB, double* dPsi, int nSweeps)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dB && dPsi && dB != dPsi, "sym-gauss-seidel: null or aliased argument");
+        need(nSweeps >= 0, "sym-gauss-seidel: negative sweep count");
+        DBuf<double> bPrime(mesh->m.nCells);
+        launchGaussSeidel(ctx->c, mesh->m, view(mesh, A), dB, dPsi, nSweeps, bPrime.p);
+        hipCheck(hipStreamSynchronize(ctx->c.stream), "sym-gauss-seidel sync");
+    });
+}
+
+int mfv_gamg_default_controls(mfv_gamg_controls* c)
+{
+    if (!c) return MFV_ERR_INVALID;
+    const GamgControls d;
+    c->smoother = (int)d.smoother;
+    c->tolerance = d.tolerance;
+    c->relTol = d.relTol;
+    c->maxIter = d.maxIter;
+    c->minIter = d.minIter;
+    c->nPreSweeps = d.nPreSweeps;
+    c->preSweepsLevelMultiplier = d.preSweepsLevelMultiplier;
+    c->maxPreSweeps = d.maxPreSweeps;
+    c->nPostSweeps = d.nPostSweeps;
+    c->postSweepsLevelMultiplier = d.postSweepsLevelMultiplier;
+    c->maxPostSweeps = d.maxPostSweeps;
+    c->nFinestSweeps = d.nFinestSweeps;
+    c->scaleCorrection = d.scaleCorrection ? 1 : 0;
+    c->nVcycles = d.nVcycles;
+    return MFV_OK;
+}
+
+int mfv_gamg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
+                   const mfv_gamg_controls* controls, mfv_perf* perf)
+{
+    return guarded(ctx, [&] {
+        const LduView v = gamgArgs(ctx, mesh, amg, A, dB, dPsi, controls);
+        const PcgPerf r = gamgSolve(ctx->c, mesh->m, amg->a, v.diag, v.upper, dB, dPsi, gamgControls(*controls));
+        putPerf(r, perf);
+    });
+}
+
+int mfv_pcg_gamg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
+                       double tolerance, double relTol, int maxIter, int minIter, const mfv_gamg_controls* controls,
+                       mfv_perf* perf)
+{
+    return guarded(ctx, [&] {
+        const LduView v = gamgArgs(ctx, mesh, amg, A, dB, dPsi, controls);
+        const PcgPerf r = pcgGamgSolve(ctx->c, mesh->m, amg->a, v.diag, v.upper, dB, dPsi, tolerance, relTol, maxIter,
+                                       minIter, gamgControls(*controls));
+        putPerf(r, perf);
     });
 }
 

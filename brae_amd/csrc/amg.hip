@@ -22,6 +22,7 @@ namespace mfv {
 namespace {
 
 constexpr int kTpb = 256;
+constexpr int kMaxDenseCoarsest = 1024;   // the dense inverse's column buffer in LDS
 inline int blocks(long long n) { return (int)((n + kTpb - 1) / kTpb); }
 
 // ---------------- V-cycle kernels ----------------
@@ -531,7 +532,7 @@ __global__ __launch_bounds__(kTpb) void coarseInverseK(int n, int nf, const int*
 {
     const int tid = threadIdx.x;
     const int w = 2 * n;
-    __shared__ double fcol[1024];
+    __shared__ double fcol[kMaxDenseCoarsest];
     __shared__ double shScale, shPiv;
     __shared__ int shRow;
     for (int i = tid; i < n * w; i += kTpb) M[i] = 0.0;
@@ -977,15 +978,11 @@ void amgBuildFrom(Ctx& c, const Mesh& m, Amg& a, std::vector<AggLevel> levels, c
         if (gi < nG - 1 || gi == 0) buildSell(c, g.sell, g.n, own, nei);
     }
     a.nCoarsest = a.grids.back().n;
-    if (a.nCoarsest > 1024) invalid("amg build: the coarsest grid has " + std::to_string(a.nCoarsest) +
-                                    " cells; the dense coarsest solve takes at most 1024");
     Grid& gl = a.grids.back();
     gl.diag64.resize(gl.n);
     gl.upper64.resize(gl.nFaces);
     gl.denseOwn.upload(gl.hOwn, c.stream);
     gl.denseNei.upload(gl.hNei, c.stream);
-    a.coarseInv.resize((size_t)a.nCoarsest * a.nCoarsest);
-    a.coarseWork.resize((size_t)a.nCoarsest * 2 * a.nCoarsest);
     a.scales.resize(nG);
     std::vector<double> ones(nG, 1.0);
     a.scales.upload(ones, c.stream);
@@ -1049,7 +1046,7 @@ void buildTail(Ctx& c, Amg& a)
 
 } // namespace
 
-void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper, Prec prec)
+void amgCoarseMatrices(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper)
 {
     const int nG = (int)a.grids.size();
     if (nG < 2) invalid("amg: the mesh is too small for a coarse grid");
@@ -1066,6 +1063,19 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
                                                                 gc.upper64.p);
     }
     hipCheck(hipGetLastError(), "galerkin");
+}
+
+void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper, Prec prec)
+{
+    // the brae cycle's coarsest solve is a dense inverse in one workgroup (OpenFOAM's GAMG solves any size)
+    if (a.nCoarsest > kMaxDenseCoarsest)
+        throw Error(kErrUnsupported, "mfv: amg galerkin: the coarsest grid has " + std::to_string(a.nCoarsest) +
+                                         " cells; the dense coarsest solve takes at most " +
+                                         std::to_string(kMaxDenseCoarsest));
+    a.coarseInv.resize((size_t)a.nCoarsest * a.nCoarsest);
+    a.coarseWork.resize((size_t)a.nCoarsest * 2 * a.nCoarsest);
+    amgCoarseMatrices(c, a, fineDiag, fineUpper);
+    const int nG = (int)a.grids.size();
     // V-cycle copies of every non-coarsest grid in the working precision
     for (int gi = 0; gi + 1 < nG; ++gi)
     {

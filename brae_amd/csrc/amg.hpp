@@ -86,6 +86,18 @@ struct PcgWork
     DBuf<double> seqA, seqB;   // per-element terms for sequential (OpenFOAM-order) reductions
 };
 
+// Device state of OpenFOAM's GAMG cycle on a hierarchy (of_gamg.hip): every coarse grid's LDU
+// addressing, its correction and source, the DIC smoother's reciprocal diagonals, and the solve's
+// FP64 work vectors.
+struct OfWork
+{
+    std::vector<std::unique_ptr<Mesh>> mesh;        // [grid], grids >= 1
+    std::vector<DBuf<double>> corr, src, rD;        // [grid]: coarseCorrFields, coarseSources, rD (grid 0: rD only)
+    DBuf<double> ACf, preSmoothed, tR, tW, bPrime;  // fine-sized scratch
+    DBuf<double> Apsi, fCorr, fRes, pA, wA, rA, AwA;
+    DBuf<double> st;                                // device scalars
+};
+
 struct Amg
 {
     int nFine = 0;
@@ -107,6 +119,7 @@ struct Amg
     bool allocatedVectors = false;
     Prec vectorsFor = Prec::F32;
     std::unique_ptr<PcgWork> work;
+    std::unique_ptr<OfWork> of;
 };
 
 // Build the hierarchy (host agglomeration + device structures).
@@ -127,6 +140,11 @@ CacheRead amgReadLevels(const std::string& path, uint64_t signature, std::vector
 // Coarse matrices from the fine FP64 matrix (symmetric: lower == upper), V-cycle copies in `prec`,
 // the coarsest inverse.
 void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper, Prec prec);
+// The FP64 coarse matrices alone (every grid's diag64/upper64), as OpenFOAM's GAMGSolver agglomerates
+// its matrix (GAMGSolverAgglomerateMatrix.C): each coarse cell's diag is its fine cells' diags in
+// ascending order, then twice each fine face inside it, ascending; each coarse face the sum of its fine
+// faces, ascending.
+void amgCoarseMatrices(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper);
 
 struct VcycleControls
 {
@@ -170,6 +188,31 @@ struct PcgPerf
     int nIterations = 0;
     bool converged = false;
 };
+
+// ---- OpenFOAM's GAMG (src/matrices/lduMatrix/solvers/GAMG/gamg_solver_cpp.cuh:67-85, GamgControls) ----
+enum class OfSmoother : int { GaussSeidel = 0, SymGaussSeidel = 1, DIC = 2, DICGaussSeidel = 3 };
+struct GamgControls
+{
+    OfSmoother smoother = OfSmoother::GaussSeidel;
+    double tolerance = 1e-6, relTol = 0.0;
+    int maxIter = 1000, minIter = 0;
+    int nPreSweeps = 0, preSweepsLevelMultiplier = 1, maxPreSweeps = 4;
+    int nPostSweeps = 2, postSweepsLevelMultiplier = 1, maxPostSweeps = 4;
+    int nFinestSweeps = 2;
+    bool scaleCorrection = true;
+    int nVcycles = 2;   // per preconditioner application (GamgPreconditionerControls)
+};
+
+// GAMGSolver::solve on a folded symmetric system (gamgSolveFolded, gamg_solver_cpp.cu:979-1023):
+// builds the FP64 coarse matrices from (diag, upper), then V-cycles until converged. psi in place.
+// Synchronises once per V-cycle (the stopping test).
+PcgPerf gamgSolve(Ctx& c, Mesh& m, Amg& a, const double* diag, const double* upper, const double* b, double* psi,
+                  const GamgControls& g);
+// PCG with the GAMG preconditioner (pcgGamgSolve, gamg_solver_cpp.cu:1027-1143): each application runs
+// g.nVcycles cycles from zero; tolerance..minIter are the PCG's, g's tolerance/relTol the coarsest
+// solve's. Synchronises once per iteration.
+PcgPerf pcgGamgSolve(Ctx& c, Mesh& m, Amg& a, const double* diag, const double* upper, const double* b, double* psi,
+                     double tolerance, double relTol, int maxIter, int minIter, const GamgControls& g);
 
 // AMG-preconditioned CG on the folded fine matrix (diag, upper symmetric). psi updated in place.
 // normFactor: device scalar, or null to compute OpenFOAM's normFactor here.

@@ -118,15 +118,39 @@ __global__ __launch_bounds__(kTpb) void gsFwdK(int cnt, const int* __restrict__ 
                                                const label* __restrict__ losortStart, const label* __restrict__ losort,
                                                const label* __restrict__ losortOwner, const double* __restrict__ diag,
                                                const double* __restrict__ upper, const double* __restrict__ b,
-                                               double* __restrict__ psi)
+                                               double* __restrict__ psi, double* __restrict__ bPrime)
 {
     const int i = blockIdx.x * kTpb + threadIdx.x;
     if (i >= cnt) return;
     const int c = cells[i];
     double t = b[c];
     for (int k = losortStart[c]; k < losortStart[c + 1]; ++k) t -= upper[losort[k]] * psi[losortOwner[k]];
+    if (bPrime) bPrime[c] = t;
     for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f) t -= upper[f] * psi[neighbour[f]];
     psi[c] = t / diag[c];
+}
+
+// symGaussSeidel's reverse walk (smooth_solver_cpp.cu:141-163), cells on the backward schedule: the
+// forward walk's bPrime[c] (b less the lower neighbours' forward values), less the upper neighbours'
+// new values in ascending face order, over diag[c]
+__global__ __launch_bounds__(kTpb) void gsBwdK(int cnt, const int* __restrict__ cells,
+                                               const label* __restrict__ ownerStart,
+                                               const label* __restrict__ neighbour, const double* __restrict__ diag,
+                                               const double* __restrict__ upper,
+                                               const double* __restrict__ bPrime, double* __restrict__ psi)
+{
+    const int i = blockIdx.x * kTpb + threadIdx.x;
+    if (i >= cnt) return;
+    const int c = cells[i];
+    double t = bPrime[c];
+    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f) t -= upper[f] * psi[neighbour[f]];
+    psi[c] = t / diag[c];
+}
+
+__global__ __launch_bounds__(kTpb) void addToK(int n, const double* __restrict__ a, double* __restrict__ y)
+{
+    const int c = blockIdx.x * kTpb + threadIdx.x;
+    if (c < n) y[c] += a[c];
 }
 
 __global__ __launch_bounds__(kTpb) void subK(int n, const double* __restrict__ b, const double* __restrict__ ax,
@@ -137,19 +161,22 @@ __global__ __launch_bounds__(kTpb) void subK(int n, const double* __restrict__ b
 }
 
 // ---- the PCG-DIC recurrence (pcg.cu:6-149); device state in st/fl ----
-enum : int { D_RHO = 0, D_RHOOLD, D_PQ, D_NF, D_INIT, D_FINAL, D_TOL, D_RELTOL, D_T0, D_T1, D_T2, D_COUNT = 16 };
+enum : int { D_RHO = 0, D_RHOOLD, D_PQ, D_NF, D_INIT, D_FINAL, D_TOL, D_RELTOL, D_T0, D_T1, D_T2, D_RELMIN,
+             D_COUNT = 16 };
 enum : int { G_ITER = 0, G_STOP, G_CONV, G_MAXIT, G_MINIT, G_COUNT = 8 };
 
 __device__ __forceinline__ bool dicConverged(const double* st, double res)
 {
-    return res < st[D_TOL] || (st[D_RELTOL] > 0.0 && res < st[D_RELTOL] * st[D_INIT]);
+    return res < st[D_TOL] || (st[D_RELTOL] > st[D_RELMIN] && res < st[D_RELTOL] * st[D_INIT]);
 }
 
-__global__ void dicSetupK(double* st, int* fl, double tol, double relTol, int maxIter, int minIter, const double* nf)
+__global__ void dicSetupK(double* st, int* fl, double tol, double relTol, double relMin, int maxIter, int minIter,
+                          const double* nf)
 {
     if (threadIdx.x || blockIdx.x) return;
     st[D_TOL] = tol;
     st[D_RELTOL] = relTol;
+    st[D_RELMIN] = relMin;
     if (nf) st[D_NF] = *nf;
     fl[G_MAXIT] = maxIter;
     fl[G_MINIT] = minIter;
@@ -189,6 +216,13 @@ __global__ __launch_bounds__(kTpb) void dicDirK(int n, const double* __restrict_
         const double beta = st[D_RHO] / st[D_RHOOLD];
         p[c] = w[c] + beta * p[c];
     }
+}
+
+// the guarded solver's singularity test: no update, no count, stop
+__global__ void dicGuardK(double* st, int* fl)
+{
+    if (threadIdx.x || blockIdx.x || fl[G_STOP]) return;
+    if (fabs(st[D_T1]) / st[D_NF] < 1e-300) fl[G_STOP] = 1;
 }
 
 __global__ __launch_bounds__(kTpb) void dicUpdK(int n, const double* __restrict__ p, const double* __restrict__ q,
@@ -256,25 +290,58 @@ void launchDicPrecondition(Ctx& c, Mesh& m, const LduView& A, const double* r, d
     hipCheck(hipGetLastError(), "dic");
 }
 
-void launchGaussSeidel(Ctx& c, Mesh& m, const LduView& A, const double* b, double* psi, int nSweeps)
+void launchGaussSeidel(Ctx& c, Mesh& m, const LduView& A, const double* b, double* psi, int nSweeps, double* bPrime)
 {
     if (A.lower != A.upper) throw Error(kErrUnsupported, "mfv: Gauss-Seidel here takes a symmetric matrix");
     if (!m.nCells) return;
     buildLevels(c, m);
     const LevelSched& s = *m.sched;
     for (int sw = 0; sw < nSweeps; ++sw)
+    {
         for (size_t l = 0; l + 1 < s.fwdStart.size(); ++l)
         {
             const int a = s.fwdStart[l], cnt = s.fwdStart[l + 1] - a;
             gsFwdK<<<blocks(cnt), kTpb, 0, c.stream>>>(cnt, s.fwdCells.p + a, m.ownerStart.p, m.neighbour.p,
                                                        m.losortStart.p, m.losort.p, m.losortOwner.p, A.diag, A.upper,
-                                                       b, psi);
+                                                       b, psi, bPrime);
         }
+        if (!bPrime) continue;
+        for (size_t l = 0; l + 1 < s.bwdStart.size(); ++l)
+        {
+            const int a = s.bwdStart[l], cnt = s.bwdStart[l + 1] - a;
+            gsBwdK<<<blocks(cnt), kTpb, 0, c.stream>>>(cnt, s.bwdCells.p + a, m.ownerStart.p, m.neighbour.p, A.diag,
+                                                       A.upper, bPrime, psi);
+        }
+    }
     hipCheck(hipGetLastError(), "gauss-seidel");
 }
 
+void launchDicFactor(Ctx& c, Mesh& m, const LduView& A, double* rD)
+{
+    if (A.lower != A.upper) throw Error(kErrUnsupported, "mfv: DIC needs a symmetric matrix (lower == upper)");
+    if (!m.nCells) return;
+    buildLevels(c, m);
+    dicFactor(c, m, A, rD, nullptr);
+    hipCheck(hipGetLastError(), "dic factor");
+}
+
+void launchDicSmooth(Ctx& c, Mesh& m, const LduView& A, const double* rD, const double* b, double* psi, int nSweeps,
+                     double* rA, double* w)
+{
+    if (A.lower != A.upper) throw Error(kErrUnsupported, "mfv: the DIC smoother needs a symmetric matrix");
+    if (!m.nCells) return;
+    buildLevels(c, m);
+    for (int sw = 0; sw < nSweeps; ++sw)
+    {
+        launchResidual(c, m, A, psi, b, rA);
+        dicApply(c, m, A, rD, rA, w, nullptr);
+        addToK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, w, psi);
+    }
+    hipCheck(hipGetLastError(), "dic smoother");
+}
+
 PcgPerf pcgDicSolve(Ctx& c, Mesh& m, const LduView& A, const double* b, double* psi, const double* normFactor,
-                    const PcgControls& pc)
+                    const PcgControls& pc, bool guarded)
 {
     if (A.lower != A.upper) throw Error(kErrUnsupported, "mfv: PCG-DIC solves symmetric matrices (lower == upper)");
     const int n = m.nCells;
@@ -284,7 +351,8 @@ PcgPerf pcgDicSolve(Ctx& c, Mesh& m, const LduView& A, const double* b, double* 
     double* S = st.p;
     int* F = fl.p;
     const int* stop = F + G_STOP;
-    dicSetupK<<<1, 1, 0, c.stream>>>(S, F, pc.tolerance, pc.relTol, pc.maxIter, pc.minIter, normFactor);
+    dicSetupK<<<1, 1, 0, c.stream>>>(S, F, pc.tolerance, pc.relTol, guarded ? 1e-20 : 0.0, pc.maxIter,
+                                     guarded ? 0 : pc.minIter, normFactor);
     // r = b - A psi (the host loop's Amul, then the subtraction)
     launchAmul(c, m, A, psi, q.p);
     subK<<<blocks(n), kTpb, 0, c.stream>>>(n, b, q.p, r.p);
@@ -301,6 +369,7 @@ PcgPerf pcgDicSolve(Ctx& c, Mesh& m, const LduView& A, const double* b, double* 
         dicDirK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.p, p.p, S, F);
         launchAmul(c, m, A, p.p, q.p);
         launchDot(c, n, q.p, p.p, S + D_T1, 1);
+        if (guarded) dicGuardK<<<1, 1, 0, c.stream>>>(S, F);
         dicUpdK<<<blocks(n), kTpb, 0, c.stream>>>(n, p.p, q.p, psi, r.p, S, F);
         launchSumMag(c, n, r.p, S + D_T2, 2);
         dicResK<<<1, 1, 0, c.stream>>>(S, F);

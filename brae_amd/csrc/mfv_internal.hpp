@@ -168,6 +168,10 @@ struct Mesh
     std::unique_ptr<LevelSched> sched;   // built on first use by the sequential sweeps (DIC, Gauss-Seidel)
 };
 
+// ownerStart, stable losort by neighbour, boundary gather (abi.cpp)
+void buildMesh(Ctx& c, Mesh& m, int nCells, int nFaces, const int* owner, const int* neighbour, int nB,
+               const int* bFaceCell);
+
 // ---- reductions / BLAS (kernels_blas.hip) ----
 void launchDot(Ctx& c, int n, const double* a, const double* b, double* out, int slot);
 void launchSumMag(Ctx& c, int n, const double* a, double* out, int slot);
@@ -210,9 +214,18 @@ void launchLaplacianCorr(Ctx& c, const Mesh& m, const double* grad, const double
 struct PcgControls;
 struct PcgPerf;
 void launchDicPrecondition(Ctx& c, Mesh& m, const LduView& A, const double* r, double* w, double* rD);
-void launchGaussSeidel(Ctx& c, Mesh& m, const LduView& A, const double* b, double* psi, int nSweeps);
+// bPrime (n, may be null): symGaussSeidel, each sweep followed by the reverse walk from the forward walk's bPrime
+void launchGaussSeidel(Ctx& c, Mesh& m, const LduView& A, const double* b, double* psi, int nSweeps,
+                       double* bPrime = nullptr);
+// DIC's reciprocal diagonal (calcReciprocalD)
+void launchDicFactor(Ctx& c, Mesh& m, const LduView& A, double* rD);
+// DICSmoother::smooth (gamg_solver_cpp.cu:277-293): per sweep psi += DIC(b - A psi); rA, w: n scratch each
+void launchDicSmooth(Ctx& c, Mesh& m, const LduView& A, const double* rD, const double* b, double* psi, int nSweeps,
+                     double* rA, double* w);
+// guarded: OpenFOAM GAMG's coarsest-level PCG-DIC (gamg_solver_cpp.cu:321-383): stop without an update
+// when |p.Ap|/normFactor < VSMALL, relTol counted only above 1e-20, no minIter
 PcgPerf pcgDicSolve(Ctx& c, Mesh& m, const LduView& A, const double* b, double* psi, const double* normFactor,
-                    const PcgControls& pc);
+                    const PcgControls& pc, bool guarded = false);
 
 // ---- geometry (kernels_geom.hip) ----
 void launchGeometry(Ctx& c, const Mesh& m, const double* points, const int* faceOffsets, const int* faceVerts,

@@ -49,6 +49,30 @@ PROF_PCG_SPMV64, PROF_PCG_UPDATE, PROF_PCG_ZR, PROF_VCYCLE, PROF_AMG_RESIDUAL0, 
 PREC_FP64, PREC_FP32, PREC_FP16 = 64, 32, 16
 
 
+SMOOTHER_GAUSS_SEIDEL, SMOOTHER_SYM_GAUSS_SEIDEL, SMOOTHER_DIC, SMOOTHER_DIC_GAUSS_SEIDEL = range(4)
+_SMOOTHER_NAMES = {"GaussSeidel": 0, "symGaussSeidel": 1, "DIC": 2, "DICGaussSeidel": 3}
+
+
+class GamgControls(ctypes.Structure):
+    _fields_ = [("smoother", I), ("tolerance", D), ("relTol", D), ("maxIter", I), ("minIter", I),
+                ("nPreSweeps", I), ("preSweepsLevelMultiplier", I), ("maxPreSweeps", I),
+                ("nPostSweeps", I), ("postSweepsLevelMultiplier", I), ("maxPostSweeps", I),
+                ("nFinestSweeps", I), ("scaleCorrection", I), ("nVcycles", I)]
+
+
+def gamg_controls(**kw):
+    """GAMGSolver's controls with its defaults; `smoother` may be OpenFOAM's name."""
+    c = GamgControls()
+    lib().mfv_gamg_default_controls(ctypes.byref(c))
+    for k, v in kw.items():
+        if k == "smoother" and isinstance(v, str):
+            v = _SMOOTHER_NAMES[v]
+        if not hasattr(c, k):
+            raise AttributeError("gamg controls have no field " + k)
+        setattr(c, k, v)
+    return c
+
+
 class ServeRequest(ctypes.Structure):
     _fields_ = [("nx", I), ("ny", I), ("nz", I), ("lx", D), ("ly", D), ("lz", D), ("jitter", D),
                 ("seed", ctypes.c_ulonglong), ("nSteps", I), ("tolerance", D), ("arrival", D)]
@@ -110,6 +134,10 @@ _SIGS = {
     "mfv_dic_precondition": [P, P, P, P, P],
     "mfv_gauss_seidel": [P, P, P, P, P, I],
     "mfv_pcg_dic_solve": [P, P, P, P, P, P, P, P],
+    "mfv_sym_gauss_seidel": [P, P, P, P, P, I],
+    "mfv_gamg_default_controls": [P],
+    "mfv_gamg_solve": [P, P, P, P, P, P, P, P],
+    "mfv_pcg_gamg_solve": [P, P, P, P, P, P, D, D, I, I, P, P],
     "mfv_server_create": [I, I, D, I, P],
     "mfv_server_destroy": [P],
     "mfv_server_request_bytes": [P, P],
@@ -375,6 +403,25 @@ class Amg:
         perf = Perf()
         self.ctx.call("mfv_amg_pcg_solve", self.mesh.h, self.h, ctypes.byref(A), ptr(b), ptr(psi),
                       ptr(normFactor), ctypes.byref(c), ctypes.byref(perf))
+        return perf
+
+    def gamg_solve(self, diag, upper, b, psi, **kw):
+        """OpenFOAM's GAMGSolver on the folded system (psi in place)."""
+        c = gamg_controls(**kw)
+        perf = Perf()
+        A = ldu(diag, upper)
+        self.ctx.call("mfv_gamg_solve", self.mesh.h, self.h, ctypes.byref(A), ptr(b), ptr(psi), ctypes.byref(c),
+                      ctypes.byref(perf))
+        return perf
+
+    def pcg_gamg_solve(self, diag, upper, b, psi, tol, relTol=0.0, maxIter=1000, minIter=0, **kw):
+        """PCG with the GAMG preconditioner: tol..minIter are the PCG's, kw the preconditioner's GAMG
+        controls (its tolerance and relTol are the coarsest solve's)."""
+        c = gamg_controls(**kw)
+        perf = Perf()
+        A = ldu(diag, upper)
+        self.ctx.call("mfv_pcg_gamg_solve", self.mesh.h, self.h, ctypes.byref(A), ptr(b), ptr(psi), tol, relTol,
+                      maxIter, minIter, ctypes.byref(c), ctypes.byref(perf))
         return perf
 
     def close(self):

@@ -277,6 +277,48 @@ int mfv_gauss_seidel(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double*
  * mfv_amg_pcg_solve. Synchronises. */
 int mfv_pcg_dic_solve(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dB, double* dPsi,
                       const double* dNormFactor, const mfv_pcg_controls* controls, mfv_perf* perf);
+/* symGaussSeidel: each of nSweeps is the forward sweep, then the reverse walk from the forward walk's
+ * accumulated lower-neighbour terms (src/OpenFOAM/matrices/smooth_solver_cpp.cu:87-163) */
+int mfv_sym_gauss_seidel(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dB, double* dPsi, int nSweeps);
+
+/* ---------------------------------------------------------------------------------------------
+ * OpenFOAM's GAMG (src/matrices/lduMatrix/solvers/GAMG/gamg_solver_cpp.cuh): the V-cycle of
+ * GAMGSolver -- residual restricted by summation, pre-sweeps (default none), the coarsest level solved
+ * by PCG-DIC from zero to the GAMG's tolerance, injection, the correction scaling, post-sweeps
+ * min(nPostSweeps + postSweepsLevelMultiplier*level, maxPostSweeps), the finest level's correction
+ * scaled against the residual and psi smoothed nFinestSweeps times -- in FP64 on the hierarchy's grids
+ * (build it with MFV_AGG_OPENFOAM for OpenFOAM's faceAreaPair levels). With the context's sequential
+ * reductions a solve equals the host solve to the bit.
+ * ------------------------------------------------------------------------------------------- */
+#define MFV_SMOOTHER_GAUSS_SEIDEL 0
+#define MFV_SMOOTHER_SYM_GAUSS_SEIDEL 1
+#define MFV_SMOOTHER_DIC 2
+#define MFV_SMOOTHER_DIC_GAUSS_SEIDEL 3
+
+/* GamgControls (gamg_solver_cpp.cuh:67-85) and GamgPreconditionerControls::nVcycles (:127-131) */
+typedef struct
+{
+    int smoother;                       /* MFV_SMOOTHER_* */
+    double tolerance, relTol;           /* the solver's; as a preconditioner, the coarsest solve's */
+    int maxIter, minIter;
+    int nPreSweeps, preSweepsLevelMultiplier, maxPreSweeps;
+    int nPostSweeps, postSweepsLevelMultiplier, maxPostSweeps;
+    int nFinestSweeps;
+    int scaleCorrection;
+    int nVcycles;                       /* V-cycles per preconditioner application */
+} mfv_gamg_controls;
+
+/* GAMGSolver.C's defaults: GaussSeidel, 1e-6, 0, 1000, 0, 0/1/4, 2/1/4, 2, scaled, nVcycles 2 */
+int mfv_gamg_default_controls(mfv_gamg_controls* controls);
+/* gamgSolveFolded (gamg_solver_cpp.cuh:185-195): A the folded symmetric matrix; the FP64 coarse matrices
+ * are agglomerated from it here; psi updated in place. Synchronises once per V-cycle. */
+int mfv_gamg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
+                   const mfv_gamg_controls* controls, mfv_perf* perf);
+/* pcgGamgSolve (gamg_solver_cpp.cuh:197-216): PCG with the GAMG preconditioner; tolerance..minIter are
+ * the PCG's. Synchronises once per iteration. */
+int mfv_pcg_gamg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
+                       double tolerance, double relTol, int maxIter, int minIter, const mfv_gamg_controls* controls,
+                       mfv_perf* perf);
 
 /* ---------------------------------------------------------------------------------------------
  * Serving: a stream of pressure-Poisson requests (no counterpart in the reference, which runs one

@@ -1127,15 +1127,15 @@ static void of_gauss_seidel(const of_mat* A, double* psi, const double* b, int n
         }
         if (symmetric)
         {
-            memcpy(bPrime, b, (size_t)nC * sizeof(double));
+            /* the reverse walk starts from the bPrime the forward walk left (smooth_solver_cpp.cu:141-163):
+             * each cell's lower-neighbour terms carry the forward sweep's values */
             for (int c = nC - 1; c >= 0; --c)
             {
                 const int fStart = A->ownStart[c], fEnd = A->ownStart[c + 1];
                 double psii = bPrime[c];
                 for (int f = fStart; f < fEnd; ++f) psii -= A->upper[f] * psi[A->up[f]];
                 psii /= A->diag[c];
-                for (int f = fStart; f < fEnd; ++f) bPrime[A->up[f]] -= A->upper[f] * psii;
-                psi[c] = psii;
+                psi[c] = psii;   /* nothing to distribute: the upper neighbours are not visited again */
             }
         }
     }
@@ -1322,6 +1322,14 @@ void orc_gauss_seidel(int nCells, int nFaces, const int* owner, const int* neigh
 {
     of_mat m = of_wrap(nCells, nFaces, owner, neighbour, diag, upper);
     of_gauss_seidel(&m, psi, b, nSweeps, 0);
+    of_mat_free(&m);
+}
+
+void orc_sym_gauss_seidel(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                          const double* upper, const double* b, double* psi, int nSweeps)
+{
+    of_mat m = of_wrap(nCells, nFaces, owner, neighbour, diag, upper);
+    of_gauss_seidel(&m, psi, b, nSweeps, 1);
     of_mat_free(&m);
 }
 

@@ -162,6 +162,10 @@ orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* up
  * precondition, gamg_solver_cpp.cu:194-231) on a folded symmetric system, as the GAMG restatement runs them */
 void orc_gauss_seidel(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
                       const double* upper, const double* b, double* psi, int nSweeps);
+/* symGaussSeidel: each sweep is the forward sweep, then a reverse walk (cells descending) from the
+ * forward walk's accumulated lower-neighbour terms (smooth_solver_cpp.cu:141-163) */
+void orc_sym_gauss_seidel(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                          const double* upper, const double* b, double* psi, int nSweeps);
 void orc_dic_precondition(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
                           const double* upper, const double* r, double* w);
 

@@ -57,6 +57,7 @@ def lib():
             "orc_face_flux": (None, [I, P, P, P, P, P, P]),
             "orc_grad_gauss": (None, [I, I, P, P, I, P, P, P, P, P, P, P, P]),
             "orc_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
+            "orc_sym_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
             "orc_dic_precondition": (None, [I, I, P, P, P, P, P, P]),
             "orc_laplacian_corr": (None, [I, I, P, P, P, P, P, P, P, P, P]),
             "orc_flux_correct": (None, [I, P, P, P, P, P, P, P, P]),
@@ -214,10 +215,10 @@ def flux_correct(own, nei, upper, lower, psi, ffc, phiHbyA):
     return phi
 
 
-def gauss_seidel(nC, own, nei, diag, upper, b, psi, nSweeps):
+def gauss_seidel(nC, own, nei, diag, upper, b, psi, nSweeps, symmetric=False):
     psi = f64(psi).copy()
-    lib().orc_gauss_seidel(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(diag)), p(f64(upper)), p(f64(b)), p(psi),
-                           nSweeps)
+    fn = lib().orc_sym_gauss_seidel if symmetric else lib().orc_gauss_seidel
+    fn(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(diag)), p(f64(upper)), p(f64(b)), p(psi), nSweeps)
     return psi
 
 

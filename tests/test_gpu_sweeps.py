@@ -88,3 +88,14 @@ def test_pcg_dic(system, sequential):
     else:
         assert abs(perf.nIterations - ref.nIterations) <= 1 and perf.finalResidual < 1e-8
         assert np.abs(x - ref_psi).max() <= 1e-6 * np.abs(ref_psi).max()
+
+
+def test_sym_gauss_seidel_bit_exact(system):
+    label, nC, own, nei, bfc, diag, up, b, _ = system
+    print("case:", label, "symGaussSeidel", flush=True)
+    case = G.Case(nC, own, nei, bfc)
+    A = mfv.ldu(G.dev(diag), G.dev(up))
+    x0 = np.random.default_rng(3).normal(size=nC)
+    dx = G.dev(x0)
+    case.call("mfv_sym_gauss_seidel", case.mesh.h, ctypes.byref(A), mfv.ptr(G.dev(b)), mfv.ptr(dx), 2)
+    assert np.array_equal(G.host(dx), O.gauss_seidel(nC, own, nei, diag, up, b, x0, 2, symmetric=True))

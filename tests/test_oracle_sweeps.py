@@ -2,6 +2,8 @@
 pinned by closed forms. CPU only.
 
 - One forward Gauss-Seidel sweep is x_new = (D + L)^-1 (b - U x_old): a lower-triangular solve.
+- symGaussSeidel's reverse walk keeps the forward walk's lower-neighbour terms:
+  x_rev = (D + U)^-1 (b - L x_fwd), an upper-triangular solve (smooth_solver_cpp.cu:141-163).
 - DIC on a chain (tridiagonal matrix, no fill-in) is the exact Cholesky solve: w = A^-1 r.
 - On a 3-D mesh, DIC's factor satisfies the incomplete-Cholesky identity: (L_D) D^-1 (L_D)^T agrees
   with A on the diagonal and on every face, L_D = D + lower part of A with D = 1/rD.
@@ -45,6 +47,23 @@ def test_gauss_seidel_sweep_is_triangular_solve():
     x2 = O.gauss_seidel(nC, own, nei, diag, up, b, x0, 2)
     ref2 = solve_triangular(np.tril(A), b - np.triu(A, 1) @ ref, lower=True)
     assert np.abs(x2 - ref2).max() < 1e-12 * np.abs(ref2).max()
+
+
+def test_sym_gauss_seidel_sweep_is_forward_then_upper_solve():
+    nC, own, nei, diag, up = system((7, 5, 4), 6)
+    A = dense(nC, own, nei, diag, up)
+    rng = np.random.default_rng(7)
+    b, x0 = rng.normal(size=nC), rng.normal(size=nC)
+    ref = x0
+    for _ in range(2):
+        fwd = solve_triangular(np.tril(A), b - np.triu(A, 1) @ ref, lower=True)
+        ref = solve_triangular(np.triu(A), b - np.tril(A, -1) @ fwd, lower=False)
+        x = O.gauss_seidel(nC, own, nei, diag, up, b, x0, _ + 1, symmetric=True)
+        assert np.abs(x - ref).max() < 1e-12 * np.abs(ref).max()
+    # a reverse walk restarted from b (dropping the lower terms) is a different sweep
+    wrong = solve_triangular(np.triu(A), b, lower=False)
+    x1 = O.gauss_seidel(nC, own, nei, diag, up, b, x0, 1, symmetric=True)
+    assert np.abs(x1 - wrong).max() > 1e-3 * np.abs(wrong).max()
 
 
 def test_dic_is_exact_on_a_chain():
