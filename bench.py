@@ -6,10 +6,11 @@ channel, 20,000,000 cells, nNonOrthogonalCorrectors 1): rAUf = interpolate(rAU);
 coefficients); Galerkin re-coarsening; then twice: grad(p), the explicit non-orthogonal correction flux
 and its divergence into the source, source -= div(phiHbyA)*V, fold, normFactor, AMG-preconditioned CG to
 a normalised residual of 1e-6 (from p = 0, then from the first solution); finally
-phi = phiHbyA - pEqn.flux() with the stored correction flux. The mixed-precision path runs the V-cycle in FP32 (or FP16) inside FP64
-flexible CG; the FP64 path runs the same solve with an FP64 cycle. Both are timed; `value` is the
-mixed path's seconds per step (lower is better), and the FP64 time, the ratio, iterations, residuals
-and the mixed/FP64 solution difference are reported beside it.
+phi = phiHbyA - pEqn.flux() with the stored correction flux. The mixed-precision path runs the V-cycle
+in FP16 with per-grid power-of-two scaling (--precision fp32: in FP32) inside FP64 flexible CG; the FP64
+path runs the same solve with an FP64 cycle, and the other mixed precision is timed beside them.
+`value` is the mixed path's seconds per step (lower is better); the FP64 time, the ratio, iterations,
+residuals and the mixed/FP64 solution difference are reported beside it.
 
 Inputs are synthetic and seeded: perturbed plug flow U = (10,0,0) + uniform(-1,1) per component
 (inlet fixedValue (10,0,0), outlet zeroGradient, walls no-slip), phiHbyA = interpolate(U).Sf,
@@ -43,11 +44,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", default="500x250x160")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16"])
+    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16"],
+                    help="the mixed path's V-cycle (the other one is timed beside it)")
+    ap.add_argument("--no-other", action="store_true", help="skip timing the other mixed precision")
     ap.add_argument("--tolerance", type=float, default=1e-6)
     ap.add_argument("--cpu-sample", default="250x125x80", help="channel the CPU oracle is timed on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-every", type=int, default=4)
+    ap.add_argument("--graph", type=int, default=0, help="1: replay captured groups of PCG iterations")
     ap.add_argument("--no-scale", action="store_true", help="plain injection instead of the scaled correction")
     ap.add_argument("--npre", type=int, default=1)
     ap.add_argument("--npost", type=int, default=1)
@@ -132,7 +136,9 @@ def gpu_run(args):
     from brae_amd import hexmesh, mfv
 
     nx, ny, nz = (int(v) for v in args.size.split("x"))
-    stream = torch.cuda.current_stream()
+    # one stream (default priority) for torch's own work and the library's, so the two stay in order
+    stream = torch.cuda.Stream(device="cuda:0")
+    torch.cuda.set_stream(stream)
     ctx = mfv.Context(0, stream=stream.cuda_stream)
     ch = Channel(nx, ny, nz, torch, mfv, hexmesh, ctx)
     t0 = time.time()
@@ -167,7 +173,7 @@ def gpu_run(args):
             perfs.append(amg.solve(ch.diagC, ch.upper, ch.b, ch.p, tolerance=args.tolerance, precision=prec,
                                    flexible=int(prec != mfv.PREC_FP64 or not args.no_scale),
                                    checkEvery=args.check_every, scaleCorrection=int(not args.no_scale),
-                                   nPreSweeps=args.npre, nPostSweeps=args.npost))
+                                   nPreSweeps=args.npre, nPostSweeps=args.npost, captureVcycle=args.graph))
         ctx.call("mfv_flux_correct", ch.mesh.h, ctypes.byref(A), P(ch.p), P(ch.ffc), P(ch.phiHbyA), -1.0, P(ch.phi))
         return perfs
 
@@ -191,11 +197,14 @@ def gpu_run(args):
         return dt, perfs, prof
 
     res = {}
-    for name, prec in (("mixed", prec_mixed), ("fp64", mfv.PREC_FP64)):
+    other = mfv.PREC_FP16 if prec_mixed == mfv.PREC_FP32 else mfv.PREC_FP32
+    runs = [("mixed", prec_mixed), ("fp64", mfv.PREC_FP64)] + ([] if args.no_other else [("other", other)])
+    for name, prec in runs:
         dt, perfs, _ = timed(prec, profile=False)
         res[name] = dict(dt=dt, perfs=perfs, p=ch.p.clone())
-        log("%s: %.4f s/step, iterations %s, final residual %.3e" % (
-            name, dt, [q.nIterations for q in perfs], perfs[-1].finalResidual))
+        log("%s (%s): %.4f s/step, iterations %s, final residual %.3e" % (
+            name, {64: "fp64", 32: "fp32", 16: "fp16"}[prec], dt, [q.nIterations for q in perfs],
+            perfs[-1].finalResidual))
     # per-kernel timing: one more step of the mixed path with events around the tagged launches (kept out
     # of the timed steps above)
     saved = args.steps, args.warmup
@@ -206,6 +215,14 @@ def gpu_run(args):
     res["mixed"]["prof_perfs"] = prof_perfs
     pm, p64 = res["mixed"]["p"], res["fp64"]["p"]
     rel_l2 = float((pm - p64).norm() / p64.norm())
+    other_out = None
+    if "other" in res:
+        po = res["other"]
+        other_out = {"v_cycle": {16: "fp16", 32: "fp32"}[other], "s_per_iter": round(po["dt"], 5),
+                     "speedup_vs_fp64": round(res["fp64"]["dt"] / po["dt"], 3),
+                     "pcg_iterations": [q.nIterations for q in po["perfs"]],
+                     "final_residual": po["perfs"][-1].finalResidual,
+                     "p_rel_l2_vs_fp64": float((po["p"] - p64).norm() / p64.norm())}
 
     # roofline of the dominant kernel of the mixed step (largest summed event time)
     N, F, Nc = ch.nC, ch.nI, sizes[1][0]
@@ -235,6 +252,7 @@ def gpu_run(args):
     free_b, total_b = torch.cuda.mem_get_info()
     hbm_used_gib = (total_b - free_b) / 2 ** 30
     copy_gbs = copy_peak(torch, N)
+    triad_gbs = triad_peak(torch, N)
     vc_n, vc_ms = prof[mfv.PROF_VCYCLE]
     grids = []
     for gi in range(amg.nGrids):
@@ -252,9 +270,9 @@ def gpu_run(args):
         roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                       frac=round(achieved / PEAK_HBM_GBS, 4), traffic=measured_traffic(label), kernel=label,
                       alg_bytes_per_launch=int(alg_bytes), avg_launch_us=round(avg_ms * 1e3, 2),
-                      copy_peak_GBps=round(copy_gbs, 1)),
+                      copy_peak_GBps=round(copy_gbs, 1), triad_GBps=round(triad_gbs, 1)),
         per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1), grids=grids,
-        hbm_used_gib=round(hbm_used_gib, 2),
+        hbm_used_gib=round(hbm_used_gib, 2), other=other_out,
     )
     amg.close()
     ch.mesh.close()
@@ -315,6 +333,26 @@ def copy_peak(torch, n):
     ms = e0.elapsed_time(e1) / reps
     del src, dst
     return 16.0 * n / (ms * 1e-3) / 1e9
+
+
+def triad_peak(torch, n):
+    """Measured streaming rate of c = a + s*b over n doubles (24 bytes per element / time): a plain
+    elementwise stream, the box's practical HBM ceiling beside the published 8 TB/s."""
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.rand(n, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+    for _ in range(3):
+        torch.add(a, b, alpha=0.5, out=c)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        torch.add(a, b, alpha=0.5, out=c)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b, c
+    return 24.0 * n / (ms * 1e-3) / 1e9
 
 
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_pressure_corrector_20M.md")
@@ -436,6 +474,7 @@ def main():
                 "cycle": "V(%d,%d) weighted Jacobi%s" % (args.npre, args.npost,
                                                         "" if args.no_scale else ", scaled correction"),
                 "per_grid_us": r["grids"]},
+        "other_mixed": r["other"],
         "hbm_used_gib": r["hbm_used_gib"],
         "roofline": r["roofline"],
         "per_kernel": r["per_kernel"],

@@ -4,6 +4,7 @@
 #include "mfv.h"
 #include "mfv_internal.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -678,6 +679,7 @@ int mfv_pcg_default_controls(mfv_pcg_controls* c)
     c->checkEvery = 4;
     c->nVcycles = 1;
     c->scaleCorrection = 0;
+    c->captureVcycle = 0;
     return MFV_OK;
 }
 
@@ -712,6 +714,20 @@ int mfv_sym_gauss_seidel(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const dou
         launchGaussSeidel(ctx->c, mesh->m, view(mesh, A), dB, dPsi, nSweeps, bPrime.p);
         hipCheck(hipStreamSynchronize(ctx->c.stream), "sym-gauss-seidel sync");
     });
+}
+
+int mfv_faceareapair_weights(int nFaces, const double* hSf, const double* hMagSf, double* hWeights)
+{
+    if (nFaces < 0 || (nFaces && (!hSf || !hMagSf || !hWeights))) return MFV_ERR_INVALID;
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const double root = std::sqrt(hMagSf[f]);
+        const double wx = hSf[3 * f] / root;
+        const double wy = (hSf[3 * f + 1] / root) * 1.01;
+        const double wz = (hSf[3 * f + 2] / root) * 1.02;
+        hWeights[f] = std::sqrt(wx * wx + wy * wy + wz * wz);
+    }
+    return MFV_OK;
 }
 
 int mfv_gamg_default_controls(mfv_gamg_controls* c)
@@ -805,6 +821,7 @@ int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A,
         pc.flexible = controls->flexible != 0;
         pc.scaleCorrection = controls->scaleCorrection != 0;
         pc.checkEvery = controls->checkEvery;
+        pc.graph = controls->captureVcycle != 0;
         const PcgPerf r = amgPcgSolve(ctx->c, mesh->m, amg->a, v.diag, v.upper, dB, dPsi, dNormFactor, pc);
         if (perf)
         {

@@ -514,11 +514,18 @@ __global__ __launch_bounds__(kReduceBlock) void maxAbsK(int n, const double* __r
     }
 }
 
-__global__ void maxPow2FinishK(int nParts, const double* __restrict__ partials, double* __restrict__ out)
+__global__ __launch_bounds__(kFinishBlock) void maxPow2FinishK(int nParts, const double* __restrict__ partials,
+                                                                double* __restrict__ out)
 {
-    if (threadIdx.x || blockIdx.x) return;
+    __shared__ double lds[kFinishBlock / 64];
     double mm = 0.0;
-    for (int i = 0; i < nParts; ++i) mm = fmax(mm, partials[i]);
+    for (int i = threadIdx.x; i < nParts; i += kFinishBlock) mm = fmax(mm, partials[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mm = fmax(mm, __shfl_xor(mm, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mm;
+    __syncthreads();
+    if (threadIdx.x) return;
+    for (int w = 0; w < kFinishBlock / 64; ++w) mm = fmax(mm, lds[w]);
     *out = mm > 0.0 ? exp2(ceil(log2(mm))) : 1.0;
 }
 
@@ -1114,7 +1121,7 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
                     const int gm = reduceGrid(g.nFaces);
                     double* pm = c.partials.p + 4 * (size_t)kMaxReduceBlocks;
                     maxAbsK<<<gm, kReduceBlock, 0, c.stream>>>(g.nFaces, u, pm);
-                    maxPow2FinishK<<<1, 64, 0, c.stream>>>(gm, pm, a.scales.p + gi);
+                    maxPow2FinishK<<<1, kFinishBlock, 0, c.stream>>>(gm, pm, a.scales.p + gi);
                     fillK<__half><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.sell.slotLo.p, g.sell.slotHi.p,
                                                                            u, u, a.scales.p + gi, g.off16.p);
                 }
@@ -1176,6 +1183,34 @@ void* amgVcycle(Ctx& c, Amg& a, const VcycleControls& vc, const int* stop)
     case Prec::F16: return vcycleAt<__half, float>(c, a, 0, vc, stop, vecs32, Scatter<float>{});
     }
     return nullptr;
+}
+
+uint64_t amgLayoutKey(const Amg& a)
+{
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+    auto mix = [&h](uint64_t v) {
+        h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+        h *= 0xff51afd7ed558ccdull;
+    };
+    auto ptr = [&mix](const void* p) { mix(reinterpret_cast<uintptr_t>(p)); };
+    mix((uint64_t)a.prec);
+    mix((uint64_t)a.tailStart);
+    mix(a.grids.size());
+    for (const Grid& g : a.grids)
+    {
+        for (const void* p : {(const void*)g.sell.slicePtr.p, (const void*)g.sell.cols.p, (const void*)g.sell.pattern.p,
+                              (const void*)g.sell.patOff.p, (const void*)g.off64.p, (const void*)g.off32.p,
+                              (const void*)g.off16.p, (const void*)g.diagV64.p, (const void*)g.diagV32.p,
+                              (const void*)g.b64.p, (const void*)g.x64.p, (const void*)g.y64.p, (const void*)g.r64.p,
+                              (const void*)g.e64.p, (const void*)g.b32.p, (const void*)g.x32.p, (const void*)g.y32.p,
+                              (const void*)g.r32.p, (const void*)g.e32.p, (const void*)g.map.p,
+                              (const void*)g.cellStart.p, (const void*)g.cellList.p})
+            ptr(p);
+    }
+    for (const void* p : {(const void*)a.coarseInv.p, (const void*)a.scales.p, (const void*)a.alphas.p,
+                          (const void*)a.seqA.p, (const void*)a.seqB.p, (const void*)a.tailDesc.p})
+        ptr(p);
+    return h;
 }
 
 void amgResultToDouble(Ctx& c, Amg& a, const void* res, double* z)

@@ -78,12 +78,22 @@ struct AmgBuildControls
     int mergeLevels = 1;   // pairwise passes composed into each level
 };
 
-// Work vectors and device state of the PCG solve on a hierarchy (FP64, one per cell).
+// Work vectors and device state of the PCG solve on a hierarchy (FP64, one per cell), and the captured
+// graph of a group of iterations with the key it was captured for.
 struct PcgWork
 {
     DBuf<double> r, p0, p1, q, rOld, st;
     DBuf<int> fl;
     DBuf<double> seqA, seqB;   // per-element terms for sequential (OpenFOAM-order) reductions
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graphKey;
+    PcgWork() = default;
+    PcgWork(const PcgWork&) = delete;
+    PcgWork& operator=(const PcgWork&) = delete;
+    ~PcgWork()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
 };
 
 // Device state of OpenFOAM's GAMG cycle on a hierarchy (of_gamg.hip): every coarse grid's LDU
@@ -161,6 +171,10 @@ struct VcycleControls
 // int, may be null) makes every kernel return at once when nonzero.
 void* amgVcycle(Ctx& c, Amg& a, const VcycleControls& vc, const int* stop);
 
+// Every device address a V-cycle's kernels take as an argument, hashed: a captured cycle stays valid
+// while this is unchanged (values may change; buffers may not move).
+uint64_t amgLayoutKey(const Amg& a);
+
 // z (FP64) = the V-cycle result `res` (as amgVcycle returned it)
 void amgResultToDouble(Ctx& c, Amg& a, const void* res, double* z);
 
@@ -180,6 +194,7 @@ struct PcgControls
     bool flexible = false;
     bool scaleCorrection = false;
     int checkEvery = 4;
+    bool graph = false;   // replay captured groups of iterations (captureVcycle)
 };
 
 struct PcgPerf

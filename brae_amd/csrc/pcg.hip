@@ -210,9 +210,9 @@ __global__ __launch_bounds__(kReduceBlock) void pcgDirAmulK(SellView S, const do
     MFV_ROWS(S.n, c)
     {
         const double pc = P(c);
-        const double ax = rowOffSum<double, double>(S, c, val, P, diag[c] * pc);
-        pNew[c] = pc;
-        q[c] = ax;
+        const double ax = rowOffSum<double, double>(S, c, val, P, ldStream(diag + c) * pc);
+        __builtin_nontemporal_store(pc, pNew + c);
+        __builtin_nontemporal_store(ax, q + c);
         if (seqA)
             seqA[c] = ax * pc;
         else
@@ -327,9 +327,8 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     vc.scale = pc.scaleCorrection;
     amgStartFrom(c, a, w.r.p, vc, w.fl.p + F_STOP);
     double* p[2] = {w.p0.p, w.p1.p};
-    int it = 0;
-    for (; it < pc.maxIter; ++it)
-    {
+    // one iteration, enqueued; `it` selects the direction buffers (they alternate)
+    auto iteration = [&](int it) {
         const Tv* z;
         {
             ProfScope ps(c, kProfVcycle);
@@ -367,11 +366,61 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         else
             finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, UpdPost{d}, stopF);
         hipCheck(hipGetLastError(), "pcg iteration");
-        if ((it + 1) % std::max(1, pc.checkEvery) == 0 || it + 1 == pc.maxIter)
+    };
+    auto stopped = [&] {
+        hipCheck(hipMemcpyAsync(c.mailbox, w.fl.p, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost, c.stream), "poll");
+        hipCheck(hipStreamSynchronize(c.stream), "poll sync");
+        return reinterpret_cast<const int*>(c.mailbox)[F_STOP] != 0;
+    };
+    if (pc.graph && !c.profiling && c.stream != nullptr)
+    {
+        // A group of iterations captured once and replayed between polls. The group starts on an even
+        // iteration, so the direction buffers alternate as in the enqueued loop; the device stops every
+        // kernel at convergence or maxIter, so the iterations of a group past either do nothing.
+        const int group = (std::max(1, pc.checkEvery) + 1) & ~1;
+        std::vector<uint64_t> key = {amgLayoutKey(a), reinterpret_cast<uintptr_t>(c.stream),
+                                     reinterpret_cast<uintptr_t>(psi), reinterpret_cast<uintptr_t>(diag),
+                                     reinterpret_cast<uintptr_t>(c.partials.p), reinterpret_cast<uintptr_t>(w.r.p),
+                                     reinterpret_cast<uintptr_t>(w.p0.p), reinterpret_cast<uintptr_t>(w.p1.p),
+                                     reinterpret_cast<uintptr_t>(w.q.p), reinterpret_cast<uintptr_t>(w.rOld.p),
+                                     reinterpret_cast<uintptr_t>(w.st.p), reinterpret_cast<uintptr_t>(w.fl.p),
+                                     reinterpret_cast<uintptr_t>(sA), reinterpret_cast<uintptr_t>(sB),
+                                     (uint64_t)group, (uint64_t)pc.flexible, (uint64_t)pc.nPre, (uint64_t)pc.nPost,
+                                     (uint64_t)pc.scaleCorrection, (uint64_t)seq};
+        if (!w.graph || w.graphKey != key)
         {
-            hipCheck(hipMemcpyAsync(c.mailbox, w.fl.p, sizeof(int) * F_COUNT, hipMemcpyDeviceToHost, c.stream), "poll");
-            hipCheck(hipStreamSynchronize(c.stream), "poll sync");
-            if (reinterpret_cast<const int*>(c.mailbox)[F_STOP]) break;
+            if (w.graph) (void)hipGraphExecDestroy(w.graph);
+            w.graph = nullptr;
+            hipGraph_t gr = nullptr;
+            hipCheck(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "capture begin");
+            try
+            {
+                for (int j = 0; j < group; ++j) iteration(j);
+            }
+            catch (...)
+            {
+                (void)hipStreamEndCapture(c.stream, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            hipCheck(hipStreamEndCapture(c.stream, &gr), "capture end");
+            const hipError_t e = hipGraphInstantiate(&w.graph, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            hipCheck(e, "graph instantiate");
+            w.graphKey = key;
+        }
+        for (int it = 0; it < pc.maxIter; it += group)
+        {
+            hipCheck(hipGraphLaunch(w.graph, c.stream), "graph launch");
+            if (stopped()) break;
+        }
+    }
+    else
+    {
+        for (int it = 0; it < pc.maxIter; ++it)
+        {
+            iteration(it);
+            if (((it + 1) % std::max(1, pc.checkEvery) == 0 || it + 1 == pc.maxIter) && stopped()) break;
         }
     }
     hipCheck(hipMemcpyAsync(c.mailbox, w.st.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, c.stream), "report");

@@ -14,6 +14,20 @@ __device__ __forceinline__ T safeDiag(T d)
     return d < T(0) ? -fl : fl;
 }
 
+// A load of data a pass reads once (coefficients, explicit columns, row patterns, diagonals):
+// nontemporal, so the stream does not push the gathered vectors -- reread by the rows a plane of
+// cells away -- out of the XCD's L2.
+template <typename T>
+__device__ __forceinline__ T ldStream(const T* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ __half ldStream<__half>(const __half* p)
+{
+    return __ushort_as_half(__builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p)));
+}
+
 // a stored coefficient in the arithmetic precision Tv
 template <typename Tv, typename Tm>
 __device__ __forceinline__ Tv cvt(Tm v) { return (Tv)v; }
@@ -54,8 +68,8 @@ __device__ __forceinline__ Tv rowGroup(const SellView& S, int c, int lane, int b
     for (int k = 0; k < W; ++k)
     {
         const int e = base + (k << 6) + lane;
-        a[k] = cvt<Tv>(val[e]);
-        col[k] = Pat ? c + off[k] : S.cols[e];
+        a[k] = cvt<Tv>(ldStream(val + e));
+        col[k] = Pat ? c + off[k] : ldStream(S.cols + e);
     }
     Tv xv[W];
 #pragma unroll
@@ -94,7 +108,8 @@ __device__ __forceinline__ Tv rowOffSum(const SellView& S, int c, const Tm* __re
     const int p0 = __builtin_amdgcn_readfirstlane(S.slicePtr[s]);
     const int w = __builtin_amdgcn_readfirstlane((S.slicePtr[s + 1] - p0) >> 6);
     if (S.pattern)
-        return rowEntries<true, Tm, Tv>(S, c, lane, p0, w, S.patOff + (int)S.pattern[c] * S.width, val, X, acc);
+        return rowEntries<true, Tm, Tv>(S, c, lane, p0, w, S.patOff + (int)ldStream(S.pattern + c) * S.width, val, X,
+                                        acc);
     return rowEntries<false, Tm, Tv>(S, c, lane, p0, w, nullptr, val, X, acc);
 }
 
@@ -105,14 +120,15 @@ __device__ __forceinline__ Tv rowAx(const SellView& S, int c, const Tm* __restri
                                     const double* __restrict__ scale, XF X)
 {
     const Tv xc = X(c);
+    const Tv dc = ldStream(diag + c);
     if constexpr (sizeof(Tm) == 2)
     {
         const Tv off = rowOffSum<Tm, Tv>(S, c, val, X, Tv(0));
-        return diag[c] * xc + (Tv)(*scale) * off;
+        return dc * xc + (Tv)(*scale) * off;
     }
     else
     {
-        return rowOffSum<Tm, Tv>(S, c, val, X, diag[c] * xc);
+        return rowOffSum<Tm, Tv>(S, c, val, X, dc * xc);
     }
 }
 

@@ -39,7 +39,7 @@ class AmgControls(ctypes.Structure):
 class PcgControls(ctypes.Structure):
     _fields_ = [("tolerance", D), ("relTol", D), ("maxIter", I), ("minIter", I), ("precision", I),
                 ("nPreSweeps", I), ("nPostSweeps", I), ("flexible", I), ("smoother", I), ("checkEvery", I),
-                ("nVcycles", I), ("scaleCorrection", I)]
+                ("nVcycles", I), ("scaleCorrection", I), ("captureVcycle", I)]
 
 
 AGG_BRAE, AGG_FACEAREAPAIR = 0, 1
@@ -136,6 +136,7 @@ _SIGS = {
     "mfv_pcg_dic_solve": [P, P, P, P, P, P, P, P],
     "mfv_sym_gauss_seidel": [P, P, P, P, P, I],
     "mfv_gamg_default_controls": [P],
+    "mfv_faceareapair_weights": [I, P, P, P],
     "mfv_gamg_solve": [P, P, P, P, P, P, P, P],
     "mfv_pcg_gamg_solve": [P, P, P, P, P, P, D, D, I, I, P, P],
     "mfv_server_create": [I, I, D, I, P],

@@ -428,13 +428,12 @@ inline void amgGalerkin(AMGData& A, const DeviceBuffer<scalar>& fineDiag, const 
 }
 
 // deviceAMGPCG (device_amg.cuh:550-553) with the normFactor on the device (NULL: computed here);
-// captureVcycle is accepted and ignored (kernels are enqueued ahead on the stream)
+// captureVcycle replays captured groups of iterations (a HIP graph per hierarchy)
 inline DeviceSolverPerf deviceAMGPCG(const DeviceLduView& Afine, AMGData& amg, const DeviceBuffer<scalar>& b,
                                      DeviceBuffer<scalar>& psi, const scalar* dNormFactor, scalar tol, scalar relTol,
                                      int maxIter, bool captureVcycle = false, int checkEvery = 1,
                                      bool corrScaling = false, int minIter = 0)
 {
-    (void)captureVcycle;
     mfv_ctx ctx = threadContext();
     mfv_pcg_controls c;
     mfv_pcg_default_controls(&c);
@@ -446,6 +445,7 @@ inline DeviceSolverPerf deviceAMGPCG(const DeviceLduView& Afine, AMGData& amg, c
     c.nPreSweeps = amg.nPreSweeps;
     c.nPostSweeps = amg.nPostSweeps;
     c.checkEvery = checkEvery;
+    c.captureVcycle = captureVcycle ? 1 : 0;
     c.scaleCorrection = corrScaling ? 1 : 0;
     c.flexible = corrScaling ? 1 : 0;   // a scaled cycle is nonlinear: flexible CG, as the reference switches
     if (Afine.nCells != amg.nFine) throw std::runtime_error("brae hip: deviceAMGPCG: matrix and hierarchy sizes differ");
@@ -466,6 +466,134 @@ inline DeviceSolverPerf deviceAMGPCG(const DeviceLduView& Afine, AMGData& amg, c
     DeviceBuffer<scalar> nf(std::vector<scalar>{normFactor});
     return deviceAMGPCG(Afine, amg, b, psi, nf.data(), tol, relTol, maxIter, captureVcycle, checkEvery, corrScaling,
                         minIter);
+}
+
+// ---------------- OpenFOAM's GAMG (src/matrices/lduMatrix/solvers/GAMG/gamg_solver_cpp.cuh) ----------------
+
+// SolverPerformance (src/OpenFOAM/matrices/pcg.cuh:16-21)
+struct SolverPerformance
+{
+    scalar initialResidual = 0, finalResidual = 0;
+    int nIterations = 0;
+    bool converged = false;
+};
+
+// GamgControls (gamg_solver_cpp.cuh:67-85) with GAMGSolver.C's defaults
+struct GamgControls
+{
+    std::string smoother = "GaussSeidel";
+    scalar tolerance = 1e-6;
+    scalar relTol = 0;
+    int maxIter = 1000;
+    int minIter = 0;
+    int nPreSweeps = 0;
+    int preSweepsLevelMultiplier = 1;
+    int maxPreSweeps = 4;
+    int nPostSweeps = 2;
+    int postSweepsLevelMultiplier = 1;
+    int maxPostSweeps = 4;
+    int nFinestSweeps = 2;
+    bool scaleCorrection = true;
+    label nCellsInCoarsestLevel = 10;
+};
+
+// GamgPreconditionerControls (gamg_solver_cpp.cuh:127-131)
+struct GamgPreconditionerControls
+{
+    GamgControls gamg;
+    int nVcycles = 2;
+};
+
+// gamgSmootherPorted (gamg_solver_cpp.cuh:134): the smoothers the GAMG runs
+inline bool gamgSmootherPorted(const std::string& s)
+{
+    return s == "GaussSeidel" || s == "symGaussSeidel" || s == "DIC" || s == "DICGaussSeidel";
+}
+
+inline mfv_gamg_controls toMfv(const GamgControls& g, int nVcycles, const char* who)
+{
+    mfv_gamg_controls c;
+    mfv_gamg_default_controls(&c);
+    if (g.smoother == "GaussSeidel")
+        c.smoother = MFV_SMOOTHER_GAUSS_SEIDEL;
+    else if (g.smoother == "symGaussSeidel")
+        c.smoother = MFV_SMOOTHER_SYM_GAUSS_SEIDEL;
+    else if (g.smoother == "DIC")
+        c.smoother = MFV_SMOOTHER_DIC;
+    else if (g.smoother == "DICGaussSeidel")
+        c.smoother = MFV_SMOOTHER_DIC_GAUSS_SEIDEL;
+    else
+        throw std::runtime_error(std::string("brae hip: ") + who + ": smoother " + g.smoother + " is not carried");
+    c.tolerance = g.tolerance;
+    c.relTol = g.relTol;
+    c.maxIter = g.maxIter;
+    c.minIter = g.minIter;
+    c.nPreSweeps = g.nPreSweeps;
+    c.preSweepsLevelMultiplier = g.preSweepsLevelMultiplier;
+    c.maxPreSweeps = g.maxPreSweeps;
+    c.nPostSweeps = g.nPostSweeps;
+    c.postSweepsLevelMultiplier = g.postSweepsLevelMultiplier;
+    c.maxPostSweeps = g.maxPostSweeps;
+    c.nFinestSweeps = g.nFinestSweeps;
+    c.scaleCorrection = g.scaleCorrection ? 1 : 0;
+    c.nVcycles = nVcycles;
+    return c;
+}
+
+// faceAreaPairGamgAgglomeration (pair_gamg_agglomeration_cpp.cu:286-345): OpenFOAM's pairwise levels
+// from the face weights |(Sf_x, 1.01 Sf_y, 1.02 Sf_z)|/sqrt|Sf| down to nCellsInCoarsestLevel. Sf is
+// xyz-interleaved per internal face.
+inline AMGData faceAreaPairGamgAgglomeration(const std::vector<label>& fineOwner, const std::vector<label>& fineNei,
+                                             const std::vector<scalar>& Sf, const std::vector<scalar>& magSf,
+                                             int nFine, label nCellsInCoarsestLevel = 10)
+{
+    mfv_ctx ctx = threadContext();
+    const int nF = static_cast<int>(fineOwner.size());
+    if (Sf.size() != 3 * (size_t)nF || magSf.size() != (size_t)nF)
+        throw std::runtime_error("brae hip: faceAreaPairGamgAgglomeration: Sf/magSf sizes differ from the faces");
+    std::vector<scalar> w(nF);
+    mfvCheck(mfv_faceareapair_weights(nF, Sf.data(), magSf.data(), w.data()), ctx, "faceAreaPair weights");
+    AMGData a;
+    a.nFine = nFine;
+    a.precision = MFV_PREC_FP64;
+    mfvCheck(mfv_mesh_create(ctx, nFine, nF, fineOwner.data(), fineNei.data(), 0, nullptr, &a.addressing), ctx,
+             "faceAreaPairGamgAgglomeration addressing");
+    mfv_amg_controls c{MFV_AGG_FACEAREAPAIR, nCellsInCoarsestLevel, 1};
+    mfvCheck(mfv_amg_build(ctx, a.addressing, w.data(), &c, &a.amg), ctx, "faceAreaPairGamgAgglomeration");
+    return a;
+}
+
+// gamgSolveFolded (gamg_solver_cpp.cuh:185-195) on the device: the folded symmetric system (diag,
+// upper), psi in place; the coarse matrices are agglomerated from it on every call
+inline SolverPerformance gamgSolveFolded(AMGData& agglomeration, const DeviceBuffer<scalar>& diag,
+                                         const DeviceBuffer<scalar>& upper, const DeviceBuffer<scalar>& source,
+                                         DeviceBuffer<scalar>& psi, const GamgControls& controls)
+{
+    mfv_ctx ctx = threadContext();
+    const mfv_gamg_controls c = toMfv(controls, 2, "gamgSolveFolded");
+    const mfv_ldu l{diag.data(), upper.data(), upper.data()};
+    mfv_perf perf{};
+    mfvCheck(mfv_gamg_solve(ctx, agglomeration.addressing, agglomeration.amg, &l, source.data(), psi.data(), &c,
+                            &perf),
+             ctx, "gamgSolveFolded");
+    return SolverPerformance{perf.initialResidual, perf.finalResidual, perf.nIterations, perf.converged != 0};
+}
+
+// pcgGamgSolve (gamg_solver_cpp.cuh:197-216) on the device, on a folded symmetric system: PCG's
+// tolerance..minIter, the preconditioner's GAMG controls and nVcycles
+inline SolverPerformance pcgGamgSolveFolded(AMGData& agglomeration, const DeviceBuffer<scalar>& diag,
+                                            const DeviceBuffer<scalar>& upper, const DeviceBuffer<scalar>& source,
+                                            DeviceBuffer<scalar>& psi, scalar tolerance, scalar relTol, int maxIter,
+                                            int minIter, const GamgPreconditionerControls& precond)
+{
+    mfv_ctx ctx = threadContext();
+    const mfv_gamg_controls c = toMfv(precond.gamg, precond.nVcycles, "pcgGamgSolve");
+    const mfv_ldu l{diag.data(), upper.data(), upper.data()};
+    mfv_perf perf{};
+    mfvCheck(mfv_pcg_gamg_solve(ctx, agglomeration.addressing, agglomeration.amg, &l, source.data(), psi.data(),
+                                tolerance, relTol, maxIter, minIter, &c, &perf),
+             ctx, "pcgGamgSolve");
+    return SolverPerformance{perf.initialResidual, perf.finalResidual, perf.nIterations, perf.converged != 0};
 }
 
 } // namespace brae

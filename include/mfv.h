@@ -186,6 +186,10 @@ typedef struct
  * device_amg.cu:1346-1431), and for MFV_AGG_FACEAREAPAIR faceAreaPairGamgAgglomeration
  * (src/matrices/lduMatrix/solvers/GAMG/GAMGAgglomerations/pair_gamg_agglomeration_cpp.cu:286-345).
  * hFaceWeights: one per internal face (brae passes |Sf|; faceAreaPair its perturbed weights). */
+/* faceAreaPair's face weights |(Sf_x, 1.01 Sf_y, 1.02 Sf_z)| / sqrt(|Sf|) for MFV_AGG_FACEAREAPAIR
+ * (src/matrices/lduMatrix/solvers/GAMG/GAMGAgglomerations/pair_gamg_agglomeration_cpp.cu:301-311); host
+ * arrays: hSf [3*nFaces] xyz-interleaved, hMagSf, hWeights [nFaces] */
+int mfv_faceareapair_weights(int nFaces, const double* hSf, const double* hMagSf, double* hWeights);
 int mfv_amg_build(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights, const mfv_amg_controls* controls,
                   mfv_amg* out);
 int mfv_amg_destroy(mfv_amg amg);
@@ -246,6 +250,10 @@ typedef struct
     int scaleCorrection;   /* 1: each grid's coarse correction scaled by (r.e)/(e.Ae) (the reference's
                               corrScaling, device_amg_vcycle.cu:1113-1126), fused with the first
                               post-sweep; use with flexible = 1 */
+    int captureVcycle;     /* 1: the iterations between two polls (checkEvery, rounded up to even) are
+                              captured once into a HIP graph and replayed (the reference's captureVcycle,
+                              deviceAMGPCGGraph, device_amg_pcg.cu:183-411); the result is the same as
+                              without. Needs a context stream other than the legacy default stream. */
 } mfv_pcg_controls;
 
 /* SolverPerformance (src/OpenFOAM/matrices/pcg.cuh:16-21) */

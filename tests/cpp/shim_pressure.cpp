@@ -1,6 +1,7 @@
 // A C++ caller of the pressure path through brae_mfv.hpp, written the way brae's own drivers call it
 // (simpleFoam.cu:343-381): assemble -laplacian on a box, fold, reference cell, buildAMG, amgGalerkin,
-// deviceAMGPCG, residual check with deviceAmul and deviceNormFactorInto, flux; and the error
+// deviceAMGPCG, residual check with deviceAmul and deviceNormFactorInto, flux; OpenFOAM's GAMG on the
+// same system (faceAreaPairGamgAgglomeration, gamgSolveFolded, pcgGamgSolveFolded); and the error
 // behaviour (refusals throw std::runtime_error("brae hip: ...")). Prints one JSON line; exit status 0
 // when every check holds.
 #include "brae_mfv.hpp"
@@ -161,11 +162,33 @@ int main(int argc, char** argv)
     DeviceBuffer<scalar> lower(std::vector<scalar>(nI, -0.5));
     const bool refusesAsym = throwsBrae([&] { amgGalerkin(amg, diagC, upper, lower); });
 
+    // OpenFOAM's GAMG on the same folded system: as the solver and as the PCG's preconditioner
+    const std::vector<scalar> SfAll = dm.Sf.download();
+    const std::vector<scalar> SfI(SfAll.begin(), SfAll.begin() + 3 * (size_t)nI);
+    AMGData gamg = faceAreaPairGamgAgglomeration(own, box.neighbour, SfI, magSf, nC, 10);
+    GamgControls gc;
+    gc.tolerance = 1e-8;
+    DeviceBuffer<scalar> psiG(std::vector<scalar>(nC, 0.0)), psiP(std::vector<scalar>(nC, 0.0));
+    const SolverPerformance pg = gamgSolveFolded(gamg, diagC, upper, b, psiG, gc);
+    const SolverPerformance pp = pcgGamgSolveFolded(gamg, diagC, upper, b, psiP, 1e-8, 0.0, 500, 0,
+                                                    GamgPreconditionerControls{});
+    const bool refusesSmoother = throwsBrae([&] {
+        GamgControls x;
+        x.smoother = "nonBlockingGaussSeidel";
+        DeviceBuffer<scalar> t(std::vector<scalar>(nC, 0.0));
+        gamgSolveFolded(gamg, diagC, upper, b, t, x);
+    });
+
     const bool ok = perf.nIterations > 0 && perf.nIterations < 500 && perf.finalResidual < 1e-8 && res < 2e-8 &&
-                    refusesSA && refusesAsym && flux.size() == (size_t)nI;
+                    refusesSA && refusesAsym && flux.size() == (size_t)nI && pg.converged && pg.finalResidual < 1e-8 &&
+                    pp.converged && pp.finalResidual < 1e-8 && refusesSmoother;
     std::printf("{\"cells\": %d, \"levels\": %d, \"iterations\": %d, \"initial\": %.6e, \"final\": %.6e, "
-                "\"recomputed\": %.6e, \"refuses_sa\": %s, \"refuses_asym\": %s, \"ok\": %s}\n",
+                "\"recomputed\": %.6e, \"refuses_sa\": %s, \"refuses_asym\": %s, \"gamg_levels\": %d, "
+                "\"gamg_cycles\": %d, \"gamg_final\": %.6e, \"pcg_gamg_iterations\": %d, \"pcg_gamg_final\": %.6e, "
+                "\"refuses_smoother\": %s, \"ok\": %s}\n",
                 nC, amg.nLevels(), perf.nIterations, perf.initialResidual, perf.finalResidual, res,
-                refusesSA ? "true" : "false", refusesAsym ? "true" : "false", ok ? "true" : "false");
+                refusesSA ? "true" : "false", refusesAsym ? "true" : "false", gamg.nLevels(), pg.nIterations,
+                pg.finalResidual, pp.nIterations, pp.finalResidual, refusesSmoother ? "true" : "false",
+                ok ? "true" : "false");
     return ok ? 0 : 1;
 }

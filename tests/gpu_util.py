@@ -11,17 +11,23 @@ def torch_cuda():
 
 
 def dev(a, dtype=None):
+    """a device copy, complete before return (the library's contexts run on their own streams)"""
     torch = torch_cuda()
     a = np.ascontiguousarray(a)
     t = torch.from_numpy(a.copy())
     if dtype is not None:
         t = t.to(dtype)
-    return t.to("cuda:0").contiguous()
+    t = t.to("cuda:0").contiguous()
+    torch.cuda.synchronize()
+    return t
 
 
 def zeros(n, dtype="float64"):
+    """zeros on the device, written before return"""
     torch = torch_cuda()
-    return torch.zeros(int(n), dtype=getattr(torch, dtype), device="cuda:0")
+    t = torch.zeros(int(n), dtype=getattr(torch, dtype), device="cuda:0")
+    torch.cuda.synchronize()
+    return t
 
 
 def host(t):

@@ -56,3 +56,18 @@ def test_amg_signature_fingerprints_inputs():
     assert mfv.amg_signature(m["nCells"], own, nei, w, mergeLevels=2) != s0
     assert mfv.amg_signature(m["nCells"], own, nei, w, agglomeration=mfv.AGG_FACEAREAPAIR) != s0
     assert mfv.amg_signature(m["nCells"], own, nei, w, nCellsInCoarsestLevel=8) != s0
+
+
+def test_faceareapair_weights_match_the_oracle():
+    """The library's host restatement of faceAreaPair's face weights equals the oracle's to the bit."""
+    import numpy as np
+    import oracle_lib as O
+    rng = np.random.default_rng(4)
+    nF = 1000
+    Sf = rng.normal(size=(nF, 3))
+    magSf = np.linalg.norm(Sf, axis=1)
+    w = np.zeros(nF)
+    Sf_c = np.ascontiguousarray(Sf.ravel())
+    assert mfv.lib().mfv_faceareapair_weights(nF, Sf_c.ctypes.data, magSf.ctypes.data, w.ctypes.data) == mfv.MFV_OK
+    assert np.array_equal(w, O.faceareapair_weights(Sf, magSf))
+    assert mfv.lib().mfv_faceareapair_weights(-1, None, None, None) == mfv.MFV_ERR_INVALID

@@ -233,6 +233,7 @@ def test_case20_trace_fp32():
     dd, du, db = G.dev(d["m_diag"]), G.dev(d["m_upper"]), G.dev(d["rhs"])
     amg.galerkin(dd, du, mfv.PREC_FP32)
     nf = torch.tensor([np.abs(d["rhs"]).sum() + 1e-20], dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
     cyc = d["brae_cycles"]
     for k in range(1, 8):
         psi = G.zeros(m.nCells)
@@ -327,3 +328,31 @@ def test_hierarchy_cache_and_clone(systems, tmp_path):
     assert ik == ia and np.array_equal(xk, xa)
     for o in (a, b, c, m2, e, f, k):
         o.close()
+
+
+@pytest.mark.parametrize("prec", [mfv.PREC_FP32, mfv.PREC_FP16])
+def test_pcg_graph_replays_match_enqueued(systems, prec):
+    """captureVcycle: groups of iterations captured once and replayed. Every replayed solve -- with eager
+    solves, a new right-hand side in the same buffer and synchronisations in between -- equals the
+    enqueued solve to the bit (same kernels, same reduction order)."""
+    S = systems[1]
+    print("case:", S.label, "graph", prec, flush=True)
+    ag, amg = build_pair(S, "brae")
+    amg.galerkin(S.dd, S.du, prec)
+    kw = dict(tolerance=1e-8, precision=prec, flexible=1, scaleCorrection=1, checkEvery=3)
+    rhs = [S.b, np.random.default_rng(11).normal(size=S.nC)]
+    torch = G.torch_cuda()
+    db, psi_e, psi_g = G.dev(rhs[0]), G.zeros(S.nC), G.zeros(S.nC)   # the same buffers throughout
+    for rep in range(3):
+        for k, bh in enumerate(rhs):
+            db.copy_(G.dev(bh))
+            psi_e.zero_()
+            psi_g.zero_()
+            torch.cuda.synchronize()
+            pe = amg.solve(S.dd, S.du, db, psi_e, captureVcycle=0, **kw)
+            pg = amg.solve(S.dd, S.du, db, psi_g, captureVcycle=1, **kw)
+            print("  rep", rep, "rhs", k, "iterations", pe.nIterations, pg.nIterations, flush=True)
+            assert pg.nIterations == pe.nIterations and pg.finalResidual == pe.finalResidual
+            assert np.array_equal(G.host(psi_g), G.host(psi_e))
+            assert pe.converged
+    amg.close()
