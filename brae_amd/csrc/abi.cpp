@@ -346,6 +346,17 @@ int mfv_div_upwind_coeffs(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, double
     });
 }
 
+int mfv_div_central_coeffs(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double* dWeights, double* dDiag,
+                           double* dUpper, double* dLower)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPhi && dWeights && dDiag && dUpper && dLower, "div: null argument");
+        need(dUpper != dLower, "div: a convection matrix is asymmetric; upper and lower must be separate buffers");
+        launchDivCentral(ctx->c, mesh->m, dPhi, dWeights, dDiag, dUpper, dLower);
+    });
+}
+
 int mfv_div_bcoeffs(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue, const double* dPhib,
                     double* dIC, double* dBC)
 {
@@ -884,6 +895,23 @@ int mfv_laplacian_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dGrad, const do
         need(!mesh->m.nFaces || (dGrad && dWeights && dCorrVec && dGammaf && dMagSf && dFfc),
              "laplacian correction: null argument");
         launchLaplacianCorr(ctx->c, mesh->m, dGrad, dWeights, dCorrVec, dGammaf, dMagSf, dFfc, dSource);
+    });
+}
+
+int mfv_laplacian_corr_limited(mfv_ctx ctx, mfv_mesh mesh, const double* dGrad, const double* dWeights,
+                               const double* dCorrVec, const double* dGammaf, const double* dMagSf,
+                               const double* dNonOrthDeltaCoeffs, const double* dVf, double limitCoeff,
+                               double* dFfc, double* dSource)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(!mesh->m.nFaces || (dGrad && dWeights && dCorrVec && dGammaf && dMagSf && dFfc),
+             "laplacian correction: null argument");
+        const bool limited = limitCoeff > 0.0 && limitCoeff < 1.0;
+        need(!limited || !mesh->m.nFaces || (dNonOrthDeltaCoeffs && dVf),
+             "limited laplacian correction: nonOrthDeltaCoeffs and the field are needed");
+        launchLaplacianCorr(ctx->c, mesh->m, dGrad, dWeights, dCorrVec, dGammaf, dMagSf, dFfc, dSource,
+                            dNonOrthDeltaCoeffs, dVf, limitCoeff);
     });
 }
 

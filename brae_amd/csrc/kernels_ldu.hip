@@ -203,6 +203,20 @@ __global__ __launch_bounds__(kTpb) void divFaceK(int nF, const double* __restric
     upper[f] = lo + ph;
 }
 
+// fvm::div(phi, psi) Gauss linear faces: the geometric weight in place of pos0(phi)
+// (divFaceLinearKernel, device_fvm.cu:43-61)
+__global__ __launch_bounds__(kTpb) void divFaceLinearK(int nF, const double* __restrict__ phi,
+                                                       const double* __restrict__ w, double* __restrict__ upper,
+                                                       double* __restrict__ lower)
+{
+    const int f = xcdBlock() * kTpb + threadIdx.x;
+    if (f >= nF) return;
+    const double ph = phi[f];
+    const double lo = -w[f] * ph;
+    lower[f] = lo;
+    upper[f] = lo + ph;
+}
+
 // diag -= lower (owned faces), -= upper (neighbour faces), in face order
 __global__ __launch_bounds__(kTpb) void divDiagK(int n, const label* __restrict__ ownerStart,
                                                  const label* __restrict__ losortStart,
@@ -405,6 +419,31 @@ __global__ __launch_bounds__(kTpb) void corrFluxK(int nF, const label* __restric
     ffc[f] = (gammaf[f] * magSf[f]) * corr;
 }
 
+// `limited k corrected` (limitedSnGrad; fvm.cuh:170-211, lapCorrFaceLimitedKernel device_fvm.cu:395-437): the
+// correction capped per face against the orthogonal snGrad, lim = min(k|orth| / ((1-k)|corr| + 1e-15), 1)
+__global__ __launch_bounds__(kTpb) void corrFluxLimitedK(int nF, const label* __restrict__ owner,
+                                                         const label* __restrict__ neighbour,
+                                                         const double* __restrict__ grad, const double* __restrict__ w,
+                                                         const double* __restrict__ cv,
+                                                         const double* __restrict__ gammaf,
+                                                         const double* __restrict__ magSf,
+                                                         const double* __restrict__ nonOrthDc,
+                                                         const double* __restrict__ vf, double k,
+                                                         double* __restrict__ ffc)
+{
+    const int f = xcdBlock() * kTpb + threadIdx.x;
+    if (f >= nF) return;
+    const int o = owner[f], nb = neighbour[f];
+    const double wf = w[f], wn = 1.0 - wf;
+    const double fx = wf * grad[3 * o] + wn * grad[3 * nb];
+    const double fy = wf * grad[3 * o + 1] + wn * grad[3 * nb + 1];
+    const double fz = wf * grad[3 * o + 2] + wn * grad[3 * nb + 2];
+    const double corr = cv[3 * f] * fx + cv[3 * f + 1] * fy + cv[3 * f + 2] * fz;
+    const double orth = nonOrthDc[f] * (vf[nb] - vf[o]);
+    const double lim = fmin(k * fabs(orth) / ((1.0 - k) * fabs(corr) + 1e-15), 1.0);
+    ffc[f] = (gammaf[f] * magSf[f]) * (lim * corr);
+}
+
 // source -= sum over the cell's faces of (+ffc owned, -ffc neighbour-side), ascending face index
 // (fvm.cuh:293-299, pEqn_cpp.cu:228)
 __global__ __launch_bounds__(kTpb) void corrSourceK(int n, const label* __restrict__ ownerStart,
@@ -460,9 +499,14 @@ void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcV
 }
 
 void launchLaplacianCorr(Ctx& c, const Mesh& m, const double* grad, const double* w, const double* corrVec,
-                         const double* gammaf, const double* magSf, double* ffc, double* source)
+                         const double* gammaf, const double* magSf, double* ffc, double* source,
+                         const double* nonOrthDc, const double* vf, double limitCoeff)
 {
-    if (m.nFaces)
+    const bool limited = limitCoeff > 0.0 && limitCoeff < 1.0;
+    if (m.nFaces && limited)
+        corrFluxLimitedK<<<blocks(m.nFaces), kTpb, 0, c.stream>>>(m.nFaces, m.owner.p, m.neighbour.p, grad, w, corrVec,
+                                                                  gammaf, magSf, nonOrthDc, vf, limitCoeff, ffc);
+    else if (m.nFaces)
         corrFluxK<<<blocks(m.nFaces), kTpb, 0, c.stream>>>(m.nFaces, m.owner.p, m.neighbour.p, grad, w, corrVec, gammaf,
                                                            magSf, ffc);
     if (source && m.nCells)
@@ -539,6 +583,16 @@ void launchDivUpwind(Ctx& c, const Mesh& m, const double* phi, double* diag, dou
         divDiagK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.ownerStart.p, m.losortStart.p, m.losort.p,
                                                           upper, lower, diag);
     hipCheck(hipGetLastError(), "div upwind");
+}
+
+void launchDivCentral(Ctx& c, const Mesh& m, const double* phi, const double* weights, double* diag, double* upper,
+                      double* lower)
+{
+    if (m.nFaces) divFaceLinearK<<<blocks(m.nFaces), kTpb, 0, c.stream>>>(m.nFaces, phi, weights, upper, lower);
+    if (m.nCells)
+        divDiagK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.ownerStart.p, m.losortStart.p, m.losort.p,
+                                                          upper, lower, diag);
+    hipCheck(hipGetLastError(), "div linear");
 }
 
 void launchDivBCoeffs(Ctx& c, int nB, const int* bcType, const double* bcValue, const double* phib, double* iC,

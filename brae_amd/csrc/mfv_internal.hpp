@@ -194,6 +194,8 @@ void launchLaplacian(Ctx& c, const Mesh& m, const double* dc, const double* gamm
 void launchLaplacianBCoeffs(Ctx& c, int nB, const int* bcType, const double* bcValue, const double* gammab,
                             const double* bMagSf, const double* bDelta, double* iC, double* bC);
 void launchDivUpwind(Ctx& c, const Mesh& m, const double* phi, double* diag, double* upper, double* lower);
+void launchDivCentral(Ctx& c, const Mesh& m, const double* phi, const double* weights, double* diag, double* upper,
+                      double* lower);
 void launchDivBCoeffs(Ctx& c, int nB, const int* bcType, const double* bcValue, const double* phib, double* iC,
                       double* bC);
 void launchFold(Ctx& c, const Mesh& m, const double* diag, const double* source, const double* iC, const double* bC,
@@ -207,8 +209,10 @@ void launchFluxCorrect(Ctx& c, const Mesh& m, const LduView& A, const double* ps
                        const double* phiHbyA, double sign, double* phi);
 void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcValue, const double* Sf,
                      const double* w, const double* V, const double* psi, double* grad);
+// limitCoeff in (0, 1): `limited k corrected` against nonOrthDc*(vf[nei] - vf[own]); otherwise unlimited
 void launchLaplacianCorr(Ctx& c, const Mesh& m, const double* grad, const double* w, const double* corrVec,
-                         const double* gammaf, const double* magSf, double* ffc, double* source);
+                         const double* gammaf, const double* magSf, double* ffc, double* source,
+                         const double* nonOrthDc = nullptr, const double* vf = nullptr, double limitCoeff = 0.0);
 
 // ---- sequential-order sweeps on the level schedule (dic.hip) ----
 struct PcgControls;

@@ -101,6 +101,7 @@ _SIGS = {
     "mfv_laplacian_bcoeffs": [P, P, P, P, P, P, P, P, P],
     "mfv_div_upwind_coeffs": [P, P, P, P, P, P],
     "mfv_div_bcoeffs": [P, P, P, P, P, P, P],
+    "mfv_div_central_coeffs": [P, P, P, P, P, P, P],
     "mfv_fold": [P, P, P, P, P, P, P, P],
     "mfv_set_reference": [P, P, P, I, D],
     "mfv_face_flux": [P, P, P, P, P],
@@ -114,6 +115,7 @@ _SIGS = {
     "mfv_flux_correct": [P, P, P, P, P, P, D, P],
     "mfv_grad_gauss": [P, P, P, P, P, P, P, P, P],
     "mfv_laplacian_corr": [P, P, P, P, P, P, P, P, P],
+    "mfv_laplacian_corr_limited": [P, P, P, P, P, P, P, P, P, D, P, P],
     "mfv_mesh_geometry": [P, P, P, P, P] + [P] * 10,
     "mfv_ctx_profile": [P, I],
     "mfv_ctx_profile_read": [P, I, P, P],

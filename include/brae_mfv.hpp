@@ -272,6 +272,20 @@ inline void deviceDivUpwindCoeffs(const DeviceMesh& dm, const DeviceBuffer<scala
              "deviceDivUpwindCoeffs");
 }
 
+// deviceDivCentralCoeffs (device_fvm.cu:628-643): Gauss linear convection on the mesh's weights
+inline void deviceDivCentralCoeffs(const DeviceMesh& dm, const DeviceBuffer<scalar>& phiInt,
+                                   DeviceBuffer<scalar>& diag, DeviceBuffer<scalar>& upper,
+                                   DeviceBuffer<scalar>& lower)
+{
+    mfv_ctx ctx = threadContext();
+    diag.resize(dm.nCells);
+    upper.resize(dm.nInternalFaces);
+    lower.resize(dm.nInternalFaces);
+    mfvCheck(mfv_div_central_coeffs(ctx, dm.mesh, phiInt.data(), dm.weights.data(), diag.data(), upper.data(),
+                                    lower.data()),
+             ctx, "deviceDivCentralCoeffs");
+}
+
 // deviceFold (device_fvm.cu:895-912): diagC = rawDiag + iC, b = source + bC (flattened boundary faces)
 inline void deviceFold(const DeviceMesh& dm, const DeviceBuffer<scalar>& rawDiag, const DeviceBuffer<scalar>& source,
                        const DeviceBuffer<scalar>& iC, const DeviceBuffer<scalar>& bC, DeviceBuffer<scalar>& diagC,
@@ -324,6 +338,20 @@ inline void deviceLaplacianCorrFlux(const DeviceMesh& dm, const DeviceBuffer<sca
     mfvCheck(mfv_laplacian_corr(ctx, dm.mesh, grad.data(), dm.weights.data(), dm.corrVec.data(), gammafInt.data(),
                                 dm.magSfInternal.data(), ffc.data(), src ? src->data() : nullptr),
              ctx, "deviceLaplacianCorrFlux");
+}
+
+// deviceLaplacianCorrFluxLimited (device_fvm.cu:497-517): `limited psi corrected`, the correction capped per
+// face against the orthogonal snGrad of phi; psi >= 1 is the unlimited correction
+inline void deviceLaplacianCorrFluxLimited(const DeviceMesh& dm, const DeviceBuffer<scalar>& gammafInt,
+                                           const DeviceBuffer<scalar>& phi, const DeviceBuffer<scalar>& grad,
+                                           scalar psi, DeviceBuffer<scalar>& ffc, DeviceBuffer<scalar>* src = nullptr)
+{
+    mfv_ctx ctx = threadContext();
+    ffc.resize(dm.nInternalFaces);
+    mfvCheck(mfv_laplacian_corr_limited(ctx, dm.mesh, grad.data(), dm.weights.data(), dm.corrVec.data(),
+                                        gammafInt.data(), dm.magSfInternal.data(), dm.nonOrthDeltaCoeffs.data(),
+                                        phi.data(), psi, ffc.data(), src ? src->data() : nullptr),
+             ctx, "deviceLaplacianCorrFluxLimited");
 }
 
 // ---------------- products and norms ----------------

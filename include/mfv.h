@@ -99,6 +99,10 @@ int mfv_laplacian_bcoeffs(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const 
 /* deviceDivUpwindCoeffs (src/cuda/device_fvm.cu:646-662): lower = -pos0(phi)*phi, upper = lower + phi */
 int mfv_div_upwind_coeffs(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, double* dDiag, double* dUpper,
                           double* dLower);
+/* deviceDivCentralCoeffs (src/cuda/device_fvm.cu:628-643; Gauss linear): lower = -w*phi, upper = lower + phi
+ * with the geometric weights dWeights [nFaces] (mfv_mesh_geometry's) */
+int mfv_div_central_coeffs(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double* dWeights, double* dDiag,
+                           double* dUpper, double* dLower);
 /* fvm::div boundary coefficients (fvm.cuh:545-552) */
 int mfv_div_bcoeffs(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue, const double* dPhib,
                     double* dIC, double* dBC);
@@ -137,6 +141,14 @@ int mfv_grad_gauss(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double*
 int mfv_laplacian_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dGrad, const double* dWeights,
                        const double* dCorrVec, const double* dGammaf, const double* dMagSf, double* dFfc,
                        double* dSource);
+/* `limited k corrected` (deviceLaplacianCorrFluxLimited, src/cuda/device_fvm.cu:497-517; fvm.cuh:170-211):
+ * as mfv_laplacian_corr with each face's correction scaled by min(k|orth| / ((1-k)|corr| + 1e-15), 1),
+ * orth = dNonOrthDeltaCoeffs*(dVf[nei] - dVf[own]) the orthogonal snGrad of the corrected field dVf;
+ * k outside (0, 1) is the unlimited correction */
+int mfv_laplacian_corr_limited(mfv_ctx ctx, mfv_mesh mesh, const double* dGrad, const double* dWeights,
+                               const double* dCorrVec, const double* dGammaf, const double* dMagSf,
+                               const double* dNonOrthDeltaCoeffs, const double* dVf, double limitCoeff,
+                               double* dFfc, double* dSource);
 
 /* FvGeometry::build (src/finiteVolume/fvMesh/fv_geometry.cu:13-158) and the patches' deltaCoeffs
  * (fv_patch.cu:160-165) on the device. Faces are polygons given by dFaceOffsets [nFaces+1] into

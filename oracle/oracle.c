@@ -247,6 +247,21 @@ void orc_div_upwind(int nCells, int nFaces, const int* owner, const int* neighbo
     }
 }
 
+/* fvm::div(phi, psi) Gauss linear (deviceDivCentralCoeffs, src/cuda/device_fvm.cu:43-61,628-643): the upwind
+ * coefficients with the geometric weights w in place of pos0(phi) */
+void orc_div_linear(int nCells, int nFaces, const int* owner, const int* neighbour, const double* phi,
+                    const double* weights, double* diag, double* upper, double* lower)
+{
+    for (int c = 0; c < nCells; ++c) diag[c] = 0.0;
+    for (int f = 0; f < nFaces; ++f)
+    {
+        lower[f] = -weights[f] * phi[f];
+        upper[f] = lower[f] + phi[f];
+        diag[owner[f]] -= lower[f];
+        diag[neighbour[f]] -= upper[f];
+    }
+}
+
 /* fvm.cuh:545-552: iC = phi_b*valueInternalCoeffs, bC = -phi_b*valueBoundaryCoeffs; fixedValue
  * (0, value), zeroGradient (1, 0). */
 void orc_div_bcoeffs(int nBFaces, const int* bcType, const double* bcValue, const double* phib,
@@ -413,6 +428,41 @@ void orc_laplacian_corr(int nCells, int nFaces, const int* owner, const int* nei
         const vec3 gf = v_add(v_scale(w, v_get(grad, owner[f])), v_scale(1.0 - w, v_get(grad, neighbour[f])));
         const vec3 cv = v_get(corrVec, f);
         const double corr = cv.x * gf.x + cv.y * gf.y + cv.z * gf.z;
+        ffc[f] = (gammaf[f] * magSf[f]) * corr;
+    }
+    if (!source) return;
+    double* src = (double*)xcalloc((size_t)nCells, sizeof(double));
+    for (int f = 0; f < nFaces; ++f)
+    {
+        src[owner[f]] += ffc[f];
+        src[neighbour[f]] -= ffc[f];
+    }
+    for (int c = 0; c < nCells; ++c) source[c] -= src[c];
+    free(src);
+}
+
+/* `limited k corrected` (fvm.cuh:170-211, OpenFOAM limitedSnGrad): per face the correction corr is capped
+ * against the orthogonal snGrad orth = nonOrthDc*(vf[nei] - vf[own]),
+ *   lim = min(k*|orth| / ((1 - k)*|corr| + 1e-15), 1),  ffc = (gamma*magSf)*(lim*corr);
+ * k outside (0, 1) is the unlimited correction. The source as in orc_laplacian_corr. */
+void orc_laplacian_corr_limited(int nCells, int nFaces, const int* owner, const int* neighbour, const double* grad,
+                                const double* weights, const double* corrVec, const double* gammaf,
+                                const double* magSf, const double* nonOrthDc, const double* vf, double limitCoeff,
+                                double* ffc, double* source)
+{
+    const int limited = limitCoeff > 0.0 && limitCoeff < 1.0;
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const double w = weights[f];
+        const vec3 gf = v_add(v_scale(w, v_get(grad, owner[f])), v_scale(1.0 - w, v_get(grad, neighbour[f])));
+        const vec3 cv = v_get(corrVec, f);
+        double corr = cv.x * gf.x + cv.y * gf.y + cv.z * gf.z;
+        if (limited)
+        {
+            const double orth = nonOrthDc[f] * (vf[neighbour[f]] - vf[owner[f]]);
+            const double lim = fmin(limitCoeff * fabs(orth) / ((1.0 - limitCoeff) * fabs(corr) + 1e-15), 1.0);
+            corr = lim * corr;
+        }
         ffc[f] = (gammaf[f] * magSf[f]) * corr;
     }
     if (!source) return;

@@ -59,6 +59,8 @@ void orc_laplacian_bcoeffs(int nBFaces, const int* bcType, const double* bcValue
 /* ---- fvm::div(phi, psi) upwind interior + boundary: fvm.cuh:476-554, 556-619 ---- */
 void orc_div_upwind(int nCells, int nFaces, const int* owner, const int* neighbour, const double* phi,
                     double* diag, double* upper, double* lower);
+void orc_div_linear(int nCells, int nFaces, const int* owner, const int* neighbour, const double* phi,
+                    const double* weights, double* diag, double* upper, double* lower);
 void orc_div_bcoeffs(int nBFaces, const int* bcType, const double* bcValue, const double* phib,
                      double* iC, double* bC);
 
@@ -89,6 +91,10 @@ void orc_grad_gauss(int nCells, int nFaces, const int* owner, const int* neighbo
 void orc_laplacian_corr(int nCells, int nFaces, const int* owner, const int* neighbour, const double* grad,
                         const double* weights, const double* corrVec, const double* gammaf, const double* magSf,
                         double* ffc, double* source);
+void orc_laplacian_corr_limited(int nCells, int nFaces, const int* owner, const int* neighbour, const double* grad,
+                                const double* weights, const double* corrVec, const double* gammaf,
+                                const double* magSf, const double* nonOrthDc, const double* vf, double limitCoeff,
+                                double* ffc, double* source);
 void orc_flux_correct(int nFaces, const int* owner, const int* neighbour, const double* upper, const double* lower,
                       const double* psi, const double* ffc, const double* phiHbyA, double* phi);
 

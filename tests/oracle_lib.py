@@ -47,6 +47,8 @@ def lib():
             "orc_laplacian_bcoeffs": (None, [I, P, P, P, P, P, P, P]),
             "orc_div_upwind": (None, [I, I, P, P, P, P, P, P]),
             "orc_div_bcoeffs": (None, [I, P, P, P, P, P]),
+            "orc_div_linear": (None, [I, I, P, P, P, P, P, P, P]),
+            "orc_laplacian_corr_limited": (None, [I, I, P, P, P, P, P, P, P, P, P, D, P, P]),
             "orc_fold": (None, [I, P, P, I, P, P, P, P, P, P]),
             "orc_set_reference": (None, [P, P, I, D]),
             "orc_amul": (None, [I, I, P, P, P, P, P, P, P]),
@@ -205,6 +207,16 @@ def laplacian_corr(nC, own, nei, grad, weights, corrVec, gammaf, magSf, source=N
     src = None if source is None else f64(source).copy()
     lib().orc_laplacian_corr(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(grad)), p(f64(weights)), p(f64(corrVec)),
                              p(f64(gammaf)), p(f64(magSf)), p(ffc), p(src))
+    return ffc, src
+
+
+def laplacian_corr_limited(nC, own, nei, grad, weights, corrVec, gammaf, magSf, nonOrthDc, vf, k, source=None):
+    """`limited k corrected` correction flux (and source -= div(ffc) when a source is given)"""
+    ffc = np.zeros(len(own))
+    src = None if source is None else f64(source).copy()
+    lib().orc_laplacian_corr_limited(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(grad)), p(f64(weights)),
+                                     p(f64(corrVec)), p(f64(gammaf)), p(f64(magSf)), p(f64(nonOrthDc)), p(f64(vf)),
+                                     float(k), p(ffc), p(src))
     return ffc, src
 
 

@@ -325,6 +325,17 @@ def test_nonorth_correction_bit_exact(shape, jitter):
     case.sync()
     assert np.array_equal(G.host(dffc), ref_ffc)
     assert np.array_equal(G.host(dsrc), ref_src)
+    # `limited k corrected` against the orthogonal snGrad of psi
+    dndc = G.dev(m["nonOrthDeltaCoeffs"])
+    for k in (0.333, 0.5):
+        lim_ffc, lim_src = O.laplacian_corr_limited(nC, own, nei, ref_g, m["weights"], m["corrVec"], gamma,
+                                                    m["magSf"][:nI], m["nonOrthDeltaCoeffs"], psi, k, source=src0)
+        dffl, dsrl = G.zeros(nI), G.dev(src0)
+        case.call("mfv_laplacian_corr_limited", case.mesh.h, mfv.ptr(dg), mfv.ptr(dw), mfv.ptr(dcv), mfv.ptr(dgam),
+                  mfv.ptr(dms), mfv.ptr(dndc), mfv.ptr(dpsi), k, mfv.ptr(dffl), mfv.ptr(dsrl))
+        case.sync()
+        assert np.array_equal(G.host(dffl), lim_ffc), k
+        assert np.array_equal(G.host(dsrl), lim_src), k
     # phi = phiHbyA - sign*(flux + ffc): with sign -1 and the negated matrix this is the reference's
     # phi = phiHbyA - (flux(A) + ffc) where A = -A'
     up = rng.normal(size=nI)
@@ -336,3 +347,19 @@ def test_nonorth_correction_bit_exact(shape, jitter):
               mfv.ptr(out))
     case.sync()
     assert np.array_equal(G.host(out), ref_phi)
+
+
+def test_div_central_bit_exact(pitz):
+    """Gauss linear convection on the mesh's weights, against the oracle (deviceDivCentralCoeffs)."""
+    d, mesh, case = pitz
+    print("case: pitzDaily div linear", flush=True)
+    nC, nI = mesh.nCells, mesh.nInternal
+    phi, w = d["phi_internal"], mesh.weights
+    od, ou, ol = np.zeros(nC), np.zeros(nI), np.zeros(nI)
+    O.lib().orc_div_linear(nC, nI, O.p(mesh.own()), O.p(mesh.neighbour), O.p(O.f64(phi)), O.p(O.f64(w)), O.p(od),
+                           O.p(ou), O.p(ol))
+    diag, up, lo = G.zeros(nC), G.zeros(nI), G.zeros(nI)
+    case.call("mfv_div_central_coeffs", case.mesh.h, mfv.ptr(G.dev(phi)), mfv.ptr(G.dev(w)), mfv.ptr(diag),
+              mfv.ptr(up), mfv.ptr(lo))
+    case.sync()
+    assert np.array_equal(G.host(diag), od) and np.array_equal(G.host(up), ou) and np.array_equal(G.host(lo), ol)

@@ -79,6 +79,39 @@ def test_div_upwind_matches_openfoam_dump(pitz):
         assert rel(bC[s], d["div_bC_" + name]) <= 1e-12, name
 
 
+def test_div_linear_pinned_by_upwind_and_divergence(pitz):
+    """Gauss linear convection (deviceDivCentralCoeffs): with the weights set to pos0(phi) it is the upwind
+    matrix that matches OpenFOAM's dump above, bit for bit; with the mesh's geometric weights (any
+    weights) the matrix applied to a constant field is the internal-face divergence of phi; and at the
+    weight's end points a face takes the neighbour's (w = 0) or the owner's (w = 1) value alone."""
+    d, mesh = pitz
+    nC, nI = mesh.nCells, mesh.nInternal
+    own, nei = mesh.own(), mesh.neighbour
+    phi = O.f64(d["phi_internal"])
+
+    def lin(w):
+        diag, upper, lower = np.zeros(nC), np.zeros(nI), np.zeros(nI)
+        O.lib().orc_div_linear(nC, nI, O.p(own), O.p(nei), O.p(phi), O.p(O.f64(w)), O.p(diag), O.p(upper), O.p(lower))
+        return diag, upper, lower
+
+    ud, uu, ul = lin((phi >= 0).astype(np.float64))
+    assert rel(ud, d["div_diag"]) <= 1e-12 and rel(uu, d["div_upper"]) <= 1e-12 and rel(ul, d["div_lower"]) <= 1e-12
+    ref = [np.zeros(nC), np.zeros(nI), np.zeros(nI)]
+    O.lib().orc_div_upwind(nC, nI, O.p(own), O.p(nei), O.p(phi), *[O.p(a) for a in ref])
+    assert all(np.array_equal(a, b) for a, b in zip((ud, uu, ul), ref))
+    diag, upper, lower = lin(mesh.weights)
+    one = O.amul(nC, own, nei, diag, upper, lower, np.ones(nC))
+    div = np.zeros(nC)
+    np.add.at(div, own, phi)
+    np.add.at(div, nei, -phi)
+    assert np.abs(one - div).max() <= 1e-12 * np.abs(phi).max()
+    # the end points of the weight: w = 0 takes the neighbour's value (lower 0, upper phi), w = 1 the owner's
+    d0, u0, l0 = lin(np.zeros(nI))
+    d1, u1, l1 = lin(np.ones(nI))
+    assert np.array_equal(l0, np.zeros(nI)) and np.array_equal(u0, phi)
+    assert np.array_equal(l1, -phi) and np.array_equal(u1, np.zeros(nI))
+
+
 def test_amul_matches_openfoam_dump(pitz):
     """test_spmv.cu:42-47: A.psi against amul.dat at rel <= 1e-12 (A = laplacian(1,T), unfolded)."""
     d, mesh = pitz

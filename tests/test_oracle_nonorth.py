@@ -109,3 +109,37 @@ def test_flux_correct_adds_the_correction():
     phi0 = O.flux_correct(own, nei, up, None, psi, None, phiH)
     assert np.allclose(phi0 - phi, ffc, rtol=0, atol=1e-12)
     assert np.allclose(phi0, phiH - (up * psi[nei] - up * psi[own]), rtol=0, atol=1e-12)
+
+
+def test_limited_correction_caps_against_the_orthogonal_part():
+    """`limited k corrected`: k outside (0, 1) is the unlimited correction to the bit; inside, a face keeps
+    its full correction (to the bit) where k|orth| >= (1-k)|corr|, and elsewhere its flux magnitude is
+    exactly gamma|Sf| k|orth| |corr| / ((1-k)|corr| + 1e-15) <= gamma|Sf| k/(1-k) |orth|; the source
+    loses exactly the divergence of the limited flux."""
+    m, own, nei, bfc = mesh(0.3, seed=5)
+    nC, nI = m["nCells"], len(own)
+    rng = np.random.default_rng(6)
+    vf = rng.normal(size=nC)
+    grad = rng.normal(size=(nC, 3)) * 20.0   # a rough gradient: large corrections on the jittered faces
+    gam = rng.uniform(0.5, 2.0, nI)
+    args = (nC, own, nei, grad, m["weights"], m["corrVec"], gam, m["magSf"][:nI])
+    full, _ = O.laplacian_corr(*args)
+    for k in (0.0, 1.0, 1.5, -0.2):
+        ffc, _ = O.laplacian_corr_limited(*args, m["nonOrthDeltaCoeffs"], vf, k)
+        assert np.array_equal(ffc, full), k
+    for k in (0.333, 0.5, 0.9):
+        ffc, src = O.laplacian_corr_limited(*args, m["nonOrthDeltaCoeffs"], vf, k, source=np.zeros(nC))
+        orth = m["nonOrthDeltaCoeffs"] * (vf[nei] - vf[own])
+        gm = gam * m["magSf"][:nI]
+        corr = np.where(gm != 0, full / gm, 0.0)
+        keep = k * np.abs(orth) >= (1 - k) * np.abs(corr) + 1e-15
+        assert keep.any() and (~keep).any(), k
+        assert np.array_equal(ffc[keep], full[keep])
+        capped = np.abs(gm[~keep]) * k * np.abs(orth[~keep]) * np.abs(corr[~keep]) / (
+            (1 - k) * np.abs(corr[~keep]) + 1e-15)
+        assert np.allclose(np.abs(ffc[~keep]), capped, rtol=1e-12, atol=0)
+        assert (np.abs(ffc) <= np.abs(gm) * k / (1 - k) * np.abs(orth) * (1 + 1e-12) + np.abs(full) * keep).all()
+        div = np.zeros(nC)
+        np.add.at(div, own, ffc)
+        np.add.at(div, nei, -ffc)
+        assert np.allclose(src, -div, rtol=0, atol=1e-12 * np.abs(ffc).max())
