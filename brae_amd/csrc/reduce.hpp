@@ -25,6 +25,16 @@ inline int reduceGrid(int n)
     return b;
 }
 
+// a partition of `rows` rows per thread
+inline int reduceGridRows(int n, int rows)
+{
+    const long long per = (long long)kReduceBlock * rows;
+    long long b = ((long long)n + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
+    return (int)b;
+}
+
 // rows per workgroup for a reduceGrid partition (a multiple of kReduceBlock)
 __host__ __device__ __forceinline__ int chunkRows(int n, int grid)
 {
