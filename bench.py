@@ -226,19 +226,22 @@ def gpu_run(args):
 
     # roofline of the dominant kernel of the mixed step (largest summed event time)
     N, F, Nc = ch.nC, ch.nI, sizes[1][0]
-    zb = 4  # the cycle's vector width (FP32 vectors for the FP32 and FP16 cycles)
+    # widths in the mixed cycle: vectors (FP16 cycle: FP16; FP32 cycle: FP32), coefficients, the diagonal
+    # (FP32 in both)
+    zb = 4 if prec_mixed == mfv.PREC_FP32 else 2
     vb = 4 if prec_mixed == mfv.PREC_FP32 else 2
+    dw = 4
     iters = sum(q.nIterations for q in res["mixed"]["prof_perfs"])
     names = {
         mfv.PROF_PCG_SPMV64: ("pcgDirAmulK: p = z + beta*p, q = A.p (FP64), p.q", 8 * (N + F) + 8 * F + zb * N + 24 * N,
                               iters),
-        mfv.PROF_PCG_UPDATE: ("pcgUpdateK: psi, r update, sumMag(r), next cycle entry", 8 * (2 * N + 4 * N + N) + 3 * zb * N,
-                              iters),
+        mfv.PROF_PCG_UPDATE: ("pcgUpdateK: psi, r update, sumMag(r), next cycle entry",
+                              8 * (2 * N + 4 * N + N) + 2 * zb * N + dw * N, iters),
         mfv.PROF_PCG_ZR: ("pcgZrK: z.r, z.r_old", zb * N + 16 * N, iters),
-        mfv.PROF_AMG_RESIDUAL0: ("residualK grid 0 (cycle precision)", zb * N + vb * F + 3 * zb * N + 8 * F, iters),
-        mfv.PROF_AMG_CORR_AMUL0: ("corrAmulK grid 0: A e, r.e, e.Ae", zb * N + vb * F + 3 * zb * N + 8 * F, iters),
-        mfv.PROF_AMG_CORR_SMOOTH0: ("corrSmoothK grid 0", 6 * zb * N, iters),
-        mfv.PROF_AMG_RESTRICT0: ("restrictK grid 0 -> 1", zb * N + 4 * N + 4 * Nc + 3 * zb * Nc, iters),
+        mfv.PROF_AMG_RESIDUAL0: ("residualK grid 0 (cycle precision)", vb * F + dw * N + 3 * zb * N + 8 * F, iters),
+        mfv.PROF_AMG_CORR_AMUL0: ("corrAmulK grid 0: A e, r.e, e.Ae", vb * F + dw * N + 3 * zb * N + 8 * F, iters),
+        mfv.PROF_AMG_CORR_SMOOTH0: ("corrSmoothK grid 0", 5 * zb * N + dw * N, iters),
+        mfv.PROF_AMG_RESTRICT0: ("restrictK grid 0 -> 1", zb * N + 4 * N + 4 * Nc + 2 * zb * Nc + dw * Nc, iters),
     }
     prof = res["mixed"]["prof"]
     dom = max(names, key=lambda t: prof[t][1])

@@ -26,78 +26,88 @@ constexpr int kMaxDenseCoarsest = 1024;   // the dense inverse's column buffer i
 inline int blocks(long long n) { return (int)((n + kTpb - 1) / kTpb); }
 
 // ---------------- V-cycle kernels ----------------
+//
+// Every kernel takes its grid's vectors as stored (Ts: the arithmetic type Tv, or FP16 for the FP16
+// cycle) with `xm`, the grid's multiplier of solution-type vectors (null outside the FP16 cycle); see
+// CycVec.
 
 // r = b - A x  (amgSpmvF then residualT)
-template <typename Tm, typename Tv>
+template <typename Tm, typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void residualK(SellView S, const Tm* __restrict__ val, const Tv* __restrict__ diag,
-                                                  const double* __restrict__ scale, const Tv* __restrict__ x,
-                                                  const Tv* __restrict__ b, Tv* __restrict__ r,
-                                                  const int* __restrict__ stop)
+                                                  const double* __restrict__ scale, const Ts* __restrict__ x,
+                                                  const Ts* __restrict__ b, Ts* __restrict__ r,
+                                                  const float* __restrict__ xm, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
-    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, PlainX<Tv>{x});
-    r[c] = b[c] - ax;
+    const CycVec<Tv, Ts> X(x, xm), B(b, nullptr), R(r, nullptr);
+    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X);
+    R.set(c, B(c) - ax);
 }
 
 // The coarse-to-fine injection, fused into whichever kernel finishes a coarse grid's cycle: coarse cell
 // C's final value goes to every fine cell of its agglomerate (the fine grid's cellStart/cellList), so
 // the fine grid reads its correction e = xc[map] as a plain vector (prolongT's values, written once).
-template <typename Tv>
+template <typename Tv, typename Ts>
 struct Scatter
 {
     const int* __restrict__ start;
     const int* __restrict__ list;
-    Tv* __restrict__ e;   // null: no fine grid waits for this one (grid 0)
+    Ts* __restrict__ e;        // null: no fine grid waits for this one (grid 0)
+    const float* xm;           // the fine grid's multiplier
     __device__ void operator()(int C, Tv v) const
     {
         if (!e) return;
-        for (int k = start[C]; k < start[C + 1]; ++k) e[list[k]] = v;
+        const CycVec<Tv, Ts> E(e, xm);
+        for (int k = start[C]; k < start[C + 1]; ++k) E.set(list[k], v);
     }
 };
 
 // xOut = x + omega*(b - A x)/D  (amgSpmvF then smoothT)
-template <typename Tm, typename Tv>
+template <typename Tm, typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void smoothK(SellView S, const Tm* __restrict__ val, const Tv* __restrict__ diag,
-                                                const double* __restrict__ scale, const Tv* __restrict__ x,
-                                                const Tv* __restrict__ b, Tv* __restrict__ xOut, Scatter<Tv> sc,
+                                                const double* __restrict__ scale, const Ts* __restrict__ x,
+                                                const Ts* __restrict__ b, Ts* __restrict__ xOut,
+                                                const float* __restrict__ xm, Scatter<Tv, Ts> sc,
                                                 const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
-    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, PlainX<Tv>{x});
-    const Tv xo = x[c] + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
-    xOut[c] = xo;
+    const CycVec<Tv, Ts> X(x, xm), B(b, nullptr), XO(xOut, xm);
+    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X);
+    const Tv xo = X(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(diag[c]);
+    XO.set(c, xo);
     sc(c, xo);
 }
 
 // x + e at any cell: the prolonged iterate a fused first post-sweep works on
-template <typename Tv>
+template <typename Tv, typename Ts>
 struct SumX
 {
-    const Tv* __restrict__ x;
-    const Tv* __restrict__ e;
-    __device__ Tv operator()(int j) const { return x[j] + e[j]; }
+    CycVec<Tv, Ts> x, e;
+    __device__ Tv operator()(int j) const { return x(j) + e(j); }
 };
 
 // prolongation fused into the first post-sweep: xin = x + e (prolongT), then
 // xOut = xin + omega*(b - A xin)/D
-template <typename Tm, typename Tv>
+template <typename Tm, typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void prolongSmoothK(SellView S, const Tm* __restrict__ val,
                                                        const Tv* __restrict__ diag, const double* __restrict__ scale,
-                                                       const Tv* __restrict__ e, const Tv* __restrict__ x,
-                                                       const Tv* __restrict__ b, Tv* __restrict__ xOut,
-                                                       Scatter<Tv> sc, const int* __restrict__ stop)
+                                                       const Ts* __restrict__ e, const Ts* __restrict__ x,
+                                                       const Ts* __restrict__ b, Ts* __restrict__ xOut,
+                                                       const float* __restrict__ xm, Scatter<Tv, Ts> sc,
+                                                       const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
-    const SumX<Tv> X{x, e};
+    const SumX<Tv, Ts> X{CycVec<Tv, Ts>(x, xm), CycVec<Tv, Ts>(e, xm)};
+    const CycVec<Tv, Ts> B(b, nullptr), XO(xOut, xm);
     const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X);
-    const Tv xo = X(c) + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
-    xOut[c] = xo;
+    const Tv xo = X(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(diag[c]);
+    XO.set(c, xo);
     sc(c, xo);
 }
 
@@ -110,23 +120,24 @@ struct AlphaPost
     __device__ void operator()(double num, double den) const { *alpha = den > 1e-30 ? num / den : 1.0; }
 };
 
-template <typename Tm, typename Tv>
+template <typename Tm, typename Tv, typename Ts>
 __global__ __launch_bounds__(kReduceBlock) void corrAmulK(SellView S, const Tm* __restrict__ val,
                                                           const Tv* __restrict__ diag, const double* __restrict__ scale,
-                                                          const Tv* __restrict__ e, const Tv* __restrict__ r,
-                                                          Tv* __restrict__ Ae, double* pa, double* pb, double* seqA,
-                                                          double* seqB, const int* __restrict__ stop)
+                                                          const Ts* __restrict__ e, const Ts* __restrict__ r,
+                                                          Ts* __restrict__ Ae, const float* __restrict__ xm, double* pa,
+                                                          double* pb, double* seqA, double* seqB,
+                                                          const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     __shared__ double lds[kReduceBlock / 64];
-    const PlainX<Tv> E{e};
+    const CycVec<Tv, Ts> E(e, xm), R(r, nullptr), AE(Ae, nullptr);
     double num = 0.0, den = 0.0;
     MFV_ROWS(S.n, c)
     {
         const Tv ae = rowAx<Tm, Tv>(S, c, val, diag, scale, E);
-        Ae[c] = ae;
+        AE.set(c, ae);
         const double ev = (double)E(c);
-        const double tn = ev * (double)r[c], td = ev * (double)ae;
+        const double tn = ev * (double)R(c), td = ev * (double)ae;
         if (seqA)
         {
             seqA[c] = tn;
@@ -149,91 +160,143 @@ __global__ __launch_bounds__(kReduceBlock) void corrAmulK(SellView S, const Tm* 
 }
 
 // second half, in place: x = x + alpha*e + omega*(r - alpha*Ae)/D (smooth) or x + alpha*e (no sweep)
-template <typename Tv>
-__global__ __launch_bounds__(kTpb) void corrSmoothK(int n, const Tv* __restrict__ e, const Tv* __restrict__ r,
-                                                    const Tv* __restrict__ Ae, const Tv* __restrict__ diag,
-                                                    const double* __restrict__ alpha, int smooth,
-                                                    Tv* __restrict__ x, Scatter<Tv> sc, const int* __restrict__ stop)
+template <typename Tv, typename Ts>
+__global__ __launch_bounds__(kTpb) void corrSmoothK(int n, const Ts* __restrict__ e, const Ts* __restrict__ r,
+                                                    const Ts* __restrict__ Ae, const Tv* __restrict__ diag,
+                                                    const double* __restrict__ alpha, int smooth, Ts* __restrict__ x,
+                                                    const float* __restrict__ xm, Scatter<Tv, Ts> sc,
+                                                    const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
+    const CycVec<Tv, Ts> E(e, xm), R(r, nullptr), AE(Ae, nullptr), X(x, xm);
     const Tv al = (Tv)(*alpha);
-    const Tv ev = e[c];
-    const Tv xo = smooth ? x[c] + al * ev + Tv(kOmega) * (r[c] - al * Ae[c]) / safeDiag(diag[c]) : x[c] + al * ev;
-    x[c] = xo;
+    const Tv ev = E(c);
+    const Tv xo = smooth ? X(c) + al * ev + Tv(kOmega) * (R(c) - al * AE(c)) / safeDiag(diag[c]) : X(c) + al * ev;
+    X.set(c, xo);
     sc(c, xo);
 }
 
 // prolongation alone (no post-sweep): x += e
-template <typename Tv>
-__global__ __launch_bounds__(kTpb) void prolongK(int n, const Tv* __restrict__ e, Tv* __restrict__ x, Scatter<Tv> sc,
+template <typename Tv, typename Ts>
+__global__ __launch_bounds__(kTpb) void prolongK(int n, const Ts* __restrict__ e, Ts* __restrict__ x,
+                                                 const float* __restrict__ xm, Scatter<Tv, Ts> sc,
                                                  const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
-    const Tv xo = x[c] + e[c];
-    x[c] = xo;
+    const CycVec<Tv, Ts> E(e, xm), X(x, xm);
+    const Tv xo = X(c) + E(c);
+    X.set(c, xo);
     sc(c, xo);
 }
 
 // coarse rhs = sum of the fine residual over each agglomerate, ascending fine cell (restrictGatherT);
 // with fromZero the coarse grid's first sweep from zero x = 0 + omega*(b - 0)/D (smoothFromZeroT)
-template <typename Tv>
+template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict__ cellStart,
-                                                  const int* __restrict__ cellList, const Tv* __restrict__ r,
-                                                  Tv* __restrict__ bc, const Tv* __restrict__ diagC,
-                                                  Tv* __restrict__ xc, int fromZero, const int* __restrict__ stop)
+                                                  const int* __restrict__ cellList, const Ts* __restrict__ r,
+                                                  Ts* __restrict__ bc, const Tv* __restrict__ diagC,
+                                                  Ts* __restrict__ xc, const float* __restrict__ xmC, int fromZero,
+                                                  const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int C = xcdBlock() * kTpb + threadIdx.x;
     if (C >= nc) return;
+    const CycVec<Tv, Ts> R(r, nullptr), BC(bc, nullptr), XC(xc, xmC);
     Tv s = Tv(0);
-    for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += r[cellList[k]];
-    bc[C] = s;
-    if (fromZero) xc[C] = Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(diagC[C]);
+    for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += R(cellList[k]);
+    BC.set(C, s);
+    if (fromZero) XC.set(C, Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(diagC[C]));
 }
 
-template <typename Tv>
-__global__ __launch_bounds__(kTpb) void zeroK(int n, Tv* __restrict__ x, const int* __restrict__ stop)
+template <typename Ts>
+__global__ __launch_bounds__(kTpb) void zeroK(int n, Ts* __restrict__ x, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
-    if (c < n) x[c] = Tv(0);
+    if (c < n) x[c] = Ts(0.0f);
 }
 
-// grid 0 entry from the FP64 residual: b = (Tv) r, x = omega*b/D when pre-smoothing from zero
-template <typename Tv>
+// grid 0 entry from the FP64 residual: b = (Tv) r (the FP16 cycle: r over its power-of-two scale sig),
+// x = omega*b/D when pre-smoothing from zero
+template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void startK(int n, const double* __restrict__ r, const Tv* __restrict__ diag,
-                                               Tv* __restrict__ b, Tv* __restrict__ x, int fromZero,
+                                               Ts* __restrict__ b, Ts* __restrict__ x, const float* __restrict__ xm,
+                                               const double* __restrict__ sig, int fromZero,
                                                const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
-    const Tv bv = (Tv)r[c];
-    b[c] = bv;
-    x[c] = fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(diag[c]) : Tv(0);
+    const CycVec<Tv, Ts> B(b, nullptr), X(x, xm);
+    const Tv bv = sig ? (Tv)(r[c] * (1.0 / *sig)) : (Tv)r[c];
+    B.set(c, bv);
+    X.set(c, fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(diag[c]) : Tv(0));
 }
 
 // coarsest grid: x = Ainv * b in FP64, ascending column per row (one workgroup)
-template <typename Tv>
-__global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __restrict__ inv, const Tv* __restrict__ b,
-                                                     Tv* __restrict__ x, Scatter<Tv> sc, const int* __restrict__ stop)
+template <typename Tv, typename Ts>
+__global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __restrict__ inv, const Ts* __restrict__ b,
+                                                     Ts* __restrict__ x, const float* __restrict__ xm,
+                                                     Scatter<Tv, Ts> sc, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    __shared__ double sb[1024];
-    for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = (double)b[j];
+    __shared__ double sb[kMaxDenseCoarsest];
+    const CycVec<Tv, Ts> B(b, nullptr), X(x, xm);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = (double)B(j);
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x)
     {
         double s = 0.0;
         const double* row = inv + (size_t)i * n;
         for (int j = 0; j < n; ++j) s += row[j] * sb[j];
-        x[i] = (Tv)s;
+        X.set(i, (Tv)s);
         sc(i, (Tv)s);
     }
+}
+
+// max and min of |d| over a workgroup's rows, for the FP16 cycle's solution-vector multiplier
+__global__ __launch_bounds__(kReduceBlock) void diagRangeK(int n, const double* __restrict__ d, double* pmax,
+                                                           double* pmin)
+{
+    __shared__ double lds[kReduceBlock / 64];
+    double mx = 0.0, mn = 1e308;
+    MFV_ROWS(n, i)
+    {
+        const double v = fabs(d[i]);
+        mx = fmax(mx, v);
+        if (v > 0.0) mn = fmin(mn, v);
+    }
+    const double bmx = blockMax(mx, lds);
+    const double bmn = -blockMax(-mn, lds);
+    if (threadIdx.x == 0)
+    {
+        pmax[blockIdx.x] = bmx;
+        pmin[blockIdx.x] = bmn;
+    }
+}
+
+// the grid's multiplier: the power of two at or below the geometric mean of max|d| and min|d| (the
+// solution vectors, about b/D, then sit near 1 whatever the units)
+__global__ __launch_bounds__(kFinishBlock) void xmulFinishK(int nParts, const double* __restrict__ pmax,
+                                                            const double* __restrict__ pmin, float* __restrict__ out)
+{
+    __shared__ double lds[kFinishBlock / 64];
+    double mx = 0.0, mn = 1e308;
+    for (int i = threadIdx.x; i < nParts; i += kFinishBlock)
+    {
+        mx = fmax(mx, pmax[i]);
+        mn = fmin(mn, pmin[i]);
+    }
+    mx = blockMax<kFinishBlock>(mx, lds);
+    mn = -blockMax<kFinishBlock>(-mn, lds);
+    if (threadIdx.x) return;
+    if (!(mx > 0.0)) mx = 1.0;
+    if (!(mn > 0.0) || mn > mx) mn = mx;
+    *out = (float)exp2(floor(0.5 * (log2(mx) + log2(mn))));
 }
 
 // ---------------- the small grids in one workgroup ----------------
@@ -255,46 +318,46 @@ struct TailGrid
     const void* diag;
     const double* scale;
     void *b, *x, *y, *r, *e;
+    const float* xm;        // solution-vector multiplier (FP16 cycle), else null
     const int* cellStart;   // this grid's cells per coarse cell of the next grid
     const int* cellList;
     int n;
 };
 
-template <typename Tm, typename Tv>
-__device__ __forceinline__ void tailSweep(const TailGrid& G, const Tv* __restrict__ xin, Tv* __restrict__ xout,
-                                          const Scatter<Tv>& sc)
+template <typename Tm, typename Tv, typename Ts>
+__device__ __forceinline__ void tailSweep(const TailGrid& G, Ts* xin, Ts* xout, const Scatter<Tv, Ts>& sc)
 {
     const Tm* val = static_cast<const Tm*>(G.val);
     const Tv* diag = static_cast<const Tv*>(G.diag);
-    const Tv* b = static_cast<const Tv*>(G.b);
+    const CycVec<Tv, Ts> X(xin, G.xm), XO(xout, G.xm), B(static_cast<Ts*>(G.b), nullptr);
     for (int c = threadIdx.x; c < G.n; c += kTailThreads)
     {
-        const Tv ax = rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, PlainX<Tv>{xin});
-        const Tv xo = xin[c] + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
-        xout[c] = xo;
+        const Tv ax = rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, X);
+        const Tv xo = X(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(diag[c]);
+        XO.set(c, xo);
         sc(c, xo);
     }
 }
 
-template <typename Tm, typename Tv>
+template <typename Tm, typename Tv, typename Ts>
 __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict__ D, int T,
                                                      const double* __restrict__ inv, int nPre, int nPost,
-                                                     int scaled, Scatter<Tv> up, const int* __restrict__ stop)
+                                                     int scaled, Scatter<Tv, Ts> up, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     __shared__ double lds[kTailThreads / 64];
     __shared__ double sb[1024];
     __shared__ double shAlpha;
     const int tid = threadIdx.x;
-    const Scatter<Tv> none{nullptr, nullptr, nullptr};
+    const Scatter<Tv, Ts> none{nullptr, nullptr, nullptr, nullptr};
     // grid g's current iterate is its x buffer, or y after an odd number of ping-pong sweeps (bit g)
     unsigned flip = 0;
-    auto X = [&](int g) { return static_cast<Tv*>((flip >> g) & 1u ? D[g].y : D[g].x); };
-    auto Y = [&](int g) { return static_cast<Tv*>((flip >> g) & 1u ? D[g].x : D[g].y); };
+    auto X = [&](int g) { return static_cast<Ts*>((flip >> g) & 1u ? D[g].y : D[g].x); };
+    auto Y = [&](int g) { return static_cast<Ts*>((flip >> g) & 1u ? D[g].x : D[g].y); };
     // the finishing step of grid g hands its values to grid g-1's e, or out of the tail for g = 0
     auto upOf = [&](int g) {
         if (g == 0) return up;
-        return Scatter<Tv>{D[g - 1].cellStart, D[g - 1].cellList, static_cast<Tv*>(D[g - 1].e)};
+        return Scatter<Tv, Ts>{D[g - 1].cellStart, D[g - 1].cellList, static_cast<Ts*>(D[g - 1].e), D[g - 1].xm};
     };
     // ---- down ----
     for (int g = 0; g + 1 < T; ++g)
@@ -302,51 +365,47 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
         const TailGrid G = D[g];
         const Tm* val = static_cast<const Tm*>(G.val);
         const Tv* diag = static_cast<const Tv*>(G.diag);
-        const Tv* b = static_cast<const Tv*>(G.b);
-        Tv* r = static_cast<Tv*>(G.r);
+        const CycVec<Tv, Ts> B(static_cast<Ts*>(G.b), nullptr), R(static_cast<Ts*>(G.r), nullptr);
         for (int s = 1; s < nPre; ++s)
         {
-            tailSweep<Tm, Tv>(G, X(g), Y(g), none);
+            tailSweep<Tm, Tv, Ts>(G, X(g), Y(g), none);
             __syncthreads();
             flip ^= 1u << g;
         }
-        Tv* x = X(g);
+        const CycVec<Tv, Ts> Xg(X(g), G.xm);
         if (nPre == 0)
         {
-            for (int c = tid; c < G.n; c += kTailThreads) x[c] = Tv(0);
+            for (int c = tid; c < G.n; c += kTailThreads) Xg.set(c, Tv(0));
             __syncthreads();
         }
-        for (int c = tid; c < G.n; c += kTailThreads)
-            r[c] = b[c] - rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, PlainX<Tv>{x});
+        for (int c = tid; c < G.n; c += kTailThreads) R.set(c, B(c) - rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, Xg));
         __syncthreads();
         const TailGrid C = D[g + 1];
         const Tv* dC = static_cast<const Tv*>(C.diag);
-        Tv* bC = static_cast<Tv*>(C.b);
-        Tv* xC = X(g + 1);
+        const CycVec<Tv, Ts> BC(static_cast<Ts*>(C.b), nullptr), XC(X(g + 1), C.xm);
         const int fromZero = (g + 2 < T && nPre > 0) ? 1 : 0;
         for (int k = tid; k < C.n; k += kTailThreads)
         {
             Tv s = Tv(0);
-            for (int q = G.cellStart[k]; q < G.cellStart[k + 1]; ++q) s += r[G.cellList[q]];
-            bC[k] = s;
-            if (fromZero) xC[k] = Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(dC[k]);
+            for (int q = G.cellStart[k]; q < G.cellStart[k + 1]; ++q) s += R(G.cellList[q]);
+            BC.set(k, s);
+            if (fromZero) XC.set(k, Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(dC[k]));
         }
         __syncthreads();
     }
     // ---- coarsest: x = Ainv b ----
     {
         const TailGrid G = D[T - 1];
-        const Tv* b = static_cast<const Tv*>(G.b);
-        Tv* x = X(T - 1);
-        for (int j = tid; j < G.n; j += kTailThreads) sb[j] = (double)b[j];
+        const CycVec<Tv, Ts> B(static_cast<Ts*>(G.b), nullptr), Xg(X(T - 1), G.xm);
+        for (int j = tid; j < G.n; j += kTailThreads) sb[j] = (double)B(j);
         __syncthreads();
-        const Scatter<Tv> sc = upOf(T - 1);
+        const Scatter<Tv, Ts> sc = upOf(T - 1);
         for (int i = tid; i < G.n; i += kTailThreads)
         {
             double s = 0.0;
             const double* row = inv + (size_t)i * G.n;
             for (int j = 0; j < G.n; ++j) s += row[j] * sb[j];
-            x[i] = (Tv)s;
+            Xg.set(i, (Tv)s);
             sc(i, (Tv)s);
         }
         __syncthreads();
@@ -358,20 +417,18 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
         const TailGrid G = D[g];
         const Tm* val = static_cast<const Tm*>(G.val);
         const Tv* diag = static_cast<const Tv*>(G.diag);
-        const Tv* r = static_cast<const Tv*>(G.r);
-        const Tv* e = static_cast<const Tv*>(G.e);
-        const Scatter<Tv> sc = upOf(g);
+        const CycVec<Tv, Ts> R(static_cast<Ts*>(G.r), nullptr), E(static_cast<Ts*>(G.e), G.xm);
+        const Scatter<Tv, Ts> sc = upOf(g);
         if (scaled)
         {
-            Tv* x = X(g);
-            Tv* Ae = Y(g);
+            const CycVec<Tv, Ts> Xg(X(g), G.xm), AE(Y(g), nullptr);
             double num = 0.0, den = 0.0;
             for (int c = tid; c < G.n; c += kTailThreads)
             {
-                const Tv ae = rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, PlainX<Tv>{e});
-                Ae[c] = ae;
-                const double ev = (double)e[c];
-                num += ev * (double)r[c];
+                const Tv ae = rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, E);
+                AE.set(c, ae);
+                const double ev = (double)E(c);
+                num += ev * (double)R(c);
                 den += ev * (double)ae;
             }
             num = blockSum<kTailThreads>(num, lds);
@@ -381,24 +438,23 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
             const Tv al = (Tv)shAlpha;
             for (int c = tid; c < G.n; c += kTailThreads)
             {
-                const Tv ev = e[c];
-                const Tv xo = nPost > 0 ? x[c] + al * ev + Tv(kOmega) * (r[c] - al * Ae[c]) / safeDiag(diag[c])
-                                        : x[c] + al * ev;
-                x[c] = xo;
+                const Tv ev = E(c);
+                const Tv xo = nPost > 0 ? Xg(c) + al * ev + Tv(kOmega) * (R(c) - al * AE(c)) / safeDiag(diag[c])
+                                        : Xg(c) + al * ev;
+                Xg.set(c, xo);
                 if (!nExtra) sc(c, xo);
             }
             __syncthreads();
         }
         else if (nPost > 0)
         {
-            const Tv* b = static_cast<const Tv*>(G.b);
-            const SumX<Tv> XE{X(g), e};
-            Tv* y = Y(g);
+            const CycVec<Tv, Ts> B(static_cast<Ts*>(G.b), nullptr), Yg(Y(g), G.xm);
+            const SumX<Tv, Ts> XE{CycVec<Tv, Ts>(X(g), G.xm), E};
             for (int c = tid; c < G.n; c += kTailThreads)
             {
                 const Tv ax = rowAx<Tm, Tv>(G.S, c, val, diag, G.scale, XE);
-                const Tv xo = XE(c) + Tv(kOmega) * (b[c] - ax) / safeDiag(diag[c]);
-                y[c] = xo;
+                const Tv xo = XE(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(diag[c]);
+                Yg.set(c, xo);
                 if (!nExtra) sc(c, xo);
             }
             __syncthreads();
@@ -406,11 +462,11 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
         }
         else
         {
-            Tv* x = X(g);
+            const CycVec<Tv, Ts> Xg(X(g), G.xm);
             for (int c = tid; c < G.n; c += kTailThreads)
             {
-                const Tv xo = x[c] + e[c];
-                x[c] = xo;
+                const Tv xo = Xg(c) + E(c);
+                Xg.set(c, xo);
                 sc(c, xo);
             }
             __syncthreads();
@@ -418,7 +474,7 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
         if (scaled || nPost > 0)
             for (int s = 0; s < nExtra; ++s)
             {
-                tailSweep<Tm, Tv>(G, X(g), Y(g), s + 1 == nExtra ? sc : none);
+                tailSweep<Tm, Tv, Ts>(G, X(g), Y(g), s + 1 == nExtra ? sc : none);
                 __syncthreads();
                 flip ^= 1u << g;
             }
@@ -487,10 +543,15 @@ __global__ __launch_bounds__(kTpb) void castK(int n, const double* __restrict__ 
     if (i < n) b[i] = (Tv)a[i];
 }
 
-__global__ __launch_bounds__(kTpb) void widenK(int n, const float* __restrict__ a, double* __restrict__ b)
+// the cycle's grid-0 result in FP64 (the FP16 cycle's times its input scale)
+template <typename Tv, typename Ts>
+__global__ __launch_bounds__(kTpb) void widenK(int n, const Ts* __restrict__ a, const float* __restrict__ xm,
+                                               const double* __restrict__ sig, double* __restrict__ b)
 {
     const int i = xcdBlock() * kTpb + threadIdx.x;
-    if (i < n) b[i] = (double)a[i];
+    if (i >= n) return;
+    const CycVec<Tv, Ts> A(a, xm);
+    b[i] = sig ? (double)A(i) * *sig : (double)A(i);
 }
 
 // per-grid FP16 scale: the smallest power of two >= max|upper| (exact scaling); max is exact, so the
@@ -527,6 +588,21 @@ __global__ __launch_bounds__(kFinishBlock) void maxPow2FinishK(int nParts, const
     if (threadIdx.x) return;
     for (int w = 0; w < kFinishBlock / 64; ++w) mm = fmax(mm, lds[w]);
     *out = mm > 0.0 ? exp2(ceil(log2(mm))) : 1.0;
+}
+
+// the FP16 cycle's input scale from max|r| (partials of maxAbsK): sig[0] (this cycle) = sig[1] (the next
+// one's, until the solver's update replaces it) = the power of two at or above the max
+__global__ __launch_bounds__(kFinishBlock) void sigFinishK(int nParts, const double* __restrict__ partials,
+                                                           double* __restrict__ sig)
+{
+    __shared__ double lds[kFinishBlock / 64];
+    double mm = 0.0;
+    for (int i = threadIdx.x; i < nParts; i += kFinishBlock) mm = fmax(mm, partials[i]);
+    mm = blockMax<kFinishBlock>(mm, lds);
+    if (threadIdx.x) return;
+    const double s = mm > 0.0 ? exp2(ceil(log2(mm))) : 1.0;
+    sig[0] = s;
+    sig[1] = s;
 }
 
 // Gauss-Jordan inverse with partial pivoting (first largest |a_ik|, i >= k) on [A | I], the
@@ -789,40 +865,51 @@ DBuf<float>& offOf<float>(Grid& g) { return g.off32; }
 template <>
 DBuf<__half>& offOf<__half>(Grid& g) { return g.off16; }
 
-template <typename Tv>
+template <typename Tv, typename Ts>
 struct VecSet
 {
-    Tv *b, *x, *y, *r, *e;
+    Ts *b, *x, *y, *r, *e;
     const Tv* diag;
+    const float* xm;   // solution-vector multiplier (FP16 cycle), else null
 };
 
-VecSet<double> vecs64(Grid& g) { return {g.b64.p, g.x64.p, g.y64.p, g.r64.p, g.e64.p, g.diagV64.p}; }
-VecSet<float> vecs32(Grid& g) { return {g.b32.p, g.x32.p, g.y32.p, g.r32.p, g.e32.p, g.diagV32.p}; }
+VecSet<double, double> vecs64(Amg&, Grid& g)
+{
+    return {g.b64.p, g.x64.p, g.y64.p, g.r64.p, g.e64.p, g.diagV64.p, nullptr};
+}
+VecSet<float, float> vecs32(Amg&, Grid& g)
+{
+    return {g.b32.p, g.x32.p, g.y32.p, g.r32.p, g.e32.p, g.diagV32.p, nullptr};
+}
+VecSet<float, __half> vecs16(Amg& a, Grid& g)
+{
+    return {g.b16.p, g.x16.p, g.y16.p, g.r16.p, g.e16.p, g.diagV32.p, a.xmul.p + (&g - a.grids.data())};
+}
 
 // Runs the cycle from grid gi down and back. Grid gi's final values go to up (the finer grid's e, or
 // nowhere for grid 0); the return value is the buffer that holds them (x or y: the ping-pong sweeps
 // leave it in either).
-template <typename Tm, typename Tv>
-Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, VecSet<Tv> (*vs)(Grid&),
-             Scatter<Tv> up)
+template <typename Tm, typename Tv, typename Ts>
+Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
+             VecSet<Tv, Ts> (*vs)(Amg&, Grid&), Scatter<Tv, Ts> up)
 {
     Grid& g = a.grids[gi];
-    VecSet<Tv> v = vs(g);
+    VecSet<Tv, Ts> v = vs(a, g);
     const int n = g.n;
     const int last = (int)a.grids.size() - 1;
-    const Scatter<Tv> none{nullptr, nullptr, nullptr};
+    const Scatter<Tv, Ts> none{nullptr, nullptr, nullptr, nullptr};
     if (gi == last)
     {
         ProfScope ps(c, kProfGridDown + gi);
-        coarseApplyK<Tv><<<1, kTpb, 0, c.stream>>>(n, a.coarseInv.p, v.b, v.x, up, stop);
+        coarseApplyK<Tv, Ts><<<1, kTpb, 0, c.stream>>>(n, a.coarseInv.p, v.b, v.x, v.xm, up, stop);
         return v.x;
     }
     if (gi > 0 && gi == a.tailStart && !c.sequentialReduce)
     {
         ProfScope ps(c, kProfGridDown + gi);
-        tailK<Tm, Tv><<<1, kTailThreads, 0, c.stream>>>(reinterpret_cast<const TailGrid*>(a.tailDesc.p),
-                                                         last - gi + 1, a.coarseInv.p, vc.nPre, vc.nPost,
-                                                         vc.scale ? 1 : 0, up, stop);
+        tailK<Tm, Tv, Ts><<<1, kTailThreads, 0, c.stream>>>(reinterpret_cast<const TailGrid*>(a.tailDesc.p),
+                                                             last - gi + 1, a.coarseInv.p, vc.nPre, vc.nPost,
+                                                             vc.scale ? 1 : 0, up, stop);
         hipCheck(hipGetLastError(), "vcycle tail");
         return nullptr;   // grid gi's values went up through `up`; no caller reads them here
     }
@@ -833,25 +920,26 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
     // pre-sweeps after the first (the first ran from zero where b was made)
     for (int s = 1; s < vc.nPre; ++s)
     {
-        smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, none, stop);
+        smoothK<Tm, Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, v.xm, none, stop);
         std::swap(v.x, v.y);
     }
-    if (vc.nPre == 0) zeroK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, v.x, stop);
+    if (vc.nPre == 0) zeroK<Ts><<<blocks(n), kTpb, 0, c.stream>>>(n, v.x, stop);
     {
         ProfScope ps(c, gi == 0 ? kProfAmgResidual0 : kProfCount);
-        residualK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.r, stop);
+        residualK<Tm, Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.r, v.xm, stop);
     }
     Grid& gc = a.grids[gi + 1];
-    VecSet<Tv> w = vs(gc);
+    VecSet<Tv, Ts> w = vs(a, gc);
     const int coarseIsLast = (gi + 1 == last);
     {
         ProfScope ps(c, gi == 0 ? kProfAmgRestrict0 : kProfCount);
-        restrictK<Tv><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, v.r, w.b, w.diag, w.x,
-                                                            (!coarseIsLast && vc.nPre > 0) ? 1 : 0, stop);
+        restrictK<Tv, Ts><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, v.r, w.b, w.diag,
+                                                                w.x, w.xm, (!coarseIsLast && vc.nPre > 0) ? 1 : 0,
+                                                                stop);
     }
     psDown.close();
     // the coarse grid leaves its result, injected, in this grid's e
-    vcycleAt<Tm, Tv>(c, a, gi + 1, vc, stop, vs, Scatter<Tv>{g.cellStart.p, g.cellList.p, v.e});
+    vcycleAt<Tm, Tv, Ts>(c, a, gi + 1, vc, stop, vs, Scatter<Tv, Ts>{g.cellStart.p, g.cellList.p, v.e, v.xm});
     ProfScope psUp(c, kProfGridUp + gi);
     // the kernel that finishes this grid hands its values on to the finer grid
     const int nExtra = std::max(0, vc.nPost - 1);
@@ -864,9 +952,9 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
         double* p2 = c.partials.p + 2 * (size_t)kMaxReduceBlocks;
         {
             ProfScope ps(c, gi == 0 ? kProfAmgCorrAmul0 : kProfCount);
-            corrAmulK<Tm, Tv><<<gr, kReduceBlock, 0, c.stream>>>(S, val, v.diag, sc, v.e, v.r, v.y, p1, p2,
-                                                                 seq ? a.seqA.p : nullptr, seq ? a.seqB.p : nullptr,
-                                                                 stop);
+            corrAmulK<Tm, Tv, Ts><<<gr, kReduceBlock, 0, c.stream>>>(S, val, v.diag, sc, v.e, v.r, v.y, v.xm, p1, p2,
+                                                                     seq ? a.seqA.p : nullptr,
+                                                                     seq ? a.seqB.p : nullptr, stop);
         }
         if (seq)
             seqFinish2K<<<1, 64, 0, c.stream>>>(n, a.seqA.p, a.seqB.p, AlphaPost{a.alphas.p + gi}, stop);
@@ -874,13 +962,14 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
             finish2K<<<1, kFinishBlock, 0, c.stream>>>(gr, p1, p2, AlphaPost{a.alphas.p + gi}, stop);
         {
             ProfScope ps(c, gi == 0 ? kProfAmgCorrSmooth0 : kProfCount);
-            corrSmoothK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, v.e, v.r, v.y, v.diag, a.alphas.p + gi,
-                                                              vc.nPost > 0 ? 1 : 0, v.x, nExtra ? none : up, stop);
+            corrSmoothK<Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(n, v.e, v.r, v.y, v.diag, a.alphas.p + gi,
+                                                                  vc.nPost > 0 ? 1 : 0, v.x, v.xm, nExtra ? none : up,
+                                                                  stop);
         }
         for (int s = 0; s < nExtra; ++s)
         {
-            smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y,
-                                                              s + 1 == nExtra ? up : none, stop);
+            smoothK<Tm, Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, v.xm,
+                                                                  s + 1 == nExtra ? up : none, stop);
             std::swap(v.x, v.y);
         }
     }
@@ -888,20 +977,20 @@ Tv* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop, 
     {
         {
             ProfScope ps(c, gi == 0 ? kProfAmgCorrSmooth0 : kProfCount);
-            prolongSmoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.e, v.x, v.b, v.y,
-                                                                      nExtra ? none : up, stop);
+            prolongSmoothK<Tm, Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.e, v.x, v.b, v.y, v.xm,
+                                                                          nExtra ? none : up, stop);
         }
         std::swap(v.x, v.y);
         for (int s = 0; s < nExtra; ++s)
         {
-            smoothK<Tm, Tv><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y,
-                                                              s + 1 == nExtra ? up : none, stop);
+            smoothK<Tm, Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, v.xm,
+                                                                  s + 1 == nExtra ? up : none, stop);
             std::swap(v.x, v.y);
         }
     }
     else
     {
-        prolongK<Tv><<<blocks(n), kTpb, 0, c.stream>>>(n, v.e, v.x, up, stop);
+        prolongK<Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(n, v.e, v.x, v.xm, up, stop);
     }
     hipCheck(hipGetLastError(), "vcycle");
     return v.x;
@@ -926,6 +1015,15 @@ void allocVectors<float>(Ctx& c, Grid& g)
     zeroBuf(c, g.y32, g.n);
     zeroBuf(c, g.r32, g.n);
     zeroBuf(c, g.e32, g.n);
+}
+template <>
+void allocVectors<__half>(Ctx& c, Grid& g)
+{
+    zeroBuf(c, g.b16, g.n);
+    zeroBuf(c, g.x16, g.n);
+    zeroBuf(c, g.y16, g.n);
+    zeroBuf(c, g.r16, g.n);
+    zeroBuf(c, g.e16, g.n);
 }
 
 } // namespace
@@ -1040,11 +1138,18 @@ void buildTail(Ctx& c, Amg& a)
             t.diag = G.diagV64.p;
             t.b = G.b64.p; t.x = G.x64.p; t.y = G.y64.p; t.r = G.r64.p; t.e = G.e64.p;
         }
-        else
+        else if (a.prec == Prec::F32)
         {
-            t.val = a.prec == Prec::F32 ? (const void*)G.off32.p : (const void*)G.off16.p;
+            t.val = G.off32.p;
             t.diag = G.diagV32.p;
             t.b = G.b32.p; t.x = G.x32.p; t.y = G.y32.p; t.r = G.r32.p; t.e = G.e32.p;
+        }
+        else
+        {
+            t.val = G.off16.p;
+            t.diag = G.diagV32.p;
+            t.b = G.b16.p; t.x = G.x16.p; t.y = G.y16.p; t.r = G.r16.p; t.e = G.e16.p;
+            t.xm = a.xmul.p + g;
         }
         d.push_back(t);
     }
@@ -1151,12 +1256,31 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
         {
             if (prec == Prec::F64)
                 allocVectors<double>(c, g);
-            else
+            else if (prec == Prec::F32)
                 allocVectors<float>(c, g);
+            else
+                allocVectors<__half>(c, g);
         }
         a.allocatedVectors = true;
         a.vectorsFor = prec;
         a.tailFor = -1;
+    }
+    if (prec == Prec::F16)
+    {
+        // each grid's solution-vector multiplier from its FP64 diagonal
+        a.xmul.resize(nG);
+        a.sig.resize(2);
+        double* pmx = c.partials.p + 6 * (size_t)kMaxReduceBlocks;
+        double* pmn = c.partials.p + 7 * (size_t)kMaxReduceBlocks;
+        for (int gi = 0; gi < nG; ++gi)
+        {
+            const Grid& g = a.grids[gi];
+            const double* d = gi == 0 ? fineDiag : g.diag64.p;
+            const int gr = reduceGrid(g.n);
+            diagRangeK<<<gr, kReduceBlock, 0, c.stream>>>(g.n, d, pmx, pmn);
+            xmulFinishK<<<1, kFinishBlock, 0, c.stream>>>(gr, pmx, pmn, a.xmul.p + gi);
+        }
+        hipCheck(hipGetLastError(), "fp16 vector multipliers");
     }
     a.prec = prec;
     buildTail(c, a);
@@ -1168,9 +1292,22 @@ void amgStartFrom(Ctx& c, Amg& a, const double* r, const VcycleControls& vc, con
     Grid& g = a.grids[0];
     const int fromZero = vc.nPre > 0 ? 1 : 0;
     if (a.prec == Prec::F64)
-        startK<double><<<blocks(g.n), kTpb, 0, c.stream>>>(g.n, r, g.diagV64.p, g.b64.p, g.x64.p, fromZero, stop);
+        startK<double, double><<<blocks(g.n), kTpb, 0, c.stream>>>(g.n, r, g.diagV64.p, g.b64.p, g.x64.p, nullptr,
+                                                                    nullptr, fromZero, stop);
+    else if (a.prec == Prec::F32)
+        startK<float, float><<<blocks(g.n), kTpb, 0, c.stream>>>(g.n, r, g.diagV32.p, g.b32.p, g.x32.p, nullptr,
+                                                                  nullptr, fromZero, stop);
     else
-        startK<float><<<blocks(g.n), kTpb, 0, c.stream>>>(g.n, r, g.diagV32.p, g.b32.p, g.x32.p, fromZero, stop);
+    {
+        // the FP16 cycle's input scale: the power of two at or above max|r|, for this cycle and (until the
+        // solver's update replaces it) the next
+        const int gr = reduceGrid(g.n);
+        double* pm = c.partials.p + 4 * (size_t)kMaxReduceBlocks;
+        maxAbsK<<<gr, kReduceBlock, 0, c.stream>>>(g.n, r, pm);
+        sigFinishK<<<1, kFinishBlock, 0, c.stream>>>(gr, pm, a.sig.p);
+        startK<float, __half><<<blocks(g.n), kTpb, 0, c.stream>>>(g.n, r, g.diagV32.p, g.b16.p, g.x16.p, a.xmul.p,
+                                                                   a.sig.p, fromZero, stop);
+    }
     hipCheck(hipGetLastError(), "vcycle start");
 }
 
@@ -1179,9 +1316,9 @@ void* amgVcycle(Ctx& c, Amg& a, const VcycleControls& vc, const int* stop)
     if (!a.galerkinDone) throw Error(kErrState, "mfv: amg_galerkin must run before a V-cycle");
     switch (a.prec)
     {
-    case Prec::F64: return vcycleAt<double, double>(c, a, 0, vc, stop, vecs64, Scatter<double>{});
-    case Prec::F32: return vcycleAt<float, float>(c, a, 0, vc, stop, vecs32, Scatter<float>{});
-    case Prec::F16: return vcycleAt<__half, float>(c, a, 0, vc, stop, vecs32, Scatter<float>{});
+    case Prec::F64: return vcycleAt<double, double, double>(c, a, 0, vc, stop, vecs64, Scatter<double, double>{});
+    case Prec::F32: return vcycleAt<float, float, float>(c, a, 0, vc, stop, vecs32, Scatter<float, float>{});
+    case Prec::F16: return vcycleAt<__half, float, __half>(c, a, 0, vc, stop, vecs16, Scatter<float, __half>{});
     }
     return nullptr;
 }
@@ -1208,8 +1345,13 @@ uint64_t amgLayoutKey(const Amg& a)
                               (const void*)g.cellStart.p, (const void*)g.cellList.p})
             ptr(p);
     }
+    for (const Grid& g : a.grids)
+        for (const void* p : {(const void*)g.b16.p, (const void*)g.x16.p, (const void*)g.y16.p, (const void*)g.r16.p,
+                              (const void*)g.e16.p})
+            ptr(p);
     for (const void* p : {(const void*)a.coarseInv.p, (const void*)a.scales.p, (const void*)a.alphas.p,
-                          (const void*)a.seqA.p, (const void*)a.seqB.p, (const void*)a.tailDesc.p})
+                          (const void*)a.seqA.p, (const void*)a.seqB.p, (const void*)a.tailDesc.p,
+                          (const void*)a.xmul.p, (const void*)a.sig.p})
         ptr(p);
     return h;
 }
@@ -1219,8 +1361,11 @@ void amgResultToDouble(Ctx& c, Amg& a, const void* res, double* z)
     const int n = a.grids[0].n;
     if (a.prec == Prec::F64)
         hipCheck(hipMemcpyAsync(z, res, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream), "result copy");
+    else if (a.prec == Prec::F32)
+        widenK<float, float><<<blocks(n), kTpb, 0, c.stream>>>(n, static_cast<const float*>(res), nullptr, nullptr, z);
     else
-        widenK<<<blocks(n), kTpb, 0, c.stream>>>(n, static_cast<const float*>(res), z);
+        widenK<float, __half><<<blocks(n), kTpb, 0, c.stream>>>(n, static_cast<const __half*>(res), a.xmul.p,
+                                                                a.sig.p, z);
     hipCheck(hipGetLastError(), "result");
 }
 

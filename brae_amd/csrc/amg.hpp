@@ -61,6 +61,7 @@ struct Grid
     // V-cycle vectors in the vector precision (double for the FP64 cycle, float otherwise)
     DBuf<double> b64, x64, y64, r64, e64;   // e: the coarse correction prolonged onto this grid
     DBuf<float> b32, x32, y32, r32, e32;
+    DBuf<__half> b16, x16, y16, r16, e16;   // the FP16 cycle's (see the row engine's CycVec)
     // restriction/prolongation to the next grid (absent on the coarsest)
     DBuf<int> map;                       // [n] fine -> coarse
     DBuf<int> cellStart, cellList;       // coarse cell <- fine cells, ascending
@@ -120,6 +121,8 @@ struct Amg
     DBuf<double> coarseWork;       // Gauss-Jordan augmented matrix
     DBuf<double> scales;           // per grid: FP16 scale (1 otherwise)
     DBuf<double> alphas;           // per grid: the scaled correction's alpha
+    DBuf<float> xmul;              // per grid: the FP16 cycle's solution-vector multiplier
+    DBuf<double> sig;              // the FP16 cycle's input scale: [0] this cycle's, [1] the next one's
     DBuf<double> seqA, seqB;       // per-cell terms for sequential (OpenFOAM-order) reductions
     Prec prec = Prec::F32;         // precision the V-cycle copies are filled in
     int tailStart = 0;             // first grid the single-workgroup tail kernel runs (0: none)

@@ -145,17 +145,34 @@ struct ZrPost
     }
 };
 
-template <typename Tv>
-__global__ __launch_bounds__(kReduceBlock) void pcgZrK(int n, const Tv* __restrict__ z, const double* __restrict__ r,
+// the V-cycle's result z at row i in FP64: the stored value, times the FP16 cycle's input scale
+template <typename Tv, typename Ts>
+struct CycleOut
+{
+    CycVec<Tv, Ts> z;
+    double sig;
+    __device__ double operator()(int i) const { return sig != 0.0 ? (double)z(i) * sig : (double)z(i); }
+};
+
+template <typename Tv, typename Ts>
+__device__ __forceinline__ CycleOut<Tv, Ts> cycleOut(const Ts* z, const float* xm, const double* sig)
+{
+    return CycleOut<Tv, Ts>{CycVec<Tv, Ts>(z, xm), sig ? *sig : 0.0};
+}
+
+template <typename Tv, typename Ts>
+__global__ __launch_bounds__(kReduceBlock) void pcgZrK(int n, const Ts* __restrict__ zs, const float* __restrict__ xm,
+                                                       const double* __restrict__ sig, const double* __restrict__ r,
                                                        const double* __restrict__ rOld, int flexible, PcgDev d,
                                                        double* pA, double* pB, double* seqA, double* seqB)
 {
     if (d.fl[F_STOP]) return;
     __shared__ double lds[kReduceBlock / 64];
+    const CycleOut<Tv, Ts> z = cycleOut<Tv, Ts>(zs, xm, sig);
     double a = 0.0, o = 0.0;
     MFV_ROWS(n, i)
     {
-        const double zi = (double)z[i];
+        const double zi = z(i);
         const double ea = zi * r[i];
         const double eo = flexible ? zi * rOld[i] : 0.0;
         if (seqA)
@@ -175,16 +192,16 @@ __global__ __launch_bounds__(kReduceBlock) void pcgZrK(int n, const Tv* __restri
     storePartials2(sa, so, pA, pB);
 }
 
-template <typename Tv>
+template <typename Tv, typename Ts>
 struct DirX
 {
-    const Tv* __restrict__ z;
+    CycleOut<Tv, Ts> z;
     const double* __restrict__ pOld;
     double beta;
     int first;
     __device__ double operator()(int j) const
     {
-        const double zj = (double)z[j];
+        const double zj = z(j);
         return first ? zj : beta * pOld[j] + zj;
     }
 };
@@ -196,16 +213,18 @@ struct PqPost
     __device__ void operator()(double s) const { d.st[S_PQ] = s; }
 };
 
-template <typename Tv>
+template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kReduceBlock) void pcgDirAmulK(SellView S, const double* __restrict__ val,
-                                                            const double* __restrict__ diag, const Tv* __restrict__ z,
+                                                            const double* __restrict__ diag, const Ts* __restrict__ z,
+                                                            const float* __restrict__ xm,
+                                                            const double* __restrict__ sig,
                                                             const double* __restrict__ pOld,
                                                             double* __restrict__ pNew, double* __restrict__ q, PcgDev d,
                                                             double* partials, double* seqA)
 {
     if (d.fl[F_STOP]) return;
     __shared__ double lds[kReduceBlock / 64];
-    const DirX<Tv> P{z, pOld, d.st[S_BETA], d.fl[F_ITER] == 0 ? 1 : 0};
+    const DirX<Tv, Ts> P{cycleOut<Tv, Ts>(z, xm, sig), pOld, d.st[S_BETA], d.fl[F_ITER] == 0 ? 1 : 0};
     double s = 0.0;
     MFV_ROWS(S.n, c)
     {
@@ -222,10 +241,19 @@ __global__ __launch_bounds__(kReduceBlock) void pcgDirAmulK(SellView S, const do
     storePartial(blockSum(s, lds), partials);
 }
 
-// psi += alpha p, r -= alpha q, sum|r|; the stop decision; the next cycle's grid-0 entry
+// psi += alpha p, r -= alpha q, sum|r|; the stop decision; the next cycle's grid-0 entry. The FP16
+// cycle's input scale rolls on: the update stored the next cycle's input at sig[1], which becomes that
+// cycle's scale sig[0]; sig[1] becomes the power of two at or above the new max|r|.
 struct UpdPost
 {
     PcgDev d;
+    double* sig;
+    __device__ void roll(double maxR) const
+    {
+        if (!sig) return;
+        sig[0] = sig[1];
+        if (maxR > 0.0) sig[1] = exp2(ceil(log2(maxR)));
+    }
     __device__ void operator()(double sumR) const
     {
         const double res = sumR / d.st[S_NF];
@@ -238,19 +266,24 @@ struct UpdPost
     }
 };
 
-template <typename Tv>
+// with `sig` (the FP16 cycle) the next cycle's input is r/sig[1], and max|r| goes to pMax
+template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kReduceBlock) void pcgUpdateK(int n, const double* __restrict__ p,
                                                            const double* __restrict__ q, double* __restrict__ psi,
                                                            double* __restrict__ r, double* __restrict__ rOld,
                                                            int flexible, const Tv* __restrict__ diag0,
-                                                           Tv* __restrict__ b0, Tv* __restrict__ x0, int fromZero,
-                                                           PcgDev d, double* partials, double* seqA)
+                                                           Ts* __restrict__ b0, Ts* __restrict__ x0,
+                                                           const float* __restrict__ xm, const double* __restrict__ sig,
+                                                           int fromZero, PcgDev d, double* partials, double* pMax,
+                                                           double* seqA)
 {
     if (d.fl[F_STOP]) return;
     __shared__ double lds[kReduceBlock / 64];
     const double pq = d.st[S_PQ];
     const double alpha = fabs(pq) > 1e-300 ? d.st[S_RHO] / pq : 0.0;
-    double s = 0.0;
+    const double isig = sig ? 1.0 / sig[1] : 1.0;
+    const CycVec<Tv, Ts> B(b0, nullptr), X(x0, xm);
+    double s = 0.0, mr = 0.0;
     MFV_ROWS(n, i)
     {
         psi[i] += alpha * p[i];
@@ -262,12 +295,49 @@ __global__ __launch_bounds__(kReduceBlock) void pcgUpdateK(int n, const double* 
             seqA[i] = fabs(rn);
         else
             s += fabs(rn);
-        const Tv bv = (Tv)rn;
-        b0[i] = bv;
-        x0[i] = fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(diag0[i]) : Tv(0);
+        mr = fmax(mr, fabs(rn));
+        const Tv bv = sig ? (Tv)(rn * isig) : (Tv)rn;
+        B.set(i, bv);
+        X.set(i, fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(diag0[i]) : Tv(0));
+    }
+    if (pMax)
+    {
+        const double m = blockMax(mr, lds);
+        if (threadIdx.x == 0) pMax[blockIdx.x] = m;
     }
     if (seqA) return;
     storePartial(blockSum(s, lds), partials);
+}
+
+// the update's finish with the FP16 cycle's max|r|
+__global__ __launch_bounds__(kFinishBlock) void updFinishK(int nParts, const double* __restrict__ pa,
+                                                           const double* __restrict__ pMax, UpdPost post,
+                                                           const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    __shared__ double lds[kFinishBlock / 64];
+    const double s = blockSum<kFinishBlock>(finishShare(nParts, pa), lds);
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nParts; i += kFinishBlock) m = fmax(m, pMax[i]);
+    m = blockMax<kFinishBlock>(m, lds);
+    if (threadIdx.x == 0)
+    {
+        post(s);
+        post.roll(m);
+    }
+}
+
+// the sequential-order finish of the update with the FP16 cycle's max|r| (one thread)
+__global__ void updSeqFinishK(int n, const double* __restrict__ a, int nParts, const double* __restrict__ pMax,
+                              UpdPost post, const int* __restrict__ stop)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    if (stop && *stop) return;
+    double s = 0.0, m = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i];
+    for (int i = 0; i < nParts; ++i) m = fmax(m, pMax[i]);
+    post(s);
+    post.roll(m);
 }
 
 __global__ void pcgSetupK(PcgDev d, double tol, double relTol, int maxIter, int minIter)
@@ -281,10 +351,15 @@ __global__ void pcgSetupK(PcgDev d, double tol, double relTol, int maxIter, int 
     d.fl[F_ITER] = 0;
 }
 
-template <typename Tv>
+template <typename Tv, typename Ts>
 PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* b, double* psi,
-               const double* normFactor, const PcgControls& pc, Tv* b0, Tv* x0, const Tv* diag0)
+               const double* normFactor, const PcgControls& pc, Ts* b0, Ts* x0, const Tv* diag0)
 {
+    // the FP16 cycle: grid 0's solution-vector multiplier and the input scales
+    const bool half = pc.prec == Prec::F16;
+    const float* xm0 = half ? a.xmul.p : nullptr;
+    double* sig = half ? a.sig.p : nullptr;
+    double* pMax = half ? c.partials.p + 7 * (size_t)kMaxReduceBlocks : nullptr;
     const int n = m.nCells;
     PcgWork& w = *a.work;
     PcgDev d{w.st.p, w.fl.p};
@@ -334,15 +409,15 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     double* p[2] = {w.p0.p, w.p1.p};
     // one iteration, enqueued; `it` selects the direction buffers (they alternate)
     auto iteration = [&](int it) {
-        const Tv* z;
+        const Ts* z;
         {
             ProfScope ps(c, kProfVcycle);
-            z = static_cast<const Tv*>(amgVcycle(c, a, vc, w.fl.p + F_STOP));
+            z = static_cast<const Ts*>(amgVcycle(c, a, vc, w.fl.p + F_STOP));
         }
         {
             ProfScope ps(c, kProfPcgZr);
-            pcgZrK<Tv><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, w.r.p, w.rOld.p, pc.flexible ? 1 : 0, d, pa, pb, sA,
-                                                          sB);
+            pcgZrK<Tv, Ts><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, xm0, sig, w.r.p, w.rOld.p, pc.flexible ? 1 : 0, d,
+                                                              pa, pb, sA, sB);
         }
         {
             const ZrPost post{d, pc.flexible ? 1 : 0};
@@ -353,8 +428,8 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         }
         {
             ProfScope ps(c, kProfPcgSpmv64);
-            pcgDirAmulK<Tv><<<gRd, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, z, p[it & 1], p[(it + 1) & 1],
-                                                               w.q.p, d, pa, sA);
+            pcgDirAmulK<Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, z, xm0, sig, p[it & 1],
+                                                                   p[(it + 1) & 1], w.q.p, d, pa, sA);
         }
         if (seq)
             seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, PqPost{d}, stopF);
@@ -362,14 +437,19 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
             finishK<<<1, kFinishBlock, 0, c.stream>>>(gRd, pa, PqPost{d}, stopF);
         {
             ProfScope ps(c, kProfPcgUpdate);
-            pcgUpdateK<Tv><<<gRu, kReduceBlock, 0, c.stream>>>(n, p[(it + 1) & 1], w.q.p, psi, w.r.p, w.rOld.p,
-                                                             pc.flexible ? 1 : 0, diag0, b0, x0, pc.nPre > 0 ? 1 : 0,
-                                                             d, pa, sA);
+            pcgUpdateK<Tv, Ts><<<gRu, kReduceBlock, 0, c.stream>>>(n, p[(it + 1) & 1], w.q.p, psi, w.r.p, w.rOld.p,
+                                                                 pc.flexible ? 1 : 0, diag0, b0, x0, xm0, sig,
+                                                                 pc.nPre > 0 ? 1 : 0, d, pa, pMax, sA);
         }
-        if (seq)
-            seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, UpdPost{d}, stopF);
+        const UpdPost upd{d, sig};
+        if (half && seq)
+            updSeqFinishK<<<1, 64, 0, c.stream>>>(n, sA, gRu, pMax, upd, stopF);
+        else if (half)
+            updFinishK<<<1, kFinishBlock, 0, c.stream>>>(gRu, pa, pMax, upd, stopF);
+        else if (seq)
+            seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, upd, stopF);
         else
-            finishK<<<1, kFinishBlock, 0, c.stream>>>(gRu, pa, UpdPost{d}, stopF);
+            finishK<<<1, kFinishBlock, 0, c.stream>>>(gRu, pa, upd, stopF);
         hipCheck(hipGetLastError(), "pcg iteration");
     };
     auto stopped = [&] {
@@ -467,8 +547,11 @@ PcgPerf amgPcgSolve(Ctx& c, const Mesh& m, Amg& a, const double* diag, const dou
         hipCheck(hipMemsetAsync(w.rOld.p, 0, n * sizeof(double), c.stream), "pcg work");
     }
     Grid& g0 = a.grids[0];
-    if (pc.prec == Prec::F64) return solveT<double>(c, m, a, diag, b, psi, normFactor, pc, g0.b64.p, g0.x64.p, g0.diagV64.p);
-    return solveT<float>(c, m, a, diag, b, psi, normFactor, pc, g0.b32.p, g0.x32.p, g0.diagV32.p);
+    if (pc.prec == Prec::F64)
+        return solveT<double, double>(c, m, a, diag, b, psi, normFactor, pc, g0.b64.p, g0.x64.p, g0.diagV64.p);
+    if (pc.prec == Prec::F32)
+        return solveT<float, float>(c, m, a, diag, b, psi, normFactor, pc, g0.b32.p, g0.x32.p, g0.diagV32.p);
+    return solveT<float, __half>(c, m, a, diag, b, psi, normFactor, pc, g0.b16.p, g0.x16.p, g0.diagV32.p);
 }
 
 } // namespace mfv

@@ -57,6 +57,22 @@ __device__ __forceinline__ double waveSum(double v)
     return v;
 }
 
+// Max over a workgroup of N threads (exact, so any order); the result is valid in thread 0.
+template <int N = kReduceBlock>
+__device__ __forceinline__ double blockMax(double v, double* lds)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double m = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < N / 64; ++w) m = fmax(m, lds[w]);
+    return m;
+}
+
 // Sum over a workgroup of N threads; the result is valid in thread 0.
 template <int N = kReduceBlock>
 __device__ __forceinline__ double blockSum(double v, double* lds)

@@ -139,5 +139,33 @@ struct PlainX
     __device__ Tv operator()(int j) const { return x[j]; }
 };
 
+// A V-cycle vector of one grid, stored as Ts with arithmetic in Tv. The FP64 and FP32 cycles store
+// their arithmetic type as is. The FP16 cycle stores FP16: its input is normalised by a power of two
+// (the cycle is positively homogeneous, so that is exact), right-hand-side vectors (b, r, A e) are then
+// O(1) on every grid, and solution vectors (x, e), which scale like b/D, are stored multiplied by the
+// grid's power-of-two typical diagonal `mul`; stores saturate at the FP16 range. Usable as the row
+// engine's operand functor.
+template <typename Tv, typename Ts>
+struct CycVec
+{
+    Ts* __restrict__ p;
+    __device__ CycVec(const Ts* q, const float*) : p(const_cast<Ts*>(q)) {}
+    __device__ Tv operator()(int i) const { return p[i]; }
+    __device__ void set(int i, Tv v) const { p[i] = v; }
+};
+
+template <>
+struct CycVec<float, __half>
+{
+    __half* __restrict__ p;
+    float m, im;
+    __device__ CycVec(const __half* q, const float* mul)
+        : p(const_cast<__half*>(q)), m(mul ? *mul : 1.0f), im(mul ? 1.0f / *mul : 1.0f)
+    {
+    }
+    __device__ float operator()(int i) const { return __half2float(p[i]) * im; }
+    __device__ void set(int i, float v) const { p[i] = __float2half(fminf(fmaxf(v * m, -65504.0f), 65504.0f)); }
+};
+
 
 } // namespace mfv
