@@ -27,6 +27,12 @@ __device__ __forceinline__ __half ldStream<__half>(const __half* p)
 {
     return __ushort_as_half(__builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p)));
 }
+template <>
+__device__ __forceinline__ float4 ldStream<float4>(const float4* p)
+{
+    const float4 v = *p;
+    return v;
+}
 
 // a stored coefficient in the arithmetic precision Tv
 template <typename Tv, typename Tm>
