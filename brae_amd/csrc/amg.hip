@@ -57,10 +57,21 @@ struct Scatter
     const int* __restrict__ list;
     Ts* __restrict__ e;        // null: no fine grid waits for this one (grid 0)
     const float* xm;           // the fine grid's multiplier
+    const int2* pairs;         // the fine grid's agglomerate pairs (may be null: the lists alone)
     __device__ void operator()(int C, Tv v) const
     {
         if (!e) return;
         const CycVec<Tv, Ts> E(e, xm);
+        if (pairs)
+        {
+            const int2 pc = pairs[C];
+            if (pc.x >= 0)
+            {
+                E.set(pc.x, v);
+                if (pc.y >= 0) E.set(pc.y, v);
+                return;
+            }
+        }
         for (int k = start[C]; k < start[C + 1]; ++k) E.set(list[k], v);
     }
 };
@@ -248,9 +259,10 @@ __global__ __launch_bounds__(kTpb) void prolongK(int n, const Ts* __restrict__ e
 // with fromZero the coarse grid's first sweep from zero x = 0 + omega*(b - 0)/D (smoothFromZeroT)
 template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict__ cellStart,
-                                                  const int* __restrict__ cellList, const Ts* __restrict__ r,
-                                                  Ts* __restrict__ bc, const Tv* __restrict__ diagC,
-                                                  Ts* __restrict__ xc, const float* __restrict__ xmC, int fromZero,
+                                                  const int* __restrict__ cellList, const int2* __restrict__ pairs,
+                                                  const Ts* __restrict__ r, Ts* __restrict__ bc,
+                                                  const Tv* __restrict__ diagC, Ts* __restrict__ xc,
+                                                  const float* __restrict__ xmC, int fromZero,
                                                   const int* __restrict__ stop)
 {
     if (stop && *stop) return;
@@ -258,7 +270,14 @@ __global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict_
     if (C >= nc) return;
     const CycVec<Tv, Ts> R(r, nullptr), BC(bc, nullptr), XC(xc, xmC);
     Tv s = Tv(0);
-    for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += R(cellList[k]);
+    const int2 pc = pairs[C];
+    if (pc.x >= 0)
+    {
+        s += R(pc.x);
+        if (pc.y >= 0) s += R(pc.y);
+    }
+    else
+        for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += R(cellList[k]);
     BC.set(C, s);
     if (fromZero) XC.set(C, Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(diagC[C]));
 }
@@ -400,7 +419,7 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
     __shared__ double sb[1024];
     __shared__ double shAlpha;
     const int tid = threadIdx.x;
-    const Scatter<Tv, Ts> none{nullptr, nullptr, nullptr, nullptr};
+    const Scatter<Tv, Ts> none{nullptr, nullptr, nullptr, nullptr, nullptr};
     // grid g's current iterate is its x buffer, or y after an odd number of ping-pong sweeps (bit g)
     unsigned flip = 0;
     auto X = [&](int g) { return static_cast<Ts*>((flip >> g) & 1u ? D[g].y : D[g].x); };
@@ -408,7 +427,8 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
     // the finishing step of grid g hands its values to grid g-1's e, or out of the tail for g = 0
     auto upOf = [&](int g) {
         if (g == 0) return up;
-        return Scatter<Tv, Ts>{D[g - 1].cellStart, D[g - 1].cellList, static_cast<Ts*>(D[g - 1].e), D[g - 1].xm};
+        return Scatter<Tv, Ts>{D[g - 1].cellStart, D[g - 1].cellList, static_cast<Ts*>(D[g - 1].e), D[g - 1].xm,
+                               nullptr};
     };
     // ---- down ----
     for (int g = 0; g + 1 < T; ++g)
@@ -898,9 +918,21 @@ void restrictionLists(Ctx& c, Grid& G, const AggLevel& L)
                 dl[dat[-1 - r]++] = f;
         }
     }
+    std::vector<int2> pr(nC);
+    for (int C = 0; C < nC; ++C)
+    {
+        const int k = cs[C + 1] - cs[C];
+        if (k == 1)
+            pr[C] = int2{cl[cs[C]], -1};
+        else if (k == 2)
+            pr[C] = int2{cl[cs[C]], cl[cs[C] + 1]};
+        else
+            pr[C] = int2{-1, C};
+    }
     G.map.upload(L.map, c.stream);
     G.cellStart.upload(cs, c.stream);
     G.cellList.upload(cl, c.stream);
+    G.pairs.upload(pr, c.stream);
     G.dfaceStart.upload(ds, c.stream);
     G.dfaceList.upload(dl, c.stream);
     G.faceStart.upload(fs, c.stream);
@@ -948,7 +980,7 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
     VecSet<Tv, Ts> v = vs(a, g);
     const int n = g.n;
     const int last = (int)a.grids.size() - 1;
-    const Scatter<Tv, Ts> none{nullptr, nullptr, nullptr, nullptr};
+    const Scatter<Tv, Ts> none{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (gi == last)
     {
         ProfScope ps(c, kProfGridDown + gi);
@@ -984,13 +1016,14 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
     const int coarseIsLast = (gi + 1 == last);
     {
         ProfScope ps(c, gi == 0 ? kProfAmgRestrict0 : kProfCount);
-        restrictK<Tv, Ts><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, v.r, w.b, w.diag,
-                                                                w.x, w.xm, (!coarseIsLast && vc.nPre > 0) ? 1 : 0,
-                                                                stop);
+        restrictK<Tv, Ts><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, g.pairs.p, v.r,
+                                                                w.b, w.diag, w.x, w.xm,
+                                                                (!coarseIsLast && vc.nPre > 0) ? 1 : 0, stop);
     }
     psDown.close();
     // the coarse grid leaves its result, injected, in this grid's e
-    vcycleAt<Tm, Tv, Ts>(c, a, gi + 1, vc, stop, vs, Scatter<Tv, Ts>{g.cellStart.p, g.cellList.p, v.e, v.xm});
+    vcycleAt<Tm, Tv, Ts>(c, a, gi + 1, vc, stop, vs,
+                         Scatter<Tv, Ts>{g.cellStart.p, g.cellList.p, v.e, v.xm, g.pairs.p});
     ProfScope psUp(c, kProfGridUp + gi);
     // the kernel that finishes this grid hands its values on to the finer grid
     const int nExtra = std::max(0, vc.nPost - 1);
@@ -1405,7 +1438,7 @@ uint64_t amgLayoutKey(const Amg& a)
                               (const void*)g.b64.p, (const void*)g.x64.p, (const void*)g.y64.p, (const void*)g.r64.p,
                               (const void*)g.e64.p, (const void*)g.b32.p, (const void*)g.x32.p, (const void*)g.y32.p,
                               (const void*)g.r32.p, (const void*)g.e32.p, (const void*)g.map.p,
-                              (const void*)g.cellStart.p, (const void*)g.cellList.p})
+                              (const void*)g.cellStart.p, (const void*)g.cellList.p, (const void*)g.pairs.p})
             ptr(p);
     }
     for (const Grid& g : a.grids)
