@@ -57,18 +57,20 @@ struct Scatter
     const int* __restrict__ list;
     Ts* __restrict__ e;        // null: no fine grid waits for this one (grid 0)
     const float* xm;           // the fine grid's multiplier
-    const int2* pairs;         // the fine grid's agglomerate pairs (may be null: the lists alone)
+    const int4* quads;         // the fine grid's agglomerates of up to four (may be null: the lists alone)
     __device__ void operator()(int C, Tv v) const
     {
         if (!e) return;
         const CycVec<Tv, Ts> E(e, xm);
-        if (pairs)
+        if (quads)
         {
-            const int2 pc = pairs[C];
-            if (pc.x >= 0)
+            const int4 q = quads[C];
+            if (q.x >= 0)
             {
-                E.set(pc.x, v);
-                if (pc.y >= 0) E.set(pc.y, v);
+                E.set(q.x, v);
+                if (q.y >= 0) E.set(q.y, v);
+                if (q.z >= 0) E.set(q.z, v);
+                if (q.w >= 0) E.set(q.w, v);
                 return;
             }
         }
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(kTpb) void prolongK(int n, const Ts* __restrict__ e
 // with fromZero the coarse grid's first sweep from zero x = 0 + omega*(b - 0)/D (smoothFromZeroT)
 template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict__ cellStart,
-                                                  const int* __restrict__ cellList, const int2* __restrict__ pairs,
+                                                  const int* __restrict__ cellList, const int4* __restrict__ quads,
                                                   const Ts* __restrict__ r, Ts* __restrict__ bc,
                                                   const Tv* __restrict__ diagC, Ts* __restrict__ xc,
                                                   const float* __restrict__ xmC, int fromZero,
@@ -270,11 +272,13 @@ __global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict_
     if (C >= nc) return;
     const CycVec<Tv, Ts> R(r, nullptr), BC(bc, nullptr), XC(xc, xmC);
     Tv s = Tv(0);
-    const int2 pc = pairs[C];
-    if (pc.x >= 0)
+    const int4 q = quads[C];
+    if (q.x >= 0)
     {
-        s += R(pc.x);
-        if (pc.y >= 0) s += R(pc.y);
+        s += R(q.x);
+        if (q.y >= 0) s += R(q.y);
+        if (q.z >= 0) s += R(q.z);
+        if (q.w >= 0) s += R(q.w);
     }
     else
         for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += R(cellList[k]);
@@ -918,21 +922,24 @@ void restrictionLists(Ctx& c, Grid& G, const AggLevel& L)
                 dl[dat[-1 - r]++] = f;
         }
     }
-    std::vector<int2> pr(nC);
+    std::vector<int4> qd(nC);
     for (int C = 0; C < nC; ++C)
     {
         const int k = cs[C + 1] - cs[C];
-        if (k == 1)
-            pr[C] = int2{cl[cs[C]], -1};
-        else if (k == 2)
-            pr[C] = int2{cl[cs[C]], cl[cs[C] + 1]};
+        int v[4] = {-1, -1, -1, -1};
+        if (k <= 4)
+            for (int j = 0; j < k; ++j) v[j] = cl[cs[C] + j];
         else
-            pr[C] = int2{-1, C};
+        {
+            v[0] = -1;
+            v[1] = C;
+        }
+        qd[C] = int4{v[0], v[1], v[2], v[3]};
     }
     G.map.upload(L.map, c.stream);
     G.cellStart.upload(cs, c.stream);
     G.cellList.upload(cl, c.stream);
-    G.pairs.upload(pr, c.stream);
+    G.quads.upload(qd, c.stream);
     G.dfaceStart.upload(ds, c.stream);
     G.dfaceList.upload(dl, c.stream);
     G.faceStart.upload(fs, c.stream);
@@ -1016,14 +1023,14 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
     const int coarseIsLast = (gi + 1 == last);
     {
         ProfScope ps(c, gi == 0 ? kProfAmgRestrict0 : kProfCount);
-        restrictK<Tv, Ts><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, g.pairs.p, v.r,
+        restrictK<Tv, Ts><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, g.quads.p, v.r,
                                                                 w.b, w.diag, w.x, w.xm,
                                                                 (!coarseIsLast && vc.nPre > 0) ? 1 : 0, stop);
     }
     psDown.close();
     // the coarse grid leaves its result, injected, in this grid's e
     vcycleAt<Tm, Tv, Ts>(c, a, gi + 1, vc, stop, vs,
-                         Scatter<Tv, Ts>{g.cellStart.p, g.cellList.p, v.e, v.xm, g.pairs.p});
+                         Scatter<Tv, Ts>{g.cellStart.p, g.cellList.p, v.e, v.xm, g.quads.p});
     ProfScope psUp(c, kProfGridUp + gi);
     // the kernel that finishes this grid hands its values on to the finer grid
     const int nExtra = std::max(0, vc.nPost - 1);
@@ -1438,7 +1445,7 @@ uint64_t amgLayoutKey(const Amg& a)
                               (const void*)g.b64.p, (const void*)g.x64.p, (const void*)g.y64.p, (const void*)g.r64.p,
                               (const void*)g.e64.p, (const void*)g.b32.p, (const void*)g.x32.p, (const void*)g.y32.p,
                               (const void*)g.r32.p, (const void*)g.e32.p, (const void*)g.map.p,
-                              (const void*)g.cellStart.p, (const void*)g.cellList.p, (const void*)g.pairs.p})
+                              (const void*)g.cellStart.p, (const void*)g.cellList.p, (const void*)g.quads.p})
             ptr(p);
     }
     for (const Grid& g : a.grids)

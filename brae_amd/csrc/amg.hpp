@@ -65,8 +65,8 @@ struct Grid
     // restriction/prolongation to the next grid (absent on the coarsest)
     DBuf<int> map;                       // [n] fine -> coarse
     DBuf<int> cellStart, cellList;       // coarse cell <- fine cells, ascending
-    DBuf<int2> pairs;                    // coarse cell <- its first and second fine cell (-1: none), or
-                                         // {-1, C} for an agglomerate of three or more (use the lists)
+    DBuf<int4> quads;                    // coarse cell <- its first four fine cells, ascending (-1: none), or
+                                         // {-1, C, -1, -1} for an agglomerate of five or more (the lists)
     DBuf<int> dfaceStart, dfaceList;     // coarse cell <- fine faces inside it, ascending
     DBuf<int> faceStart, faceList;       // coarse face <- fine faces, ascending
     DBuf<int> denseOwn, denseNei;        // coarsest grid: addressing for the dense assembly

@@ -208,7 +208,9 @@ void serveOne(mfv_ctx ctx, const mfv_serve_request& q, int precision, mfv_serve_
         std::vector<double> hw(nI);
         hipCheck(hipMemcpyAsync(hw.data(), magSf.p, sizeof(double) * nI, hipMemcpyDeviceToHost, st), "weights");
         hipCheck(hipStreamSynchronize(st), "weights sync");
-        mfv_amg_controls ac{MFV_AGG_BRAE, 64, 1};
+        // levels of two composed pairwise passes and a V(2,1) cycle (the fastest measured cycle on the
+        // 20 M-cell channel)
+        mfv_amg_controls ac{MFV_AGG_BRAE, 64, 2};
         check(mfv_amg_build(ctx, mesh, hw.data(), &ac, &amg), ctx, "amg");
         check(mfv_amg_galerkin(ctx, amg, diag.p, upper.p, nullptr, precision), ctx, "galerkin");
         mfv_pcg_controls pc;
@@ -217,6 +219,8 @@ void serveOne(mfv_ctx ctx, const mfv_serve_request& q, int precision, mfv_serve_
         pc.precision = precision;
         pc.scaleCorrection = 1;   // scaled correction: a nonlinear cycle, so flexible CG
         pc.flexible = 1;
+        pc.nPreSweeps = 2;
+        pc.nPostSweeps = 1;
         const mfv_ldu A{diag.p, upper.p, upper.p};
         hipCheck(hipMemsetAsync(p.p, 0, sizeof(double) * nC, st), "p clear");
         const int *dOwn = nullptr, *dNei = nullptr;

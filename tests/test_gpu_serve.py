@@ -23,9 +23,10 @@ MIX = [req((40, 30, 20), (2.0, 1.0, 1.0), 0.1, seed=11, steps=3),
        req((16, 16, 16), seed=16, steps=5, arrival=0.03)]
 
 
-def test_results_identical_alone_and_in_a_mix():
-    print("case: serving mix of", len(MIX), flush=True)
-    rc, mixed = mfv.serve(MIX, nSlots=4, hbmBudgetBytes=50e9)
+@pytest.mark.parametrize("precision", [32, 16])
+def test_results_identical_alone_and_in_a_mix(precision):
+    print("case: serving mix of", len(MIX), "precision", precision, flush=True)
+    rc, mixed = mfv.serve(MIX, nSlots=4, hbmBudgetBytes=50e9, precision=precision)
     assert rc == mfv.MFV_OK
     for r in mixed:
         assert r.status == 0 and r.finalResidual < 1e-6 and r.totalIterations > 0
@@ -33,7 +34,7 @@ def test_results_identical_alone_and_in_a_mix():
     spans = sorted((r.admitted, r.finished) for r in mixed)
     assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
     for i in (0, 2, 4):
-        rc, alone = mfv.serve([dict(MIX[i], arrival=0.0)], nSlots=1, hbmBudgetBytes=50e9)
+        rc, alone = mfv.serve([dict(MIX[i], arrival=0.0)], nSlots=1, hbmBudgetBytes=50e9, precision=precision)
         assert rc == mfv.MFV_OK
         assert alone[0].fingerprint == mixed[i].fingerprint, i
         assert alone[0].totalIterations == mixed[i].totalIterations
