@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--serving", type=int, default=512, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--gamg-step", type=int, default=0,
+                    help="1: also time the step with OpenFOAM's GAMG-PCG on the device (the CPU baseline's solver)")
     return ap.parse_args()
 
 
@@ -152,10 +154,11 @@ def gpu_run(args):
     A = mfv.ldu(ch.diagC, ch.upper)
     prec_mixed = mfv.PREC_FP32 if args.precision == "fp32" else mfv.PREC_FP16
 
-    def step(prec):
+    def step(prec, gamg=None):
         """one SIMPLE pressure corrector: nNonOrthogonalCorrectors + 1 assemblies and solves of the same
         matrix, the explicit non-orthogonal correction rebuilt from the latest p before each, and the flux
-        (with the stored correction flux) after the last"""
+        (with the stored correction flux) after the last. With `gamg` the solve is OpenFOAM's PCG with
+        the GAMG preconditioner (GaussSeidel, nVcycles 2) on that hierarchy, as the CPU path solves it."""
         ch.p.zero_()
         ctx.call("mfv_interpolate", ch.mesh.h, P(ch.w), P(ch.negRAU), P(ch.gf), P(ch.gb))
         ctx.call("mfv_laplacian_coeffs", ch.mesh.h, P(ch.ndc), P(ch.gf), P(ch.magSfI), P(ch.diag), P(ch.upper), None)
@@ -170,6 +173,10 @@ def gpu_run(args):
                      P(ch.ffc), P(ch.source))
             ctx.call("mfv_div_source", ch.mesh.h, P(ch.phiHbyA), P(ch.phiB), P(ch.V), -1.0, P(ch.source))
             ctx.call("mfv_fold", ch.mesh.h, P(ch.diag), P(ch.source), P(ch.iC), P(ch.bC), P(ch.diagC), P(ch.b))
+            if gamg is not None:
+                perfs.append(gamg.pcg_gamg_solve(ch.diagC, ch.upper, ch.b, ch.p, args.tolerance,
+                                                 smoother="GaussSeidel"))
+                continue
             if corr == 0:
                 amg.galerkin(ch.diagC, ch.upper, prec)
             perfs.append(amg.solve(ch.diagC, ch.upper, ch.b, ch.p, tolerance=args.tolerance, precision=prec,
@@ -279,6 +286,27 @@ def gpu_run(args):
         per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1), grids=grids,
         hbm_used_gib=round(hbm_used_gib, 2), other=other_out,
     )
+    if args.gamg_step:
+        # the CPU baseline's own algorithm on the device: OpenFOAM's faceAreaPair hierarchy and GAMG-PCG
+        Sf = ch.Sf[:3 * ch.nI].cpu().numpy()
+        wts = np.zeros(ch.nI)
+        mfv.lib().mfv_faceareapair_weights(ch.nI, Sf.ctypes.data, w_host.ctypes.data, wts.ctypes.data)
+        t0 = time.time()
+        gamg = mfv.Amg(ch.mesh, wts, mfv.AGG_FACEAREAPAIR, 10)
+        build_s = time.time() - t0
+        step(None, gamg)   # first use: the coarse grids' addressing and sweep schedules
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        gp = step(None, gamg)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        log("openfoam gamg-pcg on the device: %.4f s/step, iterations %s (%d grids, %.1fs host build)" % (
+            dt, [q.nIterations for q in gp], gamg.nGrids, build_s))
+        out["gamg_step"] = dict(s_per_step=round(dt, 4), pcg_iterations=[q.nIterations for q in gp],
+                                final_residual=gp[-1].finalResidual, grids=gamg.nGrids,
+                                solver="OpenFOAM PCG + GAMG preconditioner (faceAreaPair, GaussSeidel, "
+                                       "V(0,2,2), nVcycles 2), FP64, tree reductions")
+        gamg.close()
     amg.close()
     ch.mesh.close()
     ctx.close()
@@ -481,6 +509,7 @@ def main():
                                                         "" if args.no_scale else ", scaled correction"),
                 "per_grid_us": r["grids"]},
         "other_mixed": r["other"],
+        "openfoam_gamg_gpu": r.get("gamg_step"),
         "hbm_used_gib": r["hbm_used_gib"],
         "roofline": r["roofline"],
         "per_kernel": r["per_kernel"],
