@@ -242,8 +242,9 @@ def gpu_run(args):
     dw = 4
     iters = sum(q.nIterations for q in res["mixed"]["prof_perfs"])
     names = {
-        mfv.PROF_PCG_SPMV64: ("pcgDirAmulK: p = z + beta*p, q = A.p (FP64), p.q", 8 * (N + F) + 8 * F + zb * N + 24 * N,
-                              iters),
+        mfv.PROF_PCG_SPMV64: (("pcgDirDiaK: p = z + beta*p, q = A.p (FP64 diagonals), p.q" if amg.fine_stencil() else
+                               "pcgDirAmulK: p = z + beta*p, q = A.p (FP64), p.q"),
+                              8 * (N + F) + 8 * F + zb * N + 24 * N, iters),
         mfv.PROF_PCG_UPDATE: ("pcgUpdateK: psi, r update, sumMag(r), next cycle entry",
                               8 * (2 * N + 4 * N + N) + 2 * zb * N + dw * N, iters),
         mfv.PROF_PCG_ZR: ("pcgZrK: z.r, z.r_old", zb * N + 16 * N, iters),

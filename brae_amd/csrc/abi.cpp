@@ -602,6 +602,15 @@ int mfv_amg_grid_size(mfv_amg amg, int grid, int* nCells, int* nFaces)
     return MFV_OK;
 }
 
+int mfv_amg_fine_stencil(mfv_amg amg, int* nOffsets, int* hOffsets)
+{
+    if (!amg || amg->a.grids.empty() || !nOffsets) return MFV_ERR_INVALID;
+    const Grid& g = amg->a.grids[0];
+    *nOffsets = g.diaM;
+    if (hOffsets) std::copy(g.diaOff, g.diaOff + g.diaM, hOffsets);
+    return MFV_OK;
+}
+
 int mfv_amg_level_addressing(mfv_amg amg, int level, int* hMap, int* hCoarseOwner, int* hCoarseNeighbour,
                              int* hFaceRestrict, signed char* hFaceFlip)
 {

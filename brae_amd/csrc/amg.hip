@@ -259,13 +259,14 @@ __global__ __launch_bounds__(kTpb) void prolongK(int n, const Ts* __restrict__ e
 
 // coarse rhs = sum of the fine residual over each agglomerate, ascending fine cell (restrictGatherT);
 // with fromZero the coarse grid's first sweep from zero x = 0 + omega*(b - 0)/D (smoothFromZeroT)
-template <typename Tv, typename Ts>
+// (FromZero a template parameter: its diagonal load then issues with the children's, not behind a
+// branch; without it diagC may be null -- the coarsest grid.)
+template <typename Tv, typename Ts, bool FromZero>
 __global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict__ cellStart,
                                                   const int* __restrict__ cellList, const int4* __restrict__ quads,
                                                   const Ts* __restrict__ r, Ts* __restrict__ bc,
                                                   const Tv* __restrict__ diagC, Ts* __restrict__ xc,
-                                                  const float* __restrict__ xmC, int fromZero,
-                                                  const int* __restrict__ stop)
+                                                  const float* __restrict__ xmC, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int C = xcdBlock() * kTpb + threadIdx.x;
@@ -273,17 +274,22 @@ __global__ __launch_bounds__(kTpb) void restrictK(int nc, const int* __restrict_
     const CycVec<Tv, Ts> R(r, nullptr), BC(bc, nullptr), XC(xc, xmC);
     Tv s = Tv(0);
     const int4 q = quads[C];
+    const Tv dC = FromZero ? diagC[C] : Tv(1);
     if (q.x >= 0)
     {
-        s += R(q.x);
-        if (q.y >= 0) s += R(q.y);
-        if (q.z >= 0) s += R(q.z);
-        if (q.w >= 0) s += R(q.w);
+        // the four children's loads issue together (an absent child reads the first one), and only the
+        // present ones are summed, in ascending order
+        const Tv v0 = R(q.x), v1 = R(q.y >= 0 ? q.y : q.x), v2 = R(q.z >= 0 ? q.z : q.x),
+                 v3 = R(q.w >= 0 ? q.w : q.x);
+        s += v0;
+        s = q.y >= 0 ? s + v1 : s;
+        s = q.z >= 0 ? s + v2 : s;
+        s = q.w >= 0 ? s + v3 : s;
     }
     else
         for (int k = cellStart[C]; k < cellStart[C + 1]; ++k) s += R(cellList[k]);
     BC.set(C, s);
-    if (fromZero) XC.set(C, Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(diagC[C]));
+    if (FromZero) XC.set(C, Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(dC));
 }
 
 template <typename Ts>
@@ -618,6 +624,20 @@ __global__ __launch_bounds__(kTpb) void castK(int n, const double* __restrict__ 
     if (i < n) b[i] = (Tv)a[i];
 }
 
+// grid 0's diagonal copy: each face's upper coefficient into its owner's slot of its offset, in
+// precision Tm, scaled by 1/s (s from the device) as fillK
+template <typename Tm>
+__global__ __launch_bounds__(kTpb) void diaFillK(int nf, const int* __restrict__ slot, const double* __restrict__ up,
+                                                 const double* __restrict__ scale, Tm* __restrict__ dia)
+{
+    const int f = xcdBlock() * kTpb + threadIdx.x;
+    if (f >= nf) return;
+    if constexpr (sizeof(Tm) == 2)
+        dia[slot[f]] = __float2half((float)(up[f] * (1.0 / *scale)));
+    else
+        dia[slot[f]] = (Tm)up[f];
+}
+
 // the cycle's grid-0 result in FP64 (the FP16 cycle's times its input scale)
 template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void widenK(int n, const Ts* __restrict__ a, const float* __restrict__ xm,
@@ -890,6 +910,42 @@ void zeroBuf(Ctx& c, DBuf<T>& b, size_t n)
     if (n) hipCheck(hipMemsetAsync(b.p, 0, n * sizeof(T), c.stream), "zero");
 }
 
+// Grid 0's diagonal copy, when its faces form a fixed stencil of at most kDiaMax positive offsets in
+// (owner, neighbour) order and the diagonals hold at most two slots per face (a hex mesh: three
+// offsets, about one slot per face). Otherwise diaM stays 0 and the grid keeps its sliced rows only.
+void buildDia(Ctx& c, Grid& g, const std::vector<int>& own, const std::vector<int>& nei)
+{
+    g.diaM = 0;
+    const int nF = (int)own.size();
+    if (nF == 0) return;
+    std::vector<int> offs;
+    for (int f = 0; f < nF; ++f)
+    {
+        if (f > 0 && (own[f] < own[f - 1] || (own[f] == own[f - 1] && nei[f] <= nei[f - 1]))) return;
+        const int d = nei[f] - own[f];
+        if (std::find(offs.begin(), offs.end(), d) == offs.end())
+        {
+            if ((int)offs.size() == kDiaMax) return;
+            offs.push_back(d);
+        }
+    }
+    std::sort(offs.begin(), offs.end());
+    const long long pad = ((long long)g.n + kSlice - 1) / kSlice * kSlice;
+    const long long slots = (long long)offs.size() * pad;
+    if (slots > 2LL * nF + (long long)kDiaMax * kSlice || slots > 0x7fffffffLL) return;
+    std::vector<int> slot(nF);
+    for (int f = 0; f < nF; ++f)
+    {
+        const int k = (int)(std::lower_bound(offs.begin(), offs.end(), nei[f] - own[f]) - offs.begin());
+        slot[f] = (int)(k * pad + own[f]);
+    }
+    g.diaM = (int)offs.size();
+    for (int k = 0; k < g.diaM; ++k) g.diaOff[k] = offs[k];
+    g.diaPad = (int)pad;
+    g.diaSlot.upload(slot, c.stream);
+    zeroBuf(c, g.dia64, (size_t)slots);
+}
+
 void restrictionLists(Ctx& c, Grid& G, const AggLevel& L)
 {
     const int nC = L.nCoarse, nCF = L.nCoarseFaces, nF = L.nFineFaces;
@@ -945,6 +1001,31 @@ void restrictionLists(Ctx& c, Grid& G, const AggLevel& L)
     G.faceStart.upload(fs, c.stream);
     G.faceList.upload(fl, c.stream);
 }
+
+// grid 0's V-cycle reads its diagonal copies when the stencil has two or three offsets (a 2-D or 3-D
+// structured mesh); any other grid, or stencil, its sliced rows
+bool cycleDia(const Grid& g) { return g.diaM == 2 || g.diaM == 3; }
+
+SellView cycleView(const Grid& g, bool dia)
+{
+    SellView S = viewOf(g.sell);
+    if (dia)
+    {
+        S.diaM = g.diaM;
+        S.diaPad = g.diaPad;
+        for (int k = 0; k < kDiaMax; ++k) S.diaOff[k] = g.diaOff[k];
+    }
+    return S;
+}
+
+template <typename Tm>
+DBuf<Tm>& diaOf(Grid& g);
+template <>
+DBuf<double>& diaOf<double>(Grid& g) { return g.dia64; }
+template <>
+DBuf<float>& diaOf<float>(Grid& g) { return g.dia32; }
+template <>
+DBuf<__half>& diaOf<__half>(Grid& g) { return g.dia16; }
 
 template <typename Tm>
 DBuf<Tm>& offOf(Grid& g);
@@ -1004,8 +1085,9 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
         return nullptr;   // grid gi's values went up through `up`; no caller reads them here
     }
     ProfScope psDown(c, kProfGridDown + gi);
-    const SellView S = viewOf(g.sell);
-    const Tm* val = offOf<Tm>(g).p;
+    const bool dia = gi == 0 && cycleDia(g);
+    const SellView S = cycleView(g, dia);
+    const Tm* val = dia ? diaOf<Tm>(g).p : offOf<Tm>(g).p;
     const double* sc = a.scales.p + gi;
     // pre-sweeps after the first (the first ran from zero where b was made)
     for (int s = 1; s < vc.nPre; ++s)
@@ -1023,9 +1105,12 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
     const int coarseIsLast = (gi + 1 == last);
     {
         ProfScope ps(c, gi == 0 ? kProfAmgRestrict0 : kProfCount);
-        restrictK<Tv, Ts><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, g.quads.p, v.r,
-                                                                w.b, w.diag, w.x, w.xm,
-                                                                (!coarseIsLast && vc.nPre > 0) ? 1 : 0, stop);
+        if (!coarseIsLast && vc.nPre > 0)
+            restrictK<Tv, Ts, true><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p, g.quads.p,
+                                                                          v.r, w.b, w.diag, w.x, w.xm, stop);
+        else
+            restrictK<Tv, Ts, false><<<blocks(gc.n), kTpb, 0, c.stream>>>(gc.n, g.cellStart.p, g.cellList.p,
+                                                                           g.quads.p, v.r, w.b, w.diag, w.x, w.xm, stop);
     }
     psDown.close();
     // the coarse grid leaves its result, injected, in this grid's e
@@ -1185,6 +1270,7 @@ void amgBuildFrom(Ctx& c, const Mesh& m, Amg& a, std::vector<AggLevel> levels, c
             g.upper64.resize(g.nFaces);
         }
         if (gi < nG - 1 || gi == 0) buildSell(c, g.sell, g.n, own, nei);
+        if (gi == 0) buildDia(c, g, own, nei);
     }
     a.nCoarsest = a.grids.back().n;
     Grid& gl = a.grids.back();
@@ -1305,6 +1391,8 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
             if (g.nFaces)
                 fillK<double><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.sell.slotLo.p, g.sell.slotHi.p, u, u,
                                                                        nullptr, g.off64.p);
+            if (gi == 0 && g.diaM && g.nFaces)
+                diaFillK<double><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.diaSlot.p, u, nullptr, g.dia64.p);
         }
         if (prec == Prec::F64)
         {
@@ -1321,6 +1409,12 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
                 if (g.nFaces)
                     fillK<float><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.sell.slotLo.p, g.sell.slotHi.p, u,
                                                                           u, nullptr, g.off32.p);
+                if (gi == 0 && cycleDia(g) && g.nFaces)
+                {
+                    if (g.dia32.n != g.dia64.n) zeroBuf(c, g.dia32, g.dia64.n);
+                    diaFillK<float><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.diaSlot.p, u, nullptr,
+                                                                             g.dia32.p);
+                }
             }
             else
             {
@@ -1333,6 +1427,12 @@ void amgGalerkin(Ctx& c, Amg& a, const double* fineDiag, const double* fineUpper
                     maxPow2FinishK<<<1, kFinishBlock, 0, c.stream>>>(gm, pm, a.scales.p + gi);
                     fillK<__half><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.sell.slotLo.p, g.sell.slotHi.p,
                                                                            u, u, a.scales.p + gi, g.off16.p);
+                    if (gi == 0 && cycleDia(g))
+                    {
+                        if (g.dia16.n != g.dia64.n) zeroBuf(c, g.dia16, g.dia64.n);
+                        diaFillK<__half><<<blocks(g.nFaces), kTpb, 0, c.stream>>>(g.nFaces, g.diaSlot.p, u,
+                                                                                  a.scales.p + gi, g.dia16.p);
+                    }
                 }
             }
         }
@@ -1445,7 +1545,8 @@ uint64_t amgLayoutKey(const Amg& a)
                               (const void*)g.b64.p, (const void*)g.x64.p, (const void*)g.y64.p, (const void*)g.r64.p,
                               (const void*)g.e64.p, (const void*)g.b32.p, (const void*)g.x32.p, (const void*)g.y32.p,
                               (const void*)g.r32.p, (const void*)g.e32.p, (const void*)g.map.p,
-                              (const void*)g.cellStart.p, (const void*)g.cellList.p, (const void*)g.quads.p})
+                              (const void*)g.cellStart.p, (const void*)g.cellList.p, (const void*)g.quads.p,
+                              (const void*)g.dia64.p, (const void*)g.dia32.p, (const void*)g.dia16.p})
             ptr(p);
     }
     for (const Grid& g : a.grids)

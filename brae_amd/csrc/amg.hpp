@@ -25,6 +25,7 @@
 namespace mfv {
 
 constexpr int kSlice = 64;
+constexpr int kDiaMax = 4;       // offsets of a fixed-stencil grid (a hex mesh has three)
 constexpr double kOmega = 0.8;   // weighted-Jacobi relaxation (device_amg_detail.cuh:39)
 
 enum class Prec : int { F64 = 64, F32 = 32, F16 = 16 };
@@ -70,6 +71,18 @@ struct Grid
     DBuf<int> dfaceStart, dfaceList;     // coarse cell <- fine faces inside it, ascending
     DBuf<int> faceStart, faceList;       // coarse face <- fine faces, ascending
     DBuf<int> denseOwn, denseNei;        // coarsest grid: addressing for the dense assembly
+    // Grid 0 as diagonals, when its faces form a fixed stencil: every face's neighbour - owner is one of
+    // diaM ascending positive offsets and the faces are in (owner, neighbour) order, so a row's entries in
+    // face order are its columns ascending. The FP64 upper coefficient of face (r, r + diaOff[k]) sits at
+    // dia64[k * diaPad + r] (0 where the face is absent); row r's lower entry at -diaOff[k] is the same
+    // value at dia64[k * diaPad + r - diaOff[k]], so each coefficient is stored and streamed once.
+    int diaM = 0;
+    int diaOff[kDiaMax] = {};
+    int diaPad = 0;                      // n rounded up to a slice
+    DBuf<int> diaSlot;                   // [nFaces] k * diaPad + owner
+    DBuf<double> dia64;                  // [diaM * diaPad]
+    DBuf<float> dia32;                   // the V-cycle's copies in its coefficient precision (the FP16 one
+    DBuf<__half> dia16;                  // divided by the grid's power-of-two scale, as off16)
     // host copies for inspection
     std::vector<int> hOwn, hNei;
 };

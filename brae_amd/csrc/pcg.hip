@@ -160,11 +160,13 @@ __device__ __forceinline__ CycleOut<Tv, Ts> cycleOut(const Ts* z, const float* x
     return CycleOut<Tv, Ts>{CycVec<Tv, Ts>(z, xm), sig ? *sig : 0.0};
 }
 
-template <typename Tv, typename Ts>
+// Flex (the flexible beta's z.r_old) is a template parameter: a load behind a run-time flag is issued
+// behind a branch, apart from the row's other loads.
+template <typename Tv, typename Ts, bool Flex>
 __global__ __launch_bounds__(kReduceBlock) void pcgZrK(int n, const Ts* __restrict__ zs, const float* __restrict__ xm,
                                                        const double* __restrict__ sig, const double* __restrict__ r,
-                                                       const double* __restrict__ rOld, int flexible, PcgDev d,
-                                                       double* pA, double* pB, double* seqA, double* seqB)
+                                                       const double* __restrict__ rOld, PcgDev d, double* pA,
+                                                       double* pB, double* seqA, double* seqB)
 {
     if (d.fl[F_STOP]) return;
     __shared__ double lds[kReduceBlock / 64];
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(kReduceBlock) void pcgZrK(int n, const Ts* __restri
     {
         const double zi = z(i);
         const double ea = zi * r[i];
-        const double eo = flexible ? zi * rOld[i] : 0.0;
+        const double eo = Flex ? zi * rOld[i] : 0.0;
         if (seqA)
         {
             seqA[i] = ea;
@@ -199,10 +201,14 @@ struct DirX
     const double* __restrict__ pOld;
     double beta;
     int first;
+    // pOld is loaded on the first iteration too (its buffer is allocated; the value is discarded), so the
+    // loads of a row's neighbours issue as one batch instead of behind a branch each
     __device__ double operator()(int j) const
     {
+        const double po = pOld[j];
         const double zj = z(j);
-        return first ? zj : beta * pOld[j] + zj;
+        const double pj = beta * po + zj;
+        return first ? zj : pj;
     }
 };
 
@@ -236,6 +242,102 @@ __global__ __launch_bounds__(kReduceBlock) void pcgDirAmulK(SellView S, const do
             seqA[c] = ax * pc;
         else
             s += ax * pc;
+    }
+    if (seqA) return;
+    storePartial(blockSum(s, lds), partials);
+}
+
+// Grid 0 as diagonals (a fixed stencil of M positive offsets), by value.
+struct DiaView
+{
+    int n, pad;
+    int off[kDiaMax];
+};
+
+// The same product on grid 0's diagonal copy, each thread on R rows at once (c, c + kReduceBlock, ...)
+// with every load of all R issued before the sums. Row c sums D p_c, then its lower entries at columns
+// c - off[M-1] .. c - off[0], then its upper ones at c + off[0] .. c + off[M-1]: its face order, with a
+// zero coefficient (times a finite p) where a face is absent. The upper coefficients stream once
+// (nontemporal); the lower ones are the rows above's upper coefficients, reread through the cache. A
+// column outside the grid reads the row itself.
+template <int R, int M, typename XF>
+__device__ __forceinline__ void diaRows(const DiaView& D, int c0, int end, const double* __restrict__ up,
+                                        const double* __restrict__ diag, XF P, double* ax, double* pc)
+{
+    double a[R][2 * M], xv[R][2 * M], dg[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+    {
+        const int ci = c0 + i * kReduceBlock;
+        const int c = ci < end ? ci : c0;
+        dg[i] = ldStream(diag + c);
+        int col[2 * M];
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+        {
+            const int kk = M - 1 - k;
+            const int j = c - D.off[kk];
+            const bool in = j >= 0;
+            col[k] = in ? j : c;
+            const double v = up[(size_t)kk * D.pad + col[k]];
+            a[i][k] = in ? v : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+        {
+            const int j = c + D.off[k];
+            col[M + k] = j < D.n ? j : c;
+            a[i][M + k] = ldStream(up + (size_t)k * D.pad + c);
+        }
+        pc[i] = P(c);
+#pragma unroll
+        for (int k = 0; k < 2 * M; ++k) xv[i][k] = P(col[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+    {
+        double acc = dg[i] * pc[i];
+#pragma unroll
+        for (int k = 0; k < 2 * M; ++k) acc += a[i][k] * xv[i][k];
+        ax[i] = acc;
+    }
+}
+
+// rows per thread of the diagonal product (one pass: launched on reduceGridRows(n, kDiaRows))
+constexpr int kDiaRows = 1;
+
+template <int M, typename Tv, typename Ts>
+__global__ __launch_bounds__(kReduceBlock) void pcgDirDiaK(DiaView D, const double* __restrict__ up,
+                                                           const double* __restrict__ diag, const Ts* __restrict__ z,
+                                                           const float* __restrict__ xm,
+                                                           const double* __restrict__ sig,
+                                                           const double* __restrict__ pOld,
+                                                           double* __restrict__ pNew, double* __restrict__ q, PcgDev d,
+                                                           double* partials, double* seqA)
+{
+    if (d.fl[F_STOP]) return;
+    __shared__ double lds[kReduceBlock / 64];
+    const DirX<Tv, Ts> P{cycleOut<Tv, Ts>(z, xm, sig), pOld, d.st[S_BETA], d.fl[F_ITER] == 0 ? 1 : 0};
+    const int chunk = chunkRows(D.n, gridDim.x);
+    const int begin = xcdBlock() * chunk;
+    const int end = begin + chunk < D.n ? begin + chunk : D.n;
+    double s = 0.0;
+    for (int c = begin + (int)threadIdx.x; c < end; c += kDiaRows * kReduceBlock)
+    {
+        double ax[kDiaRows], pc[kDiaRows];
+        diaRows<kDiaRows, M>(D, c, end, up, diag, P, ax, pc);
+#pragma unroll
+        for (int i = 0; i < kDiaRows; ++i)
+        {
+            const int ci = c + i * kReduceBlock;
+            if (ci >= end) continue;
+            __builtin_nontemporal_store(pc[i], pNew + ci);
+            __builtin_nontemporal_store(ax[i], q + ci);
+            if (seqA)
+                seqA[ci] = ax[i] * pc[i];
+            else
+                s += ax[i] * pc[i];
+        }
     }
     if (seqA) return;
     storePartial(blockSum(s, lds), partials);
@@ -366,9 +468,12 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     const Grid& g0 = a.grids[0];
     const SellView S = viewOf(g0.sell);
     const int gR = reduceGrid(n);
-    // rows per thread, measured per kernel on the 20 M channel: the FP64 direction product 4, z.r 8,
-    // the update 2 (reduceGrid)
-    const int gRd = reduceGridRows(n, 4);
+    // rows per thread, measured per kernel on the 20 M channel: the FP64 direction product 4 (kDiaRows
+    // on the diagonal copy), z.r 8, the update 2 (reduceGrid)
+    const bool dia = g0.diaM > 0;
+    DiaView D{n, g0.diaPad, {}};
+    for (int k = 0; k < kDiaMax; ++k) D.off[k] = g0.diaOff[k];
+    const int gRd = reduceGridRows(n, dia ? kDiaRows : 4);
     const int gRz = reduceGridRows(n, 8);
     const int gRu = reduceGrid(n);
     double* pa = c.partials.p + 5 * (size_t)kMaxReduceBlocks;
@@ -416,8 +521,12 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         }
         {
             ProfScope ps(c, kProfPcgZr);
-            pcgZrK<Tv, Ts><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, xm0, sig, w.r.p, w.rOld.p, pc.flexible ? 1 : 0, d,
-                                                              pa, pb, sA, sB);
+            if (pc.flexible)
+                pcgZrK<Tv, Ts, true><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, xm0, sig, w.r.p, w.rOld.p, d, pa, pb, sA,
+                                                                        sB);
+            else
+                pcgZrK<Tv, Ts, false><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, xm0, sig, w.r.p, w.rOld.p, d, pa, pb,
+                                                                         sA, sB);
         }
         {
             const ZrPost post{d, pc.flexible ? 1 : 0};
@@ -428,8 +537,30 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         }
         {
             ProfScope ps(c, kProfPcgSpmv64);
-            pcgDirAmulK<Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, z, xm0, sig, p[it & 1],
-                                                                   p[(it + 1) & 1], w.q.p, d, pa, sA);
+            const double* pOld = p[it & 1];
+            double* pNew = p[(it + 1) & 1];
+            switch (dia ? g0.diaM : 0)
+            {
+            case 1:
+                pcgDirDiaK<1, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
+                                                                         w.q.p, d, pa, sA);
+                break;
+            case 2:
+                pcgDirDiaK<2, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
+                                                                         w.q.p, d, pa, sA);
+                break;
+            case 3:
+                pcgDirDiaK<3, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
+                                                                         w.q.p, d, pa, sA);
+                break;
+            case 4:
+                pcgDirDiaK<4, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
+                                                                         w.q.p, d, pa, sA);
+                break;
+            default:
+                pcgDirAmulK<Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(S, g0.off64.p, diag, z, xm0, sig, pOld, pNew,
+                                                                       w.q.p, d, pa, sA);
+            }
         }
         if (seq)
             seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, PqPost{d}, stopF);

@@ -40,7 +40,8 @@ __device__ __forceinline__ Tv cvt(Tm v) { return (Tv)v; }
 template <>
 __device__ __forceinline__ float cvt<float, __half>(__half v) { return __half2float(v); }
 
-// Sliced rows of one grid, as kernel arguments.
+// Sliced rows of one grid, as kernel arguments. With diaM (2 or 3) the values a kernel gets are the
+// grid's diagonal copy instead (value of face (r, r + diaOff[k]) at k * diaPad + r), read by diaOffSum.
 struct SellView
 {
     int n;
@@ -49,12 +50,15 @@ struct SellView
     const uint8_t* pattern;
     const int* patOff;
     int width;
+    int diaM;
+    int diaPad;
+    int diaOff[kDiaMax];
 };
 
 inline SellView viewOf(const Sell& s)
 {
     return SellView{s.n, s.slicePtr.p, s.patterned ? nullptr : s.cols.p, s.patterned ? s.pattern.p : nullptr,
-                    s.patterned ? s.patOff.p : nullptr, s.width};
+                    s.patterned ? s.patOff.p : nullptr, s.width, 0, 0, {}};
 }
 
 // sum_k A'_ck * X(col_k) over row c's entries in face order, in the vector precision. X is a functor
@@ -107,9 +111,46 @@ __device__ __forceinline__ Tv rowEntries(const SellView& S, int c, int lane, int
     }
 }
 
+// The same sum on a grid's diagonal copy (M offsets): row c's lower entries at columns c - off[M-1] ..
+// c - off[0], then its upper ones at c + off[0] .. c + off[M-1] -- its face order -- with a zero
+// coefficient where a face is absent (a column outside the grid reads the row itself). No row pattern or
+// column is loaded: every load's address follows from c, so all of them issue at once. The upper
+// coefficients stream (nontemporal); a lower one is the upper coefficient of the row above it, reread
+// through the cache.
+template <int M, typename Tm, typename Tv, typename XF>
+__device__ __forceinline__ Tv diaOffSum(const SellView& S, int c, const Tm* __restrict__ val, XF X, Tv acc)
+{
+    Tv a[2 * M], xv[2 * M];
+    int col[2 * M];
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+    {
+        const int kk = M - 1 - k;
+        const int j = c - S.diaOff[kk];
+        const bool in = j >= 0;
+        col[k] = in ? j : c;
+        const Tv v = cvt<Tv>(val[(size_t)kk * S.diaPad + col[k]]);
+        a[k] = in ? v : Tv(0);
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+    {
+        const int j = c + S.diaOff[k];
+        col[M + k] = j < S.n ? j : c;
+        a[M + k] = cvt<Tv>(ldStream(val + (size_t)k * S.diaPad + c));
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * M; ++k) xv[k] = X(col[k]);
+#pragma unroll
+    for (int k = 0; k < 2 * M; ++k) acc += a[k] * xv[k];
+    return acc;
+}
+
 template <typename Tm, typename Tv, typename XF>
 __device__ __forceinline__ Tv rowOffSum(const SellView& S, int c, const Tm* __restrict__ val, XF X, Tv acc)
 {
+    if (S.diaM == 3) return diaOffSum<3, Tm, Tv>(S, c, val, X, acc);
+    if (S.diaM == 2) return diaOffSum<2, Tm, Tv>(S, c, val, X, acc);
     const int s = __builtin_amdgcn_readfirstlane(c >> 6), lane = c & 63;
     const int p0 = __builtin_amdgcn_readfirstlane(S.slicePtr[s]);
     const int w = __builtin_amdgcn_readfirstlane((S.slicePtr[s + 1] - p0) >> 6);

@@ -127,6 +127,7 @@ _SIGS = {
     "mfv_amg_clone": [P, P, P],
     "mfv_amg_num_grids": [P, P],
     "mfv_amg_grid_size": [P, I, P, P],
+    "mfv_amg_fine_stencil": [P, P, P],
     "mfv_amg_level_addressing": [P, I, P, P, P, P, P],
     "mfv_amg_galerkin": [P, P, P, P, P, I],
     "mfv_amg_grid_values": [P, P, I, P, P],
@@ -373,6 +374,13 @@ class Amg:
         nc, nf = ctypes.c_int(), ctypes.c_int()
         lib().mfv_amg_grid_size(self.h, g, ctypes.byref(nc), ctypes.byref(nf))
         return nc.value, nf.value
+
+    def fine_stencil(self):
+        """the fine grid's stencil offsets when the outer product reads it as diagonals, else []"""
+        m = ctypes.c_int()
+        offs = (ctypes.c_int * 4)()
+        self.ctx._check(lib().mfv_amg_fine_stencil(self.h, ctypes.byref(m), offs), "amg_fine_stencil")
+        return list(offs[:m.value])
 
     def level(self, i):
         import numpy as np

@@ -228,6 +228,12 @@ int mfv_amg_clone(mfv_ctx ctx, mfv_amg src, mfv_amg* out);
 /* number of grids (the mesh plus its coarse levels) */
 int mfv_amg_num_grids(mfv_amg amg, int* nGrids);
 int mfv_amg_grid_size(mfv_amg amg, int grid, int* nCells, int* nFaces);
+/* Inspection, no counterpart in the reference: the fine grid's fixed stencil, if it has one. When every
+ * face's neighbour - owner is one of at most four positive offsets and the faces are in (owner,
+ * neighbour) order, the outer product reads the fine matrix as diagonals (each coefficient stored once):
+ * *nOffsets is their number and hOffsets[0..*nOffsets) (may be NULL; room for four) the ascending
+ * offsets. 0 offsets: the fine matrix is read as sliced rows. */
+int mfv_amg_fine_stencil(mfv_amg amg, int* nOffsets, int* hOffsets);
 /* host copies of the step from grid `level` to grid `level+1` (any pointer may be NULL):
  * restrict map [nCells(level)], coarse owner/neighbour [nFaces(level+1)], faceRestrict and faceFlip
  * [nFaces(level)] -- AMGLevel's map/cOwn/cNei/faceRestrict/faceFlip (device_amg.cuh:119-182) */

@@ -23,7 +23,7 @@ class System:
 
     def __init__(self, nC, own, nei, bFaceCell, magSf, Sf, diag, upper, b, label):
         self.label = label
-        self.nC, self.own, self.nei = nC, O.i32(own), O.i32(nei)
+        self.nC, self.own, self.nei, self.bfc = nC, O.i32(own), O.i32(nei), O.i32(bFaceCell)
         self.magSf, self.Sf = O.f64(magSf), O.f64(Sf)
         self.diag, self.upper, self.b = O.f64(diag), O.f64(upper), O.f64(b)
         self.case = G.Case(nC, self.own, self.nei, O.i32(bFaceCell))
@@ -181,6 +181,52 @@ def test_pcg_bit_exact_with_sequential_reductions(systems, prec, flexible, scale
         assert perf.finalResidual == ref.finalResidual
         assert np.array_equal(G.host(psi), ref_psi)
         amg.close()
+
+
+def test_fine_stencil_layouts(systems):
+    """The hex channel's faces are a fixed stencil (+1, +nx, +nx*ny): its outer product reads the fine
+    matrix as diagonals. pitzDaily's blocks give more offsets: sliced rows."""
+    pitz, chan = systems
+    _, a = build_pair(chan, "brae")
+    assert a.fine_stencil() == [1, 40, 40 * 20]
+    a.close()
+    _, a = build_pair(pitz, "brae")
+    assert a.fine_stencil() == []
+    a.close()
+
+
+def neighbour_order_broken(S):
+    """The same system with the faces of every owner that has two or more in descending neighbour order:
+    still owner-sorted (a valid LDU mesh), but no longer a row's columns in ascending order."""
+    own, nei = S.own.copy(), S.nei.copy()
+    perm = np.arange(own.size)
+    starts = np.flatnonzero(np.r_[True, own[1:] != own[:-1]])
+    ends = np.r_[starts[1:], own.size]
+    for a, b in zip(starts, ends):
+        perm[a:b] = perm[a:b][::-1]
+    return System(S.nC, own[perm], nei[perm], S.bfc, S.magSf[perm], S.Sf[perm], S.diag, S.upper[perm], S.b,
+                  S.label + " (reordered)")
+
+
+@pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32])
+def test_pcg_bit_exact_without_the_stencil_order(systems, prec):
+    """A hex mesh whose owned faces are out of neighbour order keeps the sliced rows (the diagonals would
+    sum its rows in another order) and stays bit-exact with the oracle, which sums in face order."""
+    S = neighbour_order_broken(systems[1])
+    print("case:", S.label, prec, flush=True)
+    ag, amg = build_pair(S, "brae")
+    assert amg.fine_stencil() == []
+    amg.galerkin(S.dd, S.du, prec)
+    S.case.ctx.sequential_reduce(True)
+    psi = G.zeros(S.nC)
+    perf = amg.solve(S.dd, S.du, S.db, psi, tolerance=1e-9, precision=prec, flexible=1, checkEvery=3)
+    S.case.ctx.sequential_reduce(False)
+    ref_psi, ref = O.brae_amgpcg(ag, S.diag, S.upper, S.b, np.zeros(S.nC), 1e-9, flexible=True,
+                                 single=(prec == mfv.PREC_FP32))
+    assert perf.nIterations == ref.nIterations
+    assert perf.finalResidual == ref.finalResidual
+    assert np.array_equal(G.host(psi), ref_psi)
+    amg.close()
 
 
 @pytest.mark.parametrize("prec", [mfv.PREC_FP64, mfv.PREC_FP32, mfv.PREC_FP16])
