@@ -455,6 +455,9 @@ AmgBuildControls buildControls(const mfv_amg_controls* controls)
         throw Error(kErrUnsupported, "mfv: amg build: unknown agglomeration " + std::to_string(controls->agglomeration));
     if (controls->mergeLevels > 4) throw Error(kErrUnsupported, "mfv: amg build: mergeLevels above 4 is not carried");
     bc.mergeLevels = std::max(1, controls->mergeLevels);
+    if (controls->fineLayout != MFV_LAYOUT_AUTO && controls->fineLayout != MFV_LAYOUT_ROWS)
+        throw Error(kErrUnsupported, "mfv: amg build: unknown fine layout " + std::to_string(controls->fineLayout));
+    bc.fineRows = controls->fineLayout == MFV_LAYOUT_ROWS;
     bc.kind = controls->agglomeration == MFV_AGG_BRAE ? AggKind::Brae : AggKind::OpenFOAM;
     bc.target = controls->nCellsInCoarsestLevel > 0 ? controls->nCellsInCoarsestLevel
                                                     : (bc.kind == AggKind::Brae ? 64 : 10);
@@ -544,6 +547,7 @@ int mfv_amg_build_or_load(mfv_ctx ctx, mfv_mesh mesh, const double* hFaceWeights
         std::vector<AggLevel> levels;
         AmgBuildControls stored;
         CacheRead r = amgReadLevels(path, sig, levels, stored);
+        stored.fineRows = bc.fineRows;
         if (r == CacheRead::Loaded)
         {
             try

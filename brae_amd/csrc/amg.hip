@@ -1089,10 +1089,14 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
     const SellView S = cycleView(g, dia);
     const Tm* val = dia ? diaOf<Tm>(g).p : offOf<Tm>(g).p;
     const double* sc = a.scales.p + gi;
+    // x -> y by one Jacobi sweep
+    auto smooth = [&](const Ts* x, Ts* y, const Scatter<Tv, Ts>& to) {
+        smoothK<Tm, Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, x, v.b, y, v.xm, to, stop);
+    };
     // pre-sweeps after the first (the first ran from zero where b was made)
     for (int s = 1; s < vc.nPre; ++s)
     {
-        smoothK<Tm, Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, v.xm, none, stop);
+        smooth(v.x, v.y, none);
         std::swap(v.x, v.y);
     }
     if (vc.nPre == 0) zeroK<Ts><<<blocks(n), kTpb, 0, c.stream>>>(n, v.x, stop);
@@ -1156,8 +1160,7 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
         }
         for (int s = 0; s < nExtra; ++s)
         {
-            smoothK<Tm, Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(S, val, v.diag, sc, v.x, v.b, v.y, v.xm,
-                                                                  s + 1 == nExtra ? up : none, stop);
+            smooth(v.x, v.y, s + 1 == nExtra ? up : none);
             std::swap(v.x, v.y);
         }
     }
@@ -1270,7 +1273,7 @@ void amgBuildFrom(Ctx& c, const Mesh& m, Amg& a, std::vector<AggLevel> levels, c
             g.upper64.resize(g.nFaces);
         }
         if (gi < nG - 1 || gi == 0) buildSell(c, g.sell, g.n, own, nei);
-        if (gi == 0) buildDia(c, g, own, nei);
+        if (gi == 0 && !ctl.fineRows) buildDia(c, g, own, nei);
     }
     a.nCoarsest = a.grids.back().n;
     Grid& gl = a.grids.back();

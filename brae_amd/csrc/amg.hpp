@@ -92,6 +92,7 @@ struct AmgBuildControls
     AggKind kind = AggKind::Brae;
     int target = 64;
     int mergeLevels = 1;   // pairwise passes composed into each level
+    bool fineRows = false; // grid 0 as sliced rows only, even on a fixed stencil (no diagonal copies)
 };
 
 // Work vectors and device state of the PCG solve on a hierarchy (FP64, one per cell), and the captured

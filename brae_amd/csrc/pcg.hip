@@ -303,11 +303,10 @@ __device__ __forceinline__ void diaRows(const DiaView& D, int c0, int end, const
     }
 }
 
-// rows per thread of the diagonal product (one pass: launched on reduceGridRows(n, kDiaRows))
-constexpr int kDiaRows = 1;
-
+// One row per thread: with a slab partition (part.plane > 0, launched on slabBlocks) its rows, else
+// the contiguous XCD-ordered chunks of a reduceGridRows(n, 1) launch.
 template <int M, typename Tv, typename Ts>
-__global__ __launch_bounds__(kReduceBlock) void pcgDirDiaK(DiaView D, const double* __restrict__ up,
+__global__ __launch_bounds__(kReduceBlock) void pcgDirDiaK(DiaView D, SlabPart part, const double* __restrict__ up,
                                                            const double* __restrict__ diag, const Ts* __restrict__ z,
                                                            const float* __restrict__ xm,
                                                            const double* __restrict__ sig,
@@ -318,26 +317,28 @@ __global__ __launch_bounds__(kReduceBlock) void pcgDirDiaK(DiaView D, const doub
     if (d.fl[F_STOP]) return;
     __shared__ double lds[kReduceBlock / 64];
     const DirX<Tv, Ts> P{cycleOut<Tv, Ts>(z, xm, sig), pOld, d.st[S_BETA], d.fl[F_ITER] == 0 ? 1 : 0};
-    const int chunk = chunkRows(D.n, gridDim.x);
-    const int begin = xcdBlock() * chunk;
-    const int end = begin + chunk < D.n ? begin + chunk : D.n;
     double s = 0.0;
-    for (int c = begin + (int)threadIdx.x; c < end; c += kDiaRows * kReduceBlock)
+    auto row = [&](int c) {
+        double ax[1], pc[1];
+        diaRows<1, M>(D, c, c + 1, up, diag, P, ax, pc);
+        __builtin_nontemporal_store(pc[0], pNew + c);
+        __builtin_nontemporal_store(ax[0], q + c);
+        if (seqA)
+            seqA[c] = ax[0] * pc[0];
+        else
+            s += ax[0] * pc[0];
+    };
+    if (part.plane)
     {
-        double ax[kDiaRows], pc[kDiaRows];
-        diaRows<kDiaRows, M>(D, c, end, up, diag, P, ax, pc);
-#pragma unroll
-        for (int i = 0; i < kDiaRows; ++i)
+        for (int r = 0; r < part.rpt; ++r)
         {
-            const int ci = c + i * kReduceBlock;
-            if (ci >= end) continue;
-            __builtin_nontemporal_store(pc[i], pNew + ci);
-            __builtin_nontemporal_store(ax[i], q + ci);
-            if (seqA)
-                seqA[ci] = ax[i] * pc[i];
-            else
-                s += ax[i] * pc[i];
+            const int c = slabRow(part, r);
+            if (c >= 0) row(c);
         }
+    }
+    else
+    {
+        MFV_ROWS(D.n, c) row(c);
     }
     if (seqA) return;
     storePartial(blockSum(s, lds), partials);
@@ -468,12 +469,15 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     const Grid& g0 = a.grids[0];
     const SellView S = viewOf(g0.sell);
     const int gR = reduceGrid(n);
-    // rows per thread, measured per kernel on the 20 M channel: the FP64 direction product 4 (kDiaRows
-    // on the diagonal copy), z.r 8, the update 2 (reduceGrid)
+    // rows per thread, measured per kernel on the 20 M channel: the FP64 direction product 4 (1 on the
+    // diagonal copy), z.r 8, the update 2 (reduceGrid)
     const bool dia = g0.diaM > 0;
     DiaView D{n, g0.diaPad, {}};
     for (int k = 0; k < kDiaMax; ++k) D.off[k] = g0.diaOff[k];
-    const int gRd = reduceGridRows(n, dia ? kDiaRows : 4);
+    // the diagonal product: one row per thread, on plane slabs when the stencil's planes are wide enough
+    const int plane = dia ? g0.diaOff[g0.diaM - 1] : 0;
+    const SlabPart part = plane >= kSlabMinPlane ? slabPart(n, plane) : SlabPart{n, 0, 1};
+    const int gRd = !dia ? reduceGridRows(n, 4) : part.plane ? slabBlocks(n, part.plane, part.rpt) : reduceGridRows(n, 1);
     const int gRz = reduceGridRows(n, 8);
     const int gRu = reduceGrid(n);
     double* pa = c.partials.p + 5 * (size_t)kMaxReduceBlocks;
@@ -542,19 +546,19 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
             switch (dia ? g0.diaM : 0)
             {
             case 1:
-                pcgDirDiaK<1, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
+                pcgDirDiaK<1, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, part, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
                                                                          w.q.p, d, pa, sA);
                 break;
             case 2:
-                pcgDirDiaK<2, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
+                pcgDirDiaK<2, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, part, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
                                                                          w.q.p, d, pa, sA);
                 break;
             case 3:
-                pcgDirDiaK<3, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
+                pcgDirDiaK<3, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, part, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
                                                                          w.q.p, d, pa, sA);
                 break;
             case 4:
-                pcgDirDiaK<4, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
+                pcgDirDiaK<4, Tv, Ts><<<gRd, kReduceBlock, 0, c.stream>>>(D, part, g0.dia64.p, diag, z, xm0, sig, pOld, pNew,
                                                                          w.q.p, d, pa, sA);
                 break;
             default:

@@ -170,6 +170,48 @@ struct StoreTo
     __device__ void operator()(double s) const { *out = s; }
 };
 
+// Plane-slab partition of a grid whose rows form planes of `plane` rows (a fixed stencil's largest
+// offset; the last plane may be partial). XCD k -- workgroup b runs on XCD b % 8 -- takes the k-th
+// eighth (slab) of every plane, plane after plane, and each of its workgroups the next rpt*kReduceBlock
+// rows of that sequence, thread t rows t, t + kReduceBlock, ... A row's neighbour one plane away was
+// then touched by the same XCD one slab earlier, still in its L2, where the contiguous partition has it
+// an eighth of the whole grid earlier. Every row falls to exactly one thread. plane >= 8 * 64.
+struct SlabPart
+{
+    int n, plane, rpt;
+};
+
+constexpr int kSlabMinPlane = 8 * 64;
+
+// workgroups of a slab partition with rpt rows per thread
+inline int slabBlocks(int n, int plane, int rpt)
+{
+    const long long nPl = ((long long)n + plane - 1) / plane;
+    const long long slabMax = (plane + 7) / 8;
+    const long long per = (long long)kReduceBlock * rpt;
+    return (int)(8 * ((nPl * slabMax + per - 1) / per));
+}
+
+// the fewest rows per thread that keep the workgroups within kMaxReduceBlocks partials
+inline SlabPart slabPart(int n, int plane)
+{
+    int rpt = 1;
+    while (slabBlocks(n, plane, rpt) > kMaxReduceBlocks) ++rpt;
+    return SlabPart{n, plane, rpt};
+}
+
+// this thread's r-th row (r < rpt), or -1
+__device__ __forceinline__ int slabRow(const SlabPart& m, int r)
+{
+    const int k = blockIdx.x & 7;
+    const int s0 = (int)((long long)k * m.plane >> 3), s1 = (int)((long long)(k + 1) * m.plane >> 3);
+    const int slab = s1 - s0;
+    const int p = ((int)(blockIdx.x >> 3) * m.rpt + r) * kReduceBlock + (int)threadIdx.x;
+    const int z = p / slab;
+    const long long row = (long long)z * m.plane + s0 + (p - z * slab);
+    return row < m.n ? (int)row : -1;
+}
+
 // Row loop over this workgroup's chunk of a reduceGrid partition, XCD-ordered.
 #define MFV_ROWS(n, c)                                                                                            \
     const int mfvChunk_ = chunkRows((n), gridDim.x);                                                             \

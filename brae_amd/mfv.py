@@ -33,7 +33,7 @@ class Ldu(ctypes.Structure):
 
 
 class AmgControls(ctypes.Structure):
-    _fields_ = [("agglomeration", I), ("nCellsInCoarsestLevel", I), ("mergeLevels", I)]
+    _fields_ = [("agglomeration", I), ("nCellsInCoarsestLevel", I), ("mergeLevels", I), ("fineLayout", I)]
 
 
 class PcgControls(ctypes.Structure):
@@ -43,6 +43,7 @@ class PcgControls(ctypes.Structure):
 
 
 AGG_BRAE, AGG_FACEAREAPAIR = 0, 1
+LAYOUT_AUTO, LAYOUT_ROWS = 0, 1
 CACHE_LOADED, CACHE_ABSENT, CACHE_OTHER_MESH_OR_BUILD, CACHE_UNREADABLE = range(4)
 PROF_PCG_SPMV64, PROF_PCG_UPDATE, PROF_PCG_ZR, PROF_VCYCLE, PROF_AMG_RESIDUAL0, PROF_AMG_CORR_SMOOTH0, \
     PROF_AMG_RESTRICT0, PROF_AMG_CORR_AMUL0 = range(8)
@@ -333,7 +334,7 @@ class Amg:
     """AMG hierarchy of a mesh (host agglomeration, device levels)."""
 
     def __init__(self, mesh, weights, agglomeration=AGG_BRAE, nCellsInCoarsestLevel=0, mergeLevels=1,
-                 cacheDir=None, writeCache=False, _handle=None):
+                 cacheDir=None, writeCache=False, _handle=None, fineLayout=LAYOUT_AUTO):
         """build the hierarchy; with cacheDir, load <cacheDir>/.mfv_amgcache when it is this mesh's
         (self.how tells what the file held: CACHE_*)"""
         import numpy as np
@@ -346,7 +347,7 @@ class Amg:
         else:
             w = np.ascontiguousarray(weights, np.float64)
             assert w.size == mesh.nFaces
-            c = AmgControls(agglomeration, nCellsInCoarsestLevel, mergeLevels)
+            c = AmgControls(agglomeration, nCellsInCoarsestLevel, mergeLevels, fineLayout)
             if cacheDir is None:
                 self.ctx._check(lib().mfv_amg_build(self.ctx.h, mesh.h, ptr(w), ctypes.byref(c), ctypes.byref(self.h)),
                                 "amg_build")

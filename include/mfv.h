@@ -183,7 +183,14 @@ int mfv_norm_factor(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* 
 
 #define MFV_PREC_FP64 64
 #define MFV_PREC_FP32 32
-#define MFV_PREC_FP16 16        /* FP16 coefficients scaled per grid by a power of two, FP32 vectors */
+#define MFV_PREC_FP16 16        /* FP16 coefficients and vectors scaled per grid by powers of two, FP32
+                                   arithmetic */
+
+/* How the fine grid's matrix is read (mfv_amg_controls.fineLayout). Automatic: as diagonals when its faces
+ * form a fixed stencil (mfv_amg_fine_stencil), else as sliced rows; rows: always sliced rows. Both sum
+ * every row in face order; the choice changes speed, not results. */
+#define MFV_LAYOUT_AUTO 0
+#define MFV_LAYOUT_ROWS 1
 
 typedef struct
 {
@@ -192,6 +199,7 @@ typedef struct
                                    faceAreaPair: nCellsInCoarsestLevel (default 10) */
     int mergeLevels;            /* pairwise passes composed into one level (GAMG's mergeLevels, 1..4;
                                    0 or 1: every pass is a level, the reference's only setting) */
+    int fineLayout;             /* MFV_LAYOUT_*; not part of the hierarchy's signature */
 } mfv_amg_controls;
 
 /* buildAMG (src/matrices/lduMatrix/preconditioners/GAMGPreconditioner/device_amg.cuh:393,
