@@ -245,8 +245,10 @@ def gpu_run(args):
         mfv.PROF_PCG_SPMV64: (("pcgDirDiaK: p = z + beta*p, q = A.p (FP64 diagonals), p.q" if amg.fine_stencil() else
                                "pcgDirAmulK: p = z + beta*p, q = A.p (FP64), p.q"),
                               8 * (N + F) + 8 * F + zb * N + 24 * N, iters),
+        # reads p, q, psi, r; writes psi and the next r (into the other of two alternating residual
+        # buffers: the flexible beta's r_old needs no copy); the next cycle's b and x; the diagonal
         mfv.PROF_PCG_UPDATE: ("pcgUpdateK: psi, r update, sumMag(r), next cycle entry",
-                              8 * (2 * N + 4 * N + N) + 2 * zb * N + dw * N, iters),
+                              8 * (4 * N + 2 * N) + 2 * zb * N + dw * N, iters),
         mfv.PROF_PCG_ZR: ("pcgZrK: z.r, z.r_old", zb * N + 16 * N, iters),
         mfv.PROF_AMG_RESIDUAL0: ("residualK grid 0 (cycle precision)", vb * F + dw * N + 3 * zb * N + 8 * F, iters),
         mfv.PROF_AMG_CORR_AMUL0: ("corrAmulK grid 0: A e, r.e, e.Ae", vb * F + dw * N + 3 * zb * N + 8 * F, iters),

@@ -90,8 +90,9 @@ __global__ __launch_bounds__(kTpb) void smoothK(SellView S, const Tm* __restrict
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= S.n) return;
     const CycVec<Tv, Ts> X(x, xm), B(b, nullptr), XO(xOut, xm);
-    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X);
-    const Tv xo = X(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(diag[c]);
+    Tv dc;
+    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X, &dc);
+    const Tv xo = X(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(dc);
     XO.set(c, xo);
     sc(c, xo);
 }
@@ -119,8 +120,9 @@ __global__ __launch_bounds__(kTpb) void prolongSmoothK(SellView S, const Tm* __r
     if (c >= S.n) return;
     const SumX<Tv, Ts> X{CycVec<Tv, Ts>(x, xm), CycVec<Tv, Ts>(e, xm)};
     const CycVec<Tv, Ts> B(b, nullptr), XO(xOut, xm);
-    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X);
-    const Tv xo = X(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(diag[c]);
+    Tv dc;
+    const Tv ax = rowAx<Tm, Tv>(S, c, val, diag, scale, X, &dc);
+    const Tv xo = X(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(dc);
     XO.set(c, xo);
     sc(c, xo);
 }

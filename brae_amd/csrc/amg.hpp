@@ -99,7 +99,7 @@ struct AmgBuildControls
 // graph of a group of iterations with the key it was captured for.
 struct PcgWork
 {
-    DBuf<double> r, p0, p1, q, rOld, st;
+    DBuf<double> r, p0, p1, q, rOld, st;   // r and rOld alternate as the residual and its predecessor
     DBuf<int> fl;
     DBuf<double> seqA, seqB;   // per-element terms for sequential (OpenFOAM-order) reductions
     hipGraphExec_t graph = nullptr;

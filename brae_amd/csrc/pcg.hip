@@ -369,12 +369,14 @@ struct UpdPost
     }
 };
 
-// with `sig` (the FP16 cycle) the next cycle's input is r/sig[1], and max|r| goes to pMax
+// with `sig` (the FP16 cycle) the next cycle's input is r/sig[1], and max|r| goes to pMax. The residual
+// goes from rIn to rOut: the two buffers alternate, so rIn is the flexible beta's r_old one iteration on
+// (no copy of r).
 template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kReduceBlock) void pcgUpdateK(int n, const double* __restrict__ p,
                                                            const double* __restrict__ q, double* __restrict__ psi,
-                                                           double* __restrict__ r, double* __restrict__ rOld,
-                                                           int flexible, const Tv* __restrict__ diag0,
+                                                           const double* __restrict__ rIn, double* __restrict__ rOut,
+                                                           const Tv* __restrict__ diag0,
                                                            Ts* __restrict__ b0, Ts* __restrict__ x0,
                                                            const float* __restrict__ xm, const double* __restrict__ sig,
                                                            int fromZero, PcgDev d, double* partials, double* pMax,
@@ -390,10 +392,8 @@ __global__ __launch_bounds__(kReduceBlock) void pcgUpdateK(int n, const double* 
     MFV_ROWS(n, i)
     {
         psi[i] += alpha * p[i];
-        const double ro = r[i];
-        if (flexible) rOld[i] = ro;
-        const double rn = ro - alpha * q[i];
-        r[i] = rn;
+        const double rn = rIn[i] - alpha * q[i];
+        rOut[i] = rn;
         if (seqA)
             seqA[i] = fabs(rn);
         else
@@ -516,7 +516,10 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     vc.scale = pc.scaleCorrection;
     amgStartFrom(c, a, w.r.p, vc, w.fl.p + F_STOP);
     double* p[2] = {w.p0.p, w.p1.p};
-    // one iteration, enqueued; `it` selects the direction buffers (they alternate)
+    // the residual of iteration it is in R[it & 1], the previous one (the flexible beta's r_old) in the
+    // other, which the update then overwrites with the next residual
+    double* R[2] = {w.r.p, w.rOld.p};
+    // one iteration, enqueued; `it` selects the direction and residual buffers (they alternate)
     auto iteration = [&](int it) {
         const Ts* z;
         {
@@ -526,11 +529,11 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         {
             ProfScope ps(c, kProfPcgZr);
             if (pc.flexible)
-                pcgZrK<Tv, Ts, true><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, xm0, sig, w.r.p, w.rOld.p, d, pa, pb, sA,
-                                                                        sB);
+                pcgZrK<Tv, Ts, true><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, xm0, sig, R[it & 1], R[(it + 1) & 1], d,
+                                                                        pa, pb, sA, sB);
             else
-                pcgZrK<Tv, Ts, false><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, xm0, sig, w.r.p, w.rOld.p, d, pa, pb,
-                                                                         sA, sB);
+                pcgZrK<Tv, Ts, false><<<gRz, kReduceBlock, 0, c.stream>>>(n, z, xm0, sig, R[it & 1], R[(it + 1) & 1],
+                                                                         d, pa, pb, sA, sB);
         }
         {
             const ZrPost post{d, pc.flexible ? 1 : 0};
@@ -572,8 +575,8 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
             finishK<<<1, kFinishBlock, 0, c.stream>>>(gRd, pa, PqPost{d}, stopF);
         {
             ProfScope ps(c, kProfPcgUpdate);
-            pcgUpdateK<Tv, Ts><<<gRu, kReduceBlock, 0, c.stream>>>(n, p[(it + 1) & 1], w.q.p, psi, w.r.p, w.rOld.p,
-                                                                 pc.flexible ? 1 : 0, diag0, b0, x0, xm0, sig,
+            pcgUpdateK<Tv, Ts><<<gRu, kReduceBlock, 0, c.stream>>>(n, p[(it + 1) & 1], w.q.p, psi, R[it & 1],
+                                                                 R[(it + 1) & 1], diag0, b0, x0, xm0, sig,
                                                                  pc.nPre > 0 ? 1 : 0, d, pa, pMax, sA);
         }
         const UpdPost upd{d, sig};

@@ -162,12 +162,14 @@ __device__ __forceinline__ Tv rowOffSum(const SellView& S, int c, const Tm* __re
 
 // A x at row c: D x + sum (FP64/FP32: accumulated term by term after D x, the reference's order);
 // FP16: D x + s * sum(A' x) with A' = A/s stored.
+// (dOut: the diagonal it loaded, for a caller that also divides by it)
 template <typename Tm, typename Tv, typename XF>
 __device__ __forceinline__ Tv rowAx(const SellView& S, int c, const Tm* __restrict__ val, const Tv* __restrict__ diag,
-                                    const double* __restrict__ scale, XF X)
+                                    const double* __restrict__ scale, XF X, Tv* dOut = nullptr)
 {
     const Tv xc = X(c);
     const Tv dc = ldStream(diag + c);
+    if (dOut) *dOut = dc;
     if constexpr (sizeof(Tm) == 2)
     {
         const Tv off = rowOffSum<Tm, Tv>(S, c, val, X, Tv(0));
