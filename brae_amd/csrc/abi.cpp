@@ -887,6 +887,108 @@ int mfv_flux_correct(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double*
     });
 }
 
+int mfv_fold_boundary_diag(mfv_ctx ctx, mfv_mesh mesh, const double* dDiag, const double* dIcX,
+                           const double* dIcY, const double* dIcZ, double* dD)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dDiag && dD && (mesh->m.nBFaces == 0 || (dIcX && dIcY && dIcZ)), "fold boundary diag: null argument");
+        launchFoldBoundaryDiag(ctx->c, mesh->m, dIcX, dIcY, dIcZ, dDiag, dD);
+    });
+}
+
+int mfv_reciprocal_v(mfv_ctx ctx, mfv_mesh mesh, const double* dD, const double* dV, double* dRAU)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dD && dV && dRAU, "reciprocal V: null argument");
+        launchReciprocalV(ctx->c, mesh->m, dD, dV, dRAU);
+    });
+}
+
+int mfv_boundary_diag_term(mfv_ctx ctx, mfv_mesh mesh, const double* dIcX, const double* dIcY, const double* dIcZ,
+                           int k, double* dBdDiag)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(k >= 0 && k < 3, "boundary diag term: component must be 0, 1 or 2");
+        need(mesh->m.nBFaces == 0 || (dIcX && dIcY && dIcZ && dBdDiag), "boundary diag term: null argument");
+        launchBoundaryDiagTerm(ctx->c, mesh->m.nBFaces, dIcX, dIcY, dIcZ, k, dBdDiag);
+    });
+}
+
+int mfv_matrix_h(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dSource,
+                 const double* dBdDiag, const double* dBdSrc, const double* dV, const double* dRAU, int valid,
+                 double* dOut)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dOut != nullptr, "matrix H: null output");
+        if (!valid)
+        {
+            launchZero(ctx->c, mesh->m.nCells, dOut);
+            return;
+        }
+        const LduView L = view(mesh, A);
+        need(dPsi && dSource && dV && (mesh->m.nBFaces == 0 || (dBdDiag && dBdSrc)), "matrix H: null argument");
+        launchMatrixH(ctx->c, mesh->m, L.upper, L.lower, dPsi, dSource, dBdDiag, dBdSrc, dV, dRAU, dOut);
+    });
+}
+
+int mfv_constrain_hbya(mfv_ctx ctx, mfv_mesh mesh, const int* dTakeU, const double* dUb, const double* dHbyA,
+                       double* dHbyAb)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(mesh->m.nBFaces == 0 || (dTakeU && dUb && dHbyA && dHbyAb), "constrain HbyA: null argument");
+        launchConstrainHbyA(ctx->c, mesh->m, dTakeU, dUb, dHbyA, dHbyAb);
+    });
+}
+
+int mfv_vector_flux(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dSf, const double* dUx,
+                    const double* dUy, const double* dUz, double* dPhi)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(mesh->m.nFaces == 0 || (dWeights && dSf && dUx && dUy && dUz && dPhi), "vector flux: null argument");
+        launchVectorFlux(ctx->c, mesh->m, dWeights, dSf, dUx, dUy, dUz, dPhi);
+    });
+}
+
+int mfv_boundary_flux(mfv_ctx ctx, mfv_mesh mesh, const double* dSf, const double* dUxb, const double* dUyb,
+                      const double* dUzb, const int* dIsEmpty, double* dPhiB)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(mesh->m.nBFaces == 0 || (dSf && dUxb && dUyb && dUzb && dPhiB), "boundary flux: null argument");
+        launchBoundaryFlux(ctx->c, mesh->m, dSf ? dSf + 3 * (size_t)mesh->m.nFaces : nullptr, dUxb, dUyb, dUzb,
+                           dIsEmpty, dPhiB);
+    });
+}
+
+int mfv_adjust_phi(mfv_ctx ctx, int nBFaces, const int* dAdjustable, double* dPhiB, int nInternalFaces,
+                   const double* dPhiInt, double* massCorr, int* closedVolume)
+{
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "null context");
+        need(nBFaces >= 0 && nInternalFaces >= 0, "adjust phi: negative size");
+        need(nBFaces == 0 || (dAdjustable && dPhiB), "adjust phi: null argument");
+        need(nInternalFaces == 0 || dPhiInt, "adjust phi: internal flux missing");
+        Ctx& c = ctx->c;
+        double* decision = c.scalars.p + 40;
+        launchAdjustPhi(c, nBFaces, dAdjustable, dPhiB, nInternalFaces, dPhiInt, decision);
+        hipCheck(hipMemcpyAsync(c.mailbox, decision, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                 "adjust phi read-back");
+        hipCheck(hipStreamSynchronize(c.stream), "adjust phi sync");
+        const double mc = c.mailbox[0], refused = c.mailbox[1], closed = c.mailbox[2];
+        if (massCorr) *massCorr = refused != 0.0 ? 1.0 : mc;
+        if (closedVolume) *closedVolume = closed != 0.0 ? 1 : 0;
+        if (refused != 0.0)
+            invalid("adjust phi: the continuity error cannot be removed by adjusting the outflow (adjustPhi.C): "
+                    "check the velocity boundary conditions");
+    });
+}
+
 int mfv_grad_gauss(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double* dBcValue, const double* dSf,
                    const double* dWeights, const double* dV, const double* dPsi, double* dGrad)
 {

@@ -1,0 +1,325 @@
+// The SIMPLE pressure predictor's momentum glue on the device: A()'s folded diagonal and rAU, H() per
+// component (and HbyA = rAU*H), constrainHbyA, phiHbyA = flux(HbyA) with its boundary faces, and
+// adjustPhi. Reference: pressurePredictor (src/applications/solvers/simpleFoam/pEqn.cu:94-335) and its
+// host twin (pEqn_cpp.cu:40-133, fv_matrix_ops.cuh:211-340, fvc.cu:545-580).
+//
+// Every per-cell sum is a gather in the host loop's order, term by term: a cell's boundary faces
+// ascending (the patch order the host walks), and its internal faces ascending -- the losort run, then
+// the owned run (OpenFOAM's upper-triangular order puts every face a cell neighbours before the faces it
+// owns). No atomics, so the FP64 results are the host loops' to the bit.
+#include "reduce.hpp"
+
+namespace mfv {
+
+namespace {
+
+constexpr int kTpb = 256;
+inline int blocks(int n) { return (n + kTpb - 1) / kTpb; }
+
+// D = diag + cmptAv(internalCoeffs) face by face (fv_matrix_ops.cuh:211-226; pEqn.cu:117-138, the
+// cmptAv and foldBoundaryDiag kernels), cmptAv = (x + y + z)/3
+__global__ __launch_bounds__(kTpb) void foldBoundaryDiagK(int n, const label* __restrict__ bndStart,
+                                                          const label* __restrict__ bndFaces,
+                                                          const double* __restrict__ icX,
+                                                          const double* __restrict__ icY,
+                                                          const double* __restrict__ icZ,
+                                                          const double* __restrict__ diag, double* __restrict__ D)
+{
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    if (c >= n) return;
+    double d = diag[c];
+    for (int k = bndStart[c]; k < bndStart[c + 1]; ++k)
+    {
+        const int i = bndFaces[k];
+        d += (icX[i] + icY[i] + icZ[i]) / 3.0;
+    }
+    D[c] = d;
+}
+
+// rAU = 1/A, A = D/V (device_simple.cu reciprocalVKernel; fv_matrix_ops.cuh:224-225)
+__global__ __launch_bounds__(kTpb) void reciprocalVK(int n, const double* __restrict__ D, const double* __restrict__ V,
+                                                     double* __restrict__ rAU)
+{
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    if (c >= n) return;
+    const double A = D[c] / V[c];
+    rAU[c] = 1.0 / A;
+}
+
+// cmptAv(ic) - ic_k per boundary face (pEqn.cu:146-149; fv_matrix_ops.cuh:290-300)
+__global__ __launch_bounds__(kTpb) void boundaryDiagTermK(int nB, const double* __restrict__ icX,
+                                                          const double* __restrict__ icY,
+                                                          const double* __restrict__ icZ, int k,
+                                                          double* __restrict__ out)
+{
+    const int i = xcdBlock() * kTpb + threadIdx.x;
+    if (i >= nB) return;
+    const double x = icX[i], y = icY[i], z = icZ[i];
+    const double av = (x + y + z) / 3.0;
+    out[i] = av - (k == 0 ? x : (k == 1 ? y : z));
+}
+
+// H() of one component (fv_matrix_ops.cuh:277-340; device_simple.cu matrixHKernel), optionally times rAU
+// (HbyA, pEqn_cpp.cu:52-53): from zero, bdDiag*psi per boundary face, minus lower*psi[owner] and
+// upper*psi[neighbour] per internal face, plus the source, plus boundaryCoeffs per boundary face, over V
+__global__ __launch_bounds__(kTpb) void matrixHK(int n, const label* __restrict__ ownerStart,
+                                                 const label* __restrict__ losortStart,
+                                                 const label* __restrict__ losort,
+                                                 const label* __restrict__ losortOwner,
+                                                 const label* __restrict__ neighbour,
+                                                 const label* __restrict__ bndStart,
+                                                 const label* __restrict__ bndFaces,
+                                                 const double* __restrict__ upper, const double* __restrict__ lower,
+                                                 const double* __restrict__ psi, const double* __restrict__ source,
+                                                 const double* __restrict__ bdDiag, const double* __restrict__ bdSrc,
+                                                 const double* __restrict__ V, const double* __restrict__ rAU,
+                                                 double* __restrict__ out)
+{
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    if (c >= n) return;
+    const int b0 = bndStart[c], b1 = bndStart[c + 1];
+    const double pc = psi[c];
+    double h = 0.0;
+    for (int k = b0; k < b1; ++k) h += bdDiag[bndFaces[k]] * pc;
+    for (int k = losortStart[c]; k < losortStart[c + 1]; ++k) h = h - lower[losort[k]] * psi[losortOwner[k]];
+    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f) h = h - upper[f] * psi[neighbour[f]];
+    h += source[c];
+    for (int k = b0; k < b1; ++k) h += bdSrc[bndFaces[k]];
+    h = h / V[c];
+    out[c] = rAU ? rAU[c] * h : h;
+}
+
+// constrainHbyA (pEqn.cu constrainHbyAKernel; pEqn_cpp.cu:55-72): U's boundary value on a face whose U
+// is not assignable, the owner cell's HbyA elsewhere
+__global__ __launch_bounds__(kTpb) void constrainHbyAK(int nB, const label* __restrict__ bFaceCell,
+                                                       const int* __restrict__ takeU, const double* __restrict__ Ub,
+                                                       const double* __restrict__ HbyA, double* __restrict__ HbyAb)
+{
+    const int i = xcdBlock() * kTpb + threadIdx.x;
+    if (i >= nB) return;
+    HbyAb[i] = takeU[i] ? Ub[i] : HbyA[bFaceCell[i]];
+}
+
+// flux of a cell vector field on the internal faces: (w U_own + (1-w) U_nei) . Sf (fvc.cu:545-563)
+__global__ __launch_bounds__(kTpb) void vectorFluxK(int nF, const label* __restrict__ owner,
+                                                    const label* __restrict__ neighbour,
+                                                    const double* __restrict__ w, const double* __restrict__ Sf,
+                                                    const double* __restrict__ Ux, const double* __restrict__ Uy,
+                                                    const double* __restrict__ Uz, double* __restrict__ phi)
+{
+    const int f = xcdBlock() * kTpb + threadIdx.x;
+    if (f >= nF) return;
+    const int o = owner[f], n = neighbour[f];
+    const double wf = w[f];
+    const double ux = wf * Ux[o] + (1.0 - wf) * Ux[n];
+    const double uy = wf * Uy[o] + (1.0 - wf) * Uy[n];
+    const double uz = wf * Uz[o] + (1.0 - wf) * Uz[n];
+    const double* s = Sf + 3 * (size_t)f;
+    phi[f] = ux * s[0] + uy * s[1] + uz * s[2];
+}
+
+// ...and on the boundary faces: Ub . Sf_b, zero on an empty face (fvc.cu:564-580)
+__global__ __launch_bounds__(kTpb) void boundaryFluxK(int nB, const double* __restrict__ SfB,
+                                                      const double* __restrict__ uxb, const double* __restrict__ uyb,
+                                                      const double* __restrict__ uzb, const int* __restrict__ isEmpty,
+                                                      double* __restrict__ phiB)
+{
+    const int i = xcdBlock() * kTpb + threadIdx.x;
+    if (i >= nB) return;
+    const double* s = SfB + 3 * (size_t)i;
+    phiB[i] = (isEmpty && isEmpty[i]) ? 0.0 : uxb[i] * s[0] + uyb[i] * s[1] + uzb[i] * s[2];
+}
+
+// adjustPhi's sums (pEqn_cpp.cu:85-105): per workgroup, sum|phi| over internal faces, and the boundary's
+// massIn, fixed outflow and adjustable outflow (four partial slots)
+__global__ __launch_bounds__(kReduceBlock) void adjustSumsK(int nF, const double* __restrict__ phiInt, int nB,
+                                                            const int* __restrict__ adjustable,
+                                                            const double* __restrict__ phiB, double* pTot,
+                                                            double* pIn, double* pFix, double* pAdj)
+{
+    __shared__ double lds[kReduceBlock / 64];
+    const int n = nF > nB ? nF : nB;
+    double t = 0.0, in = 0.0, fix = 0.0, adj = 0.0;
+    MFV_ROWS(n, i)
+    {
+        if (i < nF) t += fabs(phiInt[i]);
+        if (i < nB)
+        {
+            const double v = phiB[i];
+            const bool a = adjustable[i] != 0;
+            const bool out = !(v < 0.0);   // the host's else-branch (a NaN included)
+            in += out ? 0.0 : -v;
+            fix += (out && !a) ? v : 0.0;
+            adj += (out && a) ? v : 0.0;
+        }
+    }
+    const double st = blockSum(t, lds);
+    const double si = blockSum(in, lds);
+    const double sf = blockSum(fix, lds);
+    const double sa = blockSum(adj, lds);
+    if (threadIdx.x == 0)
+    {
+        pTot[blockIdx.x] = st;
+        pIn[blockIdx.x] = si;
+        pFix[blockIdx.x] = sf;
+        pAdj[blockIdx.x] = sa;
+    }
+}
+
+// the decision (pEqn_cpp.cu:107-123; device_simple.cu:282-346): out = {massCorr, refused, closedVolume}
+__device__ void adjustDecide(double tot, double in, double fix, double adj, double* out)
+{
+    const double vsmall = 1e-300, small = 1e-15;
+    const double totalFlux = tot;
+    double mc = 1.0, refused = 0.0;
+    const double magAdj = fabs(adj);
+    if (magAdj > vsmall && magAdj / totalFlux > small)
+        mc = (in - fix) / adj;
+    else if (fabs(fix - in) / totalFlux > 1e-8)
+        refused = 1.0;
+    out[0] = mc;
+    out[1] = refused;
+    out[2] = (fabs(in) / totalFlux < small && fabs(fix) / totalFlux < small && fabs(adj) / totalFlux < small) ? 1.0
+                                                                                                              : 0.0;
+}
+
+__global__ __launch_bounds__(kFinishBlock) void adjustFinishK(int nParts, const double* __restrict__ pTot,
+                                                              const double* __restrict__ pIn,
+                                                              const double* __restrict__ pFix,
+                                                              const double* __restrict__ pAdj, double* out)
+{
+    __shared__ double lds[kFinishBlock / 64];
+    const double t = blockSum<kFinishBlock>(finishShare(nParts, pTot), lds);
+    const double in = blockSum<kFinishBlock>(finishShare(nParts, pIn), lds);
+    const double fix = blockSum<kFinishBlock>(finishShare(nParts, pFix), lds);
+    const double adj = blockSum<kFinishBlock>(finishShare(nParts, pAdj), lds);
+    if (threadIdx.x == 0) adjustDecide(1e-300 + t, in, fix, adj, out);
+}
+
+// the same sums in the host loops' order on one thread (the sequential-reduction mode)
+__global__ void adjustSeqK(int nF, const double* __restrict__ phiInt, int nB, const int* __restrict__ adjustable,
+                           const double* __restrict__ phiB, double* out)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    double t = 1e-300, in = 0.0, fix = 0.0, adj = 0.0;
+    for (int f = 0; f < nF; ++f) t += fabs(phiInt[f]);
+    for (int i = 0; i < nB; ++i)
+    {
+        const double v = phiB[i];
+        if (v < 0.0)
+            in -= v;
+        else if (!adjustable[i])
+            fix += v;
+        else
+            adj += v;
+    }
+    adjustDecide(t, in, fix, adj, out);
+}
+
+// scale the adjustable outflow (pEqn_cpp.cu:125-131); nothing when the correction was refused
+__global__ __launch_bounds__(kTpb) void adjustScaleK(int nB, const int* __restrict__ adjustable,
+                                                     const double* __restrict__ decision, double* __restrict__ phiB)
+{
+    const int i = xcdBlock() * kTpb + threadIdx.x;
+    if (i >= nB) return;
+    const double mc = decision[0];
+    if (decision[1] == 0.0 && adjustable[i] && phiB[i] > 0.0) phiB[i] *= mc;
+}
+
+// zeros (a kernel, not a memset, so the call can sit in a captured graph)
+__global__ __launch_bounds__(kTpb) void zeroK(int n, double* __restrict__ x)
+{
+    const int i = xcdBlock() * kTpb + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+} // namespace
+
+void launchZero(Ctx& c, int n, double* x)
+{
+    if (n <= 0) return;
+    zeroK<<<blocks(n), kTpb, 0, c.stream>>>(n, x);
+    hipCheck(hipGetLastError(), "zero");
+}
+
+void launchFoldBoundaryDiag(Ctx& c, const Mesh& m, const double* icX, const double* icY, const double* icZ,
+                            const double* diag, double* D)
+{
+    if (m.nCells == 0) return;
+    foldBoundaryDiagK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.bndStart.p, m.bndFaces.p, icX, icY, icZ,
+                                                                diag, D);
+    hipCheck(hipGetLastError(), "fold boundary diag");
+}
+
+void launchReciprocalV(Ctx& c, const Mesh& m, const double* D, const double* V, double* rAU)
+{
+    if (m.nCells == 0) return;
+    reciprocalVK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, D, V, rAU);
+    hipCheck(hipGetLastError(), "reciprocal V");
+}
+
+void launchBoundaryDiagTerm(Ctx& c, int nB, const double* icX, const double* icY, const double* icZ, int k,
+                            double* out)
+{
+    if (nB == 0) return;
+    boundaryDiagTermK<<<blocks(nB), kTpb, 0, c.stream>>>(nB, icX, icY, icZ, k, out);
+    hipCheck(hipGetLastError(), "boundary diag term");
+}
+
+void launchMatrixH(Ctx& c, const Mesh& m, const double* upper, const double* lower, const double* psi,
+                   const double* source, const double* bdDiag, const double* bdSrc, const double* V,
+                   const double* rAU, double* out)
+{
+    if (m.nCells == 0) return;
+    matrixHK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.ownerStart.p, m.losortStart.p, m.losort.p,
+                                                       m.losortOwner.p, m.neighbour.p, m.bndStart.p, m.bndFaces.p,
+                                                       upper, lower, psi, source, bdDiag, bdSrc, V, rAU, out);
+    hipCheck(hipGetLastError(), "matrix H");
+}
+
+void launchConstrainHbyA(Ctx& c, const Mesh& m, const int* takeU, const double* Ub, const double* HbyA,
+                         double* HbyAb)
+{
+    if (m.nBFaces == 0) return;
+    constrainHbyAK<<<blocks(m.nBFaces), kTpb, 0, c.stream>>>(m.nBFaces, m.bFaceCell.p, takeU, Ub, HbyA, HbyAb);
+    hipCheck(hipGetLastError(), "constrain HbyA");
+}
+
+void launchVectorFlux(Ctx& c, const Mesh& m, const double* w, const double* Sf, const double* Ux, const double* Uy,
+                      const double* Uz, double* phi)
+{
+    if (m.nFaces == 0) return;
+    vectorFluxK<<<blocks(m.nFaces), kTpb, 0, c.stream>>>(m.nFaces, m.owner.p, m.neighbour.p, w, Sf, Ux, Uy, Uz,
+                                                          phi);
+    hipCheck(hipGetLastError(), "vector flux");
+}
+
+void launchBoundaryFlux(Ctx& c, const Mesh& m, const double* SfB, const double* uxb, const double* uyb,
+                        const double* uzb, const int* isEmpty, double* phiB)
+{
+    if (m.nBFaces == 0) return;
+    boundaryFluxK<<<blocks(m.nBFaces), kTpb, 0, c.stream>>>(m.nBFaces, SfB, uxb, uyb, uzb, isEmpty, phiB);
+    hipCheck(hipGetLastError(), "boundary flux");
+}
+
+void launchAdjustPhi(Ctx& c, int nB, const int* adjustable, double* phiB, int nF, const double* phiInt,
+                     double* decision)
+{
+    if (!phiInt) nF = 0;
+    if (c.sequentialReduce)
+        adjustSeqK<<<1, 64, 0, c.stream>>>(nF, phiInt, nB, adjustable, phiB, decision);
+    else
+    {
+        const int g = reduceGrid(nF > nB ? nF : nB);
+        double* pa = c.partials.p;
+        adjustSumsK<<<g, kReduceBlock, 0, c.stream>>>(nF, phiInt, nB, adjustable, phiB, pa, pa + kMaxReduceBlocks,
+                                                      pa + 2 * (size_t)kMaxReduceBlocks,
+                                                      pa + 3 * (size_t)kMaxReduceBlocks);
+        adjustFinishK<<<1, kFinishBlock, 0, c.stream>>>(g, pa, pa + kMaxReduceBlocks, pa + 2 * (size_t)kMaxReduceBlocks,
+                                                        pa + 3 * (size_t)kMaxReduceBlocks, decision);
+    }
+    if (nB) adjustScaleK<<<blocks(nB), kTpb, 0, c.stream>>>(nB, adjustable, decision, phiB);
+    hipCheck(hipGetLastError(), "adjust phi");
+}
+
+} // namespace mfv

@@ -207,6 +207,25 @@ void launchDivSource(Ctx& c, const Mesh& m, const double* phi, const double* phi
                      double* source);
 void launchFluxCorrect(Ctx& c, const Mesh& m, const LduView& A, const double* psi, const double* ffc,
                        const double* phiHbyA, double sign, double* phi);
+// ---- the pressure predictor's momentum glue (kernels_predictor.hip) ----
+void launchZero(Ctx& c, int n, double* x);
+void launchFoldBoundaryDiag(Ctx& c, const Mesh& m, const double* icX, const double* icY, const double* icZ,
+                            const double* diag, double* D);
+void launchReciprocalV(Ctx& c, const Mesh& m, const double* D, const double* V, double* rAU);
+void launchBoundaryDiagTerm(Ctx& c, int nB, const double* icX, const double* icY, const double* icZ, int k,
+                            double* out);
+void launchMatrixH(Ctx& c, const Mesh& m, const double* upper, const double* lower, const double* psi,
+                   const double* source, const double* bdDiag, const double* bdSrc, const double* V,
+                   const double* rAU, double* out);
+void launchConstrainHbyA(Ctx& c, const Mesh& m, const int* takeU, const double* Ub, const double* HbyA,
+                         double* HbyAb);
+void launchVectorFlux(Ctx& c, const Mesh& m, const double* w, const double* Sf, const double* Ux, const double* Uy,
+                      const double* Uz, double* phi);
+void launchBoundaryFlux(Ctx& c, const Mesh& m, const double* SfB, const double* uxb, const double* uyb,
+                        const double* uzb, const int* isEmpty, double* phiB);
+// decision: 3 device doubles {massCorr, refused, closedVolume}
+void launchAdjustPhi(Ctx& c, int nB, const int* adjustable, double* phiB, int nF, const double* phiInt,
+                     double* decision);
 void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcValue, const double* Sf,
                      const double* w, const double* V, const double* psi, double* grad);
 // limitCoeff in (0, 1): `limited k corrected` against nonOrthDc*(vf[nei] - vf[own]); otherwise unlimited

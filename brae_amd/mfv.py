@@ -129,6 +129,14 @@ _SIGS = {
     "mfv_amg_num_grids": [P, P],
     "mfv_amg_grid_size": [P, I, P, P],
     "mfv_amg_fine_stencil": [P, P, P],
+    "mfv_fold_boundary_diag": [P, P, P, P, P, P, P],
+    "mfv_reciprocal_v": [P, P, P, P, P],
+    "mfv_boundary_diag_term": [P, P, P, P, P, I, P],
+    "mfv_matrix_h": [P, P, P, P, P, P, P, P, P, I, P],
+    "mfv_constrain_hbya": [P, P, P, P, P, P],
+    "mfv_vector_flux": [P, P, P, P, P, P, P, P],
+    "mfv_boundary_flux": [P, P, P, P, P, P, P, P],
+    "mfv_adjust_phi": [P, I, P, P, I, P, P, P],
     "mfv_amg_level_addressing": [P, I, P, P, P, P, P],
     "mfv_amg_galerkin": [P, P, P, P, P, I],
     "mfv_amg_grid_values": [P, P, I, P, P],

@@ -315,6 +315,114 @@ inline void deviceMatrixFluxInternal(const DeviceLduView& A, const DeviceBuffer<
     mfvCheck(mfv_face_flux(ctx, A.mesh, &l, p.data(), fluxInt.data()), ctx, "deviceMatrixFluxInternal");
 }
 
+// ---- the SIMPLE pressure predictor's momentum glue (pressurePredictor, pEqn.cu:94-335) ----
+
+// A()'s numerator (pEqn.cu:117-138): D = diag + cmptAv(internalCoeffs), iC[3] the momentum matrix's
+// internalCoeffs per boundary face and component
+inline void deviceFoldBoundaryDiag(const DeviceMesh& dm, const DeviceBuffer<scalar>& diag,
+                                   const DeviceBuffer<scalar>* iC, DeviceBuffer<scalar>& D)
+{
+    mfv_ctx ctx = threadContext();
+    D.resize(dm.nCells);
+    mfvCheck(mfv_fold_boundary_diag(ctx, dm.mesh, diag.data(), iC[0].data(), iC[1].data(), iC[2].data(), D.data()),
+             ctx, "deviceFoldBoundaryDiag");
+}
+
+// deviceReciprocalV (src/cuda/device_simple.cuh:26): rAU = 1/(diagC/V), diagC already folded
+inline void deviceReciprocalV(const DeviceMesh& dm, const DeviceBuffer<scalar>& diagC, DeviceBuffer<scalar>& rAU)
+{
+    mfv_ctx ctx = threadContext();
+    rAU.resize(dm.nCells);
+    mfvCheck(mfv_reciprocal_v(ctx, dm.mesh, diagC.data(), dm.V.data(), rAU.data()), ctx, "deviceReciprocalV");
+}
+
+// bdDiagK = cmptAv(iC) - iC_k (pEqn.cu:146-149)
+inline void deviceBoundaryDiagTerm(const DeviceMesh& dm, const DeviceBuffer<scalar>* iC, int k,
+                                   DeviceBuffer<scalar>& bdDiagK)
+{
+    mfv_ctx ctx = threadContext();
+    bdDiagK.resize(dm.nBoundaryFaces);
+    mfvCheck(mfv_boundary_diag_term(ctx, dm.mesh, iC[0].data(), iC[1].data(), iC[2].data(), k, bdDiagK.data()), ctx,
+             "deviceBoundaryDiagTerm");
+}
+
+// deviceMatrixH (src/cuda/device_simple.cuh:14-23): H = (source + lduMatrix::H(psi) + bdDiag*psi + bdSrc)/V;
+// valid = false: a component the mesh's empty patches knock out (zeros)
+inline void deviceMatrixH(const DeviceLduView& A, const DeviceMesh& dm, const DeviceBuffer<scalar>& psiK,
+                          const DeviceBuffer<scalar>& sourceK, const DeviceBuffer<scalar>& bdDiagK,
+                          const DeviceBuffer<scalar>& bdSrcK, DeviceBuffer<scalar>& Hk, bool valid = true)
+{
+    mfv_ctx ctx = threadContext();
+    Hk.resize(dm.nCells);
+    const mfv_ldu l = A.ldu();
+    mfvCheck(mfv_matrix_h(ctx, dm.mesh, &l, psiK.data(), sourceK.data(), bdDiagK.data(), bdSrcK.data(), dm.V.data(),
+                          nullptr, valid ? 1 : 0, Hk.data()),
+             ctx, "deviceMatrixH");
+}
+
+// HbyA_k = rAU*H_k in one pass (pEqn.cu:150-185: deviceMatrixH, then deviceHadamard)
+inline void deviceMatrixHbyA(const DeviceLduView& A, const DeviceMesh& dm, const DeviceBuffer<scalar>& psiK,
+                             const DeviceBuffer<scalar>& sourceK, const DeviceBuffer<scalar>& bdDiagK,
+                             const DeviceBuffer<scalar>& bdSrcK, const DeviceBuffer<scalar>& rAU,
+                             DeviceBuffer<scalar>& HbyAk, bool valid = true)
+{
+    mfv_ctx ctx = threadContext();
+    HbyAk.resize(dm.nCells);
+    const mfv_ldu l = A.ldu();
+    mfvCheck(mfv_matrix_h(ctx, dm.mesh, &l, psiK.data(), sourceK.data(), bdDiagK.data(), bdSrcK.data(), dm.V.data(),
+                          rAU.data(), valid ? 1 : 0, HbyAk.data()),
+             ctx, "deviceMatrixHbyA");
+}
+
+// constrainHbyA (pEqn.cu:187-203): HbyAb = Ub where takeU (U not assignable), the owner cell's HbyA elsewhere
+inline void deviceConstrainHbyA(const DeviceMesh& dm, const DeviceBuffer<int>& takeU, const DeviceBuffer<scalar>& Ub,
+                                const DeviceBuffer<scalar>& HbyA, DeviceBuffer<scalar>& HbyAb)
+{
+    mfv_ctx ctx = threadContext();
+    HbyAb.resize(dm.nBoundaryFaces);
+    mfvCheck(mfv_constrain_hbya(ctx, dm.mesh, takeU.data(), Ub.data(), HbyA.data(), HbyAb.data()), ctx,
+             "deviceConstrainHbyA");
+}
+
+// deviceVectorFlux (src/cuda/device_simple.cuh:48): phiInt = interpolate(U) . Sf
+inline void deviceVectorFlux(const DeviceMesh& dm, const DeviceBuffer<scalar>& Ux, const DeviceBuffer<scalar>& Uy,
+                             const DeviceBuffer<scalar>& Uz, DeviceBuffer<scalar>& phiInt)
+{
+    mfv_ctx ctx = threadContext();
+    phiInt.resize(dm.nInternalFaces);
+    mfvCheck(mfv_vector_flux(ctx, dm.mesh, dm.weights.data(), dm.Sf.data(), Ux.data(), Uy.data(), Uz.data(),
+                             phiInt.data()),
+             ctx, "deviceVectorFlux");
+}
+
+// deviceBoundaryFlux (src/cuda/device_simple.cuh:59): phiB = Ub . Sf_b (zero on the faces isEmpty marks)
+inline void deviceBoundaryFlux(const DeviceMesh& dm, const DeviceBuffer<scalar>& uxb, const DeviceBuffer<scalar>& uyb,
+                               const DeviceBuffer<scalar>& uzb, DeviceBuffer<scalar>& phiB,
+                               const DeviceBuffer<int>* isEmpty = nullptr)
+{
+    mfv_ctx ctx = threadContext();
+    phiB.resize(dm.nBoundaryFaces);
+    mfvCheck(mfv_boundary_flux(ctx, dm.mesh, dm.Sf.data(), uxb.data(), uyb.data(), uzb.data(),
+                               isEmpty ? isEmpty->data() : nullptr, phiB.data()),
+             ctx, "deviceBoundaryFlux");
+}
+
+// deviceAdjustPhi (src/cuda/device_simple.cuh:99-102): the adjustable outflow of phiB scaled in place; returns
+// massCorr; throws when the continuity error cannot be removed (adjustPhi.C's FatalError)
+inline scalar deviceAdjustPhi(const DeviceBuffer<label>& adjustable, DeviceBuffer<scalar>& phiB,
+                              const DeviceBuffer<scalar>* phiInt = nullptr, bool* closedVolume = nullptr)
+{
+    mfv_ctx ctx = threadContext();
+    double mc = 1.0;
+    int closed = 0;
+    mfvCheck(mfv_adjust_phi(ctx, static_cast<int>(phiB.size()), adjustable.data(), phiB.data(),
+                            phiInt ? static_cast<int>(phiInt->size()) : 0, phiInt ? phiInt->data() : nullptr, &mc,
+                            &closed),
+             ctx, "deviceAdjustPhi");
+    if (closedVolume) *closedVolume = closed != 0;
+    return mc;
+}
+
 // fvc::gaussGrad with the patch values of a field whose patches are fixedValue (bcType 1, value
 // bcValue) or zeroGradient (0) or empty (2); grad is xyz-interleaved (3*nCells)
 inline void deviceGaussGrad(const DeviceMesh& dm, const DeviceBuffer<scalar>& vol, const DeviceBuffer<int>& bcType,

@@ -128,6 +128,49 @@ int mfv_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double*
  * correction); sign -1 when A is the negated (positive-definite) pressure matrix */
 int mfv_flux_correct(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dFfc,
                      const double* dPhiHbyA, double sign, double* dPhi);
+/* ---------------------------------------------------------------------------------------------
+ * The SIMPLE pressure predictor's momentum glue (pressurePredictor,
+ * src/applications/solvers/simpleFoam/pEqn.cu:94-335; host twin pEqn_cpp.cu:40-133): rAU, H() per
+ * component and HbyA, constrainHbyA, phiHbyA and adjustPhi, for uncoupled patches (no SIMPLEC, MRF or
+ * cyclic pair). The momentum matrix comes as an mfv_ldu (upper/lower) plus, per boundary face and
+ * component, its internalCoeffs (dIcX/Y/Z) and boundaryCoeffs. Boundary arrays in boundary-face order.
+ * ------------------------------------------------------------------------------------------- */
+/* A()'s numerator (pEqn.cu:117-138, cmptAvKernel + foldBoundaryDiagKernel; fv_matrix_ops.cuh:211-226):
+ * D = diag + cmptAv(internalCoeffs) face by face, cmptAv = (x + y + z)/3; dDiag the (relaxed) diagonal */
+int mfv_fold_boundary_diag(mfv_ctx ctx, mfv_mesh mesh, const double* dDiag, const double* dIcX,
+                           const double* dIcY, const double* dIcZ, double* dD);
+/* deviceReciprocalV (src/cuda/device_simple.cuh:26): rAU = 1/(D/V) */
+int mfv_reciprocal_v(mfv_ctx ctx, mfv_mesh mesh, const double* dD, const double* dV, double* dRAU);
+/* H()'s boundary-diagonal factor of component k (pEqn.cu:146-149): cmptAv(ic) - ic_k per boundary face */
+int mfv_boundary_diag_term(mfv_ctx ctx, mfv_mesh mesh, const double* dIcX, const double* dIcY, const double* dIcZ,
+                           int k, double* dBdDiag);
+/* deviceMatrixH (src/cuda/device_simple.cuh:14-23) of one component: H = (bdDiag*psi + lduMatrix::H(psi) +
+ * source + bdSrc)/V in the host loop's order (fv_matrix_ops.cuh:277-340). A's upper and lower are read.
+ * dRAU (may be NULL): the output is HbyA = rAU*H (pEqn.cu:185, deviceHadamard). valid = 0: a component
+ * the mesh's empty patches knock out (fvMatrix::H's validComponents): zeros. */
+int mfv_matrix_h(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dSource,
+                 const double* dBdDiag, const double* dBdSrc, const double* dV, const double* dRAU, int valid,
+                 double* dOut);
+/* constrainHbyA (pEqn.cu:187-203, constrainHbyAKernel): HbyAb = Ub where dTakeU is set (U not assignable),
+ * the owner cell's HbyA elsewhere; one component */
+int mfv_constrain_hbya(mfv_ctx ctx, mfv_mesh mesh, const int* dTakeU, const double* dUb, const double* dHbyA,
+                       double* dHbyAb);
+/* deviceVectorFlux (src/cuda/device_simple.cuh:48): phi = (w U_own + (1-w) U_nei) . Sf per internal face;
+ * dSf xyz per face, internal faces first (as mfv_mesh_geometry writes it) */
+int mfv_vector_flux(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dSf, const double* dUx,
+                    const double* dUy, const double* dUz, double* dPhi);
+/* deviceBoundaryFlux (src/cuda/device_simple.cuh:59): phiB = Ub . Sf_b per boundary face, zero where dIsEmpty
+ * (may be NULL) is set; dSf the same whole-mesh array as mfv_vector_flux's */
+int mfv_boundary_flux(mfv_ctx ctx, mfv_mesh mesh, const double* dSf, const double* dUxb, const double* dUyb,
+                      const double* dUzb, const int* dIsEmpty, double* dPhiB);
+/* deviceAdjustPhi (src/cuda/device_simple.cuh:99; adjustPhi, pEqn_cpp.cu:80-133): scales the adjustable
+ * (dAdjustable) outflow of dPhiB [nBFaces] in place so the boundary flux balances; dPhiInt [nInternalFaces]
+ * (NULL with 0) is the internal flux of the normaliser 1e-300 + sum|phi|. massCorr/closedVolume may be NULL.
+ * Synchronises. MFV_ERR_INVALID when the continuity error cannot be removed (OpenFOAM's FatalError; dPhiB
+ * untouched). The sums are the context's reductions (tree, or sequential in the host's order). */
+int mfv_adjust_phi(mfv_ctx ctx, int nBFaces, const int* dAdjustable, double* dPhiB, int nInternalFaces,
+                   const double* dPhiInt, double* massCorr, int* closedVolume);
+
 /* fvc::gaussGrad of a cell field with its patch values (src/finiteVolume/finiteVolume/fvc.cu:67-121):
  * dSf xyz per face (internal then boundary, as mfv_mesh_geometry writes it), dWeights per internal face,
  * patch values from dBcType/dBcValue (fixedValue: the value; zeroGradient: the cell's; empty: none);

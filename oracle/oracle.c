@@ -2062,3 +2062,114 @@ orc_perf orc_brae_amgpcg(const orc_aggl* a, const double* diag, const double* up
     free(pA); free(wA); free(rA); free(rOld);
     return perf;
 }
+
+/* ---------------------------------------------------------------------------------------------
+ * The SIMPLE pressure predictor's momentum glue: the host loops of pEqn_cpp.cu:40-133 and
+ * fv_matrix_ops.cuh:211-340, as written (no contraction).
+ * ------------------------------------------------------------------------------------------- */
+
+void orc_fold_boundary_diag(int nCells, int nBFaces, const int* bFaceCell, const double* icX, const double* icY,
+                            const double* icZ, const double* diag, double* D)
+{
+    memcpy(D, diag, sizeof(double) * (size_t)nCells);
+    for (int i = 0; i < nBFaces; ++i) D[bFaceCell[i]] += (icX[i] + icY[i] + icZ[i]) / 3.0;
+}
+
+void orc_reciprocal_v(int nCells, const double* D, const double* V, double* rAU)
+{
+    for (int c = 0; c < nCells; ++c)
+    {
+        const double A = D[c] / V[c];
+        rAU[c] = 1.0 / A;
+    }
+}
+
+void orc_boundary_diag_term(int nBFaces, const double* icX, const double* icY, const double* icZ, int k,
+                            double* out)
+{
+    const double* ic = k == 0 ? icX : (k == 1 ? icY : icZ);
+    for (int i = 0; i < nBFaces; ++i)
+    {
+        const double av = (icX[i] + icY[i] + icZ[i]) / 3.0;
+        out[i] = av - ic[i];
+    }
+}
+
+void orc_matrix_h(int nCells, int nFaces, const int* owner, const int* neighbour, const double* upper,
+                  const double* lower, const double* psi, const double* source, int nBFaces, const int* bFaceCell,
+                  const double* bdDiag, const double* bdSrc, const double* V, double* H)
+{
+    memset(H, 0, sizeof(double) * (size_t)nCells);
+    for (int i = 0; i < nBFaces; ++i)
+    {
+        const int c = bFaceCell[i];
+        H[c] += bdDiag[i] * psi[c];
+    }
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const int o = owner[f], n = neighbour[f];
+        H[n] = H[n] - lower[f] * psi[o];
+        H[o] = H[o] - upper[f] * psi[n];
+    }
+    for (int c = 0; c < nCells; ++c) H[c] += source[c];
+    for (int i = 0; i < nBFaces; ++i) H[bFaceCell[i]] += bdSrc[i];
+    for (int c = 0; c < nCells; ++c) H[c] = H[c] / V[c];
+}
+
+void orc_vector_flux(int nFaces, const int* owner, const int* neighbour, const double* weights, const double* Sf,
+                     const double* Ux, const double* Uy, const double* Uz, double* phi)
+{
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const int o = owner[f], n = neighbour[f];
+        const double w = weights[f];
+        const double ux = w * Ux[o] + (1.0 - w) * Ux[n];
+        const double uy = w * Uy[o] + (1.0 - w) * Uy[n];
+        const double uz = w * Uz[o] + (1.0 - w) * Uz[n];
+        phi[f] = ux * Sf[3 * (size_t)f] + uy * Sf[3 * (size_t)f + 1] + uz * Sf[3 * (size_t)f + 2];
+    }
+}
+
+void orc_boundary_flux(int nBFaces, const double* SfB, const double* uxb, const double* uyb, const double* uzb,
+                       const int* isEmpty, double* phiB)
+{
+    for (int i = 0; i < nBFaces; ++i)
+        phiB[i] = (isEmpty && isEmpty[i]) ? 0.0
+                                          : uxb[i] * SfB[3 * (size_t)i] + uyb[i] * SfB[3 * (size_t)i + 1] +
+                                                uzb[i] * SfB[3 * (size_t)i + 2];
+}
+
+int orc_adjust_phi(int nFaces, const double* phiInt, int nBFaces, const int* adjustable, double* phiB,
+                   double* massCorr, int* closedVolume)
+{
+    const double vsmall = 1e-300, small = 1e-15;
+    double massIn = 0.0, fixedOut = 0.0, adjOut = 0.0, totalFlux = vsmall;
+    for (int f = 0; f < nFaces; ++f) totalFlux += fabs(phiInt[f]);
+    for (int i = 0; i < nBFaces; ++i)
+    {
+        const double v = phiB[i];
+        if (v < 0.0)
+            massIn -= v;
+        else if (!adjustable[i])
+            fixedOut += v;
+        else
+            adjOut += v;
+    }
+    double mc = 1.0;
+    const double magAdj = fabs(adjOut);
+    if (closedVolume)
+        *closedVolume = fabs(massIn) / totalFlux < small && fabs(fixedOut) / totalFlux < small &&
+                        fabs(adjOut) / totalFlux < small;
+    if (magAdj > vsmall && magAdj / totalFlux > small)
+        mc = (massIn - fixedOut) / adjOut;
+    else if (fabs(fixedOut - massIn) / totalFlux > 1e-8)
+    {
+        if (massCorr) *massCorr = 1.0;
+        return 1;
+    }
+    for (int i = 0; i < nBFaces; ++i)
+        if (adjustable[i] && phiB[i] > 0.0) phiB[i] *= mc;
+    if (massCorr) *massCorr = mc;
+    return 0;
+}
+

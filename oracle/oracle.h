@@ -179,6 +179,37 @@ void orc_dic_precondition(int nCells, int nFaces, const int* owner, const int* n
 void orc_brae_vcycle(const orc_aggl* a, const double* diag, const double* upper, const double* r, double* z,
                      int nPre, int nPost, int single, int scale);
 
+/* ---- The SIMPLE pressure predictor's momentum glue (src/applications/solvers/simpleFoam/pEqn_cpp.cu:40-133):
+ * rAU = 1/A, H() per component and HbyA = rAU*H, constrainHbyA, phiHbyA = fvc::flux(HbyA), adjustPhi.
+ * Boundary arrays per boundary face in face (patch) order; Sf xyz-interleaved. ---- */
+/* A()'s numerator (fv_matrix_ops.cuh:211-226, matrixA): D = diag + cmptAv(internalCoeffs) face by face,
+ * cmptAv = (x + y + z)/3 */
+void orc_fold_boundary_diag(int nCells, int nBFaces, const int* bFaceCell, const double* icX, const double* icY,
+                            const double* icZ, const double* diag, double* D);
+/* rAU = 1/A, A = D/V (fv_matrix_ops.cuh:224-225; pEqn_cpp.cu:44-47) */
+void orc_reciprocal_v(int nCells, const double* D, const double* V, double* rAU);
+/* H()'s boundary-diagonal factor of component k: cmptAv(ic) - ic_k (fv_matrix_ops.cuh:290-300) */
+void orc_boundary_diag_term(int nBFaces, const double* icX, const double* icY, const double* icZ, int k,
+                            double* out);
+/* H() of one component (fv_matrix_ops.cuh:277-340, matrixH; uncoupled patches): from zero, += bdDiag*psi per
+ * boundary face; per internal face in order H[nei] -= lower*psi[own], H[own] -= upper*psi[nei]; += source;
+ * += boundaryCoeffs per boundary face; /= V */
+void orc_matrix_h(int nCells, int nFaces, const int* owner, const int* neighbour, const double* upper,
+                  const double* lower, const double* psi, const double* source, int nBFaces, const int* bFaceCell,
+                  const double* bdDiag, const double* bdSrc, const double* V, double* H);
+/* fvc::flux of a cell vector field (fvc.cu:545-580): internal (w*U_own + (1-w)*U_nei) . Sf */
+void orc_vector_flux(int nFaces, const int* owner, const int* neighbour, const double* weights, const double* Sf,
+                     const double* Ux, const double* Uy, const double* Uz, double* phi);
+/* ...and its boundary faces: Ub . Sf_b, zero on an empty face (SfB: the boundary faces' Sf) */
+void orc_boundary_flux(int nBFaces, const double* SfB, const double* uxb, const double* uyb, const double* uzb,
+                       const int* isEmpty, double* phiB);
+/* adjustPhi (pEqn_cpp.cu:80-133): totalFlux = 1e-300 + sum|phi| over internal faces; massIn, fixed and adjustable
+ * outflow over the boundary; massCorr = (massIn - fixedOut)/adjOut when adjOut is not negligible, and the
+ * adjustable outflow scaled by it. Returns 0, or 1 when the continuity error cannot be removed (phiB
+ * untouched). closedVolume (nullable): all three negligible against totalFlux (device_simple.cu:282-346). */
+int orc_adjust_phi(int nFaces, const double* phiInt, int nBFaces, const int* adjustable, double* phiB,
+                   double* massCorr, int* closedVolume);
+
 #ifdef __cplusplus
 }
 #endif

@@ -179,16 +179,80 @@ int main(int argc, char** argv)
         gamgSolveFolded(gamg, diagC, upper, b, t, x);
     });
 
+    // the pressure predictor's glue on an upwind momentum matrix: H()'s defining identity
+    // V*H = source + bdDiag*psi + bdSrc - (off-diagonal A) psi against deviceAmul, the flux of a uniform
+    // field (U . Sf), and adjustPhi closing continuity
+    std::vector<scalar> hPhi(nI);
+    for (auto& v : hPhi) v = g(rng);
+    DeviceBuffer<scalar> phiM(hPhi), mDiag, mUpper, mLower;
+    deviceDivUpwindCoeffs(dm, phiM, mDiag, mUpper, mLower);
+    const DeviceLduView M = lduView(dm, mDiag, mUpper, &mLower);
+    DeviceBuffer<scalar> iCs[3] = {DeviceBuffer<scalar>(std::vector<scalar>(nB, 0.3)),
+                                   DeviceBuffer<scalar>(std::vector<scalar>(nB, 0.5)),
+                                   DeviceBuffer<scalar>(std::vector<scalar>(nB, 0.7))};
+    DeviceBuffer<scalar> Dm, rAU, bd, Hk, HbyAk, zeroDiag(std::vector<scalar>(nC, 0.0)), offPsi;
+    deviceFoldBoundaryDiag(dm, mDiag, iCs, Dm);
+    deviceReciprocalV(dm, Dm, rAU);
+    deviceBoundaryDiagTerm(dm, iCs, 1, bd);
+    std::vector<scalar> hU(nC), hBc(nB);
+    for (auto& v : hU) v = g(rng);
+    for (auto& v : hBc) v = g(rng);
+    DeviceBuffer<scalar> Uk(hU), bcK(hBc);
+    deviceMatrixH(M, dm, Uk, source, bd, bcK, Hk);
+    deviceMatrixHbyA(M, dm, Uk, source, bd, bcK, rAU, HbyAk);
+    deviceAmul(lduView(dm, zeroDiag, mUpper, &mLower), Uk, offPsi);
+    const std::vector<scalar> hH = Hk.download(), hHb = HbyAk.download(), hOff = offPsi.download(),
+                              hV = dm.V.download(), hR = rAU.download(), hBd = bd.download();
+    std::vector<scalar> rhs(src);
+    for (int i = 0; i < nB; ++i) rhs[box.owner[nI + i]] += hBd[i] * hU[box.owner[nI + i]] + hBc[i];
+    double hErr = 0.0, hScale = 0.0, hbErr = 0.0;
+    for (int c = 0; c < nC; ++c)
+    {
+        hErr = std::max(hErr, std::fabs(hV[c] * hH[c] - (rhs[c] - hOff[c])));
+        hScale = std::max(hScale, std::fabs(rhs[c]) + std::fabs(hOff[c]));
+        hbErr = std::max(hbErr, std::fabs(hHb[c] - hR[c] * hH[c]));
+    }
+    DeviceBuffer<scalar> u1(std::vector<scalar>(nC, 0.25)), u2(std::vector<scalar>(nC, -0.5)),
+        u3(std::vector<scalar>(nC, 1.0)), phiU;
+    deviceVectorFlux(dm, u1, u2, u3, phiU);
+    const std::vector<scalar> hPhiU = phiU.download();
+    double fluxErr = 0.0;
+    for (int f = 0; f < nI; ++f)
+        fluxErr = std::max(fluxErr, std::fabs(hPhiU[f] - (0.25 * SfAll[3 * f] - 0.5 * SfAll[3 * f + 1] +
+                                                          SfAll[3 * f + 2])));
+    std::vector<scalar> hPb(nB);
+    std::vector<label> hAdj(nB);
+    for (int i = 0; i < nB; ++i)
+    {
+        hPb[i] = g(rng);
+        hAdj[i] = (i % 3 == 0) ? 1 : 0;
+    }
+    DeviceBuffer<scalar> phiB(hPb);
+    DeviceBuffer<label> adjustable(hAdj);
+    bool closedVolume = true;
+    deviceAdjustPhi(adjustable, phiB, &phiU, &closedVolume);
+    const std::vector<scalar> hPa = phiB.download();
+    double net = 0.0, tot = 0.0;
+    for (int i = 0; i < nB; ++i)
+    {
+        net += hPa[i];
+        tot += std::fabs(hPb[i]);
+    }
+    const bool predictorOk =
+        hErr <= 1e-12 * hScale && hbErr == 0.0 && fluxErr <= 1e-14 && std::fabs(net) <= 1e-12 * tot && !closedVolume;
+
     const bool ok = perf.nIterations > 0 && perf.nIterations < 500 && perf.finalResidual < 1e-8 && res < 2e-8 &&
+                    predictorOk &&
                     refusesSA && refusesAsym && flux.size() == (size_t)nI && pg.converged && pg.finalResidual < 1e-8 &&
                     pp.converged && pp.finalResidual < 1e-8 && refusesSmoother;
     std::printf("{\"cells\": %d, \"levels\": %d, \"iterations\": %d, \"initial\": %.6e, \"final\": %.6e, "
                 "\"recomputed\": %.6e, \"refuses_sa\": %s, \"refuses_asym\": %s, \"gamg_levels\": %d, "
                 "\"gamg_cycles\": %d, \"gamg_final\": %.6e, \"pcg_gamg_iterations\": %d, \"pcg_gamg_final\": %.6e, "
-                "\"refuses_smoother\": %s, \"ok\": %s}\n",
+                "\"refuses_smoother\": %s, \"predictor_h_err\": %.3e, \"predictor_flux_err\": %.3e, "
+                "\"predictor_net_flux\": %.3e, \"ok\": %s}\n",
                 nC, amg.nLevels(), perf.nIterations, perf.initialResidual, perf.finalResidual, res,
                 refusesSA ? "true" : "false", refusesAsym ? "true" : "false", gamg.nLevels(), pg.nIterations,
                 pg.finalResidual, pp.nIterations, pp.finalResidual, refusesSmoother ? "true" : "false",
-                ok ? "true" : "false");
+                hErr / (hScale > 0.0 ? hScale : 1.0), fluxErr, net, ok ? "true" : "false");
     return ok ? 0 : 1;
 }

@@ -57,6 +57,13 @@ def lib():
             "orc_sum_mag": (D, [I, P]),
             "orc_sum_prod": (D, [I, P, P]),
             "orc_face_flux": (None, [I, P, P, P, P, P, P]),
+            "orc_fold_boundary_diag": (None, [I, I, P, P, P, P, P, P]),
+            "orc_reciprocal_v": (None, [I, P, P, P]),
+            "orc_boundary_diag_term": (None, [I, P, P, P, I, P]),
+            "orc_matrix_h": (None, [I, I, P, P, P, P, P, P, I, P, P, P, P, P]),
+            "orc_vector_flux": (None, [I, P, P, P, P, P, P, P, P]),
+            "orc_boundary_flux": (None, [I, P, P, P, P, P, P]),
+            "orc_adjust_phi": (I, [I, P, I, P, P, P, P]),
             "orc_grad_gauss": (None, [I, I, P, P, I, P, P, P, P, P, P, P, P]),
             "orc_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
             "orc_sym_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
@@ -225,6 +232,54 @@ def flux_correct(own, nei, upper, lower, psi, ffc, phiHbyA):
     lib().orc_flux_correct(len(own), p(i32(own)), p(i32(nei)), p(f64(upper)), p(None if lower is None else f64(lower)),
                            p(f64(psi)), p(None if ffc is None else f64(ffc)), p(f64(phiHbyA)), p(phi))
     return phi
+
+
+def fold_boundary_diag(nC, bfc, icX, icY, icZ, diag):
+    D = np.zeros(nC)
+    lib().orc_fold_boundary_diag(nC, len(bfc), p(i32(bfc)), p(f64(icX)), p(f64(icY)), p(f64(icZ)), p(f64(diag)), p(D))
+    return D
+
+
+def reciprocal_v(D, V):
+    r = np.zeros(len(D))
+    lib().orc_reciprocal_v(len(D), p(f64(D)), p(f64(V)), p(r))
+    return r
+
+
+def boundary_diag_term(icX, icY, icZ, k):
+    out = np.zeros(len(icX))
+    lib().orc_boundary_diag_term(len(icX), p(f64(icX)), p(f64(icY)), p(f64(icZ)), int(k), p(out))
+    return out
+
+
+def matrix_h(nC, own, nei, upper, lower, psi, source, bfc, bdDiag, bdSrc, V):
+    H = np.zeros(nC)
+    lib().orc_matrix_h(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(upper)), p(f64(lower)), p(f64(psi)),
+                       p(f64(source)), len(bfc), p(i32(bfc)), p(f64(bdDiag)), p(f64(bdSrc)), p(f64(V)), p(H))
+    return H
+
+
+def vector_flux(own, nei, weights, Sf, Ux, Uy, Uz):
+    phi = np.zeros(len(own))
+    lib().orc_vector_flux(len(own), p(i32(own)), p(i32(nei)), p(f64(weights)), p(f64(Sf)), p(f64(Ux)), p(f64(Uy)),
+                          p(f64(Uz)), p(phi))
+    return phi
+
+
+def boundary_flux(SfB, uxb, uyb, uzb, isEmpty=None):
+    phiB = np.zeros(len(uxb))
+    lib().orc_boundary_flux(len(uxb), p(f64(SfB)), p(f64(uxb)), p(f64(uyb)), p(f64(uzb)),
+                            p(None if isEmpty is None else i32(isEmpty)), p(phiB))
+    return phiB
+
+
+def adjust_phi(phiInt, adjustable, phiB):
+    """(status, massCorr, closedVolume, adjusted phiB): status 1 = continuity error not removable"""
+    out = f64(phiB).copy()
+    mc, closed = ctypes.c_double(), ctypes.c_int()
+    rc = lib().orc_adjust_phi(len(phiInt), p(f64(phiInt)), len(out), p(i32(adjustable)), p(out),
+                              ctypes.byref(mc), ctypes.byref(closed))
+    return rc, mc.value, bool(closed.value), out
 
 
 def gauss_seidel(nC, own, nei, diag, upper, b, psi, nSweeps, symmetric=False):

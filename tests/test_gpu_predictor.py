@@ -1,0 +1,150 @@
+"""GPU parity of the pressure predictor's momentum glue (rAU, H(), HbyA, constrainHbyA, phiHbyA,
+adjustPhi) against the oracle: bit-exact, the adjustPhi sums in the context's sequential mode; with the
+default tree sums the mass correction agrees to rounding and continuity closes."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import gpu_util as G
+import oracle_lib as O
+from brae_amd import hexmesh, mfv
+from brae_amd.mfv import ptr
+
+pytestmark = pytest.mark.gpu
+
+
+class Case:
+    """An asymmetric momentum matrix (upwind convection + laplacian) on a jittered hex box, boundary
+    coefficients per face and component, sources, a velocity, host and device copies."""
+
+    def __init__(self, nx, ny, nz, seed):
+        m = hexmesh.box_mesh(nx, ny, nz, length=(2.0, 1.0, 0.5), grading=(1.0, 4.0, 4.0), jitter=0.1, seed=seed)
+        nI, nC = m["nInternal"], m["nCells"]
+        self.m, self.nI, self.nC = m, nI, nC
+        self.own, self.nei, self.bfc = O.i32(m["owner"][:nI]), O.i32(m["neighbour"]), O.i32(m["owner"][nI:])
+        nB = self.nB = len(self.bfc)
+        rng = np.random.default_rng(seed)
+        phi = rng.normal(size=nI) * m["magSf"][:nI]
+        dU, uU, lU = np.zeros(nC), np.zeros(nI), np.zeros(nI)
+        O.lib().orc_div_upwind(nC, nI, O.p(self.own), O.p(self.nei), O.p(O.f64(phi)), O.p(dU), O.p(uU), O.p(lU))
+        dL, uL, lL = np.zeros(nC), np.zeros(nI), np.zeros(nI)
+        O.lib().orc_laplacian(nC, nI, O.p(self.own), O.p(self.nei), O.p(O.f64(m["deltaCoeffs"])),
+                              O.p(np.full(nI, 1e-2)), O.p(O.f64(m["magSf"][:nI])), O.p(dL), O.p(uL), O.p(lL))
+        self.diag = dU - dL + 2.0 * np.abs(uU).max()
+        self.upper, self.lower = uU - uL, lU - lL
+        self.ic = [rng.uniform(0.1, 1.0, nB) for _ in range(3)]
+        self.bc = [rng.normal(size=nB) for _ in range(3)]
+        self.src = [rng.normal(size=nC) * m["V"] for _ in range(3)]
+        self.U = [rng.normal(size=nC) for _ in range(3)]
+        self.Ub = [rng.normal(size=nB) for _ in range(3)]
+        self.takeU = (rng.uniform(size=nB) < 0.4).astype(np.int32)
+        self.V, self.w, self.Sf = O.f64(m["V"]), O.f64(m["weights"]), O.f64(m["Sf"])
+        self.case = G.Case(nC, self.own, self.nei, self.bfc)
+        self.d = {k: G.dev(v) for k, v in dict(diag=self.diag, upper=self.upper, lower=self.lower, V=self.V,
+                                                 w=self.w, Sf=self.Sf.ravel()).items()}
+        self.dic = [G.dev(a) for a in self.ic]
+
+    def call(self, name, *args):
+        self.case.call(name, self.case.mesh.h, *args)
+
+    def ldu(self):
+        return mfv.ldu(self.d["diag"], self.d["upper"], self.d["lower"])
+
+
+@pytest.fixture(scope="module", params=[(9, 7, 5, 3), (61, 37, 23, 11)], ids=["9x7x5", "61x37x23"])
+def case(request):
+    return Case(*request.param)
+
+
+def test_rau_and_hbya_bit_exact(case):
+    C = case
+    print("case:", C.nC, flush=True)
+    D_ref = O.fold_boundary_diag(C.nC, C.bfc, *C.ic, C.diag)
+    rAU_ref = O.reciprocal_v(D_ref, C.V)
+    dD, dR = G.zeros(C.nC), G.zeros(C.nC)
+    C.call("mfv_fold_boundary_diag", ptr(C.d["diag"]), *(ptr(a) for a in C.dic), ptr(dD))
+    C.call("mfv_reciprocal_v", ptr(dD), ptr(C.d["V"]), ptr(dR))
+    assert np.array_equal(G.host(dD), D_ref)
+    assert np.array_equal(G.host(dR), rAU_ref)
+    A = C.ldu()
+    for k in range(3):
+        bd_ref = O.boundary_diag_term(*C.ic, k)
+        dbd = G.zeros(C.nB)
+        C.call("mfv_boundary_diag_term", *(ptr(a) for a in C.dic), k, ptr(dbd))
+        assert np.array_equal(G.host(dbd), bd_ref), k
+        H_ref = O.matrix_h(C.nC, C.own, C.nei, C.upper, C.lower, C.U[k], C.src[k], C.bfc, bd_ref, C.bc[k], C.V)
+        dU, dS, dB, dH, dHb = G.dev(C.U[k]), G.dev(C.src[k]), G.dev(C.bc[k]), G.zeros(C.nC), G.zeros(C.nC)
+        C.call("mfv_matrix_h", ctypes.byref(A), ptr(dU), ptr(dS), ptr(dbd), ptr(dB), ptr(C.d["V"]), None, 1, ptr(dH))
+        C.call("mfv_matrix_h", ctypes.byref(A), ptr(dU), ptr(dS), ptr(dbd), ptr(dB), ptr(C.d["V"]), ptr(dR), 1,
+               ptr(dHb))
+        assert np.array_equal(G.host(dH), H_ref), k
+        assert np.array_equal(G.host(dHb), rAU_ref * H_ref), k
+    # a knocked-out component comes back as zeros
+    dZ = G.dev(np.ones(C.nC))
+    C.call("mfv_matrix_h", ctypes.byref(A), ptr(dU), ptr(dS), ptr(dbd), ptr(dB), ptr(C.d["V"]), ptr(dR), 0, ptr(dZ))
+    assert not G.host(dZ).any()
+
+
+def test_constrain_and_flux_bit_exact(case):
+    C = case
+    print("case:", C.nC, flush=True)
+    HbyA = [np.random.default_rng(20 + k).normal(size=C.nC) for k in range(3)]
+    dHbyA = [G.dev(h) for h in HbyA]
+    dTake = G.dev(C.takeU)
+    Hb = []
+    for k in range(3):
+        ref = np.where(C.takeU != 0, C.Ub[k], HbyA[k][C.bfc])
+        out = G.zeros(C.nB)
+        C.call("mfv_constrain_hbya", ptr(dTake), ptr(G.dev(C.Ub[k])), ptr(dHbyA[k]), ptr(out))
+        assert np.array_equal(G.host(out), ref), k
+        Hb.append(ref)
+    phi_ref = O.vector_flux(C.own, C.nei, C.w, C.Sf[:C.nI], *HbyA)
+    dphi = G.zeros(C.nI)
+    C.call("mfv_vector_flux", ptr(C.d["w"]), ptr(C.d["Sf"]), *(ptr(h) for h in dHbyA), ptr(dphi))
+    assert np.array_equal(G.host(dphi), phi_ref)
+    empty = np.zeros(C.nB, np.int32)
+    empty[::5] = 1
+    for e in (None, empty):
+        phiB_ref = O.boundary_flux(C.Sf[C.nI:], *Hb, isEmpty=e)
+        dpb = G.zeros(C.nB)
+        C.call("mfv_boundary_flux", ptr(C.d["Sf"]), *(ptr(G.dev(h)) for h in Hb),
+               None if e is None else ptr(G.dev(e)), ptr(dpb))
+        assert np.array_equal(G.host(dpb), phiB_ref)
+
+
+def adjust(C, phiInt, adjustable, phiB):
+    dpb = G.dev(phiB)
+    mc, closed = ctypes.c_double(), ctypes.c_int()
+    rc = mfv.lib().mfv_adjust_phi(C.case.ctx.h, len(phiB), ptr(G.dev(adjustable)), ptr(dpb), len(phiInt),
+                                  ptr(G.dev(phiInt)), ctypes.byref(mc), ctypes.byref(closed))
+    return rc, mc.value, bool(closed.value), G.host(dpb)
+
+
+def test_adjust_phi(case):
+    C = case
+    print("case:", C.nC, flush=True)
+    rng = np.random.default_rng(5)
+    phiInt = rng.normal(size=C.nI)
+    phiB = rng.normal(size=C.nB)
+    adj = (rng.uniform(size=C.nB) < 0.3).astype(np.int32)
+    rc_ref, mc_ref, closed_ref, out_ref = O.adjust_phi(phiInt, adj, phiB)
+    assert rc_ref == 0
+    # the host loops' order: bit-exact
+    C.case.ctx.sequential_reduce(True)
+    try:
+        rc, mc, closed, out = adjust(C, phiInt, adj, phiB)
+    finally:
+        C.case.ctx.sequential_reduce(False)
+    assert rc == 0 and mc == mc_ref and closed == closed_ref
+    assert np.array_equal(out, out_ref)
+    # tree sums: the correction to rounding, continuity closed
+    rc, mc, closed, out = adjust(C, phiInt, adj, phiB)
+    assert rc == 0 and abs(mc - mc_ref) <= 1e-12 * abs(mc_ref)
+    assert abs(out.sum()) <= 1e-11 * np.abs(phiB).sum()
+    # no adjustable outflow and an imbalance: refused, the flux untouched
+    none = np.zeros(C.nB, np.int32)
+    assert O.adjust_phi(phiInt, none, phiB)[0] == 1
+    rc, _, _, out = adjust(C, phiInt, none, phiB)
+    assert rc == mfv.MFV_ERR_INVALID and np.array_equal(out, phiB)
+    assert "continuity" in mfv.lib().mfv_last_error(C.case.ctx.h).decode()
