@@ -17,6 +17,7 @@ Inputs are synthetic and seeded: perturbed plug flow U = (10,0,0) + uniform(-1,1
 rAU = 0.05*V^(1/3)/|U|; p: fixedValue 0 at the outlet, zeroGradient elsewhere.
 """
 import argparse
+import collections
 import ctypes
 import json
 import os
@@ -62,6 +63,8 @@ def parse():
     ap.add_argument("--serving", type=int, default=512, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--predictor", type=int, default=1,
+                    help="1: also time the pressure predictor's glue (rAU, HbyA, constrainHbyA, phiHbyA, adjustPhi)")
     ap.add_argument("--gamg-step", type=int, default=0,
                     help="1: also time the step with OpenFOAM's GAMG-PCG on the device (the CPU baseline's solver)")
     return ap.parse_args()
@@ -289,6 +292,8 @@ def gpu_run(args):
         per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1), grids=grids,
         hbm_used_gib=round(hbm_used_gib, 2), other=other_out,
     )
+    if args.predictor:
+        out["predictor"] = predictor_run(ch, ctx, patches=ch.patches)
     if args.gamg_step:
         # the CPU baseline's own algorithm on the device: OpenFOAM's faceAreaPair hierarchy and GAMG-PCG
         Sf = ch.Sf[:3 * ch.nI].cpu().numpy()
@@ -314,6 +319,100 @@ def gpu_run(args):
     ch.mesh.close()
     ctx.close()
     return out
+
+
+def predictor_run(ch, ctx, patches, reps=5):
+    """The SIMPLE pressure predictor's glue (pEqn.cu pressurePredictor: A()'s folded diagonal and rAU,
+    HbyA = rAU*H() per component, constrainHbyA, phiHbyA with its boundary faces, adjustPhi) on the
+    channel, with a synthetic momentum matrix: upwind convection of the channel's flux made diagonally
+    dominant, seeded internalCoeffs/boundaryCoeffs and sources, a seeded velocity. Inlet and walls fix U
+    (constrainHbyA takes U there), the outlet is adjustable. Per kernel: HIP events on the library's
+    stream, algorithmic bytes as for the pressure path (each array once, int32 owner/neighbour credited)."""
+    import torch
+    from brae_amd import mfv
+
+    P, N, F, B = mfv.ptr, ch.nC, ch.nI, ch.nB
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(11)
+
+    def rnd(n, lo=-1.0, hi=1.0):
+        return torch.rand(n, dtype=torch.float64, device="cuda:0", generator=g) * (hi - lo) + lo
+
+    def z(n):
+        return torch.zeros(n, dtype=torch.float64, device="cuda:0")
+
+    mD, mU, mL = z(N), z(F), z(F)
+    ctx.call("mfv_div_upwind_coeffs", ch.mesh.h, P(ch.phiHbyA), P(mD), P(mU), P(mL))
+    mD += 6.0 * mU.abs().max()
+    ic = [rnd(B, 0.1, 1.0) for _ in range(3)]
+    bc = [rnd(B) for _ in range(3)]
+    src = [rnd(N) * ch.V for _ in range(3)]
+    U = [rnd(N) for _ in range(3)]
+    Ub = [rnd(B) for _ in range(3)]
+    takeU = torch.ones(B, dtype=torch.int32, device="cuda:0")
+    adjustable = torch.zeros(B, dtype=torch.int32, device="cuda:0")
+    for name, start, size in patches:
+        if name == "outlet":
+            takeU[start - F:start - F + size] = 0
+            adjustable[start - F:start - F + size] = 1
+    A = mfv.ldu(mD, mU, mL)
+    D, rAU, phi, phiB = z(N), z(N), z(F), z(B)
+    bd = [z(B) for _ in range(3)]
+    HbyA = [z(N) for _ in range(3)]
+    Hb = [z(B) for _ in range(3)]
+    stream = torch.cuda.current_stream()
+    events = collections.defaultdict(list)
+
+    def timed(name, fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        events[name].append((e0, e1))
+
+    mesh = ch.mesh.h
+    massCorr = []
+    for _ in range(reps + 1):
+        timed("foldBoundaryDiagK", lambda: ctx.call("mfv_fold_boundary_diag", mesh, P(mD), *(P(a) for a in ic), P(D)))
+        timed("reciprocalVK", lambda: ctx.call("mfv_reciprocal_v", mesh, P(D), P(ch.V), P(rAU)))
+        for k in range(3):
+            timed("boundaryDiagTermK", lambda: ctx.call("mfv_boundary_diag_term", mesh, *(P(a) for a in ic), k,
+                                                        P(bd[k])))
+        # HbyA of the three components in one pass over the momentum matrix
+        timed("matrixH3K (HbyA, 3 components)",
+              lambda: ctx.call("mfv_matrix_h3", mesh, ctypes.byref(A), mfv.ptr3(U), mfv.ptr3(src), mfv.ptr3(bd),
+                               mfv.ptr3(bc), P(ch.V), P(rAU), None, mfv.ptr3(HbyA)))
+        for k in range(3):
+            timed("constrainHbyAK", lambda: ctx.call("mfv_constrain_hbya", mesh, P(takeU), P(Ub[k]), P(HbyA[k]),
+                                                     P(Hb[k])))
+        timed("vectorFluxK", lambda: ctx.call("mfv_vector_flux", mesh, P(ch.w), P(ch.Sf), *(P(h) for h in HbyA),
+                                              P(phi)))
+        timed("boundaryFluxK", lambda: ctx.call("mfv_boundary_flux", mesh, P(ch.Sf), *(P(h) for h in Hb), None,
+                                                P(phiB)))
+        mc = ctypes.c_double()
+        timed("adjustPhi (sums, decision, scale)",
+              lambda: ctx.call("mfv_adjust_phi", B, P(adjustable), P(phiB), F, P(phi), ctypes.byref(mc), None))
+        massCorr.append(mc.value)
+    torch.cuda.synchronize()
+    alg = {"foldBoundaryDiagK": 16 * N + 4 * N + 24 * B + 4 * B, "reciprocalVK": 24 * N,
+           "boundaryDiagTermK": 32 * B,
+           "matrixH3K (HbyA, 3 components)": 16 * F + 8 * F + 3 * 24 * N + 16 * N + 3 * 16 * B,
+           "constrainHbyAK": 32 * B, "vectorFluxK": 48 * F + 24 * N, "boundaryFluxK": 56 * B,
+           "adjustPhi (sums, decision, scale)": 8 * F + 20 * B}
+    kernels, total_ms = {}, 0.0
+    for name, ev in events.items():
+        per = len(ev) // (reps + 1)
+        ms = [a.elapsed_time(b) for a, b in ev[per:]]   # the first round warms up
+        avg = sum(ms) / len(ms)
+        total_ms += sum(ms) / reps
+        gbs = alg[name] / (avg * 1e-3) / 1e9
+        kernels[name] = {"launches_per_step": per, "avg_us": round(avg * 1e3, 2), "alg_bytes": int(alg[name]),
+                         "alg_GBps": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4)}
+    log("pressure predictor glue: %.3f ms per step (%s)" % (total_ms, ", ".join(
+        "%s %.1f us" % (k, v["avg_us"]) for k, v in kernels.items())))
+    return {"ms_per_step": round(total_ms, 3), "mass_correction": massCorr[-1], "per_kernel": kernels,
+            "workload": "rAU, HbyA (3 components), constrainHbyA, phiHbyA, adjustPhi on the %dx%dx%d channel"
+                        % ch.shape}
 
 
 def serving_run(args):
@@ -513,6 +612,7 @@ def main():
                 "per_grid_us": r["grids"]},
         "other_mixed": r["other"],
         "openfoam_gamg_gpu": r.get("gamg_step"),
+        "pressure_predictor": r.get("predictor"),
         "hbm_used_gib": r["hbm_used_gib"],
         "roofline": r["roofline"],
         "per_kernel": r["per_kernel"],

@@ -935,6 +935,32 @@ int mfv_matrix_h(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPs
     });
 }
 
+int mfv_matrix_h3(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* const* dPsi,
+                  const double* const* dSource, const double* const* dBdDiag, const double* const* dBdSrc,
+                  const double* dV, const double* dRAU, const int* valid, double* const* dOut)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dPsi && dSource && dBdDiag && dBdSrc && dOut && dV, "matrix H3: null argument");
+        bool any = false;
+        for (int k = 0; k < 3; ++k)
+        {
+            need(dOut[k] != nullptr, "matrix H3: null output");
+            if (valid && !valid[k]) continue;
+            any = true;
+            need(dPsi[k] && dSource[k] && (mesh->m.nBFaces == 0 || (dBdDiag[k] && dBdSrc[k])),
+                 "matrix H3: null argument");
+        }
+        if (!any)
+        {
+            for (int k = 0; k < 3; ++k) launchZero(ctx->c, mesh->m.nCells, dOut[k]);
+            return;
+        }
+        const LduView L = view(mesh, A);
+        launchMatrixH3(ctx->c, mesh->m, L.upper, L.lower, dPsi, dSource, dBdDiag, dBdSrc, dV, dRAU, valid, dOut);
+    });
+}
+
 int mfv_constrain_hbya(mfv_ctx ctx, mfv_mesh mesh, const int* dTakeU, const double* dUb, const double* dHbyA,
                        double* dHbyAb)
 {

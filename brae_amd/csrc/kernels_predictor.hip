@@ -89,6 +89,78 @@ __global__ __launch_bounds__(kTpb) void matrixHK(int n, const label* __restrict_
     out[c] = rAU ? rAU[c] * h : h;
 }
 
+// The three components of a vector field at once: one pass over the matrix and the addressing
+struct Vec3Args
+{
+    const double* psi[3];
+    const double* source[3];
+    const double* bdDiag[3];
+    const double* bdSrc[3];
+    double* out[3];
+    int valid[3];
+};
+
+// H() (or HbyA) of the three components in one pass: each component's terms in matrixHK's order, so the
+// values are three matrixHK launches' to the bit; a component that is not valid is written as zeros
+__global__ __launch_bounds__(kTpb) void matrixH3K(int n, const label* __restrict__ ownerStart,
+                                                  const label* __restrict__ losortStart,
+                                                  const label* __restrict__ losort,
+                                                  const label* __restrict__ losortOwner,
+                                                  const label* __restrict__ neighbour,
+                                                  const label* __restrict__ bndStart,
+                                                  const label* __restrict__ bndFaces,
+                                                  const double* __restrict__ upper, const double* __restrict__ lower,
+                                                  Vec3Args a, const double* __restrict__ V,
+                                                  const double* __restrict__ rAU)
+{
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    if (c >= n) return;
+    const int b0 = bndStart[c], b1 = bndStart[c + 1];
+    double h[3], pc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        h[k] = 0.0;
+        pc[k] = a.psi[k][c];
+    }
+    for (int j = b0; j < b1; ++j)
+    {
+        const int i = bndFaces[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) h[k] += a.bdDiag[k][i] * pc[k];
+    }
+    for (int j = losortStart[c]; j < losortStart[c + 1]; ++j)
+    {
+        const double l = lower[losort[j]];
+        const int o = losortOwner[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) h[k] = h[k] - l * a.psi[k][o];
+    }
+    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f)
+    {
+        const double u = upper[f];
+        const int nb = neighbour[f];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) h[k] = h[k] - u * a.psi[k][nb];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h[k] += a.source[k][c];
+    for (int j = b0; j < b1; ++j)
+    {
+        const int i = bndFaces[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) h[k] += a.bdSrc[k][i];
+    }
+    const double v = V[c];
+    const double r = rAU ? rAU[c] : 1.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        const double hk = h[k] / v;
+        a.out[k][c] = !a.valid[k] ? 0.0 : (rAU ? r * hk : hk);
+    }
+}
+
 // constrainHbyA (pEqn.cu constrainHbyAKernel; pEqn_cpp.cu:55-72): U's boundary value on a face whose U
 // is not assignable, the owner cell's HbyA elsewhere
 __global__ __launch_bounds__(kTpb) void constrainHbyAK(int nB, const label* __restrict__ bFaceCell,
@@ -275,6 +347,29 @@ void launchMatrixH(Ctx& c, const Mesh& m, const double* upper, const double* low
                                                        m.losortOwner.p, m.neighbour.p, m.bndStart.p, m.bndFaces.p,
                                                        upper, lower, psi, source, bdDiag, bdSrc, V, rAU, out);
     hipCheck(hipGetLastError(), "matrix H");
+}
+
+void launchMatrixH3(Ctx& c, const Mesh& m, const double* upper, const double* lower, const double* const* psi,
+                    const double* const* source, const double* const* bdDiag, const double* const* bdSrc,
+                    const double* V, const double* rAU, const int* valid, double* const* out)
+{
+    if (m.nCells == 0) return;
+    Vec3Args a{};
+    for (int k = 0; k < 3; ++k)
+    {
+        a.valid[k] = valid ? (valid[k] != 0) : 1;
+        // a knocked-out component reads the first valid one's inputs (in bounds) and writes zeros
+        const int s = a.valid[k] ? k : (a.valid[0] ? 0 : (a.valid[1] ? 1 : 2));
+        a.psi[k] = psi[s];
+        a.source[k] = source[s];
+        a.bdDiag[k] = bdDiag[s];
+        a.bdSrc[k] = bdSrc[s];
+        a.out[k] = out[k];
+    }
+    matrixH3K<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.ownerStart.p, m.losortStart.p, m.losort.p,
+                                                        m.losortOwner.p, m.neighbour.p, m.bndStart.p, m.bndFaces.p,
+                                                        upper, lower, a, V, rAU);
+    hipCheck(hipGetLastError(), "matrix H (three components)");
 }
 
 void launchConstrainHbyA(Ctx& c, const Mesh& m, const int* takeU, const double* Ub, const double* HbyA,

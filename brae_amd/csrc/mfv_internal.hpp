@@ -217,6 +217,10 @@ void launchBoundaryDiagTerm(Ctx& c, int nB, const double* icX, const double* icY
 void launchMatrixH(Ctx& c, const Mesh& m, const double* upper, const double* lower, const double* psi,
                    const double* source, const double* bdDiag, const double* bdSrc, const double* V,
                    const double* rAU, double* out);
+// the three components in one pass (pointers per component; valid may be null: all)
+void launchMatrixH3(Ctx& c, const Mesh& m, const double* upper, const double* lower, const double* const* psi,
+                    const double* const* source, const double* const* bdDiag, const double* const* bdSrc,
+                    const double* V, const double* rAU, const int* valid, double* const* out);
 void launchConstrainHbyA(Ctx& c, const Mesh& m, const int* takeU, const double* Ub, const double* HbyA,
                          double* HbyAb);
 void launchVectorFlux(Ctx& c, const Mesh& m, const double* w, const double* Sf, const double* Ux, const double* Uy,

@@ -133,6 +133,7 @@ _SIGS = {
     "mfv_reciprocal_v": [P, P, P, P, P],
     "mfv_boundary_diag_term": [P, P, P, P, P, I, P],
     "mfv_matrix_h": [P, P, P, P, P, P, P, P, P, I, P],
+    "mfv_matrix_h3": [P, P, P, P, P, P, P, P, P, P, P],
     "mfv_constrain_hbya": [P, P, P, P, P, P],
     "mfv_vector_flux": [P, P, P, P, P, P, P, P],
     "mfv_boundary_flux": [P, P, P, P, P, P, P, P],
@@ -454,3 +455,11 @@ class Amg:
             self.close()
         except Exception:
             pass
+
+
+def ptr3(ts):
+    """a host array of three device pointers (per-component arguments)"""
+    a = (ctypes.c_void_p * 3)(*[ptr(t) for t in ts])
+    _KEEP.append(a)
+    return a
+

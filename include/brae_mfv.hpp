@@ -374,6 +374,35 @@ inline void deviceMatrixHbyA(const DeviceLduView& A, const DeviceMesh& dm, const
              ctx, "deviceMatrixHbyA");
 }
 
+// HbyA of the three components in one pass over the matrix (no counterpart of its own: pEqn.cu:143-185's three
+// deviceMatrixH + deviceHadamard calls); U, source, bdDiag, bdSrc, HbyA: arrays of three; valid may be null
+inline void deviceMatrixHbyA3(const DeviceLduView& A, const DeviceMesh& dm, const DeviceBuffer<scalar>* U,
+                              const DeviceBuffer<scalar>* source, const DeviceBuffer<scalar>* bdDiag,
+                              const DeviceBuffer<scalar>* bdSrc, const DeviceBuffer<scalar>& rAU,
+                              DeviceBuffer<scalar>* HbyA, const bool* valid = nullptr)
+{
+    mfv_ctx ctx = threadContext();
+    const double* psi[3];
+    const double* src[3];
+    const double* bd[3];
+    const double* bs[3];
+    double* out[3];
+    int v[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        HbyA[k].resize(dm.nCells);
+        psi[k] = U[k].data();
+        src[k] = source[k].data();
+        bd[k] = bdDiag[k].data();
+        bs[k] = bdSrc[k].data();
+        out[k] = HbyA[k].data();
+        v[k] = valid ? (valid[k] ? 1 : 0) : 1;
+    }
+    const mfv_ldu l = A.ldu();
+    mfvCheck(mfv_matrix_h3(ctx, dm.mesh, &l, psi, src, bd, bs, dm.V.data(), rAU.data(), v, out), ctx,
+             "deviceMatrixHbyA3");
+}
+
 // constrainHbyA (pEqn.cu:187-203): HbyAb = Ub where takeU (U not assignable), the owner cell's HbyA elsewhere
 inline void deviceConstrainHbyA(const DeviceMesh& dm, const DeviceBuffer<int>& takeU, const DeviceBuffer<scalar>& Ub,
                                 const DeviceBuffer<scalar>& HbyA, DeviceBuffer<scalar>& HbyAb)

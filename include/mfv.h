@@ -151,6 +151,12 @@ int mfv_boundary_diag_term(mfv_ctx ctx, mfv_mesh mesh, const double* dIcX, const
 int mfv_matrix_h(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, const double* dSource,
                  const double* dBdDiag, const double* dBdSrc, const double* dV, const double* dRAU, int valid,
                  double* dOut);
+/* No counterpart of its own: the three deviceMatrixH calls of pressurePredictor (pEqn.cu:143-185) in one pass
+ * over the matrix and the addressing. Host arrays of three device pointers per argument (component x, y, z);
+ * valid (host, may be NULL: all valid) knocks components out. The values are three mfv_matrix_h calls'. */
+int mfv_matrix_h3(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* const* dPsi,
+                  const double* const* dSource, const double* const* dBdDiag, const double* const* dBdSrc,
+                  const double* dV, const double* dRAU, const int* valid, double* const* dOut);
 /* constrainHbyA (pEqn.cu:187-203, constrainHbyAKernel): HbyAb = Ub where dTakeU is set (U not assignable),
  * the owner cell's HbyA elsewhere; one component */
 int mfv_constrain_hbya(mfv_ctx ctx, mfv_mesh mesh, const int* dTakeU, const double* dUb, const double* dHbyA,

@@ -86,6 +86,37 @@ def test_rau_and_hbya_bit_exact(case):
     assert not G.host(dZ).any()
 
 
+def test_three_components_in_one_pass(case):
+    """mfv_matrix_h3 gives three mfv_matrix_h calls' values (and the oracle's), with and without rAU, and
+    zeros for a knocked-out component"""
+    C = case
+    print("case:", C.nC, flush=True)
+    A = C.ldu()
+    rAU = O.reciprocal_v(O.fold_boundary_diag(C.nC, C.bfc, *C.ic, C.diag), C.V)
+    dR = G.dev(rAU)
+    bd = [O.boundary_diag_term(*C.ic, k) for k in range(3)]
+    dPsi, dSrc = [G.dev(u) for u in C.U], [G.dev(s) for s in C.src]
+    dBd, dBs = [G.dev(b) for b in bd], [G.dev(b) for b in C.bc]
+    for r in (None, dR):
+        for valid in (None, [1, 0, 1]):
+            outs = [G.dev(np.full(C.nC, 7.0)) for _ in range(3)]
+            v = None if valid is None else (ctypes.c_int * 3)(*valid)
+            C.call("mfv_matrix_h3", ctypes.byref(A), mfv.ptr3(dPsi), mfv.ptr3(dSrc), mfv.ptr3(dBd), mfv.ptr3(dBs),
+                   ptr(C.d["V"]), None if r is None else ptr(r), v, mfv.ptr3(outs))
+            for k in range(3):
+                got = G.host(outs[k])
+                if valid is not None and not valid[k]:
+                    assert not got.any(), k
+                    continue
+                H = O.matrix_h(C.nC, C.own, C.nei, C.upper, C.lower, C.U[k], C.src[k], C.bfc, bd[k], C.bc[k], C.V)
+                ref = H if r is None else rAU * H
+                single = G.zeros(C.nC)
+                C.call("mfv_matrix_h", ctypes.byref(A), ptr(dPsi[k]), ptr(dSrc[k]), ptr(dBd[k]), ptr(dBs[k]),
+                       ptr(C.d["V"]), None if r is None else ptr(r), 1, ptr(single))
+                assert np.array_equal(got, ref), k
+                assert np.array_equal(got, G.host(single)), k
+
+
 def test_constrain_and_flux_bit_exact(case):
     C = case
     print("case:", C.nC, flush=True)
