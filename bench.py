@@ -322,7 +322,8 @@ def gpu_run(args):
 
 
 def predictor_run(ch, ctx, patches, reps=5):
-    """The SIMPLE pressure predictor's glue (pEqn.cu pressurePredictor: A()'s folded diagonal and rAU,
+    """The SIMPLE pressure predictor's glue (UEqn.relax() on the momentum matrix, then pEqn.cu
+    pressurePredictor: A()'s folded diagonal and rAU,
     HbyA = rAU*H() per component, constrainHbyA, phiHbyA with its boundary faces, adjustPhi) on the
     channel, with a synthetic momentum matrix: upwind convection of the channel's flux made diagonally
     dominant, seeded internalCoeffs/boundaryCoeffs and sources, a seeded velocity. Inlet and walls fix U
@@ -356,7 +357,7 @@ def predictor_run(ch, ctx, patches, reps=5):
             takeU[start - F:start - F + size] = 0
             adjustable[start - F:start - F + size] = 1
     A = mfv.ldu(mD, mU, mL)
-    D, rAU, phi, phiB = z(N), z(N), z(F), z(B)
+    D, rAU, phi, phiB, R, delta, icMag, icMin = z(N), z(N), z(F), z(B), z(N), z(N), z(B), z(B)
     bd = [z(B) for _ in range(3)]
     HbyA = [z(N) for _ in range(3)]
     Hb = [z(B) for _ in range(3)]
@@ -373,7 +374,14 @@ def predictor_run(ch, ctx, patches, reps=5):
     mesh = ch.mesh.h
     massCorr = []
     for _ in range(reps + 1):
-        timed("foldBoundaryDiagK", lambda: ctx.call("mfv_fold_boundary_diag", mesh, P(mD), *(P(a) for a in ic), P(D)))
+        # UEqn.relax(): cmptMax(cmptMag(iC)), cmptMin(iC), then the relaxed diagonal and the three sources
+        timed("cmpt3K (max|iC|, min iC)", lambda: (
+            ctx.call("mfv_cmpt_max_mag3", B, *(P(a) for a in ic), P(icMag)),
+            ctx.call("mfv_cmpt_min3", B, *(P(a) for a in ic), P(icMin))))
+        timed("relaxK (diagonal, 3 sources)",
+              lambda: ctx.call("mfv_relax_diag", mesh, ctypes.byref(A), P(ic[0]), 0.7, None, P(icMag), P(icMin),
+                               P(R), P(delta), 3, mfv.ptr3(U), mfv.ptr3(src)))
+        timed("foldBoundaryDiagK", lambda: ctx.call("mfv_fold_boundary_diag", mesh, P(R), *(P(a) for a in ic), P(D)))
         timed("reciprocalVK", lambda: ctx.call("mfv_reciprocal_v", mesh, P(D), P(ch.V), P(rAU)))
         for k in range(3):
             timed("boundaryDiagTermK", lambda: ctx.call("mfv_boundary_diag_term", mesh, *(P(a) for a in ic), k,
@@ -394,7 +402,9 @@ def predictor_run(ch, ctx, patches, reps=5):
               lambda: ctx.call("mfv_adjust_phi", B, P(adjustable), P(phiB), F, P(phi), ctypes.byref(mc), None))
         massCorr.append(mc.value)
     torch.cuda.synchronize()
-    alg = {"foldBoundaryDiagK": 16 * N + 4 * N + 24 * B + 4 * B, "reciprocalVK": 24 * N,
+    alg = {"cmpt3K (max|iC|, min iC)": 2 * 32 * B,
+           "relaxK (diagonal, 3 sources)": 16 * F + 4 * F + 8 * N + 16 * N + 3 * 24 * N + 16 * B,
+           "foldBoundaryDiagK": 16 * N + 4 * N + 24 * B + 4 * B, "reciprocalVK": 24 * N,
            "boundaryDiagTermK": 32 * B,
            "matrixH3K (HbyA, 3 components)": 16 * F + 8 * F + 3 * 24 * N + 16 * N + 3 * 16 * B,
            "constrainHbyAK": 32 * B, "vectorFluxK": 48 * F + 24 * N, "boundaryFluxK": 56 * B,
@@ -411,7 +421,8 @@ def predictor_run(ch, ctx, patches, reps=5):
     log("pressure predictor glue: %.3f ms per step (%s)" % (total_ms, ", ".join(
         "%s %.1f us" % (k, v["avg_us"]) for k, v in kernels.items())))
     return {"ms_per_step": round(total_ms, 3), "mass_correction": massCorr[-1], "per_kernel": kernels,
-            "workload": "rAU, HbyA (3 components), constrainHbyA, phiHbyA, adjustPhi on the %dx%dx%d channel"
+            "workload": "UEqn relax, rAU, HbyA (3 components), constrainHbyA, phiHbyA, adjustPhi on the %dx%dx%d "
+                        "channel"
                         % ch.shape}
 
 

@@ -79,6 +79,7 @@ void buildMesh(Ctx& c, Mesh& m, int nCells, int nFaces, const int* owner, const 
     m.nCells = nCells;
     m.nFaces = nFaces;
     m.nBFaces = nB;
+    m.band = 0;
     m.hOwner.assign(owner, owner + nFaces);
     m.hNeighbour.assign(neighbour, neighbour + nFaces);
     m.hBFaceCell.assign(bFaceCell, bFaceCell + nB);
@@ -93,6 +94,7 @@ void buildMesh(Ctx& c, Mesh& m, int nCells, int nFaces, const int* owner, const 
                     " breaks OpenFOAM's upper-triangular order (faces sorted by owner)");
         ++os[o + 1];
         ++ls[n + 1];
+        if (n - o > m.band) m.band = n - o;
     }
     for (int i = 0; i < nCells; ++i)
     {
@@ -958,6 +960,52 @@ int mfv_matrix_h3(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* co
         }
         const LduView L = view(mesh, A);
         launchMatrixH3(ctx->c, mesh->m, L.upper, L.lower, dPsi, dSource, dBdDiag, dBdSrc, dV, dRAU, valid, dOut);
+    });
+}
+
+int mfv_relax_diag(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dICbnd, double alpha,
+                   const double* dCycSumOff, const double* dICmaxMag, const double* dICmin, double* dRelaxedDiag,
+                   double* dDelta, int nComp, const double* const* dPsi, double* const* dSource)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dRelaxedDiag || dDelta || nComp > 0, "relax: no output");
+        need(nComp >= 0 && nComp <= 3, "relax: nComp must be 0..3");
+        need(mesh->m.nBFaces == 0 || dICbnd || (dICmaxMag && dICmin), "relax: null internalCoeffs");
+        if (nComp) need(dPsi && dSource, "relax: null psi/source");
+        for (int k = 0; k < nComp; ++k) need(dPsi[k] && dSource[k], "relax: null psi/source");
+        if (!(alpha > 0.0)) invalid("relax: alpha must be positive (fvMatrix::relax skips alpha <= 0)");
+        const LduView L = view(mesh, A);
+        launchRelaxDiag(ctx->c, mesh->m, L.upper, L.lower, L.diag, dICbnd, dICmaxMag, dICmin, dCycSumOff, alpha, nComp,
+                        dPsi, dSource, dRelaxedDiag, dDelta);
+    });
+}
+
+int mfv_cmpt_max_mag3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut)
+{
+    return guarded(ctx, [&] {
+        need(n >= 0, "cmpt max mag: negative size");
+        need(n == 0 || (dA && dB && dC && dOut), "cmpt max mag: null argument");
+        launchCmpt3(ctx->c, n, dA, dB, dC, false, dOut);
+    });
+}
+
+int mfv_cmpt_min3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut)
+{
+    return guarded(ctx, [&] {
+        need(n >= 0, "cmpt min: negative size");
+        need(n == 0 || (dA && dB && dC && dOut), "cmpt min: null argument");
+        launchCmpt3(ctx->c, n, dA, dB, dC, true, dOut);
+    });
+}
+
+int mfv_relax_field(mfv_ctx ctx, int n, double* dP, const double* dPPrev, double alpha)
+{
+    return guarded(ctx, [&] {
+        need(n >= 0, "relax field: negative size");
+        if (alpha >= 1.0 || alpha <= 0.0 || n == 0) return;
+        need(dP && dPPrev, "relax field: null argument");
+        launchRelaxField(ctx->c, n, dP, dPPrev, alpha);
     });
 }
 

@@ -14,7 +14,27 @@ namespace mfv {
 namespace {
 
 constexpr int kTpb = 256;
+static_assert(kTpb == kReduceBlock, "slabRow numbers threads of kReduceBlock-wide workgroups");
 inline int blocks(int n) { return (n + kTpb - 1) / kTpb; }
+
+// The cell of a per-cell gather kernel's thread, or -1. Slab: the plane-slab partition of the matrix's
+// band (a cell's neighbour one band back -- lower[], psi[] of its losort run -- was read by the same XCD
+// one slab earlier and is still in that XCD's L2); else the XCD-contiguous order.
+template <bool Slab>
+__device__ __forceinline__ int gatherCell(int n, const SlabPart& part)
+{
+    if (Slab) return slabRow(part, 0);
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    return c < n ? c : -1;
+}
+
+// the partition of a mesh's cells for gather kernels: slabs of the band when the cells span several
+// bands of at least kSlabMinPlane cells, one row per thread
+inline SlabPart gatherPart(const Mesh& m)
+{
+    if (m.band >= kSlabMinPlane && m.band < m.nCells) return SlabPart{m.nCells, m.band, 1};
+    return SlabPart{m.nCells, 0, 1};
+}
 
 // D = diag + cmptAv(internalCoeffs) face by face (fv_matrix_ops.cuh:211-226; pEqn.cu:117-138, the
 // cmptAv and foldBoundaryDiag kernels), cmptAv = (x + y + z)/3
@@ -62,7 +82,8 @@ __global__ __launch_bounds__(kTpb) void boundaryDiagTermK(int nB, const double* 
 // H() of one component (fv_matrix_ops.cuh:277-340; device_simple.cu matrixHKernel), optionally times rAU
 // (HbyA, pEqn_cpp.cu:52-53): from zero, bdDiag*psi per boundary face, minus lower*psi[owner] and
 // upper*psi[neighbour] per internal face, plus the source, plus boundaryCoeffs per boundary face, over V
-__global__ __launch_bounds__(kTpb) void matrixHK(int n, const label* __restrict__ ownerStart,
+template <bool Slab>
+__global__ __launch_bounds__(kTpb) void matrixHK(int n, SlabPart part, const label* __restrict__ ownerStart,
                                                  const label* __restrict__ losortStart,
                                                  const label* __restrict__ losort,
                                                  const label* __restrict__ losortOwner,
@@ -75,8 +96,8 @@ __global__ __launch_bounds__(kTpb) void matrixHK(int n, const label* __restrict_
                                                  const double* __restrict__ V, const double* __restrict__ rAU,
                                                  double* __restrict__ out)
 {
-    const int c = xcdBlock() * kTpb + threadIdx.x;
-    if (c >= n) return;
+    const int c = gatherCell<Slab>(n, part);
+    if (c < 0) return;
     const int b0 = bndStart[c], b1 = bndStart[c + 1];
     const double pc = psi[c];
     double h = 0.0;
@@ -101,8 +122,15 @@ struct Vec3Args
 };
 
 // H() (or HbyA) of the three components in one pass: each component's terms in matrixHK's order, so the
-// values are three matrixHK launches' to the bit; a component that is not valid is written as zeros
-__global__ __launch_bounds__(kTpb) void matrixH3K(int n, const label* __restrict__ ownerStart,
+// values are three matrixHK launches' to the bit; a component that is not valid is written as zeros.
+// The first kBatch faces of the cell's losort run and of its owned run are loaded together, each level of
+// the gather (face index, coefficient and neighbour, psi) at once for both runs: clamped indices keep
+// every load in bounds and unconditional, and a face beyond its run is dropped from the sum, not added as
+// zero. Faces past the first kBatch of a run follow in a loop.
+constexpr int kBatch = 3;
+
+template <bool Slab>
+__global__ __launch_bounds__(kTpb) void matrixH3K(int n, SlabPart part, int nF, const label* __restrict__ ownerStart,
                                                   const label* __restrict__ losortStart,
                                                   const label* __restrict__ losort,
                                                   const label* __restrict__ losortOwner,
@@ -113,9 +141,11 @@ __global__ __launch_bounds__(kTpb) void matrixH3K(int n, const label* __restrict
                                                   Vec3Args a, const double* __restrict__ V,
                                                   const double* __restrict__ rAU)
 {
-    const int c = xcdBlock() * kTpb + threadIdx.x;
-    if (c >= n) return;
+    const int c = gatherCell<Slab>(n, part);
+    if (c < 0) return;
     const int b0 = bndStart[c], b1 = bndStart[c + 1];
+    const int l0 = losortStart[c], l1 = losortStart[c + 1];
+    const int f0 = ownerStart[c], f1 = ownerStart[c + 1];
     double h[3], pc[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -123,25 +153,61 @@ __global__ __launch_bounds__(kTpb) void matrixH3K(int n, const label* __restrict
         h[k] = 0.0;
         pc[k] = a.psi[k][c];
     }
+    double lv[kBatch] = {}, uv[kBatch] = {}, lp[kBatch][3] = {}, up[kBatch][3] = {};
+    if (nF > 0)   // uniform: a mesh with internal faces
+    {
+        const int last = nF - 1;
+        int lf[kBatch], lo[kBatch], nb[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            const int j = min(l0 + u, last), f = min(f0 + u, last);
+            lf[u] = losort[j];
+            lo[u] = losortOwner[j];
+            nb[u] = neighbour[f];
+            uv[u] = upper[f];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            lv[u] = lower[lf[u]];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                lp[u][k] = a.psi[k][lo[u]];
+                up[u][k] = a.psi[k][nb[u]];
+            }
+        }
+    }
     for (int j = b0; j < b1; ++j)
     {
         const int i = bndFaces[j];
 #pragma unroll
         for (int k = 0; k < 3; ++k) h[k] += a.bdDiag[k][i] * pc[k];
     }
-    for (int j = losortStart[c]; j < losortStart[c + 1]; ++j)
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (l0 + u < l1) h[k] = h[k] - lv[u] * lp[u][k];
+    for (int j = l0 + kBatch; j < l1; ++j)
     {
         const double l = lower[losort[j]];
         const int o = losortOwner[j];
 #pragma unroll
         for (int k = 0; k < 3; ++k) h[k] = h[k] - l * a.psi[k][o];
     }
-    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f)
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (f0 + u < f1) h[k] = h[k] - uv[u] * up[u][k];
+    for (int f = f0 + kBatch; f < f1; ++f)
     {
         const double u = upper[f];
-        const int nb = neighbour[f];
+        const int nbf = neighbour[f];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) h[k] = h[k] - u * a.psi[k][nb];
+        for (int k = 0; k < 3; ++k) h[k] = h[k] - u * a.psi[k][nbf];
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) h[k] += a.source[k][c];
@@ -159,6 +225,99 @@ __global__ __launch_bounds__(kTpb) void matrixH3K(int n, const label* __restrict
         const double hk = h[k] / v;
         a.out[k][c] = !a.valid[k] ? 0.0 : (rAU ? r * hk : hk);
     }
+}
+
+// fvMatrix::relax for uncoupled patches (fv_matrix_ops.cuh:158-207; device_simple.cu relaxKernel) per cell,
+// in the host loops' order: sumOff = |lower| over the losort run then |upper| over the owned run (face
+// order), plus the coupled-interface magnitude; d = diag + cmptMagMax(ic) face by face; d = max(|d|,
+// sumOff)/alpha; d -= cmptMin(ic) face by face; delta = d - diag; and, for nComp components, source_k +=
+// delta*psi_k in the same pass. icMag/icMin NULL: |icBnd| and icBnd (equal components).
+struct RelaxArgs
+{
+    const double* icBnd;
+    const double* icMag;
+    const double* icMin;
+    const double* cycSumOff;
+    double alpha;
+    int nComp;
+    const double* psi[3];
+    double* source[3];
+};
+
+template <bool Slab>
+__global__ __launch_bounds__(kTpb) void relaxK(int n, SlabPart part, int nF, const label* __restrict__ ownerStart,
+                                               const label* __restrict__ losortStart,
+                                               const label* __restrict__ losort, const label* __restrict__ bndStart,
+                                               const label* __restrict__ bndFaces, const double* __restrict__ upper,
+                                               const double* __restrict__ lower, const double* __restrict__ diag,
+                                               RelaxArgs a, double* __restrict__ relaxed, double* __restrict__ delta)
+{
+    const int c = gatherCell<Slab>(n, part);
+    if (c < 0) return;
+    const int b0 = bndStart[c], b1 = bndStart[c + 1];
+    const int l0 = losortStart[c], l1 = losortStart[c + 1];
+    const int f0 = ownerStart[c], f1 = ownerStart[c + 1];
+    double lv[kBatch] = {}, uv[kBatch] = {};
+    if (nF > 0)   // uniform
+    {
+        const int last = nF - 1;
+        int lf[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            lf[u] = losort[min(l0 + u, last)];
+            uv[u] = upper[min(f0 + u, last)];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) lv[u] = lower[lf[u]];
+    }
+    const double d0 = diag[c];
+    double sumOff = 0.0;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+        if (l0 + u < l1) sumOff += fabs(lv[u]);
+    for (int j = l0 + kBatch; j < l1; ++j) sumOff += fabs(lower[losort[j]]);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+        if (f0 + u < f1) sumOff += fabs(uv[u]);
+    for (int f = f0 + kBatch; f < f1; ++f) sumOff += fabs(upper[f]);
+    if (a.cycSumOff) sumOff += a.cycSumOff[c];
+    double d = d0;
+    for (int j = b0; j < b1; ++j)
+    {
+        const int i = bndFaces[j];
+        d += a.icMag ? a.icMag[i] : fabs(a.icBnd[i]);
+    }
+    d = fmax(fabs(d), sumOff) / a.alpha;
+    for (int j = b0; j < b1; ++j)
+    {
+        const int i = bndFaces[j];
+        d -= a.icMin ? a.icMin[i] : a.icBnd[i];
+    }
+    const double dd = d - d0;
+    if (relaxed) relaxed[c] = d;
+    if (delta) delta[c] = dd;
+    for (int k = 0; k < a.nComp; ++k) a.source[k][c] += dd * a.psi[k][c];
+}
+
+// per element: out = max(|a|, |b|, |c|) (cmptMax(cmptMag)), or min(a, b, c) (cmptMin); cf_types.cuh:199-202
+template <bool Min>
+__global__ __launch_bounds__(kTpb) void cmpt3K(int n, const double* __restrict__ x, const double* __restrict__ y,
+                                               const double* __restrict__ z, double* __restrict__ out)
+{
+    const int i = xcdBlock() * kTpb + threadIdx.x;
+    if (i >= n) return;
+    out[i] = Min ? fmin(fmin(x[i], y[i]), z[i]) : fmax(fmax(fabs(x[i]), fabs(y[i])), fabs(z[i]));
+}
+
+// GeometricField::relax (pEqn.cu:422-433; pEqn_cpp.cu:295-300): p = pPrev + alpha*(p - pPrev)
+__global__ __launch_bounds__(kTpb) void relaxFieldK(int n, double* __restrict__ p, const double* __restrict__ pPrev,
+                                                    double alpha)
+{
+    const int i = xcdBlock() * kTpb + threadIdx.x;
+    if (i >= n) return;
+    const double q = pPrev[i];
+    p[i] = q + alpha * (p[i] - q);
 }
 
 // constrainHbyA (pEqn.cu constrainHbyAKernel; pEqn_cpp.cu:55-72): U's boundary value on a face whose U
@@ -202,20 +361,33 @@ __global__ __launch_bounds__(kTpb) void boundaryFluxK(int nB, const double* __re
     phiB[i] = (isEmpty && isEmpty[i]) ? 0.0 : uxb[i] * s[0] + uyb[i] * s[1] + uzb[i] * s[2];
 }
 
-// adjustPhi's sums (pEqn_cpp.cu:85-105): per workgroup, sum|phi| over internal faces, and the boundary's
-// massIn, fixed outflow and adjustable outflow (four partial slots)
+// adjustPhi's sums (pEqn_cpp.cu:85-105): per workgroup, sum|phi| over its chunk of the internal faces (eight
+// loads in flight per thread), and massIn, fixed outflow and adjustable outflow over its chunk of the
+// boundary faces (four partial slots)
 __global__ __launch_bounds__(kReduceBlock) void adjustSumsK(int nF, const double* __restrict__ phiInt, int nB,
                                                             const int* __restrict__ adjustable,
                                                             const double* __restrict__ phiB, double* pTot,
                                                             double* pIn, double* pFix, double* pAdj)
 {
     __shared__ double lds[kReduceBlock / 64];
-    const int n = nF > nB ? nF : nB;
+    const int wb = xcdBlock();
     double t = 0.0, in = 0.0, fix = 0.0, adj = 0.0;
-    MFV_ROWS(n, i)
     {
-        if (i < nF) t += fabs(phiInt[i]);
-        if (i < nB)
+        const int ch = chunkRows(nF, gridDim.x), b = wb * ch, e = b + ch < nF ? b + ch : nF;
+        int i = b + (int)threadIdx.x;
+        for (; i + 7 * kReduceBlock < e; i += 8 * kReduceBlock)
+        {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = phiInt[i + u * kReduceBlock];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) t += fabs(v[u]);
+        }
+        for (; i < e; i += kReduceBlock) t += fabs(phiInt[i]);
+    }
+    {
+        const int ch = chunkRows(nB, gridDim.x), b = wb * ch, e = b + ch < nB ? b + ch : nB;
+        for (int i = b + (int)threadIdx.x; i < e; i += kReduceBlock)
         {
             const double v = phiB[i];
             const bool a = adjustable[i] != 0;
@@ -343,9 +515,15 @@ void launchMatrixH(Ctx& c, const Mesh& m, const double* upper, const double* low
                    const double* rAU, double* out)
 {
     if (m.nCells == 0) return;
-    matrixHK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.ownerStart.p, m.losortStart.p, m.losort.p,
-                                                       m.losortOwner.p, m.neighbour.p, m.bndStart.p, m.bndFaces.p,
-                                                       upper, lower, psi, source, bdDiag, bdSrc, V, rAU, out);
+    const SlabPart part = gatherPart(m);
+    if (part.plane)
+        matrixHK<true><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p, m.bndStart.p,
+            m.bndFaces.p, upper, lower, psi, source, bdDiag, bdSrc, V, rAU, out);
+    else
+        matrixHK<false><<<blocks(m.nCells), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p, m.bndStart.p,
+            m.bndFaces.p, upper, lower, psi, source, bdDiag, bdSrc, V, rAU, out);
     hipCheck(hipGetLastError(), "matrix H");
 }
 
@@ -366,10 +544,63 @@ void launchMatrixH3(Ctx& c, const Mesh& m, const double* upper, const double* lo
         a.bdSrc[k] = bdSrc[s];
         a.out[k] = out[k];
     }
-    matrixH3K<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.ownerStart.p, m.losortStart.p, m.losort.p,
-                                                        m.losortOwner.p, m.neighbour.p, m.bndStart.p, m.bndFaces.p,
-                                                        upper, lower, a, V, rAU);
+    const SlabPart part = gatherPart(m);
+    if (part.plane)
+        matrixH3K<true><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p,
+            m.bndStart.p, m.bndFaces.p, upper, lower, a, V, rAU);
+    else
+        matrixH3K<false><<<blocks(m.nCells), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p,
+            m.bndStart.p, m.bndFaces.p, upper, lower, a, V, rAU);
     hipCheck(hipGetLastError(), "matrix H (three components)");
+}
+
+void launchRelaxDiag(Ctx& c, const Mesh& m, const double* upper, const double* lower, const double* diag,
+                     const double* icBnd, const double* icMag, const double* icMin, const double* cycSumOff,
+                     double alpha, int nComp, const double* const* psi, double* const* source, double* relaxed,
+                     double* delta)
+{
+    if (m.nCells == 0) return;
+    RelaxArgs a{};
+    a.icBnd = icBnd;
+    a.icMag = icMag;
+    a.icMin = icMin;
+    a.cycSumOff = cycSumOff;
+    a.alpha = alpha;
+    a.nComp = nComp;
+    for (int k = 0; k < nComp; ++k)
+    {
+        a.psi[k] = psi[k];
+        a.source[k] = source[k];
+    }
+    const SlabPart part = gatherPart(m);
+    if (part.plane)
+        relaxK<true><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.bndStart.p, m.bndFaces.p, upper,
+            lower, diag, a, relaxed, delta);
+    else
+        relaxK<false><<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, part, m.nFaces, m.ownerStart.p,
+                                                               m.losortStart.p, m.losort.p, m.bndStart.p,
+                                                               m.bndFaces.p, upper, lower, diag, a, relaxed, delta);
+    hipCheck(hipGetLastError(), "relax diag");
+}
+
+void launchCmpt3(Ctx& c, int n, const double* x, const double* y, const double* z, bool min, double* out)
+{
+    if (n <= 0) return;
+    if (min)
+        cmpt3K<true><<<blocks(n), kTpb, 0, c.stream>>>(n, x, y, z, out);
+    else
+        cmpt3K<false><<<blocks(n), kTpb, 0, c.stream>>>(n, x, y, z, out);
+    hipCheck(hipGetLastError(), "component max/min");
+}
+
+void launchRelaxField(Ctx& c, int n, double* p, const double* pPrev, double alpha)
+{
+    if (n <= 0) return;
+    relaxFieldK<<<blocks(n), kTpb, 0, c.stream>>>(n, p, pPrev, alpha);
+    hipCheck(hipGetLastError(), "relax field");
 }
 
 void launchConstrainHbyA(Ctx& c, const Mesh& m, const int* takeU, const double* Ub, const double* HbyA,
@@ -405,7 +636,7 @@ void launchAdjustPhi(Ctx& c, int nB, const int* adjustable, double* phiB, int nF
         adjustSeqK<<<1, 64, 0, c.stream>>>(nF, phiInt, nB, adjustable, phiB, decision);
     else
     {
-        const int g = reduceGrid(nF > nB ? nF : nB);
+        const int g = reduceGridRows(nF > nB ? nF : nB, 8);
         double* pa = c.partials.p;
         adjustSumsK<<<g, kReduceBlock, 0, c.stream>>>(nF, phiInt, nB, adjustable, phiB, pa, pa + kMaxReduceBlocks,
                                                       pa + 2 * (size_t)kMaxReduceBlocks,

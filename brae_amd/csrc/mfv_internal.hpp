@@ -164,6 +164,7 @@ struct Mesh
     DBuf<label> bFaceCell;     // [nBFaces]
     DBuf<label> bndStart;      // [nCells+1] into bndFaces
     DBuf<label> bndFaces;      // boundary face indices (0-based within the boundary) per cell, ascending
+    int band = 0;              // largest neighbour - owner over the internal faces
     std::vector<label> hOwner, hNeighbour, hBFaceCell;
     std::unique_ptr<LevelSched> sched;   // built on first use by the sequential sweeps (DIC, Gauss-Seidel)
 };
@@ -221,6 +222,12 @@ void launchMatrixH(Ctx& c, const Mesh& m, const double* upper, const double* low
 void launchMatrixH3(Ctx& c, const Mesh& m, const double* upper, const double* lower, const double* const* psi,
                     const double* const* source, const double* const* bdDiag, const double* const* bdSrc,
                     const double* V, const double* rAU, const int* valid, double* const* out);
+void launchRelaxDiag(Ctx& c, const Mesh& m, const double* upper, const double* lower, const double* diag,
+                     const double* icBnd, const double* icMag, const double* icMin, const double* cycSumOff,
+                     double alpha, int nComp, const double* const* psi, double* const* source, double* relaxed,
+                     double* delta);
+void launchCmpt3(Ctx& c, int n, const double* x, const double* y, const double* z, bool min, double* out);
+void launchRelaxField(Ctx& c, int n, double* p, const double* pPrev, double alpha);
 void launchConstrainHbyA(Ctx& c, const Mesh& m, const int* takeU, const double* Ub, const double* HbyA,
                          double* HbyAb);
 void launchVectorFlux(Ctx& c, const Mesh& m, const double* w, const double* Sf, const double* Ux, const double* Uy,

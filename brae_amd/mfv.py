@@ -138,6 +138,10 @@ _SIGS = {
     "mfv_vector_flux": [P, P, P, P, P, P, P, P],
     "mfv_boundary_flux": [P, P, P, P, P, P, P, P],
     "mfv_adjust_phi": [P, I, P, P, I, P, P, P],
+    "mfv_relax_diag": [P, P, P, P, D, P, P, P, P, P, I, P, P],
+    "mfv_cmpt_max_mag3": [P, I, P, P, P, P],
+    "mfv_cmpt_min3": [P, I, P, P, P, P],
+    "mfv_relax_field": [P, I, P, P, D],
     "mfv_amg_level_addressing": [P, I, P, P, P, P, P],
     "mfv_amg_galerkin": [P, P, P, P, P, I],
     "mfv_amg_grid_values": [P, P, I, P, P],

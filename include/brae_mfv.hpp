@@ -413,6 +413,70 @@ inline void deviceConstrainHbyA(const DeviceMesh& dm, const DeviceBuffer<int>& t
              "deviceConstrainHbyA");
 }
 
+// deviceRelaxDiag (src/cuda/device_simple.cuh:118-120): fvMatrix::relax's diagonal for uncoupled patches,
+// relaxedDiag = max(|diag + cmptMagMax(iC)|, sumOff)/alpha - cmptMin(iC) and delta = relaxedDiag - diag.
+// iCmaxMag/iCmin nullptr: |iCbnd| and iCbnd.
+inline void deviceRelaxDiag(const DeviceLduView& A, const DeviceMesh& dm, const DeviceBuffer<scalar>& iCbnd, scalar alpha,
+                            DeviceBuffer<scalar>& relaxedDiag, DeviceBuffer<scalar>& delta,
+                            const scalar* cycSumOff = nullptr, const scalar* iCmaxMag = nullptr,
+                            const scalar* iCmin = nullptr)
+{
+    mfv_ctx ctx = threadContext();
+    relaxedDiag.resize(dm.nCells);
+    delta.resize(dm.nCells);
+    const mfv_ldu l = A.ldu();
+    mfvCheck(mfv_relax_diag(ctx, dm.mesh, &l, iCbnd.data(), alpha, cycSumOff, iCmaxMag, iCmin, relaxedDiag.data(),
+                            delta.data(), 0, nullptr, nullptr),
+             ctx, "deviceRelaxDiag");
+}
+
+// UEqn.relax() in one pass (UEqn.cu:724-760: deviceRelaxDiag, then source_k += delta*U_k per component)
+inline void deviceRelaxMomentum(const DeviceLduView& A, const DeviceMesh& dm, const DeviceBuffer<scalar>* iC,
+                                scalar alpha, const DeviceBuffer<scalar>* U, DeviceBuffer<scalar>* source,
+                                DeviceBuffer<scalar>& relaxedDiag, DeviceBuffer<scalar>& delta)
+{
+    mfv_ctx ctx = threadContext();
+    DeviceBuffer<scalar> iCmaxMag(dm.nBoundaryFaces), iCmin(dm.nBoundaryFaces);
+    mfvCheck(mfv_cmpt_max_mag3(ctx, dm.nBoundaryFaces, iC[0].data(), iC[1].data(), iC[2].data(), iCmaxMag.data()),
+             ctx, "deviceCmptMaxMag3");
+    mfvCheck(mfv_cmpt_min3(ctx, dm.nBoundaryFaces, iC[0].data(), iC[1].data(), iC[2].data(), iCmin.data()), ctx,
+             "deviceCmptMin3");
+    relaxedDiag.resize(dm.nCells);
+    delta.resize(dm.nCells);
+    const mfv_ldu l = A.ldu();
+    const scalar* psi[3] = {U[0].data(), U[1].data(), U[2].data()};
+    scalar* src[3] = {source[0].data(), source[1].data(), source[2].data()};
+    mfvCheck(mfv_relax_diag(ctx, dm.mesh, &l, iC[0].data(), alpha, nullptr, iCmaxMag.data(), iCmin.data(),
+                            relaxedDiag.data(), delta.data(), 3, psi, src),
+             ctx, "deviceRelaxMomentum");
+}
+
+// deviceCmptMaxMag3 / deviceCmptMin3 (src/cuda/device_simple.cuh:123-128)
+inline void deviceCmptMaxMag3(const DeviceBuffer<scalar>& a, const DeviceBuffer<scalar>& b,
+                              const DeviceBuffer<scalar>& c, DeviceBuffer<scalar>& out)
+{
+    mfv_ctx ctx = threadContext();
+    out.resize(a.size());
+    mfvCheck(mfv_cmpt_max_mag3(ctx, static_cast<int>(a.size()), a.data(), b.data(), c.data(), out.data()), ctx,
+             "deviceCmptMaxMag3");
+}
+
+inline void deviceCmptMin3(const DeviceBuffer<scalar>& a, const DeviceBuffer<scalar>& b, const DeviceBuffer<scalar>& c,
+                           DeviceBuffer<scalar>& out)
+{
+    mfv_ctx ctx = threadContext();
+    out.resize(a.size());
+    mfvCheck(mfv_cmpt_min3(ctx, static_cast<int>(a.size()), a.data(), b.data(), c.data(), out.data()), ctx,
+             "deviceCmptMin3");
+}
+
+// relaxField (src/applications/solvers/simpleFoam/pEqn.cuh:154): p = pPrev + alpha*(p - pPrev)
+inline void relaxField(DeviceBuffer<scalar>& p, const DeviceBuffer<scalar>& pPrev, scalar alpha)
+{
+    mfv_ctx ctx = threadContext();
+    mfvCheck(mfv_relax_field(ctx, static_cast<int>(p.size()), p.data(), pPrev.data(), alpha), ctx, "relaxField");
+}
+
 // deviceVectorFlux (src/cuda/device_simple.cuh:48): phiInt = interpolate(U) . Sf
 inline void deviceVectorFlux(const DeviceMesh& dm, const DeviceBuffer<scalar>& Ux, const DeviceBuffer<scalar>& Uy,
                              const DeviceBuffer<scalar>& Uz, DeviceBuffer<scalar>& phiInt)

@@ -157,6 +157,23 @@ int mfv_matrix_h(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPs
 int mfv_matrix_h3(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* const* dPsi,
                   const double* const* dSource, const double* const* dBdDiag, const double* const* dBdSrc,
                   const double* dV, const double* dRAU, const int* valid, double* const* dOut);
+/* deviceRelaxDiag (src/cuda/device_simple.cuh:118-120; fvMatrix::relax, fv_matrix_ops.cuh:158-207) for
+ * uncoupled patches: sumOff = sum|off-diagonal| of the row (+ dCycSumOff, may be NULL); D = diag +
+ * cmptMagMax(ic) per boundary face; D = max(|D|, sumOff)/alpha; D -= cmptMin(ic) per boundary face.
+ * dICmaxMag/dICmin (may be NULL: |dICbnd| and dICbnd) are per boundary face. Writes dRelaxedDiag = D and
+ * dDelta = D - diag (either may be NULL). nComp (0..3) components also get their source update in the same
+ * pass, dSource[k] += dDelta*dPsi[k] (UEqn.cu:753-760), host arrays of device pointers. Sums in the host
+ * loops' order. MFV_ERR_INVALID for alpha <= 0 (the host skips the relaxation: leave the matrix as is). */
+int mfv_relax_diag(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dICbnd, double alpha,
+                   const double* dCycSumOff, const double* dICmaxMag, const double* dICmin, double* dRelaxedDiag,
+                   double* dDelta, int nComp, const double* const* dPsi, double* const* dSource);
+/* deviceCmptMaxMag3 / deviceCmptMin3 (src/cuda/device_simple.cuh:123-128): per element max(|a|,|b|,|c|) and
+ * min(a,b,c) */
+int mfv_cmpt_max_mag3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut);
+int mfv_cmpt_min3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut);
+/* relaxField (src/applications/solvers/simpleFoam/pEqn.cuh:154): p = pPrev + alpha*(p - pPrev) in place;
+ * nothing unless 0 < alpha < 1 */
+int mfv_relax_field(mfv_ctx ctx, int n, double* dP, const double* dPPrev, double alpha);
 /* constrainHbyA (pEqn.cu:187-203, constrainHbyAKernel): HbyAb = Ub where dTakeU is set (U not assignable),
  * the owner cell's HbyA elsewhere; one component */
 int mfv_constrain_hbya(mfv_ctx ctx, mfv_mesh mesh, const int* dTakeU, const double* dUb, const double* dHbyA,

@@ -2116,6 +2116,62 @@ void orc_matrix_h(int nCells, int nFaces, const int* owner, const int* neighbour
     for (int c = 0; c < nCells; ++c) H[c] = H[c] / V[c];
 }
 
+/* fvMatrix::relax(alpha) for uncoupled patches (fv_matrix_ops.cuh:158-207): sumOff from |upper| (owner) and
+ * |lower| (neighbour) in face order; diag += cmptMagMax(ic) face by face; diag = max(|diag|, sumOff)/alpha;
+ * diag -= cmptMin(ic) face by face; source_k += (diag - D0)*psi_k for each of nComp components. icMag/icMin:
+ * per boundary face cmptMagMax / cmptMin of the internalCoeffs. cycSumOff (may be NULL) adds each cell's
+ * coupled-interface magnitude after the internal faces. alpha <= 0: nothing changes. diag and the
+ * sources in place; delta (may be NULL) = relaxed - D0 */
+void orc_relax_matrix(int nCells, int nFaces, const int* owner, const int* neighbour, const double* upper,
+                      const double* lower, int nBFaces, const int* bFaceCell, const double* icMag,
+                      const double* icMin, const double* cycSumOff, double alpha, double* diag, int nComp,
+                      const double* const* psi, double* const* source, double* delta)
+{
+    if (alpha <= 0.0)
+    {
+        if (delta) memset(delta, 0, sizeof(double) * (size_t)nCells);
+        return;
+    }
+    double* D0 = (double*)malloc(sizeof(double) * (size_t)nCells);
+    double* sumOff = (double*)calloc((size_t)nCells, sizeof(double));
+    memcpy(D0, diag, sizeof(double) * (size_t)nCells);
+    for (int f = 0; f < nFaces; ++f)
+    {
+        sumOff[owner[f]] += fabs(upper[f]);
+        sumOff[neighbour[f]] += fabs(lower[f]);
+    }
+    if (cycSumOff)
+        for (int c = 0; c < nCells; ++c) sumOff[c] += cycSumOff[c];
+    for (int i = 0; i < nBFaces; ++i) diag[bFaceCell[i]] += icMag[i];
+    for (int c = 0; c < nCells; ++c) diag[c] = fmax(fabs(diag[c]), sumOff[c]) / alpha;
+    for (int i = 0; i < nBFaces; ++i) diag[bFaceCell[i]] -= icMin[i];
+    for (int c = 0; c < nCells; ++c)
+    {
+        const double d = diag[c] - D0[c];
+        for (int k = 0; k < nComp; ++k) source[k][c] += d * psi[k][c];
+        if (delta) delta[c] = d;
+    }
+    free(D0);
+    free(sumOff);
+}
+
+/* cmptMax(cmptMag(v)) and cmptMin(v) of the internalCoeffs per boundary face (cf_types.cuh:199-202) */
+void orc_cmpt_mag_max_min(int n, const double* x, const double* y, const double* z, double* mag, double* mn)
+{
+    for (int i = 0; i < n; ++i)
+    {
+        mag[i] = fmax(fmax(fabs(x[i]), fabs(y[i])), fabs(z[i]));
+        mn[i] = fmin(fmin(x[i], y[i]), z[i]);
+    }
+}
+
+/* GeometricField::relax (pEqn_cpp.cu:295-300): p = pPrev + alpha*(p - pPrev); nothing unless 0 < alpha < 1 */
+void orc_relax_field(int n, double* p, const double* pPrev, double alpha)
+{
+    if (alpha >= 1.0 || alpha <= 0.0) return;
+    for (int c = 0; c < n; ++c) p[c] = pPrev[c] + alpha * (p[c] - pPrev[c]);
+}
+
 void orc_vector_flux(int nFaces, const int* owner, const int* neighbour, const double* weights, const double* Sf,
                      const double* Ux, const double* Uy, const double* Uz, double* phi)
 {

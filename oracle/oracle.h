@@ -194,6 +194,12 @@ void orc_boundary_diag_term(int nBFaces, const double* icX, const double* icY, c
 /* H() of one component (fv_matrix_ops.cuh:277-340, matrixH; uncoupled patches): from zero, += bdDiag*psi per
  * boundary face; per internal face in order H[nei] -= lower*psi[own], H[own] -= upper*psi[nei]; += source;
  * += boundaryCoeffs per boundary face; /= V */
+void orc_relax_matrix(int nCells, int nFaces, const int* owner, const int* neighbour, const double* upper,
+                      const double* lower, int nBFaces, const int* bFaceCell, const double* icMag,
+                      const double* icMin, const double* cycSumOff, double alpha, double* diag, int nComp,
+                      const double* const* psi, double* const* source, double* delta);
+void orc_cmpt_mag_max_min(int n, const double* x, const double* y, const double* z, double* mag, double* mn);
+void orc_relax_field(int n, double* p, const double* pPrev, double alpha);
 void orc_matrix_h(int nCells, int nFaces, const int* owner, const int* neighbour, const double* upper,
                   const double* lower, const double* psi, const double* source, int nBFaces, const int* bFaceCell,
                   const double* bdDiag, const double* bdSrc, const double* V, double* H);

@@ -64,6 +64,9 @@ def lib():
             "orc_vector_flux": (None, [I, P, P, P, P, P, P, P, P]),
             "orc_boundary_flux": (None, [I, P, P, P, P, P, P]),
             "orc_adjust_phi": (I, [I, P, I, P, P, P, P]),
+            "orc_relax_matrix": (None, [I, I, P, P, P, P, I, P, P, P, P, D, P, I, P, P, P]),
+            "orc_cmpt_mag_max_min": (None, [I, P, P, P, P, P]),
+            "orc_relax_field": (None, [I, P, P, D]),
             "orc_grad_gauss": (None, [I, I, P, P, I, P, P, P, P, P, P, P, P]),
             "orc_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
             "orc_sym_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
@@ -407,3 +410,33 @@ def brae_vcycle(aggl, diag, upper, r, nPre=1, nPost=1, single=False, scale=False
     z = np.zeros(len(r))
     lib().orc_brae_vcycle(aggl.h, p(f64(diag)), p(f64(upper)), p(f64(r)), p(z), nPre, nPost, int(single), int(scale))
     return z
+
+
+def cmpt_mag_max_min(x, y, z):
+    """per boundary face cmptMax(cmptMag(ic)) and cmptMin(ic)"""
+    mag, mn = np.zeros(len(x)), np.zeros(len(x))
+    lib().orc_cmpt_mag_max_min(len(x), p(f64(x)), p(f64(y)), p(f64(z)), p(mag), p(mn))
+    return mag, mn
+
+
+def relax_matrix(nC, own, nei, upper, lower, bfc, icMag, icMin, alpha, diag, psi=(), source=(), cycSumOff=None):
+    """fvMatrix::relax: (relaxed diag, delta, relaxed sources) for the components psi/source"""
+    d = f64(diag).copy()
+    src = [f64(s).copy() for s in source]
+    ps = [f64(x) for x in psi]
+    k = len(src)
+    PA = ctypes.c_void_p * max(k, 1)
+    pp = PA(*[x.ctypes.data for x in ps]) if k else None
+    sp = PA(*[x.ctypes.data for x in src]) if k else None
+    delta = np.zeros(nC)
+    lib().orc_relax_matrix(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(upper)), p(f64(lower)), len(bfc),
+                           p(i32(bfc)), p(f64(icMag)), p(f64(icMin)), p(None if cycSumOff is None else f64(cycSumOff)),
+                           alpha, p(d), k, ctypes.cast(pp, ctypes.c_void_p) if k else None,
+                           ctypes.cast(sp, ctypes.c_void_p) if k else None, p(delta))
+    return d, delta, src
+
+
+def relax_field(pNew, pPrev, alpha):
+    out = f64(pNew).copy()
+    lib().orc_relax_field(len(out), p(out), p(f64(pPrev)), alpha)
+    return out

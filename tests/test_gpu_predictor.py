@@ -179,3 +179,56 @@ def test_adjust_phi(case):
     rc, _, _, out = adjust(C, phiInt, none, phiB)
     assert rc == mfv.MFV_ERR_INVALID and np.array_equal(out, phiB)
     assert "continuity" in mfv.lib().mfv_last_error(C.case.ctx.h).decode()
+
+
+def test_relax_bit_exact(case):
+    """fvMatrix::relax (the momentum matrix's diagonal, and the three sources in the same pass) and its
+    component reductions: the oracle's values to the bit"""
+    C = case
+    print("case:", C.nC, flush=True)
+    ic = [a.copy() for a in C.ic]
+    ic[1][::3] *= -1.5                                   # slip-like faces: the components differ
+    dic = [G.dev(a) for a in ic]
+    mag_ref, mn_ref = O.cmpt_mag_max_min(*ic)
+    dmag, dmn = G.zeros(C.nB), G.zeros(C.nB)
+    lib = mfv.lib()
+    assert lib.mfv_cmpt_max_mag3(C.case.ctx.h, C.nB, *(ptr(a) for a in dic), ptr(dmag)) == 0
+    assert lib.mfv_cmpt_min3(C.case.ctx.h, C.nB, *(ptr(a) for a in dic), ptr(dmn)) == 0
+    assert np.array_equal(G.host(dmag), mag_ref) and np.array_equal(G.host(dmn), mn_ref)
+    diag = C.diag.copy()
+    diag[::4] *= 0.05                                    # rows where the off-diagonal bound wins
+    A = mfv.ldu(G.dev(diag), C.d["upper"], C.d["lower"])
+    cyc = np.zeros(C.nC)
+    cyc[::17] = 3.0
+    for alpha, cy in ((0.7, None), (0.3, cyc), (1.0, None)):
+        d_ref, delta_ref, src_ref = O.relax_matrix(C.nC, C.own, C.nei, C.upper, C.lower, C.bfc, mag_ref, mn_ref, alpha,
+                                                   diag, psi=C.U, source=C.src, cycSumOff=cy)
+        dR, dD = G.zeros(C.nC), G.zeros(C.nC)
+        dS = [G.dev(s) for s in C.src]
+        dP = [G.dev(u) for u in C.U]
+        C.call("mfv_relax_diag", ctypes.byref(A), ptr(dic[0]), alpha, None if cy is None else ptr(G.dev(cy)),
+               ptr(dmag), ptr(dmn), ptr(dR), ptr(dD), 3, mfv.ptr3(dP), mfv.ptr3(dS))
+        assert np.array_equal(G.host(dR), d_ref), alpha
+        assert np.array_equal(G.host(dD), delta_ref), alpha
+        for k in range(3):
+            assert np.array_equal(G.host(dS[k]), src_ref[k]), (alpha, k)
+    # the reference's scalar form: one internalCoeffs array, |iC| added and iC removed; no source update
+    d_ref, delta_ref, _ = O.relax_matrix(C.nC, C.own, C.nei, C.upper, C.lower, C.bfc, np.abs(C.ic[0]), C.ic[0], 0.5,
+                                         diag)
+    dR = G.zeros(C.nC)
+    C.call("mfv_relax_diag", ctypes.byref(A), ptr(C.dic[0]), 0.5, None, None, None, ptr(dR), None, 0, None, None)
+    assert np.array_equal(G.host(dR), d_ref)
+    # alpha <= 0 is refused by name (the host skips the relaxation)
+    rc = lib.mfv_relax_diag(C.case.ctx.h, C.case.mesh.h, ctypes.byref(A), ptr(C.dic[0]), 0.0, None, None, None,
+                            ptr(dR), None, 0, None, None)
+    assert rc == mfv.MFV_ERR_INVALID and "alpha" in lib.mfv_last_error(C.case.ctx.h).decode()
+
+
+def test_relax_field(case):
+    C = case
+    rng = np.random.default_rng(12)
+    p, prev = rng.normal(size=C.nC), rng.normal(size=C.nC)
+    for alpha in (0.3, 0.7, 1.0, 0.0):
+        dp = G.dev(p)
+        assert mfv.lib().mfv_relax_field(C.case.ctx.h, C.nC, ptr(dp), ptr(G.dev(prev)), alpha) == 0
+        assert np.array_equal(G.host(dp), O.relax_field(p, prev, alpha)), alpha
