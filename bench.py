@@ -322,7 +322,8 @@ def gpu_run(args):
 
 
 def predictor_run(ch, ctx, patches, reps=5):
-    """The SIMPLE pressure predictor's glue (UEqn.relax() on the momentum matrix, then pEqn.cu
+    """The SIMPLE pressure predictor's glue (UEqn.relax() on the momentum matrix and linearUpwind's
+    deferred correction, then pEqn.cu
     pressurePredictor: A()'s folded diagonal and rAU,
     HbyA = rAU*H() per component, constrainHbyA, phiHbyA with its boundary faces, adjustPhi) on the
     channel, with a synthetic momentum matrix: upwind convection of the channel's flux made diagonally
@@ -358,6 +359,8 @@ def predictor_run(ch, ctx, patches, reps=5):
             adjustable[start - F:start - F + size] = 1
     A = mfv.ldu(mD, mU, mL)
     D, rAU, phi, phiB, R, delta, icMag, icMin = z(N), z(N), z(F), z(B), z(N), z(N), z(B), z(B)
+    gU = [rnd(3 * N) for _ in range(3)]
+    corrU = [z(N) for _ in range(3)]
     bd = [z(B) for _ in range(3)]
     HbyA = [z(N) for _ in range(3)]
     Hb = [z(B) for _ in range(3)]
@@ -381,6 +384,10 @@ def predictor_run(ch, ctx, patches, reps=5):
         timed("relaxK (diagonal, 3 sources)",
               lambda: ctx.call("mfv_relax_diag", mesh, ctypes.byref(A), P(ic[0]), 0.7, None, P(icMag), P(icMin),
                                P(R), P(delta), 3, mfv.ptr3(U), mfv.ptr3(src)))
+        # linearUpwind's deferred correction of div(phi, U), the three components in one pass
+        timed("linearUpwindCorrK (3 components)",
+              lambda: ctx.call("mfv_linear_upwind_corr", mesh, P(ch.phiHbyA), P(ch.Cf), P(ch.C), 3, mfv.ptr3(gU),
+                               mfv.ptr3(corrU)))
         timed("foldBoundaryDiagK", lambda: ctx.call("mfv_fold_boundary_diag", mesh, P(R), *(P(a) for a in ic), P(D)))
         timed("reciprocalVK", lambda: ctx.call("mfv_reciprocal_v", mesh, P(D), P(ch.V), P(rAU)))
         for k in range(3):
@@ -404,6 +411,7 @@ def predictor_run(ch, ctx, patches, reps=5):
     torch.cuda.synchronize()
     alg = {"cmpt3K (max|iC|, min iC)": 2 * 32 * B,
            "relaxK (diagonal, 3 sources)": 16 * F + 4 * F + 8 * N + 16 * N + 3 * 24 * N + 16 * B,
+           "linearUpwindCorrK (3 components)": 8 * F + 24 * F + 8 * F + 24 * N + 72 * N + 24 * N,
            "foldBoundaryDiagK": 16 * N + 4 * N + 24 * B + 4 * B, "reciprocalVK": 24 * N,
            "boundaryDiagTermK": 32 * B,
            "matrixH3K (HbyA, 3 components)": 16 * F + 8 * F + 3 * 24 * N + 16 * N + 3 * 16 * B,
@@ -421,7 +429,7 @@ def predictor_run(ch, ctx, patches, reps=5):
     log("pressure predictor glue: %.3f ms per step (%s)" % (total_ms, ", ".join(
         "%s %.1f us" % (k, v["avg_us"]) for k, v in kernels.items())))
     return {"ms_per_step": round(total_ms, 3), "mass_correction": massCorr[-1], "per_kernel": kernels,
-            "workload": "UEqn relax, rAU, HbyA (3 components), constrainHbyA, phiHbyA, adjustPhi on the %dx%dx%d "
+            "workload": "UEqn relax, linearUpwind correction, rAU, HbyA (3 components), constrainHbyA, phiHbyA, adjustPhi on the %dx%dx%d "
                         "channel"
                         % ch.shape}
 

@@ -981,6 +981,19 @@ int mfv_relax_diag(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* d
     });
 }
 
+int mfv_linear_upwind_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double* dCf, const double* dC,
+                           int nComp, const double* const* dGrad, double* const* dCorr)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(nComp >= 1 && nComp <= 3, "linearUpwind correction: nComp must be 1..3");
+        need(dGrad && dCorr, "linearUpwind correction: null gradient/correction");
+        for (int k = 0; k < nComp; ++k) need(dGrad[k] && dCorr[k], "linearUpwind correction: null gradient/correction");
+        need(mesh->m.nFaces == 0 || (dPhi && dCf && dC), "linearUpwind correction: null phi/Cf/C");
+        launchLinearUpwindCorr(ctx->c, mesh->m, dPhi, dCf, dC, nComp, dGrad, dCorr);
+    });
+}
+
 int mfv_cmpt_max_mag3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut)
 {
     return guarded(ctx, [&] {

@@ -300,6 +300,138 @@ __global__ __launch_bounds__(kTpb) void relaxK(int n, SlabPart part, int nF, con
     for (int k = 0; k < a.nComp; ++k) a.source[k][c] += dd * a.psi[k][c];
 }
 
+// linearUpwind's deferred correction of div(phi, U_k) for NC components (fvm.cuh:395-433;
+// device_deferred_correction.cu linearUpwindCorrKernel): per cell, over its internal faces in face order
+// (the losort run, then the owned run), fc = phi * ((Cf - C_up) . grad_k(up)) with up the owner where
+// phi > 0, subtracted on a neighbour's face and added on an owned one. The dot product is x + y + z,
+// uncontracted, as the host source writes it. The first kBatch faces of each run are loaded together
+// (clamped indices, dropped from the sum past the run's end), the rest in a loop.
+struct LinUpArgs
+{
+    const double* grad[3];   // xyz per cell
+    double* corr[3];
+};
+
+// one face's fc for NC components: the upwind cell's centre and gradient are selected between this
+// cell's (held in registers) and the other cell's, both loaded before phi is known
+template <int NC>
+struct LinUpCell
+{
+    double x, y, z, g[NC][3];
+};
+
+template <int NC>
+__device__ __forceinline__ void linUpLoad(int cell, const double* __restrict__ C, const LinUpArgs& a,
+                                          LinUpCell<NC>& o)
+{
+    o.x = C[3 * (size_t)cell];
+    o.y = C[3 * (size_t)cell + 1];
+    o.z = C[3 * (size_t)cell + 2];
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) o.g[k][i] = a.grad[k][3 * (size_t)cell + i];
+}
+
+template <int NC>
+__device__ __forceinline__ void linUpFace(double pf, bool otherUp, const LinUpCell<NC>& self,
+                                          const LinUpCell<NC>& other, double fx, double fy, double fz,
+                                          double (&fc)[NC])
+{
+    const double dx = fx - (otherUp ? other.x : self.x);
+    const double dy = fy - (otherUp ? other.y : self.y);
+    const double dz = fz - (otherUp ? other.z : self.z);
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+    {
+        const double gx = otherUp ? other.g[k][0] : self.g[k][0];
+        const double gy = otherUp ? other.g[k][1] : self.g[k][1];
+        const double gz = otherUp ? other.g[k][2] : self.g[k][2];
+        fc[k] = pf * (dx * gx + dy * gy + dz * gz);
+    }
+}
+
+template <bool Slab, int NC>
+__global__ __launch_bounds__(kTpb) void linearUpwindCorrK(int n, SlabPart part, int nF,
+                                                          const label* __restrict__ ownerStart,
+                                                          const label* __restrict__ losortStart,
+                                                          const label* __restrict__ losort,
+                                                          const label* __restrict__ losortOwner,
+                                                          const label* __restrict__ neighbour,
+                                                          const double* __restrict__ phi,
+                                                          const double* __restrict__ Cf,
+                                                          const double* __restrict__ C, LinUpArgs a)
+{
+    const int c = gatherCell<Slab>(n, part);
+    if (c < 0) return;
+    const int l0 = losortStart[c], l1 = losortStart[c + 1];
+    const int f0 = ownerStart[c], f1 = ownerStart[c + 1];
+    LinUpCell<NC> self;
+    linUpLoad<NC>(c, C, a, self);
+    double s[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) s[k] = 0.0;
+    // the losort run (this cell the neighbour; the owner is upwind where phi > 0): fc subtracted
+    for (int j0 = l0; j0 < l1; j0 += kBatch)
+    {
+        int lf[kBatch];
+        LinUpCell<NC> o[kBatch];
+        double pl[kBatch], fx[kBatch], fy[kBatch], fz[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            const int j = min(j0 + u, l1 - 1);
+            lf[u] = losort[j];
+            linUpLoad<NC>(losortOwner[j], C, a, o[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            pl[u] = phi[lf[u]];
+            fx[u] = Cf[3 * (size_t)lf[u]];
+            fy[u] = Cf[3 * (size_t)lf[u] + 1];
+            fz[u] = Cf[3 * (size_t)lf[u] + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            double fc[NC];
+            linUpFace<NC>(pl[u], pl[u] > 0.0, self, o[u], fx[u], fy[u], fz[u], fc);
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                if (j0 + u < l1) s[k] -= fc[k];
+        }
+    }
+    // the owned run (this cell upwind where phi > 0): fc added
+    for (int g0 = f0; g0 < f1; g0 += kBatch)
+    {
+        int of[kBatch];
+        LinUpCell<NC> o[kBatch];
+        double po[kBatch], fx[kBatch], fy[kBatch], fz[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            of[u] = min(g0 + u, f1 - 1);
+            po[u] = phi[of[u]];
+            fx[u] = Cf[3 * (size_t)of[u]];
+            fy[u] = Cf[3 * (size_t)of[u] + 1];
+            fz[u] = Cf[3 * (size_t)of[u] + 2];
+            linUpLoad<NC>(neighbour[of[u]], C, a, o[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            double fc[NC];
+            linUpFace<NC>(po[u], !(po[u] > 0.0), self, o[u], fx[u], fy[u], fz[u], fc);
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                if (g0 + u < f1) s[k] += fc[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) a.corr[k][c] = s[k];
+}
+
 // per element: out = max(|a|, |b|, |c|) (cmptMax(cmptMag)), or min(a, b, c) (cmptMin); cf_types.cuh:199-202
 template <bool Min>
 __global__ __launch_bounds__(kTpb) void cmpt3K(int n, const double* __restrict__ x, const double* __restrict__ y,
@@ -584,6 +716,40 @@ void launchRelaxDiag(Ctx& c, const Mesh& m, const double* upper, const double* l
                                                                m.losortStart.p, m.losort.p, m.bndStart.p,
                                                                m.bndFaces.p, upper, lower, diag, a, relaxed, delta);
     hipCheck(hipGetLastError(), "relax diag");
+}
+
+template <int NC>
+static void launchLinUp(Ctx& c, const Mesh& m, const double* phi, const double* Cf, const double* C,
+                        const LinUpArgs& a)
+{
+    const SlabPart part = gatherPart(m);
+    if (part.plane)
+        linearUpwindCorrK<true, NC><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p, phi,
+            Cf, C, a);
+    else
+        linearUpwindCorrK<false, NC><<<blocks(m.nCells), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p, phi,
+            Cf, C, a);
+}
+
+void launchLinearUpwindCorr(Ctx& c, const Mesh& m, const double* phi, const double* Cf, const double* C, int nComp,
+                            const double* const* grad, double* const* corr)
+{
+    if (m.nCells == 0) return;
+    LinUpArgs a{};
+    for (int k = 0; k < nComp; ++k)
+    {
+        a.grad[k] = grad[k];
+        a.corr[k] = corr[k];
+    }
+    if (nComp == 3)
+        launchLinUp<3>(c, m, phi, Cf, C, a);
+    else if (nComp == 2)
+        launchLinUp<2>(c, m, phi, Cf, C, a);
+    else
+        launchLinUp<1>(c, m, phi, Cf, C, a);
+    hipCheck(hipGetLastError(), "linearUpwind correction");
 }
 
 void launchCmpt3(Ctx& c, int n, const double* x, const double* y, const double* z, bool min, double* out)

@@ -226,6 +226,8 @@ void launchRelaxDiag(Ctx& c, const Mesh& m, const double* upper, const double* l
                      const double* icBnd, const double* icMag, const double* icMin, const double* cycSumOff,
                      double alpha, int nComp, const double* const* psi, double* const* source, double* relaxed,
                      double* delta);
+void launchLinearUpwindCorr(Ctx& c, const Mesh& m, const double* phi, const double* Cf, const double* C, int nComp,
+                            const double* const* grad, double* const* corr);
 void launchCmpt3(Ctx& c, int n, const double* x, const double* y, const double* z, bool min, double* out);
 void launchRelaxField(Ctx& c, int n, double* p, const double* pPrev, double alpha);
 void launchConstrainHbyA(Ctx& c, const Mesh& m, const int* takeU, const double* Ub, const double* HbyA,

@@ -139,6 +139,7 @@ _SIGS = {
     "mfv_boundary_flux": [P, P, P, P, P, P, P, P],
     "mfv_adjust_phi": [P, I, P, P, I, P, P, P],
     "mfv_relax_diag": [P, P, P, P, D, P, P, P, P, P, I, P, P],
+    "mfv_linear_upwind_corr": [P, P, P, P, P, I, P, P],
     "mfv_cmpt_max_mag3": [P, I, P, P, P, P],
     "mfv_cmpt_min3": [P, I, P, P, P, P],
     "mfv_relax_field": [P, I, P, P, D],

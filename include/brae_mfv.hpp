@@ -451,6 +451,36 @@ inline void deviceRelaxMomentum(const DeviceLduView& A, const DeviceMesh& dm, co
              ctx, "deviceRelaxMomentum");
 }
 
+// deviceLinearUpwindCorr (src/cuda/device_simple.cuh:34-36): linearUpwind's deferred correction of
+// div(phi, U_k) per cell; the caller does source -= corrSource. grad is xyz per cell, the layout
+// deviceGaussGrad writes here (the reference takes three arrays gx, gy, gz).
+inline void deviceLinearUpwindCorr(const DeviceMesh& dm, const DeviceBuffer<scalar>& phiInt,
+                                   const DeviceBuffer<scalar>& grad, DeviceBuffer<scalar>& corrSource)
+{
+    mfv_ctx ctx = threadContext();
+    corrSource.resize(dm.nCells);
+    const scalar* g[1] = {grad.data()};
+    scalar* out[1] = {corrSource.data()};
+    mfvCheck(mfv_linear_upwind_corr(ctx, dm.mesh, phiInt.data(), dm.Cf.data(), dm.C.data(), 1, g, out), ctx,
+             "deviceLinearUpwindCorr");
+}
+
+// the three components' corrections in one pass over the faces (three deviceLinearUpwindCorr calls' values)
+inline void deviceLinearUpwindCorr3(const DeviceMesh& dm, const DeviceBuffer<scalar>& phiInt,
+                                    const DeviceBuffer<scalar>* grad, DeviceBuffer<scalar>* corrSource)
+{
+    mfv_ctx ctx = threadContext();
+    const scalar* g[3] = {grad[0].data(), grad[1].data(), grad[2].data()};
+    scalar* out[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        corrSource[k].resize(dm.nCells);
+        out[k] = corrSource[k].data();
+    }
+    mfvCheck(mfv_linear_upwind_corr(ctx, dm.mesh, phiInt.data(), dm.Cf.data(), dm.C.data(), 3, g, out), ctx,
+             "deviceLinearUpwindCorr3");
+}
+
 // deviceCmptMaxMag3 / deviceCmptMin3 (src/cuda/device_simple.cuh:123-128)
 inline void deviceCmptMaxMag3(const DeviceBuffer<scalar>& a, const DeviceBuffer<scalar>& b,
                               const DeviceBuffer<scalar>& c, DeviceBuffer<scalar>& out)

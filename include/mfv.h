@@ -167,6 +167,14 @@ int mfv_matrix_h3(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* co
 int mfv_relax_diag(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dICbnd, double alpha,
                    const double* dCycSumOff, const double* dICmaxMag, const double* dICmin, double* dRelaxedDiag,
                    double* dDelta, int nComp, const double* const* dPsi, double* const* dSource);
+/* deviceLinearUpwindCorr (src/cuda/device_simple.cuh:34-36; fvm.cuh:395-433): linearUpwind's deferred
+ * correction of div(phi, U_k), corr = sum over internal faces of +-phi*((Cf - C_up) . grad_k(up)) (+ on the
+ * owner, - on the neighbour; up the owner where phi > 0), for nComp (1..3) components in one pass. dGrad[k]
+ * xyz per cell (mfv_grad_gauss's layout), dCf and dC mfv_mesh_geometry's. Uncoupled boundary faces add
+ * nothing. The caller subtracts dCorr[k] from component k's source (fvMatrix += surfaceIntegrate). Face
+ * order per cell, uncontracted: the host loop's bits. */
+int mfv_linear_upwind_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double* dCf, const double* dC,
+                           int nComp, const double* const* dGrad, double* const* dCorr);
 /* deviceCmptMaxMag3 / deviceCmptMin3 (src/cuda/device_simple.cuh:123-128): per element max(|a|,|b|,|c|) and
  * min(a,b,c) */
 int mfv_cmpt_max_mag3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut);

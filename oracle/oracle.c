@@ -2155,6 +2155,29 @@ void orc_relax_matrix(int nCells, int nFaces, const int* owner, const int* neigh
     free(sumOff);
 }
 
+/* linearUpwind's deferred correction of div(phi, psi) for one component (fvm.cuh:395-433): per internal
+ * face, up = owner if phi > 0 else neighbour, fc = phi * ((Cf - C_up) . grad_up) with the dot product
+ * x + y + z as written; corr[owner] += fc, corr[neighbour] -= fc in face order. Boundary faces of uncoupled
+ * patches contribute nothing. Cf [3*nFaces], C and grad [3*nCells], xyz interleaved. The caller subtracts
+ * corr from the source. */
+void orc_linear_upwind_corr(int nCells, int nFaces, const int* owner, const int* neighbour, const double* phi,
+                            const double* Cf, const double* C, const double* grad, double* corr)
+{
+    memset(corr, 0, sizeof(double) * (size_t)nCells);
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const double pf = phi[f];
+        const int up = (pf > 0.0) ? owner[f] : neighbour[f];
+        const double dx = Cf[3 * (size_t)f] - C[3 * (size_t)up];
+        const double dy = Cf[3 * (size_t)f + 1] - C[3 * (size_t)up + 1];
+        const double dz = Cf[3 * (size_t)f + 2] - C[3 * (size_t)up + 2];
+        const double* g = grad + 3 * (size_t)up;
+        const double fc = pf * (dx * g[0] + dy * g[1] + dz * g[2]);
+        corr[owner[f]] += fc;
+        corr[neighbour[f]] -= fc;
+    }
+}
+
 /* cmptMax(cmptMag(v)) and cmptMin(v) of the internalCoeffs per boundary face (cf_types.cuh:199-202) */
 void orc_cmpt_mag_max_min(int n, const double* x, const double* y, const double* z, double* mag, double* mn)
 {

@@ -67,6 +67,7 @@ def lib():
             "orc_relax_matrix": (None, [I, I, P, P, P, P, I, P, P, P, P, D, P, I, P, P, P]),
             "orc_cmpt_mag_max_min": (None, [I, P, P, P, P, P]),
             "orc_relax_field": (None, [I, P, P, D]),
+            "orc_linear_upwind_corr": (None, [I, I, P, P, P, P, P, P, P]),
             "orc_grad_gauss": (None, [I, I, P, P, I, P, P, P, P, P, P, P, P]),
             "orc_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
             "orc_sym_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
@@ -440,3 +441,11 @@ def relax_field(pNew, pPrev, alpha):
     out = f64(pNew).copy()
     lib().orc_relax_field(len(out), p(out), p(f64(pPrev)), alpha)
     return out
+
+
+def linear_upwind_corr(nC, own, nei, phi, Cf, C, grad):
+    """linearUpwind's correction per cell for one component; grad [nC, 3]"""
+    corr = np.zeros(nC)
+    lib().orc_linear_upwind_corr(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(phi)), p(f64(Cf).ravel()),
+                                 p(f64(C).ravel()), p(f64(grad).ravel()), p(corr))
+    return corr

@@ -232,3 +232,26 @@ def test_relax_field(case):
         dp = G.dev(p)
         assert mfv.lib().mfv_relax_field(C.case.ctx.h, C.nC, ptr(dp), ptr(G.dev(prev)), alpha) == 0
         assert np.array_equal(G.host(dp), O.relax_field(p, prev, alpha)), alpha
+
+
+def test_linear_upwind_corr_bit_exact(case):
+    """linearUpwind's deferred correction: three components in one pass and one at a time, the oracle's
+    face-order sums to the bit (zero fluxes included: the neighbour is the upwind cell there)"""
+    C = case
+    print("case:", C.nC, flush=True)
+    rng = np.random.default_rng(31)
+    phi = rng.normal(size=C.nI) * C.m["magSf"][:C.nI]
+    phi[::7] = 0.0
+    grads = [rng.normal(size=(C.nC, 3)) for _ in range(3)]
+    Cf, Cc = O.f64(C.m["Cf"]), O.f64(C.m["C"])
+    dphi, dCf, dC = G.dev(phi), G.dev(Cf.ravel()), G.dev(Cc.ravel())
+    dg = [G.dev(g.ravel()) for g in grads]
+    refs = [O.linear_upwind_corr(C.nC, C.own, C.nei, phi, Cf[:C.nI], Cc, g) for g in grads]
+    outs = [G.dev(np.full(C.nC, 9.0)) for _ in range(3)]
+    C.call("mfv_linear_upwind_corr", ptr(dphi), ptr(dCf), ptr(dC), 3, mfv.ptr3(dg), mfv.ptr3(outs))
+    for k in range(3):
+        assert np.array_equal(G.host(outs[k]), refs[k]), k
+        one = G.zeros(C.nC)
+        C.call("mfv_linear_upwind_corr", ptr(dphi), ptr(dCf), ptr(dC), 1, (ctypes.c_void_p * 1)(dg[k].data_ptr()),
+               (ctypes.c_void_p * 1)(one.data_ptr()))
+        assert np.array_equal(G.host(one), refs[k]), k
