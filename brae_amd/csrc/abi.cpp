@@ -994,6 +994,30 @@ int mfv_linear_upwind_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const
     });
 }
 
+int mfv_scaled_dev2t(mfv_ctx ctx, int n, const double* dGradU, const double* dNu, double* dSigma)
+{
+    return guarded(ctx, [&] {
+        need(n >= 0, "scaled dev2: negative size");
+        need(n == 0 || (dGradU && dNu && dSigma), "scaled dev2: null argument");
+        need(dGradU != dSigma, "scaled dev2: sigma must not alias gradU (the transpose reads across rows)");
+        launchScaledDev2T(ctx->c, n, dGradU, dNu, dSigma);
+    });
+}
+
+int mfv_tensor_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dSf,
+                          const int* dIsEmpty, const double* dTcell, const double* dTbnd, double* dSrcX,
+                          double* dSrcY, double* dSrcZ)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dTcell && dSrcX && dSrcY && dSrcZ, "tensor div: null argument");
+        need(mesh->m.nFaces + mesh->m.nBFaces == 0 || dSf, "tensor div: null Sf");
+        need(mesh->m.nFaces == 0 || dWeights, "tensor div: null weights");
+        need(mesh->m.nBFaces == 0 || dTbnd, "tensor div: null boundary tensors");
+        launchTensorDiv(ctx->c, mesh->m, dWeights, dSf, dIsEmpty, dTcell, dTbnd, dSrcX, dSrcY, dSrcZ);
+    });
+}
+
 int mfv_cmpt_max_mag3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut)
 {
     return guarded(ctx, [&] {

@@ -432,6 +432,140 @@ __global__ __launch_bounds__(kTpb) void linearUpwindCorrK(int n, SlabPart part, 
     for (int k = 0; k < NC; ++k) a.corr[k][c] = s[k];
 }
 
+// nuEff*dev2(T(grad(U))) per cell (linearViscousStress_cpp.cu:75-79; device_divdevreff.cu sigmaKernel):
+// tensors component-major t[(i*3 + j)*n + c], gradU_ij = d(U_j)/d(x_i); sigma_ij = nu*(gradU_ji -
+// (2/3)*tr on the diagonal)
+__global__ __launch_bounds__(kTpb) void dev2tK(int n, const double* __restrict__ gradU, const double* __restrict__ nu,
+                                               double* __restrict__ sigma)
+{
+    const int c = xcdBlock() * kTpb + threadIdx.x;
+    if (c >= n) return;
+    double g[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] = gradU[q * (size_t)n + c];
+    const double k = (2.0 / 3.0) * (g[0] + g[4] + g[8]);
+    const double v = nu[c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            const double t = g[j * 3 + i];
+            sigma[(i * 3 + j) * (size_t)n + c] = v * (i == j ? t - k : t);
+        }
+}
+
+// V*fvc::div of a tensor field (fvc.cu:867-905 before the division by V; device_divdevreff.cu
+// tensorDivKernel): per cell, its internal faces in face order -- the losort run subtracted, the owned
+// run added -- with T_f = w*T_own + (1-w)*T_nei and (Sf & T_f)_j = Sx*T_xj + Sy*T_yj + Sz*T_zj, then its
+// boundary faces not marked empty, Sf_b & T_b added. The cell's own tensor stays in registers; the
+// other cell's tensor, w and Sf of up to kBatch faces are loaded together (clamped indices).
+__device__ __forceinline__ void tensorFaceFlux(double wf, bool selfOwns, const double (&ts)[9],
+                                               const double (&to)[9], double sx, double sy, double sz,
+                                               double (&r)[3])
+{
+    double tf[9];
+    const double wn = 1.0 - wf;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tf[q] = selfOwns ? wf * ts[q] + wn * to[q] : wf * to[q] + wn * ts[q];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r[j] = sx * tf[j] + sy * tf[3 + j] + sz * tf[6 + j];
+}
+
+template <bool Slab>
+__global__ __launch_bounds__(kTpb) void tensorDivK(int n, SlabPart part, int nF, int nB,
+                                                   const label* __restrict__ ownerStart,
+                                                   const label* __restrict__ losortStart,
+                                                   const label* __restrict__ losort,
+                                                   const label* __restrict__ losortOwner,
+                                                   const label* __restrict__ neighbour,
+                                                   const label* __restrict__ bndStart,
+                                                   const label* __restrict__ bndFaces,
+                                                   const int* __restrict__ isEmpty, const double* __restrict__ w,
+                                                   const double* __restrict__ Sf, const double* __restrict__ Tc,
+                                                   const double* __restrict__ Tb, double* __restrict__ dX,
+                                                   double* __restrict__ dY, double* __restrict__ dZ)
+{
+    const int c = gatherCell<Slab>(n, part);
+    if (c < 0) return;
+    const int l0 = losortStart[c], l1 = losortStart[c + 1];
+    const int f0 = ownerStart[c], f1 = ownerStart[c + 1];
+    const int b0 = bndStart[c], b1 = bndStart[c + 1];
+    double ts[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ts[q] = Tc[q * (size_t)n + c];
+    double d[3] = {0.0, 0.0, 0.0};
+    for (int j0 = l0; j0 < l1; j0 += kBatch)
+    {
+        int lf[kBatch];
+        double to[kBatch][9], wf[kBatch], sx[kBatch], sy[kBatch], sz[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            const int j = min(j0 + u, l1 - 1);
+            lf[u] = losort[j];
+            const int o = losortOwner[j];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) to[u][q] = Tc[q * (size_t)n + o];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            wf[u] = w[lf[u]];
+            sx[u] = Sf[3 * (size_t)lf[u]];
+            sy[u] = Sf[3 * (size_t)lf[u] + 1];
+            sz[u] = Sf[3 * (size_t)lf[u] + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            double r[3];
+            tensorFaceFlux(wf[u], false, ts, to[u], sx[u], sy[u], sz[u], r);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (j0 + u < l1) d[k] = d[k] - r[k];
+        }
+    }
+    for (int g0 = f0; g0 < f1; g0 += kBatch)
+    {
+        double to[kBatch][9], wf[kBatch], sx[kBatch], sy[kBatch], sz[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            const int f = min(g0 + u, f1 - 1);
+            const int nb = neighbour[f];
+            wf[u] = w[f];
+            sx[u] = Sf[3 * (size_t)f];
+            sy[u] = Sf[3 * (size_t)f + 1];
+            sz[u] = Sf[3 * (size_t)f + 2];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) to[u][q] = Tc[q * (size_t)n + nb];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            double r[3];
+            tensorFaceFlux(wf[u], true, ts, to[u], sx[u], sy[u], sz[u], r);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (g0 + u < f1) d[k] += r[k];
+        }
+    }
+    for (int j = b0; j < b1; ++j)
+    {
+        const int i = bndFaces[j];
+        if (isEmpty && isEmpty[i]) continue;
+        const double* s = Sf + 3 * ((size_t)nF + i);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            d[k] += s[0] * Tb[(0 * 3 + k) * (size_t)nB + i] + s[1] * Tb[(1 * 3 + k) * (size_t)nB + i] +
+                    s[2] * Tb[(2 * 3 + k) * (size_t)nB + i];
+    }
+    dX[c] = d[0];
+    dY[c] = d[1];
+    dZ[c] = d[2];
+}
+
 // per element: out = max(|a|, |b|, |c|) (cmptMax(cmptMag)), or min(a, b, c) (cmptMin); cf_types.cuh:199-202
 template <bool Min>
 __global__ __launch_bounds__(kTpb) void cmpt3K(int n, const double* __restrict__ x, const double* __restrict__ y,
@@ -750,6 +884,29 @@ void launchLinearUpwindCorr(Ctx& c, const Mesh& m, const double* phi, const doub
     else
         launchLinUp<1>(c, m, phi, Cf, C, a);
     hipCheck(hipGetLastError(), "linearUpwind correction");
+}
+
+void launchScaledDev2T(Ctx& c, int n, const double* gradU, const double* nu, double* sigma)
+{
+    if (n <= 0) return;
+    dev2tK<<<blocks(n), kTpb, 0, c.stream>>>(n, gradU, nu, sigma);
+    hipCheck(hipGetLastError(), "nu*dev2(T(gradU))");
+}
+
+void launchTensorDiv(Ctx& c, const Mesh& m, const double* w, const double* Sf, const int* isEmpty, const double* Tc,
+                     const double* Tb, double* dX, double* dY, double* dZ)
+{
+    if (m.nCells == 0) return;
+    const SlabPart part = gatherPart(m);
+    if (part.plane)
+        tensorDivK<true><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.nBFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p,
+            m.neighbour.p, m.bndStart.p, m.bndFaces.p, isEmpty, w, Sf, Tc, Tb, dX, dY, dZ);
+    else
+        tensorDivK<false><<<blocks(m.nCells), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.nBFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p,
+            m.neighbour.p, m.bndStart.p, m.bndFaces.p, isEmpty, w, Sf, Tc, Tb, dX, dY, dZ);
+    hipCheck(hipGetLastError(), "tensor divergence");
 }
 
 void launchCmpt3(Ctx& c, int n, const double* x, const double* y, const double* z, bool min, double* out)

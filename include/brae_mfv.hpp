@@ -481,6 +481,33 @@ inline void deviceLinearUpwindCorr3(const DeviceMesh& dm, const DeviceBuffer<sca
              "deviceLinearUpwindCorr3");
 }
 
+// deviceScaledDev2T (src/cuda/device_divdevreff.cuh:102-103): sigma = nu*dev2(T(gradU)), tensors packed
+// [(i*3 + j)*n + c]
+inline void deviceScaledDev2T(int n, const DeviceBuffer<scalar>& gradU, const DeviceBuffer<scalar>& nu,
+                              DeviceBuffer<scalar>& sigma)
+{
+    mfv_ctx ctx = threadContext();
+    sigma.resize(9 * static_cast<std::size_t>(n));
+    mfvCheck(mfv_scaled_dev2t(ctx, n, gradU.data(), nu.data(), sigma.data()), ctx, "deviceScaledDev2T");
+}
+
+// deviceTensorDivSource (src/cuda/device_divdevreff.cuh:104-107): V*fvc::div(T) per cell from the cell
+// tensors and the boundary faces' tensors (both packed [(i*3 + j)*n + e]); empty faces per isEmpty (may be
+// nullptr). Cyclic and AMI interfaces are not carried.
+inline void deviceTensorDivSource(const DeviceMesh& dm, const DeviceBuffer<scalar>& Tcell,
+                                  const DeviceBuffer<scalar>& Tbnd, DeviceBuffer<scalar>& srcX,
+                                  DeviceBuffer<scalar>& srcY, DeviceBuffer<scalar>& srcZ,
+                                  const DeviceBuffer<int>* isEmpty = nullptr)
+{
+    mfv_ctx ctx = threadContext();
+    srcX.resize(dm.nCells);
+    srcY.resize(dm.nCells);
+    srcZ.resize(dm.nCells);
+    mfvCheck(mfv_tensor_div_source(ctx, dm.mesh, dm.weights.data(), dm.Sf.data(), isEmpty ? isEmpty->data() : nullptr,
+                                   Tcell.data(), Tbnd.data(), srcX.data(), srcY.data(), srcZ.data()),
+             ctx, "deviceTensorDivSource");
+}
+
 // deviceCmptMaxMag3 / deviceCmptMin3 (src/cuda/device_simple.cuh:123-128)
 inline void deviceCmptMaxMag3(const DeviceBuffer<scalar>& a, const DeviceBuffer<scalar>& b,
                               const DeviceBuffer<scalar>& c, DeviceBuffer<scalar>& out)

@@ -175,6 +175,18 @@ int mfv_relax_diag(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* d
  * order per cell, uncontracted: the host loop's bits. */
 int mfv_linear_upwind_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const double* dCf, const double* dC,
                            int nComp, const double* const* dGrad, double* const* dCorr);
+/* deviceScaledDev2T (src/cuda/device_divdevreff.cuh:102-103; linearViscousStress_cpp.cu:75-79): sigma = nu*dev2(T(gradU))
+ * per cell, tensors component-major [(i*3 + j)*n + c] with gradU_ij = d(U_j)/d(x_i); dSigma must not
+ * alias dGradU */
+int mfv_scaled_dev2t(mfv_ctx ctx, int n, const double* dGradU, const double* dNu, double* dSigma);
+/* deviceTensorDivSource (src/cuda/device_divdevreff.cuh:104-107; fvc.cu:867-905 before its /V): V*fvc::div(T) per
+ * cell, linear interpolation on internal faces, dTbnd [9*nBFaces] component-major on the boundary faces not
+ * marked in dIsEmpty (may be NULL); dSf xyz per face, internal then boundary (mfv_mesh_geometry's), dWeights
+ * its weights. Outputs the raw sums per component. Face order per cell, uncontracted: the host loop's bits.
+ * The boundary tensors (the corrected boundary gradient's stress) are the caller's. */
+int mfv_tensor_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dSf,
+                          const int* dIsEmpty, const double* dTcell, const double* dTbnd, double* dSrcX,
+                          double* dSrcY, double* dSrcZ);
 /* deviceCmptMaxMag3 / deviceCmptMin3 (src/cuda/device_simple.cuh:123-128): per element max(|a|,|b|,|c|) and
  * min(a,b,c) */
 int mfv_cmpt_max_mag3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut);

@@ -2178,6 +2178,60 @@ void orc_linear_upwind_corr(int nCells, int nFaces, const int* owner, const int*
     }
 }
 
+/* nuEff*dev2(T(grad(U))) per cell (linearViscousStress_cpp.cu:75-79, cf_types.cuh:174-179): tensors
+ * component-major, t[(i*3 + j)*n + c]; gradU_ij = d(U_j)/d(x_i); T(gradU)_ij = gradU_ji; dev2 subtracts
+ * (2/3)*tr on the diagonal, tr = xx + yy + zz */
+void orc_scaled_dev2t(int n, const double* gradU, const double* nu, double* sigma)
+{
+    for (int c = 0; c < n; ++c)
+    {
+        const double tr = gradU[0 * (size_t)n + c] + gradU[4 * (size_t)n + c] + gradU[8 * (size_t)n + c];
+        const double k = (2.0 / 3.0) * tr;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+            {
+                const double t = gradU[(j * 3 + i) * (size_t)n + c];
+                sigma[(i * 3 + j) * (size_t)n + c] = nu[c] * (i == j ? t - k : t);
+            }
+    }
+}
+
+/* V*fvc::div of a tensor field (fvc.cu:867-905 before its division by V): per internal face
+ * T_f = w*T_own + (1-w)*T_nei, (Sf & T_f)_j = Sx*T_xj + Sy*T_yj + Sz*T_zj, added to the owner and
+ * subtracted from the neighbour in face order; then per boundary face not marked empty, Sf_b & T_b added
+ * in boundary order. Tcell [9*nC] and Tbnd [9*nB] component-major; Sf xyz per face, internal then boundary
+ * (the boundary's at 3*(nFaces + i)). isEmpty may be NULL. */
+void orc_tensor_div(int nCells, int nFaces, const int* owner, const int* neighbour, const double* w,
+                    const double* Sf, int nBFaces, const int* bFaceCell, const int* isEmpty, const double* Tcell,
+                    const double* Tbnd, double* dX, double* dY, double* dZ)
+{
+    double* d[3] = {dX, dY, dZ};
+    for (int j = 0; j < 3; ++j) memset(d[j], 0, sizeof(double) * (size_t)nCells);
+    const size_t nC = (size_t)nCells, nB = (size_t)nBFaces;
+    for (int f = 0; f < nFaces; ++f)
+    {
+        const int o = owner[f], ne = neighbour[f];
+        const double wf = w[f];
+        double Tf[9];
+        for (int q = 0; q < 9; ++q) Tf[q] = wf * Tcell[q * nC + o] + (1.0 - wf) * Tcell[q * nC + ne];
+        const double* s = Sf + 3 * (size_t)f;
+        for (int j = 0; j < 3; ++j)
+        {
+            const double r = s[0] * Tf[0 * 3 + j] + s[1] * Tf[1 * 3 + j] + s[2] * Tf[2 * 3 + j];
+            d[j][o] += r;
+            d[j][ne] = d[j][ne] - r;
+        }
+    }
+    for (int i = 0; i < nBFaces; ++i)
+    {
+        if (isEmpty && isEmpty[i]) continue;
+        const double* s = Sf + 3 * ((size_t)nFaces + i);
+        for (int j = 0; j < 3; ++j)
+            d[j][bFaceCell[i]] +=
+                s[0] * Tbnd[(0 * 3 + j) * nB + i] + s[1] * Tbnd[(1 * 3 + j) * nB + i] + s[2] * Tbnd[(2 * 3 + j) * nB + i];
+    }
+}
+
 /* cmptMax(cmptMag(v)) and cmptMin(v) of the internalCoeffs per boundary face (cf_types.cuh:199-202) */
 void orc_cmpt_mag_max_min(int n, const double* x, const double* y, const double* z, double* mag, double* mn)
 {

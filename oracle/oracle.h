@@ -200,6 +200,10 @@ void orc_relax_matrix(int nCells, int nFaces, const int* owner, const int* neigh
                       const double* const* psi, double* const* source, double* delta);
 void orc_linear_upwind_corr(int nCells, int nFaces, const int* owner, const int* neighbour, const double* phi,
                             const double* Cf, const double* C, const double* grad, double* corr);
+void orc_scaled_dev2t(int n, const double* gradU, const double* nu, double* sigma);
+void orc_tensor_div(int nCells, int nFaces, const int* owner, const int* neighbour, const double* w,
+                    const double* Sf, int nBFaces, const int* bFaceCell, const int* isEmpty, const double* Tcell,
+                    const double* Tbnd, double* dX, double* dY, double* dZ);
 void orc_cmpt_mag_max_min(int n, const double* x, const double* y, const double* z, double* mag, double* mn);
 void orc_relax_field(int n, double* p, const double* pPrev, double alpha);
 void orc_matrix_h(int nCells, int nFaces, const int* owner, const int* neighbour, const double* upper,

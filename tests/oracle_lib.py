@@ -68,6 +68,8 @@ def lib():
             "orc_cmpt_mag_max_min": (None, [I, P, P, P, P, P]),
             "orc_relax_field": (None, [I, P, P, D]),
             "orc_linear_upwind_corr": (None, [I, I, P, P, P, P, P, P, P]),
+            "orc_scaled_dev2t": (None, [I, P, P, P]),
+            "orc_tensor_div": (None, [I, I, P, P, P, P, I, P, P, P, P, P, P, P]),
             "orc_grad_gauss": (None, [I, I, P, P, I, P, P, P, P, P, P, P, P]),
             "orc_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
             "orc_sym_gauss_seidel": (None, [I, I, P, P, P, P, P, P, I]),
@@ -449,3 +451,20 @@ def linear_upwind_corr(nC, own, nei, phi, Cf, C, grad):
     lib().orc_linear_upwind_corr(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(phi)), p(f64(Cf).ravel()),
                                  p(f64(C).ravel()), p(f64(grad).ravel()), p(corr))
     return corr
+
+
+
+def scaled_dev2t(gradU, nu):
+    """nu*dev2(T(gradU)); gradU [9, n] component-major (row q = i*3 + j is d(U_j)/d(x_i))"""
+    g = f64(gradU)
+    sig = np.zeros_like(g)
+    lib().orc_scaled_dev2t(g.shape[1], p(g), p(f64(nu)), p(sig))
+    return sig
+
+
+def tensor_div(nC, own, nei, w, Sf, bfc, Tcell, Tbnd, isEmpty=None):
+    """V*div(T): (dX, dY, dZ); Tcell [9, nC], Tbnd [9, nB]; Sf [nFaces + nB, 3]"""
+    d = [np.zeros(nC) for _ in range(3)]
+    lib().orc_tensor_div(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(w)), p(f64(Sf).ravel()), len(bfc), p(i32(bfc)),
+                         p(None if isEmpty is None else i32(isEmpty)), p(f64(Tcell)), p(f64(Tbnd)), *(p(x) for x in d))
+    return d

@@ -255,3 +255,27 @@ def test_linear_upwind_corr_bit_exact(case):
         C.call("mfv_linear_upwind_corr", ptr(dphi), ptr(dCf), ptr(dC), 1, (ctypes.c_void_p * 1)(dg[k].data_ptr()),
                (ctypes.c_void_p * 1)(one.data_ptr()))
         assert np.array_equal(G.host(one), refs[k]), k
+
+
+def test_viscous_stress_bit_exact(case):
+    """nu*dev2(T(grad U)) and V*div of the stress tensor (the explicit part of divDevReff), the oracle's
+    values to the bit; empty boundary faces dropped"""
+    C = case
+    print("case:", C.nC, flush=True)
+    rng = np.random.default_rng(41)
+    gradU = rng.normal(size=(9, C.nC))
+    nu = rng.uniform(1e-3, 1e-1, C.nC)
+    sig_ref = O.scaled_dev2t(gradU, nu)
+    dsig = G.zeros(9 * C.nC)
+    assert mfv.lib().mfv_scaled_dev2t(C.case.ctx.h, C.nC, ptr(G.dev(gradU.ravel())), ptr(G.dev(nu)), ptr(dsig)) == 0
+    assert np.array_equal(G.host(dsig).reshape(9, C.nC), sig_ref)
+    Tb = rng.normal(size=(9, C.nB))
+    empty = np.zeros(C.nB, np.int32)
+    empty[::4] = 1
+    for e in (None, empty):
+        ref = O.tensor_div(C.nC, C.own, C.nei, C.w, C.Sf, C.bfc, sig_ref, Tb, isEmpty=e)
+        out = [G.dev(np.full(C.nC, 5.0)) for _ in range(3)]
+        C.call("mfv_tensor_div_source", ptr(C.d["w"]), ptr(C.d["Sf"]), None if e is None else ptr(G.dev(e)), ptr(dsig),
+               ptr(G.dev(Tb.ravel())), *(ptr(o) for o in out))
+        for k in range(3):
+            assert np.array_equal(G.host(out[k]), ref[k]), k
