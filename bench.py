@@ -360,7 +360,7 @@ def predictor_run(ch, ctx, patches, reps=5):
     A = mfv.ldu(mD, mU, mL)
     D, rAU, phi, phiB, R, delta, icMag, icMin = z(N), z(N), z(F), z(B), z(N), z(N), z(B), z(B)
     gU = [rnd(3 * N) for _ in range(3)]
-    gradU9, nuEff, sigma, sigmaB = rnd(9 * N), rnd(N, 1e-3, 1e-1), z(9 * N), rnd(9 * B)
+    gradU9, nuEff, sigmaB = rnd(9 * N), rnd(N, 1e-3, 1e-1), rnd(9 * B)
     divSig = [z(N) for _ in range(3)]
     corrU = [z(N) for _ in range(3)]
     bd = [z(B) for _ in range(3)]
@@ -391,10 +391,10 @@ def predictor_run(ch, ctx, patches, reps=5):
               lambda: ctx.call("mfv_linear_upwind_corr", mesh, P(ch.phiHbyA), P(ch.Cf), P(ch.C), 3, mfv.ptr3(gU),
                                mfv.ptr3(corrU)))
         # divDevReff's explicit stress: nuEff*dev2(T(grad U)), then V*div of it with the boundary faces' stress
-        timed("dev2tK (stress)", lambda: ctx.call("mfv_scaled_dev2t", N, P(gradU9), P(nuEff), P(sigma)))
-        timed("tensorDivK (V div stress)",
-              lambda: ctx.call("mfv_tensor_div_source", mesh, P(ch.w), P(ch.Sf), None, P(sigma), P(sigmaB),
-                               *(P(t) for t in divSig)))
+        # (formed where each cell's stress is read: one pass, the stress never stored)
+        timed("tensorDivK (V div nuEff dev2(T(grad U)), fused)",
+              lambda: ctx.call("mfv_div_dev_reff_explicit", mesh, P(ch.w), P(ch.Sf), None, P(gradU9), P(nuEff),
+                               P(sigmaB), *(P(t) for t in divSig)))
         timed("foldBoundaryDiagK", lambda: ctx.call("mfv_fold_boundary_diag", mesh, P(R), *(P(a) for a in ic), P(D)))
         timed("reciprocalVK", lambda: ctx.call("mfv_reciprocal_v", mesh, P(D), P(ch.V), P(rAU)))
         for k in range(3):
@@ -419,8 +419,7 @@ def predictor_run(ch, ctx, patches, reps=5):
     alg = {"cmpt3K (max|iC|, min iC)": 2 * 32 * B,
            "relaxK (diagonal, 3 sources)": 16 * F + 4 * F + 8 * N + 16 * N + 3 * 24 * N + 16 * B,
            "linearUpwindCorrK (3 components)": 8 * F + 24 * F + 8 * F + 24 * N + 72 * N + 24 * N,
-           "dev2tK (stress)": 72 * N + 8 * N + 72 * N,
-           "tensorDivK (V div stress)": 8 * F + 24 * F + 8 * F + 72 * N + 24 * N + 72 * B + 24 * B,
+           "tensorDivK (V div nuEff dev2(T(grad U)), fused)": 8 * F + 24 * F + 8 * F + 80 * N + 24 * N + 72 * B + 24 * B,
            "foldBoundaryDiagK": 16 * N + 4 * N + 24 * B + 4 * B, "reciprocalVK": 24 * N,
            "boundaryDiagTermK": 32 * B,
            "matrixH3K (HbyA, 3 components)": 16 * F + 8 * F + 3 * 24 * N + 16 * N + 3 * 16 * B,

@@ -1014,7 +1014,21 @@ int mfv_tensor_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, co
         need(mesh->m.nFaces + mesh->m.nBFaces == 0 || dSf, "tensor div: null Sf");
         need(mesh->m.nFaces == 0 || dWeights, "tensor div: null weights");
         need(mesh->m.nBFaces == 0 || dTbnd, "tensor div: null boundary tensors");
-        launchTensorDiv(ctx->c, mesh->m, dWeights, dSf, dIsEmpty, dTcell, dTbnd, dSrcX, dSrcY, dSrcZ);
+        launchTensorDiv(ctx->c, mesh->m, dWeights, dSf, dIsEmpty, dTcell, nullptr, dTbnd, dSrcX, dSrcY, dSrcZ);
+    });
+}
+
+int mfv_div_dev_reff_explicit(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dSf,
+                              const int* dIsEmpty, const double* dGradU, const double* dNu, const double* dSigmaB,
+                              double* dSrcX, double* dSrcY, double* dSrcZ)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dGradU && dNu && dSrcX && dSrcY && dSrcZ, "divDevReff: null argument");
+        need(mesh->m.nFaces + mesh->m.nBFaces == 0 || dSf, "divDevReff: null Sf");
+        need(mesh->m.nFaces == 0 || dWeights, "divDevReff: null weights");
+        need(mesh->m.nBFaces == 0 || dSigmaB, "divDevReff: null boundary stress");
+        launchTensorDiv(ctx->c, mesh->m, dWeights, dSf, dIsEmpty, dGradU, dNu, dSigmaB, dSrcX, dSrcY, dSrcZ);
     });
 }
 

@@ -472,7 +472,30 @@ __device__ __forceinline__ void tensorFaceFlux(double wf, bool selfOwns, const d
     for (int j = 0; j < 3; ++j) r[j] = sx * tf[j] + sy * tf[3 + j] + sz * tf[6 + j];
 }
 
-template <bool Slab>
+// a cell's tensor: stored (Tc component-major), or nu*dev2(T(gradU)) formed from the gradient with
+// dev2tK's arithmetic, so both give the same bits
+template <bool FromGrad>
+__device__ __forceinline__ void loadTensor(int n, int cell, const double* __restrict__ T,
+                                           const double* __restrict__ nu, double (&t)[9])
+{
+    if (!FromGrad)
+    {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) t[q] = T[q * (size_t)n + cell];
+        return;
+    }
+    double g[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] = T[q * (size_t)n + cell];
+    const double k = (2.0 / 3.0) * (g[0] + g[4] + g[8]);
+    const double v = nu[cell];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t[i * 3 + j] = v * (i == j ? g[j * 3 + i] - k : g[j * 3 + i]);
+}
+
+template <bool Slab, bool FromGrad>
 __global__ __launch_bounds__(kTpb) void tensorDivK(int n, SlabPart part, int nF, int nB,
                                                    const label* __restrict__ ownerStart,
                                                    const label* __restrict__ losortStart,
@@ -483,7 +506,8 @@ __global__ __launch_bounds__(kTpb) void tensorDivK(int n, SlabPart part, int nF,
                                                    const label* __restrict__ bndFaces,
                                                    const int* __restrict__ isEmpty, const double* __restrict__ w,
                                                    const double* __restrict__ Sf, const double* __restrict__ Tc,
-                                                   const double* __restrict__ Tb, double* __restrict__ dX,
+                                                   const double* __restrict__ nu, const double* __restrict__ Tb,
+                                                   double* __restrict__ dX,
                                                    double* __restrict__ dY, double* __restrict__ dZ)
 {
     const int c = gatherCell<Slab>(n, part);
@@ -492,8 +516,7 @@ __global__ __launch_bounds__(kTpb) void tensorDivK(int n, SlabPart part, int nF,
     const int f0 = ownerStart[c], f1 = ownerStart[c + 1];
     const int b0 = bndStart[c], b1 = bndStart[c + 1];
     double ts[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) ts[q] = Tc[q * (size_t)n + c];
+    loadTensor<FromGrad>(n, c, Tc, nu, ts);
     double d[3] = {0.0, 0.0, 0.0};
     for (int j0 = l0; j0 < l1; j0 += kBatch)
     {
@@ -504,9 +527,7 @@ __global__ __launch_bounds__(kTpb) void tensorDivK(int n, SlabPart part, int nF,
         {
             const int j = min(j0 + u, l1 - 1);
             lf[u] = losort[j];
-            const int o = losortOwner[j];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) to[u][q] = Tc[q * (size_t)n + o];
+            loadTensor<FromGrad>(n, losortOwner[j], Tc, nu, to[u]);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u)
@@ -538,8 +559,7 @@ __global__ __launch_bounds__(kTpb) void tensorDivK(int n, SlabPart part, int nF,
             sx[u] = Sf[3 * (size_t)f];
             sy[u] = Sf[3 * (size_t)f + 1];
             sz[u] = Sf[3 * (size_t)f + 2];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) to[u][q] = Tc[q * (size_t)n + nb];
+            loadTensor<FromGrad>(n, nb, Tc, nu, to[u]);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u)
@@ -893,19 +913,31 @@ void launchScaledDev2T(Ctx& c, int n, const double* gradU, const double* nu, dou
     hipCheck(hipGetLastError(), "nu*dev2(T(gradU))");
 }
 
-void launchTensorDiv(Ctx& c, const Mesh& m, const double* w, const double* Sf, const int* isEmpty, const double* Tc,
-                     const double* Tb, double* dX, double* dY, double* dZ)
+template <bool FromGrad>
+static void launchTensorDivT(Ctx& c, const Mesh& m, const double* w, const double* Sf, const int* isEmpty,
+                             const double* Tc, const double* nu, const double* Tb, double* dX, double* dY, double* dZ)
 {
-    if (m.nCells == 0) return;
     const SlabPart part = gatherPart(m);
     if (part.plane)
-        tensorDivK<true><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+        tensorDivK<true, FromGrad><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
             m.nCells, part, m.nFaces, m.nBFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p,
-            m.neighbour.p, m.bndStart.p, m.bndFaces.p, isEmpty, w, Sf, Tc, Tb, dX, dY, dZ);
+            m.neighbour.p, m.bndStart.p, m.bndFaces.p, isEmpty, w, Sf, Tc, nu, Tb, dX, dY, dZ);
     else
-        tensorDivK<false><<<blocks(m.nCells), kTpb, 0, c.stream>>>(
+        tensorDivK<false, FromGrad><<<blocks(m.nCells), kTpb, 0, c.stream>>>(
             m.nCells, part, m.nFaces, m.nBFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p,
-            m.neighbour.p, m.bndStart.p, m.bndFaces.p, isEmpty, w, Sf, Tc, Tb, dX, dY, dZ);
+            m.neighbour.p, m.bndStart.p, m.bndFaces.p, isEmpty, w, Sf, Tc, nu, Tb, dX, dY, dZ);
+}
+
+// nu == nullptr: Tc holds the tensors; else Tc holds grad(U) and each cell's nu*dev2(T(grad U)) is formed
+// where it is read
+void launchTensorDiv(Ctx& c, const Mesh& m, const double* w, const double* Sf, const int* isEmpty, const double* Tc,
+                     const double* nu, const double* Tb, double* dX, double* dY, double* dZ)
+{
+    if (m.nCells == 0) return;
+    if (nu)
+        launchTensorDivT<true>(c, m, w, Sf, isEmpty, Tc, nu, Tb, dX, dY, dZ);
+    else
+        launchTensorDivT<false>(c, m, w, Sf, isEmpty, Tc, nu, Tb, dX, dY, dZ);
     hipCheck(hipGetLastError(), "tensor divergence");
 }
 

@@ -230,7 +230,7 @@ void launchLinearUpwindCorr(Ctx& c, const Mesh& m, const double* phi, const doub
                             const double* const* grad, double* const* corr);
 void launchScaledDev2T(Ctx& c, int n, const double* gradU, const double* nu, double* sigma);
 void launchTensorDiv(Ctx& c, const Mesh& m, const double* w, const double* Sf, const int* isEmpty, const double* Tc,
-                     const double* Tb, double* dX, double* dY, double* dZ);
+                     const double* nu, const double* Tb, double* dX, double* dY, double* dZ);
 void launchCmpt3(Ctx& c, int n, const double* x, const double* y, const double* z, bool min, double* out);
 void launchRelaxField(Ctx& c, int n, double* p, const double* pPrev, double alpha);
 void launchConstrainHbyA(Ctx& c, const Mesh& m, const int* takeU, const double* Ub, const double* HbyA,

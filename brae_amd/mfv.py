@@ -142,6 +142,7 @@ _SIGS = {
     "mfv_linear_upwind_corr": [P, P, P, P, P, I, P, P],
     "mfv_scaled_dev2t": [P, I, P, P, P],
     "mfv_tensor_div_source": [P, P, P, P, P, P, P, P, P, P],
+    "mfv_div_dev_reff_explicit": [P, P, P, P, P, P, P, P, P, P, P],
     "mfv_cmpt_max_mag3": [P, I, P, P, P, P],
     "mfv_cmpt_min3": [P, I, P, P, P, P],
     "mfv_relax_field": [P, I, P, P, D],

@@ -508,6 +508,23 @@ inline void deviceTensorDivSource(const DeviceMesh& dm, const DeviceBuffer<scala
              ctx, "deviceTensorDivSource");
 }
 
+// deviceDivDevReff's sigma and tensor-divergence stages (src/cuda/device_divdevreff.cu:408-620) in one
+// pass: V*div(nu*dev2(T(gradU))) without storing sigma; sigmaB is the boundary faces' stress
+inline void deviceDivDevReffExplicit(const DeviceMesh& dm, const DeviceBuffer<scalar>& gradU,
+                                     const DeviceBuffer<scalar>& nu, const DeviceBuffer<scalar>& sigmaB,
+                                     DeviceBuffer<scalar>& srcX, DeviceBuffer<scalar>& srcY,
+                                     DeviceBuffer<scalar>& srcZ, const DeviceBuffer<int>* isEmpty = nullptr)
+{
+    mfv_ctx ctx = threadContext();
+    srcX.resize(dm.nCells);
+    srcY.resize(dm.nCells);
+    srcZ.resize(dm.nCells);
+    mfvCheck(mfv_div_dev_reff_explicit(ctx, dm.mesh, dm.weights.data(), dm.Sf.data(),
+                                       isEmpty ? isEmpty->data() : nullptr, gradU.data(), nu.data(), sigmaB.data(),
+                                       srcX.data(), srcY.data(), srcZ.data()),
+             ctx, "deviceDivDevReffExplicit");
+}
+
 // deviceCmptMaxMag3 / deviceCmptMin3 (src/cuda/device_simple.cuh:123-128)
 inline void deviceCmptMaxMag3(const DeviceBuffer<scalar>& a, const DeviceBuffer<scalar>& b,
                               const DeviceBuffer<scalar>& c, DeviceBuffer<scalar>& out)

@@ -187,6 +187,13 @@ int mfv_scaled_dev2t(mfv_ctx ctx, int n, const double* dGradU, const double* dNu
 int mfv_tensor_div_source(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dSf,
                           const int* dIsEmpty, const double* dTcell, const double* dTbnd, double* dSrcX,
                           double* dSrcY, double* dSrcZ);
+/* The sigma and tensor-divergence stages of deviceDivDevReff (src/cuda/device_divdevreff.cu:408-620) in one
+ * pass: V*div(nu*dev2(T(gradU))) with each cell's stress formed from dGradU [9*nCells] and dNu where it is
+ * read, never stored; dSigmaB the boundary faces' stress [9*nBFaces]. The values are mfv_scaled_dev2t's
+ * followed by mfv_tensor_div_source's, to the bit. */
+int mfv_div_dev_reff_explicit(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dSf,
+                              const int* dIsEmpty, const double* dGradU, const double* dNu, const double* dSigmaB,
+                              double* dSrcX, double* dSrcY, double* dSrcZ);
 /* deviceCmptMaxMag3 / deviceCmptMin3 (src/cuda/device_simple.cuh:123-128): per element max(|a|,|b|,|c|) and
  * min(a,b,c) */
 int mfv_cmpt_max_mag3(mfv_ctx ctx, int n, const double* dA, const double* dB, const double* dC, double* dOut);

@@ -279,3 +279,9 @@ def test_viscous_stress_bit_exact(case):
                ptr(G.dev(Tb.ravel())), *(ptr(o) for o in out))
         for k in range(3):
             assert np.array_equal(G.host(out[k]), ref[k]), k
+        # the fused pass forms each cell's stress from grad(U) where it is read: the same bits
+        fused = [G.dev(np.full(C.nC, 5.0)) for _ in range(3)]
+        C.call("mfv_div_dev_reff_explicit", ptr(C.d["w"]), ptr(C.d["Sf"]), None if e is None else ptr(G.dev(e)),
+               ptr(G.dev(gradU.ravel())), ptr(G.dev(nu)), ptr(G.dev(Tb.ravel())), *(ptr(o) for o in fused))
+        for k in range(3):
+            assert np.array_equal(G.host(fused[k]), ref[k]), k
