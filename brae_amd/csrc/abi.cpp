@@ -137,7 +137,7 @@ namespace {
 
 LduView view(const mfv_mesh mesh, const mfv_ldu* A)
 {
-    need(A && A->dDiag && A->dUpper, "matrix: diag/upper missing");
+    need(A && A->dDiag && (A->dUpper || mesh->m.nFaces == 0), "matrix: diag/upper missing");
     return LduView{mesh->m.nCells, mesh->m.nFaces, A->dDiag, A->dUpper, A->dLower ? A->dLower : A->dUpper};
 }
 

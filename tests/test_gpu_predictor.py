@@ -31,7 +31,7 @@ class Case:
         dL, uL, lL = np.zeros(nC), np.zeros(nI), np.zeros(nI)
         O.lib().orc_laplacian(nC, nI, O.p(self.own), O.p(self.nei), O.p(O.f64(m["deltaCoeffs"])),
                               O.p(np.full(nI, 1e-2)), O.p(O.f64(m["magSf"][:nI])), O.p(dL), O.p(uL), O.p(lL))
-        self.diag = dU - dL + 2.0 * np.abs(uU).max()
+        self.diag = dU - dL + 2.0 * (np.abs(uU).max() if nI else 1.0)
         self.upper, self.lower = uU - uL, lU - lL
         self.ic = [rng.uniform(0.1, 1.0, nB) for _ in range(3)]
         self.bc = [rng.normal(size=nB) for _ in range(3)]
@@ -52,7 +52,9 @@ class Case:
         return mfv.ldu(self.d["diag"], self.d["upper"], self.d["lower"])
 
 
-@pytest.fixture(scope="module", params=[(9, 7, 5, 3), (61, 37, 23, 11)], ids=["9x7x5", "61x37x23"])
+# one cell (no internal faces), a ragged 3x2x1 box, a small and a 52 k-cell box (plane slabs)
+@pytest.fixture(scope="module", params=[(1, 1, 1, 5), (3, 2, 1, 6), (9, 7, 5, 3), (61, 37, 23, 11)],
+                ids=["1x1x1", "3x2x1", "9x7x5", "61x37x23"])
 def case(request):
     return Case(*request.param)
 
@@ -159,6 +161,8 @@ def test_adjust_phi(case):
     phiInt = rng.normal(size=C.nI)
     phiB = rng.normal(size=C.nB)
     adj = (rng.uniform(size=C.nB) < 0.3).astype(np.int32)
+    if C.nB < 50:
+        adj[:] = 1                                       # a few faces: every one adjustable
     rc_ref, mc_ref, closed_ref, out_ref = O.adjust_phi(phiInt, adj, phiB)
     assert rc_ref == 0
     # the host loops' order: bit-exact
