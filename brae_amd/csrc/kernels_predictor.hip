@@ -20,13 +20,43 @@ inline int blocks(int n) { return (n + kTpb - 1) / kTpb; }
 // The cell of a per-cell gather kernel's thread, or -1. Slab: the plane-slab partition of the matrix's
 // band (a cell's neighbour one band back -- lower[], psi[] of its losort run -- was read by the same XCD
 // one slab earlier and is still in that XCD's L2); else the XCD-contiguous order.
-template <bool Slab>
+// With Split > 1 each XCD's slab is cut into that many column parts, and the XCD walks each part through
+// every plane before the next: the reuse distance of a band-back neighbour shrinks to one part, at the
+// price of the cells next to a part boundary, whose neighbours across it were read a whole walk earlier.
+// Measured at 20 M cells: two parts speed up linearUpwindCorrK (1762 -> 1662 us) and slow the other
+// gathers by 1-5 %, four parts are slower for all of them.
+template <bool Slab, int Split = 1>
 __device__ __forceinline__ int gatherCell(int n, const SlabPart& part)
 {
-    if (Slab) return slabRow(part, 0);
-    const int c = xcdBlock() * kTpb + threadIdx.x;
-    return c < n ? c : -1;
+    if (!Slab)
+    {
+        const int c = xcdBlock() * kTpb + threadIdx.x;
+        return c < n ? c : -1;
+    }
+    const int k = blockIdx.x & 7;
+    const int s0 = (int)((long long)k * part.plane >> 3), s1 = (int)((long long)(k + 1) * part.plane >> 3);
+    const int slab = s1 - s0, sub = (slab + Split - 1) / Split;
+    const int nPl = (n + part.plane - 1) / part.plane;
+    const long long p = (long long)(blockIdx.x >> 3) * kTpb + threadIdx.x;
+    const long long per = (long long)nPl * sub;
+    const int col = (int)(p / per);
+    if (col >= Split) return -1;
+    const long long q = p - col * per;
+    const int z = (int)(q / sub), x = col * sub + (int)(q - (long long)z * sub);
+    if (x >= slab) return -1;
+    const long long row = (long long)z * part.plane + s0 + x;
+    return row < n ? (int)row : -1;
 }
+
+// workgroups of the gather partition
+inline int gatherBlocks(int n, int plane, int split = 1)
+{
+    const long long nPl = ((long long)n + plane - 1) / plane;
+    const long long slabMax = (plane + 7) / 8, sub = (slabMax + split - 1) / split;
+    return (int)(8 * ((split * nPl * sub + kTpb - 1) / kTpb));
+}
+
+constexpr int kLinUpSplit = 2;   // linearUpwindCorrK's column parts
 
 // the partition of a mesh's cells for gather kernels: slabs of the band when the cells span several
 // bands of at least kSlabMinPlane cells, one row per thread
@@ -362,7 +392,7 @@ __global__ __launch_bounds__(kTpb) void linearUpwindCorrK(int n, SlabPart part, 
                                                           const double* __restrict__ Cf,
                                                           const double* __restrict__ C, LinUpArgs a)
 {
-    const int c = gatherCell<Slab>(n, part);
+    const int c = gatherCell<Slab, kLinUpSplit>(n, part);
     if (c < 0) return;
     const int l0 = losortStart[c], l1 = losortStart[c + 1];
     const int f0 = ownerStart[c], f1 = ownerStart[c + 1];
@@ -803,7 +833,7 @@ void launchMatrixH(Ctx& c, const Mesh& m, const double* upper, const double* low
     if (m.nCells == 0) return;
     const SlabPart part = gatherPart(m);
     if (part.plane)
-        matrixHK<true><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+        matrixHK<true><<<gatherBlocks(m.nCells, part.plane), kTpb, 0, c.stream>>>(
             m.nCells, part, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p, m.bndStart.p,
             m.bndFaces.p, upper, lower, psi, source, bdDiag, bdSrc, V, rAU, out);
     else
@@ -832,7 +862,7 @@ void launchMatrixH3(Ctx& c, const Mesh& m, const double* upper, const double* lo
     }
     const SlabPart part = gatherPart(m);
     if (part.plane)
-        matrixH3K<true><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+        matrixH3K<true><<<gatherBlocks(m.nCells, part.plane), kTpb, 0, c.stream>>>(
             m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p,
             m.bndStart.p, m.bndFaces.p, upper, lower, a, V, rAU);
     else
@@ -862,7 +892,7 @@ void launchRelaxDiag(Ctx& c, const Mesh& m, const double* upper, const double* l
     }
     const SlabPart part = gatherPart(m);
     if (part.plane)
-        relaxK<true><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+        relaxK<true><<<gatherBlocks(m.nCells, part.plane), kTpb, 0, c.stream>>>(
             m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.bndStart.p, m.bndFaces.p, upper,
             lower, diag, a, relaxed, delta);
     else
@@ -878,7 +908,7 @@ static void launchLinUp(Ctx& c, const Mesh& m, const double* phi, const double* 
 {
     const SlabPart part = gatherPart(m);
     if (part.plane)
-        linearUpwindCorrK<true, NC><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+        linearUpwindCorrK<true, NC><<<gatherBlocks(m.nCells, part.plane, kLinUpSplit), kTpb, 0, c.stream>>>(
             m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p, phi,
             Cf, C, a);
     else
@@ -919,7 +949,7 @@ static void launchTensorDivT(Ctx& c, const Mesh& m, const double* w, const doubl
 {
     const SlabPart part = gatherPart(m);
     if (part.plane)
-        tensorDivK<true, FromGrad><<<slabBlocks(m.nCells, part.plane, 1), kTpb, 0, c.stream>>>(
+        tensorDivK<true, FromGrad><<<gatherBlocks(m.nCells, part.plane), kTpb, 0, c.stream>>>(
             m.nCells, part, m.nFaces, m.nBFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p,
             m.neighbour.p, m.bndStart.p, m.bndFaces.p, isEmpty, w, Sf, Tc, nu, Tb, dX, dY, dZ);
     else
