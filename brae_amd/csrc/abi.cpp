@@ -380,12 +380,16 @@ int mfv_fold(mfv_ctx ctx, mfv_mesh mesh, const double* dDiag, const double* dSou
     });
 }
 
-int mfv_set_reference(mfv_ctx ctx, double* dDiag, double* dB, int refCell, double refValue)
+int mfv_set_reference(mfv_ctx ctx, int nCells, double* dDiag, double* dB, int refCell, double refValue)
 {
     return guarded(ctx, [&] {
         need(ctx != nullptr, "null context");
-        need(dDiag && dB, "setReference: null argument");
+        need(nCells >= 0, "setReference: negative nCells");
         if (refCell < 0) return;
+        need(dDiag && dB, "setReference: null argument");
+        if (refCell >= nCells)
+            invalid("setReference: refCell " + std::to_string(refCell) + " is outside the " + std::to_string(nCells) +
+                    " cells");
         launchSetReference(ctx->c, dDiag, dB, refCell, refValue);
     });
 }
@@ -990,6 +994,12 @@ int mfv_linear_upwind_corr(mfv_ctx ctx, mfv_mesh mesh, const double* dPhi, const
         need(dGrad && dCorr, "linearUpwind correction: null gradient/correction");
         for (int k = 0; k < nComp; ++k) need(dGrad[k] && dCorr[k], "linearUpwind correction: null gradient/correction");
         need(mesh->m.nFaces == 0 || (dPhi && dCf && dC), "linearUpwind correction: null phi/Cf/C");
+        if (mesh->m.nFaces == 0)
+        {
+            // no internal face: the correction is zero, and phi/Cf/C may be null
+            for (int k = 0; k < nComp; ++k) launchZero(ctx->c, mesh->m.nCells, dCorr[k]);
+            return;
+        }
         launchLinearUpwindCorr(ctx->c, mesh->m, dPhi, dCf, dC, nComp, dGrad, dCorr);
     });
 }

@@ -204,7 +204,7 @@ void serveOne(mfv_ctx ctx, const mfv_serve_request& q, int precision, mfv_serve_
         gammaf.upload(minusOne, st);
         check(mfv_laplacian_coeffs(ctx, mesh, dc.p, gammaf.p, magSf.p, diag.p, upper.p, nullptr), ctx, "laplacian");
         hipCheck(hipMemsetAsync(b.p, 0, sizeof(double) * nC, st), "rhs clear");
-        check(mfv_set_reference(ctx, diag.p, b.p, 0, 0.0), ctx, "reference");
+        check(mfv_set_reference(ctx, nC, diag.p, b.p, 0, 0.0), ctx, "reference");
         std::vector<double> hw(nI);
         hipCheck(hipMemcpyAsync(hw.data(), magSf.p, sizeof(double) * nI, hipMemcpyDeviceToHost, st), "weights");
         hipCheck(hipStreamSynchronize(st), "weights sync");

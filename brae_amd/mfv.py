@@ -104,7 +104,7 @@ _SIGS = {
     "mfv_div_bcoeffs": [P, P, P, P, P, P, P],
     "mfv_div_central_coeffs": [P, P, P, P, P, P, P],
     "mfv_fold": [P, P, P, P, P, P, P, P],
-    "mfv_set_reference": [P, P, P, I, D],
+    "mfv_set_reference": [P, I, P, P, I, D],
     "mfv_face_flux": [P, P, P, P, P],
     "mfv_amul": [P, P, P, P, P],
     "mfv_residual": [P, P, P, P, P, P],

@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -302,7 +303,11 @@ inline void deviceFold(const DeviceMesh& dm, const DeviceBuffer<scalar>& rawDiag
 inline void deviceSetReference(DeviceBuffer<scalar>& diag, DeviceBuffer<scalar>& b, label refCell, scalar refValue)
 {
     mfv_ctx ctx = threadContext();
-    mfvCheck(mfv_set_reference(ctx, diag.data(), b.data(), refCell, refValue), ctx, "deviceSetReference");
+    if (refCell >= 0 && (static_cast<std::size_t>(refCell) >= diag.size() || static_cast<std::size_t>(refCell) >= b.size()))
+        throw std::runtime_error("brae hip: deviceSetReference: refCell " + std::to_string(refCell) +
+                                 " is outside the diagonal or the source");
+    const int n = static_cast<int>(std::min(diag.size(), b.size()));
+    mfvCheck(mfv_set_reference(ctx, n, diag.data(), b.data(), refCell, refValue), ctx, "deviceSetReference");
 }
 
 // deviceMatrixFluxInternal (device_simple.cuh:51-52): flux = upper*p[nei] - lower*p[own]

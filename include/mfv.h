@@ -109,8 +109,10 @@ int mfv_div_bcoeffs(mfv_ctx ctx, mfv_mesh mesh, const int* dBcType, const double
 /* deviceFold (src/cuda/device_fvm.cu:895-912): diagC = diag + sum iC, b = source + sum bC */
 int mfv_fold(mfv_ctx ctx, mfv_mesh mesh, const double* dDiag, const double* dSource, const double* dIC,
              const double* dBC, double* dDiagC, double* dB);
-/* deviceSetReference (src/applications/solvers/simpleFoam/device_simple.cuh:104-106) */
-int mfv_set_reference(mfv_ctx ctx, double* dDiag, double* dB, int refCell, double refValue);
+/* deviceSetReference (src/applications/solvers/simpleFoam/device_simple.cuh:104-106):
+ * b[refCell] += diag[refCell]*refValue, then diag[refCell] += diag[refCell] (the source takes the OLD diagonal).
+ * nCells is the length of both arrays; refCell < 0 is a no-op, refCell >= nCells is refused. */
+int mfv_set_reference(mfv_ctx ctx, int nCells, double* dDiag, double* dB, int refCell, double refValue);
 /* deviceMatrixFluxInternal (device_simple.cuh:51-52): flux = upper*psi[nei] - lower*psi[own] */
 int mfv_face_flux(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dPsi, double* dFlux);
 

@@ -58,3 +58,23 @@ def hex_case(nx, ny, nz, **kw):
     nI = m["nInternal"]
     c = Case(m["nCells"], m["owner"][:nI], m["neighbour"], m["owner"][nI:])
     return m, c
+
+
+_HIP = None
+
+
+def d2h(dptr, n, dtype):
+    """copy n elements of a raw device pointer the library returned to a new host array"""
+    global _HIP
+    import ctypes
+    torch_cuda().cuda.synchronize()
+    if _HIP is None:
+        _HIP = ctypes.CDLL("libamdhip64.so")
+        _HIP.hipMemcpy.restype = ctypes.c_int
+        _HIP.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _HIP.hipDeviceSynchronize.restype = ctypes.c_int
+    out = np.zeros(int(n), dtype)
+    if n:
+        assert _HIP.hipDeviceSynchronize() == 0
+        assert _HIP.hipMemcpy(out.ctypes.data, dptr, out.nbytes, 2) == 0   # hipMemcpyDeviceToHost
+    return out

@@ -363,3 +363,53 @@ def test_div_central_bit_exact(pitz):
               mfv.ptr(up), mfv.ptr(lo))
     case.sync()
     assert np.array_equal(G.host(diag), od) and np.array_equal(G.host(up), ou) and np.array_equal(G.host(lo), ol)
+
+
+@pytest.mark.parametrize("which", ["pitzDaily", "channel"])
+def test_mesh_addressing_arrays_bit_exact(pitz, which):
+    """every array mfv_mesh_addressing exposes equals the oracle's lduAddressing (device_ldu.cuh:109-146)"""
+    if which == "pitzDaily":
+        _, mesh, case = pitz
+        nC, own, nei = mesh.nCells, mesh.own(), mesh.neighbour
+    else:
+        m, case = G.hex_case(13, 9, 6, length=(4.0, 1.0, 0.5), grading=(1.0, 20.0, 20.0), jitter=0.1, seed=8)
+        nI = m["nInternal"]
+        nC, own, nei = m["nCells"], O.i32(m["owner"][:nI]), O.i32(m["neighbour"])
+    print("case: addressing", which, nC, flush=True)
+    nF = len(own)
+    ptrs = [ctypes.c_void_p() for _ in range(5)]
+    assert mfv.lib().mfv_mesh_addressing(case.mesh.h, *[ctypes.byref(x) for x in ptrs]) == 0
+    d_own, d_nei, d_os, d_ls, d_lo = [G.d2h(x.value, n, np.int32) for x, n in
+                                       zip(ptrs, (nF, nF, nC + 1, nC + 1, nF))]
+    o_os = np.zeros(nC + 1, np.int32)
+    o_ls = np.zeros(nC + 1, np.int32)
+    o_lo = np.zeros(nF, np.int32)
+    assert O.lib().orc_ldu_addressing(nC, nF, O.p(own), O.p(nei), O.p(o_os), O.p(o_ls), O.p(o_lo)) == 0
+    assert np.array_equal(d_own, own) and np.array_equal(d_nei, nei)
+    assert np.array_equal(d_os, o_os)
+    assert np.array_equal(d_ls, o_ls)
+    assert np.array_equal(d_lo, o_lo)
+
+
+def test_set_reference_bit_exact(pitz):
+    """mfv_set_reference against orc_set_reference: the source takes the OLD diagonal times refValue"""
+    _, mesh, case = pitz
+    nC = mesh.nCells
+    print("case: setReference", nC, flush=True)
+    rng = np.random.default_rng(21)
+    diag, b = rng.uniform(-3.0, -1.0, nC), rng.normal(size=nC)
+    for ref, value in ((0, 0.0), (17, 3.75), (nC - 1, -2.5), (-1, 1.0)):
+        dd, db = G.dev(diag), G.dev(b)
+        case.call("mfv_set_reference", nC, mfv.ptr(dd), mfv.ptr(db), ref, value)
+        case.sync()
+        od, ob = diag.copy(), b.copy()
+        O.lib().orc_set_reference(O.p(od), O.p(ob), ref, value)
+        assert np.array_equal(G.host(dd), od)
+        assert np.array_equal(G.host(db), ob)
+        if ref >= 0:
+            assert G.host(db)[ref] == b[ref] + diag[ref] * value
+    # a reference cell outside the mesh is refused, nothing is written
+    dd, db = G.dev(diag), G.dev(b)
+    with pytest.raises(mfv.MfvError):
+        case.call("mfv_set_reference", nC, mfv.ptr(dd), mfv.ptr(db), nC, 1.0)
+    assert np.array_equal(G.host(dd), diag)

@@ -14,12 +14,50 @@ from brae_amd.mfv import ptr
 pytestmark = pytest.mark.gpu
 
 
+def permuted(m, perm):
+    """The box renumbered: cell c becomes perm[c]. Internal faces are re-oriented owner < neighbour (Sf and
+    the correction vector change sign, the weight becomes 1 - w) and re-sorted into upper-triangular order;
+    boundary faces keep their order. A random permutation leaves a band close to the cell count that does
+    not divide it, so the plane-slab gathers meet a partial last plane."""
+    nI, nC = m["nInternal"], m["nCells"]
+    o, n = perm[m["owner"][:nI]], perm[m["neighbour"]]
+    flip = o > n
+    lo, hi = np.where(flip, n, o), np.where(flip, o, n)
+    order = np.lexsort((hi, lo))
+    out = dict(m)
+    out["owner"] = np.concatenate([lo[order], perm[m["owner"][nI:]]]).astype(np.int32)
+    out["neighbour"] = hi[order].astype(np.int32)
+    sgn = np.where(flip, -1.0, 1.0)[order]
+    for k in ("Sf", "corrVec"):
+        a = np.array(m[k], dtype=float)
+        a[:nI] = a[:nI][order] * sgn[:, None]
+        out[k] = a
+    for k in ("Cf",):
+        a = np.array(m[k], dtype=float)
+        a[:nI] = a[:nI][order]
+        out[k] = a
+    w = np.array(m["weights"], dtype=float)
+    w[:nI] = np.where(flip, 1.0 - w[:nI], w[:nI])[order]
+    out["weights"] = w
+    for k in ("magSf", "deltaCoeffs", "nonOrthDeltaCoeffs"):
+        a = np.array(m[k], dtype=float)
+        a[:nI] = a[:nI][order]
+        out[k] = a
+    inv = np.empty(nC, np.int64)
+    inv[perm] = np.arange(nC)
+    out["C"] = np.asarray(m["C"])[inv]
+    out["V"] = np.asarray(m["V"])[inv]
+    return out
+
+
 class Case:
     """An asymmetric momentum matrix (upwind convection + laplacian) on a jittered hex box, boundary
     coefficients per face and component, sources, a velocity, host and device copies."""
 
-    def __init__(self, nx, ny, nz, seed):
+    def __init__(self, nx, ny, nz, seed, permute=False):
         m = hexmesh.box_mesh(nx, ny, nz, length=(2.0, 1.0, 0.5), grading=(1.0, 4.0, 4.0), jitter=0.1, seed=seed)
+        if permute:
+            m = permuted(m, np.random.default_rng(seed + 100).permutation(m["nCells"]))
         nI, nC = m["nInternal"], m["nCells"]
         self.m, self.nI, self.nC = m, nI, nC
         self.own, self.nei, self.bfc = O.i32(m["owner"][:nI]), O.i32(m["neighbour"]), O.i32(m["owner"][nI:])
@@ -53,8 +91,10 @@ class Case:
 
 
 # one cell (no internal faces), a ragged 3x2x1 box, a small and a 52 k-cell box (plane slabs)
-@pytest.fixture(scope="module", params=[(1, 1, 1, 5), (3, 2, 1, 6), (9, 7, 5, 3), (61, 37, 23, 11)],
-                ids=["1x1x1", "3x2x1", "9x7x5", "61x37x23"])
+# and the 23 k-cell box renumbered at random (a band of nearly the cell count that does not divide it)
+@pytest.fixture(scope="module", params=[(1, 1, 1, 5), (3, 2, 1, 6), (9, 7, 5, 3), (61, 37, 23, 11),
+                                        (41, 29, 19, 12, True)],
+                ids=["1x1x1", "3x2x1", "9x7x5", "61x37x23", "41x29x19-permuted"])
 def case(request):
     return Case(*request.param)
 
@@ -259,6 +299,12 @@ def test_linear_upwind_corr_bit_exact(case):
         C.call("mfv_linear_upwind_corr", ptr(dphi), ptr(dCf), ptr(dC), 1, (ctypes.c_void_p * 1)(dg[k].data_ptr()),
                (ctypes.c_void_p * 1)(one.data_ptr()))
         assert np.array_equal(G.host(one), refs[k]), k
+    if C.nI == 0:
+        # a mesh without internal faces: phi, Cf and C may be null and the correction is zero
+        zs = [G.dev(np.full(C.nC, 9.0)) for _ in range(3)]
+        C.call("mfv_linear_upwind_corr", None, None, None, 3, mfv.ptr3(dg), mfv.ptr3(zs))
+        for k in range(3):
+            assert np.array_equal(G.host(zs[k]), np.zeros(C.nC)), k
 
 
 def test_viscous_stress_bit_exact(case):
