@@ -222,6 +222,76 @@ void orc_boundary_flux(int nBFaces, const double* SfB, const double* uxb, const 
 int orc_adjust_phi(int nFaces, const double* phiInt, int nBFaces, const int* adjustable, double* phiB,
                    double* massCorr, int* closedVolume);
 
+/* ---- one SIMPLE iteration (oracle/simple.c): simpleFoam_cpp.cu:15-237 ---- */
+/* PBiCGStab with DILU on an asymmetric LDU matrix plus uncoupled boundary coefficients: pbicgstab.cu:6-150 */
+orc_perf orc_pbicgstab(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                       const double* upper, const double* lower, const double* source, int nBFaces,
+                       const int* bFaceCell, const double* iC, const double* bC, double* psi, double tol,
+                       double relTol, int maxIter, int minIter);
+/* the GAMG of gamg.cu:267-400 (magSf face weights, agglomeratePair, coarsen, GaussSeidel, dense coarsest);
+ * nLevels (nullable) receives the number of levels */
+orc_perf orc_gamg_ref(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                      const double* upper, const double* lower, const double* source, int nBFaces,
+                      const int* bFaceCell, const double* iC, const double* bC, const double* magSf,
+                      double* psi, double tol, double relTol, int maxIter, int minIter, int nCellsInCoarsest,
+                      int nPostSweeps, int nFinestSweeps, int* nLevels);
+/* one agglomeratePair pass of gamg.cu:41-107; returns the number of coarse cells */
+int orc_gamg_ref_pair(int nCells, int nFaces, const int* lowerAddr, const int* upperAddr, const double* weights,
+                      int forward, int* map);
+/* the GAMG's parts: one coarsening (returns the coarse face count; arrays may be NULL to size them),
+ * GaussSeidel sweeps on an asymmetric level, the dense coarsest solve and the correction scaling */
+int orc_gamg_ref_coarsen(int nCells, int nFaces, const int* lowerAddr, const int* upperAddr, const double* diag,
+                         const double* upper, const double* lower, const double* weights, const int* map,
+                         int nCoarse, int* cLower, int* cUpper, double* cDiag, double* cUpperV, double* cLowerV,
+                         double* cWeights);
+void orc_gamg_ref_gauss_seidel(int nCells, int nFaces, const int* lowerAddr, const int* upperAddr,
+                               const double* diag, const double* upper, const double* lower, const double* source,
+                               double* psi, int nSweeps);
+void orc_gamg_ref_dense(int nCells, int nFaces, const int* lowerAddr, const int* upperAddr, const double* diag,
+                        const double* upper, const double* lower, const double* b, double* x);
+void orc_gamg_ref_scale(int nCells, int nFaces, const int* lowerAddr, const int* upperAddr, const double* diag,
+                        const double* upper, const double* lower, const double* source, double* field);
+/* fvc::gradUBoundary for fixedValue/zeroGradient/empty patches (fvc.cu:781-865); gradU [9*nCells] and
+ * gb [9*nBFaces] component-major, (i*3 + j): dU_j/dx_i */
+void orc_grad_u_boundary(int nCells, int nFaces, int nBFaces, const int* bFaceCell, const int* uType,
+                         const double* Sf, const double* magSf, const double* bDeltaCoeffs, const double* Ux,
+                         const double* Uy, const double* Uz, const double* Ubx, const double* Uby,
+                         const double* Ubz, const double* gradU, double* gb);
+
+enum { ORC_P_GAMG_REF = 0, ORC_P_BRAE_AMGPCG = 1, ORC_P_PCG_GAMG = 2 };
+typedef struct {
+    int nCells, nFaces, nBFaces;
+    const int *owner, *neighbour, *bFaceCell;        /* internal owner/neighbour, boundary faces' cells */
+    /* geometry as orc_geometry writes it: Sf, Cf, magSf over internal then boundary faces */
+    const double *Sf, *Cf, *magSf, *C, *V, *weights, *deltaCoeffs, *nonOrthDeltaCoeffs, *corrVec, *bDeltaCoeffs;
+    const int* uType;          /* per boundary face: ORC_BC_FIXED_VALUE (noSlip: value 0), ZERO_GRADIENT, EMPTY */
+    const double* uValue;      /* 3*nBFaces, xyz: the fixed values */
+    const int* pType;
+    const double* pValue;      /* nBFaces */
+    double nu;                 /* laminar: nuEff = nu in every cell and on every boundary face */
+    const double *nuEff, *nuEffB; /* nullable: a frozen nuEff per cell / boundary face instead */
+    double relaxU, relaxP;
+    int momentumPredictor, linearUpwind, correctedLaplacian, nNonOrthCorr;
+    double tolU, relTolU; int maxIterU, minIterU;
+    double tolP, relTolP; int maxIterP, minIterP;
+    int pSolver;               /* ORC_P_*: gamg.cu's GAMG (the reference CPU path), brae AMG-PCG, PCG + GAMG */
+    const orc_aggl* pAggl;     /* the hierarchy of ORC_P_BRAE_AMGPCG / ORC_P_PCG_GAMG */
+    int amgPre, amgPost, amgFlexible, amgSingle, amgScale;
+    int pRefCell; double pRefValue;
+} orc_simple_case;
+typedef struct {
+    double *Ux, *Uy, *Uz;      /* cells */
+    double *Ubx, *Uby, *Ubz;   /* boundary faces */
+    double *p, *pb, *phi, *phiB;
+} orc_simple_fields;
+typedef struct {
+    double UInitial[3], UFinal[3]; int UIters[3];
+    double pInitial, pFinal; int pIters[8];
+    double massCorr; int adjustStatus;
+} orc_simple_result;
+/* one iteration in place on the fields; returns 0, or 1 when adjustPhi refuses */
+int orc_simple_step(const orc_simple_case* cs, orc_simple_fields* fields, orc_simple_result* res);
+
 #ifdef __cplusplus
 }
 #endif

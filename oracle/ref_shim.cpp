@@ -7,8 +7,10 @@
 //   pcg                  src/OpenFOAM/matrices/pcg.cu:6-149
 //   gamgPairAgglomerate  src/matrices/lduMatrix/solvers/GAMG/GAMGAgglomerations/pair_gamg_agglomeration_cpp.cu:158-284
 //   gamgLduBand          same file, :347-365
+//   pbicgstab            src/OpenFOAM/matrices/pbicgstab.cu:6-150 (PBiCGStab with DILU)
 #include "ldu_spmv.cuh"
 #include "pair_gamg_agglomeration_cpp.cuh"
+#include "pbicgstab.cuh"
 #include "pcg.cuh"
 
 #include <exception>
@@ -80,6 +82,43 @@ int ref_pcg(int nCells, int nFaces, const int* owner, const int* neighbour, cons
         M.boundaryCoeffs.assign(1, std::vector<scalar>(bC, bC + nBFaces));
         std::vector<scalar> x(psi, psi + nCells);
         const SolverPerformance sp = pcg(M, x, m, patches, tolerance, relTol, maxIter, minIter, nullptr);
+        for (int c = 0; c < nCells; ++c) psi[c] = x[c];
+        perf[0] = sp.initialResidual;
+        perf[1] = sp.finalResidual;
+        perf[2] = sp.nIterations;
+        return 0;
+    }
+    catch (const std::exception&)
+    {
+        return 1;
+    }
+}
+
+// PBiCGStab with DILU on an asymmetric matrix whose boundary arrives as one uncoupled patch (iC/bC per
+// boundary face). psi in/out; perf = {initial, final, nIterations}.
+int ref_pbicgstab(int nCells, int nFaces, const int* owner, const int* neighbour, const double* diag,
+                  const double* upper, const double* lower, const double* source, int nBFaces, const int* bFaceCell,
+                  const double* iC, const double* bC, double* psi, double tolerance, double relTol, int maxIter,
+                  int minIter, double* perf)
+{
+    try
+    {
+        const PrimitiveMesh m = ldu_mesh(nCells, nFaces, owner, neighbour, nBFaces, bFaceCell);
+        FvScalarMatrix M;
+        M.diag.assign(diag, diag + nCells);
+        M.upper.assign(upper, upper + nFaces);
+        M.lower.assign(lower, lower + nFaces);
+        M.source.assign(source, source + nCells);
+        std::vector<FvPatch> patches(1);
+        patches[0].name = "boundary";
+        patches[0].type = "patch";
+        patches[0].start = nFaces;
+        patches[0].size = nBFaces;
+        patches[0].faceCells.assign(bFaceCell, bFaceCell + nBFaces);
+        M.internalCoeffs.assign(1, std::vector<scalar>(iC, iC + nBFaces));
+        M.boundaryCoeffs.assign(1, std::vector<scalar>(bC, bC + nBFaces));
+        std::vector<scalar> x(psi, psi + nCells);
+        const SolverPerformance sp = pbicgstab(M, x, m, patches, tolerance, relTol, maxIter, minIter);
         for (int c = 0; c < nCells; ++c) psi[c] = x[c];
         perf[0] = sp.initialResidual;
         perf[1] = sp.finalResidual;

@@ -12,6 +12,8 @@ the tests load them without an OpenFOAM parser:
                  values, 282/phi, matrix.dat (fvm::laplacian(1,T)), matrix_div.dat
                  (fvm::div(phi,T) upwind), amul.dat (psi, A.psi), solution.dat (PCG-DIC),
                  validation/matrixDumpGAMG/solution.dat (OpenFOAM GAMG)
+  simple_step.npz  validation/matrixDumpSimple/282/{U,p,phi}, constant/transportProperties (nu) and
+                 step.dat (OpenFOAM's p, U, phi after one SIMPLE iteration from those fields)
   case20.npz     demo/amgpcg/case20/constant/polyMesh, trace/{matrix,rhs,of_hierarchy,
                  brae_hierarchy,brae_levels,brae_cycles,of_gamg_cycles}.csv
 """
@@ -131,6 +133,35 @@ def read_scalar_field(path, patchNames, patchSize):
     return internal, types, vals
 
 
+def read_vector_field(path, patchNames, patchSize):
+    """internal (n, 3) values, boundary types and per-boundary-face (size, 3) values of a volVectorField;
+    noSlip is a fixedValue of zero, a patch without a value reads as zeros"""
+    txt = _body(path)
+    m = re.search(r"internalField\s+(uniform|nonuniform)", txt)
+    if m.group(1) == "uniform":
+        v = re.search(r"internalField\s+uniform\s+\(([^)]*)\)", txt).group(1)
+        internal = np.array([[float(t) for t in v.split()]])
+    else:
+        n, inner, _ = _list_block(txt, m.end())
+        internal = np.array([[float(t) for t in s.split()] for s in re.findall(r"\(([^()]*)\)", inner)])
+        assert internal.shape == (n, 3)
+    bi = txt.find("boundaryField")
+    types, vals = [], []
+    for name, size in zip(patchNames, patchSize):
+        pm = re.search(r"\b%s\s*\{" % re.escape(str(name)), txt[bi:])
+        body_start = bi + pm.end()
+        body_end = txt.find("}", body_start)
+        body = txt[body_start:body_end]
+        t = re.search(r"type\s+(\w+);", body).group(1)
+        v = np.zeros((int(size), 3))
+        um = re.search(r"value\s+uniform\s+\(([^)]*)\)", body)
+        if um:
+            v[:] = [float(x) for x in um.group(1).split()]
+        types.append(t)
+        vals.append(v)
+    return internal, types, vals
+
+
 def read_matrix_dat(path):
     tok = open(path).read().split()
     nC, nIf, npch = int(tok[0]), int(tok[1]), int(tok[2])
@@ -188,6 +219,35 @@ def pitzdaily(ref):
     np.savez_compressed(os.path.join(HERE, "pitzdaily.npz"), **out)
 
 
+def simple_step(ref):
+    """validation/matrixDumpSimple (pitzDaily, the same mesh as matrixDump): OpenFOAM's step-282 fields
+    (U, p, phi with their patch types and values) and its dumpSimpleStep output step.dat (p, U and phi
+    after one SIMPLE iteration, phi per patch), which tests/test_simple_step_cpp.cu compares at 1e-9"""
+    case = os.path.join(ref, "validation/matrixDumpSimple")
+    mesh = read_mesh(os.path.join(case, "constant/polyMesh"))
+    names, sizes = mesh["patchNames"], mesh["patchSize"]
+    out = {}
+    Ui, Ut, Uv = read_vector_field(os.path.join(case, "282/U"), names, sizes)
+    out["U0"], out["U_types"], out["U_values"] = Ui, np.array(Ut), np.concatenate(Uv)
+    pi, pt, pv = read_scalar_field(os.path.join(case, "282/p"), names, sizes)
+    out["p0"], out["p_types"], out["p_values"] = pi, np.array(pt), np.concatenate(pv)
+    phiI, _, phiB = read_scalar_field(os.path.join(case, "282/phi"), names, sizes)
+    out["phi0_internal"], out["phi0_boundary"] = phiI, np.concatenate(phiB)
+    nu = re.search(r"\bnu\s+([^;]+);", _body(os.path.join(case, "constant/transportProperties"))).group(1)
+    out["nu"] = float(nu.split()[-1])
+    tok = open(os.path.join(case, "step.dat")).read().split()
+    nC, nIf, npch = int(tok[0]), int(tok[1]), int(tok[2])
+    k = 3
+    out["step_p"] = np.array(tok[k:k + nC], dtype=np.float64); k += nC
+    out["step_U"] = np.array(tok[k:k + 3 * nC], dtype=np.float64).reshape(nC, 3); k += 3 * nC
+    out["step_phi_internal"] = np.array(tok[k:k + nIf], dtype=np.float64); k += nIf
+    for _ in range(npch):
+        name, sz = tok[k], int(tok[k + 1]); k += 2
+        out["step_phi_" + name] = np.array(tok[k:k + sz], dtype=np.float64); k += sz
+    assert k == len(tok)
+    np.savez_compressed(os.path.join(HERE, "simple_step.npz"), **out)
+
+
 def case20(ref):
     base = os.path.join(ref, "demo/amgpcg/case20")
     out = dict(read_mesh(os.path.join(base, "constant/polyMesh")))
@@ -219,6 +279,7 @@ def case20(ref):
 if __name__ == "__main__":
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     pitzdaily(ref)
+    simple_step(ref)
     case20(ref)
-    for f in ("pitzdaily.npz", "case20.npz"):
+    for f in ("pitzdaily.npz", "simple_step.npz", "case20.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

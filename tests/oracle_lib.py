@@ -91,6 +91,15 @@ def lib():
             "orc_pcg_gamg_solve": (Perf, [P, P, P, P, P, D, D, I, I, P]),
             "orc_brae_amgpcg": (Perf, [P, P, P, P, P, D, D, I, I, I, I, I, I, I, D, P]),
             "orc_brae_vcycle": (None, [P, P, P, P, P, I, I, I, I]),
+            "orc_pbicgstab": (Perf, [I, I, P, P, P, P, P, P, I, P, P, P, P, D, D, I, I]),
+            "orc_gamg_ref": (Perf, [I, I, P, P, P, P, P, P, I, P, P, P, P, P, D, D, I, I, I, I, I, P]),
+            "orc_gamg_ref_pair": (I, [I, I, P, P, P, I, P]),
+            "orc_gamg_ref_coarsen": (I, [I, I, P, P, P, P, P, P, P, I, P, P, P, P, P, P]),
+            "orc_gamg_ref_gauss_seidel": (None, [I, I, P, P, P, P, P, P, P, I]),
+            "orc_gamg_ref_dense": (None, [I, I, P, P, P, P, P, P, P]),
+            "orc_gamg_ref_scale": (None, [I, I, P, P, P, P, P, P, P]),
+            "orc_grad_u_boundary": (None, [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+            "orc_simple_step": (I, [P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -468,3 +477,91 @@ def tensor_div(nC, own, nei, w, Sf, bfc, Tcell, Tbnd, isEmpty=None):
     lib().orc_tensor_div(nC, len(own), p(i32(own)), p(i32(nei)), p(f64(w)), p(f64(Sf).ravel()), len(bfc), p(i32(bfc)),
                          p(None if isEmpty is None else i32(isEmpty)), p(f64(Tcell)), p(f64(Tbnd)), *(p(x) for x in d))
     return d
+
+
+# ---- one SIMPLE iteration (oracle/simple.c) ----
+P_GAMG_REF, P_BRAE_AMGPCG, P_PCG_GAMG = 0, 1, 2
+_VP, _I, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+
+
+class SimpleCase(ctypes.Structure):
+    _fields_ = [("nCells", _I), ("nFaces", _I), ("nBFaces", _I),
+                ("owner", _VP), ("neighbour", _VP), ("bFaceCell", _VP),
+                ("Sf", _VP), ("Cf", _VP), ("magSf", _VP), ("C", _VP), ("V", _VP), ("weights", _VP),
+                ("deltaCoeffs", _VP), ("nonOrthDeltaCoeffs", _VP), ("corrVec", _VP), ("bDeltaCoeffs", _VP),
+                ("uType", _VP), ("uValue", _VP), ("pType", _VP), ("pValue", _VP),
+                ("nu", _D), ("nuEff", _VP), ("nuEffB", _VP), ("relaxU", _D), ("relaxP", _D),
+                ("momentumPredictor", _I), ("linearUpwind", _I), ("correctedLaplacian", _I), ("nNonOrthCorr", _I),
+                ("tolU", _D), ("relTolU", _D), ("maxIterU", _I), ("minIterU", _I),
+                ("tolP", _D), ("relTolP", _D), ("maxIterP", _I), ("minIterP", _I),
+                ("pSolver", _I), ("pAggl", _VP), ("amgPre", _I), ("amgPost", _I), ("amgFlexible", _I),
+                ("amgSingle", _I), ("amgScale", _I), ("pRefCell", _I), ("pRefValue", _D)]
+
+
+class SimpleFields(ctypes.Structure):
+    _fields_ = [(n, _VP) for n in ("Ux", "Uy", "Uz", "Ubx", "Uby", "Ubz", "p", "pb", "phi", "phiB")]
+
+
+class SimpleResult(ctypes.Structure):
+    _fields_ = [("UInitial", _D * 3), ("UFinal", _D * 3), ("UIters", _I * 3), ("pInitial", _D), ("pFinal", _D),
+                ("pIters", _I * 8), ("massCorr", _D), ("adjustStatus", _I)]
+
+
+def bc_codes(types):
+    """patch type names to the oracle's codes: fixedValue and noSlip fix a value, empty drops out"""
+    m = {"fixedValue": BC_FIXED_VALUE, "noSlip": BC_FIXED_VALUE, "zeroGradient": BC_ZERO_GRADIENT,
+         "empty": BC_EMPTY}
+    return i32([m[str(t)] for t in types])
+
+
+def evaluate_boundary(bfc, types, values, cell):
+    """patch values after evaluate(): fixedValue keeps its value, zeroGradient the owner cell's, empty 0"""
+    out = np.where(types == BC_FIXED_VALUE, values, 0.0)
+    zg = types == BC_ZERO_GRADIENT
+    out[zg] = cell[bfc[zg]]
+    return out
+
+
+class SimpleState:
+    """Host fields of one SIMPLE run: U (3 cell arrays + 3 boundary arrays), p, p's patch values, phi."""
+
+    def __init__(self, U, Ub, p, pb, phi, phiB):
+        self.U = [f64(u).copy() for u in U]
+        self.Ub = [f64(u).copy() for u in Ub]
+        self.p, self.pb, self.phi, self.phiB = f64(p).copy(), f64(pb).copy(), f64(phi).copy(), f64(phiB).copy()
+
+    def copy(self):
+        return SimpleState(self.U, self.Ub, self.p, self.pb, self.phi, self.phiB)
+
+
+def simple_case(mesh, uType, uValue, pType, pValue, nu=1e-5, relaxU=0.7, relaxP=0.3, momentumPredictor=1,
+                linearUpwind=0, correctedLaplacian=0, nNonOrthCorr=0, tolU=1e-10, relTolU=0.0, maxIterU=1000,
+                minIterU=0, tolP=1e-10, relTolP=0.0, maxIterP=1000, minIterP=0, pSolver=P_GAMG_REF, aggl=None,
+                amgPre=1, amgPost=1, amgFlexible=0, amgSingle=0, amgScale=0, pRefCell=-1, pRefValue=0.0,
+                nuEff=None, nuEffB=None):
+    """The oracle's case struct for an O.Mesh (its arrays are kept alive on the returned object)."""
+    keep = dict(own=mesh.own(), nei=i32(mesh.neighbour), bfc=i32(mesh.bFaceCell), Sf=f64(mesh.Sf), Cf=f64(mesh.Cf),
+                magSf=f64(mesh.magSf), C=f64(mesh.C), V=f64(mesh.V), w=f64(mesh.weights), dc=f64(mesh.deltaCoeffs),
+                ndc=f64(mesh.nonOrthDeltaCoeffs), cv=f64(mesh.corrVec), bdc=f64(mesh.bDeltaCoeffs),
+                uType=i32(uType), uValue=f64(uValue), pType=i32(pType), pValue=f64(pValue),
+                nuEff=None if nuEff is None else f64(nuEff), nuEffB=None if nuEffB is None else f64(nuEffB))
+    a = lambda k: None if keep[k] is None else keep[k].ctypes.data
+    cs = SimpleCase(mesh.nCells, mesh.nInternal, mesh.nBFaces, a("own"), a("nei"), a("bfc"), a("Sf"), a("Cf"),
+                    a("magSf"), a("C"), a("V"), a("w"), a("dc"), a("ndc"), a("cv"), a("bdc"), a("uType"),
+                    a("uValue"), a("pType"), a("pValue"), nu, a("nuEff"), a("nuEffB"), relaxU, relaxP,
+                    momentumPredictor, linearUpwind, correctedLaplacian, nNonOrthCorr, tolU, relTolU, maxIterU,
+                    minIterU, tolP, relTolP, maxIterP, minIterP, pSolver, None if aggl is None else aggl.h,
+                    amgPre, amgPost, amgFlexible, amgSingle, amgScale, pRefCell, pRefValue)
+    cs._keep = keep
+    cs._aggl = aggl
+    return cs
+
+
+def simple_step(cs, st):
+    """one SIMPLE iteration in place on the SimpleState; returns the SimpleResult"""
+    f = SimpleFields(*[x.ctypes.data for x in st.U + st.Ub + [st.p, st.pb, st.phi, st.phiB]])
+    r = SimpleResult()
+    rc = lib().orc_simple_step(ctypes.byref(cs), ctypes.byref(f), ctypes.byref(r))
+    if rc != 0:
+        raise RuntimeError("oracle SIMPLE step refused (adjustPhi)")
+    return r
