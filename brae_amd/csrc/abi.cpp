@@ -3,6 +3,7 @@
 #include "amg.hpp"
 #include "mfv.h"
 #include "mfv_internal.hpp"
+#include "simple.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -24,6 +25,13 @@ struct mfv_amg_s
     mfv_ctx ctx;
     mfv_mesh mesh;
     mfv::Amg a;
+};
+
+struct mfv_simple_s
+{
+    mfv_ctx ctx;
+    mfv_mesh mesh;
+    std::unique_ptr<mfv::SimpleSolver> s;
 };
 
 namespace mfv {
@@ -1210,6 +1218,181 @@ int mfv_ctx_profile_read(mfv_ctx ctx, int tag, int* count, double* totalMs)
         if (count) *count = (int)v.size();
         if (totalMs) *totalMs = tot;
         v.clear();
+    });
+}
+
+
+int mfv_bicg_default_controls(mfv_bicg_controls* c)
+{
+    if (!c) return MFV_ERR_INVALID;
+    c->tolerance = 1e-10;
+    c->relTol = 0.0;
+    c->maxIter = 1000;
+    c->minIter = 0;
+    c->preconditioner = MFV_PRECOND_DILU;
+    c->checkEvery = 4;
+    return MFV_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+BicgControls bicgControls(const mfv_bicg_controls& c)
+{
+    if (c.preconditioner != MFV_PRECOND_DILU && c.preconditioner != MFV_PRECOND_DIAGONAL)
+        throw Error(kErrUnsupported, "mfv: pbicgstab: preconditioner " + std::to_string(c.preconditioner) +
+                                         " is not carried (DILU, diagonal)");
+    if (c.maxIter < 0 || c.minIter < 0) invalid("pbicgstab: negative iteration limits");
+    BicgControls b;
+    b.tolerance = c.tolerance;
+    b.relTol = c.relTol;
+    b.maxIter = c.maxIter;
+    b.minIter = c.minIter;
+    b.precond = c.preconditioner == MFV_PRECOND_DILU ? BicgPrecond::DILU : BicgPrecond::Diagonal;
+    b.checkEvery = c.checkEvery;
+    return b;
+}
+
+PcgControls pcgControls(const mfv_pcg_controls& c)
+{
+    if (c.smoother != 0)
+        throw Error(kErrUnsupported, "mfv: pcg: smoother " + std::to_string(c.smoother) +
+                                         " is not carried (0 = weighted Jacobi)");
+    if (c.nVcycles > 1) throw Error(kErrUnsupported, "mfv: pcg: nVcycles above 1 is not carried");
+    PcgControls pc;
+    pc.tolerance = c.tolerance;
+    pc.relTol = c.relTol;
+    pc.maxIter = c.maxIter;
+    pc.minIter = c.minIter;
+    pc.prec = precOf(c.precision);
+    pc.nPre = c.nPreSweeps;
+    pc.nPost = c.nPostSweeps;
+    pc.flexible = c.flexible != 0;
+    pc.scaleCorrection = c.scaleCorrection != 0;
+    pc.checkEvery = c.checkEvery;
+    pc.graph = c.captureVcycle != 0;
+    return pc;
+}
+
+} // namespace
+
+extern "C" {
+
+int mfv_pbicgstab_solve(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dSource, const double* dIC,
+                        const double* dBC, double* dPsi, const mfv_bicg_controls* controls, mfv_perf* perf)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dSource && dPsi && controls, "pbicgstab: null argument");
+        need(mesh->m.nBFaces == 0 || (dIC && dBC), "pbicgstab: boundary coefficients missing");
+        const LduView v = view(mesh, A);
+        const PcgPerf r = pbicgstabSolve(ctx->c, mesh->m, v, dSource, dIC, dBC, dPsi, bicgControls(*controls));
+        putPerf(r, perf);
+    });
+}
+
+int mfv_simple_default_controls(mfv_simple_controls* c)
+{
+    if (!c) return MFV_ERR_INVALID;
+    std::memset(c, 0, sizeof(*c));
+    c->nu = 1e-5;
+    c->relaxU = 0.7;
+    c->relaxP = 0.3;
+    c->momentumPredictor = 1;
+    c->linearUpwind = 0;
+    c->correctedLaplacian = 0;
+    c->nNonOrthCorr = 0;
+    c->pRefCell = -1;
+    c->pRefValue = 0.0;
+    mfv_bicg_default_controls(&c->U);
+    mfv_pcg_default_controls(&c->p);
+    c->p.tolerance = 1e-10;
+    c->p.precision = MFV_PREC_FP64;
+    c->mergeLevels = 1;
+    return MFV_OK;
+}
+
+int mfv_simple_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry, const int* hUType,
+                      const double* hUValue, const int* hPType, const double* hPValue,
+                      const mfv_simple_controls* controls, mfv_simple* out)
+{
+    if (out) *out = nullptr;
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(out && geometry && controls, "simple create: null argument");
+        const int nB = mesh->m.nBFaces;
+        need(nB == 0 || (hUType && hUValue && hPType && hPValue), "simple create: boundary conditions missing");
+        Geometry g;
+        g.Sf = geometry->dSf;
+        g.Cf = geometry->dCf;
+        g.magSf = geometry->dMagSf;
+        g.C = geometry->dC;
+        g.V = geometry->dV;
+        g.w = geometry->dWeights;
+        g.dc = geometry->dDeltaCoeffs;
+        g.ndc = geometry->dNonOrthDeltaCoeffs;
+        g.corrVec = geometry->dCorrVec;
+        g.bDelta = geometry->dBDeltaCoeffs;
+        SimpleControls sc;
+        sc.nu = controls->nu;
+        sc.relaxU = controls->relaxU;
+        sc.relaxP = controls->relaxP;
+        sc.momentumPredictor = controls->momentumPredictor != 0;
+        sc.linearUpwind = controls->linearUpwind != 0;
+        sc.correctedLaplacian = controls->correctedLaplacian != 0;
+        sc.nNonOrthCorr = controls->nNonOrthCorr;
+        sc.pRefCell = controls->pRefCell;
+        sc.pRefValue = controls->pRefValue;
+        sc.U = bicgControls(controls->U);
+        sc.P = pcgControls(controls->p);
+        std::vector<int> ut(hUType, hUType + nB), pt(hPType, hPType + nB);
+        std::vector<double> uv(hUValue, hUValue + 3 * (size_t)nB), pv(hPValue, hPValue + nB);
+        std::unique_ptr<mfv_simple_s> h(new mfv_simple_s{ctx, mesh, nullptr});
+        h->s.reset(new SimpleSolver(ctx->c, mesh->m, g, ut, uv, pt, pv, sc, controls->mergeLevels));
+        *out = h.release();
+    });
+}
+
+int mfv_simple_destroy(mfv_simple simple)
+{
+    if (!simple) return MFV_OK;
+    (void)hipStreamSynchronize(simple->ctx->c.stream);
+    delete simple;
+    return MFV_OK;
+}
+
+int mfv_simple_step(mfv_simple simple, const mfv_simple_fields* f, const double* dNuEff, const double* dNuEffB,
+                    mfv_simple_result* result)
+{
+    if (!simple) return MFV_ERR_INVALID;
+    return guarded(simple->ctx, [&] {
+        need(f != nullptr, "simple step: null fields");
+        SimpleFields sf;
+        sf.U[0] = f->dUx;
+        sf.U[1] = f->dUy;
+        sf.U[2] = f->dUz;
+        sf.Ub[0] = f->dUbx;
+        sf.Ub[1] = f->dUby;
+        sf.Ub[2] = f->dUbz;
+        sf.p = f->dP;
+        sf.pb = f->dPb;
+        sf.phi = f->dPhi;
+        sf.phiB = f->dPhiB;
+        const SimpleResult r = simple->s->step(sf, dNuEff, dNuEffB);
+        if (result)
+        {
+            for (int k = 0; k < 3; ++k)
+            {
+                result->UInitial[k] = r.UInitial[k];
+                result->UFinal[k] = r.UFinal[k];
+                result->UIters[k] = r.UIters[k];
+            }
+            result->pInitial = r.pInitial;
+            result->pFinal = r.pFinal;
+            for (int k = 0; k < 8; ++k) result->pIters[k] = r.pIters[k];
+            result->massCorr = r.massCorr;
+        }
     });
 }
 

@@ -223,6 +223,20 @@ struct PcgPerf
     bool converged = false;
 };
 
+// ---- PBiCGStab for asymmetric matrices (src/OpenFOAM/matrices/pbicgstab.cu:6-150; pbicgstab.hip) ----
+enum class BicgPrecond : int { DILU = 0, Diagonal = 1 };
+struct BicgControls
+{
+    double tolerance = 1e-10, relTol = 0.0;
+    int maxIter = 1000, minIter = 0;
+    BicgPrecond precond = BicgPrecond::DILU;
+    int checkEvery = 4;
+};
+// the raw matrix (diag, upper, lower) with its uncoupled boundary coefficients per boundary face: the solve
+// folds them (diag + iC, source + bC) as the reference's does; psi in place
+PcgPerf pbicgstabSolve(Ctx& c, Mesh& m, const LduView& A, const double* source, const double* iC, const double* bC,
+                       double* psi, const BicgControls& bc);
+
 // ---- OpenFOAM's GAMG (src/matrices/lduMatrix/solvers/GAMG/gamg_solver_cpp.cuh:67-85, GamgControls) ----
 enum class OfSmoother : int { GaussSeidel = 0, SymGaussSeidel = 1, DIC = 2, DICGaussSeidel = 3 };
 struct GamgControls

@@ -280,6 +280,8 @@ void dicFactor(Ctx& c, Mesh& m, const LduView& A, double* rD, const int* stop)
 
 } // namespace
 
+void buildLevelSchedule(Ctx& c, Mesh& m) { buildLevels(c, m); }
+
 void launchDicPrecondition(Ctx& c, Mesh& m, const LduView& A, const double* r, double* w, double* rD)
 {
     if (A.lower != A.upper) throw Error(kErrUnsupported, "mfv: DIC needs a symmetric matrix (lower == upper)");

@@ -149,6 +149,13 @@ struct LevelSched
     DBuf<int> fwdCells, bwdCells;          // [nCells] each
 };
 
+// PBiCGStab's work vectors, kept on the mesh between solves (pbicgstab.hip)
+struct BicgWork;
+struct BicgWorkDeleter
+{
+    void operator()(BicgWork* w) const;
+};
+
 // The mesh's internal-face LDU addressing on the device (lduAddressing), plus the boundary
 // gather (cells' boundary faces in ascending face order).
 struct Mesh
@@ -167,7 +174,11 @@ struct Mesh
     int band = 0;              // largest neighbour - owner over the internal faces
     std::vector<label> hOwner, hNeighbour, hBFaceCell;
     std::unique_ptr<LevelSched> sched;   // built on first use by the sequential sweeps (DIC, Gauss-Seidel)
+    std::unique_ptr<BicgWork, BicgWorkDeleter> bicg;   // PBiCGStab's vectors, on first use
 };
+
+// the level schedule of the mesh's sequential sweeps, built once (dic.hip)
+void buildLevelSchedule(Ctx& c, Mesh& m);
 
 // ownerStart, stable losort by neighbour, boundary gather (abi.cpp)
 void buildMesh(Ctx& c, Mesh& m, int nCells, int nFaces, const int* owner, const int* neighbour, int nB,

@@ -88,6 +88,35 @@ class Perf(ctypes.Structure):
     _fields_ = [("initialResidual", D), ("finalResidual", D), ("nIterations", I), ("converged", I)]
 
 
+class BicgControls(ctypes.Structure):
+    _fields_ = [("tolerance", D), ("relTol", D), ("maxIter", I), ("minIter", I), ("preconditioner", I),
+                ("checkEvery", I)]
+
+
+PRECOND_DILU, PRECOND_DIAGONAL = 0, 1
+BC_ZERO_GRADIENT, BC_FIXED_VALUE, BC_EMPTY = 0, 1, 2
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [(n, P) for n in ("dSf", "dCf", "dMagSf", "dC", "dV", "dWeights", "dDeltaCoeffs",
+                                 "dNonOrthDeltaCoeffs", "dCorrVec", "dBDeltaCoeffs")]
+
+
+class SimpleControls(ctypes.Structure):
+    _fields_ = [("nu", D), ("relaxU", D), ("relaxP", D), ("momentumPredictor", I), ("linearUpwind", I),
+                ("correctedLaplacian", I), ("nNonOrthCorr", I), ("pRefCell", I), ("pRefValue", D),
+                ("U", BicgControls), ("p", PcgControls), ("mergeLevels", I)]
+
+
+class SimpleFields(ctypes.Structure):
+    _fields_ = [(n, P) for n in ("dUx", "dUy", "dUz", "dUbx", "dUby", "dUbz", "dP", "dPb", "dPhi", "dPhiB")]
+
+
+class SimpleResult(ctypes.Structure):
+    _fields_ = [("UInitial", D * 3), ("UFinal", D * 3), ("UIters", I * 3), ("pInitial", D), ("pFinal", D),
+                ("pIters", I * 8), ("massCorr", D)]
+
+
 _SIGS = {
     "mfv_ctx_create": [I, P, P],
     "mfv_ctx_destroy": [P],
@@ -160,6 +189,12 @@ _SIGS = {
     "mfv_faceareapair_weights": [I, P, P, P],
     "mfv_gamg_solve": [P, P, P, P, P, P, P, P],
     "mfv_pcg_gamg_solve": [P, P, P, P, P, P, D, D, I, I, P, P],
+    "mfv_bicg_default_controls": [P],
+    "mfv_pbicgstab_solve": [P, P, P, P, P, P, P, P, P],
+    "mfv_simple_default_controls": [P],
+    "mfv_simple_create": [P, P, P, P, P, P, P, P, P],
+    "mfv_simple_destroy": [P],
+    "mfv_simple_step": [P, P, P, P, P],
     "mfv_server_create": [I, I, D, I, P],
     "mfv_server_destroy": [P],
     "mfv_server_request_bytes": [P, P],
@@ -471,3 +506,64 @@ def ptr3(ts):
     _KEEP.append(a)
     return a
 
+
+
+def bicg_controls(**kw):
+    """PBiCGStab controls with the library's defaults (1e-10, DILU), overridden by keyword"""
+    c = BicgControls()
+    lib().mfv_bicg_default_controls(ctypes.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError("bicg controls have no field " + k)
+        setattr(c, k, v)
+    return c
+
+
+def simple_controls(U=None, p=None, **kw):
+    """one SIMPLE iteration's controls: the library's defaults, then U/p solver fields (dicts) and the rest"""
+    c = SimpleControls()
+    lib().mfv_simple_default_controls(ctypes.byref(c))
+    for k, v in (U or {}).items():
+        setattr(c.U, k, v)
+    for k, v in (p or {}).items():
+        setattr(c.p, k, v)
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError("simple controls have no field " + k)
+        setattr(c, k, v)
+    return c
+
+
+class Simple:
+    """mfv_simple: one SIMPLE iteration's workspace and pressure hierarchy on a mesh"""
+
+    def __init__(self, mesh, geometry, uType, uValue, pType, pValue, controls):
+        import numpy as np
+        self.mesh = mesh
+        self._keep = [np.ascontiguousarray(uType, np.int32), np.ascontiguousarray(uValue, np.float64),
+                      np.ascontiguousarray(pType, np.int32), np.ascontiguousarray(pValue, np.float64)]
+        self.geometry = geometry
+        self.controls = controls
+        h = P()
+        mesh.ctx.call("mfv_simple_create", mesh.h, ctypes.byref(geometry), *[k.ctypes.data for k in self._keep],
+                      ctypes.byref(controls), ctypes.byref(h))
+        self.h = h.value
+        _LIVE.add(self)
+
+    def step(self, U, Ub, p, pb, phi, phiB, nuEff=None, nuEffB=None):
+        f = SimpleFields(*[ptr(t) for t in list(U) + list(Ub) + [p, pb, phi, phiB]])
+        r = SimpleResult()
+        rc = lib().mfv_simple_step(self.h, ctypes.byref(f), ptr(nuEff), ptr(nuEffB), ctypes.byref(r))
+        self.mesh.ctx._check(rc, "mfv_simple_step")
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mfv_simple_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -450,6 +450,94 @@ int mfv_pcg_gamg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A
                        mfv_perf* perf);
 
 /* ---------------------------------------------------------------------------------------------
+ * PBiCGStab for an asymmetric matrix with its uncoupled boundary coefficients (the reference's
+ * `pbicgstab`, src/OpenFOAM/matrices/pbicgstab.cu:6-150, the solver solveVector runs per momentum
+ * component, src/OpenFOAM/matrices/solve_vector.cuh:27-116). The raw matrix comes in with internalCoeffs
+ * and boundaryCoeffs per boundary face (dIC, dBC, in the mesh's boundary-face order); the solve folds them
+ * (diag + iC, source + bC) as the reference does. MFV_PRECOND_DILU is the reference's DILU preconditioner,
+ * walked level by level in the host loop's order (with the context's sequential reductions the solve is the
+ * host solve to the bit); MFV_PRECOND_DIAGONAL divides by the folded diagonal. Synchronises every
+ * checkEvery iterations.
+ * ------------------------------------------------------------------------------------------- */
+#define MFV_PRECOND_DILU 0
+#define MFV_PRECOND_DIAGONAL 1
+typedef struct
+{
+    double tolerance, relTol;
+    int maxIter, minIter;
+    int preconditioner;    /* MFV_PRECOND_* */
+    int checkEvery;
+} mfv_bicg_controls;
+/* the reference's defaults for a momentum component (cpu::StepInput): 1e-10, 0, 1000, 0, DILU, poll every 4 */
+int mfv_bicg_default_controls(mfv_bicg_controls* controls);
+int mfv_pbicgstab_solve(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dSource, const double* dIC,
+                        const double* dBC, double* dPsi, const mfv_bicg_controls* controls, mfv_perf* perf);
+
+/* ---------------------------------------------------------------------------------------------
+ * One SIMPLE iteration on the device: cpu::simpleStep (src/applications/solvers/simpleFoam/
+ * simpleFoam_cpp.cu:15-237; its device twin gpu::simpleStep, simpleFoam.cu:92-420) for uncoupled patches
+ * without MRF, fvOptions or SIMPLEC. UEqn = div(phi,U) (upwind, optionally linearUpwind's deferred
+ * correction) - laplacian(nuEff,U) (optionally corrected) - div(nuEff dev2(T(grad U))), relaxed; the
+ * momentum predictor with PBiCGStab per component; rAU, HbyA, constrainHbyA, phiHbyA, adjustPhi (when
+ * pRefCell >= 0); nNonOrthCorr + 1 pressure solves by AMG-PCG with phi corrected after the last; p relaxed;
+ * U corrected. Boundary conditions per boundary face: MFV_BC_* (noSlip = fixedValue 0).
+ * Geometry: the arrays mfv_mesh_geometry writes (device).
+ * ------------------------------------------------------------------------------------------- */
+#define MFV_BC_ZERO_GRADIENT 0
+#define MFV_BC_FIXED_VALUE 1
+#define MFV_BC_EMPTY 2
+typedef struct mfv_simple_s* mfv_simple;
+typedef struct
+{
+    const double *dSf, *dCf, *dMagSf;   /* internal then boundary faces (xyz for Sf, Cf) */
+    const double *dC, *dV;              /* cells */
+    const double *dWeights, *dDeltaCoeffs, *dNonOrthDeltaCoeffs, *dCorrVec;  /* internal faces */
+    const double* dBDeltaCoeffs;        /* boundary faces */
+} mfv_geometry;
+typedef struct
+{
+    double nu;                          /* laminar viscosity (nuEff when the step is given none) */
+    double relaxU, relaxP;              /* relaxationFactors: equations U, fields p */
+    int momentumPredictor;
+    int linearUpwind;                   /* div(phi,U) Gauss linearUpwind grad(U) (else upwind) */
+    int correctedLaplacian;             /* laplacianSchemes Gauss linear corrected (nonOrthDeltaCoeffs) */
+    int nNonOrthCorr;                   /* nNonOrthogonalCorrectors, 0..7 */
+    int pRefCell;                       /* -1: p is fixed on a patch (no reference, no adjustPhi) */
+    double pRefValue;
+    mfv_bicg_controls U;                /* the momentum components' PBiCGStab */
+    mfv_pcg_controls p;                 /* the pressure's AMG-PCG */
+    int mergeLevels;                    /* pairwise passes per pressure AMG level */
+} mfv_simple_controls;
+typedef struct
+{
+    double *dUx, *dUy, *dUz;            /* cells */
+    double *dUbx, *dUby, *dUbz;         /* boundary faces: U's patch values */
+    double *dP, *dPb;                   /* p in cells and on the boundary faces */
+    double *dPhi, *dPhiB;               /* face fluxes: internal, boundary */
+} mfv_simple_fields;
+typedef struct
+{
+    double UInitial[3], UFinal[3];      /* per component; OpenFOAM reports the max over solved components */
+    int UIters[3];
+    double pInitial, pFinal;            /* the first solve's initial, the last solve's final residual */
+    int pIters[8];                      /* per pressure solve */
+    double massCorr;                    /* adjustPhi's factor (1 without a reference cell) */
+} mfv_simple_result;
+/* cpu::StepInput's defaults: nu 1e-5, U 0.7, p 0.3, momentum predictor, upwind, uncorrected, no
+ * non-orthogonal correctors, no reference cell; PBiCGStab-DILU to 1e-10; AMG-PCG FP64 to 1e-10 */
+int mfv_simple_default_controls(mfv_simple_controls* controls);
+/* builds the step's workspace and the pressure hierarchy (synchronises); hUType/hPType: MFV_BC_* per
+ * boundary face, hUValue 3 per face (xyz), hPValue 1 per face */
+int mfv_simple_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry, const int* hUType,
+                      const double* hUValue, const int* hPType, const double* hPValue,
+                      const mfv_simple_controls* controls, mfv_simple* out);
+int mfv_simple_destroy(mfv_simple simple);
+/* one iteration in place on the fields; dNuEff/dNuEffB (nullable): nuEff per cell and boundary face in
+ * place of nu (nu + nut of the turbulence model's last correction). Synchronises. */
+int mfv_simple_step(mfv_simple simple, const mfv_simple_fields* fields, const double* dNuEff,
+                    const double* dNuEffB, mfv_simple_result* result);
+
+/* ---------------------------------------------------------------------------------------------
  * Serving: a stream of pressure-Poisson requests (no counterpart in the reference, which runs one
  * case per process). Each request is a box of nx*ny*nz hex cells (lengths lx, ly, lz; interior points
  * jittered by +-jitter of the spacing), solved nSteps times for right-hand sides -div(U) of seeded
