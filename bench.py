@@ -49,7 +49,7 @@ def parse():
                     help="the mixed path's V-cycle (the other one is timed beside it)")
     ap.add_argument("--no-other", action="store_true", help="skip timing the other mixed precision")
     ap.add_argument("--tolerance", type=float, default=1e-6)
-    ap.add_argument("--cpu-sample", default="250x125x80", help="channel the CPU oracle is timed on")
+    ap.add_argument("--cpu-sample", default="160x80x50", help="channel the CPU oracle is timed on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--graph", type=int, default=0, help="1: replay captured groups of PCG iterations")
@@ -65,6 +65,11 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--predictor", type=int, default=1,
                     help="1: also time the pressure predictor's glue (rAU, HbyA, constrainHbyA, phiHbyA, adjustPhi)")
+    ap.add_argument("--simple", type=int, default=1, help="1: time whole SIMPLE iterations (the headline)")
+    ap.add_argument("--nu", type=float, default=1e-5, help="laminar viscosity of the SIMPLE iterations")
+    ap.add_argument("--turbulence", type=int, default=1, help="1: k-epsilon with wall functions; 0: laminar")
+    ap.add_argument("--tol-u", type=float, default=1e-5, help="momentum solve tolerance")
+    ap.add_argument("--reltol-u", type=float, default=0.1, help="momentum solve relative tolerance")
     ap.add_argument("--gamg-step", type=int, default=0,
                     help="1: also time the step with OpenFOAM's GAMG-PCG on the device (the CPU baseline's solver)")
     return ap.parse_args()
@@ -78,12 +83,15 @@ class Channel:
         self.shape = (nx, ny, nz)
         pts = hexmesh.box_points(nx, ny, nz, (4.0, 1.0, 0.5), (1.0, 20.0, 20.0), 0.1, seed=2024)
         faces, owner, nei, patches = hexmesh.box_topology(nx, ny, nz)
+        self.points, self.faces = pts, faces
         self.nC, self.nI = nx * ny * nz, nei.size
         self.nB = owner.size - self.nI
         self.patches = patches
         log("mesh %dx%dx%d: %d cells, %d internal faces (%.1fs)" % (nx, ny, nz, self.nC, self.nI, time.time() - t0))
-        self.mesh = mfv.Mesh(ctx, self.nC, np.ascontiguousarray(owner[:self.nI]), nei,
-                             np.ascontiguousarray(owner[self.nI:]))
+        self.own = np.ascontiguousarray(owner[:self.nI])
+        self.nei = nei
+        self.bcell = np.ascontiguousarray(owner[self.nI:])
+        self.mesh = mfv.Mesh(ctx, self.nC, self.own, nei, self.bcell)
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
         z = lambda n: torch.zeros(int(n), dtype=torch.float64, device="cuda:0")
         nF = owner.size
@@ -138,6 +146,186 @@ class Channel:
         log("fields ready (%.1fs)" % (time.time() - t0))
 
 
+def channel_bcs(ch, torch):
+    """The channel's boundary conditions per boundary face: inlet U fixedValue (10,0,0) with p
+    zeroGradient; outlet U zeroGradient with p fixedValue 0; the four walls noSlip with p zeroGradient."""
+    from brae_amd import mfv
+    nB, nI = ch.nB, ch.nI
+    uT = np.full(nB, mfv.BC_FIXED_VALUE, np.int32)
+    pT = np.full(nB, mfv.BC_ZERO_GRADIENT, np.int32)
+    uV = np.zeros((nB, 3))
+    pV = np.zeros(nB)
+    for name, start, size in ch.patches:
+        s = slice(start - nI, start - nI + size)
+        if name == "inlet":
+            uV[s] = (10.0, 0.0, 0.0)
+        elif name == "outlet":
+            uT[s] = mfv.BC_ZERO_GRADIENT
+            pT[s] = mfv.BC_FIXED_VALUE
+    return uT, uV, pT, pV
+
+
+def simple_fields(ch, torch, uT, uV, seed=7):
+    """Seeded initial fields: U = (10,0,0) + uniform(-1,1) per component, U's patch values from the
+    boundary conditions, phi = interpolate(U).Sf inside and Ub.Sf on the boundary, p = 0."""
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(seed)
+    nC, nI, nB = ch.nC, ch.nI, ch.nB
+    U = torch.rand((nC, 3), dtype=torch.float64, device="cuda:0", generator=g) * 2.0 - 1.0
+    U[:, 0] += 10.0
+    bcell = torch.from_numpy(ch.bcell).to("cuda:0").long()
+    fixed = torch.from_numpy(uT == 1).to("cuda:0")
+    Ub = torch.where(fixed[:, None], torch.from_numpy(uV).to("cuda:0"), U[bcell])
+    own_d = torch.from_numpy(ch.own).to("cuda:0").long()
+    nei_d = torch.from_numpy(ch.nei).to("cuda:0").long()
+    nF = nI + nB
+    Sf = ch.Sf.view(nF, 3)
+    wv = ch.w.view(-1, 1)
+    phi = ((wv * U[own_d] + (1.0 - wv) * U[nei_d]) * Sf[:nI]).sum(1).contiguous()
+    phiB = (Ub * Sf[nI:]).sum(1).contiguous()
+    fields = dict(U=[U[:, k].contiguous() for k in range(3)], Ub=[Ub[:, k].contiguous() for k in range(3)],
+                  p=torch.zeros(nC, dtype=torch.float64, device="cuda:0"),
+                  pb=torch.zeros(nB, dtype=torch.float64, device="cuda:0"), phi=phi, phiB=phiB)
+    torch.cuda.synchronize()
+    return fields
+
+
+def channel_kepsilon(ch, torch, mfv, args):
+    """The channel's k-epsilon set-up: wall functions on the four walls (their near-wall distance from the
+    library's host nearWallDist), inlet k = 0.375 and epsilon = 14.855, zeroGradient elsewhere; initial k and
+    epsilon uniform at the inlet values, nut = Cmu k^2/epsilon (cells and boundary faces)."""
+    t0 = time.time()
+    nB, nI, nC = ch.nB, ch.nI, ch.nC
+    isWall = np.zeros(nB, np.int32)
+    ktype = np.full(nB, mfv.BC_ZERO_GRADIENT, np.int32)
+    kV, eV = np.zeros(nB), np.zeros(nB)
+    for name, start, size in ch.patches:
+        s = slice(start - nI, start - nI + size)
+        if name in ("lowerWall", "upperWall", "back", "front"):
+            isWall[s] = 1
+        elif name == "inlet":
+            ktype[s] = mfv.BC_FIXED_VALUE
+            kV[s], eV[s] = 0.375, 14.855
+    nF = nI + nB
+    y = mfv.near_wall_dist(nI, ch.points, np.arange(0, 4 * nF + 1, 4, dtype=np.int32), ch.faces.reshape(-1),
+                           ch.bcell, ch.C.cpu().numpy(), isWall)
+    log("near-wall distance on %d wall faces (%.1fs host)" % (int(isWall.sum()), time.time() - t0))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    k0 = torch.full((nC,), 0.375, dtype=torch.float64, device="cuda:0")
+    e0 = torch.full((nC,), 14.855, dtype=torch.float64, device="cuda:0")
+    bc = torch.from_numpy(ch.bcell).to("cuda:0").long()
+    fixed = dev(ktype == mfv.BC_FIXED_VALUE)
+    kb0 = torch.where(fixed, dev(kV), k0[bc])
+    eb0 = torch.where(fixed, dev(eV), e0[bc])
+    fields0 = dict(k=k0, kb=kb0.contiguous(), eps=e0, epsb=eb0.contiguous(), nut=(0.09 * k0 * k0 / e0).contiguous(),
+                   nutb=(0.09 * kb0 * kb0 / eb0).contiguous())
+    controls = mfv.kepsilon_controls(solver=dict(tolerance=args.tol_u, relTol=args.reltol_u, maxIter=1000,
+                                                 preconditioner=mfv.PRECOND_DIAGONAL, checkEvery=args.check_every),
+                                     nu=args.nu, relaxEps=0.7, relaxK=0.7)
+    return dict(isWall=isWall, y=y, type=ktype, kV=kV, eV=eV, fields0=fields0, controls=controls)
+
+
+def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
+    """Whole SIMPLE iterations on the channel (the headline): UEqn (upwind convection with linearUpwind's
+    deferred correction, corrected laplacian(nuEff, U), the explicit stress, relax 0.7), the momentum
+    predictor (PBiCGStab per component), the pressure predictor, nNonOrthogonalCorrectors + 1 pressure
+    solves (AMG-PCG to --tolerance) with the corrected laplacian, phi, p relaxed by 0.3 and U corrected.
+    The mixed path and the FP64 path run the same iterations from the same seeded fields; only the
+    pressure V-cycle's precision differs (and the outer CG is flexible when the cycle is not FP64)."""
+    uT, uV, pT, pV = channel_bcs(ch, torch)
+    geom = mfv.Geometry(*(mfv.ptr(t) for t in (ch.Sf, ch.Cf, ch.magSf, ch.C, ch.V, ch.w, ch.dc, ch.ndc, ch.corr,
+                                                ch.bDelta)))
+    U_ctl = dict(tolerance=args.tol_u, relTol=args.reltol_u, maxIter=1000,
+                 preconditioner=mfv.PRECOND_DIAGONAL, checkEvery=args.check_every)
+
+    def controls(prec):
+        return mfv.simple_controls(
+            U=U_ctl, p=dict(tolerance=args.tolerance, relTol=0.0, precision=prec, nPreSweeps=args.npre,
+                            nPostSweeps=args.npost, flexible=int(prec != mfv.PREC_FP64 or not args.no_scale),
+                            scaleCorrection=int(not args.no_scale), checkEvery=args.check_every),
+            mergeLevels=args.merge_levels, nu=args.nu, relaxU=0.7, relaxP=0.3, linearUpwind=1,
+            correctedLaplacian=1, nNonOrthCorr=args.non_orth, pRefCell=-1)
+
+    turb = None
+    if args.turbulence:
+        turb = channel_kepsilon(ch, torch, mfv, args)
+    runs = [("mixed", prec_mixed), ("fp64", mfv.PREC_FP64)] + ([] if args.no_other else [("other", other)])
+    out = {}
+    for name, prec in runs:
+        t0 = time.time()
+        sim = mfv.Simple(ch.mesh, geom, uT, uV.ravel(), pT, pV, controls(prec))
+        ke = None
+        if turb is not None:
+            ke = mfv.KEpsilon(ch.mesh, geom, turb["isWall"], turb["y"], turb["type"], turb["kV"], turb["type"],
+                              turb["eV"], turb["controls"])
+        log("SIMPLE %s: hierarchy built (%.1fs)" % (name, time.time() - t0))
+
+        def fresh():
+            f = simple_fields(ch, torch, uT, uV)
+            if turb is not None:
+                f.update({k: v.clone() for k, v in turb["fields0"].items()})
+            torch.cuda.synchronize()
+            return f
+
+        def iterate(f):
+            a = (f["U"], f["Ub"], f["p"], f["pb"], f["phi"], f["phiB"])
+            if ke is None:
+                return sim.step(*a), None
+            return mfv.simple_step_kepsilon(sim, ke, *a, f["k"], f["kb"], f["eps"], f["epsb"], f["nut"], f["nutb"])
+
+        f = fresh()
+        for _ in range(args.warmup):
+            iterate(f)
+        # the timed iterations start again from the seeded fields
+        del f
+        f = fresh()
+        t = time.perf_counter()
+        res = [iterate(f) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        # one more iteration with events around its phases (outside the timed ones)
+        ctx.profile(True)
+        for tag in mfv.PROF_SIMPLE:
+            ctx.profile_read(tag)
+        iterate(f)
+        phases = {mfv.PROF_SIMPLE[tag]: round(ctx.profile_read(tag)[1], 3) for tag in mfv.PROF_SIMPLE}
+        ctx.profile(False)
+        phases = {k: v for k, v in phases.items() if v > 0}
+        out[name] = dict(dt=dt, res=[r for r, _ in res], keres=[kr for _, kr in res], p=f["p"].clone(),
+                         U=torch.stack(f["U"]).clone(), phases_ms=phases,
+                         k=None if turb is None else f["k"].clone(), eps=None if turb is None else f["eps"].clone())
+        rr = out[name]["res"]
+        log("SIMPLE %s: %.4f s/iter; p iterations %s; U iterations %s; final p residuals %s; phases (ms) %s" % (
+            name, dt, [list(r.pIters[:args.non_orth + 1]) for r in rr], [list(r.UIters) for r in rr],
+            ["%.2e" % r.pFinal for r in rr], phases))
+        if turb is not None:
+            log("  k-epsilon: eps/k iterations %s" % [(kr.epsIters, kr.kIters) for kr in out[name]["keres"]])
+        sim.close()
+        if ke is not None:
+            ke.close()
+        del f
+        torch.cuda.synchronize()
+    m, d = out["mixed"], out["fp64"]
+    line = dict(
+        dt_mixed=m["dt"], dt_fp64=d["dt"],
+        p_iterations={k: [list(r.pIters[:args.non_orth + 1]) for r in v["res"]] for k, v in out.items()},
+        U_iterations={k: [list(r.UIters) for r in v["res"]] for k, v in out.items()},
+        final_p_residual={k: max(r.pFinal for r in v["res"]) for k, v in out.items()},
+        p_rel_l2_vs_fp64={k: float((v["p"] - d["p"]).norm() / d["p"].norm()) for k, v in out.items() if k != "fp64"},
+        U_rel_l2_vs_fp64={k: float((v["U"] - d["U"]).norm() / d["U"].norm()) for k, v in out.items() if k != "fp64"},
+        phases_ms={k: v["phases_ms"] for k, v in out.items()})
+    if turb is not None:
+        line["ke_iterations"] = {k: [(kr.epsIters, kr.kIters) for kr in v["keres"]] for k, v in out.items()}
+        line["k_rel_l2_vs_fp64"] = {k: float((v["k"] - d["k"]).norm() / d["k"].norm())
+                                    for k, v in out.items() if k != "fp64"}
+        line["eps_rel_l2_vs_fp64"] = {k: float((v["eps"] - d["eps"]).norm() / d["eps"].norm())
+                                      for k, v in out.items() if k != "fp64"}
+        line["wall_cells"] = out["mixed"]["keres"][0].nWallCells
+    if "other" in out:
+        line["dt_other"] = out["other"]["dt"]
+    return line
+
+
 def gpu_run(args):
     import torch
     from brae_amd import hexmesh, mfv
@@ -148,6 +336,9 @@ def gpu_run(args):
     torch.cuda.set_stream(stream)
     ctx = mfv.Context(0, stream=stream.cuda_stream)
     ch = Channel(nx, ny, nz, torch, mfv, hexmesh, ctx)
+    prec_mixed = mfv.PREC_FP32 if args.precision == "fp32" else mfv.PREC_FP16
+    other = mfv.PREC_FP16 if prec_mixed == mfv.PREC_FP32 else mfv.PREC_FP32
+    simple = simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other) if args.simple else None
     t0 = time.time()
     w_host = ch.magSfI.cpu().numpy()
     amg = mfv.Amg(ch.mesh, w_host, mfv.AGG_BRAE, 64, mergeLevels=args.merge_levels)
@@ -155,7 +346,6 @@ def gpu_run(args):
     log("AMG hierarchy: %d grids, coarsest %d cells (%.1fs host build)" % (amg.nGrids, sizes[-1][0], time.time() - t0))
     P = mfv.ptr
     A = mfv.ldu(ch.diagC, ch.upper)
-    prec_mixed = mfv.PREC_FP32 if args.precision == "fp32" else mfv.PREC_FP16
 
     def step(prec, gamg=None):
         """one SIMPLE pressure corrector: nNonOrthogonalCorrectors + 1 assemblies and solves of the same
@@ -209,7 +399,6 @@ def gpu_run(args):
         return dt, perfs, prof
 
     res = {}
-    other = mfv.PREC_FP16 if prec_mixed == mfv.PREC_FP32 else mfv.PREC_FP32
     runs = [("mixed", prec_mixed), ("fp64", mfv.PREC_FP64)] + ([] if args.no_other else [("other", other)])
     for name, prec in runs:
         dt, perfs, _ = timed(prec, profile=False)
@@ -301,7 +490,7 @@ def gpu_run(args):
                       launches=prof[dom][0], noop_launches=noop,
                       copy_peak_GBps=round(copy_gbs, 1), triad_GBps=round(triad_gbs, 1)),
         per_kernel=per_kernel, vcycle_avg_us=round(vc_ms / max(1, iters) * 1e3, 1), grids=grids,
-        hbm_used_gib=round(hbm_used_gib, 2), other=other_out,
+        hbm_used_gib=round(hbm_used_gib, 2), other=other_out, simple=simple,
     )
     if args.predictor:
         out["predictor"] = predictor_run(ch, ctx, patches=ch.patches)
@@ -546,69 +735,90 @@ def measured_traffic(label):
     return None
 
 
+def cpu_info():
+    """(logical CPUs, CPU model) of this host"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 0, model
+
+
 def cpu_baseline(args, n_full):
-    """The oracle's FP64 pressure corrector on a reduced channel of the same construction,
-    single-threaded, scaled to the full cell count: the same passes as the GPU step (assembly, then per
-    non-orthogonal pass grad(p), correction flux and source, div(phiHbyA), fold and OpenFOAM's
-    GAMG-preconditioned PCG (faceAreaPair, GaussSeidel, V(0,2,2), nVcycles 2), then the flux)."""
+    """The oracle's FP64 SIMPLE iteration -- the reference CPU path: PBiCGStab-DILU momentum solves,
+    gamg.cu's GAMG (agglomeratePair, GaussSeidel, dense coarsest) for the pressure, then the k-epsilon
+    correct() with PBiCGStab-DILU solves -- on a reduced channel of the same construction and boundaries,
+    single-threaded (the reference's host path has no threading), from the same kind of seeded fields.
+    One untimed iteration, then one timed; the time is scaled linearly to the full cell count."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from brae_amd import hexmesh
     nx, ny, nz = (int(v) for v in args.cpu_sample.split("x"))
     m = hexmesh.box_mesh(nx, ny, nz, (4.0, 1.0, 0.5), (1.0, 20.0, 20.0), 0.1, seed=2024)
     nC, nI = m["nCells"], m["nInternal"]
-    nB = m["faces"].shape[0] - nI
-    rng = np.random.default_rng(7)
-    U = rng.uniform(-1.0, 1.0, (nC, 3))
-    U[:, 0] += 10.0
-    own, nei = np.ascontiguousarray(m["owner"][:nI]), m["neighbour"]
-    w = m["weights"][:, None]
-    phi = ((w * U[own] + (1 - w) * U[nei]) * m["Sf"][:nI]).sum(1)
-    phiB = np.zeros(nB)
-    types = np.zeros(nB, np.int32)
-    bcell = np.ascontiguousarray(m["owner"][nI:])
+    nB = len(m["owner"]) - nI
+    mesh = O.Mesh(m["points"], np.arange(0, 4 * m["faces"].shape[0] + 1, 4), m["faces"].ravel(), m["owner"],
+                  m["neighbour"])
+    uT = np.full(nB, O.BC_FIXED_VALUE, np.int32)
+    pT = np.full(nB, O.BC_ZERO_GRADIENT, np.int32)
+    uV = np.zeros((nB, 3))
+    isWall = np.zeros(nB, np.int32)
+    kT = np.full(nB, O.BC_ZERO_GRADIENT, np.int32)
+    kV, eV = np.zeros(nB), np.zeros(nB)
     for name, start, size in m["patches"]:
         s = slice(start - nI, start - nI + size)
         if name == "inlet":
-            phiB[s] = 10.0 * m["Sf"][start:start + size, 0]
+            uV[s] = (10.0, 0.0, 0.0)
+            kT[s] = O.BC_FIXED_VALUE
+            kV[s], eV[s] = 0.375, 14.855
         elif name == "outlet":
-            phiB[s] = (U[bcell[s]] * m["Sf"][start:start + size]).sum(1)
-            types[s] = 1
-    negRAU = -0.05 * np.cbrt(m["V"]) / np.linalg.norm(U, axis=1)
-    wts = O.faceareapair_weights(m["Sf"][:nI], m["magSf"][:nI])
-    ag = O.of_agglomeration(nC, own, nei, wts, nCellsInCoarsestLevel=10)
-    t = time.perf_counter()
-    gf = m["weights"] * negRAU[own] + (1 - m["weights"]) * negRAU[nei]
-    diag, up = np.zeros(nC), np.zeros(nI)
-    O.lib().orc_laplacian(nC, nI, O.p(own), O.p(nei), O.p(m["nonOrthDeltaCoeffs"]), O.p(gf),
-                          O.p(O.f64(m["magSf"][:nI])), O.p(diag), O.p(up), None)
-    iC, bC = np.zeros(nB), np.zeros(nB)
-    O.lib().orc_laplacian_bcoeffs(nB, O.p(types), O.p(np.zeros(nB)), O.p(O.f64(negRAU[bcell])),
-                                  O.p(O.f64(m["magSf"][nI:])), O.p(O.f64(m["bDeltaCoeffs"])), O.p(iC), O.p(bC))
-    div = np.zeros(nC)
-    np.add.at(div, own, phi)
-    np.add.at(div, nei, -phi)
-    np.add.at(div, bcell, phiB)
-    psi = np.zeros(nC)
-    iters = []
-    for _ in range(args.non_orth + 1):
-        g = O.grad_gauss(nC, own, nei, bcell, types, np.zeros(nB), m["Sf"], m["weights"], m["V"], psi)
-        ffc, src = O.laplacian_corr(nC, own, nei, g, m["weights"], m["corrVec"], gf, m["magSf"][:nI],
-                                    source=np.zeros(nC))
-        src = src - (div / m["V"]) * m["V"]
-        diagC, b = np.zeros(nC), np.zeros(nC)
-        O.lib().orc_fold(nC, O.p(diag), O.p(src), nB, O.p(bcell), O.p(types), O.p(iC), O.p(bC), O.p(diagC), O.p(b))
-        psi, perf = O.pcg_gamg_solve(ag, diagC, up, b, psi, args.tolerance, smoother="GaussSeidel")
-        iters.append(perf.nIterations)
-    O.flux_correct(own, nei, up, None, psi, ffc, phi)
-    sec = time.perf_counter() - t
-    scaled = sec * n_full / nC
-    return dict(value=round(scaled, 3), unit="s/SIMPLE-iter (pressure corrector) @ full size, linear in cells",
-                cores=1, kind="port",
-                sample="oracle FP64 pressure corrector (assembly, %d non-orthogonal passes of grad/correction/"
-                       "fold/OpenFOAM GAMG-PCG GaussSeidel V(0,2,2) nVcycles 2 to %g, flux) on the %s channel "
-                       "(%d cells): %.2f s, PCG iterations %s; scaled x%.1f (hierarchy build excluded)"
-                       % (args.non_orth + 1, args.tolerance, args.cpu_sample, nC, sec, iters, n_full / nC))
+            uT[s] = O.BC_ZERO_GRADIENT
+            pT[s] = O.BC_FIXED_VALUE
+        else:
+            isWall[s] = 1
+    rng = np.random.default_rng(7)
+    U0 = rng.uniform(-1.0, 1.0, (nC, 3))
+    U0[:, 0] += 10.0
+    U = [U0[:, k].copy() for k in range(3)]
+    Ub = [O.evaluate_boundary(mesh.bFaceCell, uT, uV[:, k], U[k]) for k in range(3)]
+    own, nei = mesh.own(), mesh.neighbour
+    w = mesh.weights[:, None]
+    phi = ((w * U0[own] + (1 - w) * U0[nei]) * mesh.Sf[:nI]).sum(1)
+    phiB = (np.stack(Ub, 1) * mesh.Sf[nI:]).sum(1)
+    st = O.SimpleState(U, Ub, np.zeros(nC), np.zeros(nB), phi, phiB)
+    k = np.full(nC, 0.375)
+    e = np.full(nC, 14.855)
+    nut = 0.09 * k * k / e
+    kb = O.evaluate_boundary(mesh.bFaceCell, kT, kV, k)
+    eb = O.evaluate_boundary(mesh.bFaceCell, kT, eV, e)
+    nutb = 0.09 * kb * kb / np.where(eb > 0, eb, 1.0)
+    y = O.near_wall_dist(mesh, isWall)
+    kcs = O.kepsilon_case(mesh, isWall, y, kT, kV, kT, eV, nu=args.nu, relaxEps=0.7, relaxK=0.7, tol=args.tol_u,
+                          relTol=args.reltol_u, maxIter=1000)
+    times, iters = [], []
+    for it in range(2):
+        t = time.perf_counter()
+        cs = O.simple_case(mesh, uT, uV.ravel(), pT, np.zeros(nB), nu=args.nu, tolU=args.tol_u, relTolU=args.reltol_u,
+                           tolP=args.tolerance, pSolver=O.P_GAMG_REF, linearUpwind=1, correctedLaplacian=1,
+                           nNonOrthCorr=args.non_orth, nuEff=args.nu + nut, nuEffB=args.nu + nutb)
+        r = O.simple_step(cs, st)
+        O.kepsilon_correct(kcs, st.U, st.Ub, st.phi, st.phiB, k, kb, e, eb, nut, nutb)
+        times.append(time.perf_counter() - t)
+        iters.append(list(r.pIters[:args.non_orth + 1]))
+    sec = times[-1]
+    ncpu, model = cpu_info()
+    return dict(value=round(sec * n_full / nC, 3), unit="s/SIMPLE-iter @ full size, linear in cells",
+                cores=1, kind="port", host_cpus=ncpu, cpu_model=model,
+                sample="oracle FP64 turbulent SIMPLE iteration (kEpsilon, wall functions; U PBiCGStab-DILU to %g "
+                       "relTol %g; p gamg.cu GAMG to %g; k/epsilon PBiCGStab-DILU) on the %s channel (%d cells), one "
+                       "untimed then one timed iteration: %.2f s (pressure solves %s); scaled x%.1f"
+                       % (args.tol_u, args.reltol_u, args.tolerance, args.cpu_sample, nC, sec, iters[-1],
+                          n_full / nC))
 
 
 def main():
@@ -621,34 +831,62 @@ def main():
     r = gpu_run(args)
     serving = serving_run(args) if args.serving > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, nx * ny * nz)
+    sim = r.get("simple")
+    pc = {
+        "s_per_step": round(r["dt_mixed"], 5), "fp64_s_per_step": round(r["dt_fp64"], 5),
+        "speedup_mixed_vs_fp64": round(r["dt_fp64"] / r["dt_mixed"], 3),
+        "pcg_iterations": {"mixed": r["iters_mixed"], "fp64": r["iters_fp64"]},
+        "final_residual": {"mixed": r["final_mixed"], "fp64": r["final_fp64"]},
+        "p_rel_l2_mixed_vs_fp64": r["p_rel_l2_mixed_vs_fp64"], "other_mixed": r["other"],
+        "workload": "the pressure corrector alone (interpolate(rAU), laplacian, Galerkin, %d x (grad(p), correction, "
+                    "div(phiHbyA), fold, AMG-PCG to %g from p = 0 then the first solution), flux) on seeded "
+                    "phiHbyA and rAU" % (args.non_orth + 1, args.tolerance)}
+    if sim is not None:
+        value, fp64 = sim["dt_mixed"], sim["dt_fp64"]
+        unit = "s/SIMPLE-iter"
+        workload = ("SIMPLE iteration (%s, nu %g; linearUpwind U, corrected laplacian, nNonOrthogonalCorrectors "
+                    "%d; U%s PBiCGStab-diagonal to %g relTol %g; p AMG-PCG to %g), %s graded (ratio 20) jittered "
+                    "(10%%) channel 4x1x0.5, %d cells"
+                    % ("kEpsilon with nutk/epsilon wall functions, inlet k 0.375 epsilon 14.855, relax k/epsilon 0.7"
+                       if args.turbulence else "laminar", args.nu, args.non_orth, ", k, epsilon" if args.turbulence
+                       else "", args.tol_u, args.reltol_u, args.tolerance, args.size, nx * ny * nz))
+        data = "synthetic (seeded perturbed plug flow U = (10,0,0) + uniform(-1,1), p = 0; %d timed iterations " \
+               "from the seeded fields)" % args.steps
+    else:
+        value, fp64 = r["dt_mixed"], r["dt_fp64"]
+        unit = "s/SIMPLE-iter (pressure corrector)"
+        workload = "SIMPLE pressure corrector, %s channel, %d cells" % (args.size, nx * ny * nz)
+        data = "synthetic (seeded perturbed plug flow on the graded jittered channel; p solved from 0)"
     line = {
         "metric": METRIC,
-        "value": round(r["dt_mixed"], 5),
-        "unit": "s/SIMPLE-iter (pressure corrector)",
+        "value": round(value, 5),
+        "unit": unit,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["dt_mixed"] * 1e3, 2),
+        "ms_per_step": round(value * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 outer / %s V-cycle" % args.precision,
-        "data": "synthetic (seeded perturbed plug flow on the graded jittered channel; p solved from 0)",
-        "config": {"workload": "SIMPLE pressure corrector (nNonOrthogonalCorrectors %d), %s graded (ratio 20) "
-                               "jittered (10%%) channel 4x1x0.5, %d cells, AMG-PCG to %g"
-                               % (args.non_orth, args.size, nx * ny * nz, args.tolerance),
-                   "cells": nx * ny * nz, "tolerance": args.tolerance, "v_cycle": args.precision},
-        "fp64_s_per_iter": round(r["dt_fp64"], 5),
-        "speedup_mixed_vs_fp64": round(r["dt_fp64"] / r["dt_mixed"], 3),
-        "pcg_iterations": {"mixed": r["iters_mixed"], "fp64": r["iters_fp64"]},
-        "final_residual": {"mixed": r["final_mixed"], "fp64": r["final_fp64"]},
-        "p_rel_l2_mixed_vs_fp64": r["p_rel_l2_mixed_vs_fp64"],
+        "data": data,
+        "config": {"workload": workload, "cells": nx * ny * nz, "tolerance": args.tolerance,
+                   "v_cycle": args.precision},
+        "fp64_s_per_iter": round(fp64, 5),
+        "speedup_mixed_vs_fp64": round(fp64 / value, 3),
+        "simple": None if sim is None else {
+            "other_mixed_s_per_iter": None if "dt_other" not in sim else round(sim["dt_other"], 5),
+            "p_iterations": sim["p_iterations"], "U_iterations": sim["U_iterations"],
+            "final_p_residual": sim["final_p_residual"], "p_rel_l2_vs_fp64": sim["p_rel_l2_vs_fp64"],
+            "U_rel_l2_vs_fp64": sim["U_rel_l2_vs_fp64"], "phases_ms": sim["phases_ms"],
+            "ke_iterations": sim.get("ke_iterations"), "k_rel_l2_vs_fp64": sim.get("k_rel_l2_vs_fp64"),
+            "eps_rel_l2_vs_fp64": sim.get("eps_rel_l2_vs_fp64"), "wall_cells": sim.get("wall_cells")},
+        "pressure_corrector": pc,
         "amg": {"grids": r["n_grids"], "coarsest_cells": r["coarsest"], "vcycle_avg_us": r["vcycle_avg_us"],
                 "mergeLevels": args.merge_levels,
                 "cycle": "V(%d,%d) weighted Jacobi%s" % (args.npre, args.npost,
                                                         "" if args.no_scale else ", scaled correction"),
                 "per_grid_us": r["grids"]},
-        "other_mixed": r["other"],
         "openfoam_gamg_gpu": r.get("gamg_step"),
         "pressure_predictor": r.get("predictor"),
         "hbm_used_gib": r["hbm_used_gib"],

@@ -34,6 +34,14 @@ struct mfv_simple_s
     std::unique_ptr<mfv::SimpleSolver> s;
 };
 
+struct mfv_kepsilon_s
+{
+    mfv_ctx ctx;
+    mfv_mesh mesh;
+    std::unique_ptr<mfv::KEpsilon> k;
+    mfv::DBuf<double> nuEffC, nuEffB;   // the turbulent step's nu + nut
+};
+
 namespace mfv {
 
 namespace {
@@ -1393,6 +1401,187 @@ int mfv_simple_step(mfv_simple simple, const mfv_simple_fields* f, const double*
             for (int k = 0; k < 8; ++k) result->pIters[k] = r.pIters[k];
             result->massCorr = r.massCorr;
         }
+    });
+}
+
+} // extern "C"
+
+namespace {
+
+mfv::Geometry geometryOf(const mfv_geometry* geometry)
+{
+    mfv::Geometry g;
+    g.Sf = geometry->dSf;
+    g.Cf = geometry->dCf;
+    g.magSf = geometry->dMagSf;
+    g.C = geometry->dC;
+    g.V = geometry->dV;
+    g.w = geometry->dWeights;
+    g.dc = geometry->dDeltaCoeffs;
+    g.ndc = geometry->dNonOrthDeltaCoeffs;
+    g.corrVec = geometry->dCorrVec;
+    g.bDelta = geometry->dBDeltaCoeffs;
+    return g;
+}
+
+void putSimpleResult(const mfv::SimpleResult& r, mfv_simple_result* result)
+{
+    if (!result) return;
+    for (int k = 0; k < 3; ++k)
+    {
+        result->UInitial[k] = r.UInitial[k];
+        result->UFinal[k] = r.UFinal[k];
+        result->UIters[k] = r.UIters[k];
+    }
+    result->pInitial = r.pInitial;
+    result->pFinal = r.pFinal;
+    for (int k = 0; k < 8; ++k) result->pIters[k] = r.pIters[k];
+    result->massCorr = r.massCorr;
+}
+
+void putKeResult(const mfv::KEpsilonResult& r, mfv_kepsilon_result* out)
+{
+    if (!out) return;
+    out->epsInitial = r.epsInitial;
+    out->epsFinal = r.epsFinal;
+    out->kInitial = r.kInitial;
+    out->kFinal = r.kFinal;
+    out->epsIters = r.epsIters;
+    out->kIters = r.kIters;
+    out->nWallCells = r.nWallCells;
+}
+
+void needKeFields(const mfv_kepsilon_fields* f, int nB)
+{
+    mfv::need(f != nullptr, "kepsilon: null fields");
+    mfv::need(f->dK && f->dEps && f->dNut, "kepsilon: k, epsilon or nut missing");
+    mfv::need(nB == 0 || (f->dKb && f->dEpsb && f->dNutb), "kepsilon: boundary values missing");
+}
+
+} // namespace
+
+extern "C" {
+
+int mfv_kepsilon_default_controls(mfv_kepsilon_controls* c)
+{
+    if (!c) return MFV_ERR_INVALID;
+    std::memset(c, 0, sizeof(*c));
+    c->nu = 1e-5;
+    c->relaxEps = 1.0;
+    c->relaxK = 1.0;
+    mfv_bicg_default_controls(&c->solver);
+    const mfv::KEpsilonCoeffs d;
+    c->coeffs = mfv_kepsilon_coeffs{d.Cmu, d.C1, d.C2, d.C3, d.sigmaK, d.sigmaEps, d.kappa, d.E};
+    return MFV_OK;
+}
+
+int mfv_near_wall_dist(int nInternalFaces, int nBFaces, const double* hPoints, const int* hFaceOffsets,
+                       const int* hFaceVerts, const int* hBFaceCell, const double* hC, const int* hIsWall, double* hY)
+{
+    return mfv::guarded(nullptr, [&] {
+        mfv::need(nInternalFaces >= 0 && nBFaces >= 0, "near wall dist: negative size");
+        mfv::need(nBFaces == 0 || (hPoints && hFaceOffsets && hFaceVerts && hBFaceCell && hC && hIsWall && hY),
+                  "near wall dist: null argument");
+        mfv::nearWallDistHost(nInternalFaces, nBFaces, hPoints, hFaceOffsets, hFaceVerts, hBFaceCell, hC, hIsWall, hY);
+    });
+}
+
+int mfv_kepsilon_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry, const int* hIsWall,
+                        const double* hYWall, const int* hKType, const double* hKValue, const int* hEpsType,
+                        const double* hEpsValue, const mfv_kepsilon_controls* controls, mfv_kepsilon* out)
+{
+    if (out) *out = nullptr;
+    return mfv::guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        mfv::need(out && geometry && controls, "kepsilon create: null argument");
+        const int nB = mesh->m.nBFaces;
+        mfv::need(nB == 0 || (hIsWall && hYWall && hKType && hKValue && hEpsType && hEpsValue),
+                  "kepsilon create: boundary arrays missing");
+        mfv::KEpsilonControls kc;
+        kc.nu = controls->nu;
+        kc.relaxEps = controls->relaxEps;
+        kc.relaxK = controls->relaxK;
+        kc.solver = bicgControls(controls->solver);
+        const mfv_kepsilon_coeffs& co = controls->coeffs;
+        kc.co = mfv::KEpsilonCoeffs{co.Cmu, co.C1, co.C2, co.C3, co.sigmaK, co.sigmaEps, co.kappa, co.E};
+        std::vector<int> w(hIsWall, hIsWall + nB), kt(hKType, hKType + nB), et(hEpsType, hEpsType + nB);
+        std::vector<double> y(hYWall, hYWall + nB), kv(hKValue, hKValue + nB), ev(hEpsValue, hEpsValue + nB);
+        std::unique_ptr<mfv_kepsilon_s> h(new mfv_kepsilon_s{ctx, mesh, nullptr, {}, {}});
+        h->k.reset(new mfv::KEpsilon(ctx->c, mesh->m, geometryOf(geometry), w, y, kt, kv, et, ev, kc));
+        *out = h.release();
+    });
+}
+
+int mfv_kepsilon_destroy(mfv_kepsilon ke)
+{
+    if (!ke) return MFV_OK;
+    (void)hipStreamSynchronize(ke->ctx->c.stream);
+    delete ke;
+    return MFV_OK;
+}
+
+int mfv_kepsilon_nu_eff(mfv_kepsilon ke, const double* dNut, const double* dNutB, double* dNuEff, double* dNuEffB)
+{
+    if (!ke) return MFV_ERR_INVALID;
+    return mfv::guarded(ke->ctx, [&] {
+        const int nB = ke->mesh->m.nBFaces;
+        mfv::need(dNut && dNuEff && (nB == 0 || (dNutB && dNuEffB)), "kepsilon nuEff: null argument");
+        ke->k->nuEff(dNut, dNutB, dNuEff, dNuEffB);
+    });
+}
+
+int mfv_kepsilon_correct(mfv_kepsilon ke, const mfv_kepsilon_fields* f, mfv_kepsilon_result* result)
+{
+    if (!ke) return MFV_ERR_INVALID;
+    return mfv::guarded(ke->ctx, [&] {
+        const int nB = ke->mesh->m.nBFaces;
+        needKeFields(f, nB);
+        mfv::need(f->dUx && f->dUy && f->dUz && f->dPhi, "kepsilon correct: U or phi missing");
+        mfv::need(nB == 0 || (f->dUbx && f->dUby && f->dUbz && f->dPhiB), "kepsilon correct: boundary U or phi missing");
+        const double* U[3] = {f->dUx, f->dUy, f->dUz};
+        const double* Ub[3] = {f->dUbx, f->dUby, f->dUbz};
+        putKeResult(ke->k->correct(U, Ub, f->dPhi, f->dPhiB, f->dK, f->dKb, f->dEps, f->dEpsb, f->dNut, f->dNutb),
+                    result);
+    });
+}
+
+int mfv_simple_step_kepsilon(mfv_simple simple, const mfv_simple_fields* f, mfv_kepsilon ke,
+                             const mfv_kepsilon_fields* kf, mfv_simple_result* result, mfv_kepsilon_result* keResult)
+{
+    if (!simple || !ke) return MFV_ERR_INVALID;
+    return mfv::guarded(simple->ctx, [&] {
+        mfv::need(ke->ctx == simple->ctx && ke->mesh == simple->mesh,
+                  "simple step kepsilon: the model and the step must share a context and a mesh");
+        mfv::need(f != nullptr, "simple step: null fields");
+        const int n = simple->mesh->m.nCells, nB = simple->mesh->m.nBFaces;
+        needKeFields(kf, nB);
+        ke->nuEffC.resize(std::max(n, 1));
+        ke->nuEffB.resize(std::max(nB, 1));
+        mfv::Ctx& c = simple->ctx->c;
+        mfv::SimpleFields sf;
+        sf.U[0] = f->dUx;
+        sf.U[1] = f->dUy;
+        sf.U[2] = f->dUz;
+        sf.Ub[0] = f->dUbx;
+        sf.Ub[1] = f->dUby;
+        sf.Ub[2] = f->dUbz;
+        sf.p = f->dP;
+        sf.pb = f->dPb;
+        sf.phi = f->dPhi;
+        sf.phiB = f->dPhiB;
+        {
+            mfv::ProfScope ps(c, mfv::kProfSimpleTurbulence);
+            ke->k->nuEff(kf->dNut, kf->dNutb, ke->nuEffC.p, ke->nuEffB.p);
+        }
+        putSimpleResult(simple->s->step(sf, ke->nuEffC.p, ke->nuEffB.p), result);
+        const double* U[3] = {f->dUx, f->dUy, f->dUz};
+        const double* Ub[3] = {f->dUbx, f->dUby, f->dUbz};
+        mfv::KEpsilonResult kr;
+        {
+            mfv::ProfScope ps(c, mfv::kProfSimpleTurbulence);
+            kr = ke->k->correct(U, Ub, f->dPhi, f->dPhiB, kf->dK, kf->dKb, kf->dEps, kf->dEpsb, kf->dNut, kf->dNutb);
+        }
+        putKeResult(kr, keResult);
     });
 }
 

@@ -135,6 +135,16 @@ enum ProfTag : int
     kProfAmgCorrSmooth0,  // grid-0 correction update + first post-sweep (corrSmoothK or prolongSmoothK)
     kProfAmgRestrict0,    // grid 0 -> 1 restriction
     kProfAmgCorrAmul0,    // grid-0 scaled correction's A e and its two sums (corrAmulK)
+    // the phases of one SIMPLE iteration (simple.cpp)
+    kProfSimpleUEqn = 8,  // momentum matrix: convection, diffusion, linearUpwind and non-orthogonal
+                          // corrections, the explicit stress, relax
+    kProfSimpleUSolve,    // momentum predictor: -grad(p) source, the three component solves, U's patches
+    kProfSimplePPredictor,// rAU, HbyA, constrainHbyA, phiHbyA, adjustPhi, interpolate(rAU)
+    kProfSimplePAssembly, // per pressure pass: laplacian, boundary coefficients, correction, div, fold
+    kProfSimpleGalerkin,  // the pressure hierarchy's coarse values
+    kProfSimplePSolve,    // the pressure solves (AMG-PCG)
+    kProfSimpleCorrect,   // phi, p relax, U = HbyA - rAU grad(p), patches
+    kProfSimpleTurbulence,// the turbulence model's correct()
     kProfGridDown = 16,   // + grid: that grid's way down (sweeps, residual, restriction; the coarsest solve)
     kProfGridUp = 80,     // + grid: that grid's way up (prolongation, scaling, sweeps)
     kProfCount = 144

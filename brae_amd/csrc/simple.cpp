@@ -133,6 +133,7 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
     }
 
     // ---- UEqn ----
+    ProfScope psU(c, kProfSimpleUEqn);
     double* ic[3] = {iC[0].p, iC[1].p, iC[2].p};
     double* bcf[3] = {bC[0].p, bC[1].p, bC[2].p};
     double* s[3] = {src[0].p, src[1].p, src[2].p};
@@ -186,9 +187,12 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
         Dmom = Dm.p;
     }
 
+    psU.close();
+
     // ---- momentum predictor: solve(UEqn == -grad(p)) on a copy of the source ----
     if (ctl.momentumPredictor)
     {
+        ProfScope ps(c, kProfSimpleUSolve);
         launchGradGauss(c, m, pgType.p, f.pb, g.Sf, g.w, g.V, f.p, gradP.p);
         double* sp[3] = {srcP[0].p, srcP[1].p, srcP[2].p};
         const double* sc[3] = {s[0], s[1], s[2]};
@@ -206,6 +210,7 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
     }
 
     // ---- pressure predictor ----
+    ProfScope psPred(c, kProfSimplePPredictor);
     launchFoldBoundaryDiag(c, m, ic[0], ic[1], ic[2], Dmom, pD.p);
     launchReciprocalV(c, m, pD.p, g.V, rAU.p);
     double* hb[3] = {HbyA[0].p, HbyA[1].p, HbyA[2].p};
@@ -233,8 +238,10 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
 
     // ---- the pressure equation, nNonOrthogonalCorrectors + 1 times ----
     launchInterpolate(c, m, g.w, rAU.p, rAUf.p, rAUb.p);
+    psPred.close();
     for (int corrPass = 0; corrPass <= ctl.nNonOrthCorr; ++corrPass)
     {
+        ProfScope psAsm(c, kProfSimplePAssembly);
         if (nF) launchLaplacian(c, m, dcU, rAUf.p, g.magSf, pD.p, pU.p, pU.p);
         else launchZero(c, n, pD.p);
         launchLaplacianBCoeffs(c, nB, pType.p, pValue.p, rAUb.p, bMagSf, g.bDelta, piC.p, pbC.p);
@@ -247,8 +254,15 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
         launchDivSource(c, m, phiHbyA.p, phiHbyAB.p, g.V, 1.0, psrc.p);
         if (ctl.pRefCell >= 0) launchSetReference(c, pD.p, psrc.p, ctl.pRefCell, ctl.pRefValue);
         launchFold(c, m, pD.p, psrc.p, piC.p, pbC.p, diagC.p, b.p);
-        if (corrPass == 0) amgGalerkin(c, amg, diagC.p, pU.p, ctl.P.prec);
+        psAsm.close();
+        if (corrPass == 0)
+        {
+            ProfScope ps(c, kProfSimpleGalerkin);
+            amgGalerkin(c, amg, diagC.p, pU.p, ctl.P.prec);
+        }
+        ProfScope psSolve(c, kProfSimplePSolve);
         const PcgPerf pf = amgPcgSolve(c, m, amg, diagC.p, pU.p, b.p, f.p, nullptr, ctl.P);
+        psSolve.close();
         if (corrPass == 0) res.pInitial = pf.initialResidual;
         res.pFinal = pf.finalResidual;
         res.pIters[corrPass] = pf.nIterations;
@@ -259,6 +273,7 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
         }
         if (corrPass == ctl.nNonOrthCorr)
         {
+            ProfScope ps(c, kProfSimpleCorrect);
             if (nF)
                 launchFluxCorrect(c, m, LduView{n, nF, diagC.p, pU.p, pU.p}, f.p,
                                   ctl.correctedLaplacian ? ffc.p : nullptr, phiHbyA.p, 1.0, f.phi);
@@ -267,6 +282,7 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
     }
 
     // ---- p.relax(), then U = HbyA - rAU grad(p) ----
+    ProfScope psCorr(c, kProfSimpleCorrect);
     if (ctl.relaxP > 0.0 && ctl.relaxP < 1.0) launchRelaxField(c, n, f.p, pPrev.p, ctl.relaxP);
     {
         const double* cells[1] = {f.p};
@@ -279,6 +295,7 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
     launchCorrectU(c, n, rAU.p, gradP.p, hbc, Uo);
     double* ubo[3] = {f.Ub[0], f.Ub[1], f.Ub[2]};
     launchEvaluate(c, m, 3, 3, uType.p, uValue.p, U, ubo);
+    psCorr.close();
     hipCheck(hipStreamSynchronize(c.stream), "simple step");
     return res;
 }

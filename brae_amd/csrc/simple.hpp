@@ -89,4 +89,52 @@ struct SimpleSolver
     SimpleResult step(const SimpleFields& f, const double* nuEff, const double* nuEffB);
 };
 
+// ---- the standard k-epsilon model (kepsilon.hip; k_epsilon.cuh:221-337) ----
+struct KEpsilonCoeffs
+{
+    double Cmu = 0.09, C1 = 1.44, C2 = 1.92, C3 = 0.0, sigmaK = 1.0, sigmaEps = 1.3, kappa = 0.41, E = 9.8;
+};
+
+struct KEpsilonControls
+{
+    double nu = 1e-5;
+    double relaxEps = 1.0, relaxK = 1.0;
+    BicgControls solver;   // the k and epsilon solves' PBiCGStab
+    KEpsilonCoeffs co;
+};
+
+struct KEpsilonResult
+{
+    double epsInitial = 0, epsFinal = 0, kInitial = 0, kFinal = 0;
+    int epsIters = 0, kIters = 0, nWallCells = 0;
+};
+
+struct KEpsilon
+{
+    Ctx& c;
+    Mesh& m;
+    Geometry g;
+    KEpsilonControls ctl;
+    int nWallCells = 0;
+    DBuf<int> isWall, kType, epsType, isEmpty, gType, wallCell;
+    DBuf<double> yWall, kValue, epsValue;
+    DBuf<double> gradU, g3[3], gbn, G, divU, D, Dr, Up, Lo, S, iC, bC, Df, lD, lU, nutB0;
+
+    KEpsilon(Ctx& ctx, Mesh& mesh, const Geometry& geom, const std::vector<int>& hIsWall, const std::vector<double>& hY,
+             const std::vector<int>& hKType, const std::vector<double>& hKValue, const std::vector<int>& hEpsType,
+             const std::vector<double>& hEpsValue, const KEpsilonControls& controls);
+    // nuEff = nu + nut in the cells and on the boundary faces
+    void nuEff(const double* nut, const double* nutb, double* nuEffC, double* nuEffB);
+    KEpsilonResult correct(const double* const* U, const double* const* Ub, const double* phi, const double* phiB,
+                           double* k, double* kb, double* eps, double* epsb, double* nut, double* nutb);
+
+private:
+    PcgPerf solveScalar(bool isEps, const double* phi, const double* phiB, const double* nut, double* psi,
+                        double* psib, const double* prod, const double* eps, const double* k);
+};
+
+// nearWallDist (near_wall_dist.cuh:120-177) on the host: y per boundary face of the wall faces
+void nearWallDistHost(int nFaces, int nBFaces, const double* points, const int* faceOffsets, const int* faceVerts,
+                      const int* bFaceCell, const double* C, const int* isWall, double* y);
+
 } // namespace mfv

@@ -47,6 +47,10 @@ LAYOUT_AUTO, LAYOUT_ROWS = 0, 1
 CACHE_LOADED, CACHE_ABSENT, CACHE_OTHER_MESH_OR_BUILD, CACHE_UNREADABLE = range(4)
 PROF_PCG_SPMV64, PROF_PCG_UPDATE, PROF_PCG_ZR, PROF_VCYCLE, PROF_AMG_RESIDUAL0, PROF_AMG_CORR_SMOOTH0, \
     PROF_AMG_RESTRICT0, PROF_AMG_CORR_AMUL0 = range(8)
+# the phases of one SIMPLE iteration (mfv_simple_step)
+PROF_SIMPLE = {8: "UEqn assembly", 9: "U solve (momentum predictor)", 10: "pressure predictor",
+               11: "pressure assembly", 12: "Galerkin", 13: "pressure solve", 14: "flux and U correction",
+               15: "turbulence"}
 PREC_FP64, PREC_FP32, PREC_FP16 = 64, 32, 16
 
 
@@ -106,6 +110,24 @@ class SimpleControls(ctypes.Structure):
     _fields_ = [("nu", D), ("relaxU", D), ("relaxP", D), ("momentumPredictor", I), ("linearUpwind", I),
                 ("correctedLaplacian", I), ("nNonOrthCorr", I), ("pRefCell", I), ("pRefValue", D),
                 ("U", BicgControls), ("p", PcgControls), ("mergeLevels", I)]
+
+
+class KEpsilonCoeffs(ctypes.Structure):
+    _fields_ = [(n, D) for n in ("Cmu", "C1", "C2", "C3", "sigmaK", "sigmaEps", "kappa", "E")]
+
+
+class KEpsilonControls(ctypes.Structure):
+    _fields_ = [("nu", D), ("relaxEps", D), ("relaxK", D), ("solver", BicgControls), ("coeffs", KEpsilonCoeffs)]
+
+
+class KEpsilonFields(ctypes.Structure):
+    _fields_ = [(n, P) for n in ("dUx", "dUy", "dUz", "dUbx", "dUby", "dUbz", "dPhi", "dPhiB", "dK", "dKb", "dEps",
+                                 "dEpsb", "dNut", "dNutb")]
+
+
+class KEpsilonResult(ctypes.Structure):
+    _fields_ = [("epsInitial", D), ("epsFinal", D), ("kInitial", D), ("kFinal", D), ("epsIters", I),
+                ("kIters", I), ("nWallCells", I)]
 
 
 class SimpleFields(ctypes.Structure):
@@ -191,6 +213,13 @@ _SIGS = {
     "mfv_pcg_gamg_solve": [P, P, P, P, P, P, D, D, I, I, P, P],
     "mfv_bicg_default_controls": [P],
     "mfv_pbicgstab_solve": [P, P, P, P, P, P, P, P, P],
+    "mfv_kepsilon_default_controls": [P],
+    "mfv_near_wall_dist": [I, I, P, P, P, P, P, P, P],
+    "mfv_kepsilon_create": [P, P, P, P, P, P, P, P, P, P, P],
+    "mfv_kepsilon_destroy": [P],
+    "mfv_kepsilon_nu_eff": [P, P, P, P, P],
+    "mfv_kepsilon_correct": [P, P, P],
+    "mfv_simple_step_kepsilon": [P, P, P, P, P, P],
     "mfv_simple_default_controls": [P],
     "mfv_simple_create": [P, P, P, P, P, P, P, P, P],
     "mfv_simple_destroy": [P],
@@ -567,3 +596,86 @@ class Simple:
             self.close()
         except Exception:
             pass
+
+
+def kepsilon_controls(solver=None, coeffs=None, **kw):
+    """k-epsilon controls with the library's defaults, then solver/coeffs fields (dicts) and the rest"""
+    c = KEpsilonControls()
+    lib().mfv_kepsilon_default_controls(ctypes.byref(c))
+    for k, v in (solver or {}).items():
+        setattr(c.solver, k, v)
+    for k, v in (coeffs or {}).items():
+        setattr(c.coeffs, k, v)
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError("kepsilon controls have no field " + k)
+        setattr(c, k, v)
+    return c
+
+
+def near_wall_dist(nInternal, points, faceOffsets, faceVerts, bFaceCell, C, isWall):
+    """the library's host nearWallDist: y per boundary face (numpy in, numpy out)"""
+    import numpy as np
+    pts = np.ascontiguousarray(points, np.float64)
+    fo = np.ascontiguousarray(faceOffsets, np.int32)
+    fv = np.ascontiguousarray(faceVerts, np.int32)
+    bfc = np.ascontiguousarray(bFaceCell, np.int32)
+    cc = np.ascontiguousarray(C, np.float64)
+    w = np.ascontiguousarray(isWall, np.int32)
+    y = np.zeros(bfc.size)
+    rc = lib().mfv_near_wall_dist(nInternal, bfc.size, pts.ctypes.data, fo.ctypes.data, fv.ctypes.data,
+                                  bfc.ctypes.data, cc.ctypes.data, w.ctypes.data, y.ctypes.data)
+    if rc != MFV_OK:
+        raise MfvError("mfv_near_wall_dist failed (%d)" % rc)
+    return y
+
+
+class KEpsilon:
+    """mfv_kepsilon: the standard k-epsilon model's workspace on a mesh"""
+
+    def __init__(self, mesh, geometry, isWall, yWall, kType, kValue, epsType, epsValue, controls):
+        import numpy as np
+        self.mesh = mesh
+        self._keep = [np.ascontiguousarray(isWall, np.int32), np.ascontiguousarray(yWall, np.float64),
+                      np.ascontiguousarray(kType, np.int32), np.ascontiguousarray(kValue, np.float64),
+                      np.ascontiguousarray(epsType, np.int32), np.ascontiguousarray(epsValue, np.float64)]
+        self.geometry = geometry
+        self.controls = controls
+        h = P()
+        mesh.ctx.call("mfv_kepsilon_create", mesh.h, ctypes.byref(geometry), *[k.ctypes.data for k in self._keep],
+                      ctypes.byref(controls), ctypes.byref(h))
+        self.h = h.value
+        _LIVE.add(self)
+
+    @staticmethod
+    def fields(U, Ub, phi, phiB, k, kb, eps, epsb, nut, nutb):
+        return KEpsilonFields(*[ptr(t) for t in list(U) + list(Ub) + [phi, phiB, k, kb, eps, epsb, nut, nutb]])
+
+    def correct(self, U, Ub, phi, phiB, k, kb, eps, epsb, nut, nutb):
+        f = self.fields(U, Ub, phi, phiB, k, kb, eps, epsb, nut, nutb)
+        r = KEpsilonResult()
+        self.mesh.ctx._check(lib().mfv_kepsilon_correct(self.h, ctypes.byref(f), ctypes.byref(r)),
+                             "mfv_kepsilon_correct")
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mfv_kepsilon_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def simple_step_kepsilon(sim, ke, U, Ub, p, pb, phi, phiB, k, kb, eps, epsb, nut, nutb):
+    """one turbulent SIMPLE iteration (nuEff from nut, the step, the model's correct())"""
+    f = SimpleFields(*[ptr(t) for t in list(U) + list(Ub) + [p, pb, phi, phiB]])
+    kf = KEpsilon.fields(U, Ub, phi, phiB, k, kb, eps, epsb, nut, nutb)
+    r, kr = SimpleResult(), KEpsilonResult()
+    rc = lib().mfv_simple_step_kepsilon(sim.h, ctypes.byref(f), ke.h, ctypes.byref(kf), ctypes.byref(r),
+                                        ctypes.byref(kr))
+    sim.mesh.ctx._check(rc, "mfv_simple_step_kepsilon")
+    return r, kr

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -665,7 +666,7 @@ inline void deviceNormFactorInto(const DeviceLduView& A, const DeviceBuffer<scal
 struct AMGData
 {
     mfv_amg amg = nullptr;
-    mfv_mesh addressing = nullptr;   // the fine addressing the hierarchy was built on (owned)
+    mfv_mesh addressing = nullptr;   // the fine addressing the hierarchy was built on
     int nFine = 0;
     int precision = MFV_PREC_FP32;   // the reference's FP32 V-cycle
     int nPreSweeps = 1, nPostSweeps = 1;
@@ -681,6 +682,7 @@ struct AMGData
             reset();
             amg = o.amg;
             addressing = o.addressing;
+            owner_ = std::move(o.owner_);
             nFine = o.nFine;
             precision = o.precision;
             nPreSweeps = o.nPreSweeps;
@@ -697,13 +699,26 @@ struct AMGData
         if (amg) mfv_amg_num_grids(amg, &n);
         return n > 0 ? n - 1 : 0;
     }
+    // takes ownership of a new addressing handle (destroyed with the last hierarchy that uses it)
+    void adoptAddressing(mfv_mesh m)
+    {
+        addressing = m;
+        owner_ = std::shared_ptr<mfv_mesh_s>(m, [](mfv_mesh_s* p) { mfv_mesh_destroy(p); });
+    }
+    // shares another hierarchy's addressing (a clone lives on its source's mesh)
+    void shareAddressing(const AMGData& o)
+    {
+        addressing = o.addressing;
+        owner_ = o.owner_;
+    }
 
 private:
+    std::shared_ptr<mfv_mesh_s> owner_;
     void reset()
     {
         if (amg) mfv_amg_destroy(amg);
-        if (addressing) mfv_mesh_destroy(addressing);
         amg = nullptr;
+        owner_.reset();
         addressing = nullptr;
     }
 };
@@ -718,11 +733,70 @@ inline AMGData buildAMG(const std::vector<label>& fineOwner, const std::vector<l
     mfv_ctx ctx = threadContext();
     AMGData a;
     a.nFine = nFine;
+    mfv_mesh m = nullptr;
     mfvCheck(mfv_mesh_create(ctx, nFine, static_cast<int>(fineOwner.size()), fineOwner.data(), fineNei.data(), 0,
-                             nullptr, &a.addressing),
+                             nullptr, &m),
              ctx, "buildAMG addressing");
+    a.adoptAddressing(m);
     mfv_amg_controls c{MFV_AGG_BRAE, 64, 1};
     mfvCheck(mfv_amg_build(ctx, a.addressing, faceWeights.data(), &c, &a.amg), ctx, "buildAMG");
+    return a;
+}
+
+// AMGCacheRead (device_amg.cuh:410-416): what the cache directory held
+enum class AMGCacheRead
+{
+    loaded,
+    absent,
+    otherMeshOrBuild,
+    unreadable
+};
+
+// buildOrLoadAMG (device_amg.cuh:448-451): the hierarchy from <cacheDir>'s file when it holds this mesh's
+// and this build's signature, else built (and written there with writeCache); `how` says what the file held
+inline AMGData buildOrLoadAMG(const std::vector<label>& fineOwner, const std::vector<label>& fineNei,
+                              const std::vector<scalar>& faceWeights, int nFine, const std::string& cacheDir,
+                              bool writeCache, const bool* smoothedAggregation = nullptr,
+                              AMGCacheRead* how = nullptr)
+{
+    if (smoothedAggregation && *smoothedAggregation)
+        throw std::runtime_error("brae hip: buildOrLoadAMG: smoothed aggregation is not carried");
+    if (fineOwner.size() != fineNei.size() || faceWeights.size() != fineOwner.size())
+        throw std::runtime_error("brae hip: buildOrLoadAMG: owner, neighbour and weights differ in length");
+    mfv_ctx ctx = threadContext();
+    AMGData a;
+    a.nFine = nFine;
+    mfv_mesh m = nullptr;
+    mfvCheck(mfv_mesh_create(ctx, nFine, static_cast<int>(fineOwner.size()), fineOwner.data(), fineNei.data(), 0,
+                             nullptr, &m),
+             ctx, "buildOrLoadAMG addressing");
+    a.adoptAddressing(m);
+    mfv_amg_controls c{MFV_AGG_BRAE, 64, 1};
+    int h = MFV_CACHE_ABSENT;
+    mfvCheck(mfv_amg_build_or_load(ctx, a.addressing, faceWeights.data(), &c, cacheDir.c_str(), writeCache ? 1 : 0,
+                                   &a.amg, &h),
+             ctx, "buildOrLoadAMG");
+    if (how)
+        *how = h == MFV_CACHE_LOADED             ? AMGCacheRead::loaded
+               : h == MFV_CACHE_OTHER_MESH_OR_BUILD ? AMGCacheRead::otherMeshOrBuild
+               : h == MFV_CACHE_UNREADABLE          ? AMGCacheRead::unreadable
+                                                    : AMGCacheRead::absent;
+    return a;
+}
+
+// cloneAMG (device_amg_cache.cu:70): a second hierarchy of the same structure on the same fine addressing;
+// its coarse values start unset (amgGalerkin), its precision and sweeps are the source's
+inline AMGData cloneAMG(const AMGData& S)
+{
+    if (!S.amg || !S.addressing) throw std::runtime_error("brae hip: cloneAMG: empty hierarchy");
+    mfv_ctx ctx = threadContext();
+    AMGData a;
+    a.nFine = S.nFine;
+    a.precision = S.precision;
+    a.nPreSweeps = S.nPreSweeps;
+    a.nPostSweeps = S.nPostSweeps;
+    a.shareAddressing(S);
+    mfvCheck(mfv_amg_clone(ctx, S.amg, &a.amg), ctx, "cloneAMG");
     return a;
 }
 
@@ -864,8 +938,10 @@ inline AMGData faceAreaPairGamgAgglomeration(const std::vector<label>& fineOwner
     AMGData a;
     a.nFine = nFine;
     a.precision = MFV_PREC_FP64;
-    mfvCheck(mfv_mesh_create(ctx, nFine, nF, fineOwner.data(), fineNei.data(), 0, nullptr, &a.addressing), ctx,
+    mfv_mesh m = nullptr;
+    mfvCheck(mfv_mesh_create(ctx, nFine, nF, fineOwner.data(), fineNei.data(), 0, nullptr, &m), ctx,
              "faceAreaPairGamgAgglomeration addressing");
+    a.adoptAddressing(m);
     mfv_amg_controls c{MFV_AGG_FACEAREAPAIR, nCellsInCoarsestLevel, 1};
     mfvCheck(mfv_amg_build(ctx, a.addressing, w.data(), &c, &a.amg), ctx, "faceAreaPairGamgAgglomeration");
     return a;

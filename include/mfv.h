@@ -538,6 +538,64 @@ int mfv_simple_step(mfv_simple simple, const mfv_simple_fields* fields, const do
                     const double* dNuEffB, mfv_simple_result* result);
 
 /* ---------------------------------------------------------------------------------------------
+ * The standard k-epsilon model as the reference's host SIMPLE driver corrects it after each iteration:
+ * kepsilon::correct (src/TurbulenceModels/turbulenceModels/RAS/kEpsilon/k_epsilon.cuh:221-337; the
+ * device twin deviceKEpsilonCorrect, src/cuda/device_kepsilon.cu:1139-1378). Production G = nut*GbyNu,
+ * the epsilonWallFunction's near-wall epsilon and production (corner weights 1/nWallFaces) and the
+ * nutkWallFunction on the faces of wall patches, the epsilon equation (upwind convection, laplacian of
+ * nut/sigmaEps + nu, C2 eps/k destruction, C1 Cmu k GbyNu production, the (2/3 C1 - C3) divU SuSp term),
+ * relaxed and constrained at the near-wall cells (fvMatrix::setValues), solved by PBiCGStab and bounded;
+ * then k with the new epsilon; then nut = Cmu k^2/epsilon. Incompressible, upwind convection, uncoupled
+ * patches; k and epsilon carry zeroGradient values on the wall-function faces.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct mfv_kepsilon_s* mfv_kepsilon;
+typedef struct
+{
+    double Cmu, C1, C2, C3, sigmaK, sigmaEps, kappa, E;   /* kepsilon_coeffs.cuh:11-22 */
+} mfv_kepsilon_coeffs;
+typedef struct
+{
+    double nu;                          /* laminar viscosity */
+    double relaxEps, relaxK;            /* relaxationFactors: equations epsilon, k */
+    mfv_bicg_controls solver;           /* the two solves' PBiCGStab */
+    mfv_kepsilon_coeffs coeffs;
+} mfv_kepsilon_controls;
+typedef struct
+{
+    const double *dUx, *dUy, *dUz, *dUbx, *dUby, *dUbz;  /* U in cells and on the boundary faces */
+    const double *dPhi, *dPhiB;                          /* face fluxes: internal, boundary */
+    double *dK, *dKb, *dEps, *dEpsb, *dNut, *dNutb;      /* cells, boundary faces (in place) */
+} mfv_kepsilon_fields;
+typedef struct
+{
+    double epsInitial, epsFinal, kInitial, kFinal;
+    int epsIters, kIters, nWallCells;
+} mfv_kepsilon_result;
+/* OpenFOAM v2412's coefficients, nu 1e-5, no relaxation (factor 1), PBiCGStab-DILU to 1e-10 */
+int mfv_kepsilon_default_controls(mfv_kepsilon_controls* controls);
+/* nearWallDist (src/finiteVolume/fvMesh/near_wall_dist.cuh:120-177) on the host: y per boundary face of the
+ * faces flagged in hIsWall; hFaceOffsets/hFaceVerts over every face (internal then boundary), hC the cell
+ * centres (xyz). Needs no context. */
+int mfv_near_wall_dist(int nInternalFaces, int nBFaces, const double* hPoints, const int* hFaceOffsets,
+                       const int* hFaceVerts, const int* hBFaceCell, const double* hC, const int* hIsWall,
+                       double* hY);
+/* hIsWall, hYWall (mfv_near_wall_dist), hKType/hEpsType (MFV_BC_*) and values per boundary face */
+int mfv_kepsilon_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry, const int* hIsWall,
+                        const double* hYWall, const int* hKType, const double* hKValue, const int* hEpsType,
+                        const double* hEpsValue, const mfv_kepsilon_controls* controls, mfv_kepsilon* out);
+int mfv_kepsilon_destroy(mfv_kepsilon ke);
+/* nuEff = nu + nut per cell and boundary face (the momentum equation's viscosity) */
+int mfv_kepsilon_nu_eff(mfv_kepsilon ke, const double* dNut, const double* dNutB, double* dNuEff, double* dNuEffB);
+/* one correct() in place on k, epsilon and nut. Synchronises. */
+int mfv_kepsilon_correct(mfv_kepsilon ke, const mfv_kepsilon_fields* fields, mfv_kepsilon_result* result);
+/* one turbulent SIMPLE iteration, as cpu::simpleStep orders it (simpleFoam_cpp.cu:27-46, 208-233): nuEff
+ * from nut, the laminar step's UEqn, pressure and corrections with it, then the model's correct() from the
+ * corrected U and phi. The U/phi members of keFields are ignored (the step's fields are used). */
+int mfv_simple_step_kepsilon(mfv_simple simple, const mfv_simple_fields* fields, mfv_kepsilon ke,
+                             const mfv_kepsilon_fields* keFields, mfv_simple_result* result,
+                             mfv_kepsilon_result* keResult);
+
+/* ---------------------------------------------------------------------------------------------
  * Serving: a stream of pressure-Poisson requests (no counterpart in the reference, which runs one
  * case per process). Each request is a box of nx*ny*nz hex cells (lengths lx, ly, lz; interior points
  * jittered by +-jitter of the spacing), solved nSteps times for right-hand sides -div(U) of seeded

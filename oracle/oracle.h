@@ -292,6 +292,47 @@ typedef struct {
 /* one iteration in place on the fields; returns 0, or 1 when adjustPhi refuses */
 int orc_simple_step(const orc_simple_case* cs, orc_simple_fields* fields, orc_simple_result* res);
 
+/* ---- the standard k-epsilon model as the host SIMPLE driver calls it (kepsilon.c) ---- */
+typedef struct {
+    double Cmu, C1, C2, C3, sigmaK, sigmaEps, kappa, E;
+} orc_kepsilon_coeffs;
+void orc_kepsilon_default_coeffs(orc_kepsilon_coeffs* co);
+typedef struct {
+    int nCells, nFaces, nBFaces;
+    const int *owner, *neighbour, *bFaceCell;
+    /* geometry as orc_geometry writes it (Sf, magSf over internal then boundary faces) */
+    const double *Sf, *magSf, *V, *weights, *deltaCoeffs, *bDeltaCoeffs;
+    const int* isWall;          /* per boundary face: the face of a `wall` patch (the wall functions) */
+    const double* yWall;        /* per boundary face: the near-wall distance (wall faces) */
+    const int *kType, *epsType; /* ORC_BC_* per boundary face (the wall functions' patches: zeroGradient) */
+    const double *kValue, *epsValue;
+    double nu, relaxEps, relaxK, tol, relTol;
+    int maxIter, minIter;
+    orc_kepsilon_coeffs co;
+} orc_kepsilon_case;
+typedef struct {
+    double epsInitial, epsFinal, kInitial, kFinal;
+    int epsIters, kIters, nWallCells;
+} orc_kepsilon_result;
+/* nearWallDist (near_wall_dist.cuh:120-177): y per boundary face of a wall patch; faceOffsets/faceVerts over
+ * every face (internal then boundary), C the cell centres [3*nCells] */
+void orc_near_wall_dist(int nFaces, int nBFaces, const double* points, const int* faceOffsets,
+                        const int* faceVerts, const int* bFaceCell, const double* C, const int* isWall, double* y);
+double orc_y_plus_lam(double kappa, double E);
+double orc_nutk_wall(double Cmu, double kappa, double E, double y, double kCell, double nu);
+void orc_gbynu(int n, const double* gradU, double* out);
+/* kEqn and epsilonEqn as assembled matrices (k_epsilon.cuh:137-186): nutb NULL takes the owner cell's
+ * diffusivity on the boundary */
+void orc_kepsilon_eqns(const orc_kepsilon_case* cs, const double* k, const double* kb, const double* eps,
+                       const double* epsb, const double* nut, const double* nutb, const double* G,
+                       const double* GbyNu, const double* phi, const double* phiB, double* kD, double* kU,
+                       double* kL, double* kS, double* kIC, double* kBC, double* eD, double* eU, double* eL,
+                       double* eS, double* eIC, double* eBC);
+/* one kepsilon::correct (k_epsilon.cuh:221-337) in place on k, epsilon and nut (cells and boundary faces) */
+int orc_kepsilon_correct(const orc_kepsilon_case* cs, const double* const* U, const double* const* Ub,
+                         const double* phi, const double* phiB, double* k, double* kb, double* eps, double* epsb,
+                         double* nut, double* nutb, orc_kepsilon_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,10 +1,15 @@
 // A C++ caller of the pressure path through brae_mfv.hpp, written the way brae's own drivers call it
 // (simpleFoam.cu:343-381): assemble -laplacian on a box, fold, reference cell, buildAMG, amgGalerkin,
-// deviceAMGPCG, residual check with deviceAmul and deviceNormFactorInto, flux; OpenFOAM's GAMG on the
-// same system (faceAreaPairGamgAgglomeration, gamgSolveFolded, pcgGamgSolveFolded); and the error
+// deviceAMGPCG, residual check with deviceAmul and deviceNormFactorInto, flux; the hierarchy persisted,
+// reloaded and cloned (buildOrLoadAMG, cloneAMG); OpenFOAM's GAMG on the same system
+// (faceAreaPairGamgAgglomeration, gamgSolveFolded, pcgGamgSolveFolded, and an fvSolution entry read by
+// readGamgControls); and the error
 // behaviour (refusals throw std::runtime_error("brae hip: ...")). Prints one JSON line; exit status 0
 // when every check holds.
+#include "brae_foam.hpp"
 #include "brae_mfv.hpp"
+
+#include <unistd.h>
 
 #include <cmath>
 #include <cstdio>
@@ -259,21 +264,72 @@ int main(int argc, char** argv)
     relaxField(pNew, pPrev, 0.5);
     for (const scalar v : pNew.download()) relaxErr = std::max(relaxErr, std::fabs(v - 2.0));
 
+    // the hierarchy written and read back (buildOrLoadAMG) and a clone of it (cloneAMG): each solves the same
+    // system to the same bits as the hierarchy built above
+    const std::vector<scalar> psiRef = psi.download();
+    char dirT[] = "/tmp/brae_amgcache_XXXXXX";
+    const std::string cacheDir = mkdtemp(dirT) ? std::string(dirT) : std::string("/tmp");
+    AMGCacheRead how1 = AMGCacheRead::loaded, how2 = AMGCacheRead::absent;
+    bool cacheOk = true;
+    {
+        AMGData built = buildOrLoadAMG(own, box.neighbour, magSf, nC, cacheDir, true, nullptr, &how1);
+        AMGData loaded = buildOrLoadAMG(own, box.neighbour, magSf, nC, cacheDir, false, nullptr, &how2);
+        AMGData cl = cloneAMG(loaded);
+        for (AMGData* h : {&built, &loaded, &cl})
+        {
+            amgGalerkin(*h, diagC, upper, upper);
+            DeviceBuffer<scalar> x(std::vector<scalar>(nC, 0.0));
+            const DeviceSolverPerf q = deviceAMGPCG(A, *h, b, x, static_cast<const scalar*>(nullptr), 1e-8, 0.0, 500,
+                                                    false, 4, true, 0);
+            cacheOk = cacheOk && q.nIterations == perf.nIterations && x.download() == psiRef;
+        }
+        cacheOk = cacheOk && how1 == AMGCacheRead::absent && how2 == AMGCacheRead::loaded && cl.nLevels() == amg.nLevels();
+    }
+    std::remove((cacheDir + "/.mfv_amgcache").c_str());
+    rmdir(cacheDir.c_str());
+
+    // an fvSolution GAMG entry read by readGamgControls drives the same solve as the controls set by hand
+    const FoamDict fvSolution = parseFoamDict(
+        "solvers { p { solver GAMG; smoother GaussSeidel; tolerance 1e-8; relTol 0; nCellsInCoarsestLevel 10; } }");
+    const FoamDict& pEntry = *fvSolution.subDict("solvers")->subDict("p");
+    const FoamGamgControls fc = readGamgControls(pEntry, pEntry.scalarOr("tolerance", 1e-6), pEntry.scalarOr("relTol", 0.0),
+                                                 pEntry.intOr("maxIter", 1000), "p: ");
+    GamgControls gcRead;
+    gcRead.smoother = fc.smoother;
+    gcRead.tolerance = fc.tolerance;
+    gcRead.relTol = fc.relTol;
+    gcRead.maxIter = fc.maxIter;
+    gcRead.minIter = fc.minIter;
+    gcRead.nPreSweeps = fc.nPreSweeps;
+    gcRead.preSweepsLevelMultiplier = fc.preSweepsLevelMultiplier;
+    gcRead.maxPreSweeps = fc.maxPreSweeps;
+    gcRead.nPostSweeps = fc.nPostSweeps;
+    gcRead.postSweepsLevelMultiplier = fc.postSweepsLevelMultiplier;
+    gcRead.maxPostSweeps = fc.maxPostSweeps;
+    gcRead.nFinestSweeps = fc.nFinestSweeps;
+    gcRead.scaleCorrection = fc.scaleCorrection;
+    gcRead.nCellsInCoarsestLevel = fc.nCellsInCoarsestLevel;
+    DeviceBuffer<scalar> psiR(std::vector<scalar>(nC, 0.0));
+    const SolverPerformance pr = gamgSolveFolded(gamg, diagC, upper, b, psiR, gcRead);
+    const bool readOk = pr.nIterations == pg.nIterations && psiR.download() == psiG.download();
+
     const bool predictorOk = hErr <= 1e-12 * hScale && hbErr == 0.0 && fluxErr <= 1e-14 &&
                              std::fabs(net) <= 1e-12 * tot && !closedVolume && relaxErr <= 1e-15;
 
     const bool ok = perf.nIterations > 0 && perf.nIterations < 500 && perf.finalResidual < 1e-8 && res < 2e-8 &&
                     predictorOk &&
                     refusesSA && refusesAsym && flux.size() == (size_t)nI && pg.converged && pg.finalResidual < 1e-8 &&
-                    pp.converged && pp.finalResidual < 1e-8 && refusesSmoother;
+                    pp.converged && pp.finalResidual < 1e-8 && refusesSmoother && cacheOk && readOk;
     std::printf("{\"cells\": %d, \"levels\": %d, \"iterations\": %d, \"initial\": %.6e, \"final\": %.6e, "
                 "\"recomputed\": %.6e, \"refuses_sa\": %s, \"refuses_asym\": %s, \"gamg_levels\": %d, "
                 "\"gamg_cycles\": %d, \"gamg_final\": %.6e, \"pcg_gamg_iterations\": %d, \"pcg_gamg_final\": %.6e, "
                 "\"refuses_smoother\": %s, \"predictor_h_err\": %.3e, \"predictor_flux_err\": %.3e, "
-                "\"predictor_net_flux\": %.3e, \"relax_err\": %.3e, \"ok\": %s}\n",
+                "\"predictor_net_flux\": %.3e, \"relax_err\": %.3e, \"cache_clone_same_bits\": %s, "
+                "\"fvsolution_gamg_same_bits\": %s, \"ok\": %s}\n",
                 nC, amg.nLevels(), perf.nIterations, perf.initialResidual, perf.finalResidual, res,
                 refusesSA ? "true" : "false", refusesAsym ? "true" : "false", gamg.nLevels(), pg.nIterations,
                 pg.finalResidual, pp.nIterations, pp.finalResidual, refusesSmoother ? "true" : "false",
-                hErr / (hScale > 0.0 ? hScale : 1.0), fluxErr, net, relaxErr, ok ? "true" : "false");
+                hErr / (hScale > 0.0 ? hScale : 1.0), fluxErr, net, relaxErr, cacheOk ? "true" : "false",
+                readOk ? "true" : "false", ok ? "true" : "false");
     return ok ? 0 : 1;
 }

@@ -14,6 +14,11 @@ the tests load them without an OpenFOAM parser:
                  validation/matrixDumpGAMG/solution.dat (OpenFOAM GAMG)
   simple_step.npz  validation/matrixDumpSimple/282/{U,p,phi}, constant/transportProperties (nu) and
                  step.dat (OpenFOAM's p, U, phi after one SIMPLE iteration from those fields)
+  kepsilon.npz   validation/kEpsCase and validation/kEpsCorrect: 282/{U,k,epsilon,nut,phi}, keqn.dat and
+                 epseqn.dat (OpenFOAM's k and epsilon equations), correct.dat (k, epsilon, nut after one
+                 kEpsilon::correct())
+  foam/          OpenFOAM ASCII files as they are: case20's polyMesh, 0/T and system/fvSolution, and
+                 pitzDaily's system/fvSolution (validation/matrixDumpSimple), for the readers' tests
   case20.npz     demo/amgpcg/case20/constant/polyMesh, trace/{matrix,rhs,of_hierarchy,
                  brae_hierarchy,brae_levels,brae_cycles,of_gamg_cycles}.csv
 """
@@ -248,6 +253,42 @@ def simple_step(ref):
     np.savez_compressed(os.path.join(HERE, "simple_step.npz"), **out)
 
 
+def kepsilon(ref):
+    """validation/kEpsCase and validation/kEpsCorrect (pitzDaily, kEpsilon, nu 1e-5): OpenFOAM's step-282 fields
+    U, k, epsilon, nut and phi with their patch types and values; keqn.dat/epseqn.dat (OpenFOAM's assembled
+    k and epsilon equations from kEpsCase's fields, compared at 1e-11 by tests/test_kepsilon_eqn.cu) and
+    correct.dat (k, epsilon, nut after one kEpsilon::correct() from kEpsCorrect's fields,
+    tests/test_kepsilon_correct.cu)"""
+    out = {}
+    for tag, sub in (("eqn", "validation/kEpsCase"), ("corr", "validation/kEpsCorrect")):
+        case = os.path.join(ref, sub)
+        mesh = read_mesh(os.path.join(case, "constant/polyMesh"))
+        names, sizes = mesh["patchNames"], mesh["patchSize"]
+        if tag == "eqn":
+            out.update({"mesh_" + k: v for k, v in mesh.items()})
+        Ui, Ut, Uv = read_vector_field(os.path.join(case, "282/U"), names, sizes)
+        out[tag + "_U0"], out[tag + "_U_types"], out[tag + "_U_values"] = Ui, np.array(Ut), np.concatenate(Uv)
+        for f in ("k", "epsilon", "nut"):
+            i0, t0, v0 = read_scalar_field(os.path.join(case, "282", f), names, sizes)
+            out[tag + "_" + f + "0"], out[tag + "_" + f + "_types"] = i0, np.array(t0)
+            out[tag + "_" + f + "_values"] = np.concatenate(v0)
+        phiI, _, phiB = read_scalar_field(os.path.join(case, "282/phi"), names, sizes)
+        out[tag + "_phi0_internal"], out[tag + "_phi0_boundary"] = phiI, np.concatenate(phiB)
+    for eq in ("k", "eps"):
+        d, u, l, src, pch = read_matrix_dat(os.path.join(ref, "validation/kEpsCase", eq + "eqn.dat"))
+        out[eq + "eqn_diag"], out[eq + "eqn_upper"], out[eq + "eqn_lower"], out[eq + "eqn_source"] = d, u, l, src
+        for name in out["mesh_patchNames"]:
+            v = pch.get(str(name), np.zeros((0, 2)))
+            out[eq + "eqn_iC_" + str(name)] = v[:, 0].copy()
+            out[eq + "eqn_bC_" + str(name)] = v[:, 1].copy()
+    tok = open(os.path.join(ref, "validation/kEpsCorrect/correct.dat")).read().split()
+    nC = int(tok[0])
+    out["correct_k"] = np.array(tok[1:1 + nC], dtype=np.float64)
+    out["correct_epsilon"] = np.array(tok[1 + nC:1 + 2 * nC], dtype=np.float64)
+    out["correct_nut"] = np.array(tok[1 + 2 * nC:1 + 3 * nC], dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, "kepsilon.npz"), **out)
+
+
 def case20(ref):
     base = os.path.join(ref, "demo/amgpcg/case20")
     out = dict(read_mesh(os.path.join(base, "constant/polyMesh")))
@@ -276,10 +317,28 @@ def case20(ref):
     np.savez_compressed(os.path.join(HERE, "case20.npz"), **out)
 
 
+def foam_files(ref):
+    """OpenFOAM ASCII files kept as they are, for the readers' tests: case20's polyMesh, its 0/T and its
+    system/fvSolution (a GAMG entry with directSolveCoarsest), and pitzDaily's system/fvSolution (a GAMG p
+    entry, regex keys, relaxation factors) from validation/matrixDumpSimple"""
+    import shutil
+    src = os.path.join(ref, "demo/amgpcg/case20")
+    dst = os.path.join(HERE, "foam", "case20")
+    for sub in ("constant/polyMesh/points", "constant/polyMesh/faces", "constant/polyMesh/owner",
+                "constant/polyMesh/neighbour", "constant/polyMesh/boundary", "0/T", "system/fvSolution"):
+        os.makedirs(os.path.dirname(os.path.join(dst, sub)), exist_ok=True)
+        shutil.copyfile(os.path.join(src, sub), os.path.join(dst, sub))
+    os.makedirs(os.path.join(HERE, "foam", "pitzDaily"), exist_ok=True)
+    shutil.copyfile(os.path.join(ref, "validation/matrixDumpSimple/system/fvSolution"),
+                    os.path.join(HERE, "foam", "pitzDaily", "fvSolution"))
+
+
 if __name__ == "__main__":
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     pitzdaily(ref)
     simple_step(ref)
+    kepsilon(ref)
     case20(ref)
-    for f in ("pitzdaily.npz", "simple_step.npz", "case20.npz"):
+    foam_files(ref)
+    for f in ("pitzdaily.npz", "simple_step.npz", "kepsilon.npz", "case20.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -100,6 +100,13 @@ def lib():
             "orc_gamg_ref_scale": (None, [I, I, P, P, P, P, P, P, P]),
             "orc_grad_u_boundary": (None, [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P]),
             "orc_simple_step": (I, [P, P, P]),
+            "orc_kepsilon_default_coeffs": (None, [P]),
+            "orc_near_wall_dist": (None, [I, I, P, P, P, P, P, P, P]),
+            "orc_y_plus_lam": (D, [D, D]),
+            "orc_nutk_wall": (D, [D, D, D, D, D, D]),
+            "orc_gbynu": (None, [I, P, P]),
+            "orc_kepsilon_eqns": (None, [P] + [P] * 10 + [P] * 12),
+            "orc_kepsilon_correct": (I, [P, P, P, P, P, P, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -564,4 +571,86 @@ def simple_step(cs, st):
     rc = lib().orc_simple_step(ctypes.byref(cs), ctypes.byref(f), ctypes.byref(r))
     if rc != 0:
         raise RuntimeError("oracle SIMPLE step refused (adjustPhi)")
+    return r
+
+
+# ---- the standard k-epsilon model (oracle/kepsilon.c) ----
+class KEpsilonCoeffs(ctypes.Structure):
+    _fields_ = [(n, _D) for n in ("Cmu", "C1", "C2", "C3", "sigmaK", "sigmaEps", "kappa", "E")]
+
+
+class KEpsilonCase(ctypes.Structure):
+    _fields_ = [("nCells", _I), ("nFaces", _I), ("nBFaces", _I), ("owner", _VP), ("neighbour", _VP),
+                ("bFaceCell", _VP), ("Sf", _VP), ("magSf", _VP), ("V", _VP), ("weights", _VP),
+                ("deltaCoeffs", _VP), ("bDeltaCoeffs", _VP), ("isWall", _VP), ("yWall", _VP), ("kType", _VP),
+                ("epsType", _VP), ("kValue", _VP), ("epsValue", _VP), ("nu", _D), ("relaxEps", _D),
+                ("relaxK", _D), ("tol", _D), ("relTol", _D), ("maxIter", _I), ("minIter", _I),
+                ("co", KEpsilonCoeffs)]
+
+
+class KEpsilonResult(ctypes.Structure):
+    _fields_ = [("epsInitial", _D), ("epsFinal", _D), ("kInitial", _D), ("kFinal", _D), ("epsIters", _I),
+                ("kIters", _I), ("nWallCells", _I)]
+
+
+def kepsilon_coeffs(**kw):
+    co = KEpsilonCoeffs()
+    lib().orc_kepsilon_default_coeffs(ctypes.byref(co))
+    for k, v in kw.items():
+        setattr(co, k, v)
+    return co
+
+
+def near_wall_dist(mesh, isWall):
+    """y per boundary face of the wall patches (0 elsewhere)"""
+    nB = mesh.nFaces - mesh.nInternal
+    y = np.zeros(nB)
+    lib().orc_near_wall_dist(mesh.nInternal, nB, p(mesh.points), p(mesh.faceOffsets), p(mesh.faceVerts),
+                             p(i32(mesh.bFaceCell)), p(f64(mesh.C)), p(i32(isWall)), p(y))
+    return y
+
+
+def kepsilon_case(mesh, isWall, yWall, kType, kValue, epsType, epsValue, nu=1e-5, relaxEps=1.0, relaxK=1.0,
+                  tol=1e-10, relTol=0.0, maxIter=2000, minIter=0, co=None):
+    keep = dict(own=mesh.own(), nei=i32(mesh.neighbour), bfc=i32(mesh.bFaceCell), Sf=f64(mesh.Sf),
+                magSf=f64(mesh.magSf), V=f64(mesh.V), w=f64(mesh.weights), dc=f64(mesh.deltaCoeffs),
+                bdc=f64(mesh.bDeltaCoeffs), isWall=i32(isWall), y=f64(yWall), kT=i32(kType), eT=i32(epsType),
+                kV=f64(kValue), eV=f64(epsValue))
+    a = lambda k: keep[k].ctypes.data
+    cs = KEpsilonCase(mesh.nCells, mesh.nInternal, mesh.nFaces - mesh.nInternal, a("own"), a("nei"), a("bfc"),
+                      a("Sf"), a("magSf"), a("V"), a("w"), a("dc"), a("bdc"), a("isWall"), a("y"), a("kT"), a("eT"),
+                      a("kV"), a("eV"), nu, relaxEps, relaxK, tol, relTol, maxIter, minIter,
+                      co if co is not None else kepsilon_coeffs())
+    cs._keep = keep
+    return cs
+
+
+def gbynu(gradU9):
+    """gradU component-major [9, n] -> GbyNu [n]"""
+    g = f64(gradU9)
+    n = g.size // 9
+    out = np.zeros(n)
+    lib().orc_gbynu(n, p(g), p(out))
+    return out
+
+
+def kepsilon_eqns(cs, k, eps, nut, nutb, G, GbyNu, phi, phiB):
+    nC, nI, nB = cs.nCells, cs.nFaces, cs.nBFaces
+    out = [np.zeros(n) for n in (nC, nI, nI, nC, nB, nB, nC, nI, nI, nC, nB, nB)]
+    lib().orc_kepsilon_eqns(ctypes.byref(cs), p(f64(k)), None, p(f64(eps)), None, p(f64(nut)),
+                            None if nutb is None else p(f64(nutb)), p(f64(G)), p(f64(GbyNu)), p(f64(phi)),
+                            p(f64(phiB)), *[p(a) for a in out])
+    return out[:6], out[6:]
+
+
+def kepsilon_correct(cs, U, Ub, phi, phiB, k, kb, eps, epsb, nut, nutb):
+    """one correct() in place on the float64 arrays k, kb, eps, epsb, nut, nutb"""
+    keepU = [f64(u) for u in U] + [f64(u) for u in Ub]
+    Ua = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in keepU[:3]])
+    Uba = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in keepU[3:]])
+    r = KEpsilonResult()
+    for a in (k, kb, eps, epsb, nut, nutb):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    lib().orc_kepsilon_correct(ctypes.byref(cs), Ua, Uba, p(f64(phi)), p(f64(phiB)), p(k), p(kb), p(eps), p(epsb),
+                               p(nut), p(nutb), ctypes.byref(r))
     return r

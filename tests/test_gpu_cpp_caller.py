@@ -20,3 +20,4 @@ def test_cpp_caller_solves_pressure(n):
     assert r.returncode == 0
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["ok"] and out["cells"] == n ** 3 and out["final"] < 1e-8
+    assert out["cache_clone_same_bits"] and out["fvsolution_gamg_same_bits"]

@@ -286,3 +286,61 @@ def test_lid_cube_mixed_precision_against_cpu_path(prec):
     np.add.at(div, mesh.bFaceCell, G.host(phiB))
     assert np.abs(div).sum() < 1e-5 * np.abs(ph).sum()   # the CPU path: 3.9e-7
     sim.close()
+
+
+def channel_case(nx, ny, nz, seed=3):
+    """the graded, jittered channel (inlet U fixedValue (10,0,0), outlet zeroGradient with p fixedValue 0,
+    walls noSlip), U0 = (10,0,0) + seeded uniform(-1,1), phi0 = interpolate(U0).Sf, p0 = 0"""
+    m = hexmesh.box_mesh(nx, ny, nz, (4.0, 1.0, 0.5), (1.0, 20.0, 20.0), 0.1, seed=seed)
+    nC, nI = m["nCells"], m["nInternal"]
+    nB = len(m["owner"]) - nI
+    mesh = O.Mesh(m["points"], np.arange(0, 4 * m["faces"].shape[0] + 1, 4), m["faces"].ravel(), m["owner"],
+                  m["neighbour"], [p[0] for p in m["patches"]], [p[1] for p in m["patches"]],
+                  [p[2] for p in m["patches"]])
+    uT = np.full(nB, O.BC_FIXED_VALUE, np.int32)
+    pT = np.full(nB, O.BC_ZERO_GRADIENT, np.int32)
+    uV = np.zeros((nB, 3))
+    for name, start, size in m["patches"]:
+        s = slice(start - nI, start - nI + size)
+        if name == "inlet":
+            uV[s] = (10.0, 0.0, 0.0)
+        elif name == "outlet":
+            uT[s] = O.BC_ZERO_GRADIENT
+            pT[s] = O.BC_FIXED_VALUE
+    U0 = np.random.default_rng(seed).uniform(-1.0, 1.0, (nC, 3))
+    U0[:, 0] += 10.0
+    U = [U0[:, k].copy() for k in range(3)]
+    Ub = [O.evaluate_boundary(mesh.bFaceCell, uT, uV[:, k], U[k]) for k in range(3)]
+    own, nei = mesh.own(), mesh.neighbour
+    w = mesh.weights[:, None]
+    phi0 = ((w * U0[own] + (1 - w) * U0[nei]) * mesh.Sf[:nI]).sum(1)
+    phiB0 = (np.stack(Ub, 1) * mesh.Sf[nI:]).sum(1)
+    st = O.SimpleState(U, Ub, np.zeros(nC), np.zeros(nB), phi0, phiB0)
+    return m, mesh, uT, uV, pT, np.zeros(nB), st
+
+
+def test_simple_step_channel_corrected_linear_upwind_bit_exact():
+    """the bench's options on a small graded, jittered channel: linearUpwind U, the corrected laplacian with
+    one non-orthogonal corrector, p fixed at the outlet (no reference cell); sequential reductions and the
+    FP64 cycle give the oracle's composition to the bit, two iterations in a row"""
+    m, mesh, uT, uV, pT, pV, st = channel_case(21, 11, 9)
+    print("case: SIMPLE channel corrected linearUpwind", mesh.nCells, flush=True)
+    dc = DeviceCase(mesh, uT, uV, pT, pV)
+    out = dc.fields(st)
+    sim = dc.simple(mfv.simple_controls(U=dict(tolerance=1e-8, relTol=0.0),
+                                        p=dict(tolerance=1e-9, precision=mfv.PREC_FP64, nPreSweeps=1, nPostSweeps=1,
+                                               flexible=0, scaleCorrection=0, checkEvery=3),
+                                        mergeLevels=1, nu=1e-5, linearUpwind=1, correctedLaplacian=1, nNonOrthCorr=1,
+                                        pRefCell=-1))
+    ag = O.brae_agglomeration(mesh.nCells, mesh.own(), mesh.neighbour, O.f64(mesh.magSf[:mesh.nInternal]), target=64)
+    cs = O.simple_case(mesh, uT, uV.ravel(), pT, pV, nu=1e-5, tolU=1e-8, tolP=1e-9, pSolver=O.P_BRAE_AMGPCG, aggl=ag,
+                       amgPre=1, amgPost=1, linearUpwind=1, correctedLaplacian=1, nNonOrthCorr=1)
+    for it in range(2):
+        dc.case.ctx.sequential_reduce(True)
+        r = sim.step(*out)
+        dc.case.ctx.sequential_reduce(False)
+        ro = O.simple_step(cs, st)
+        assert list(r.UIters) == list(ro.UIters), (it, list(r.UIters), list(ro.UIters))
+        assert list(r.pIters[:2]) == list(ro.pIters[:2]), (it, list(r.pIters[:2]), list(ro.pIters[:2]))
+        compare_bits(out, st)
+    sim.close()
