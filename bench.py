@@ -75,164 +75,15 @@ def parse():
     return ap.parse_args()
 
 
-class Channel:
-    """The graded, jittered channel and its synthetic pressure-corrector inputs on the device."""
-
-    def __init__(self, nx, ny, nz, torch, mfv, hexmesh, ctx):
-        t0 = time.time()
-        self.shape = (nx, ny, nz)
-        pts = hexmesh.box_points(nx, ny, nz, (4.0, 1.0, 0.5), (1.0, 20.0, 20.0), 0.1, seed=2024)
-        faces, owner, nei, patches = hexmesh.box_topology(nx, ny, nz)
-        self.points, self.faces = pts, faces
-        self.nC, self.nI = nx * ny * nz, nei.size
-        self.nB = owner.size - self.nI
-        self.patches = patches
-        log("mesh %dx%dx%d: %d cells, %d internal faces (%.1fs)" % (nx, ny, nz, self.nC, self.nI, time.time() - t0))
-        self.own = np.ascontiguousarray(owner[:self.nI])
-        self.nei = nei
-        self.bcell = np.ascontiguousarray(owner[self.nI:])
-        self.mesh = mfv.Mesh(ctx, self.nC, self.own, nei, self.bcell)
-        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
-        z = lambda n: torch.zeros(int(n), dtype=torch.float64, device="cuda:0")
-        nF = owner.size
-        dpts = dev(pts)
-        dfo = dev(np.arange(0, 4 * nF + 1, 4, dtype=np.int32))
-        dfv = dev(faces.reshape(-1))
-        self.Cf, self.Sf, self.magSf = z(3 * nF), z(3 * nF), z(nF)
-        self.C, self.V = z(3 * self.nC), z(self.nC)
-        self.w, self.dc, self.ndc, self.corr, self.bDelta = z(self.nI), z(self.nI), z(self.nI), z(3 * self.nI), z(self.nB)
-        torch.cuda.synchronize()
-        ctx.call("mfv_mesh_geometry", self.mesh.h, *[mfv.ptr(t) for t in (
-            dpts, dfo, dfv, self.Cf, self.Sf, self.magSf, self.C, self.V, self.w, self.dc, self.ndc, self.corr,
-            self.bDelta)])
-        ctx.sync()
-        del dpts, dfo, dfv
-        log("geometry on device (%.1fs)" % (time.time() - t0))
-        # synthetic seeded fields
-        g = torch.Generator(device="cuda:0")
-        g.manual_seed(7)
-        U = torch.rand((self.nC, 3), dtype=torch.float64, device="cuda:0", generator=g) * 2.0 - 1.0
-        U[:, 0] += 10.0
-        own_d, nei_d = dev(owner[:self.nI]).long(), dev(nei).long()
-        Sf = self.Sf.view(nF, 3)
-        wv = self.w.view(-1, 1)
-        Uf = wv * U[own_d] + (1.0 - wv) * U[nei_d]
-        self.phiHbyA = (Uf * Sf[:self.nI]).sum(1).contiguous()
-        phiB = torch.zeros(self.nB, dtype=torch.float64, device="cuda:0")
-        bcell = dev(owner[self.nI:]).long()
-        self.types = torch.zeros(self.nB, dtype=torch.int32, device="cuda:0")
-        for name, start, size in patches:
-            s = slice(start - self.nI, start - self.nI + size)
-            if name == "inlet":
-                phiB[s] = 10.0 * Sf[start:start + size, 0]
-            elif name == "outlet":
-                phiB[s] = (U[bcell[s]] * Sf[start:start + size]).sum(1)
-                self.types[s] = 1
-        self.phiB = phiB.contiguous()
-        self.bMagSf = self.magSf[self.nI:].contiguous()
-        self.magSfI = self.magSf[:self.nI].contiguous()
-        Umag = U.norm(dim=1)
-        self.negRAU = (-0.05 * self.V.pow(1.0 / 3.0) / Umag).contiguous()
-        self.bcValue = z(self.nB)
-        del U, Uf, own_d, nei_d, bcell
-        # work arrays
-        self.gf, self.gb = z(self.nI), z(self.nB)
-        self.diag, self.upper = z(self.nC), z(self.nI)
-        self.iC, self.bC = z(self.nB), z(self.nB)
-        self.source, self.diagC, self.b = z(self.nC), z(self.nC), z(self.nC)
-        self.p, self.phi = z(self.nC), z(self.nI)
-        self.gradP, self.ffc = z(3 * self.nC), z(self.nI)
-        torch.cuda.synchronize()
-        log("fields ready (%.1fs)" % (time.time() - t0))
-
-
-def channel_bcs(ch, torch):
-    """The channel's boundary conditions per boundary face: inlet U fixedValue (10,0,0) with p
-    zeroGradient; outlet U zeroGradient with p fixedValue 0; the four walls noSlip with p zeroGradient."""
-    from brae_amd import mfv
-    nB, nI = ch.nB, ch.nI
-    uT = np.full(nB, mfv.BC_FIXED_VALUE, np.int32)
-    pT = np.full(nB, mfv.BC_ZERO_GRADIENT, np.int32)
-    uV = np.zeros((nB, 3))
-    pV = np.zeros(nB)
-    for name, start, size in ch.patches:
-        s = slice(start - nI, start - nI + size)
-        if name == "inlet":
-            uV[s] = (10.0, 0.0, 0.0)
-        elif name == "outlet":
-            uT[s] = mfv.BC_ZERO_GRADIENT
-            pT[s] = mfv.BC_FIXED_VALUE
-    return uT, uV, pT, pV
-
-
-def simple_fields(ch, torch, uT, uV, seed=7):
-    """Seeded initial fields: U = (10,0,0) + uniform(-1,1) per component, U's patch values from the
-    boundary conditions, phi = interpolate(U).Sf inside and Ub.Sf on the boundary, p = 0."""
-    g = torch.Generator(device="cuda:0")
-    g.manual_seed(seed)
-    nC, nI, nB = ch.nC, ch.nI, ch.nB
-    U = torch.rand((nC, 3), dtype=torch.float64, device="cuda:0", generator=g) * 2.0 - 1.0
-    U[:, 0] += 10.0
-    bcell = torch.from_numpy(ch.bcell).to("cuda:0").long()
-    fixed = torch.from_numpy(uT == 1).to("cuda:0")
-    Ub = torch.where(fixed[:, None], torch.from_numpy(uV).to("cuda:0"), U[bcell])
-    own_d = torch.from_numpy(ch.own).to("cuda:0").long()
-    nei_d = torch.from_numpy(ch.nei).to("cuda:0").long()
-    nF = nI + nB
-    Sf = ch.Sf.view(nF, 3)
-    wv = ch.w.view(-1, 1)
-    phi = ((wv * U[own_d] + (1.0 - wv) * U[nei_d]) * Sf[:nI]).sum(1).contiguous()
-    phiB = (Ub * Sf[nI:]).sum(1).contiguous()
-    fields = dict(U=[U[:, k].contiguous() for k in range(3)], Ub=[Ub[:, k].contiguous() for k in range(3)],
-                  p=torch.zeros(nC, dtype=torch.float64, device="cuda:0"),
-                  pb=torch.zeros(nB, dtype=torch.float64, device="cuda:0"), phi=phi, phiB=phiB)
-    torch.cuda.synchronize()
-    return fields
-
-
-def channel_kepsilon(ch, torch, mfv, args):
-    """The channel's k-epsilon set-up: wall functions on the four walls (their near-wall distance from the
-    library's host nearWallDist), inlet k = 0.375 and epsilon = 14.855, zeroGradient elsewhere; initial k and
-    epsilon uniform at the inlet values, nut = Cmu k^2/epsilon (cells and boundary faces)."""
-    t0 = time.time()
-    nB, nI, nC = ch.nB, ch.nI, ch.nC
-    isWall = np.zeros(nB, np.int32)
-    ktype = np.full(nB, mfv.BC_ZERO_GRADIENT, np.int32)
-    kV, eV = np.zeros(nB), np.zeros(nB)
-    for name, start, size in ch.patches:
-        s = slice(start - nI, start - nI + size)
-        if name in ("lowerWall", "upperWall", "back", "front"):
-            isWall[s] = 1
-        elif name == "inlet":
-            ktype[s] = mfv.BC_FIXED_VALUE
-            kV[s], eV[s] = 0.375, 14.855
-    nF = nI + nB
-    y = mfv.near_wall_dist(nI, ch.points, np.arange(0, 4 * nF + 1, 4, dtype=np.int32), ch.faces.reshape(-1),
-                           ch.bcell, ch.C.cpu().numpy(), isWall)
-    log("near-wall distance on %d wall faces (%.1fs host)" % (int(isWall.sum()), time.time() - t0))
-    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
-    k0 = torch.full((nC,), 0.375, dtype=torch.float64, device="cuda:0")
-    e0 = torch.full((nC,), 14.855, dtype=torch.float64, device="cuda:0")
-    bc = torch.from_numpy(ch.bcell).to("cuda:0").long()
-    fixed = dev(ktype == mfv.BC_FIXED_VALUE)
-    kb0 = torch.where(fixed, dev(kV), k0[bc])
-    eb0 = torch.where(fixed, dev(eV), e0[bc])
-    fields0 = dict(k=k0, kb=kb0.contiguous(), eps=e0, epsb=eb0.contiguous(), nut=(0.09 * k0 * k0 / e0).contiguous(),
-                   nutb=(0.09 * kb0 * kb0 / eb0).contiguous())
-    controls = mfv.kepsilon_controls(solver=dict(tolerance=args.tol_u, relTol=args.reltol_u, maxIter=1000,
-                                                 preconditioner=mfv.PRECOND_DIAGONAL, checkEvery=args.check_every),
-                                     nu=args.nu, relaxEps=0.7, relaxK=0.7)
-    return dict(isWall=isWall, y=y, type=ktype, kV=kV, eV=eV, fields0=fields0, controls=controls)
-
-
 def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
+    from brae_amd import channel as C
     """Whole SIMPLE iterations on the channel (the headline): UEqn (upwind convection with linearUpwind's
     deferred correction, corrected laplacian(nuEff, U), the explicit stress, relax 0.7), the momentum
     predictor (PBiCGStab per component), the pressure predictor, nNonOrthogonalCorrectors + 1 pressure
     solves (AMG-PCG to --tolerance) with the corrected laplacian, phi, p relaxed by 0.3 and U corrected.
     The mixed path and the FP64 path run the same iterations from the same seeded fields; only the
     pressure V-cycle's precision differs (and the outer CG is flexible when the cycle is not FP64)."""
-    uT, uV, pT, pV = channel_bcs(ch, torch)
+    uT, uV, pT, pV = C.channel_bcs(ch, torch)
     geom = mfv.Geometry(*(mfv.ptr(t) for t in (ch.Sf, ch.Cf, ch.magSf, ch.C, ch.V, ch.w, ch.dc, ch.ndc, ch.corr,
                                                 ch.bDelta)))
     U_ctl = dict(tolerance=args.tol_u, relTol=args.reltol_u, maxIter=1000,
@@ -248,7 +99,8 @@ def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
 
     turb = None
     if args.turbulence:
-        turb = channel_kepsilon(ch, torch, mfv, args)
+        turb = C.channel_kepsilon(ch, torch, mfv, nu=args.nu, tol=args.tol_u, relTol=args.reltol_u,
+                                  checkEvery=args.check_every, log=log)
     runs = [("mixed", prec_mixed), ("fp64", mfv.PREC_FP64)] + ([] if args.no_other else [("other", other)])
     out = {}
     for name, prec in runs:
@@ -261,7 +113,7 @@ def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
         log("SIMPLE %s: hierarchy built (%.1fs)" % (name, time.time() - t0))
 
         def fresh():
-            f = simple_fields(ch, torch, uT, uV)
+            f = C.simple_fields(ch, torch, uT, uV)
             if turb is not None:
                 f.update({k: v.clone() for k, v in turb["fields0"].items()})
             torch.cuda.synchronize()
@@ -328,6 +180,7 @@ def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
 
 def gpu_run(args):
     import torch
+    from brae_amd import channel as C
     from brae_amd import hexmesh, mfv
 
     nx, ny, nz = (int(v) for v in args.size.split("x"))
@@ -335,7 +188,7 @@ def gpu_run(args):
     stream = torch.cuda.Stream(device="cuda:0")
     torch.cuda.set_stream(stream)
     ctx = mfv.Context(0, stream=stream.cuda_stream)
-    ch = Channel(nx, ny, nz, torch, mfv, hexmesh, ctx)
+    ch = C.Channel(nx, ny, nz, torch, mfv, hexmesh, ctx, log=log)
     prec_mixed = mfv.PREC_FP32 if args.precision == "fp32" else mfv.PREC_FP16
     other = mfv.PREC_FP16 if prec_mixed == mfv.PREC_FP32 else mfv.PREC_FP32
     simple = simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other) if args.simple else None
@@ -348,36 +201,10 @@ def gpu_run(args):
     A = mfv.ldu(ch.diagC, ch.upper)
 
     def step(prec, gamg=None):
-        """one SIMPLE pressure corrector: nNonOrthogonalCorrectors + 1 assemblies and solves of the same
-        matrix, the explicit non-orthogonal correction rebuilt from the latest p before each, and the flux
-        (with the stored correction flux) after the last. With `gamg` the solve is OpenFOAM's PCG with
-        the GAMG preconditioner (GaussSeidel, nVcycles 2) on that hierarchy, as the CPU path solves it."""
-        ch.p.zero_()
-        ctx.call("mfv_interpolate", ch.mesh.h, P(ch.w), P(ch.negRAU), P(ch.gf), P(ch.gb))
-        ctx.call("mfv_laplacian_coeffs", ch.mesh.h, P(ch.ndc), P(ch.gf), P(ch.magSfI), P(ch.diag), P(ch.upper), None)
-        ctx.call("mfv_laplacian_bcoeffs", ch.mesh.h, P(ch.types), P(ch.bcValue), P(ch.gb), P(ch.bMagSf),
-                 P(ch.bDelta), P(ch.iC), P(ch.bC))
-        perfs = []
-        for corr in range(args.non_orth + 1):
-            ch.source.zero_()
-            ctx.call("mfv_grad_gauss", ch.mesh.h, P(ch.types), P(ch.bcValue), P(ch.Sf), P(ch.w), P(ch.V), P(ch.p),
-                     P(ch.gradP))
-            ctx.call("mfv_laplacian_corr", ch.mesh.h, P(ch.gradP), P(ch.w), P(ch.corr), P(ch.gf), P(ch.magSfI),
-                     P(ch.ffc), P(ch.source))
-            ctx.call("mfv_div_source", ch.mesh.h, P(ch.phiHbyA), P(ch.phiB), P(ch.V), -1.0, P(ch.source))
-            ctx.call("mfv_fold", ch.mesh.h, P(ch.diag), P(ch.source), P(ch.iC), P(ch.bC), P(ch.diagC), P(ch.b))
-            if gamg is not None:
-                perfs.append(gamg.pcg_gamg_solve(ch.diagC, ch.upper, ch.b, ch.p, args.tolerance,
-                                                 smoother="GaussSeidel"))
-                continue
-            if corr == 0:
-                amg.galerkin(ch.diagC, ch.upper, prec)
-            perfs.append(amg.solve(ch.diagC, ch.upper, ch.b, ch.p, tolerance=args.tolerance, precision=prec,
-                                   flexible=int(prec != mfv.PREC_FP64 or not args.no_scale),
-                                   checkEvery=args.check_every, scaleCorrection=int(not args.no_scale),
-                                   nPreSweeps=args.npre, nPostSweeps=args.npost, captureVcycle=args.graph))
-        ctx.call("mfv_flux_correct", ch.mesh.h, ctypes.byref(A), P(ch.p), P(ch.ffc), P(ch.phiHbyA), -1.0, P(ch.phi))
-        return perfs
+        """one SIMPLE pressure corrector (brae_amd.channel.pressure_corrector)"""
+        return C.pressure_corrector(ch, ctx, mfv, amg, prec, tolerance=args.tolerance, nonOrth=args.non_orth,
+                                    checkEvery=args.check_every, nPre=args.npre, nPost=args.npost,
+                                    scale=not args.no_scale, graph=args.graph, gamg=gamg)
 
     def timed(prec, profile):
         for _ in range(args.warmup):

@@ -473,14 +473,19 @@ AmgBuildControls buildControls(const mfv_amg_controls* controls)
 {
     AmgBuildControls bc;
     if (!controls) return bc;
-    if (controls->agglomeration != MFV_AGG_BRAE && controls->agglomeration != MFV_AGG_FACEAREAPAIR)
+    if (controls->agglomeration != MFV_AGG_BRAE && controls->agglomeration != MFV_AGG_FACEAREAPAIR &&
+        controls->agglomeration != MFV_AGG_PAIR)
         throw Error(kErrUnsupported, "mfv: amg build: unknown agglomeration " + std::to_string(controls->agglomeration));
+    if (controls->agglomeration == MFV_AGG_PAIR && controls->mergeLevels > 1)
+        throw Error(kErrUnsupported, "mfv: amg build: gamg.cu's levels are single pairwise passes (mergeLevels 1)");
     if (controls->mergeLevels > 4) throw Error(kErrUnsupported, "mfv: amg build: mergeLevels above 4 is not carried");
     bc.mergeLevels = std::max(1, controls->mergeLevels);
     if (controls->fineLayout != MFV_LAYOUT_AUTO && controls->fineLayout != MFV_LAYOUT_ROWS)
         throw Error(kErrUnsupported, "mfv: amg build: unknown fine layout " + std::to_string(controls->fineLayout));
     bc.fineRows = controls->fineLayout == MFV_LAYOUT_ROWS;
-    bc.kind = controls->agglomeration == MFV_AGG_BRAE ? AggKind::Brae : AggKind::OpenFOAM;
+    bc.kind = controls->agglomeration == MFV_AGG_BRAE   ? AggKind::Brae
+              : controls->agglomeration == MFV_AGG_PAIR ? AggKind::Pair
+                                                        : AggKind::OpenFOAM;
     bc.target = controls->nCellsInCoarsestLevel > 0 ? controls->nCellsInCoarsestLevel
                                                     : (bc.kind == AggKind::Brae ? 64 : 10);
     return bc;
@@ -816,6 +821,27 @@ int mfv_pcg_gamg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A
         const PcgPerf r = pcgGamgSolve(ctx->c, mesh->m, amg->a, v.diag, v.upper, dB, dPsi, tolerance, relTol, maxIter,
                                        minIter, gamgControls(*controls));
         putPerf(r, perf);
+    });
+}
+
+int mfv_gamg_ref_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
+                       double tolerance, double relTol, int maxIter, int minIter, int nPostSweeps, int nFinestSweeps,
+                       mfv_perf* perf)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(amg && amg->ctx == ctx && amg->mesh == mesh, "gamg.cu: hierarchy of another mesh or context");
+        need(dB && dPsi && dB != dPsi, "gamg.cu: null or aliased argument");
+        const LduView v = view(mesh, A);
+        if (v.lower != v.upper) throw Error(kErrUnsupported, "mfv: gamg.cu: an asymmetric matrix is not carried");
+        GamgRefControls g;
+        g.tolerance = tolerance;
+        g.relTol = relTol;
+        g.maxIter = maxIter;
+        g.minIter = minIter;
+        g.nPostSweeps = nPostSweeps;
+        g.nFinestSweeps = nFinestSweeps;
+        putPerf(gamgRefSolve(ctx->c, mesh->m, amg->a, v.diag, v.upper, dB, dPsi, g), perf);
     });
 }
 
@@ -1318,6 +1344,7 @@ int mfv_simple_default_controls(mfv_simple_controls* c)
     c->p.tolerance = 1e-10;
     c->p.precision = MFV_PREC_FP64;
     c->mergeLevels = 1;
+    c->pSolver = MFV_P_AMG_PCG;
     return MFV_OK;
 }
 
@@ -1354,6 +1381,9 @@ int mfv_simple_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry, 
         sc.pRefValue = controls->pRefValue;
         sc.U = bicgControls(controls->U);
         sc.P = pcgControls(controls->p);
+        if (controls->pSolver != MFV_P_AMG_PCG && controls->pSolver != MFV_P_GAMG_REF)
+            throw Error(kErrUnsupported, "mfv: simple: unknown pressure solver " + std::to_string(controls->pSolver));
+        sc.pSolver = controls->pSolver == MFV_P_GAMG_REF ? PSolver::GamgRef : PSolver::AmgPcg;
         std::vector<int> ut(hUType, hUType + nB), pt(hPType, hPType + nB);
         std::vector<double> uv(hUValue, hUValue + 3 * (size_t)nB), pv(hPValue, hPValue + nB);
         std::unique_ptr<mfv_simple_s> h(new mfv_simple_s{ctx, mesh, nullptr});

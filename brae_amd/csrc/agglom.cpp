@@ -151,6 +151,63 @@ void braeCoarseFaces(AggLevel& L, const std::vector<int>& own, const std::vector
     }
 }
 
+// gamg.cu agglomeratePair (gamg.cu:41-107), one pass: cells visited forward or in reverse; a cell's faces
+// are its upper-side ones, then its lower-side ones, each in face order; pair with the heaviest face whose
+// cells are both unmatched (strict >, -1e300 seed), else join the cluster across the heaviest face (the
+// larger of its two cells' numbers; nothing when both are unmatched), singletons numbered after the walk in
+// its direction; a reverse pass renumbers c -> nCoarse - 1 - c.
+std::vector<int> pairRefPairing(int n, const std::vector<int>& lower, const std::vector<int>& upper,
+                                const std::vector<double>& fw, bool fwd, int& nCoarse)
+{
+    std::vector<int> off, cf;
+    cellFaces(n, lower, upper, true, off, cf);
+    std::vector<int> map(n, -1);
+    nCoarse = 0;
+    for (int k = 0; k < n; ++k)
+    {
+        const int c = fwd ? k : n - k - 1;
+        if (map[c] >= 0) continue;
+        int mf = -1;
+        double mw = -1e300;
+        for (int o = off[c]; o < off[c + 1]; ++o)
+        {
+            const int f = cf[o];
+            if (map[upper[f]] < 0 && map[lower[f]] < 0 && fw[f] > mw)
+            {
+                mf = f;
+                mw = fw[f];
+            }
+        }
+        if (mf >= 0)
+        {
+            map[upper[mf]] = nCoarse;
+            map[lower[mf]] = nCoarse;
+            ++nCoarse;
+            continue;
+        }
+        int cm = -1;
+        double cmw = -1e300;
+        for (int o = off[c]; o < off[c + 1]; ++o)
+        {
+            const int f = cf[o];
+            if (fw[f] > cmw)
+            {
+                cm = f;
+                cmw = fw[f];
+            }
+        }
+        if (cm >= 0) map[c] = std::max(map[upper[cm]], map[lower[cm]]);
+    }
+    for (int k = 0; k < n; ++k)
+    {
+        const int c = fwd ? k : n - k - 1;
+        if (map[c] < 0) map[c] = nCoarse++;
+    }
+    if (!fwd)
+        for (int& c : map) c = nCoarse - 1 - c;
+    return map;
+}
+
 // pairGAMGAgglomeration::agglomerate, one pass (pair_gamg_agglomeration_cpp.cu:158-284)
 std::vector<int> ofPairing(int n, const std::vector<int>& lower, const std::vector<int>& upper,
                            const std::vector<double>& fw, bool& forward, int& nCoarse)
@@ -295,6 +352,29 @@ std::vector<AggLevel> agglomerate(AggKind kind, int nCells, const std::vector<in
             std::vector<double> cfw;
             braeCoarseFaces(L, *own, *nei, fw, cfw);
             if (L.nCoarse >= n || L.nCoarseFaces == 0) break;
+            levels.push_back(std::move(L));
+            fw = std::move(cfw);
+            own = &levels.back().cOwn;
+            nei = &levels.back().cNei;
+            n = levels.back().nCoarse;
+        }
+        return levels;
+    }
+    if (kind == AggKind::Pair)
+    {
+        // gamg.cu:290-312: at most 50 grids; a pass that reaches target cells or fewer, does not coarsen, or
+        // leaves fewer than 2 cells ends the hierarchy
+        bool fwd = true;
+        while ((int)levels.size() < 49)
+        {
+            AggLevel L;
+            L.nFine = n;
+            L.nFineFaces = (int)own->size();
+            L.map = pairRefPairing(n, *own, *nei, fw, fwd, L.nCoarse);
+            fwd = !fwd;
+            if (L.nCoarse <= target || L.nCoarse >= n || L.nCoarse < 2) break;
+            std::vector<double> cfw;
+            braeCoarseFaces(L, *own, *nei, fw, cfw);
             levels.push_back(std::move(L));
             fw = std::move(cfw);
             own = &levels.back().cOwn;

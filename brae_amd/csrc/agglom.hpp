@@ -7,6 +7,10 @@
 //   OpenFOAM  faceAreaPair (pair_gamg_agglomeration_cpp.cu:28-345): perturbed weights supplied by the
 //             caller, direction flag alternating every pass, coarse faces owner-major in first-appearance
 //             order, stop when a pass would go under nCellsInCoarsestLevel.
+//   Pair      the host CPU path's GAMG (src/OpenFOAM/matrices/gamg.cu:41-172 agglomeratePair + coarsen):
+//             each cell's upper-side faces then its lower-side ones, direction alternating every pass from
+//             forward, singletons numbered last, a reverse pass renumbered n - 1 - id; coarse faces as sorted
+//             unique (lo, hi) pairs; stop when a pass would reach nCellsInCoarsest cells or fewer.
 #pragma once
 
 #include <cstdint>
@@ -23,7 +27,7 @@ struct AggLevel
     std::vector<int8_t> faceFlip;   // [nFineFaces]
 };
 
-enum class AggKind : int { Brae = 0, OpenFOAM = 1 };
+enum class AggKind : int { Brae = 0, OpenFOAM = 1, Pair = 2 };
 
 // fineOwner/fineNeighbour: the mesh's internal faces; weights per internal face.
 // target: brae's TARGET (coarsen while n > target) or OpenFOAM's nCellsInCoarsestLevel.

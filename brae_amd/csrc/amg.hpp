@@ -123,6 +123,7 @@ struct OfWork
     DBuf<double> ACf, preSmoothed, tR, tW, bPrime;  // fine-sized scratch
     DBuf<double> Apsi, fCorr, fRes, pA, wA, rA, AwA;
     DBuf<double> st;                                // device scalars
+    DBuf<double> dense;                             // gamg.cu's coarsest elimination: n*n matrix + n rhs
 };
 
 struct Amg
@@ -256,6 +257,20 @@ struct GamgControls
 // Synchronises once per V-cycle (the stopping test).
 PcgPerf gamgSolve(Ctx& c, Mesh& m, Amg& a, const double* diag, const double* upper, const double* b, double* psi,
                   const GamgControls& g);
+// The host CPU path's GAMG (src/OpenFOAM/matrices/gamg.cu:267-400, the pressure solver cpu::simpleStep
+// calls): on a hierarchy built with AggKind::Pair on |Sf| down to nCellsInCoarsest, the folded symmetric
+// system (diag, upper), psi in place. Per cycle the residual is restricted straight down, the coarsest level
+// solved by Gaussian elimination with partial pivoting, then per level injection, gamg.cu's correction
+// scaling and nPostSweeps Gauss-Seidel sweeps; the finest correction scaled against the residual, added, and
+// psi smoothed nFinestSweeps times against b.
+struct GamgRefControls
+{
+    double tolerance = 1e-6, relTol = 0.0;
+    int maxIter = 1000, minIter = 0;
+    int nPostSweeps = 2, nFinestSweeps = 2;
+};
+PcgPerf gamgRefSolve(Ctx& c, Mesh& m, Amg& a, const double* diag, const double* upper, const double* b, double* psi,
+                     const GamgRefControls& g);
 // PCG with the GAMG preconditioner (pcgGamgSolve, gamg_solver_cpp.cu:1027-1143): each application runs
 // g.nVcycles cycles from zero; tolerance..minIter are the PCG's, g's tolerance/relTol the coarsest
 // solve's. Synchronises once per iteration.

@@ -448,9 +448,10 @@ KEpsilonResult KEpsilon::correct(const double* const* U, const double* const* Ub
     // nut's boundary as it enters: the equations' patch diffusivities
     if (nB) launchCopy(c, nB, nutb, nutB0.p);
     // production from the current nut, grad(U) with every non-empty face at U's patch value
-    for (int j = 0; j < 3; ++j) launchGradGauss(c, m, gType.p, Ub[j], g.Sf, g.w, g.V, U[j], g3[j].p);
-    const double* g3p[3] = {g3[0].p, g3[1].p, g3[2].p};
-    launchPackGrad(c, n, g3p, gradU.p);
+    {
+        double* g3o[3] = {g3[0].p, g3[1].p, g3[2].p};
+        launchGradGauss3(c, m, gType.p, Ub, g.Sf, g.w, g.V, U, g3o, gradU.p);
+    }
     if (n)
         keProductionK<<<blocks(n), kTpb, 0, c.stream>>>(n, m.ownerStart.p, m.losortStart.p, m.losort.p,
                                                         m.bndStart.p, m.bndFaces.p, isEmpty.p, gradU.p, nut, phi,

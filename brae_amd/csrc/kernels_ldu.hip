@@ -350,57 +350,6 @@ __global__ __launch_bounds__(kTpb) void fluxCorrectK(int nF, const label* __rest
 
 // ---------------- non-orthogonal correction ----------------
 
-// Gauss linear gradient of a cell field with its patch values (fvc.cu:67-121): per cell, its faces in
-// ascending face index (neighbour-side run, then owned run: -Sf*pf and +Sf*pf with the face value
-// w*(P - N) + N), then its boundary faces in order (empty: none; fixedValue: the bc value;
-// zeroGradient: the cell value), divided by V. Sf: xyz per face, internal then boundary; grad: xyz.
-__global__ __launch_bounds__(kTpb) void gradGaussK(int n, int nF, const label* __restrict__ ownerStart,
-                                                   const label* __restrict__ losortStart,
-                                                   const label* __restrict__ losort,
-                                                   const label* __restrict__ losortOwner,
-                                                   const label* __restrict__ neighbour,
-                                                   const label* __restrict__ bndStart,
-                                                   const label* __restrict__ bndFaces, const int* __restrict__ bcType,
-                                                   const double* __restrict__ bcValue, const double* __restrict__ Sf,
-                                                   const double* __restrict__ w, const double* __restrict__ V,
-                                                   const double* __restrict__ psi, double* __restrict__ grad)
-{
-    const int c = xcdBlock() * kTpb + threadIdx.x;
-    if (c >= n) return;
-    const double pc = psi[c];
-    double gx = 0.0, gy = 0.0, gz = 0.0;
-    for (int k = losortStart[c]; k < losortStart[c + 1]; ++k)
-    {
-        const int f = losort[k];
-        const double pf = w[f] * (psi[losortOwner[k]] - pc) + pc;
-        gx = gx - Sf[3 * f] * pf;
-        gy = gy - Sf[3 * f + 1] * pf;
-        gz = gz - Sf[3 * f + 2] * pf;
-    }
-    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f)
-    {
-        const double pf = w[f] * (pc - psi[neighbour[f]]) + psi[neighbour[f]];
-        gx = gx + Sf[3 * f] * pf;
-        gy = gy + Sf[3 * f + 1] * pf;
-        gz = gz + Sf[3 * f + 2] * pf;
-    }
-    for (int k = bndStart[c]; k < bndStart[c + 1]; ++k)
-    {
-        const int i = bndFaces[k];
-        const int t = bcType[i];
-        if (t == 2) continue;
-        const double pv = t == 1 ? bcValue[i] : pc;
-        const long long fb = 3LL * (nF + i);
-        gx = gx + Sf[fb] * pv;
-        gy = gy + Sf[fb + 1] * pv;
-        gz = gz + Sf[fb + 2] * pv;
-    }
-    const double v = V[c];
-    grad[3 * c] = gx / v;
-    grad[3 * c + 1] = gy / v;
-    grad[3 * c + 2] = gz / v;
-}
-
 // ffc = (gamma*magSf) * (corrVec . (w*grad[own] + (1 - w)*grad[nei])) per internal face (fvm.cuh:170-222)
 __global__ __launch_bounds__(kTpb) void corrFluxK(int nF, const label* __restrict__ owner,
                                                   const label* __restrict__ neighbour, const double* __restrict__ grad,
@@ -486,16 +435,6 @@ void launchFluxCorrect(Ctx& c, const Mesh& m, const LduView& A, const double* ps
     fluxCorrectK<<<blocks(m.nFaces), kTpb, 0, c.stream>>>(m.nFaces, m.owner.p, m.neighbour.p, A.upper, A.lower, psi,
                                                           ffc, phiHbyA, sign, phi);
     hipCheck(hipGetLastError(), "flux correct");
-}
-
-void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcValue, const double* Sf,
-                     const double* w, const double* V, const double* psi, double* grad)
-{
-    if (!m.nCells) return;
-    gradGaussK<<<blocks(m.nCells), kTpb, 0, c.stream>>>(m.nCells, m.nFaces, m.ownerStart.p, m.losortStart.p,
-                                                        m.losort.p, m.losortOwner.p, m.neighbour.p, m.bndStart.p,
-                                                        m.bndFaces.p, bcType, bcValue, Sf, w, V, psi, grad);
-    hipCheck(hipGetLastError(), "grad gauss");
 }
 
 void launchLaplacianCorr(Ctx& c, const Mesh& m, const double* grad, const double* w, const double* corrVec,

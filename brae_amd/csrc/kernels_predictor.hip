@@ -793,7 +793,146 @@ __global__ __launch_bounds__(kTpb) void zeroK(int n, double* __restrict__ x)
     if (i < n) x[i] = 0.0;
 }
 
+// The Gauss linear gradient of NC cell fields that share their patches' types (fvc.cu:67-121), each
+// component's arithmetic the host loop's: per cell its faces in face order -- the losort run (-Sf*pf), the owned run
+// (+Sf*pf), pf = w*(P - N) + N -- then its boundary faces in order (empty: none; fixedValue: the component's
+// bc value; zeroGradient: the cell value), over V. Each face's weight, area vector and neighbour index are
+// loaded once for all NC components. Writes each component's gradient xyz per cell (g3) and, when g9 is
+// given, the packed tensor g9[(i*3 + k)*n + c] = d(psi_k)/dx_i.
+struct Grad3Args
+{
+    const double* psi[3];
+    const double* bcValue[3];
+    double* g3[3];
+    double* g9;
+};
+
+template <bool Slab, int NC>
+__global__ __launch_bounds__(kTpb) void gradGauss3K(int n, SlabPart part, int nF, const label* __restrict__ ownerStart,
+                                                    const label* __restrict__ losortStart,
+                                                    const label* __restrict__ losort,
+                                                    const label* __restrict__ losortOwner,
+                                                    const label* __restrict__ neighbour,
+                                                    const label* __restrict__ bndStart,
+                                                    const label* __restrict__ bndFaces,
+                                                    const int* __restrict__ bcType, const double* __restrict__ Sf,
+                                                    const double* __restrict__ w, const double* __restrict__ V,
+                                                    Grad3Args a)
+{
+    const int c = gatherCell<Slab>(n, part);
+    if (c < 0) return;
+    double pc[NC], gx[NC], gy[NC], gz[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+    {
+        pc[j] = a.psi[j][c];
+        gx[j] = 0.0;
+        gy[j] = 0.0;
+        gz[j] = 0.0;
+    }
+    for (int k = losortStart[c]; k < losortStart[c + 1]; ++k)
+    {
+        const int f = losort[k], o = losortOwner[k];
+        const double wf = w[f], sx = Sf[3 * (size_t)f], sy = Sf[3 * (size_t)f + 1], sz = Sf[3 * (size_t)f + 2];
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+        {
+            const double pf = wf * (a.psi[j][o] - pc[j]) + pc[j];
+            gx[j] = gx[j] - sx * pf;
+            gy[j] = gy[j] - sy * pf;
+            gz[j] = gz[j] - sz * pf;
+        }
+    }
+    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f)
+    {
+        const int nb = neighbour[f];
+        const double wf = w[f], sx = Sf[3 * (size_t)f], sy = Sf[3 * (size_t)f + 1], sz = Sf[3 * (size_t)f + 2];
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+        {
+            const double pn = a.psi[j][nb];
+            const double pf = wf * (pc[j] - pn) + pn;
+            gx[j] = gx[j] + sx * pf;
+            gy[j] = gy[j] + sy * pf;
+            gz[j] = gz[j] + sz * pf;
+        }
+    }
+    for (int k = bndStart[c]; k < bndStart[c + 1]; ++k)
+    {
+        const int i = bndFaces[k];
+        const int t = bcType[i];
+        if (t == 2) continue;
+        const size_t fb = 3 * ((size_t)nF + i);
+        const double sx = Sf[fb], sy = Sf[fb + 1], sz = Sf[fb + 2];
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+        {
+            const double pv = t == 1 ? a.bcValue[j][i] : pc[j];
+            gx[j] = gx[j] + sx * pv;
+            gy[j] = gy[j] + sy * pv;
+            gz[j] = gz[j] + sz * pv;
+        }
+    }
+    const double v = V[c];
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+    {
+        const double x = gx[j] / v, y = gy[j] / v, z = gz[j] / v;
+        a.g3[j][3 * (size_t)c] = x;
+        a.g3[j][3 * (size_t)c + 1] = y;
+        a.g3[j][3 * (size_t)c + 2] = z;
+        if (a.g9)
+        {
+            a.g9[(size_t)(0 * 3 + j) * n + c] = x;
+            a.g9[(size_t)(1 * 3 + j) * n + c] = y;
+            a.g9[(size_t)(2 * 3 + j) * n + c] = z;
+        }
+    }
+}
+
 } // namespace
+
+template <int NC>
+static void launchGrad(Ctx& c, const Mesh& m, const int* bcType, const double* Sf, const double* w, const double* V,
+                       const Grad3Args& a)
+{
+    const SlabPart part = gatherPart(m);
+    if (part.plane)
+        gradGauss3K<true, NC><<<gatherBlocks(m.nCells, part.plane), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p,
+            m.bndStart.p, m.bndFaces.p, bcType, Sf, w, V, a);
+    else
+        gradGauss3K<false, NC><<<blocks(m.nCells), kTpb, 0, c.stream>>>(
+            m.nCells, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p,
+            m.bndStart.p, m.bndFaces.p, bcType, Sf, w, V, a);
+    hipCheck(hipGetLastError(), "grad gauss");
+}
+
+void launchGradGauss3(Ctx& c, const Mesh& m, const int* bcType, const double* const* bcValue, const double* Sf,
+                      const double* w, const double* V, const double* const* psi, double* const* g3, double* g9)
+{
+    if (!m.nCells) return;
+    Grad3Args a{};
+    for (int j = 0; j < 3; ++j)
+    {
+        a.psi[j] = psi[j];
+        a.bcValue[j] = bcValue[j];
+        a.g3[j] = g3[j];
+    }
+    a.g9 = g9;
+    launchGrad<3>(c, m, bcType, Sf, w, V, a);
+}
+
+void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcValue, const double* Sf,
+                     const double* w, const double* V, const double* psi, double* grad)
+{
+    if (!m.nCells) return;
+    Grad3Args a{};
+    a.psi[0] = psi;
+    a.bcValue[0] = bcValue;
+    a.g3[0] = grad;
+    launchGrad<1>(c, m, bcType, Sf, w, V, a);
+}
 
 void launchZero(Ctx& c, int n, double* x)
 {

@@ -79,10 +79,17 @@ struct DBuf
         }
         n = count;
     }
+    // Returns once the copy is complete: callers upload from pageable host arrays they free right after,
+    // and a copy still in flight from freed host memory would read whatever the allocator puts there next
+    // (another thread's data, in the serving loop). Uploads happen at set-up, not per iteration.
     void upload(const T* h, size_t count, hipStream_t s)
     {
         resize(count);
-        if (count) hipCheck(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+        if (count)
+        {
+            hipCheck(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+            hipCheck(hipStreamSynchronize(s), "upload sync");
+        }
     }
     void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
     std::vector<T> download(hipStream_t s) const
@@ -263,6 +270,10 @@ void launchBoundaryFlux(Ctx& c, const Mesh& m, const double* SfB, const double* 
 // decision: 3 device doubles {massCorr, refused, closedVolume}
 void launchAdjustPhi(Ctx& c, int nB, const int* adjustable, double* phiB, int nF, const double* phiInt,
                      double* decision);
+// the gradients of three fields with the same patch types in one pass: each xyz per cell (g3[k]) and, when
+// g9 is given, packed component-major g9[(i*3 + k)*n + c] = d(psi_k)/dx_i
+void launchGradGauss3(Ctx& c, const Mesh& m, const int* bcType, const double* const* bcValue, const double* Sf,
+                      const double* w, const double* V, const double* const* psi, double* const* g3, double* g9);
 void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcValue, const double* Sf,
                      const double* w, const double* V, const double* psi, double* grad);
 // limitCoeff in (0, 1): `limited k corrected` against nonOrthDc*(vf[nei] - vf[own]); otherwise unlimited

@@ -297,6 +297,79 @@ __global__ __launch_bounds__(kCoarsestLanes) void coarsestPcgDicK(CoarseLdu A, c
 }
 
 // ---- the cycle ----
+// gamg.cu's scaleCorrection update (gamg.cu:207-219): sf = num/(den >= 0 ? max(den, VSMALL) : min(den, -VSMALL)),
+// x = sf*x + (b - sf*Ax)/D
+__global__ __launch_bounds__(kTpb) void scaleRefApplyK(int n, double* __restrict__ field, const double* __restrict__ Acf,
+                                                       const double* __restrict__ source,
+                                                       const double* __restrict__ diag, const double* __restrict__ st)
+{
+    const int i = blockIdx.x * kTpb + threadIdx.x;
+    if (i >= n) return;
+    const double num = st[S_NUM], den = st[S_DEN];
+    const double sf = num / (den >= 0 ? fmax(den, kVSmall) : fmin(den, -kVSmall));
+    field[i] = sf * field[i] + (source[i] - sf * Acf[i]) / diag[i];
+}
+
+// gamg.cu's denseSolve (gamg.cu:222-263) of the coarsest level, one lane: the LDU matrix scattered into a
+// dense row-major n x n (diag, then per face upper at (own, nei) and lower at (nei, own)), Gaussian elimination
+// with partial pivoting (rows swapped, the first largest |pivot| kept, zero factors skipped), back substitution
+__global__ void denseRefSolveK(int n, int nF, const int* __restrict__ own, const int* __restrict__ nei,
+                               const double* __restrict__ diag, const double* __restrict__ upper,
+                               const double* __restrict__ lower, const double* __restrict__ b, double* __restrict__ x,
+                               double* __restrict__ A)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    double* rhs = A + (size_t)n * n;
+    for (size_t i = 0; i < (size_t)n * n; ++i) A[i] = 0.0;
+    for (int i = 0; i < n; ++i)
+    {
+        rhs[i] = b[i];
+        A[(size_t)i * n + i] = diag[i];
+    }
+    for (int f = 0; f < nF; ++f)
+    {
+        A[(size_t)own[f] * n + nei[f]] += upper[f];
+        A[(size_t)nei[f] * n + own[f]] += lower[f];
+    }
+    for (int k = 0; k < n; ++k)
+    {
+        int piv = k;
+        double best = fabs(A[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[(size_t)i * n + k]) > best)
+            {
+                best = fabs(A[(size_t)i * n + k]);
+                piv = i;
+            }
+        if (piv != k)
+        {
+            for (int j = 0; j < n; ++j)
+            {
+                const double t = A[(size_t)k * n + j];
+                A[(size_t)k * n + j] = A[(size_t)piv * n + j];
+                A[(size_t)piv * n + j] = t;
+            }
+            const double t = rhs[k];
+            rhs[k] = rhs[piv];
+            rhs[piv] = t;
+        }
+        const double dkk = A[(size_t)k * n + k];
+        for (int i = k + 1; i < n; ++i)
+        {
+            const double fct = A[(size_t)i * n + k] / dkk;
+            if (fct == 0.0) continue;
+            for (int j = k; j < n; ++j) A[(size_t)i * n + j] -= fct * A[(size_t)k * n + j];
+            rhs[i] -= fct * rhs[k];
+        }
+    }
+    for (int i = n - 1; i >= 0; --i)
+    {
+        double s = rhs[i];
+        for (int j = i + 1; j < n; ++j) s -= A[(size_t)i * n + j] * x[j];
+        x[i] = s / A[(size_t)i * n + i];
+    }
+}
+
 struct Level
 {
     Mesh* m;
@@ -371,6 +444,39 @@ public:
 
     const Level& fine() const { return lv_[0]; }
 
+    // gamg.cu's cycle (gamg.cu:356-386) on psi with b, fRes the finest residual, fCorr and Apsi scratch
+    void vcycleRef(double* psi, const double* b, double* Apsi, double* fCorr, const double* fRes, int nPost,
+                   int nFinest)
+    {
+        const int coarsest = (int)lv_.size() - 1;
+        restrictTo(1, fRes);
+        for (int gi = 1; gi < coarsest; ++gi) restrictTo(gi + 1, w_.src[gi].p);
+        {
+            Level& L = lv_[coarsest];
+            const Mesh& m = *L.m;
+            denseRefSolveK<<<1, 64, 0, c_.stream>>>(L.A.nCells, L.A.nFaces, m.owner.p, m.neighbour.p, L.A.diag,
+                                                    L.A.upper, L.A.lower, w_.src[coarsest].p, w_.corr[coarsest].p,
+                                                    w_.dense.p);
+        }
+        for (int gi = coarsest - 1; gi >= 1; --gi)
+        {
+            Level& L = lv_[gi];
+            double* corr = w_.corr[gi].p;
+            double* src = w_.src[gi].p;
+            const int n = L.A.nCells;
+            prolongK<<<blocks(n), kTpb, 0, c_.stream>>>(n, a_.grids[gi].map.p, w_.corr[gi + 1].p, corr);
+            scaleRef(L, corr, w_.ACf.p, src);
+            if (nPost > 0) launchGaussSeidel(c_, *L.m, L.A, src, corr, nPost);
+        }
+        Level& F = lv_[0];
+        const int n = F.A.nCells;
+        prolongK<<<blocks(n), kTpb, 0, c_.stream>>>(n, a_.grids[0].map.p, w_.corr[1].p, fCorr);
+        scaleRef(F, fCorr, Apsi, fRes);
+        addK<<<blocks(n), kTpb, 0, c_.stream>>>(n, fCorr, psi);
+        if (nFinest > 0) launchGaussSeidel(c_, *F.m, F.A, b, psi, nFinest);
+        hipCheck(hipGetLastError(), "gamg.cu v-cycle");
+    }
+
 private:
     bool dic() const { return g_.smoother == OfSmoother::DIC || g_.smoother == OfSmoother::DICGaussSeidel; }
 
@@ -391,6 +497,16 @@ private:
         if (g_.smoother == OfSmoother::GaussSeidel || g_.smoother == OfSmoother::DICGaussSeidel)
             launchGaussSeidel(c_, *L.m, L.A, source, psi, nSweeps);
         if (g_.smoother == OfSmoother::SymGaussSeidel) launchGaussSeidel(c_, *L.m, L.A, source, psi, nSweeps, w_.bPrime.p);
+    }
+
+    // gamg.cu's scaleCorrection: the two sums each in ascending cell order (sequential mode), then the update
+    void scaleRef(Level& L, double* field, double* Acf, const double* source)
+    {
+        const int n = L.A.nCells;
+        launchAmul(c_, *L.m, L.A, field, Acf);
+        launchDot(c_, n, field, source, w_.st.p + S_NUM, 0);
+        launchDot(c_, n, field, Acf, w_.st.p + S_DEN, 1);
+        scaleRefApplyK<<<blocks(n), kTpb, 0, c_.stream>>>(n, field, Acf, source, L.A.diag, w_.st.p);
     }
 
     void scale(Level& L, double* field, double* Acf, const double* source)
@@ -527,6 +643,57 @@ PcgPerf gamgSolve(Ctx& c, Mesh& m, Amg& a, const double* diag, const double* upp
                  perf.nIterations < g.minIter);
     }
     perf.converged = converged(perf, g.tolerance, g.relTol);
+    return perf;
+}
+
+PcgPerf gamgRefSolve(Ctx& c, Mesh& m, Amg& a, const double* diag, const double* upper, const double* b, double* psi,
+                     const GamgRefControls& g)
+{
+    if (g.maxIter < 0 || g.minIter < 0) invalid("gamg.cu: negative iteration bound");
+    if (g.nPostSweeps < 0 || g.nFinestSweeps < 0) invalid("gamg.cu: negative sweep count");
+    if (a.controls.kind != AggKind::Pair || a.controls.mergeLevels != 1)
+        throw Error(kErrUnsupported, "mfv: gamg.cu: the hierarchy must be gamg.cu's agglomeratePair levels "
+                                     "(MFV_AGG_PAIR, mergeLevels 1)");
+    prepare(c, m, a);
+    OfWork& w = *a.of;
+    const int coarseN = a.grids.back().n;
+    w.dense.resize((size_t)coarseN * coarseN + coarseN);
+    const int n = m.nCells;
+    const LduView A{n, m.nFaces, diag, upper, upper};
+    amgCoarseMatrices(c, a, diag, upper);
+    GamgControls gc;
+    gc.smoother = OfSmoother::GaussSeidel;
+    Cycle cyc(c, m, a, diag, upper, gc);
+    double* st = w.st.p;
+    launchAmul(c, m, A, psi, w.Apsi.p);
+    launchNormFactor(c, m, A, psi, b, st + S_NF);
+    subK<<<blocks(n), kTpb, 0, c.stream>>>(n, b, w.Apsi.p, w.fRes.p);
+    launchSumMag(c, n, w.fRes.p, st + S_SUMMAG, 2);
+    double h[2];
+    readScalars(c, st, S_NF, 2, h);
+    const double nf = h[0];
+    PcgPerf perf;
+    perf.initialResidual = h[1] / nf;
+    perf.finalResidual = perf.initialResidual;
+    perf.nIterations = 0;
+    // gamg.cu's stopping rule: relTol only when positive
+    auto conv = [&](double fr) { return fr < g.tolerance || (g.relTol > 0.0 && fr < g.relTol * perf.initialResidual); };
+    if (g.minIter > 0 || !conv(perf.finalResidual))
+    {
+        int it = 0;
+        do
+        {
+            cyc.vcycleRef(psi, b, w.Apsi.p, w.fCorr.p, w.fRes.p, g.nPostSweeps, g.nFinestSweeps);
+            launchAmul(c, m, A, psi, w.Apsi.p);
+            subK<<<blocks(n), kTpb, 0, c.stream>>>(n, b, w.Apsi.p, w.fRes.p);
+            launchSumMag(c, n, w.fRes.p, st + S_SUMMAG, 2);
+            readScalars(c, st, S_SUMMAG, 1, h);
+            perf.finalResidual = h[0] / nf;
+            ++it;
+        } while ((it < g.maxIter && !conv(perf.finalResidual)) || it < g.minIter);
+        perf.nIterations = it;
+    }
+    perf.converged = conv(perf.finalResidual);
     return perf;
 }
 

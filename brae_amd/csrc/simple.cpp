@@ -98,9 +98,19 @@ SimpleSolver::SimpleSolver(Ctx& ctx, Mesh& mesh, const Geometry& geom, const std
     if (nF) hipCheck(hipMemcpyAsync(w.data(), g.magSf, sizeof(double) * nF, hipMemcpyDeviceToHost, c.stream), "weights");
     hipCheck(hipStreamSynchronize(c.stream), "simple setup");
     AmgBuildControls bc;
-    bc.kind = AggKind::Brae;
-    bc.target = 64;
-    bc.mergeLevels = std::max(1, mergeLevels);
+    if (ctl.pSolver == PSolver::GamgRef)
+    {
+        // the host CPU path's GAMG: gamg.cu's agglomeratePair on |Sf| down to 10 cells, single passes
+        bc.kind = AggKind::Pair;
+        bc.target = 10;
+        bc.mergeLevels = 1;
+    }
+    else
+    {
+        bc.kind = AggKind::Brae;
+        bc.target = 64;
+        bc.mergeLevels = std::max(1, mergeLevels);
+    }
     amgBuild(c, m, amg, w, bc);
 }
 
@@ -141,13 +151,12 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
     const double* Ub[3] = {f.Ub[0], f.Ub[1], f.Ub[2]};
     launchDivUpwind(c, m, f.phi, D.p, Up.p, Lo.p);
     launchMomBoundary(c, nB, uType.p, uValue.p, f.phiB, nuB.p, bMagSf, g.bDelta, ic, bcf);
-    for (int k = 0; k < 3; ++k)
     {
-        launchGradGauss(c, m, gType.p, Ub[k], g.Sf, g.w, g.V, U[k], g3[k].p);
-        launchZero(c, n, s[k]);
+        double* g3o[3] = {g3[0].p, g3[1].p, g3[2].p};
+        launchGradGauss3(c, m, gType.p, Ub, g.Sf, g.w, g.V, U, g3o, gradU.p);
     }
+    for (int k = 0; k < 3; ++k) launchZero(c, n, s[k]);
     const double* g3p[3] = {g3[0].p, g3[1].p, g3[2].p};
-    launchPackGrad(c, n, g3p, gradU.p);
     if (ctl.linearUpwind && nF)
     {
         double* cr[3] = {corr[0].p, corr[1].p, corr[2].p};
@@ -255,13 +264,26 @@ SimpleResult SimpleSolver::step(const SimpleFields& f, const double* nuEff, cons
         if (ctl.pRefCell >= 0) launchSetReference(c, pD.p, psrc.p, ctl.pRefCell, ctl.pRefValue);
         launchFold(c, m, pD.p, psrc.p, piC.p, pbC.p, diagC.p, b.p);
         psAsm.close();
-        if (corrPass == 0)
+        if (corrPass == 0 && ctl.pSolver == PSolver::AmgPcg)
         {
             ProfScope ps(c, kProfSimpleGalerkin);
             amgGalerkin(c, amg, diagC.p, pU.p, ctl.P.prec);
         }
         ProfScope psSolve(c, kProfSimplePSolve);
-        const PcgPerf pf = amgPcgSolve(c, m, amg, diagC.p, pU.p, b.p, f.p, nullptr, ctl.P);
+        PcgPerf pf;
+        if (ctl.pSolver == PSolver::GamgRef)
+        {
+            // gamg(pEqn, p, ..., nCellsInCoarsest 10, nPreSweeps 0, nPostSweeps 2, nFinestSweeps 2)
+            // (simpleFoam_cpp.cu:156-168)
+            GamgRefControls gr;
+            gr.tolerance = ctl.P.tolerance;
+            gr.relTol = ctl.P.relTol;
+            gr.maxIter = ctl.P.maxIter;
+            gr.minIter = ctl.P.minIter;
+            pf = gamgRefSolve(c, m, amg, diagC.p, pU.p, b.p, f.p, gr);
+        }
+        else
+            pf = amgPcgSolve(c, m, amg, diagC.p, pU.p, b.p, f.p, nullptr, ctl.P);
         psSolve.close();
         if (corrPass == 0) res.pInitial = pf.initialResidual;
         res.pFinal = pf.finalResidual;

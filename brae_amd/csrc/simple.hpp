@@ -36,7 +36,7 @@ struct Geometry
     const double *dc = nullptr, *ndc = nullptr, *corrVec = nullptr, *bDelta = nullptr;
 };
 
-enum class PSolver : int { AmgPcg = 0 };
+enum class PSolver : int { AmgPcg = 0, GamgRef = 1 };
 
 struct SimpleControls
 {
@@ -47,7 +47,8 @@ struct SimpleControls
     int pRefCell = -1;
     double pRefValue = 0.0;
     BicgControls U;                 // the momentum components' PBiCGStab
-    PcgControls P;                  // the pressure's AMG-PCG
+    PcgControls P;                  // the pressure's AMG-PCG (GamgRef: its tolerance..minIter)
+    PSolver pSolver = PSolver::AmgPcg;
 };
 
 struct SimpleResult

@@ -42,7 +42,8 @@ class PcgControls(ctypes.Structure):
                 ("nVcycles", I), ("scaleCorrection", I), ("captureVcycle", I)]
 
 
-AGG_BRAE, AGG_FACEAREAPAIR = 0, 1
+AGG_BRAE, AGG_FACEAREAPAIR, AGG_PAIR = 0, 1, 2
+P_AMG_PCG, P_GAMG_REF = 0, 1
 LAYOUT_AUTO, LAYOUT_ROWS = 0, 1
 CACHE_LOADED, CACHE_ABSENT, CACHE_OTHER_MESH_OR_BUILD, CACHE_UNREADABLE = range(4)
 PROF_PCG_SPMV64, PROF_PCG_UPDATE, PROF_PCG_ZR, PROF_VCYCLE, PROF_AMG_RESIDUAL0, PROF_AMG_CORR_SMOOTH0, \
@@ -109,7 +110,7 @@ class Geometry(ctypes.Structure):
 class SimpleControls(ctypes.Structure):
     _fields_ = [("nu", D), ("relaxU", D), ("relaxP", D), ("momentumPredictor", I), ("linearUpwind", I),
                 ("correctedLaplacian", I), ("nNonOrthCorr", I), ("pRefCell", I), ("pRefValue", D),
-                ("U", BicgControls), ("p", PcgControls), ("mergeLevels", I)]
+                ("U", BicgControls), ("p", PcgControls), ("mergeLevels", I), ("pSolver", I)]
 
 
 class KEpsilonCoeffs(ctypes.Structure):
@@ -211,6 +212,7 @@ _SIGS = {
     "mfv_faceareapair_weights": [I, P, P, P],
     "mfv_gamg_solve": [P, P, P, P, P, P, P, P],
     "mfv_pcg_gamg_solve": [P, P, P, P, P, P, D, D, I, I, P, P],
+    "mfv_gamg_ref_solve": [P, P, P, P, P, P, D, D, I, I, I, I, P],
     "mfv_bicg_default_controls": [P],
     "mfv_pbicgstab_solve": [P, P, P, P, P, P, P, P, P],
     "mfv_kepsilon_default_controls": [P],
@@ -282,6 +284,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise MfvError("libmfv.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        # torch ships its own HIP runtime under the same sonames; loading it first makes the library bind to that
+        # one, so a process that creates a context before touching torch still has a single runtime
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, args in _SIGS.items():
             fn = getattr(L, name)

@@ -275,6 +275,8 @@ int mfv_norm_factor(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* 
  * ------------------------------------------------------------------------------------------- */
 #define MFV_AGG_BRAE 0          /* brae's device-path pairing: forward greedy, sorted coarse faces */
 #define MFV_AGG_FACEAREAPAIR 1  /* OpenFOAM faceAreaPair: alternating direction, first-appearance faces */
+#define MFV_AGG_PAIR 2          /* the host CPU path's agglomeratePair (gamg.cu:41-172): alternating
+                                   direction, sorted coarse faces; weights |Sf|; nCellsInCoarsestLevel 10 */
 
 #define MFV_PREC_FP64 64
 #define MFV_PREC_FP32 32
@@ -448,6 +450,15 @@ int mfv_gamg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, co
 int mfv_pcg_gamg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
                        double tolerance, double relTol, int maxIter, int minIter, const mfv_gamg_controls* controls,
                        mfv_perf* perf);
+/* gamg (src/OpenFOAM/matrices/gamg.cu:267-400): the GAMG the host CPU path's SIMPLE step solves p with
+ * (simpleFoam_cpp.cu:156), on a hierarchy built with MFV_AGG_PAIR on |Sf| (mergeLevels 1): the residual
+ * restricted straight down, the coarsest level by dense Gaussian elimination with partial pivoting, then per
+ * level injection, its correction scaling and nPostSweeps Gauss-Seidel sweeps; the finest correction scaled,
+ * added and smoothed nFinestSweeps times. The folded symmetric system (A with lower == upper), psi in place.
+ * With the context's sequential reductions the solve is the host loop's to the bit. Synchronises per cycle. */
+int mfv_gamg_ref_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
+                       double tolerance, double relTol, int maxIter, int minIter, int nPostSweeps, int nFinestSweeps,
+                       mfv_perf* perf);
 
 /* ---------------------------------------------------------------------------------------------
  * PBiCGStab for an asymmetric matrix with its uncoupled boundary coefficients (the reference's
@@ -507,7 +518,11 @@ typedef struct
     mfv_bicg_controls U;                /* the momentum components' PBiCGStab */
     mfv_pcg_controls p;                 /* the pressure's AMG-PCG */
     int mergeLevels;                    /* pairwise passes per pressure AMG level */
+    int pSolver;                        /* MFV_P_AMG_PCG, or MFV_P_GAMG_REF: the host CPU path's gamg.cu GAMG
+                                           (nCellsInCoarsest 10, V(0,2,2)) to p's tolerance..minIter */
 } mfv_simple_controls;
+#define MFV_P_AMG_PCG 0
+#define MFV_P_GAMG_REF 1
 typedef struct
 {
     double *dUx, *dUy, *dUz;            /* cells */
