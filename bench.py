@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--turbulence", type=int, default=1, help="1: k-epsilon with wall functions; 0: laminar")
     ap.add_argument("--tol-u", type=float, default=1e-5, help="momentum solve tolerance")
     ap.add_argument("--reltol-u", type=float, default=0.1, help="momentum solve relative tolerance")
+    ap.add_argument("--profile-simple", action="store_true",
+                    help="only the mixed path's SIMPLE iterations (for a profiler run): no FP64 or pressure-corrector "
+                         "legs, no serving or CPU baseline; prints one short JSON line")
     ap.add_argument("--gamg-step", type=int, default=0,
                     help="1: also time the step with OpenFOAM's GAMG-PCG on the device (the CPU baseline's solver)")
     return ap.parse_args()
@@ -102,6 +105,8 @@ def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
         turb = C.channel_kepsilon(ch, torch, mfv, nu=args.nu, tol=args.tol_u, relTol=args.reltol_u,
                                   checkEvery=args.check_every, log=log)
     runs = [("mixed", prec_mixed), ("fp64", mfv.PREC_FP64)] + ([] if args.no_other else [("other", other)])
+    if args.profile_simple:
+        runs = runs[:1]
     out = {}
     for name, prec in runs:
         t0 = time.time()
@@ -157,6 +162,10 @@ def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
             ke.close()
         del f
         torch.cuda.synchronize()
+    if args.profile_simple:
+        m = out["mixed"]
+        return dict(dt_mixed=m["dt"], p_iterations=[list(r.pIters[:args.non_orth + 1]) for r in m["res"]],
+                    final_p_residual=max(r.pFinal for r in m["res"]), phases_ms=m["phases_ms"])
     m, d = out["mixed"], out["fp64"]
     line = dict(
         dt_mixed=m["dt"], dt_fp64=d["dt"],
@@ -191,7 +200,12 @@ def gpu_run(args):
     ch = C.Channel(nx, ny, nz, torch, mfv, hexmesh, ctx, log=log)
     prec_mixed = mfv.PREC_FP32 if args.precision == "fp32" else mfv.PREC_FP16
     other = mfv.PREC_FP16 if prec_mixed == mfv.PREC_FP32 else mfv.PREC_FP32
-    simple = simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other) if args.simple else None
+    simple = simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other) if args.simple or args.profile_simple else None
+    if args.profile_simple:
+        ch.mesh.close()
+        ctx.close()
+        calibration_kernels(torch, ch.nC)
+        return dict(profile_simple=simple)
     t0 = time.time()
     w_host = ch.magSfI.cpu().numpy()
     amg = mfv.Amg(ch.mesh, w_host, mfv.AGG_BRAE, 64, mergeLevels=args.merge_levels)
@@ -506,6 +520,24 @@ def serving_run(args):
                          "1-20 RHS per request, %s V-cycle AMG-PCG to 1e-6" % args.precision.upper())
 
 
+def calibration_kernels(torch, n):
+    """Kernels of known HBM traffic for reading a profiler's byte counters: a plain copy of n doubles
+    (8n read, 8n written) and a gather of n doubles through an int64 index that reverses every run of 64
+    elements (8n index + 8n source read, 8n written: every cache line of the source is used whole)."""
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    idx = torch.arange(n, device="cuda", dtype=torch.int64)
+    m = n // 64 * 64
+    idx[:m] = idx[:m].view(-1, 64).flip(1).reshape(-1)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    for _ in range(3):
+        torch.index_select(src, 0, idx, out=dst)
+    torch.cuda.synchronize()
+    del src, dst, idx
+
+
 def copy_peak(torch, n):
     """Measured device-to-device copy rate (read + write bytes / time) of n doubles: the plain copy kernel
     the HBM byte counters are calibrated against, and the box's practical bandwidth ceiling."""
@@ -656,6 +688,9 @@ def main():
         sys.exit(2)
     nx, ny, nz = (int(v) for v in args.size.split("x"))
     r = gpu_run(args)
+    if args.profile_simple:
+        print(json.dumps(dict(r["profile_simple"], size=args.size, v_cycle=args.precision)), flush=True)
+        return
     serving = serving_run(args) if args.serving > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, nx * ny * nz)
     sim = r.get("simple")

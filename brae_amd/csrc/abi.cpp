@@ -5,19 +5,25 @@
 #include "mfv_internal.hpp"
 #include "simple.hpp"
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <new>
 
+// A context and a mesh are reference counted: the caller's handle holds one reference and every object
+// built on them (a mesh on its context; a hierarchy, SIMPLE or kEpsilon workspace on its mesh and context)
+// another, so the destroy calls may come in any order and the last one tears down.
 struct mfv_ctx_s
 {
     mfv::Ctx c;
+    std::atomic<int> refs{1};
 };
 
 struct mfv_mesh_s
 {
     mfv_ctx ctx;
     mfv::Mesh m;
+    std::atomic<int> refs{1};
 };
 
 struct mfv_amg_s
@@ -248,9 +254,14 @@ int mfv_ctx_create(int device, void* hStream, mfv_ctx* out)
     return MFV_OK;
 }
 
-int mfv_ctx_destroy(mfv_ctx ctx)
+extern "C++" {
+namespace {
+
+void retainCtx(mfv_ctx ctx) { ctx->refs.fetch_add(1); }
+
+void releaseCtx(mfv_ctx ctx)
 {
-    if (!ctx) return MFV_OK;
+    if (ctx->refs.fetch_sub(1) != 1) return;
     (void)hipStreamSynchronize(ctx->c.stream);
     for (auto& v : ctx->c.prof)
         for (auto& e : v)
@@ -262,6 +273,25 @@ int mfv_ctx_destroy(mfv_ctx ctx)
     if (ctx->c.mailbox) (void)hipHostFree(ctx->c.mailbox);
     if (ctx->c.ownsStream) (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
+}
+
+void retainMesh(mfv_mesh mesh) { mesh->refs.fetch_add(1); }
+
+void releaseMesh(mfv_mesh mesh)
+{
+    if (mesh->refs.fetch_sub(1) != 1) return;
+    const mfv_ctx ctx = mesh->ctx;
+    (void)hipStreamSynchronize(ctx->c.stream);
+    delete mesh;
+    releaseCtx(ctx);
+}
+
+} // namespace
+} // extern "C++"
+
+int mfv_ctx_destroy(mfv_ctx ctx)
+{
+    if (ctx) releaseCtx(ctx);
     return MFV_OK;
 }
 
@@ -308,14 +338,14 @@ int mfv_mesh_create(mfv_ctx ctx, int nCells, int nFaces, const int* hOwner, cons
         delete mesh;
         return rc;
     }
+    retainCtx(ctx);
     *out = mesh;
     return MFV_OK;
 }
 
 int mfv_mesh_destroy(mfv_mesh mesh)
 {
-    if (mesh) (void)hipStreamSynchronize(mesh->ctx->c.stream);
-    delete mesh;
+    if (mesh) releaseMesh(mesh);
     return MFV_OK;
 }
 
@@ -510,6 +540,8 @@ int newAmg(mfv_ctx ctx, mfv_mesh mesh, mfv_amg* out, F fill)
         delete amg;
         return rc;
     }
+    retainCtx(ctx);
+    retainMesh(mesh);
     *out = amg;
     return MFV_OK;
 }
@@ -613,8 +645,13 @@ int mfv_amg_clone(mfv_ctx ctx, mfv_amg src, mfv_amg* out)
 
 int mfv_amg_destroy(mfv_amg amg)
 {
-    if (amg) (void)hipStreamSynchronize(amg->ctx->c.stream);
+    if (!amg) return MFV_OK;
+    const mfv_ctx ctx = amg->ctx;
+    const mfv_mesh mesh = amg->mesh;
+    (void)hipStreamSynchronize(ctx->c.stream);
     delete amg;
+    releaseMesh(mesh);
+    releaseCtx(ctx);
     return MFV_OK;
 }
 
@@ -1388,6 +1425,8 @@ int mfv_simple_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry, 
         std::vector<double> uv(hUValue, hUValue + 3 * (size_t)nB), pv(hPValue, hPValue + nB);
         std::unique_ptr<mfv_simple_s> h(new mfv_simple_s{ctx, mesh, nullptr});
         h->s.reset(new SimpleSolver(ctx->c, mesh->m, g, ut, uv, pt, pv, sc, controls->mergeLevels));
+        retainCtx(ctx);
+        retainMesh(mesh);
         *out = h.release();
     });
 }
@@ -1395,8 +1434,12 @@ int mfv_simple_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry, 
 int mfv_simple_destroy(mfv_simple simple)
 {
     if (!simple) return MFV_OK;
-    (void)hipStreamSynchronize(simple->ctx->c.stream);
+    const mfv_ctx ctx = simple->ctx;
+    const mfv_mesh mesh = simple->mesh;
+    (void)hipStreamSynchronize(ctx->c.stream);
     delete simple;
+    releaseMesh(mesh);
+    releaseCtx(ctx);
     return MFV_OK;
 }
 
@@ -1538,6 +1581,8 @@ int mfv_kepsilon_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry
         std::vector<double> y(hYWall, hYWall + nB), kv(hKValue, hKValue + nB), ev(hEpsValue, hEpsValue + nB);
         std::unique_ptr<mfv_kepsilon_s> h(new mfv_kepsilon_s{ctx, mesh, nullptr, {}, {}});
         h->k.reset(new mfv::KEpsilon(ctx->c, mesh->m, geometryOf(geometry), w, y, kt, kv, et, ev, kc));
+        retainCtx(ctx);
+        retainMesh(mesh);
         *out = h.release();
     });
 }
@@ -1545,8 +1590,12 @@ int mfv_kepsilon_create(mfv_ctx ctx, mfv_mesh mesh, const mfv_geometry* geometry
 int mfv_kepsilon_destroy(mfv_kepsilon ke)
 {
     if (!ke) return MFV_OK;
-    (void)hipStreamSynchronize(ke->ctx->c.stream);
+    const mfv_ctx ctx = ke->ctx;
+    const mfv_mesh mesh = ke->mesh;
+    (void)hipStreamSynchronize(ctx->c.stream);
     delete ke;
+    releaseMesh(mesh);
+    releaseCtx(ctx);
     return MFV_OK;
 }
 

@@ -57,7 +57,10 @@ __device__ __forceinline__ double waveSum(double v)
     return v;
 }
 
-// Max over a workgroup of N threads (exact, so any order); the result is valid in thread 0.
+// Max over a workgroup of N threads (exact, so any order); the result is valid in thread 0. It starts
+// from the first wave's maximum, so negative values (a min taken as -max(-v)) come out right, and ends
+// with a barrier, as blockSum does: the LDS slots are free again for the caller's next reduction only
+// once thread 0 has read them.
 template <int N = kReduceBlock>
 __device__ __forceinline__ double blockMax(double v, double* lds)
 {
@@ -69,7 +72,11 @@ __device__ __forceinline__ double blockMax(double v, double* lds)
     __syncthreads();
     double m = 0.0;
     if (threadIdx.x == 0)
-        for (int w = 0; w < N / 64; ++w) m = fmax(m, lds[w]);
+    {
+        m = lds[0];
+        for (int w = 1; w < N / 64; ++w) m = fmax(m, lds[w]);
+    }
+    __syncthreads();
     return m;
 }
 

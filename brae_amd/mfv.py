@@ -327,6 +327,21 @@ def ptr(t):
 _LIVE = weakref.WeakSet()
 
 
+def _adopt(parent, child):
+    """record that child was built on parent (a context's meshes, a mesh's hierarchies and solvers)"""
+    parent.__dict__.setdefault("_children", weakref.WeakSet()).add(child)
+
+
+def _close_children(obj):
+    """close what was built on obj before obj itself: the garbage collector finalises the objects of a
+    reference cycle in any order, and the library needs a context and a mesh to outlive what was built on
+    them"""
+    kids = obj.__dict__.get("_children")
+    if kids:
+        for k in list(kids):
+            k.close()
+
+
 @atexit.register
 def _close_all():
     """Release library objects before the HIP runtime tears down at process exit: meshes and
@@ -376,6 +391,7 @@ class Context:
 
     def close(self):
         if self.h:
+            _close_children(self)
             lib().mfv_ctx_destroy(self.h)
             self.h = P()
 
@@ -402,9 +418,11 @@ class Mesh:
         ctx._check(lib().mfv_mesh_create(ctx.h, self.nCells, self.nFaces, ptr(self.owner), ptr(self.neighbour),
                                          self.nBFaces, ptr(self.bFaceCell), ctypes.byref(self.h)), "mesh_create")
         _LIVE.add(self)
+        _adopt(ctx, self)
 
     def close(self):
         if self.h:
+            _close_children(self)
             lib().mfv_mesh_destroy(self.h)
             self.h = P()
 
@@ -448,6 +466,7 @@ class Amg:
                                 "amg_build_or_load")
                 self.how = how.value
         _LIVE.add(self)
+        _adopt(self.mesh, self)
         n = ctypes.c_int()
         lib().mfv_amg_num_grids(self.h, ctypes.byref(n))
         self.nGrids = n.value
@@ -586,6 +605,7 @@ class Simple:
                       ctypes.byref(controls), ctypes.byref(h))
         self.h = h.value
         _LIVE.add(self)
+        _adopt(self.mesh, self)
 
     def step(self, U, Ub, p, pb, phi, phiB, nuEff=None, nuEffB=None):
         f = SimpleFields(*[ptr(t) for t in list(U) + list(Ub) + [p, pb, phi, phiB]])
@@ -654,6 +674,7 @@ class KEpsilon:
                       ctypes.byref(controls), ctypes.byref(h))
         self.h = h.value
         _LIVE.add(self)
+        _adopt(self.mesh, self)
 
     @staticmethod
     def fields(U, Ub, phi, phiB, k, kb, eps, epsb, nut, nutb):

@@ -47,6 +47,8 @@ typedef struct
  * Context: one HIP stream, the reduction scratch and a pinned read-back mailbox. Replaces the
  * process-wide singletons of reductions.cu:89-111 so concurrent contexts do not share state.
  * hStream may be NULL (the context creates its own default-priority stream).
+ * Contexts and meshes are reference counted: whatever is built on them (meshes, hierarchies, SIMPLE and
+ * kEpsilon workspaces) keeps them alive, so destroy calls may come in any order; the last one frees.
  * ------------------------------------------------------------------------------------------- */
 int mfv_ctx_create(int device, void* hStream, mfv_ctx* out);
 int mfv_ctx_destroy(mfv_ctx ctx);

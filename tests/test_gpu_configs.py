@@ -115,12 +115,7 @@ def serving_stream(n, seed=2025):
     return reqs
 
 
-FP16_MIX = pytest.mark.xfail(reason="with the FP16 cycle the largest requests (3-4 M cells, 60+ PCG iterations a "
-                                     "step) end on other bits in a concurrent mix than alone; FP64 and FP32 match",
-                              strict=False)
-
-
-@pytest.mark.parametrize("precision", [32, pytest.param(16, marks=FP16_MIX)])
+@pytest.mark.parametrize("precision", [32, 16])
 def test_serving_at_stream_sizes(precision):
     reqs = serving_stream(16)
     sizes = [r["nx"] * r["ny"] * r["nz"] for r in reqs]
