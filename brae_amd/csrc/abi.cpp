@@ -227,6 +227,7 @@ int mfv_ctx_create(int device, void* hStream, mfv_ctx* out)
     const int rc = guarded(nullptr, [&] {
         hipCheck(hipSetDevice(device), "set device");
         ctx->c.device = device;
+        staticBicgScratchInit(device);   // the no-allocation solve's workspace address on this device
         if (hStream)
             ctx->c.stream = static_cast<hipStream_t>(hStream);
         else
@@ -1359,6 +1360,22 @@ int mfv_pbicgstab_solve(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const doub
         need(mesh->m.nBFaces == 0 || (dIC && dBC), "pbicgstab: boundary coefficients missing");
         const LduView v = view(mesh, A);
         const PcgPerf r = pbicgstabSolve(ctx->c, mesh->m, v, dSource, dIC, dBC, dPsi, bicgControls(*controls));
+        putPerf(r, perf);
+    });
+}
+
+static_assert(MFV_STATIC_BICG_MAX_CELLS == kStaticBicgMaxCells, "the header states the static workspace's size");
+
+int mfv_pbicgstab_solve_noalloc(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dSource,
+                                const double* dIC, const double* dBC, double* dPsi, const mfv_bicg_controls* controls,
+                                mfv_perf* perf)
+{
+    return guarded(ctx, [&] {
+        needMesh(ctx, mesh);
+        need(dSource && dPsi && controls, "pbicgstab (no allocation): null argument");
+        need(mesh->m.nBFaces == 0 || (dIC && dBC), "pbicgstab (no allocation): boundary coefficients missing");
+        const LduView v = view(mesh, A);
+        const PcgPerf r = pbicgstabSolveStatic(ctx->c, mesh->m, v, dSource, dIC, dBC, dPsi, bicgControls(*controls));
         putPerf(r, perf);
     });
 }

@@ -237,6 +237,13 @@ struct BicgControls
 // folds them (diag + iC, source + bC) as the reference's does; psi in place
 PcgPerf pbicgstabSolve(Ctx& c, Mesh& m, const LduView& A, const double* source, const double* iC, const double* bC,
                        double* psi, const BicgControls& bc);
+// The same solve with its workspace in a static device array instead of the mesh's buffers: no device
+// allocation. Up to kStaticBicgMaxCells cells; DILU needs the mesh's level schedule built beforehand. Every
+// call on one GPU shares that GPU's array, so calls that overlap on one GPU race.
+constexpr int kStaticBicgMaxCells = 1 << 19;
+void staticBicgScratchInit(int device);   // once per device, with that device current
+PcgPerf pbicgstabSolveStatic(Ctx& c, Mesh& m, const LduView& A, const double* source, const double* iC,
+                             const double* bC, double* psi, const BicgControls& bc);
 
 // ---- OpenFOAM's GAMG (src/matrices/lduMatrix/solvers/GAMG/gamg_solver_cpp.cuh:67-85, GamgControls) ----
 enum class OfSmoother : int { GaussSeidel = 0, SymGaussSeidel = 1, DIC = 2, DICGaussSeidel = 3 };

@@ -17,6 +17,8 @@
 #include "amg.hpp"
 #include "reduce.hpp"
 
+#include <mutex>
+
 namespace mfv {
 
 // The PBiCGStab's device buffers, kept on the mesh between solves (resized when the mesh's size changes).
@@ -25,6 +27,13 @@ struct BicgWork
     DBuf<double> diagC, b, rD, r, r0, p, y, Ay, s, z, t, st;
     DBuf<int> fl;
     void size(int n);
+};
+
+// The device addresses one solve works in: the mesh's BicgWork, or the static workspace below.
+struct BicgPtrs
+{
+    double *diagC, *b, *rD, *r, *r0, *p, *y, *Ay, *s, *z, *t, *st;
+    int* fl;
 };
 
 void BicgWorkDeleter::operator()(BicgWork* w) const { delete w; }
@@ -377,7 +386,7 @@ struct BicgRun
 {
     Ctx& c;
     Mesh& m;
-    BicgWork& w;
+    const BicgPtrs& w;
     LduView A;   // the folded matrix: diag is w.diagC
     int n;
     int gR;
@@ -387,48 +396,48 @@ struct BicgRun
     {
         if (c.sequentialReduce)
         {
-            dotSeqStopK<<<1, 64, 0, c.stream>>>(n, a, b, out, w.fl.p);
+            dotSeqStopK<<<1, 64, 0, c.stream>>>(n, a, b, out, w.fl);
             return;
         }
         double* pa = c.partials.p + (size_t)slot * kMaxReduceBlocks;
-        dotStopK<<<gR, kReduceBlock, 0, c.stream>>>(n, a, b, pa, w.fl.p);
-        finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, StoreTo{out}, w.fl.p + BF_STOP);
+        dotStopK<<<gR, kReduceBlock, 0, c.stream>>>(n, a, b, pa, w.fl);
+        finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, StoreTo{out}, w.fl + BF_STOP);
     }
     void dot2(const double* t, const double* s, double* ts, double* tt)
     {
         if (c.sequentialReduce)
         {
-            dotSeqStopK<<<1, 64, 0, c.stream>>>(n, t, s, ts, w.fl.p);
-            dotSeqStopK<<<1, 64, 0, c.stream>>>(n, t, t, tt, w.fl.p);
+            dotSeqStopK<<<1, 64, 0, c.stream>>>(n, t, s, ts, w.fl);
+            dotSeqStopK<<<1, 64, 0, c.stream>>>(n, t, t, tt, w.fl);
             return;
         }
         double* pa = c.partials.p + (size_t)3 * kMaxReduceBlocks;
         double* pb = c.partials.p + (size_t)4 * kMaxReduceBlocks;
-        dot2StopK<<<gR, kReduceBlock, 0, c.stream>>>(n, t, s, pa, pb, w.fl.p);
-        finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, StoreTo{ts}, w.fl.p + BF_STOP);
-        finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pb, StoreTo{tt}, w.fl.p + BF_STOP);
+        dot2StopK<<<gR, kReduceBlock, 0, c.stream>>>(n, t, s, pa, pb, w.fl);
+        finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, StoreTo{ts}, w.fl + BF_STOP);
+        finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pb, StoreTo{tt}, w.fl + BF_STOP);
     }
     void sumMag(const double* a, double* out, int slot)
     {
         if (c.sequentialReduce)
         {
-            sumMagSeqStopK<<<1, 64, 0, c.stream>>>(n, a, out, w.fl.p);
+            sumMagSeqStopK<<<1, 64, 0, c.stream>>>(n, a, out, w.fl);
             return;
         }
         double* pa = c.partials.p + (size_t)slot * kMaxReduceBlocks;
-        sumMagStopK<<<gR, kReduceBlock, 0, c.stream>>>(n, a, pa, w.fl.p);
-        finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, StoreTo{out}, w.fl.p + BF_STOP);
+        sumMagStopK<<<gR, kReduceBlock, 0, c.stream>>>(n, a, pa, w.fl);
+        finishK<<<1, kFinishBlock, 0, c.stream>>>(gR, pa, StoreTo{out}, w.fl + BF_STOP);
     }
     void amul(const double* x, double* y)
     {
         bicgAmulK<<<blocks(n), kTpb, 0, c.stream>>>(n, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p,
-                                                    m.neighbour.p, A.diag, A.upper, A.lower, x, y, w.fl.p);
+                                                    m.neighbour.p, A.diag, A.upper, A.lower, x, y, w.fl);
     }
     void precondition(BicgPrecond pre, const double* r, double* y)
     {
         if (pre == BicgPrecond::Diagonal)
         {
-            diagApplyK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.rD.p, r, y, w.fl.p);
+            diagApplyK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.rD, r, y, w.fl);
             return;
         }
         const LevelSched& s = *m.sched;
@@ -436,20 +445,20 @@ struct BicgRun
         {
             const int a = s.fwdStart[l], cnt = s.fwdStart[l + 1] - a;
             diluFwdK<<<blocks(cnt), kTpb, 0, c.stream>>>(cnt, s.fwdCells.p + a, m.losortStart.p, m.losort.p,
-                                                         m.losortOwner.p, w.rD.p, A.lower, r, y, w.fl.p);
+                                                         m.losortOwner.p, w.rD, A.lower, r, y, w.fl);
         }
         for (size_t l = 0; l + 1 < s.bwdStart.size(); ++l)
         {
             const int a = s.bwdStart[l], cnt = s.bwdStart[l + 1] - a;
             diluBwdK<<<blocks(cnt), kTpb, 0, c.stream>>>(cnt, s.bwdCells.p + a, m.ownerStart.p, m.neighbour.p,
-                                                         w.rD.p, A.upper, y, w.fl.p);
+                                                         w.rD, A.upper, y, w.fl);
         }
     }
     void factor(BicgPrecond pre)
     {
         if (pre == BicgPrecond::Diagonal)
         {
-            diagInvK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.diagC.p, w.rD.p);
+            diagInvK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.diagC, w.rD);
             return;
         }
         const LevelSched& s = *m.sched;
@@ -457,9 +466,9 @@ struct BicgRun
         {
             const int a = s.fwdStart[l], cnt = s.fwdStart[l + 1] - a;
             diluFactorK<<<blocks(cnt), kTpb, 0, c.stream>>>(cnt, s.fwdCells.p + a, m.losortStart.p, m.losort.p,
-                                                            m.losortOwner.p, w.diagC.p, A.upper, A.lower, w.rD.p);
+                                                            m.losortOwner.p, w.diagC, A.upper, A.lower, w.rD);
         }
-        invertK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.rD.p);
+        invertK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.rD);
     }
 };
 
@@ -473,52 +482,50 @@ void BicgWork::size(int n)
     fl.resize(BF_COUNT);
 }
 
-PcgPerf pbicgstabSolve(Ctx& c, Mesh& m, const LduView& A0, const double* source, const double* iC, const double* bC,
-                       double* psi, const BicgControls& bc)
+namespace {
+
+// The solve itself, in the workspace w (every vector n long; st and fl the state). Nothing here allocates:
+// the level schedule (DILU) must already be on the mesh.
+PcgPerf bicgRunSolve(Ctx& c, Mesh& m, const LduView& A0, const double* source, const double* iC, const double* bC,
+                     double* psi, const BicgControls& bc, const BicgPtrs& w)
 {
-    if (bc.maxIter < 0 || bc.minIter < 0) invalid("pbicgstab: negative iteration limits");
-    if (m.nBFaces && (!iC || !bC)) invalid("pbicgstab: boundary coefficients missing");
     const int n = m.nCells;
-    if (!m.bicg) m.bicg.reset(new BicgWork());
-    BicgWork& w = *m.bicg;
-    w.size(n);
-    if (bc.precond == BicgPrecond::DILU) buildLevelSchedule(c, m);
-    BicgRun R{c, m, w, LduView{n, m.nFaces, w.diagC.p, A0.upper, A0.lower}, n, reduceGrid(n)};
-    double* S = w.st.p;
-    int* F = w.fl.p;
+    BicgRun R{c, m, w, LduView{n, m.nFaces, w.diagC, A0.upper, A0.lower}, n, reduceGrid(n)};
+    double* S = w.st;
+    int* F = w.fl;
     bicgSetupK<<<1, 1, 0, c.stream>>>(S, F, bc.tolerance, bc.relTol, bc.maxIter, bc.minIter);
-    bicgFoldK<<<blocks(n), kTpb, 0, c.stream>>>(n, m.bndStart.p, m.bndFaces.p, A0.diag, source, iC, bC, w.diagC.p,
-                                                w.b.p);
+    bicgFoldK<<<blocks(n), kTpb, 0, c.stream>>>(n, m.bndStart.p, m.bndFaces.p, A0.diag, source, iC, bC, w.diagC,
+                                                w.b);
     R.factor(bc.precond);
     // rA = b - A psi (Amul, then the difference), rA0 = rA; normFactor with the folded diagonal
-    R.amul(psi, w.y.p);
-    bicgSubK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.b.p, w.y.p, w.r.p, w.r0.p);
-    launchNormFactor(c, m, R.A, psi, w.b.p, S + B_NF);
-    R.sumMag(w.r.p, S + B_T2, 2);
+    R.amul(psi, w.y);
+    bicgSubK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.b, w.y, w.r, w.r0);
+    launchNormFactor(c, m, R.A, psi, w.b, S + B_NF);
+    R.sumMag(w.r, S + B_T2, 2);
     bicgInitK<<<1, 1, 0, c.stream>>>(S, F);
     hipCheck(hipGetLastError(), "pbicgstab init");
     // the reference's do-while runs its body at least once when not converged on entry
     const int limit = std::max(1, std::max(bc.maxIter, bc.minIter));
     for (int it = 0; it < limit; ++it)
     {
-        R.dot(w.r0.p, w.r.p, S + B_T0, 0);
+        R.dot(w.r0, w.r, S + B_T0, 0);
         bicgRhoK<<<1, 1, 0, c.stream>>>(S, F);
-        bicgDirK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.r.p, w.Ay.p, w.p.p, S, F);
-        R.precondition(bc.precond, w.p.p, w.y.p);
-        R.amul(w.y.p, w.Ay.p);
-        R.dot(w.r0.p, w.Ay.p, S + B_T1, 1);
+        bicgDirK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.r, w.Ay, w.p, S, F);
+        R.precondition(bc.precond, w.p, w.y);
+        R.amul(w.y, w.Ay);
+        R.dot(w.r0, w.Ay, S + B_T1, 1);
         bicgAlphaK<<<1, 1, 0, c.stream>>>(S, F);
-        bicgSK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.r.p, w.Ay.p, w.s.p, S, F);
-        R.sumMag(w.s.p, S + B_T2, 2);
+        bicgSK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.r, w.Ay, w.s, S, F);
+        R.sumMag(w.s, S + B_T2, 2);
         bicgSResK<<<1, 1, 0, c.stream>>>(S, F);
-        bicgExitK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.y.p, psi, S, F);
+        bicgExitK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.y, psi, S, F);
         bicgExitCountK<<<1, 1, 0, c.stream>>>(F);
-        R.precondition(bc.precond, w.s.p, w.z.p);
-        R.amul(w.z.p, w.t.p);
-        R.dot2(w.t.p, w.s.p, S + B_T3, S + B_T4);
+        R.precondition(bc.precond, w.s, w.z);
+        R.amul(w.z, w.t);
+        R.dot2(w.t, w.s, S + B_T3, S + B_T4);
         bicgOmegaK<<<1, 1, 0, c.stream>>>(S, F);
-        bicgUpdK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.y.p, w.z.p, w.s.p, w.t.p, psi, w.r.p, S, F);
-        R.sumMag(w.r.p, S + B_T2, 2);
+        bicgUpdK<<<blocks(n), kTpb, 0, c.stream>>>(n, w.y, w.z, w.s, w.t, psi, w.r, S, F);
+        R.sumMag(w.r, S + B_T2, 2);
         bicgResK<<<1, 1, 0, c.stream>>>(S, F);
         hipCheck(hipGetLastError(), "pbicgstab iteration");
         if ((it + 1) % std::max(1, bc.checkEvery) == 0 || it + 1 == limit)
@@ -540,6 +547,81 @@ PcgPerf pbicgstabSolve(Ctx& c, Mesh& m, const LduView& A0, const double* source,
     perf.nIterations = fl[BF_ITER];
     perf.converged = fl[BF_CONV] != 0;
     return perf;
+}
+
+
+} // namespace
+
+PcgPerf pbicgstabSolve(Ctx& c, Mesh& m, const LduView& A0, const double* source, const double* iC, const double* bC,
+                       double* psi, const BicgControls& bc)
+{
+    if (bc.maxIter < 0 || bc.minIter < 0) invalid("pbicgstab: negative iteration limits");
+    if (m.nBFaces && (!iC || !bC)) invalid("pbicgstab: boundary coefficients missing");
+    const int n = m.nCells;
+    if (!m.bicg) m.bicg.reset(new BicgWork());
+    BicgWork& w = *m.bicg;
+    w.size(n);
+    if (bc.precond == BicgPrecond::DILU) buildLevelSchedule(c, m);
+    const BicgPtrs ptrs{w.diagC.p, w.b.p, w.rD.p, w.r.p, w.r0.p, w.p.p, w.y.p, w.Ay.p, w.s.p, w.z.p, w.t.p,
+                        w.st.p, w.fl.p};
+    return bicgRunSolve(c, m, A0, source, iC, bC, psi, bc, ptrs);
+}
+
+// ---- the no-allocation variant: the workspace in one static device array ----
+//
+// scratch holds the eleven work vectors of a solve of up to kStaticBicgMaxCells cells, each at a 256-byte
+// aligned offset, then the solver state (B_COUNT doubles) and its flags (BF_COUNT ints). Each GPU the
+// library runs on holds its own copy until the process ends. In host code `scratch` names a host-side
+// stand-in, so its device address on each GPU is looked up once, with that GPU current, and kept in
+// gStaticScratch; only that address is handed to kernels.
+namespace {
+constexpr size_t kStaticVec = kStaticBicgMaxCells;      // doubles per vector: a multiple of 32 (256 bytes)
+constexpr size_t kStaticState = kStaticVec * 11;        // offset of the state, then the flags 256 bytes on
+constexpr size_t kStaticDoubles = kStaticState + 64;
+static_assert(kStaticVec % 32 == 0, "vectors stay 256-byte aligned");
+static_assert(B_COUNT <= 32 && BF_COUNT * sizeof(int) <= 32 * sizeof(double), "state fits its slot");
+constexpr int kMaxDevices = 64;
+std::mutex gStaticMutex;
+double* gStaticScratch[kMaxDevices] = {};
+} // namespace
+
+static __device__ __align__(256) double scratch[kStaticDoubles];
+
+void staticBicgScratchInit(int device)
+{
+    if (device < 0 || device >= kMaxDevices) invalid("static workspace: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(gStaticMutex);
+    if (gStaticScratch[device]) return;
+    void* ptr = nullptr;
+    hipCheck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "static workspace address");
+    if (!ptr) invalid("static workspace: no device address");
+    gStaticScratch[device] = static_cast<double*>(ptr);
+}
+
+PcgPerf pbicgstabSolveStatic(Ctx& c, Mesh& m, const LduView& A0, const double* source, const double* iC,
+                             const double* bC, double* psi, const BicgControls& bc)
+{
+    if (bc.maxIter < 0 || bc.minIter < 0) invalid("pbicgstab (no allocation): negative iteration limits");
+    if (m.nBFaces && (!iC || !bC)) invalid("pbicgstab (no allocation): boundary coefficients missing");
+    const int n = m.nCells;
+    if ((size_t)n > kStaticVec)
+        invalid("pbicgstab (no allocation): " + std::to_string(n) + " cells exceed the static workspace's " +
+                std::to_string(kStaticBicgMaxCells));
+    if (bc.precond == BicgPrecond::DILU && !m.sched)
+        invalid("pbicgstab (no allocation): DILU needs the mesh's level schedule; a first mfv_pbicgstab_solve, "
+                "mfv_pcg_dic_solve or sweep on this mesh builds it (or use the diagonal preconditioner)");
+    double* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(gStaticMutex);
+        if (c.device >= 0 && c.device < kMaxDevices) base = gStaticScratch[c.device];
+    }
+    if (!base) invalid("pbicgstab (no allocation): the static workspace of this device was not set up");
+    double* v[11];
+    for (int k = 0; k < 11; ++k) v[k] = base + (size_t)k * kStaticVec;
+    double* st = base + kStaticState;
+    int* fl = reinterpret_cast<int*>(base + kStaticState + 32);
+    const BicgPtrs ptrs{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], st, fl};
+    return bicgRunSolve(c, m, A0, source, iC, bC, psi, bc, ptrs);
 }
 
 } // namespace mfv

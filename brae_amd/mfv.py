@@ -215,6 +215,7 @@ _SIGS = {
     "mfv_gamg_ref_solve": [P, P, P, P, P, P, D, D, I, I, I, I, P],
     "mfv_bicg_default_controls": [P],
     "mfv_pbicgstab_solve": [P, P, P, P, P, P, P, P, P],
+    "mfv_pbicgstab_solve_noalloc": [P, P, P, P, P, P, P, P, P],
     "mfv_kepsilon_default_controls": [P],
     "mfv_near_wall_dist": [I, I, P, P, P, P, P, P, P],
     "mfv_kepsilon_create": [P, P, P, P, P, P, P, P, P, P, P],

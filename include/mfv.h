@@ -485,6 +485,18 @@ typedef struct
 int mfv_bicg_default_controls(mfv_bicg_controls* controls);
 int mfv_pbicgstab_solve(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dSource, const double* dIC,
                         const double* dBC, double* dPsi, const mfv_bicg_controls* controls, mfv_perf* perf);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): mfv_pbicgstab_solve with its work vectors in
+ * a static device array of the library (MFV_STATIC_BICG_MAX_CELLS cells, about 44 MiB per GPU, held until
+ * the process ends) instead of the mesh's buffers; it calls no hipMalloc or hipFree. Same arguments, same
+ * kernels, same bits. Larger meshes are refused (MFV_ERR_INVALID). DILU needs the mesh's level schedule,
+ * which a first mfv_pbicgstab_solve, mfv_pcg_dic_solve or sweep on the mesh builds; the diagonal
+ * preconditioner needs nothing. Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (two contexts, two threads) race. */
+#define MFV_STATIC_BICG_MAX_CELLS 524288
+int mfv_pbicgstab_solve_noalloc(mfv_ctx ctx, mfv_mesh mesh, const mfv_ldu* A, const double* dSource,
+                                const double* dIC, const double* dBC, double* dPsi, const mfv_bicg_controls* controls,
+                                mfv_perf* perf);
 
 /* ---------------------------------------------------------------------------------------------
  * One SIMPLE iteration on the device: cpu::simpleStep (src/applications/solvers/simpleFoam/
