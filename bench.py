@@ -1,20 +1,20 @@
-"""Benchmark: the SIMPLE pressure corrector on one MI355X.
+"""Benchmark: whole SIMPLE iterations on one MI355X (the headline), with the pressure corrector alone, the
+pressure predictor's kernels, serving and the CPU baseline beside them.
 
-Workload (one step = one SIMPLE iteration's pressure corrector on the 500x250x160 graded, jittered
-channel, 20,000,000 cells, nNonOrthogonalCorrectors 1): rAUf = interpolate(rAU); assemble
--laplacian(rAUf, p) (face coefficients on nonOrthDeltaCoeffs, diagonal, fixedValue/zeroGradient boundary
-coefficients); Galerkin re-coarsening; then twice: grad(p), the explicit non-orthogonal correction flux
-and its divergence into the source, source -= div(phiHbyA)*V, fold, normFactor, AMG-preconditioned CG to
-a normalised residual of 1e-6 (from p = 0, then from the first solution); finally
-phi = phiHbyA - pEqn.flux() with the stored correction flux. The mixed-precision path runs the V-cycle
-in FP16 with per-grid power-of-two scaling (--precision fp32: in FP32) inside FP64 flexible CG; the FP64
-path runs the same solve with an FP64 cycle, and the other mixed precision is timed beside them.
-`value` is the mixed path's seconds per step (lower is better); the FP64 time, the ratio, iterations,
-residuals and the mixed/FP64 solution difference are reported beside it.
+Headline workload (one step = one SIMPLE iteration of simpleFoam with kEpsilon on the 500x250x160
+graded (ratio 20), 10 %-jittered channel 4 x 1 x 0.5, 20,000,000 cells): UEqn (upwind convection with
+linearUpwind's deferred correction, corrected laplacian(nuEff, U), divDevReff's explicit stress, relax 0.7),
+the momentum predictor (PBiCGStab per component), the pressure predictor (rAU, HbyA, phiHbyA, adjustPhi),
+nNonOrthogonalCorrectors + 1 pressure solves (corrected laplacian, AMG-PCG to 1e-6 normalised residual),
+phi, p relaxed by 0.3, U corrected, then the k-epsilon correct() with nutk/epsilon wall functions. The
+mixed path runs the pressure V-cycle in FP16 with per-grid power-of-two scaling (--precision fp32: FP32)
+inside FP64 flexible CG; the FP64 path runs the same iterations with an FP64 cycle. `value` is the mixed
+path's seconds per iteration (lower is better); the FP64 time, the ratio, iterations, residuals and the
+mixed/FP64 field differences are reported beside it.
 
-Inputs are synthetic and seeded: perturbed plug flow U = (10,0,0) + uniform(-1,1) per component
-(inlet fixedValue (10,0,0), outlet zeroGradient, walls no-slip), phiHbyA = interpolate(U).Sf,
-rAU = 0.05*V^(1/3)/|U|; p: fixedValue 0 at the outlet, zeroGradient elsewhere.
+Inputs are synthetic and seeded: perturbed plug flow U = (10,0,0) + uniform(-1,1) per component (inlet
+fixedValue (10,0,0), k 0.375, epsilon 14.855; outlet zeroGradient with p fixedValue 0; walls no-slip with
+wall functions), p = 0, phi = interpolate(U).Sf; the timed iterations start from the seeded fields.
 """
 import argparse
 import collections
@@ -53,10 +53,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--graph", type=int, default=0, help="1: replay captured groups of PCG iterations")
-    ap.add_argument("--no-scale", action="store_true", help="plain injection instead of the scaled correction")
+    ap.add_argument("--scale", action="store_true",
+                    help="scale each grid's prolonged correction by (r.e)/(e.Ae) (brae's corrScaling); off by default: "
+                         "with V(1,1) on two-pass levels the SIMPLE iteration takes the same PCG iterations without it, "
+                         "one SpMV per grid cheaper")
     # V(2,1) on levels of two composed pairwise passes: the fastest measured cycle for the FP64 path and
     # within 1 % of the fastest for the FP16 one on the 20 M channel
-    ap.add_argument("--npre", type=int, default=2)
+    ap.add_argument("--npre", type=int, default=1)
     ap.add_argument("--npost", type=int, default=1)
     ap.add_argument("--merge-levels", type=int, default=2, help="pairwise passes per AMG level")
     ap.add_argument("--non-orth", type=int, default=1, help="nNonOrthogonalCorrectors (solves per step - 1)")
@@ -95,8 +98,8 @@ def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
     def controls(prec):
         return mfv.simple_controls(
             U=U_ctl, p=dict(tolerance=args.tolerance, relTol=0.0, precision=prec, nPreSweeps=args.npre,
-                            nPostSweeps=args.npost, flexible=int(prec != mfv.PREC_FP64 or not args.no_scale),
-                            scaleCorrection=int(not args.no_scale), checkEvery=args.check_every),
+                            nPostSweeps=args.npost, flexible=int(prec != mfv.PREC_FP64 or args.scale),
+                            scaleCorrection=int(args.scale), checkEvery=args.check_every),
             mergeLevels=args.merge_levels, nu=args.nu, relaxU=0.7, relaxP=0.3, linearUpwind=1,
             correctedLaplacian=1, nNonOrthCorr=args.non_orth, pRefCell=-1)
 
@@ -218,7 +221,7 @@ def gpu_run(args):
         """one SIMPLE pressure corrector (brae_amd.channel.pressure_corrector)"""
         return C.pressure_corrector(ch, ctx, mfv, amg, prec, tolerance=args.tolerance, nonOrth=args.non_orth,
                                     checkEvery=args.check_every, nPre=args.npre, nPost=args.npost,
-                                    scale=not args.no_scale, graph=args.graph, gamg=gamg)
+                                    scale=args.scale, graph=args.graph, gamg=gamg)
 
     def timed(prec, profile):
         for _ in range(args.warmup):
@@ -608,16 +611,13 @@ def cpu_info():
     return os.cpu_count() or 0, model
 
 
-def cpu_baseline(args, n_full):
-    """The oracle's FP64 SIMPLE iteration -- the reference CPU path: PBiCGStab-DILU momentum solves,
-    gamg.cu's GAMG (agglomeratePair, GaussSeidel, dense coarsest) for the pressure, then the k-epsilon
-    correct() with PBiCGStab-DILU solves -- on a reduced channel of the same construction and boundaries,
-    single-threaded (the reference's host path has no threading), from the same kind of seeded fields.
-    One untimed iteration, then one timed; the time is scaled linearly to the full cell count."""
+def cpu_sample(sample, nu, tol_u, reltol_u, tolerance, non_orth):
+    """The oracle's FP64 turbulent SIMPLE iteration on a reduced channel: (seconds of the second of two
+    iterations, its pressure iterations, cells). Module level so a process pool can run it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from brae_amd import hexmesh
-    nx, ny, nz = (int(v) for v in args.cpu_sample.split("x"))
+    nx, ny, nz = (int(v) for v in sample.split("x"))
     m = hexmesh.box_mesh(nx, ny, nz, (4.0, 1.0, 0.5), (1.0, 20.0, 20.0), 0.1, seed=2024)
     nC, nI = m["nCells"], m["nInternal"]
     nB = len(m["owner"]) - nI
@@ -657,27 +657,55 @@ def cpu_baseline(args, n_full):
     eb = O.evaluate_boundary(mesh.bFaceCell, kT, eV, e)
     nutb = 0.09 * kb * kb / np.where(eb > 0, eb, 1.0)
     y = O.near_wall_dist(mesh, isWall)
-    kcs = O.kepsilon_case(mesh, isWall, y, kT, kV, kT, eV, nu=args.nu, relaxEps=0.7, relaxK=0.7, tol=args.tol_u,
-                          relTol=args.reltol_u, maxIter=1000)
+    kcs = O.kepsilon_case(mesh, isWall, y, kT, kV, kT, eV, nu=nu, relaxEps=0.7, relaxK=0.7, tol=tol_u,
+                          relTol=reltol_u, maxIter=1000)
     times, iters = [], []
     for it in range(2):
         t = time.perf_counter()
-        cs = O.simple_case(mesh, uT, uV.ravel(), pT, np.zeros(nB), nu=args.nu, tolU=args.tol_u, relTolU=args.reltol_u,
-                           tolP=args.tolerance, pSolver=O.P_GAMG_REF, linearUpwind=1, correctedLaplacian=1,
-                           nNonOrthCorr=args.non_orth, nuEff=args.nu + nut, nuEffB=args.nu + nutb)
+        cs = O.simple_case(mesh, uT, uV.ravel(), pT, np.zeros(nB), nu=nu, tolU=tol_u, relTolU=reltol_u,
+                           tolP=tolerance, pSolver=O.P_GAMG_REF, linearUpwind=1, correctedLaplacian=1,
+                           nNonOrthCorr=non_orth, nuEff=nu + nut, nuEffB=nu + nutb)
         r = O.simple_step(cs, st)
         O.kepsilon_correct(kcs, st.U, st.Ub, st.phi, st.phiB, k, kb, e, eb, nut, nutb)
         times.append(time.perf_counter() - t)
-        iters.append(list(r.pIters[:args.non_orth + 1]))
-    sec = times[-1]
+        iters.append(list(r.pIters[:non_orth + 1]))
+    return times[-1], iters[-1], nC
+
+
+def cpu_baseline(args, n_full):
+    """The oracle's FP64 SIMPLE iteration -- the reference CPU path: PBiCGStab-DILU momentum solves,
+    gamg.cu's GAMG (agglomeratePair, GaussSeidel, dense coarsest) for the pressure, then the k-epsilon
+    correct() with PBiCGStab-DILU solves -- on a reduced channel of the same construction and boundaries,
+    from the same kind of seeded fields; one untimed iteration, then one timed, scaled linearly to the full
+    cell count. `value` is one thread (the reference's host path has no threading; its Gauss-Seidel and
+    DILU sweeps are sequential). `all_cores` runs one sample per host core at once (the box's CPU share, at
+    most 16): the throughput a perfectly decomposed all-core run could reach, as s/iteration at full size."""
+    import concurrent.futures
+    import multiprocessing
+    a = (args.cpu_sample, args.nu, args.tol_u, args.reltol_u, args.tolerance, args.non_orth)
+    sec, iters, nC = cpu_sample(*a)
     ncpu, model = cpu_info()
-    return dict(value=round(sec * n_full / nC, 3), unit="s/SIMPLE-iter @ full size, linear in cells",
-                cores=1, kind="port", host_cpus=ncpu, cpu_model=model,
-                sample="oracle FP64 turbulent SIMPLE iteration (kEpsilon, wall functions; U PBiCGStab-DILU to %g "
-                       "relTol %g; p gamg.cu GAMG to %g; k/epsilon PBiCGStab-DILU) on the %s channel (%d cells), one "
-                       "untimed then one timed iteration: %.2f s (pressure solves %s); scaled x%.1f"
-                       % (args.tol_u, args.reltol_u, args.tolerance, args.cpu_sample, nC, sec, iters[-1],
-                          n_full / nC))
+    out = dict(value=round(sec * n_full / nC, 3), unit="s/SIMPLE-iter @ full size, linear in cells",
+               cores=1, kind="port", host_cpus=ncpu, cpu_model=model,
+               sample="oracle FP64 turbulent SIMPLE iteration (kEpsilon, wall functions; U PBiCGStab-DILU to %g "
+                      "relTol %g; p gamg.cu GAMG to %g; k/epsilon PBiCGStab-DILU) on the %s channel (%d cells), one "
+                      "untimed then one timed iteration: %.2f s (pressure solves %s); scaled x%.1f"
+                      % (args.tol_u, args.reltol_u, args.tolerance, args.cpu_sample, nC, sec, iters, n_full / nC))
+    try:
+        P = max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        P = max(1, min(16, os.cpu_count() or 1))
+    if P > 1:
+        t = time.perf_counter()
+        with concurrent.futures.ProcessPoolExecutor(P, mp_context=multiprocessing.get_context("spawn")) as ex:
+            res = list(ex.map(cpu_sample, *[[x] * P for x in a]))
+        wall = time.perf_counter() - t
+        per = max(r[0] for r in res)
+        out["all_cores"] = dict(
+            value=round(per * n_full / (P * nC), 3), unit="s/SIMPLE-iter @ full size (aggregate of %d samples)" % P,
+            cores=P, sample="%d copies of the sample at once, one per core: the slowest timed iteration %.2f s "
+                            "(wall %.1f s with set-up); %d x %d cells per iteration" % (P, per, wall, P, nC))
+    return out
 
 
 def main():
@@ -747,7 +775,7 @@ def main():
         "amg": {"grids": r["n_grids"], "coarsest_cells": r["coarsest"], "vcycle_avg_us": r["vcycle_avg_us"],
                 "mergeLevels": args.merge_levels,
                 "cycle": "V(%d,%d) weighted Jacobi%s" % (args.npre, args.npost,
-                                                        "" if args.no_scale else ", scaled correction"),
+                                                        ", scaled correction" if args.scale else ""),
                 "per_grid_us": r["grids"]},
         "openfoam_gamg_gpu": r.get("gamg_step"),
         "pressure_predictor": r.get("predictor"),

@@ -216,6 +216,10 @@ LduView gamgArgs(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, cons
 
 } // namespace
 
+namespace {
+PcgControls pcgControls(const mfv_pcg_controls& c);   // the C controls checked and converted
+}
+
 extern "C" {
 
 int mfv_ctx_create(int device, void* hStream, mfv_ctx* out)
@@ -920,18 +924,7 @@ int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A,
             throw Error(kErrUnsupported, "mfv: pcg: smoother " + std::to_string(controls->smoother) +
                                              " is not carried (0 = weighted Jacobi)");
         if (controls->nVcycles > 1) throw Error(kErrUnsupported, "mfv: pcg: nVcycles above 1 is not carried");
-        PcgControls pc;
-        pc.tolerance = controls->tolerance;
-        pc.relTol = controls->relTol;
-        pc.maxIter = controls->maxIter;
-        pc.minIter = controls->minIter;
-        pc.prec = precOf(controls->precision);
-        pc.nPre = controls->nPreSweeps;
-        pc.nPost = controls->nPostSweeps;
-        pc.flexible = controls->flexible != 0;
-        pc.scaleCorrection = controls->scaleCorrection != 0;
-        pc.checkEvery = controls->checkEvery;
-        pc.graph = controls->captureVcycle != 0;
+        const PcgControls pc = pcgControls(*controls);
         const PcgPerf r = amgPcgSolve(ctx->c, mesh->m, amg->a, v.diag, v.upper, dB, dPsi, dNormFactor, pc);
         if (perf)
         {

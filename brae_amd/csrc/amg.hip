@@ -24,6 +24,7 @@ namespace {
 
 constexpr int kTpb = 256;
 constexpr int kMaxDenseCoarsest = 1024;   // the dense inverse's column buffer in LDS
+constexpr int kAlphaInBlockParts = 1024;  // corrAmulK partials a corrSmoothK workgroup reduces itself
 inline int blocks(long long n) { return (int)((n + kTpb - 1) / kTpb); }
 
 // ---------------- V-cycle kernels ----------------
@@ -175,19 +176,36 @@ __global__ __launch_bounds__(kReduceBlock) void corrAmulK(SellView S, const Tm* 
     }
 }
 
-// second half, in place: x = x + alpha*e + omega*(r - alpha*Ae)/D (smooth) or x + alpha*e (no sweep)
+// second half, in place: x = x + alpha*e + omega*(r - alpha*Ae)/D (smooth) or x + alpha*e (no sweep).
+// With nParts > 0 every workgroup finishes alpha itself from corrAmulK's partials pa, pb (finish2K's
+// arithmetic, so the same bits) and no finishing launch runs in between; else alpha is read.
 template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void corrSmoothK(int n, const Ts* __restrict__ e, const Ts* __restrict__ r,
                                                     const Ts* __restrict__ Ae, const Tv* __restrict__ diag,
                                                     const double* __restrict__ alpha, int smooth, Ts* __restrict__ x,
                                                     const float* __restrict__ xm, Scatter<Tv, Ts> sc,
-                                                    const int* __restrict__ stop)
+                                                    const int* __restrict__ stop, int nParts,
+                                                    const double* __restrict__ pa, const double* __restrict__ pb)
 {
+    static_assert(kTpb == kReduceBlock, "alpha is finished by kReduceBlock-thread workgroups");
     if (stop && *stop) return;
+    __shared__ double lds[2 * (kFinishBlock / 64)];
+    __shared__ double shAlpha;
+    double alv;
+    if (nParts > 0)
+    {
+        double num, den;
+        finish2InBlock(nParts, pa, pb, lds, num, den);
+        if (threadIdx.x == 0) shAlpha = den > 1e-30 ? num / den : 1.0;
+        __syncthreads();
+        alv = shAlpha;
+    }
+    else
+        alv = *alpha;
     const int c = xcdBlock() * kTpb + threadIdx.x;
     if (c >= n) return;
     const CycVec<Tv, Ts> E(e, xm), R(r, nullptr), AE(Ae, nullptr), X(x, xm);
-    const Tv al = (Tv)(*alpha);
+    const Tv al = (Tv)alv;
     const Tv ev = E(c);
     const Tv xo = smooth ? X(c) + al * ev + Tv(kOmega) * (R(c) - al * AE(c)) / safeDiag(diag[c]) : X(c) + al * ev;
     X.set(c, xo);
@@ -1138,19 +1156,22 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
                                                                      seq ? a.seqA.p : nullptr,
                                                                      seq ? a.seqB.p : nullptr, stop);
         }
+        const Scatter<Tv, Ts> to = nExtra ? none : up;
+        // four rows a thread pays where nothing is scattered (grid 0); a grid that injects its values
+        // into the finer one keeps a thread per row for the scattered writes
+        bool wide = false;
+        if constexpr (std::is_same<Ts, __half>::value) wide = to.e == nullptr;
+        // a small grid's alpha is finished by every workgroup of the update itself (same bits, one launch
+        // fewer on a latency-bound grid)
+        const bool alphaInBlock = !seq && !wide && gr <= kAlphaInBlockParts;
         if (seq)
             seqFinish2K<<<1, 64, 0, c.stream>>>(n, a.seqA.p, a.seqB.p, AlphaPost{a.alphas.p + gi}, stop);
-        else
+        else if (!alphaInBlock)
             finish2K<<<1, kFinishBlock, 0, c.stream>>>(gr, p1, p2, AlphaPost{a.alphas.p + gi}, stop);
         {
             ProfScope ps(c, gi == 0 ? kProfAmgCorrSmooth0 : kProfCount);
-            const Scatter<Tv, Ts> to = nExtra ? none : up;
-            // four rows a thread pays where nothing is scattered (grid 0); a grid that injects its values
-            // into the finer one keeps a thread per row for the scattered writes
-            bool wide = false;
             if constexpr (std::is_same<Ts, __half>::value)
             {
-                wide = to.e == nullptr;
                 if (wide)
                     corrSmoothH4K<<<blocks((n + 3) / 4), kTpb, 0, c.stream>>>(n, v.e, v.r, v.y, v.diag, a.alphas.p + gi,
                                                                                vc.nPost > 0 ? 1 : 0, v.x, v.xm, to,
@@ -1158,7 +1179,8 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
             }
             if (!wide)
                 corrSmoothK<Tv, Ts><<<blocks(n), kTpb, 0, c.stream>>>(n, v.e, v.r, v.y, v.diag, a.alphas.p + gi,
-                                                                      vc.nPost > 0 ? 1 : 0, v.x, v.xm, to, stop);
+                                                                      vc.nPost > 0 ? 1 : 0, v.x, v.xm, to, stop,
+                                                                      alphaInBlock ? gr : 0, p1, p2);
         }
         for (int s = 0; s < nExtra; ++s)
         {

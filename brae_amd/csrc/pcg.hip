@@ -470,7 +470,8 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     const SellView S = viewOf(g0.sell);
     const int gR = reduceGrid(n);
     // rows per thread, measured per kernel on the 20 M channel: the FP64 direction product 4 (1 on the
-    // diagonal copy), z.r 8, the update 2 (reduceGrid)
+    // diagonal copy; 2 measured 5 % slower), z.r 8, the update 2 (reduceGrid; 4 measured 5 % slower, more than
+    // its finish saves)
     const bool dia = g0.diaM > 0;
     DiaView D{n, g0.diaPad, {}};
     for (int k = 0; k < kDiaMax; ++k) D.off[k] = g0.diaOff[k];

@@ -106,10 +106,10 @@ __device__ __forceinline__ void storePartial(double blockTotal, double* partials
 
 // One finishing thread's share of nParts partials (i = tid, tid + kFinishBlock, ...), with four loads in
 // flight: four running sums in a fixed rotation, combined in a fixed order.
-__device__ __forceinline__ double finishShare(int nParts, const double* __restrict__ pa)
+__device__ __forceinline__ double finishShare(int nParts, const double* __restrict__ pa, int tid)
 {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int i = threadIdx.x;
+    int i = tid;
     for (; i + 3 * kFinishBlock < nParts; i += 4 * kFinishBlock)
     {
         a0 += pa[i];
@@ -119,6 +119,45 @@ __device__ __forceinline__ double finishShare(int nParts, const double* __restri
     }
     for (; i < nParts; i += kFinishBlock) a0 += pa[i];
     return (a0 + a1) + (a2 + a3);
+}
+__device__ __forceinline__ double finishShare(int nParts, const double* __restrict__ pa)
+{
+    return finishShare(nParts, pa, (int)threadIdx.x);
+}
+
+// finish2K's two sums computed by one kReduceBlock-thread workgroup, with finish2K's exact arithmetic: each
+// thread plays four of the kFinishBlock finishing threads (t, t + kReduceBlock, ...), whose shares it reduces
+// over its wave by the same butterfly; thread 0 adds the sixteen wave sums in finish2K's order. Lets a
+// kernel that needs a small reduction's result (a few hundred partials) finish it itself, in every
+// workgroup, instead of a one-workgroup kernel between two launches. Valid in thread 0; all threads call.
+__device__ __forceinline__ void finish2InBlock(int nParts, const double* __restrict__ pa, const double* __restrict__ pb,
+                                               double* lds, double& sa, double& sb)
+{
+    static_assert(kFinishBlock == 4 * kReduceBlock, "four finishing threads per workgroup thread");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const int v = (int)threadIdx.x + k * kReduceBlock;
+        const double a = waveSum(finishShare(nParts, pa, v));
+        const double b = waveSum(finishShare(nParts, pb, v));
+        if (lane == 0)
+        {
+            lds[wave + 4 * k] = a;
+            lds[16 + wave + 4 * k] = b;
+        }
+    }
+    __syncthreads();
+    sa = 0.0;
+    sb = 0.0;
+    if (threadIdx.x == 0)
+    {
+#pragma unroll
+        for (int w = 0; w < kFinishBlock / 64; ++w) sa += lds[w];
+#pragma unroll
+        for (int w = 0; w < kFinishBlock / 64; ++w) sb += lds[16 + w];
+    }
+    __syncthreads();
 }
 
 // Stage two: one workgroup sums nParts partials in a fixed order and hands the total to `post`.
