@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--turbulence", type=int, default=1, help="1: k-epsilon with wall functions; 0: laminar")
     ap.add_argument("--tol-u", type=float, default=1e-5, help="momentum solve tolerance")
     ap.add_argument("--reltol-u", type=float, default=0.1, help="momentum solve relative tolerance")
+    ap.add_argument("--c5", action="store_true",
+                    help="also time the largest single-GPU case: the 800x400x375 channel (120 M cells), turbulent SIMPLE "
+                         "iterations with the FP32 and the FP64 cycle, and the peak HBM in use (adds a few minutes)")
     ap.add_argument("--profile-simple", action="store_true",
                     help="only the mixed path's SIMPLE iterations (for a profiler run): no FP64 or pressure-corrector "
                          "legs, no serving or CPU baseline; prints one short JSON line")
@@ -81,7 +84,7 @@ def parse():
     return ap.parse_args()
 
 
-def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
+def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other, on_step=None):
     from brae_amd import channel as C
     """Whole SIMPLE iterations on the channel (the headline): UEqn (upwind convection with linearUpwind's
     deferred correction, corrected laplacian(nuEff, U), the explicit stress, relax 0.7), the momentum
@@ -143,6 +146,8 @@ def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other):
         res = [iterate(f) for _ in range(args.steps)]
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
+        if on_step:
+            on_step()
         # one more iteration with events around its phases (outside the timed ones)
         ctx.profile(True)
         for tag in mfv.PROF_SIMPLE:
@@ -293,7 +298,9 @@ def gpu_run(args):
         mfv.PROF_PCG_ZR: ("pcgZrK: z.r, z.r_old", zb * N + 16 * N, iters),
         mfv.PROF_AMG_RESIDUAL0: ("residualK grid 0 (cycle precision)", vb * F + dw * N + 3 * zb * N + idx, iters),
         mfv.PROF_AMG_CORR_AMUL0: ("corrAmulK grid 0: A e, r.e, e.Ae", vb * F + dw * N + 3 * zb * N + idx, iters),
-        mfv.PROF_AMG_CORR_SMOOTH0: ("corrSmoothK grid 0", 5 * zb * N + dw * N, iters),
+        mfv.PROF_AMG_CORR_SMOOTH0: (("corrSmoothK grid 0", 5 * zb * N + dw * N, iters) if args.scale else
+                                    ("prolongSmoothK grid 0: x + e, one sweep", vb * F + dw * N + 4 * zb * N + idx,
+                                     iters)),
         mfv.PROF_AMG_RESTRICT0: ("restrictK grid 0 -> 1", zb * N + 4 * N + 4 * Nc + 2 * zb * Nc + dw * Nc, iters),
     }
     prof = res["mixed"]["prof"]
@@ -362,6 +369,47 @@ def gpu_run(args):
     amg.close()
     ch.mesh.close()
     ctx.close()
+    return out
+
+
+def c5_run(args):
+    """The largest single-GPU configuration: turbulent SIMPLE iterations on the 800x400x375 channel (120 M
+    cells) with the FP32 cycle (the configuration's mixed precision) and with the FP64 cycle, as the headline's
+    iterations (one untimed, then --steps timed, from the seeded fields), and the peak HBM in use."""
+    import copy
+    import torch
+    from brae_amd import channel as C
+    from brae_amd import hexmesh, mfv
+    a = copy.copy(args)
+    a.size, a.no_other, a.profile_simple = "800x400x375", True, False
+    a.steps = min(args.steps, 2)
+    import gc
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()   # the 20 M legs' cached blocks back to the device for the library's allocations
+    free_b, total_b = torch.cuda.mem_get_info()
+    before = (total_b - free_b) / 2 ** 30
+    stream = torch.cuda.Stream(device="cuda:0")
+    torch.cuda.set_stream(stream)
+    ctx = mfv.Context(0, stream=stream.cuda_stream)
+    t0 = time.time()
+    ch = C.Channel(800, 400, 375, torch, mfv, hexmesh, ctx, log=log, corrector=False)
+    peak = [0.0]
+
+    def used():
+        free_b, total_b = torch.cuda.mem_get_info()
+        peak[0] = max(peak[0], (total_b - free_b) / 2 ** 30)
+    used()
+    r = simple_leg(a, ch, ctx, torch, mfv, mfv.PREC_FP32, mfv.PREC_FP16, on_step=used)
+    ch.mesh.close()
+    ctx.close()
+    out = dict(workload="turbulent SIMPLE iteration (as the headline) on the 800x400x375 channel, 120,000,000 cells",
+               fp32_s_per_iter=round(r["dt_mixed"], 4), fp64_s_per_iter=round(r["dt_fp64"], 4),
+               speedup_fp32_vs_fp64=round(r["dt_fp64"] / r["dt_mixed"], 3), p_iterations=r["p_iterations"],
+               final_p_residual=r["final_p_residual"], p_rel_l2_fp32_vs_fp64=r["p_rel_l2_vs_fp64"].get("mixed"),
+               phases_ms=r["phases_ms"], peak_hbm_gib=round(peak[0] - before, 1),
+               hbm_in_use_before_gib=round(before, 1), wall_s=round(time.time() - t0, 1))
+    log("c5: FP32 %.3f s/iter, FP64 %.3f s/iter, peak HBM %.1f GiB" % (r["dt_mixed"], r["dt_fp64"], peak[0] - before))
     return out
 
 
@@ -580,7 +628,7 @@ def triad_peak(torch, n):
     return 24.0 * n / (ms * 1e-3) / 1e9
 
 
-TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_pressure_corrector_20M.md")
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_pcg_iteration_20M.md")
 
 
 def measured_traffic(label):
@@ -719,6 +767,7 @@ def main():
     if args.profile_simple:
         print(json.dumps(dict(r["profile_simple"], size=args.size, v_cycle=args.precision)), flush=True)
         return
+    c5 = c5_run(args) if args.c5 else None
     serving = serving_run(args) if args.serving > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, nx * ny * nz)
     sim = r.get("simple")
@@ -784,6 +833,7 @@ def main():
         "per_kernel": r["per_kernel"],
         "cpu_baseline": cpu,
         "serving": serving,
+        "c5": c5,
     }
     print(json.dumps(line), flush=True)
 

@@ -3,8 +3,9 @@ grading ratio 20 on y and z, interior points jittered by +-10 % of the local spa
 U = (10, 0, 0), outlet (x-max) zeroGradient with p = 0, the four other faces walls. Built by the bench and
 the GPU tests at the configurations' sizes (20 M cells: 500 x 250 x 160; 2.1 M: 256 x 128 x 64).
 
-Channel           the mesh, its geometry (mfv_mesh_geometry) and synthetic pressure-corrector inputs (seeded
-                  perturbed plug flow U = (10, 0, 0) + uniform(-1, 1): phiHbyA = interpolate(U).Sf, rAU)
+Channel           the mesh, its geometry (mfv_mesh_geometry) and, unless built for SIMPLE alone, synthetic
+                  pressure-corrector inputs (seeded perturbed plug flow U = (10, 0, 0) + uniform(-1, 1):
+                  phiHbyA = interpolate(U).Sf, rAU) with their work arrays
 pressure_corrector one SIMPLE pressure corrector on those inputs: nNonOrthogonalCorrectors + 1 assemblies and
                   AMG-PCG solves, then the flux
 channel_bcs       U and p boundary conditions per boundary face
@@ -18,9 +19,12 @@ import numpy as np
 
 
 class Channel:
-    """The graded, jittered channel and its synthetic pressure-corrector inputs on the device."""
+    """The graded, jittered channel and its synthetic pressure-corrector inputs on the device. With
+    corrector=False only the mesh and its geometry are built (what a SIMPLE run reads): at 120 M cells that
+    leaves out about 24 GiB of inputs and work arrays. Torch's cached blocks from the set-up's temporaries
+    are handed back to the device at the end, so the library's own allocations can use them."""
 
-    def __init__(self, nx, ny, nz, torch, mfv, hexmesh, ctx, log=None):
+    def __init__(self, nx, ny, nz, torch, mfv, hexmesh, ctx, log=None, corrector=True):
         log = log or (lambda *a: None)
         t0 = time.time()
         self.shape = (nx, ny, nz)
@@ -51,6 +55,10 @@ class Channel:
         ctx.sync()
         del dpts, dfo, dfv
         log("geometry on device (%.1fs)" % (time.time() - t0))
+        self.corrector = corrector
+        if not corrector:
+            torch.cuda.empty_cache()
+            return
         # synthetic seeded fields
         g = torch.Generator(device="cuda:0")
         g.manual_seed(7)
@@ -86,6 +94,7 @@ class Channel:
         self.p, self.phi = z(self.nC), z(self.nI)
         self.gradP, self.ffc = z(3 * self.nC), z(self.nI)
         torch.cuda.synchronize()
+        torch.cuda.empty_cache()
         log("fields ready (%.1fs)" % (time.time() - t0))
 
 
@@ -129,7 +138,9 @@ def simple_fields(ch, torch, uT, uV, seed=7):
     fields = dict(U=[U[:, k].contiguous() for k in range(3)], Ub=[Ub[:, k].contiguous() for k in range(3)],
                   p=torch.zeros(nC, dtype=torch.float64, device="cuda:0"),
                   pb=torch.zeros(nB, dtype=torch.float64, device="cuda:0"), phi=phi, phiB=phiB)
+    del U, Ub, own_d, nei_d, bcell
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     return fields
 
 

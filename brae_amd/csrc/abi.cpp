@@ -675,6 +675,16 @@ int mfv_amg_grid_size(mfv_amg amg, int grid, int* nCells, int* nFaces)
     return MFV_OK;
 }
 
+int mfv_amg_coarse_mode(mfv_amg amg, int mode, int* tailStart, int* ldsGrid, int* ldsBytes)
+{
+    if (!amg || mode < -1 || mode > 2) return MFV_ERR_INVALID;
+    if (mode >= 0) amg->a.coarseMode = mode;
+    if (tailStart) *tailStart = amg->a.tailStart;
+    if (ldsGrid) *ldsGrid = amg->a.tailLdsBytes ? amg->a.tailLdsGrid : 0;
+    if (ldsBytes) *ldsBytes = amg->a.tailLdsBytes;
+    return MFV_OK;
+}
+
 int mfv_amg_fine_stencil(mfv_amg amg, int* nOffsets, int* hOffsets)
 {
     if (!amg || amg->a.grids.empty() || !nOffsets) return MFV_ERR_INVALID;
@@ -1668,7 +1678,10 @@ int mfv_simple_step_kepsilon(mfv_simple simple, const mfv_simple_fields* f, mfv_
         mfv::KEpsilonResult kr;
         {
             mfv::ProfScope ps(c, mfv::kProfSimpleTurbulence);
-            kr = ke->k->correct(U, Ub, f->dPhi, f->dPhiB, kf->dK, kf->dKb, kf->dEps, kf->dEpsb, kf->dNut, kf->dNutb);
+            // the step's packed grad(U) is dead once its momentum equation is assembled: the model's
+            // gradient of the corrected U goes there
+            kr = ke->k->correct(U, Ub, f->dPhi, f->dPhiB, kf->dK, kf->dKb, kf->dEps, kf->dEpsb, kf->dNut, kf->dNutb,
+                                simple->s->gradU.p);
         }
         putKeResult(kr, keResult);
     });

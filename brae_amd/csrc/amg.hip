@@ -13,6 +13,8 @@
 #include "sell_device.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <cstring>
 #include <cmath>
 #include <map>
 #include <numeric>
@@ -338,6 +340,28 @@ __global__ __launch_bounds__(kTpb) void startK(int n, const double* __restrict__
 }
 
 // coarsest grid: x = Ainv * b in FP64, ascending column per row (one workgroup)
+// sum_j row[j] * v[j] in ascending j (the dense coarsest apply's row), eight loads in flight ahead of
+// the adds
+__device__ __forceinline__ double denseRowDot(const double* row, const double* v, int n)
+{
+    double s = 0.0;
+    int j = 0;
+    for (; j + 8 <= n; j += 8)
+    {
+        double a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            a[u] = row[j + u];
+            b[u] = v[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += a[u] * b[u];
+    }
+    for (; j < n; ++j) s += row[j] * v[j];
+    return s;
+}
+
 template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __restrict__ inv, const Ts* __restrict__ b,
                                                      Ts* __restrict__ x, const float* __restrict__ xm,
@@ -350,9 +374,7 @@ __global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __rest
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x)
     {
-        double s = 0.0;
-        const double* row = inv + (size_t)i * n;
-        for (int j = 0; j < n; ++j) s += row[j] * sb[j];
+        const double s = denseRowDot(inv + (size_t)i * n, sb, n);
         X.set(i, (Tv)s);
         sc(i, (Tv)s);
     }
@@ -385,12 +407,8 @@ __global__ __launch_bounds__(kFinishBlock) void xmulFinishK(int nParts, const do
                                                             const double* __restrict__ pmin, float* __restrict__ out)
 {
     __shared__ double lds[kFinishBlock / 64];
-    double mx = 0.0, mn = 1e308;
-    for (int i = threadIdx.x; i < nParts; i += kFinishBlock)
-    {
-        mx = fmax(mx, pmax[i]);
-        mn = fmin(mn, pmin[i]);
-    }
+    double mx = finishMax(nParts, pmax, 0.0);
+    double mn = finishMax<true>(nParts, pmin, 1e308);
     mx = blockMax<kFinishBlock>(mx, lds);
     mn = -blockMax<kFinishBlock>(-mn, lds);
     if (threadIdx.x) return;
@@ -503,9 +521,7 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
         const Scatter<Tv, Ts> sc = upOf(T - 1);
         for (int i = tid; i < G.n; i += kTailThreads)
         {
-            double s = 0.0;
-            const double* row = inv + (size_t)i * G.n;
-            for (int j = 0; j < G.n; ++j) s += row[j] * sb[j];
+            const double s = denseRowDot(inv + (size_t)i * G.n, sb, G.n);
             Xg.set(i, (Tv)s);
             sc(i, (Tv)s);
         }
@@ -579,6 +595,329 @@ __global__ __launch_bounds__(kTailThreads) void tailK(const TailGrid* __restrict
                 __syncthreads();
                 flip ^= 1u << g;
             }
+    }
+}
+
+// ---------------- the small grids resident in LDS ----------------
+//
+// tailK's cycle with the coarsest grids' arrays staged into the workgroup's LDS when the launch starts.
+// A tail step is a chain of dependent loads (slice start, columns and coefficients, the gathered
+// vector); from HBM that chain costs microseconds per step, from LDS a fraction of one. The tail's
+// grids from TG on (the longest run of coarsest grids that fits the LDS a workgroup may hold) are
+// staged: slice starts, 16-bit columns and 16-bit restriction lists from a blob built once per
+// hierarchy and precision (the same byte offsets in the blob and in LDS), the coefficients and
+// diagonals copied from the grids' own arrays at every launch (a Galerkin may have changed them), the
+// vectors held in LDS only. The grids before TG are read where they are, as tailK reads them. Values
+// cross into and out of the tail where tailK's do: the first grid's right-hand side and start iterate
+// from its buffers, the injection into the finer grid's e through `up`. Each row's sum is rowAx's (entry
+// order, padded slots included) and each step tailK's, so the result is the same bits. The scaled
+// correction keeps tailK.
+
+struct LdsTailGrid
+{
+    int n, nnz;
+    int oSlice, oCols, oStart, oList;    // static arrays (blob and LDS byte offsets)
+    int oVal, oDiag, oB, oX, oY, oR, oE; // staged and LDS-only arrays (LDS byte offsets)
+};
+
+// one contiguous run of 4-byte words copied from global memory into LDS when the launch starts
+struct StageSeg
+{
+    const void* src;
+    int dst;     // LDS byte offset
+    int first;   // its first word in the launch's copy plan
+    int words;
+};
+constexpr int kMaxStageSegs = 2 + 2 * kMaxTail;   // the blob, the inverse, each staged grid's values and diagonal
+
+struct LdsTail
+{
+    int T, TG;         // the tail's grids; the first TG are read where they are
+    int oSb;           // the coarsest right-hand side in FP64
+    int oInv;          // the coarsest inverse (n*n FP64) staged, or -1: read where it is
+    int bytes;         // dynamic LDS the launch needs
+    int nSegs, words;  // the copy plan: segments and their total words
+    StageSeg seg[kMaxStageSegs];
+    LdsTailGrid g[kMaxTail];   // [t]: grid tailStart + t, for t >= TG
+};
+constexpr int kStageBatch = 16;   // words a thread loads before it stores any
+
+// one tail grid as the kernel sees it: vectors, diagonal and coefficients where they are (global or
+// LDS), its rows either as sliced rows in global memory or as the staged 16-bit form
+struct MixGrid
+{
+    int n, staged;
+    SellView S;
+    const void* val;
+    const void* diag;
+    const int* sp;
+    const unsigned short* cols;
+    void *b, *x, *y, *r, *e;
+    const int* cs32;
+    const int* cl32;
+    const unsigned short* cs16;
+    const unsigned short* cl16;
+    double scale;
+    float m, im;
+    __device__ int start(int k) const { return cs16 ? (int)cs16[k] : cs32[k]; }
+    __device__ int list(int k) const { return cl16 ? (int)cl16[k] : cl32[k]; }
+};
+
+// A x at row c of a staged grid: rowAx's sum on the 16-bit columns
+template <typename Tm, typename Tv, typename XF>
+__device__ __forceinline__ Tv stagedRowAx(const MixGrid& G, int c, XF X, Tv* dOut)
+{
+    const Tm* val = static_cast<const Tm*>(G.val);
+    const Tv xc = X(c);
+    const Tv dc = static_cast<const Tv*>(G.diag)[c];
+    *dOut = dc;
+    const int s = c >> 6, lane = c & 63;
+    const int p0 = G.sp[s], w = (G.sp[s + 1] - p0) >> 6;
+    Tv acc = sizeof(Tm) == 2 ? Tv(0) : dc * xc;
+    // eight entries' loads issued before their gathers, the sum in entry order (a coarse grid's rows
+    // are tens of entries wide: one LDS round trip per entry would serialise the step)
+    int k = 0;
+    for (; k + 8 <= w; k += 8)
+    {
+        Tv a[8], xv[8];
+        int col[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const int e = p0 + ((k + u) << 6) + lane;
+            a[u] = cvt<Tv>(val[e]);
+            col[u] = G.cols[e];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = X(col[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += a[u] * xv[u];
+    }
+    for (; k < w; ++k)
+    {
+        const int e = p0 + (k << 6) + lane;
+        acc += cvt<Tv>(val[e]) * X((int)G.cols[e]);
+    }
+    if constexpr (sizeof(Tm) == 2)
+        return dc * xc + (Tv)G.scale * acc;
+    else
+        return acc;
+}
+
+template <typename Tm, typename Tv, typename XF>
+__device__ __forceinline__ Tv mixRowAx(const MixGrid& G, int c, XF X, Tv* dOut)
+{
+    if (G.staged) return stagedRowAx<Tm, Tv>(G, c, X, dOut);
+    return rowAx<Tm, Tv>(G.S, c, static_cast<const Tm*>(G.val), static_cast<const Tv*>(G.diag), &G.scale, X, dOut);
+}
+
+template <typename Tm, typename Tv, typename Ts>
+__global__ __launch_bounds__(kTailThreads) void tailMixK(const TailGrid* __restrict__ Dg,
+                                                        const LdsTail* __restrict__ Lg,
+                                                        const double* __restrict__ inv, int nPre, int nPost,
+                                                        Scatter<Tv, Ts> up, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    extern __shared__ __align__(16) uint8_t L[];
+    __shared__ MixGrid M[kMaxTail];
+    __shared__ StageSeg sSeg[kMaxStageSegs];
+    __shared__ int sInv, sSb, sSegs, sWords;
+    const int tid = threadIdx.x;
+    // the descriptors and the copy plan into LDS (one round of loads), then the copy itself with
+    // kStageBatch loads in flight per thread: the staging costs a few memory round trips, not one per
+    // array and chunk
+    const int T = Lg->T, TG = Lg->TG;
+    if (tid == 0)
+    {
+        sInv = Lg->oInv;
+        sSb = Lg->oSb;
+        sSegs = Lg->nSegs;
+        sWords = Lg->words;
+    }
+    if (tid < Lg->nSegs) sSeg[tid] = Lg->seg[tid];
+    if (tid < T)
+    {
+        const TailGrid& D = Dg[tid];
+        MixGrid q;
+        q.n = D.n;
+        q.staged = tid >= TG;
+        q.S = D.S;
+        q.scale = *D.scale;
+        q.m = D.xm ? *D.xm : 1.0f;
+        q.im = D.xm ? 1.0f / *D.xm : 1.0f;
+        if (!q.staged)
+        {
+            q.val = D.val;
+            q.diag = D.diag;
+            q.sp = nullptr;
+            q.cols = nullptr;
+            q.b = D.b; q.x = D.x; q.y = D.y; q.r = D.r; q.e = D.e;
+            q.cs32 = D.cellStart;
+            q.cl32 = D.cellList;
+            q.cs16 = nullptr;
+            q.cl16 = nullptr;
+        }
+        else
+        {
+            const LdsTailGrid& G = Lg->g[tid];
+            q.val = L + G.oVal;
+            q.diag = L + G.oDiag;
+            q.sp = reinterpret_cast<const int*>(L + G.oSlice);
+            q.cols = reinterpret_cast<const unsigned short*>(L + G.oCols);
+            q.b = L + G.oB; q.x = L + G.oX; q.y = L + G.oY; q.r = L + G.oR; q.e = L + G.oE;
+            q.cs32 = nullptr;
+            q.cl32 = nullptr;
+            q.cs16 = reinterpret_cast<const unsigned short*>(L + G.oStart);
+            q.cl16 = reinterpret_cast<const unsigned short*>(L + G.oList);
+        }
+        M[tid] = q;
+    }
+    __syncthreads();
+    {
+        const int nSegs = sSegs, words = sWords;
+        for (int base = tid; base < words; base += kStageBatch * kTailThreads)
+        {
+            unsigned v[kStageBatch];
+            int at[kStageBatch];
+            int sg = 0;
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u)
+            {
+                const int w = base + u * kTailThreads;
+                at[u] = -1;
+                if (w < words)
+                {
+                    while (sg + 1 < nSegs && w >= sSeg[sg + 1].first) ++sg;
+                    const int k = w - sSeg[sg].first;
+                    v[u] = static_cast<const unsigned*>(sSeg[sg].src)[k];
+                    at[u] = sSeg[sg].dst + 4 * k;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u)
+                if (at[u] >= 0) *reinterpret_cast<unsigned*>(L + at[u]) = v[u];
+        }
+    }
+    // the first grid's b and x when it is staged itself (the finer grid's restriction left them)
+    if (TG == 0)
+        for (int i = tid; i < M[0].n; i += kTailThreads)
+        {
+            const Ts bv = static_cast<const Ts*>(Dg[0].b)[i], xv = static_cast<const Ts*>(Dg[0].x)[i];
+            static_cast<Ts*>(M[0].b)[i] = bv;
+            static_cast<Ts*>(M[0].x)[i] = xv;
+        }
+    __syncthreads();
+    const int nL = M[T - 1].n;
+    const double* A = sInv >= 0 ? reinterpret_cast<const double*>(L + sInv) : inv;
+    double* sb = reinterpret_cast<double*>(L + sSb);
+    auto vec = [&](int g, void* p) { return CycVec<Tv, Ts>(static_cast<Ts*>(p), M[g].m, M[g].im); };
+    auto rhs = [&](void* p) { return CycVec<Tv, Ts>(static_cast<Ts*>(p), nullptr); };
+    unsigned flip = 0;
+    auto X = [&](int g) { return (flip >> g) & 1u ? M[g].y : M[g].x; };
+    auto Y = [&](int g) { return (flip >> g) & 1u ? M[g].x : M[g].y; };
+    // grid g's finished value v at row c: into the finer grid's e, or out through `up` for grid 0
+    auto inject = [&](int g, int c, Tv v) {
+        if (g == 0)
+        {
+            up(c, v);
+            return;
+        }
+        const MixGrid& F = M[g - 1];
+        const CycVec<Tv, Ts> E = vec(g - 1, F.e);
+        for (int k = F.start(c); k < F.start(c + 1); ++k) E.set(F.list(k), v);
+    };
+    // one Jacobi sweep of grid g: X -> Y, injecting the result when `last`
+    auto sweep = [&](int g, bool last) {
+        const MixGrid& G = M[g];
+        const CycVec<Tv, Ts> Xg = vec(g, X(g)), Yg = vec(g, Y(g)), B = rhs(G.b);
+        for (int c = tid; c < G.n; c += kTailThreads)
+        {
+            Tv dc;
+            const Tv ax = mixRowAx<Tm, Tv>(G, c, Xg, &dc);
+            const Tv xo = Xg(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(dc);
+            Yg.set(c, xo);
+            if (last) inject(g, c, xo);
+        }
+        __syncthreads();
+        flip ^= 1u << g;
+    };
+    // ---- down ----
+    for (int g = 0; g + 1 < T; ++g)
+    {
+        const MixGrid& G = M[g];
+        for (int s = 1; s < nPre; ++s) sweep(g, false);
+        const CycVec<Tv, Ts> Xg = vec(g, X(g)), B = rhs(G.b), R = rhs(G.r);
+        if (nPre == 0)
+        {
+            for (int c = tid; c < G.n; c += kTailThreads) Xg.set(c, Tv(0));
+            __syncthreads();
+        }
+        for (int c = tid; c < G.n; c += kTailThreads)
+        {
+            Tv dc;
+            R.set(c, B(c) - mixRowAx<Tm, Tv>(G, c, Xg, &dc));
+        }
+        __syncthreads();
+        const MixGrid& C = M[g + 1];
+        const Tv* dC = static_cast<const Tv*>(C.diag);
+        const CycVec<Tv, Ts> BC = rhs(C.b), XC = vec(g + 1, X(g + 1));
+        const int fromZero = (g + 2 < T && nPre > 0) ? 1 : 0;
+        for (int k = tid; k < C.n; k += kTailThreads)
+        {
+            Tv s = Tv(0);
+            for (int q = G.start(k); q < G.start(k + 1); ++q) s += R(G.list(q));
+            BC.set(k, s);
+            if (fromZero) XC.set(k, Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(dC[k]));
+        }
+        __syncthreads();
+    }
+    // ---- coarsest: x = Ainv b ----
+    {
+        const MixGrid& G = M[T - 1];
+        const CycVec<Tv, Ts> B = rhs(G.b), Xg = vec(T - 1, X(T - 1));
+        for (int j = tid; j < G.n; j += kTailThreads) sb[j] = (double)B(j);
+        __syncthreads();
+        for (int i = tid; i < G.n; i += kTailThreads)
+        {
+            const double s = denseRowDot(A + (size_t)i * G.n, sb, G.n);
+            Xg.set(i, (Tv)s);
+            inject(T - 1, i, (Tv)s);
+        }
+        __syncthreads();
+    }
+    // ---- up ----
+    const int nExtra = nPost > 1 ? nPost - 1 : 0;
+    for (int g = T - 2; g >= 0; --g)
+    {
+        const MixGrid& G = M[g];
+        const CycVec<Tv, Ts> E = vec(g, G.e);
+        if (nPost > 0)
+        {
+            const CycVec<Tv, Ts> B = rhs(G.b), Yg = vec(g, Y(g));
+            const SumX<Tv, Ts> XE{vec(g, X(g)), E};
+            for (int c = tid; c < G.n; c += kTailThreads)
+            {
+                Tv dc;
+                const Tv ax = mixRowAx<Tm, Tv>(G, c, XE, &dc);
+                const Tv xo = XE(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(dc);
+                Yg.set(c, xo);
+                if (!nExtra) inject(g, c, xo);
+            }
+            __syncthreads();
+            flip ^= 1u << g;
+            for (int s = 0; s < nExtra; ++s) sweep(g, s + 1 == nExtra);
+        }
+        else
+        {
+            const CycVec<Tv, Ts> Xg = vec(g, X(g));
+            for (int c = tid; c < G.n; c += kTailThreads)
+            {
+                const Tv xo = Xg(c) + E(c);
+                Xg.set(c, xo);
+                inject(g, c, xo);
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -694,8 +1033,7 @@ __global__ __launch_bounds__(kFinishBlock) void maxPow2FinishK(int nParts, const
                                                                 double* __restrict__ out)
 {
     __shared__ double lds[kFinishBlock / 64];
-    double mm = 0.0;
-    for (int i = threadIdx.x; i < nParts; i += kFinishBlock) mm = fmax(mm, partials[i]);
+    double mm = finishMax(nParts, partials, 0.0);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mm = fmax(mm, __shfl_xor(mm, o, 64));
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mm;
@@ -711,9 +1049,7 @@ __global__ __launch_bounds__(kFinishBlock) void sigFinishK(int nParts, const dou
                                                            double* __restrict__ sig)
 {
     __shared__ double lds[kFinishBlock / 64];
-    double mm = 0.0;
-    for (int i = threadIdx.x; i < nParts; i += kFinishBlock) mm = fmax(mm, partials[i]);
-    mm = blockMax<kFinishBlock>(mm, lds);
+    double mm = blockMax<kFinishBlock>(finishMax(nParts, partials, 0.0), lds);
     if (threadIdx.x) return;
     const double s = mm > 0.0 ? exp2(ceil(log2(mm))) : 1.0;
     sig[0] = s;
@@ -1095,12 +1431,35 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
         coarseApplyK<Tv, Ts><<<1, kTpb, 0, c.stream>>>(n, a.coarseInv.p, v.b, v.x, v.xm, up, stop);
         return v.x;
     }
-    if (gi > 0 && gi == a.tailStart && !c.sequentialReduce)
+    if (gi > 0 && gi == a.tailStart && !c.sequentialReduce && a.coarseMode > 0)
     {
         ProfScope ps(c, kProfGridDown + gi);
-        tailK<Tm, Tv, Ts><<<1, kTailThreads, 0, c.stream>>>(reinterpret_cast<const TailGrid*>(a.tailDesc.p),
-                                                             last - gi + 1, a.coarseInv.p, vc.nPre, vc.nPost,
-                                                             vc.scale ? 1 : 0, up, stop);
+        bool lds = a.coarseMode == 2 && a.tailLdsBytes > 0 && !vc.scale;
+        // dynamic LDS beyond the default limit is opted into once per instantiation, to the largest tail
+        // seen; a device that refuses keeps tailK for this hierarchy
+        static std::atomic<int> optedIn{0};
+        if (lds && a.tailLdsBytes > optedIn.load())
+        {
+            if (hipFuncSetAttribute(reinterpret_cast<const void*>(&tailMixK<Tm, Tv, Ts>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, a.tailLdsBytes) == hipSuccess)
+                optedIn.store(a.tailLdsBytes);
+            else
+            {
+                (void)hipGetLastError();
+                a.tailLdsBytes = 0;
+                lds = false;
+            }
+        }
+        if (lds)
+        {
+            tailMixK<Tm, Tv, Ts><<<1, kTailThreads, a.tailLdsBytes, c.stream>>>(
+                reinterpret_cast<const TailGrid*>(a.tailDesc.p), reinterpret_cast<const LdsTail*>(a.tailLdsDesc.p),
+                a.coarseInv.p, vc.nPre, vc.nPost, up, stop);
+        }
+        else
+            tailK<Tm, Tv, Ts><<<1, kTailThreads, 0, c.stream>>>(reinterpret_cast<const TailGrid*>(a.tailDesc.p),
+                                                                 last - gi + 1, a.coarseInv.p, vc.nPre, vc.nPost,
+                                                                 vc.scale ? 1 : 0, up, stop);
         hipCheck(hipGetLastError(), "vcycle tail");
         return nullptr;   // grid gi's values went up through `up`; no caller reads them here
     }
@@ -1321,6 +1680,163 @@ namespace {
 // The tail's per-grid descriptors for the current precision (pointers into the grids' buffers, which
 // stay put until the next allocation): grids from the first one of at most kTailCells cells (never
 // grid 0) to the coarsest.
+template <typename T>
+std::vector<T> download(Ctx& c, const T* p, size_t n)
+{
+    std::vector<T> h(n);
+    if (n) hipCheck(hipMemcpyAsync(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, c.stream), "tail download");
+    hipCheck(hipStreamSynchronize(c.stream), "tail download");
+    return h;
+}
+
+// The LDS-staged part of the tail (tailMixK) for the current precision: the longest run of coarsest
+// grids whose arrays fit the LDS a workgroup may hold beside the kernel's own, its static blob (slice
+// starts, 16-bit columns -- a patterned grid's written out -- and 16-bit restriction lists) and its LDS
+// layout. tailLdsBytes stays 0 when not even the two coarsest grids fit.
+void buildTailLds(Ctx& c, Amg& a)
+{
+    a.tailLdsBytes = 0;
+    a.tailLdsGrid = 0;
+    const int nG = (int)a.grids.size();
+    const int T = a.tailStart ? nG - a.tailStart : 0;
+    if (T < 2) return;
+    const bool f64 = a.prec == Prec::F64, f32 = a.prec == Prec::F32;
+    const size_t sm = f64 ? 8 : f32 ? 4 : 2;   // coefficient
+    const size_t sv = f64 ? 8 : 4;             // arithmetic (diagonal)
+    const size_t ss = f64 ? 8 : f32 ? 4 : 2;   // stored vector
+    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    auto u16 = [](int v) { return (unsigned short)(v >= 0 && v <= 65535 ? v : 0); };
+    int maxShared = 0;
+    hipCheck(hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device), "lds size");
+    const size_t own = sizeof(MixGrid) * kMaxTail + sizeof(StageSeg) * kMaxStageSegs + 256;   // the kernel's static LDS
+    if ((size_t)maxShared <= own) return;
+    const size_t budget = (size_t)maxShared - own;
+    // per grid: the static arrays and the staged ones
+    struct Host
+    {
+        std::vector<int> sp;
+        std::vector<unsigned short> cols, cs, cl;
+        size_t nnz = 0;
+    };
+    std::vector<Host> h(T);
+    std::vector<size_t> need(T);
+    for (int t = 0; t < T; ++t)
+    {
+        Grid& G = a.grids[a.tailStart + t];
+        const Sell& S = G.sell;
+        const bool coarsest = t + 1 == T;   // no sliced rows: its dense inverse is applied
+        if (G.n > 65535) return;
+        Host& q = h[t];
+        q.nnz = coarsest ? 0 : (size_t)S.nnz;
+        q.sp = coarsest ? std::vector<int>(1, 0) : download(c, S.slicePtr.p, (size_t)S.nSlices + 1);
+        q.cols.assign(q.nnz, 0);
+        if (!coarsest && !S.patterned)
+        {
+            const std::vector<int> cl = download(c, S.cols.p, q.nnz);
+            for (size_t e = 0; e < cl.size(); ++e) q.cols[e] = u16(cl[e]);
+        }
+        else if (!coarsest)
+        {
+            const std::vector<uint8_t> pat = download(c, S.pattern.p, (size_t)G.n);
+            const std::vector<int> po = download(c, S.patOff.p, S.patOff.n);
+            for (int r = 0; r < G.n; ++r)
+            {
+                const int sl = r >> 6, lane = r & 63, w = (q.sp[sl + 1] - q.sp[sl]) >> 6;
+                for (int k = 0; k < w; ++k)
+                    q.cols[(size_t)q.sp[sl] + ((size_t)k << 6) + lane] = u16(r + po[(size_t)pat[r] * S.width + k]);
+            }
+        }
+        if (!coarsest)
+        {
+            const int nc = a.grids[a.tailStart + t + 1].n;
+            for (int v : download(c, G.cellStart.p, (size_t)nc + 1)) q.cs.push_back(u16(v));
+            for (int v : download(c, G.cellList.p, (size_t)G.n)) q.cl.push_back(u16(v));
+        }
+        need[t] = al(4 * q.sp.size()) + al(2 * q.cols.size()) + al(2 * q.cs.size()) + al(2 * q.cl.size()) +
+                  al(q.nnz * sm) + al((size_t)G.n * sv) + 5 * al((size_t)G.n * ss);
+    }
+    const size_t nL = (size_t)a.grids.back().n;
+    size_t total = al(nL * sizeof(double));   // the coarsest right-hand side in FP64
+    int TG = T;
+    while (TG > 0 && total + need[TG - 1] <= budget)
+    {
+        total += need[TG - 1];
+        --TG;
+    }
+    if (TG > T - 2) return;   // fewer than the two coarsest grids fit
+    LdsTail d{};
+    d.T = T;
+    d.TG = TG;
+    std::vector<uint8_t> blob;
+    auto put = [&](const void* src, size_t bytes) {
+        const size_t o = blob.size();
+        blob.resize(al(o + bytes), 0);
+        if (bytes) std::memcpy(blob.data() + o, src, bytes);
+        return (int)o;
+    };
+    for (int t = TG; t < T; ++t)
+    {
+        LdsTailGrid& q = d.g[t];
+        q.n = a.grids[a.tailStart + t].n;
+        q.nnz = (int)h[t].nnz;
+        q.oSlice = put(h[t].sp.data(), sizeof(int) * h[t].sp.size());
+        q.oCols = put(h[t].cols.data(), 2 * h[t].cols.size());
+        q.oStart = put(h[t].cs.data(), 2 * h[t].cs.size());
+        q.oList = put(h[t].cl.data(), 2 * h[t].cl.size());
+    }
+    const size_t staticBytes = blob.size();
+    size_t o = blob.size();
+    auto take = [&](size_t bytes) {
+        const size_t r = o;
+        o = al(o + bytes);
+        return (int)r;
+    };
+    for (int t = TG; t < T; ++t)
+    {
+        LdsTailGrid& q = d.g[t];
+        q.oVal = take((size_t)q.nnz * sm);
+        q.oDiag = take((size_t)q.n * sv);
+        q.oB = take((size_t)q.n * ss);
+        q.oX = take((size_t)q.n * ss);
+        q.oY = take((size_t)q.n * ss);
+        q.oR = take((size_t)q.n * ss);
+        q.oE = take((size_t)q.n * ss);
+    }
+    d.oSb = take(nL * sizeof(double));
+    d.oInv = -1;
+    if (o + nL * nL * sizeof(double) <= budget) d.oInv = take(nL * nL * sizeof(double));
+    if (o > budget) return;
+    d.bytes = (int)o;
+    a.tailLdsBlob.upload(blob, c.stream);
+    // the copy plan: the blob, each staged grid's coefficients and diagonal, the inverse
+    int words = 0;
+    auto seg = [&](const void* src, int dst, size_t bytes) {
+        if (!bytes) return;
+        StageSeg& q = d.seg[d.nSegs++];
+        q.src = src;
+        q.dst = dst;
+        q.first = words;
+        q.words = (int)(bytes / 4);
+        words += q.words;
+    };
+    seg(a.tailLdsBlob.p, 0, staticBytes);
+    for (int t = TG; t < T; ++t)
+    {
+        const LdsTailGrid& q = d.g[t];
+        Grid& G = a.grids[a.tailStart + t];
+        if (q.nnz)
+            seg(f64 ? (const void*)G.off64.p : f32 ? (const void*)G.off32.p : (const void*)G.off16.p, q.oVal,
+                (size_t)q.nnz * sm);
+        seg(f64 ? (const void*)G.diagV64.p : (const void*)G.diagV32.p, q.oDiag, (size_t)q.n * sv);
+    }
+    if (d.oInv >= 0) seg(a.coarseInv.p, d.oInv, nL * nL * sizeof(double));
+    d.words = words;
+    a.tailLdsDesc.upload(reinterpret_cast<const uint8_t*>(&d), sizeof(LdsTail), c.stream);
+    hipCheck(hipStreamSynchronize(c.stream), "lds tail");
+    a.tailLdsBytes = d.bytes;
+    a.tailLdsGrid = a.tailStart + TG;
+}
+
 void buildTail(Ctx& c, Amg& a)
 {
     if (a.tailFor == (int)a.prec) return;
@@ -1371,6 +1887,7 @@ void buildTail(Ctx& c, Amg& a)
     }
     a.tailDesc.upload(reinterpret_cast<const uint8_t*>(d.data()), d.size() * sizeof(TailGrid), c.stream);
     hipCheck(hipStreamSynchronize(c.stream), "tail descriptors");
+    buildTailLds(c, a);
 }
 
 } // namespace

@@ -145,6 +145,12 @@ struct Amg
     int tailStart = 0;             // first grid the single-workgroup tail kernel runs (0: none)
     DBuf<uint8_t> tailDesc;        // per tail grid: pointers and sizes for the tail kernel
     int tailFor = -1;              // precision the descriptors were built for (-1: stale)
+    int coarseMode = 2;            // 0: per-grid kernels; 1: the one-workgroup tail; 2: that tail with its
+                                   // coarsest grids staged in LDS, as many as fit (else 1)
+    DBuf<uint8_t> tailLdsBlob;     // the LDS-staged tail's slice starts, 16-bit columns and lists
+    DBuf<uint8_t> tailLdsDesc;     // its descriptor
+    int tailLdsBytes = 0;          // LDS it needs (0: the tail does not fit; tailK runs)
+    int tailLdsGrid = 0;           // the first grid staged in LDS (the tail's grids before it are read in place)
     bool galerkinDone = false;
     bool allocatedVectors = false;
     Prec vectorsFor = Prec::F32;

@@ -357,8 +357,6 @@ KEpsilon::KEpsilon(Ctx& ctx, Mesh& mesh, const Geometry& geom, const std::vector
     for (DBuf<double>* b : {&Up, &Lo, &Df, &lU}) b->resize(F);
     for (DBuf<double>* b : {&iC, &bC, &nutB0}) b->resize(B);
     wallCell.resize(N);
-    gradU.resize(9 * N);
-    for (int k = 0; k < 3; ++k) g3[k].resize(3 * N);
     // the cells the wall functions constrain: those with a wall face
     std::vector<char> wc(n, 0);
     for (int i = 0; i < nB; ++i)
@@ -439,7 +437,7 @@ PcgPerf KEpsilon::solveScalar(bool isEps, const double* phi, const double* phiB,
 
 KEpsilonResult KEpsilon::correct(const double* const* U, const double* const* Ub, const double* phi,
                                  const double* phiB, double* k, double* kb, double* eps, double* epsb, double* nut,
-                                 double* nutb)
+                                 double* nutb, double* gradScratch)
 {
     const int n = m.nCells, nB = m.nBFaces;
     const KEpsilonCoeffs& co = ctl.co;
@@ -447,14 +445,21 @@ KEpsilonResult KEpsilon::correct(const double* const* U, const double* const* Ub
     res.nWallCells = nWallCells;
     // nut's boundary as it enters: the equations' patch diffusivities
     if (nB) launchCopy(c, nB, nutb, nutB0.p);
-    // production from the current nut, grad(U) with every non-empty face at U's patch value
+    // production from the current nut, grad(U) with every non-empty face at U's patch value, into the
+    // caller's dead 9 n doubles when it lends them, else into the model's own (allocated on first use)
+    double* gU = gradScratch;
+    if (!gU)
     {
-        double* g3o[3] = {g3[0].p, g3[1].p, g3[2].p};
-        launchGradGauss3(c, m, gType.p, Ub, g.Sf, g.w, g.V, U, g3o, gradU.p);
+        gradU.resize(9 * (size_t)std::max(n, 1));
+        gU = gradU.p;
+    }
+    {
+        double* none[3] = {nullptr, nullptr, nullptr};   // only the packed gradient is read here
+        launchGradGauss3(c, m, gType.p, Ub, g.Sf, g.w, g.V, U, none, gU);
     }
     if (n)
         keProductionK<<<blocks(n), kTpb, 0, c.stream>>>(n, m.ownerStart.p, m.losortStart.p, m.losort.p,
-                                                        m.bndStart.p, m.bndFaces.p, isEmpty.p, gradU.p, nut, phi,
+                                                        m.bndStart.p, m.bndFaces.p, isEmpty.p, gU, nut, phi,
                                                         phiB, g.V, gbn.p, G.p, divU.p);
     WallArgs wa{std::pow(co.Cmu, 0.25), std::pow(co.Cmu, 0.75), co.kappa, co.E, yPlusLamHost(co.kappa, co.E),
                 ctl.nu};

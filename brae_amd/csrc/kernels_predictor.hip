@@ -17,54 +17,7 @@ constexpr int kTpb = 256;
 static_assert(kTpb == kReduceBlock, "slabRow numbers threads of kReduceBlock-wide workgroups");
 inline int blocks(int n) { return (n + kTpb - 1) / kTpb; }
 
-// The cell of a per-cell gather kernel's thread, or -1. Slab: the plane-slab partition of the matrix's
-// band (a cell's neighbour one band back -- lower[], psi[] of its losort run -- was read by the same XCD
-// one slab earlier and is still in that XCD's L2); else the XCD-contiguous order.
-// With Split > 1 each XCD's slab is cut into that many column parts, and the XCD walks each part through
-// every plane before the next: the reuse distance of a band-back neighbour shrinks to one part, at the
-// price of the cells next to a part boundary, whose neighbours across it were read a whole walk earlier.
-// Measured at 20 M cells: two parts speed up linearUpwindCorrK (1762 -> 1662 us) and slow the other
-// gathers by 1-5 %, four parts are slower for all of them.
-template <bool Slab, int Split = 1>
-__device__ __forceinline__ int gatherCell(int n, const SlabPart& part)
-{
-    if (!Slab)
-    {
-        const int c = xcdBlock() * kTpb + threadIdx.x;
-        return c < n ? c : -1;
-    }
-    const int k = blockIdx.x & 7;
-    const int s0 = (int)((long long)k * part.plane >> 3), s1 = (int)((long long)(k + 1) * part.plane >> 3);
-    const int slab = s1 - s0, sub = (slab + Split - 1) / Split;
-    const int nPl = (n + part.plane - 1) / part.plane;
-    const long long p = (long long)(blockIdx.x >> 3) * kTpb + threadIdx.x;
-    const long long per = (long long)nPl * sub;
-    const int col = (int)(p / per);
-    if (col >= Split) return -1;
-    const long long q = p - col * per;
-    const int z = (int)(q / sub), x = col * sub + (int)(q - (long long)z * sub);
-    if (x >= slab) return -1;
-    const long long row = (long long)z * part.plane + s0 + x;
-    return row < n ? (int)row : -1;
-}
-
-// workgroups of the gather partition
-inline int gatherBlocks(int n, int plane, int split = 1)
-{
-    const long long nPl = ((long long)n + plane - 1) / plane;
-    const long long slabMax = (plane + 7) / 8, sub = (slabMax + split - 1) / split;
-    return (int)(8 * ((split * nPl * sub + kTpb - 1) / kTpb));
-}
-
 constexpr int kLinUpSplit = 2;   // linearUpwindCorrK's column parts
-
-// the partition of a mesh's cells for gather kernels: slabs of the band when the cells span several
-// bands of at least kSlabMinPlane cells, one row per thread
-inline SlabPart gatherPart(const Mesh& m)
-{
-    if (m.band >= kSlabMinPlane && m.band < m.nCells) return SlabPart{m.nCells, m.band, 1};
-    return SlabPart{m.nCells, 0, 1};
-}
 
 // D = diag + cmptAv(internalCoeffs) face by face (fv_matrix_ops.cuh:211-226; pEqn.cu:117-138, the
 // cmptAv and foldBoundaryDiag kernels), cmptAv = (x + y + z)/3
@@ -878,9 +831,12 @@ __global__ __launch_bounds__(kTpb) void gradGauss3K(int n, SlabPart part, int nF
     for (int j = 0; j < NC; ++j)
     {
         const double x = gx[j] / v, y = gy[j] / v, z = gz[j] / v;
-        a.g3[j][3 * (size_t)c] = x;
-        a.g3[j][3 * (size_t)c + 1] = y;
-        a.g3[j][3 * (size_t)c + 2] = z;
+        if (a.g3[j])
+        {
+            a.g3[j][3 * (size_t)c] = x;
+            a.g3[j][3 * (size_t)c + 1] = y;
+            a.g3[j][3 * (size_t)c + 2] = z;
+        }
         if (a.g9)
         {
             a.g9[(size_t)(0 * 3 + j) * n + c] = x;

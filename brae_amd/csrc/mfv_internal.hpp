@@ -270,8 +270,8 @@ void launchBoundaryFlux(Ctx& c, const Mesh& m, const double* SfB, const double* 
 // decision: 3 device doubles {massCorr, refused, closedVolume}
 void launchAdjustPhi(Ctx& c, int nB, const int* adjustable, double* phiB, int nF, const double* phiInt,
                      double* decision);
-// the gradients of three fields with the same patch types in one pass: each xyz per cell (g3[k]) and, when
-// g9 is given, packed component-major g9[(i*3 + k)*n + c] = d(psi_k)/dx_i
+// the gradients of three fields with the same patch types in one pass: each xyz per cell (g3[k], where not
+// null) and, when g9 is given, packed component-major g9[(i*3 + k)*n + c] = d(psi_k)/dx_i
 void launchGradGauss3(Ctx& c, const Mesh& m, const int* bcType, const double* const* bcValue, const double* Sf,
                       const double* w, const double* V, const double* const* psi, double* const* g3, double* g9);
 void launchGradGauss(Ctx& c, const Mesh& m, const int* bcType, const double* bcValue, const double* Sf,

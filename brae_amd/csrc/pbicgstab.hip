@@ -87,8 +87,16 @@ __global__ __launch_bounds__(kTpb) void bicgFoldK(int n, const label* __restrict
     b[c] = s;
 }
 
-// y = A x over a cell's faces in face order (lower-side ascending, then owned ascending), unless stopped
-__global__ __launch_bounds__(kTpb) void bicgAmulK(int n, const label* __restrict__ ownerStart,
+// y = A x over a cell's faces in face order (the losort run ascending, then the owned run ascending), unless
+// stopped. The first kAmulBatch faces of both runs are loaded together, each level of the gather (face,
+// coefficient and neighbour, then x) at once: clamped indices keep every load in bounds and unconditional,
+// and a face beyond its run is dropped from the sum, not added as zero; later faces follow in a loop. With
+// Slab the cells follow the plane-slab partition of the mesh's band, so a band-back neighbour's x and lower
+// coefficient were read by the same XCD one slab earlier and are still in its L2.
+constexpr int kAmulBatch = 3;
+
+template <bool Slab>
+__global__ __launch_bounds__(kTpb) void bicgAmulK(int n, SlabPart part, int nF, const label* __restrict__ ownerStart,
                                                   const label* __restrict__ losortStart,
                                                   const label* __restrict__ losort,
                                                   const label* __restrict__ losortOwner,
@@ -98,11 +106,41 @@ __global__ __launch_bounds__(kTpb) void bicgAmulK(int n, const label* __restrict
                                                   const int* __restrict__ fl)
 {
     if (fl[BF_STOP]) return;
-    const int c = xcdBlock() * kTpb + threadIdx.x;
-    if (c >= n) return;
+    const int c = gatherCell<Slab>(n, part);
+    if (c < 0) return;
+    const int l0 = losortStart[c], l1 = losortStart[c + 1];
+    const int f0 = ownerStart[c], f1 = ownerStart[c + 1];
     double s = diag[c] * x[c];
-    for (int k = losortStart[c]; k < losortStart[c + 1]; ++k) s += lower[losort[k]] * x[losortOwner[k]];
-    for (int f = ownerStart[c]; f < ownerStart[c + 1]; ++f) s += upper[f] * x[neighbour[f]];
+    double lv[kAmulBatch] = {}, lx[kAmulBatch] = {}, uv[kAmulBatch] = {}, ux[kAmulBatch] = {};
+    if (nF > 0)   // uniform: a mesh with internal faces
+    {
+        const int last = nF - 1;
+        int lf[kAmulBatch], lo[kAmulBatch], nb[kAmulBatch];
+#pragma unroll
+        for (int u = 0; u < kAmulBatch; ++u)
+        {
+            const int j = min(l0 + u, last), f = min(f0 + u, last);
+            lf[u] = losort[j];
+            lo[u] = losortOwner[j];
+            nb[u] = neighbour[f];
+            uv[u] = upper[f];
+        }
+#pragma unroll
+        for (int u = 0; u < kAmulBatch; ++u)
+        {
+            lv[u] = lower[lf[u]];
+            lx[u] = x[lo[u]];
+            ux[u] = x[nb[u]];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kAmulBatch; ++u)
+        if (l0 + u < l1) s += lv[u] * lx[u];
+    for (int k = l0 + kAmulBatch; k < l1; ++k) s += lower[losort[k]] * x[losortOwner[k]];
+#pragma unroll
+    for (int u = 0; u < kAmulBatch; ++u)
+        if (f0 + u < f1) s += uv[u] * ux[u];
+    for (int f = f0 + kAmulBatch; f < f1; ++f) s += upper[f] * x[neighbour[f]];
     y[c] = s;
 }
 
@@ -430,8 +468,15 @@ struct BicgRun
     }
     void amul(const double* x, double* y)
     {
-        bicgAmulK<<<blocks(n), kTpb, 0, c.stream>>>(n, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p,
-                                                    m.neighbour.p, A.diag, A.upper, A.lower, x, y, w.fl);
+        const SlabPart part = gatherPart(m);
+        if (part.plane)
+            bicgAmulK<true><<<gatherBlocks(n, part.plane), kTpb, 0, c.stream>>>(
+                n, part, m.nFaces, m.ownerStart.p, m.losortStart.p, m.losort.p, m.losortOwner.p, m.neighbour.p,
+                A.diag, A.upper, A.lower, x, y, w.fl);
+        else
+            bicgAmulK<false><<<blocks(n), kTpb, 0, c.stream>>>(n, part, m.nFaces, m.ownerStart.p, m.losortStart.p,
+                                                               m.losort.p, m.losortOwner.p, m.neighbour.p, A.diag,
+                                                               A.upper, A.lower, x, y, w.fl);
     }
     void precondition(BicgPrecond pre, const double* r, double* y)
     {

@@ -419,10 +419,10 @@ __global__ __launch_bounds__(kFinishBlock) void updFinishK(int nParts, const dou
 {
     if (stop && *stop) return;
     __shared__ double lds[kFinishBlock / 64];
-    const double s = blockSum<kFinishBlock>(finishShare(nParts, pa), lds);
-    double m = 0.0;
-    for (int i = threadIdx.x; i < nParts; i += kFinishBlock) m = fmax(m, pMax[i]);
-    m = blockMax<kFinishBlock>(m, lds);
+    double h, mp;
+    finishShareMax(nParts, pa, pMax, h, mp);
+    const double s = blockSum<kFinishBlock>(h, lds);
+    const double m = blockMax<kFinishBlock>(mp, lds);
     if (threadIdx.x == 0)
     {
         post(s);

@@ -104,12 +104,47 @@ __device__ __forceinline__ void storePartial(double blockTotal, double* partials
     if (threadIdx.x == 0) partials[blockIdx.x] = blockTotal;
 }
 
-// One finishing thread's share of nParts partials (i = tid, tid + kFinishBlock, ...), with four loads in
-// flight: four running sums in a fixed rotation, combined in a fixed order.
-__device__ __forceinline__ double finishShare(int nParts, const double* __restrict__ pa, int tid)
+// One finishing thread's share of nParts partials (i = tid, tid + kFinishBlock, ...): four running sums in
+// a fixed rotation, combined in a fixed order. With Batch 32 a large partition (the 20 M-row kernels leave
+// tens of thousands of partials) is read 32 and then 16 loads at a time, all issued before the first add,
+// so the finish waits for a few memory round trips instead of one per four partials; the adds keep the
+// rotation, so every Batch gives the same bits. Batch 4 holds no array of loads (for kernels that inline
+// a small finish and must keep their registers).
+template <int Batch>
+__device__ __forceinline__ double finishShareB(int nParts, const double* __restrict__ pa, int tid)
 {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int i = tid;
+    if constexpr (Batch >= 32)
+        for (; i + 31 * kFinishBlock < nParts; i += 32 * kFinishBlock)
+        {
+            double v[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) v[u] = pa[i + u * kFinishBlock];
+#pragma unroll
+            for (int u = 0; u < 32; u += 4)
+            {
+                a0 += v[u];
+                a1 += v[u + 1];
+                a2 += v[u + 2];
+                a3 += v[u + 3];
+            }
+        }
+    if constexpr (Batch >= 16)
+        for (; i + 15 * kFinishBlock < nParts; i += 16 * kFinishBlock)
+        {
+            double v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v[u] = pa[i + u * kFinishBlock];
+#pragma unroll
+            for (int u = 0; u < 16; u += 4)
+            {
+                a0 += v[u];
+                a1 += v[u + 1];
+                a2 += v[u + 2];
+                a3 += v[u + 3];
+            }
+        }
     for (; i + 3 * kFinishBlock < nParts; i += 4 * kFinishBlock)
     {
         a0 += pa[i];
@@ -120,9 +155,76 @@ __device__ __forceinline__ double finishShare(int nParts, const double* __restri
     for (; i < nParts; i += kFinishBlock) a0 += pa[i];
     return (a0 + a1) + (a2 + a3);
 }
+__device__ __forceinline__ double finishShare(int nParts, const double* __restrict__ pa, int tid)
+{
+    return finishShareB<4>(nParts, pa, tid);
+}
+// the share of a kFinishBlock-thread finishing kernel
 __device__ __forceinline__ double finishShare(int nParts, const double* __restrict__ pa)
 {
-    return finishShare(nParts, pa, (int)threadIdx.x);
+    return finishShareB<32>(nParts, pa, (int)threadIdx.x);
+}
+
+// finishShare(nParts, pa) and finishMax(nParts, pm, 0) together, the two arrays' loads issued in the same
+// rounds (sixteen of each in flight); the sum keeps finishShare's rotation, so it is the same bits.
+__device__ __forceinline__ void finishShareMax(int nParts, const double* __restrict__ pa,
+                                               const double* __restrict__ pm, double& share, double& mx)
+{
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, m = 0.0;
+    int i = threadIdx.x;
+    for (; i + 15 * kFinishBlock < nParts; i += 16 * kFinishBlock)
+    {
+        double v[16], w[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+        {
+            v[u] = pa[i + u * kFinishBlock];
+            w[u] = pm[i + u * kFinishBlock];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u += 4)
+        {
+            a0 += v[u];
+            a1 += v[u + 1];
+            a2 += v[u + 2];
+            a3 += v[u + 3];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) m = fmax(m, w[u]);
+    }
+    for (; i + 3 * kFinishBlock < nParts; i += 4 * kFinishBlock)
+    {
+        a0 += pa[i];
+        a1 += pa[i + kFinishBlock];
+        a2 += pa[i + 2 * kFinishBlock];
+        a3 += pa[i + 3 * kFinishBlock];
+        m = fmax(fmax(fmax(fmax(m, pm[i]), pm[i + kFinishBlock]), pm[i + 2 * kFinishBlock]), pm[i + 3 * kFinishBlock]);
+    }
+    for (; i < nParts; i += kFinishBlock)
+    {
+        a0 += pa[i];
+        m = fmax(m, pm[i]);
+    }
+    share = (a0 + a1) + (a2 + a3);
+    mx = m;
+}
+
+// One finishing thread's maximum (Min: minimum) of nParts partials from m (i = threadIdx.x, + kFinishBlock,
+// ...), eight loads in flight per round (a maximum is exact in any order).
+template <bool Min = false>
+__device__ __forceinline__ double finishMax(int nParts, const double* __restrict__ p, double m)
+{
+    int i = threadIdx.x;
+    for (; i + 7 * kFinishBlock < nParts; i += 8 * kFinishBlock)
+    {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[i + u * kFinishBlock];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m = Min ? fmin(m, v[u]) : fmax(m, v[u]);
+    }
+    for (; i < nParts; i += kFinishBlock) m = Min ? fmin(m, p[i]) : fmax(m, p[i]);
+    return m;
 }
 
 // finish2K's two sums computed by one kReduceBlock-thread workgroup, with finish2K's exact arithmetic: each
@@ -178,8 +280,10 @@ __global__ __launch_bounds__(kFinishBlock) void finish2K(int nParts, const doubl
 {
     if (stop && *stop) return;
     __shared__ double lds[kFinishBlock / 64];
-    const double sa = blockSum<kFinishBlock>(finishShare(nParts, pa), lds);
-    const double sb = blockSum<kFinishBlock>(finishShare(nParts, pb), lds);
+    // both shares' loads issue before the first barrier
+    const double ha = finishShare(nParts, pa), hb = finishShare(nParts, pb);
+    const double sa = blockSum<kFinishBlock>(ha, lds);
+    const double sb = blockSum<kFinishBlock>(hb, lds);
     if (threadIdx.x == 0) post(sa, sb);
 }
 
@@ -256,6 +360,53 @@ __device__ __forceinline__ int slabRow(const SlabPart& m, int r)
     const int z = p / slab;
     const long long row = (long long)z * m.plane + s0 + (p - z * slab);
     return row < m.n ? (int)row : -1;
+}
+
+// The cell of a per-cell gather kernel's thread, or -1. Slab: the plane-slab partition of the matrix's
+// band (a cell's neighbour one band back -- lower[], psi[] of its losort run -- was read by the same XCD
+// one slab earlier and is still in that XCD's L2); else the XCD-contiguous order.
+// With Split > 1 each XCD's slab is cut into that many column parts, and the XCD walks each part through
+// every plane before the next: the reuse distance of a band-back neighbour shrinks to one part, at the
+// price of the cells next to a part boundary, whose neighbours across it were read a whole walk earlier.
+// Measured at 20 M cells: two parts speed up linearUpwindCorrK (1762 -> 1662 us) and slow the other
+// gathers by 1-5 %, four parts are slower for all of them.
+template <bool Slab, int Split = 1>
+__device__ __forceinline__ int gatherCell(int n, const SlabPart& part)
+{
+    if (!Slab)
+    {
+        const int c = xcdBlock() * kReduceBlock + threadIdx.x;
+        return c < n ? c : -1;
+    }
+    const int k = blockIdx.x & 7;
+    const int s0 = (int)((long long)k * part.plane >> 3), s1 = (int)((long long)(k + 1) * part.plane >> 3);
+    const int slab = s1 - s0, sub = (slab + Split - 1) / Split;
+    const int nPl = (n + part.plane - 1) / part.plane;
+    const long long p = (long long)(blockIdx.x >> 3) * kReduceBlock + threadIdx.x;
+    const long long per = (long long)nPl * sub;
+    const int col = (int)(p / per);
+    if (col >= Split) return -1;
+    const long long q = p - col * per;
+    const int z = (int)(q / sub), x = col * sub + (int)(q - (long long)z * sub);
+    if (x >= slab) return -1;
+    const long long row = (long long)z * part.plane + s0 + x;
+    return row < n ? (int)row : -1;
+}
+
+// workgroups of the gather partition
+inline int gatherBlocks(int n, int plane, int split = 1)
+{
+    const long long nPl = ((long long)n + plane - 1) / plane;
+    const long long slabMax = (plane + 7) / 8, sub = (slabMax + split - 1) / split;
+    return (int)(8 * ((split * nPl * sub + kReduceBlock - 1) / kReduceBlock));
+}
+
+// the partition of a mesh's cells for gather kernels: slabs of the band when the cells span several
+// bands of at least kSlabMinPlane cells, one row per thread
+inline SlabPart gatherPart(const Mesh& m)
+{
+    if (m.band >= kSlabMinPlane && m.band < m.nCells) return SlabPart{m.nCells, m.band, 1};
+    return SlabPart{m.nCells, 0, 1};
 }
 
 // Row loop over this workgroup's chunk of a reduceGrid partition, XCD-ordered.

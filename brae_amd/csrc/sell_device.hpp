@@ -199,6 +199,7 @@ struct CycVec
 {
     Ts* __restrict__ p;
     __device__ CycVec(const Ts* q, const float*) : p(const_cast<Ts*>(q)) {}
+    __device__ CycVec(const Ts* q, float, float) : p(const_cast<Ts*>(q)) {}
     __device__ Tv operator()(int i) const { return p[i]; }
     __device__ void set(int i, Tv v) const { p[i] = v; }
 };
@@ -212,6 +213,8 @@ struct CycVec<float, __half>
         : p(const_cast<__half*>(q)), m(mul ? *mul : 1.0f), im(mul ? 1.0f / *mul : 1.0f)
     {
     }
+    // the multiplier and its reciprocal already read (mul ? *mul : 1, mul ? 1 / *mul : 1)
+    __device__ CycVec(const __half* q, float mul, float inv) : p(const_cast<__half*>(q)), m(mul), im(inv) {}
     __device__ float operator()(int i) const { return __half2float(p[i]) * im; }
     __device__ void set(int i, float v) const { p[i] = __float2half(fminf(fmaxf(v * m, -65504.0f), 65504.0f)); }
 };

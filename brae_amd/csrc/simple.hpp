@@ -119,15 +119,19 @@ struct KEpsilon
     int nWallCells = 0;
     DBuf<int> isWall, kType, epsType, isEmpty, gType, wallCell;
     DBuf<double> yWall, kValue, epsValue;
-    DBuf<double> gradU, g3[3], gbn, G, divU, D, Dr, Up, Lo, S, iC, bC, Df, lD, lU, nutB0;
+    DBuf<double> gradU, gbn, G, divU, D, Dr, Up, Lo, S, iC, bC, Df, lD, lU, nutB0;
 
     KEpsilon(Ctx& ctx, Mesh& mesh, const Geometry& geom, const std::vector<int>& hIsWall, const std::vector<double>& hY,
              const std::vector<int>& hKType, const std::vector<double>& hKValue, const std::vector<int>& hEpsType,
              const std::vector<double>& hEpsValue, const KEpsilonControls& controls);
     // nuEff = nu + nut in the cells and on the boundary faces
     void nuEff(const double* nut, const double* nutb, double* nuEffC, double* nuEffB);
+    // gradScratch, when not null, is 9 nCells doubles whose contents the caller no longer needs (the SIMPLE
+    // step's packed grad(U) once its momentum assembly is done); the gradient is built there instead of in
+    // the model's own array, which is then never allocated
     KEpsilonResult correct(const double* const* U, const double* const* Ub, const double* phi, const double* phiB,
-                           double* k, double* kb, double* eps, double* epsb, double* nut, double* nutb);
+                           double* k, double* kb, double* eps, double* epsb, double* nut, double* nutb,
+                           double* gradScratch = nullptr);
 
 private:
     PcgPerf solveScalar(bool isEps, const double* phi, const double* phiB, const double* nut, double* psi,

@@ -181,6 +181,7 @@ _SIGS = {
     "mfv_amg_num_grids": [P, P],
     "mfv_amg_grid_size": [P, I, P, P],
     "mfv_amg_fine_stencil": [P, P, P],
+    "mfv_amg_coarse_mode": [P, I, P, P, P],
     "mfv_fold_boundary_diag": [P, P, P, P, P, P, P],
     "mfv_reciprocal_v": [P, P, P, P, P],
     "mfv_boundary_diag_term": [P, P, P, P, P, I, P],
@@ -491,6 +492,15 @@ class Amg:
         offs = (ctypes.c_int * 4)()
         self.ctx._check(lib().mfv_amg_fine_stencil(self.h, ctypes.byref(m), offs), "amg_fine_stencil")
         return list(offs[:m.value])
+
+    def coarse_mode(self, mode=-1):
+        """sets how the coarsest grids run (0 per-grid kernels, 1 one workgroup, 2 one workgroup with the
+        coarsest grids staged in LDS; -1 leaves it) and returns (the tail's first grid, the first grid
+        staged, the LDS bytes the staged grids take)"""
+        t, g, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.ctx._check(lib().mfv_amg_coarse_mode(self.h, mode, ctypes.byref(t), ctypes.byref(g), ctypes.byref(b)),
+                        "amg_coarse_mode")
+        return t.value, g.value, b.value
 
     def level(self, i):
         import numpy as np

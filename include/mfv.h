@@ -341,6 +341,15 @@ int mfv_amg_grid_size(mfv_amg amg, int grid, int* nCells, int* nFaces);
  * *nOffsets is their number and hOffsets[0..*nOffsets) (may be NULL; room for four) the ascending
  * offsets. 0 offsets: the fine matrix is read as sliced rows. */
 int mfv_amg_fine_stencil(mfv_amg amg, int* nOffsets, int* hOffsets);
+/* How the coarsest grids' cycle runs, no counterpart in the reference (its small levels run one kernel
+ * per step). Below 4096 cells the grids from the first such one down run in one workgroup (the tail):
+ * mode 2 (default) stages the tail's coarsest grids -- as many as fit the device's LDS per workgroup --
+ * in that workgroup's LDS and reads the others where they are; mode 1 reads them all where they are;
+ * mode 0 runs every grid with its own kernels. All modes give the same bits (the scaled correction
+ * keeps mode 1's kernel). mode -1 only queries. *tailStart: the tail's first grid (0: no tail);
+ * *ldsGrid: the first grid staged (0: none); *ldsBytes: the LDS the staged grids take, in the precision
+ * of the last mfv_amg_galerkin; any of the three may be NULL. */
+int mfv_amg_coarse_mode(mfv_amg amg, int mode, int* tailStart, int* ldsGrid, int* ldsBytes);
 /* host copies of the step from grid `level` to grid `level+1` (any pointer may be NULL):
  * restrict map [nCells(level)], coarse owner/neighbour [nFaces(level+1)], faceRestrict and faceFlip
  * [nFaces(level)] -- AMGLevel's map/cOwn/cNei/faceRestrict/faceFlip (device_amg.cuh:119-182) */

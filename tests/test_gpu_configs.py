@@ -8,6 +8,8 @@
 - 2.1 M-cell channel (256 x 128 x 64, kEpsilon with wall functions): three turbulent SIMPLE iterations with the
   FP16 and the FP64 pressure cycle from the same seeded fields; U, p, k and epsilon agree within 1e-3 relative
   L2 and every pressure solve reaches 1e-6.
+- 120 M-cell channel (800 x 400 x 375, the largest single-GPU configuration): one turbulent SIMPLE iteration with
+  the FP32 cycle; the pressure solves reach 1e-6, the fields stay finite, the HBM in use fits the card.
 - Serving at the stream's sizes: 16 requests of log-uniform 32 k - 4 M cells (aspect 1-8, 30 % jittered,
   1-20 steps) with the FP32 and the FP16 cycle; each request's fingerprint is the same alone and in the mix,
   and every residual is below 1e-6.
@@ -98,6 +100,63 @@ def test_turbulent_channel_2M_mixed_against_fp64():
         assert d < 1e-3, (k, d)
     ch.mesh.close()
     ctx.close()
+
+
+def release(torch):
+    """the previous tests' device memory back to the device: their objects collected, torch's cache emptied"""
+    import gc
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
+def test_turbulent_channel_120M_fp32():
+    """the largest single-GPU configuration: one turbulent SIMPLE iteration on 800 x 400 x 375 (120 M cells) with
+    the FP32 cycle; every pressure solve reaches 1e-6, the fields stay finite, and the HBM the case takes (the
+    device's in-use memory over what was in use before it, the library's allocations included) fits the card"""
+    torch = torch_cuda()
+    release(torch)
+    free_b, total_b = torch.cuda.mem_get_info()
+    before = (total_b - free_b) / 2 ** 30
+    ctx = mfv.Context(0)
+    print("case: turbulent SIMPLE 800x400x375 FP32 (%.1f GiB in use before)" % before, flush=True)
+    objs = {}
+    try:
+        ch = objs["ch"] = C.Channel(800, 400, 375, torch, mfv, hexmesh, ctx, log=print, corrector=False)
+        uT, uV, pT, pV = C.channel_bcs(ch, torch)
+        turb = objs["turb"] = C.channel_kepsilon(ch, torch, mfv, tol=1e-5, relTol=0.1, log=print)
+        geom = mfv.Geometry(*(mfv.ptr(t) for t in (ch.Sf, ch.Cf, ch.magSf, ch.C, ch.V, ch.w, ch.dc, ch.ndc, ch.corr,
+                                                    ch.bDelta)))
+        ctl = mfv.simple_controls(U=dict(tolerance=1e-5, relTol=0.1, preconditioner=mfv.PRECOND_DIAGONAL),
+                                  p=dict(tolerance=1e-6, relTol=0.0, precision=mfv.PREC_FP32, nPreSweeps=1,
+                                         nPostSweeps=1, flexible=1, scaleCorrection=0),
+                                  mergeLevels=2, nu=1e-5, linearUpwind=1, correctedLaplacian=1, nNonOrthCorr=1)
+        sim = objs["sim"] = mfv.Simple(ch.mesh, geom, uT, uV.ravel(), pT, pV, ctl)
+        ke = objs["ke"] = mfv.KEpsilon(ch.mesh, geom, turb["isWall"], turb["y"], turb["type"], turb["kV"],
+                                       turb["type"], turb["eV"], turb["controls"])
+        f = objs["f"] = C.simple_fields(ch, torch, uT, uV)
+        f.update({k: v.clone() for k, v in turb["fields0"].items()})
+        r, kr = mfv.simple_step_kepsilon(sim, ke, f["U"], f["Ub"], f["p"], f["pb"], f["phi"], f["phiB"], f["k"],
+                                         f["kb"], f["eps"], f["epsb"], f["nut"], f["nutb"])
+        torch.cuda.synchronize()
+        free_b, total_b = torch.cuda.mem_get_info()
+        used = (total_b - free_b) / 2 ** 30 - before
+        print("p iterations", list(r.pIters[:2]), "final", r.pFinal, "U", list(r.UIters), "k-epsilon", kr.epsIters,
+              kr.kIters, "HBM taken by the case %.1f GiB" % used, flush=True)
+        assert r.pFinal < 1e-6 and kr.nWallCells > 0
+        for name in ("p", "k", "eps", "nut"):
+            assert bool(torch.isfinite(f[name]).all()), name
+        assert all(bool(torch.isfinite(u).all()) for u in f["U"])
+        assert used < 250.0   # of the card's 288 GB (268 GiB)
+    finally:
+        for k in ("sim", "ke"):
+            if k in objs:
+                objs[k].close()
+        if "ch" in objs:
+            objs["ch"].mesh.close()
+        objs.clear()
+        ctx.close()
+        release(torch)
 
 
 def serving_stream(n, seed=2025):
