@@ -102,7 +102,8 @@ def simple_leg(args, ch, ctx, torch, mfv, prec_mixed, other, on_step=None):
         return mfv.simple_controls(
             U=U_ctl, p=dict(tolerance=args.tolerance, relTol=0.0, precision=prec, nPreSweeps=args.npre,
                             nPostSweeps=args.npost, flexible=int(prec != mfv.PREC_FP64 or args.scale),
-                            scaleCorrection=int(args.scale), checkEvery=args.check_every),
+                            scaleCorrection=int(args.scale), checkEvery=args.check_every,
+                            captureVcycle=args.graph),
             mergeLevels=args.merge_levels, nu=args.nu, relaxU=0.7, relaxP=0.3, linearUpwind=1,
             correctedLaplacian=1, nNonOrthCorr=args.non_orth, pRefCell=-1)
 

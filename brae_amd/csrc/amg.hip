@@ -13,7 +13,6 @@
 #include "sell_device.hpp"
 
 #include <algorithm>
-#include <atomic>
 #include <cstring>
 #include <cmath>
 #include <map>
@@ -1434,23 +1433,7 @@ Ts* vcycleAt(Ctx& c, Amg& a, int gi, const VcycleControls& vc, const int* stop,
     if (gi > 0 && gi == a.tailStart && !c.sequentialReduce && a.coarseMode > 0)
     {
         ProfScope ps(c, kProfGridDown + gi);
-        bool lds = a.coarseMode == 2 && a.tailLdsBytes > 0 && !vc.scale;
-        // dynamic LDS beyond the default limit is opted into once per instantiation, to the largest tail
-        // seen; a device that refuses keeps tailK for this hierarchy
-        static std::atomic<int> optedIn{0};
-        if (lds && a.tailLdsBytes > optedIn.load())
-        {
-            if (hipFuncSetAttribute(reinterpret_cast<const void*>(&tailMixK<Tm, Tv, Ts>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, a.tailLdsBytes) == hipSuccess)
-                optedIn.store(a.tailLdsBytes);
-            else
-            {
-                (void)hipGetLastError();
-                a.tailLdsBytes = 0;
-                lds = false;
-            }
-        }
-        if (lds)
+        if (a.coarseMode == 2 && a.tailLdsBytes > 0 && !vc.scale)
         {
             tailMixK<Tm, Tv, Ts><<<1, kTailThreads, a.tailLdsBytes, c.stream>>>(
                 reinterpret_cast<const TailGrid*>(a.tailDesc.p), reinterpret_cast<const LdsTail*>(a.tailLdsDesc.p),
@@ -1807,6 +1790,16 @@ void buildTailLds(Ctx& c, Amg& a)
     if (o + nL * nL * sizeof(double) <= budget) d.oInv = take(nL * nL * sizeof(double));
     if (o > budget) return;
     d.bytes = (int)o;
+    // dynamic LDS beyond the default limit is opted into here, on the host before any launch (a capture
+    // may follow); a device that refuses keeps tailK for this hierarchy
+    const void* kern = f64 ? reinterpret_cast<const void*>(&tailMixK<double, double, double>)
+                     : f32 ? reinterpret_cast<const void*>(&tailMixK<float, float, float>)
+                           : reinterpret_cast<const void*>(&tailMixK<__half, float, __half>);
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, d.bytes) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return;
+    }
     a.tailLdsBlob.upload(blob, c.stream);
     // the copy plan: the blob, each staged grid's coefficients and diagonal, the inverse
     int words = 0;
