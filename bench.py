@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--serving", type=int, default=512, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-resident", type=float, default=0.0,
+                    help="GB of the serving budget that keep systems between requests (the stream repeats no box)")
     ap.add_argument("--predictor", type=int, default=1,
                     help="1: also time the pressure predictor's glue (rAU, HbyA, constrainHbyA, phiHbyA, adjustPhi)")
     ap.add_argument("--simple", type=int, default=1, help="1: time whole SIMPLE iterations (the headline)")
@@ -554,7 +556,8 @@ def serving_run(args):
                          nSteps=int(rng.integers(1, 21)), tolerance=1e-6, arrival=t))
     t0 = time.perf_counter()
     prec = 16 if args.precision == "fp16" else 32
-    rc, res = mfv.serve(reqs, nSlots=args.serve_slots, hbmBudgetBytes=200e9, precision=prec)
+    rc, res = mfv.serve(reqs, nSlots=args.serve_slots, hbmBudgetBytes=200e9, precision=prec,
+                        residentBytes=getattr(args, "serve_resident", 0.0) * 1e9)
     wall = time.perf_counter() - t0
     if rc != mfv.MFV_OK:
         raise RuntimeError("serving run failed (%d)" % rc)
@@ -568,6 +571,7 @@ def serving_run(args):
                 p99_latency_s=round(float(np.percentile(lat, 99)), 4), makespan_s=round(makespan, 2),
                 wall_s=round(wall, 2), arrival_rate_per_s=args.serve_rate, slots=args.serve_slots,
                 hbm_budget_gb=200, max_residual=max(r.finalResidual for r in res),
+                resident_gb=getattr(args, "serve_resident", 0.0), resident_hits=int(sum(r.resident for r in res)),
                 workload="pressure Poisson on hex boxes, log-uniform 32k-4M cells, x-aspect 1-8, 30%% jittered, "
                          "1-20 RHS per request, %s V-cycle AMG-PCG to 1e-6" % args.precision.upper())
 

@@ -16,6 +16,10 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <list>
+#include <map>
+#include <memory>
+#include <tuple>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -176,90 +180,136 @@ void check(int rc, mfv_ctx ctx, const char* what)
 
 } // namespace
 
-// one request, start to end, on the calling thread's context
-void serveOne(mfv_ctx ctx, const mfv_serve_request& q, int precision, mfv_serve_result& res)
+// A request's system on the device: the box's mesh, geometry, matrix and hierarchy (Galerkin done), and
+// its work vectors. Kept between requests when the server holds resident systems.
+struct System
 {
-    const Box box = makeBox(q.nx, q.ny, q.nz, q.lx, q.ly, q.lz, q.jitter, q.seed);
-    const int nC = box.nC, nI = (int)box.neighbour.size(), nB = (int)box.owner.size() - nI;
-    const int nF = (int)box.owner.size();
-    hipStream_t st = static_cast<hipStream_t>(mfv_ctx_stream(ctx));
     mfv_mesh mesh = nullptr;
     mfv_amg amg = nullptr;
-    try
+    int nC = 0, nI = 0, nB = 0;
+    double bytes = 0;
+    DBuf<double> Cf, Sf, magSf, C, V, w, dc, ndc, corr, bDelta, diag, upper, b, p, U, phi, zeroB;
+    DBuf<uint64_t> parts;
+    System() = default;
+    System(const System&) = delete;
+    System& operator=(const System&) = delete;
+    ~System()
     {
-        const std::vector<int> bfc(box.owner.begin() + nI, box.owner.end());
-        check(mfv_mesh_create(ctx, nC, nI, box.owner.data(), box.neighbour.data(), nB, bfc.data(), &mesh), ctx, "mesh");
-        DBuf<double> pts, Cf(3 * (size_t)nF), Sf(3 * (size_t)nF), magSf(nF), C(3 * (size_t)nC), V(nC), w(nI), dc(nI),
-            ndc(nI), corr(3 * (size_t)nI), bDelta(nB), gammaf(nI), diag(nC), upper(nI), b(nC), p(nC), U(3 * (size_t)nC),
-            phi(nI), zeroB(nB);
+        if (amg) mfv_amg_destroy(amg);
+        if (mesh) mfv_mesh_destroy(mesh);
+    }
+};
+
+// the box parameters a system is built from: equal keys, equal systems
+struct BoxKey
+{
+    int nx, ny, nz;
+    double lx, ly, lz, jitter;
+    unsigned long long seed;
+    bool operator<(const BoxKey& o) const
+    {
+        return std::tie(nx, ny, nz, lx, ly, lz, jitter, seed) < std::tie(o.nx, o.ny, o.nz, o.lx, o.ly, o.lz, o.jitter, o.seed);
+    }
+    bool operator==(const BoxKey& o) const { return !(*this < o) && !(o < *this); }
+};
+BoxKey keyOf(const mfv_serve_request& q) { return BoxKey{q.nx, q.ny, q.nz, q.lx, q.ly, q.lz, q.jitter, q.seed}; }
+
+// the request's box on the device, its laplacian and its hierarchy, on the calling thread's context
+std::unique_ptr<System> buildSystem(mfv_ctx ctx, const mfv_serve_request& q, int precision, double bytes)
+{
+    const Box box = makeBox(q.nx, q.ny, q.nz, q.lx, q.ly, q.lz, q.jitter, q.seed);
+    auto S = std::make_unique<System>();
+    const int nC = box.nC, nI = (int)box.neighbour.size(), nB = (int)box.owner.size() - nI;
+    const int nF = (int)box.owner.size();
+    S->nC = nC;
+    S->nI = nI;
+    S->nB = nB;
+    S->bytes = bytes;
+    hipStream_t st = static_cast<hipStream_t>(mfv_ctx_stream(ctx));
+    const std::vector<int> bfc(box.owner.begin() + nI, box.owner.end());
+    check(mfv_mesh_create(ctx, nC, nI, box.owner.data(), box.neighbour.data(), nB, bfc.data(), &S->mesh), ctx, "mesh");
+    for (DBuf<double>* d : {&S->Cf, &S->Sf}) d->resize(3 * (size_t)nF);
+    S->magSf.resize(nF);
+    S->C.resize(3 * (size_t)nC);
+    S->U.resize(3 * (size_t)nC);
+    for (DBuf<double>* d : {&S->V, &S->diag, &S->b, &S->p}) d->resize(nC);
+    for (DBuf<double>* d : {&S->w, &S->dc, &S->ndc, &S->upper, &S->phi}) d->resize(nI);
+    S->corr.resize(3 * (size_t)nI);
+    S->bDelta.resize(nB);
+    S->zeroB.resize(nB);
+    {
+        DBuf<double> pts, gammaf;
         DBuf<int> off, verts;
         pts.upload(box.points, st);
         off.upload(box.faceOffsets, st);
         verts.upload(box.faceVerts, st);
-        check(mfv_mesh_geometry(ctx, mesh, pts.p, off.p, verts.p, Cf.p, Sf.p, magSf.p, C.p, V.p, w.p, dc.p, ndc.p,
-                                corr.p, bDelta.p),
+        check(mfv_mesh_geometry(ctx, S->mesh, pts.p, off.p, verts.p, S->Cf.p, S->Sf.p, S->magSf.p, S->C.p, S->V.p,
+                                S->w.p, S->dc.p, S->ndc.p, S->corr.p, S->bDelta.p),
               ctx, "geometry");
         // -laplacian(1, p) held positive (brae's sign), all walls zeroGradient, reference cell 0
         std::vector<double> minusOne(nI, -1.0);
         gammaf.upload(minusOne, st);
-        check(mfv_laplacian_coeffs(ctx, mesh, dc.p, gammaf.p, magSf.p, diag.p, upper.p, nullptr), ctx, "laplacian");
-        hipCheck(hipMemsetAsync(b.p, 0, sizeof(double) * nC, st), "rhs clear");
-        check(mfv_set_reference(ctx, nC, diag.p, b.p, 0, 0.0), ctx, "reference");
-        std::vector<double> hw(nI);
-        hipCheck(hipMemcpyAsync(hw.data(), magSf.p, sizeof(double) * nI, hipMemcpyDeviceToHost, st), "weights");
-        hipCheck(hipStreamSynchronize(st), "weights sync");
-        // levels of two composed pairwise passes and a V(2,1) cycle (the fastest measured cycle on the
-        // 20 M-cell channel)
-        mfv_amg_controls ac{MFV_AGG_BRAE, 64, 2};
-        check(mfv_amg_build(ctx, mesh, hw.data(), &ac, &amg), ctx, "amg");
-        check(mfv_amg_galerkin(ctx, amg, diag.p, upper.p, nullptr, precision), ctx, "galerkin");
-        mfv_pcg_controls pc;
-        mfv_pcg_default_controls(&pc);
-        pc.tolerance = q.tolerance;
-        pc.precision = precision;
-        pc.scaleCorrection = 1;   // scaled correction: a nonlinear cycle, so flexible CG
-        pc.flexible = 1;
-        pc.nPreSweeps = 2;
-        pc.nPostSweeps = 1;
-        const mfv_ldu A{diag.p, upper.p, upper.p};
-        hipCheck(hipMemsetAsync(p.p, 0, sizeof(double) * nC, st), "p clear");
-        const int *dOwn = nullptr, *dNei = nullptr;
-        check(mfv_mesh_addressing(mesh, &dOwn, &dNei, nullptr, nullptr, nullptr), ctx, "addressing");
-        hipCheck(hipMemsetAsync(zeroB.p, 0, sizeof(double) * nB, st), "phib clear");
-        const int nParts = std::min(1024, blocks(nC));
-        DBuf<uint64_t> parts(nParts);
-        std::vector<uint64_t> hp(nParts);
-        uint64_t fp = 0;
-        res.totalIterations = 0;
-        for (int s = 0; s < q.nSteps; ++s)
-        {
-            velocityK<<<blocks(3LL * nC), kTpb, 0, st>>>(nC, q.seed * 1000003ull + (uint64_t)s, U.p);
-            faceFluxUK<<<blocks(nI), kTpb, 0, st>>>(nI, dOwn, dNei, w.p, U.p, Sf.p, phi.p);
-            hipCheck(hipMemsetAsync(b.p, 0, sizeof(double) * nC, st), "rhs clear");
-            check(mfv_div_source(ctx, mesh, phi.p, zeroB.p, V.p, -1.0, b.p), ctx, "div");
-            mfv_perf perf{};
-            check(mfv_amg_pcg_solve(ctx, mesh, amg, &A, b.p, p.p, nullptr, &pc, &perf), ctx, "solve");
-            res.totalIterations += perf.nIterations;
-            res.finalResidual = perf.finalResidual;
-            bitsSumK<<<nParts, kTpb, 0, st>>>(nC, p.p, parts.p);
-            hipCheck(hipMemcpyAsync(hp.data(), parts.p, sizeof(uint64_t) * nParts, hipMemcpyDeviceToHost, st), "fp");
-            hipCheck(hipStreamSynchronize(st), "fp sync");
-            uint64_t t = 0;
-            for (uint64_t v : hp) t += v;
-            fp = fp * 0x100000001b3ull ^ t;
-        }
-        res.fingerprint = fp;
-        res.nCells = nC;
-        res.nSteps = q.nSteps;
+        check(mfv_laplacian_coeffs(ctx, S->mesh, S->dc.p, gammaf.p, S->magSf.p, S->diag.p, S->upper.p, nullptr), ctx,
+              "laplacian");
+        hipCheck(hipStreamSynchronize(st), "geometry sync");
     }
-    catch (...)
+    hipCheck(hipMemsetAsync(S->b.p, 0, sizeof(double) * nC, st), "rhs clear");
+    check(mfv_set_reference(ctx, nC, S->diag.p, S->b.p, 0, 0.0), ctx, "reference");
+    std::vector<double> hw(nI);
+    hipCheck(hipMemcpyAsync(hw.data(), S->magSf.p, sizeof(double) * nI, hipMemcpyDeviceToHost, st), "weights");
+    hipCheck(hipStreamSynchronize(st), "weights sync");
+    // levels of two composed pairwise passes and a V(2,1) cycle (the fastest measured cycle on the
+    // 20 M-cell channel's pressure corrector)
+    mfv_amg_controls ac{MFV_AGG_BRAE, 64, 2};
+    check(mfv_amg_build(ctx, S->mesh, hw.data(), &ac, &S->amg), ctx, "amg");
+    check(mfv_amg_galerkin(ctx, S->amg, S->diag.p, S->upper.p, nullptr, precision), ctx, "galerkin");
+    hipCheck(hipMemsetAsync(S->zeroB.p, 0, sizeof(double) * nB, st), "phib clear");
+    S->parts.resize(std::min(1024, blocks(nC)));
+    return S;
+}
+
+// the request's steps on a built system: from p = 0, each step's right-hand side -div(U) of a seeded
+// velocity, the solve warm-started from the previous step's p, the solution's fingerprint
+void solveSteps(mfv_ctx ctx, System& S, const mfv_serve_request& q, int precision, mfv_serve_result& res)
+{
+    const int nC = S.nC, nI = S.nI;
+    hipStream_t st = static_cast<hipStream_t>(mfv_ctx_stream(ctx));
+    mfv_pcg_controls pc;
+    mfv_pcg_default_controls(&pc);
+    pc.tolerance = q.tolerance;
+    pc.precision = precision;
+    pc.scaleCorrection = 1;   // scaled correction: a nonlinear cycle, so flexible CG
+    pc.flexible = 1;
+    pc.nPreSweeps = 2;
+    pc.nPostSweeps = 1;
+    const mfv_ldu A{S.diag.p, S.upper.p, S.upper.p};
+    hipCheck(hipMemsetAsync(S.p.p, 0, sizeof(double) * nC, st), "p clear");
+    const int *dOwn = nullptr, *dNei = nullptr;
+    check(mfv_mesh_addressing(S.mesh, &dOwn, &dNei, nullptr, nullptr, nullptr), ctx, "addressing");
+    const int nParts = (int)S.parts.n;
+    std::vector<uint64_t> hp(nParts);
+    uint64_t fp = 0;
+    res.totalIterations = 0;
+    for (int s = 0; s < q.nSteps; ++s)
     {
-        if (amg) mfv_amg_destroy(amg);
-        if (mesh) mfv_mesh_destroy(mesh);
-        throw;
+        velocityK<<<blocks(3LL * nC), kTpb, 0, st>>>(nC, q.seed * 1000003ull + (uint64_t)s, S.U.p);
+        faceFluxUK<<<blocks(nI), kTpb, 0, st>>>(nI, dOwn, dNei, S.w.p, S.U.p, S.Sf.p, S.phi.p);
+        hipCheck(hipMemsetAsync(S.b.p, 0, sizeof(double) * nC, st), "rhs clear");
+        check(mfv_div_source(ctx, S.mesh, S.phi.p, S.zeroB.p, S.V.p, -1.0, S.b.p), ctx, "div");
+        mfv_perf perf{};
+        check(mfv_amg_pcg_solve(ctx, S.mesh, S.amg, &A, S.b.p, S.p.p, nullptr, &pc, &perf), ctx, "solve");
+        res.totalIterations += perf.nIterations;
+        res.finalResidual = perf.finalResidual;
+        bitsSumK<<<nParts, kTpb, 0, st>>>(nC, S.p.p, S.parts.p);
+        hipCheck(hipMemcpyAsync(hp.data(), S.parts.p, sizeof(uint64_t) * nParts, hipMemcpyDeviceToHost, st), "fp");
+        hipCheck(hipStreamSynchronize(st), "fp sync");
+        uint64_t t = 0;
+        for (uint64_t v : hp) t += v;
+        fp = fp * 0x100000001b3ull ^ t;
     }
-    mfv_amg_destroy(amg);
-    mfv_mesh_destroy(mesh);
+    res.fingerprint = fp;
+    res.nCells = nC;
+    res.nSteps = q.nSteps;
 }
 
 } // namespace mfv
@@ -271,6 +321,7 @@ struct mfv_server_s
     int device = 0, nSlots = 1, precision = MFV_PREC_FP32;
     double budget = 0;
     double bytesPerCell = 600.0;
+    double resident = 0;   // bytes of the budget that hold systems between requests (0: none kept)
 };
 
 extern "C" {
@@ -290,6 +341,13 @@ int mfv_server_create(int device, int nSlots, double hbmBudgetBytes, int precisi
     return MFV_OK;
 }
 
+int mfv_server_set_resident(mfv_server s, double residentBytes)
+{
+    if (!s || !(residentBytes >= 0) || residentBytes >= s->budget) return MFV_ERR_INVALID;
+    s->resident = residentBytes;
+    return MFV_OK;
+}
+
 int mfv_server_destroy(mfv_server s)
 {
     delete s;
@@ -304,11 +362,13 @@ double mfv_server_request_bytes(mfv_server s, const mfv_serve_request* q)
 int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_serve_result* results)
 {
     if (!s || nReq < 0 || (nReq && (!reqs || !results))) return MFV_ERR_INVALID;
+    // the resident systems' share of the budget is held back from admission for the whole run
+    const double admit = s->budget - s->resident;
     for (int i = 0; i < nReq; ++i)
     {
         const mfv_serve_request& q = reqs[i];
         if (q.nx < 2 || q.ny < 2 || q.nz < 2 || q.nSteps < 1 || q.tolerance <= 0 ||
-            (double)q.nx * q.ny * q.nz > 1.0e8 || mfv_server_request_bytes(s, &q) > s->budget)
+            (double)q.nx * q.ny * q.nz > 1.0e8 || mfv_server_request_bytes(s, &q) > admit)
             return MFV_ERR_INVALID;
         if (i && q.arrival < reqs[i - 1].arrival) return MFV_ERR_INVALID;   // arrivals in order
     }
@@ -317,10 +377,15 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
     std::condition_variable cv;
     int next = 0;            // first request not yet admitted (FIFO)
     double inflight = 0;     // admitted bytes
-    int idle = s->nSlots;
     int firstError = MFV_OK;
     std::string errorText;
-    auto worker = [&]() {
+    // resident systems: each slot keeps its own (they live on its context); `owner` says which slot
+    // holds a box, `idle` which slots wait for work, so a request whose box is resident on an idle slot
+    // goes to that slot
+    std::map<BoxKey, int> owner;
+    std::vector<char> idle(s->nSlots, 0);
+    const double perSlot = s->resident / s->nSlots;
+    auto worker = [&](int slot) {
         mfv_ctx ctx = nullptr;
         if (mfv_ctx_create(s->device, nullptr, &ctx) != MFV_OK)
         {
@@ -330,19 +395,35 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
             cv.notify_all();
             return;
         }
+        struct Entry
+        {
+            BoxKey key;
+            std::unique_ptr<System> sys;
+        };
+        std::list<Entry> cache;   // most recently used first
+        double cached = 0;
         for (;;)
         {
             int i;
             double bytes;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                // admit the queue's head when it has arrived and fits in what the budget has left
+                idle[slot] = 1;
+                // admit the queue's head when it has arrived, fits in what the budget has left and is not
+                // resident on another idle slot
                 for (;;)
                 {
                     if (next >= nReq) break;
                     const double wait = reqs[next].arrival - now(t0);
                     bytes = mfv_server_request_bytes(s, &reqs[next]);
-                    if (wait <= 0 && inflight + bytes <= s->budget) break;
+                    if (wait <= 0 && inflight + bytes <= admit)
+                    {
+                        const auto o = owner.find(keyOf(reqs[next]));
+                        if (o == owner.end() || o->second == slot || !idle[o->second]) break;
+                        cv.notify_all();   // the slot that holds it takes it
+                        cv.wait(lk);
+                        continue;
+                    }
                     if (wait > 0)
                         cv.wait_for(lk, std::chrono::duration<double>(wait));
                     else
@@ -351,7 +432,7 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
                 if (next >= nReq) break;
                 i = next++;
                 inflight += bytes;
-                --idle;
+                idle[slot] = 0;
                 results[i].admitted = now(t0);
                 results[i].arrival = reqs[i].arrival;
             }
@@ -360,7 +441,35 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
             std::string err;
             try
             {
-                serveOne(ctx, reqs[i], s->precision, results[i]);
+                const BoxKey key = keyOf(reqs[i]);
+                std::unique_ptr<System> sys;
+                for (auto it = cache.begin(); it != cache.end(); ++it)
+                    if (it->key == key)
+                    {
+                        sys = std::move(it->sys);
+                        cached -= sys->bytes;
+                        cache.erase(it);
+                        break;
+                    }
+                results[i].resident = sys ? 1 : 0;
+                if (!sys) sys = buildSystem(ctx, reqs[i], s->precision, bytes);
+                solveSteps(ctx, *sys, reqs[i], s->precision, results[i]);
+                if (sys->bytes <= perSlot)
+                {
+                    // keep it, the least recently used going first when the slot's share is full
+                    cached += sys->bytes;
+                    cache.push_front(Entry{key, std::move(sys)});
+                    std::lock_guard<std::mutex> g(mu);
+                    owner[key] = slot;
+                    while (cached > perSlot)
+                    {
+                        Entry& e = cache.back();
+                        cached -= e.sys->bytes;
+                        const auto o = owner.find(e.key);
+                        if (o != owner.end() && o->second == slot) owner.erase(o);
+                        cache.pop_back();
+                    }
+                }
             }
             catch (const Error& e)
             {
@@ -377,7 +486,6 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
                 results[i].finished = now(t0);
                 results[i].status = rc;
                 inflight -= bytes;
-                ++idle;
                 if (rc != MFV_OK && firstError == MFV_OK)
                 {
                     firstError = rc;
@@ -386,12 +494,22 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
             }
             cv.notify_all();
         }
+        {
+            std::lock_guard<std::mutex> g(mu);
+            for (const Entry& e : cache)
+            {
+                const auto o = owner.find(e.key);
+                if (o != owner.end() && o->second == slot) owner.erase(o);
+            }
+            idle[slot] = 0;
+        }
+        cv.notify_all();
+        cache.clear();   // the systems go before their context
         mfv_ctx_destroy(ctx);
     };
     std::vector<std::thread> threads;
-    for (int k = 0; k < s->nSlots; ++k) threads.emplace_back(worker);
+    for (int k = 0; k < s->nSlots; ++k) threads.emplace_back(worker, k);
     for (auto& t : threads) t.join();
-    (void)idle;
     return firstError;
 }
 

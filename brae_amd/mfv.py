@@ -86,7 +86,7 @@ class ServeRequest(ctypes.Structure):
 
 class ServeResult(ctypes.Structure):
     _fields_ = [("arrival", D), ("admitted", D), ("finished", D), ("status", I), ("nCells", I), ("nSteps", I),
-                ("totalIterations", I), ("finalResidual", D), ("fingerprint", ctypes.c_ulonglong)]
+                ("totalIterations", I), ("finalResidual", D), ("fingerprint", ctypes.c_ulonglong), ("resident", I)]
 
 
 class Perf(ctypes.Structure):
@@ -230,15 +230,16 @@ _SIGS = {
     "mfv_simple_step": [P, P, P, P, P],
     "mfv_server_create": [I, I, D, I, P],
     "mfv_server_destroy": [P],
+    "mfv_server_set_resident": [P, D],
     "mfv_server_request_bytes": [P, P],
     "mfv_server_run": [P, P, I, P],
 }
 _RESTYPES = {"mfv_last_error": ctypes.c_char_p, "mfv_ctx_stream": P, "mfv_server_request_bytes": D}
 
 
-def serve(requests, nSlots=8, hbmBudgetBytes=200e9, precision=32, device=0):
-    """run a request stream (list of ServeRequest or dicts of its fields) through the serving loop;
-    returns (status, [ServeResult])"""
+def serve(requests, nSlots=8, hbmBudgetBytes=200e9, precision=32, device=0, residentBytes=0.0):
+    """run a request stream (list of ServeRequest or dicts of its fields) through the serving loop, with
+    residentBytes of the budget keeping systems between requests; returns (status, [ServeResult])"""
     reqs = (ServeRequest * len(requests))()
     for i, r in enumerate(requests):
         if isinstance(r, dict):
@@ -252,6 +253,10 @@ def serve(requests, nSlots=8, hbmBudgetBytes=200e9, precision=32, device=0):
     if rc != MFV_OK:
         raise MfvError("server_create failed (%d)" % rc)
     try:
+        if residentBytes:
+            rc = lib().mfv_server_set_resident(srv, float(residentBytes))
+            if rc != MFV_OK:
+                raise MfvError("server_set_resident failed (%d)" % rc)
         rc = lib().mfv_server_run(srv, reqs, len(requests), res)
     finally:
         lib().mfv_server_destroy(srv)

@@ -641,7 +641,10 @@ int mfv_simple_step_kepsilon(mfv_simple simple, const mfv_simple_fields* fields,
  * AMG-PCG to `tolerance` in the server's cycle precision. Requests are admitted in arrival order when
  * they have arrived (seconds after mfv_server_run starts), a slot is free and their HBM estimate fits
  * the budget; each slot is a host thread with its own stream, so admitted requests overlap on the
- * device. A request's mesh, matrix and hierarchy stay resident across its steps.
+ * device. A request's mesh, matrix and hierarchy stay resident across its steps, and -- with
+ * mfv_server_set_resident -- between requests: a request for a box that a slot still holds (the same
+ * nx..seed) is handed to that slot when it is idle and solves on the system it keeps, without building
+ * anything; its result is the same bits as on a new build.
  * ------------------------------------------------------------------------------------------- */
 typedef struct mfv_server_s* mfv_server;
 typedef struct
@@ -661,9 +664,14 @@ typedef struct
     int nCells, nSteps, totalIterations;
     double finalResidual;                 /* of the last step */
     unsigned long long fingerprint;       /* of every step's solution bits */
+    int resident;                         /* 1: solved on a system a slot kept from an earlier request */
 } mfv_serve_result;
 int mfv_server_create(int device, int nSlots, double hbmBudgetBytes, int precision, mfv_server* out);
 int mfv_server_destroy(mfv_server server);
+/* bytes of the HBM budget that hold systems (mesh, geometry, matrix, hierarchy, work vectors) between
+ * requests, an equal share per slot, least recently used dropped first; held back from admission for
+ * the whole run. 0 (the default) keeps none. */
+int mfv_server_set_resident(mfv_server server, double residentBytes);
 /* the HBM estimate admission charges a request */
 double mfv_server_request_bytes(mfv_server server, const mfv_serve_request* request);
 /* runs the stream to the end; returns MFV_OK or the first request error (each result has its status) */

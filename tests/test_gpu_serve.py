@@ -1,7 +1,8 @@
 """The serving loop: requests of uneven sizes admitted by an HBM budget into concurrent slots.
 
 Each request's solution is the same bits alone and in a mix (fingerprint of every step's p), every
-request converges, admission follows arrival order and the budget, and the requests really overlap."""
+request converges, admission follows arrival order and the budget, and the requests really overlap.
+Systems kept between requests give the same bits as new builds."""
 import pytest
 
 from brae_amd import mfv
@@ -51,3 +52,38 @@ def test_admission_by_budget():
     # a request larger than the whole budget is refused up front
     rc, _ = mfv.serve([req((40, 40, 40), seed=3)], nSlots=1, hbmBudgetBytes=budget)
     assert rc == mfv.MFV_ERR_INVALID
+
+
+@pytest.mark.parametrize("precision", [32, 16])
+def test_resident_systems_between_requests(precision):
+    """With resident systems, a box asked for again (a client's next request on its mesh) is solved on
+    the system a slot kept: the same fingerprint and iterations as on a new build, most repeats resident,
+    and the first request of each box never resident. Without them nothing is kept."""
+    boxes = [req((40, 30, 20), (2.0, 1.0, 1.0), 0.1, seed=31, steps=2),
+             req((24, 24, 24), seed=32, steps=3),
+             req((48, 16, 16), (4.0, 1.0, 1.0), 0.1, seed=33, steps=2)]
+    stream = []
+    for k in range(4):
+        for b, box in enumerate(boxes):
+            stream.append(dict(box, arrival=0.05 * (3 * k + b)))
+    print("case: resident serving", len(stream), "requests, precision", precision, flush=True)
+    rc, cold = mfv.serve(stream, nSlots=2, hbmBudgetBytes=50e9, precision=precision)
+    assert rc == mfv.MFV_OK and not any(r.resident for r in cold)
+    # one slot keeps all three boxes: every repeat is resident
+    rc, one = mfv.serve(stream, nSlots=1, hbmBudgetBytes=50e9, precision=precision, residentBytes=10e9)
+    assert rc == mfv.MFV_OK
+    assert [r.resident for r in one] == [0, 0, 0] + [1] * 9
+    # two slots: a repeat goes to the slot that holds its box when that slot is idle
+    rc, two = mfv.serve(stream, nSlots=2, hbmBudgetBytes=50e9, precision=precision, residentBytes=10e9)
+    assert rc == mfv.MFV_OK
+    print("resident with two slots", [r.resident for r in two], flush=True)
+    assert not any(r.resident for r in two[:3]) and any(r.resident for r in two[3:])
+    for c, a, b in zip(cold, one, two):
+        for w in (a, b):
+            assert w.status == 0 and w.finalResidual < 1e-6
+            assert w.fingerprint == c.fingerprint and w.totalIterations == c.totalIterations
+    # a share that cannot hold a system keeps none; a share above the budget is refused
+    rc, tiny = mfv.serve(stream[:6], nSlots=2, hbmBudgetBytes=50e9, precision=precision, residentBytes=1e6)
+    assert rc == mfv.MFV_OK and not any(r.resident for r in tiny)
+    with pytest.raises(mfv.MfvError):
+        mfv.serve(stream[:1], nSlots=1, hbmBudgetBytes=1e9, precision=precision, residentBytes=2e9)
