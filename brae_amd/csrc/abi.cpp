@@ -946,6 +946,39 @@ int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A,
     });
 }
 
+int mfv_amg_pcg_solve_batch(mfv_ctx ctx, int n, const mfv_batch_solve* solves, mfv_perf* perfs)
+{
+    if (n < 0 || (n && !solves)) return MFV_ERR_INVALID;
+    return guarded(ctx, [&] {
+        std::vector<BatchSolve> b(n);
+        for (int k = 0; k < n; ++k)
+        {
+            const mfv_batch_solve& q = solves[k];
+            // the systems may live on other contexts of the same device (a serving slot's); the launch
+            // goes on this context's stream, theirs must be idle meanwhile
+            need(q.mesh && q.amg && q.amg->mesh == q.mesh, "pcg batch: null mesh, or a hierarchy of another mesh");
+            need(q.amg->ctx->c.device == ctx->c.device, "pcg batch: a system on another device");
+            need(q.dDiag && q.dB && q.dPsi, "pcg batch: null argument");
+            if (q.controls.smoother != 0 || q.controls.nVcycles > 1)
+                throw Error(kErrUnsupported, "mfv: pcg batch: weighted Jacobi and one cycle only");
+            b[k].a = &q.amg->a;
+            b[k].diag = q.dDiag;
+            b[k].b = q.dB;
+            b[k].psi = q.dPsi;
+            b[k].pc = pcgControls(q.controls);
+        }
+        amgBatchSolve(ctx->c, b);
+        if (perfs)
+            for (int k = 0; k < n; ++k)
+            {
+                perfs[k].initialResidual = b[k].perf.initialResidual;
+                perfs[k].finalResidual = b[k].perf.finalResidual;
+                perfs[k].nIterations = b[k].perf.nIterations;
+                perfs[k].converged = b[k].perf.converged ? 1 : 0;
+            }
+    });
+}
+
 int mfv_interpolate(mfv_ctx ctx, mfv_mesh mesh, const double* dWeights, const double* dVol, double* dFace,
                     double* dBFace)
 {

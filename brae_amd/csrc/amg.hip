@@ -338,7 +338,6 @@ __global__ __launch_bounds__(kTpb) void startK(int n, const double* __restrict__
     X.set(c, fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(diag[c]) : Tv(0));
 }
 
-// coarsest grid: x = Ainv * b in FP64, ascending column per row (one workgroup)
 // sum_j row[j] * v[j] in ascending j (the dense coarsest apply's row), eight loads in flight ahead of
 // the adds
 __device__ __forceinline__ double denseRowDot(const double* row, const double* v, int n)
@@ -361,6 +360,7 @@ __device__ __forceinline__ double denseRowDot(const double* row, const double* v
     return s;
 }
 
+// coarsest grid: x = Ainv * b in FP64, ascending column per row (one workgroup)
 template <typename Tv, typename Ts>
 __global__ __launch_bounds__(kTpb) void coarseApplyK(int n, const double* __restrict__ inv, const Ts* __restrict__ b,
                                                      Ts* __restrict__ x, const float* __restrict__ xm,
@@ -917,6 +917,348 @@ __global__ __launch_bounds__(kTailThreads) void tailMixK(const TailGrid* __restr
             }
             __syncthreads();
         }
+    }
+}
+
+// ---------------- small solves batched into one launch ----------------
+//
+// A solve of a few tens of thousands of rows is latency-bound on the per-grid kernels: each of the
+// thirty-odd dependent launches of a PCG iteration costs microseconds whatever its size, and the device
+// sits mostly idle. batchPcgK runs whole solves, one workgroup each, many in one launch: every workgroup
+// takes one job -- the flexible AMG-PCG of mfv_amg_pcg_solve (OpenFOAM's normFactor and residual, the
+// Polak-Ribiere+ beta, the V-cycle in the job's precision with its FP16 input scale and, when asked,
+// the scaled correction) -- and
+// iterates it to its tolerance on its own, the steps separated by workgroup barriers and every sum
+// reduced over the workgroup in a fixed order. A job's result depends on nothing else in the launch, so
+// it is the same bits in any batch. No workgroup waits for another.
+struct BatchJob
+{
+    int n, T;                  // rows; grids of the hierarchy
+    const TailGrid* grids;     // [T], grid 0 first (as sliced rows), in the cycle's precision
+    SellView S0;               // grid 0's rows for the FP64 operator
+    const double* val0;        // its FP64 coefficients
+    const double* diag;        // the folded FP64 diagonal
+    const double* b;           // right-hand side
+    double* x;                 // solution: initial guess in, result out
+    double *r0, *r1, *p, *q;   // work vectors, n each
+    const double* inv;         // the coarsest grid's dense inverse
+    double tol, relTol;
+    int maxIter, minIter, nPre, nPost, half, flexible, scaled;
+    double* outD;              // [2]: initial and final residual
+    int* outI;                 // [2]: iterations, converged
+};
+
+// threads per batched solve: 512 leaves each 256 registers (the cycle's row sums and the FP64 operator
+// are live at once), 1024 spilled
+constexpr int kBatchThreads = 512;
+
+template <typename Tm, typename Tv, typename Ts>
+__global__ __launch_bounds__(kBatchThreads) void batchPcgK(const BatchJob* __restrict__ jobs)
+{
+    __shared__ BatchJob J;
+    __shared__ MixGrid M[kMaxTail];
+    __shared__ double lds[kBatchThreads / 64];
+    __shared__ double sb[kMaxDenseCoarsest];
+    __shared__ double shA, shB;
+    const int tid = threadIdx.x;
+    static_assert(sizeof(BatchJob) % 4 == 0, "job copied as words");
+    for (int i = tid; i < (int)(sizeof(BatchJob) / 4); i += kBatchThreads)
+        reinterpret_cast<int*>(&J)[i] = reinterpret_cast<const int*>(jobs + blockIdx.x)[i];
+    __syncthreads();
+    const int n = J.n, T = J.T;
+    if (tid < T)
+    {
+        const TailGrid& D = J.grids[tid];
+        MixGrid q;
+        q.n = D.n;
+        q.staged = 0;
+        q.S = D.S;
+        q.scale = *D.scale;
+        q.m = D.xm ? *D.xm : 1.0f;
+        q.im = D.xm ? 1.0f / *D.xm : 1.0f;
+        q.val = D.val;
+        q.diag = D.diag;
+        q.sp = nullptr;
+        q.cols = nullptr;
+        q.b = D.b; q.x = D.x; q.y = D.y; q.r = D.r; q.e = D.e;
+        q.cs32 = D.cellStart;
+        q.cl32 = D.cellList;
+        q.cs16 = nullptr;
+        q.cl16 = nullptr;
+        M[tid] = q;
+    }
+    __syncthreads();
+    const double* A = J.val0;
+    const double* Dg = J.diag;
+    double* x = J.x;
+    const double* b = J.b;
+    auto vec = [&](int g, void* p) { return CycVec<Tv, Ts>(static_cast<Ts*>(p), M[g].m, M[g].im); };
+    auto rhs = [&](void* p) { return CycVec<Tv, Ts>(static_cast<Ts*>(p), nullptr); };
+    // ---- r = b - A x, OpenFOAM's normFactor about the mean of x, the initial residual ----
+    double sx = 0.0;
+    for (int c = tid; c < n; c += kBatchThreads) sx += x[c];
+    const double xRef = blockSum<kBatchThreads>(sx, lds) ;
+    if (tid == 0) shA = xRef / (double)n;
+    __syncthreads();
+    {
+        const double xr = shA;
+        double sr = 0.0, snf = 0.0;
+        for (int c = tid; c < n; c += kBatchThreads)
+        {
+            const double ax = rowOffSum<double, double>(J.S0, c, A, PlainX<double>{x}, Dg[c] * x[c]);
+            const double rc = b[c] - ax;
+            J.r0[c] = rc;
+            const double sa = rowOffSum<double, double>(J.S0, c, A, [](int) { return 1.0; }, Dg[c]);
+            const double t = sa * xr;
+            sr += fabs(rc);
+            snf += fabs(ax - t) + fabs(b[c] - t);
+        }
+        const double a = blockSum<kBatchThreads>(sr, lds);
+        const double nfs = blockSum<kBatchThreads>(snf, lds);
+        if (tid == 0)
+        {
+            shA = a;
+            shB = nfs + 1e-20;
+        }
+        __syncthreads();
+    }
+    const double nf = shB, init = shA / nf;
+    auto converged = [&](double res) { return res < J.tol || (J.relTol > 0.0 && res < J.relTol * init); };
+    double res = init, rhoOld = 1e300;
+    int it = 0;
+    bool conv = converged(init);
+    bool stop = J.minIter <= 0 && conv;
+    double* rCur = J.r0;    // this iteration's residual; rPrev the one before (the flexible beta's r_old)
+    double* rPrev = J.r1;
+    const int fromZero = J.nPre > 0 ? 1 : 0;
+    while (!stop)
+    {
+        const double* r = rCur;
+        const double* rOld = rPrev;
+        // ---- the cycle's input: b0 = r (over its power-of-two scale in FP16), the first sweep from zero ----
+        double sig = 0.0;
+        if (J.half)
+        {
+            double mx = 0.0;
+            for (int c = tid; c < n; c += kBatchThreads) mx = fmax(mx, fabs(r[c]));
+            mx = blockMax<kBatchThreads>(mx, lds);
+            if (tid == 0) shA = mx > 0.0 ? exp2(ceil(log2(mx))) : 1.0;
+            __syncthreads();
+            sig = shA;
+        }
+        {
+            const MixGrid& G = M[0];
+            const CycVec<Tv, Ts> B = rhs(G.b), X0 = vec(0, G.x);
+            const Tv* d0 = static_cast<const Tv*>(G.diag);
+            for (int c = tid; c < n; c += kBatchThreads)
+            {
+                const Tv bv = J.half ? (Tv)(r[c] * (1.0 / sig)) : (Tv)r[c];
+                B.set(c, bv);
+                X0.set(c, fromZero ? Tv(0) + Tv(kOmega) * (bv - Tv(0)) / safeDiag(d0[c]) : Tv(0));
+            }
+            __syncthreads();
+        }
+        // ---- one V-cycle from grid 0 (tailK's steps), its result in grid 0's current x ----
+        unsigned flip = 0;
+        auto X = [&](int g) { return (flip >> g) & 1u ? M[g].y : M[g].x; };
+        auto Y = [&](int g) { return (flip >> g) & 1u ? M[g].x : M[g].y; };
+        auto inject = [&](int g, int c, Tv v) {
+            if (g == 0) return;
+            const MixGrid& F = M[g - 1];
+            const CycVec<Tv, Ts> E = vec(g - 1, F.e);
+            for (int k = F.start(c); k < F.start(c + 1); ++k) E.set(F.list(k), v);
+        };
+        auto sweep = [&](int g, bool last) {
+            const MixGrid& G = M[g];
+            const CycVec<Tv, Ts> Xg = vec(g, X(g)), Yg = vec(g, Y(g)), B = rhs(G.b);
+            for (int c = tid; c < G.n; c += kBatchThreads)
+            {
+                Tv dc;
+                const Tv ax = mixRowAx<Tm, Tv>(G, c, Xg, &dc);
+                const Tv xo = Xg(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(dc);
+                Yg.set(c, xo);
+                if (last) inject(g, c, xo);
+            }
+            __syncthreads();
+            flip ^= 1u << g;
+        };
+        for (int g = 0; g + 1 < T; ++g)
+        {
+            const MixGrid& G = M[g];
+            for (int s = 1; s < J.nPre; ++s) sweep(g, false);
+            const CycVec<Tv, Ts> Xg = vec(g, X(g)), B = rhs(G.b), Rg = rhs(G.r);
+            if (J.nPre == 0)
+            {
+                for (int c = tid; c < G.n; c += kBatchThreads) Xg.set(c, Tv(0));
+                __syncthreads();
+            }
+            for (int c = tid; c < G.n; c += kBatchThreads)
+            {
+                Tv dc;
+                Rg.set(c, B(c) - mixRowAx<Tm, Tv>(G, c, Xg, &dc));
+            }
+            __syncthreads();
+            const MixGrid& C = M[g + 1];
+            const Tv* dC = static_cast<const Tv*>(C.diag);
+            const CycVec<Tv, Ts> BC = rhs(C.b), XC = vec(g + 1, X(g + 1));
+            const int fz = (g + 2 < T && J.nPre > 0) ? 1 : 0;
+            for (int k = tid; k < C.n; k += kBatchThreads)
+            {
+                Tv s = Tv(0);
+                for (int q = G.start(k); q < G.start(k + 1); ++q) s += Rg(G.list(q));
+                BC.set(k, s);
+                if (fz) XC.set(k, Tv(0) + Tv(kOmega) * (s - Tv(0)) / safeDiag(dC[k]));
+            }
+            __syncthreads();
+        }
+        {
+            const MixGrid& G = M[T - 1];
+            const CycVec<Tv, Ts> B = rhs(G.b), Xg = vec(T - 1, X(T - 1));
+            for (int j = tid; j < G.n; j += kBatchThreads) sb[j] = (double)B(j);
+            __syncthreads();
+            for (int i = tid; i < G.n; i += kBatchThreads)
+            {
+                const double s = denseRowDot(J.inv + (size_t)i * G.n, sb, G.n);
+                Xg.set(i, (Tv)s);
+                inject(T - 1, i, (Tv)s);
+            }
+            __syncthreads();
+        }
+        const int nExtra = J.nPost > 1 ? J.nPost - 1 : 0;
+        for (int g = T - 2; g >= 0; --g)
+        {
+            const MixGrid& G = M[g];
+            const CycVec<Tv, Ts> E = vec(g, G.e);
+            if (J.scaled)
+            {
+                // the scaled correction (tailK's): alpha = (r.e)/(e.Ae) over the workgroup, fused with the
+                // first post-sweep, in place
+                const CycVec<Tv, Ts> Xg = vec(g, X(g)), AE = rhs(Y(g)), Rg = rhs(G.r);
+                double num = 0.0, den = 0.0;
+                for (int c = tid; c < G.n; c += kBatchThreads)
+                {
+                    Tv dc;
+                    const Tv ae = mixRowAx<Tm, Tv>(G, c, E, &dc);
+                    AE.set(c, ae);
+                    const double ev = (double)E(c);
+                    num += ev * (double)Rg(c);
+                    den += ev * (double)ae;
+                }
+                num = blockSum<kBatchThreads>(num, lds);
+                den = blockSum<kBatchThreads>(den, lds);
+                if (tid == 0) shA = den > 1e-30 ? num / den : 1.0;
+                __syncthreads();
+                const Tv al = (Tv)shA;
+                const Tv* dg = static_cast<const Tv*>(G.diag);
+                for (int c = tid; c < G.n; c += kBatchThreads)
+                {
+                    const Tv ev = E(c);
+                    const Tv xo = J.nPost > 0 ? Xg(c) + al * ev + Tv(kOmega) * (Rg(c) - al * AE(c)) / safeDiag(dg[c])
+                                              : Xg(c) + al * ev;
+                    Xg.set(c, xo);
+                    if (!nExtra) inject(g, c, xo);
+                }
+                __syncthreads();
+                for (int s = 0; s < nExtra; ++s) sweep(g, s + 1 == nExtra);
+            }
+            else if (J.nPost > 0)
+            {
+                const CycVec<Tv, Ts> B = rhs(G.b), Yg = vec(g, Y(g));
+                const SumX<Tv, Ts> XE{vec(g, X(g)), E};
+                for (int c = tid; c < G.n; c += kBatchThreads)
+                {
+                    Tv dc;
+                    const Tv ax = mixRowAx<Tm, Tv>(G, c, XE, &dc);
+                    const Tv xo = XE(c) + Tv(kOmega) * (B(c) - ax) / safeDiag(dc);
+                    Yg.set(c, xo);
+                    if (!nExtra) inject(g, c, xo);
+                }
+                __syncthreads();
+                flip ^= 1u << g;
+                for (int s = 0; s < nExtra; ++s) sweep(g, s + 1 == nExtra);
+            }
+            else
+            {
+                const CycVec<Tv, Ts> Xg = vec(g, X(g));
+                for (int c = tid; c < G.n; c += kBatchThreads)
+                {
+                    const Tv xo = Xg(c) + E(c);
+                    Xg.set(c, xo);
+                    inject(g, c, xo);
+                }
+                __syncthreads();
+            }
+        }
+        // ---- z.r (and z.r_old), the Polak-Ribiere+ beta ----
+        const CycVec<Tv, Ts> Z = vec(0, X(0));
+        auto z = [&](int c) { return sig != 0.0 ? (double)Z(c) * sig : (double)Z(c); };
+        double zr = 0.0, zo = 0.0;
+        for (int c = tid; c < n; c += kBatchThreads)
+        {
+            const double zc = z(c);
+            zr += zc * r[c];
+            if (J.flexible && it > 0) zo += zc * rOld[c];
+        }
+        zr = blockSum<kBatchThreads>(zr, lds);
+        zo = blockSum<kBatchThreads>(zo, lds);
+        if (tid == 0) shA = zr - zo;
+        if (tid == 0) shB = zr;
+        __syncthreads();
+        const double rho = shB;
+        double beta = 0.0;
+        if (it > 0)
+        {
+            if (J.flexible)
+                beta = (rhoOld > 1e-300 || rhoOld < -1e-300) ? fmax(0.0, shA / rhoOld) : 0.0;
+            else
+                beta = fabs(rhoOld) > 1e-300 ? rho / rhoOld : 0.0;
+        }
+        rhoOld = rho;
+        // ---- p = z + beta p, q = A p, p.q ----
+        for (int c = tid; c < n; c += kBatchThreads)
+        {
+            const double zc = z(c);
+            J.p[c] = it > 0 ? beta * J.p[c] + zc : zc;
+        }
+        __syncthreads();
+        double pq = 0.0;
+        for (int c = tid; c < n; c += kBatchThreads)
+        {
+            const double pc = J.p[c];
+            const double ax = rowOffSum<double, double>(J.S0, c, A, PlainX<double>{J.p}, Dg[c] * pc);
+            J.q[c] = ax;
+            pq += ax * pc;
+        }
+        pq = blockSum<kBatchThreads>(pq, lds);
+        if (tid == 0) shA = pq;
+        __syncthreads();
+        const double alpha = fabs(shA) > 1e-300 ? rho / shA : 0.0;
+        // ---- x += alpha p, r -= alpha q (into the other buffer), sum|r| ----
+        double* rn = rPrev;   // the next residual overwrites the one before this
+        double sr = 0.0;
+        for (int c = tid; c < n; c += kBatchThreads)
+        {
+            x[c] += alpha * J.p[c];
+            const double v = r[c] - alpha * J.q[c];
+            rn[c] = v;
+            sr += fabs(v);
+        }
+        sr = blockSum<kBatchThreads>(sr, lds);
+        if (tid == 0) shA = sr;
+        __syncthreads();
+        res = shA / nf;
+        rPrev = rCur;
+        rCur = rn;
+        ++it;
+        conv = converged(res);
+        stop = ((conv || it >= J.maxIter) && it >= J.minIter);
+    }
+    if (tid == 0)
+    {
+        J.outD[0] = init;
+        J.outD[1] = res;
+        J.outI[0] = it;
+        J.outI[1] = conv ? 1 : 0;
     }
 }
 
@@ -1830,6 +2172,38 @@ void buildTailLds(Ctx& c, Amg& a)
     a.tailLdsGrid = a.tailStart + TG;
 }
 
+// grid g's pointers and sizes as the one-workgroup kernels read them, in the current precision
+TailGrid tailGridOf(Amg& a, int g)
+{
+    Grid& G = a.grids[g];
+    TailGrid t{};
+    t.S = viewOf(G.sell);
+    t.n = G.n;
+    t.scale = a.scales.p + g;
+    t.cellStart = G.cellStart.p;
+    t.cellList = G.cellList.p;
+    if (a.prec == Prec::F64)
+    {
+        t.val = G.off64.p;
+        t.diag = G.diagV64.p;
+        t.b = G.b64.p; t.x = G.x64.p; t.y = G.y64.p; t.r = G.r64.p; t.e = G.e64.p;
+    }
+    else if (a.prec == Prec::F32)
+    {
+        t.val = G.off32.p;
+        t.diag = G.diagV32.p;
+        t.b = G.b32.p; t.x = G.x32.p; t.y = G.y32.p; t.r = G.r32.p; t.e = G.e32.p;
+    }
+    else
+    {
+        t.val = G.off16.p;
+        t.diag = G.diagV32.p;
+        t.b = G.b16.p; t.x = G.x16.p; t.y = G.y16.p; t.r = G.r16.p; t.e = G.e16.p;
+        t.xm = a.xmul.p + g;
+    }
+    return t;
+}
+
 void buildTail(Ctx& c, Amg& a)
 {
     if (a.tailFor == (int)a.prec) return;
@@ -1842,42 +2216,19 @@ void buildTail(Ctx& c, Amg& a)
             a.tailStart = g;
             break;
         }
+    // every grid's descriptor, for the batched whole solves (from grid 0)
+    {
+        std::vector<TailGrid> all;
+        for (int g = 0; g < nG; ++g) all.push_back(tailGridOf(a, g));
+        a.cycleDesc.upload(reinterpret_cast<const uint8_t*>(all.data()), all.size() * sizeof(TailGrid), c.stream);
+    }
     if (!a.tailStart || nG - a.tailStart > kMaxTail)
     {
         a.tailStart = 0;
         return;
     }
     std::vector<TailGrid> d;
-    for (int g = a.tailStart; g < nG; ++g)
-    {
-        Grid& G = a.grids[g];
-        TailGrid t{};
-        t.S = viewOf(G.sell);
-        t.n = G.n;
-        t.scale = a.scales.p + g;
-        t.cellStart = G.cellStart.p;
-        t.cellList = G.cellList.p;
-        if (a.prec == Prec::F64)
-        {
-            t.val = G.off64.p;
-            t.diag = G.diagV64.p;
-            t.b = G.b64.p; t.x = G.x64.p; t.y = G.y64.p; t.r = G.r64.p; t.e = G.e64.p;
-        }
-        else if (a.prec == Prec::F32)
-        {
-            t.val = G.off32.p;
-            t.diag = G.diagV32.p;
-            t.b = G.b32.p; t.x = G.x32.p; t.y = G.y32.p; t.r = G.r32.p; t.e = G.e32.p;
-        }
-        else
-        {
-            t.val = G.off16.p;
-            t.diag = G.diagV32.p;
-            t.b = G.b16.p; t.x = G.x16.p; t.y = G.y16.p; t.r = G.r16.p; t.e = G.e16.p;
-            t.xm = a.xmul.p + g;
-        }
-        d.push_back(t);
-    }
+    for (int g = a.tailStart; g < nG; ++g) d.push_back(tailGridOf(a, g));
     a.tailDesc.upload(reinterpret_cast<const uint8_t*>(d.data()), d.size() * sizeof(TailGrid), c.stream);
     hipCheck(hipStreamSynchronize(c.stream), "tail descriptors");
     buildTailLds(c, a);
@@ -2114,6 +2465,94 @@ void sellAmul64(Ctx& c, const Grid& g, const double* diag, const double* x, doub
 {
     sellAmul64K<<<blocks(g.n), kTpb, 0, c.stream>>>(viewOf(g.sell), g.off64.p, diag, x, y);
     hipCheck(hipGetLastError(), "sell amul");
+}
+
+// The batch's solves, grouped by cycle precision, one launch of batchPcgK per group on the context's
+// stream (one workgroup per solve); their work vectors on each hierarchy. Synchronises.
+void amgBatchSolve(Ctx& c, std::vector<BatchSolve>& solves)
+{
+    for (BatchSolve& s : solves)
+    {
+        Amg& a = *s.a;
+        if (!a.galerkinDone || a.prec != s.pc.prec)
+            invalid("batch solve: the hierarchy's Galerkin values are not in the solve's precision");
+        if ((int)a.grids.size() > kMaxTail || (int)a.grids.size() < 2 || a.grids.back().n > kMaxDenseCoarsest)
+            throw Error(kErrUnsupported, "mfv: batch solve: the hierarchy is too deep or its coarsest grid too large");
+        if (a.grids[0].n > (1 << 22)) invalid("batch solve: a system above 4 M cells belongs to the per-grid kernels");
+        if (a.batchWork.n != 4 * (size_t)a.grids[0].n) a.batchWork.resize(4 * (size_t)a.grids[0].n);
+    }
+    const int nAll = (int)solves.size();
+    if (!nAll) return;
+    {
+        std::vector<const Amg*> seen;
+        for (const BatchSolve& s : solves) seen.push_back(s.a);
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+            invalid("batch solve: a hierarchy appears twice in one batch (its cycle vectors are shared)");
+    }
+    DBuf<double> outD(2 * (size_t)nAll);
+    DBuf<int> outI(2 * (size_t)nAll);
+    std::vector<DBuf<uint8_t>> jobBufs;
+    for (Prec prec : {Prec::F64, Prec::F32, Prec::F16})
+    {
+        std::vector<BatchJob> jobs;
+        for (int k = 0; k < nAll; ++k)
+        {
+            BatchSolve& s = solves[k];
+            if (s.pc.prec != prec) continue;
+            Amg& a = *s.a;
+            const Grid& g0 = a.grids[0];
+            const size_t n = (size_t)g0.n;
+            BatchJob j{};
+            j.n = g0.n;
+            j.T = (int)a.grids.size();
+            j.grids = reinterpret_cast<const TailGrid*>(a.cycleDesc.p);
+            j.S0 = viewOf(g0.sell);
+            j.val0 = g0.off64.p;
+            j.diag = s.diag;
+            j.b = s.b;
+            j.x = s.psi;
+            j.r0 = a.batchWork.p;
+            j.r1 = a.batchWork.p + n;
+            j.p = a.batchWork.p + 2 * n;
+            j.q = a.batchWork.p + 3 * n;
+            j.inv = a.coarseInv.p;
+            j.tol = s.pc.tolerance;
+            j.relTol = s.pc.relTol;
+            j.maxIter = s.pc.maxIter;
+            j.minIter = s.pc.minIter;
+            j.nPre = s.pc.nPre;
+            j.nPost = s.pc.nPost;
+            j.half = prec == Prec::F16 ? 1 : 0;
+            j.flexible = s.pc.flexible ? 1 : 0;
+            j.scaled = s.pc.scaleCorrection ? 1 : 0;
+            j.outD = outD.p + 2 * k;
+            j.outI = outI.p + 2 * k;
+            jobs.push_back(j);
+        }
+        if (jobs.empty()) continue;
+        jobBufs.emplace_back();
+        jobBufs.back().upload(reinterpret_cast<const uint8_t*>(jobs.data()), jobs.size() * sizeof(BatchJob), c.stream);
+        const BatchJob* dj = reinterpret_cast<const BatchJob*>(jobBufs.back().p);
+        const int nj = (int)jobs.size();
+        if (prec == Prec::F64)
+            batchPcgK<double, double, double><<<nj, kBatchThreads, 0, c.stream>>>(dj);
+        else if (prec == Prec::F32)
+            batchPcgK<float, float, float><<<nj, kBatchThreads, 0, c.stream>>>(dj);
+        else
+            batchPcgK<__half, float, __half><<<nj, kBatchThreads, 0, c.stream>>>(dj);
+        hipCheck(hipGetLastError(), "batch solve");
+    }
+    const std::vector<double> hd = outD.download(c.stream);
+    const std::vector<int> hi = outI.download(c.stream);
+    for (int k = 0; k < nAll; ++k)
+    {
+        PcgPerf& p = solves[k].perf;
+        p.initialResidual = hd[2 * k];
+        p.finalResidual = hd[2 * k + 1];
+        p.nIterations = hi[2 * k];
+        p.converged = hi[2 * k + 1] != 0;
+    }
 }
 
 } // namespace mfv

@@ -151,6 +151,8 @@ struct Amg
     DBuf<uint8_t> tailLdsDesc;     // its descriptor
     int tailLdsBytes = 0;          // LDS it needs (0: the tail does not fit; tailK runs)
     int tailLdsGrid = 0;           // the first grid staged in LDS (the tail's grids before it are read in place)
+    DBuf<uint8_t> cycleDesc;       // every grid's descriptor from grid 0 (batched whole solves)
+    DBuf<double> batchWork;        // a batched solve's r, r_old, p, q
     bool galerkinDone = false;
     bool allocatedVectors = false;
     Prec vectorsFor = Prec::F32;
@@ -289,6 +291,19 @@ PcgPerf gamgRefSolve(Ctx& c, Mesh& m, Amg& a, const double* diag, const double* 
 // solve's. Synchronises once per iteration.
 PcgPerf pcgGamgSolve(Ctx& c, Mesh& m, Amg& a, const double* diag, const double* upper, const double* b, double* psi,
                      double tolerance, double relTol, int maxIter, int minIter, const GamgControls& g);
+
+// Whole AMG-PCG solves batched into one launch, one workgroup each (each result is the same bits in any
+// batch). Every hierarchy's Galerkin values in its solve's precision.
+struct BatchSolve
+{
+    Amg* a = nullptr;
+    const double* diag = nullptr;   // the folded FP64 diagonal (the upper is the one Galerkin saw)
+    const double* b = nullptr;
+    double* psi = nullptr;          // initial guess in, solution out
+    PcgControls pc;
+    PcgPerf perf;                   // out
+};
+void amgBatchSolve(Ctx& c, std::vector<BatchSolve>& solves);
 
 // AMG-preconditioned CG on the folded fine matrix (diag, upper symmetric). psi updated in place.
 // normFactor: device scalar, or null to compute OpenFOAM's normFactor here.

@@ -1,13 +1,16 @@
 // The serving loop: a stream of pressure-Poisson requests on hex boxes of uneven sizes, admitted by an
 // HBM budget into a fixed number of slots, each slot a host thread with its own context (HIP stream),
 // so admitted requests run concurrently on the device. A request keeps its mesh, matrix and AMG
-// hierarchy resident across its sequence of right-hand sides and releases them at the end. There is
-// no counterpart in the reference (its solvers run one case per process); the request shape follows
-// its pressure corrector: 7-point FP64 laplacian on the box, zeroGradient walls with a reference cell,
+// hierarchy resident across its sequence of right-hand sides; with resident systems a slot keeps them
+// after the request for the next one on the same box. Small requests can hand their solves to a
+// batcher that runs every waiting solve whole, one workgroup each, in one launch. There is no
+// counterpart in the reference (its solvers run one case per process); the request shape follows its
+// pressure corrector: 7-point FP64 laplacian on the box, zeroGradient walls with a reference cell,
 // right-hand side = -div of a seeded random velocity, mixed-precision AMG-PCG.
 //
 // Results do not depend on what else is running: every kernel's arithmetic order is fixed by its
-// inputs (no atomics, per-context scratch), so a request's solution is the same bits alone or in a mix.
+// inputs (no atomics, per-context scratch, one workgroup per batched solve), so a request's solution is
+// the same bits alone or in a mix.
 #include "mfv.h"
 #include "mfv_internal.hpp"
 #include "reduce.hpp"
@@ -268,9 +271,74 @@ std::unique_ptr<System> buildSystem(mfv_ctx ctx, const mfv_serve_request& q, int
     return S;
 }
 
+// The serving loop's batcher: slots hand it the solves of their small requests (right-hand side ready,
+// their stream idle), and it launches all that are waiting as one batch of whole solves on its own
+// context (mfv_amg_pcg_solve_batch, one workgroup each); solves that arrive meanwhile go in the next
+// launch. A solve's result is the same bits in any batch.
+struct Batcher
+{
+    struct Job
+    {
+        mfv_batch_solve solve;
+        mfv_perf perf{};
+        int status = MFV_OK;
+        std::string error;
+        bool done = false;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<Job*> waiting;
+    bool quit = false;
+    mfv_ctx ctx = nullptr;
+    long long launches = 0, solves = 0;
+
+    // a slot's call: blocks until its solve is done
+    void solve(Job& j)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        waiting.push_back(&j);
+        cv.notify_all();
+        cv.wait(lk, [&] { return j.done; });
+    }
+    void run()
+    {
+        for (;;)
+        {
+            std::vector<Job*> batch;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return quit || !waiting.empty(); });
+                if (waiting.empty()) return;
+                batch.swap(waiting);
+            }
+            std::vector<mfv_batch_solve> in(batch.size());
+            std::vector<mfv_perf> out(batch.size());
+            for (size_t k = 0; k < batch.size(); ++k) in[k] = batch[k]->solve;
+            const int rc = mfv_amg_pcg_solve_batch(ctx, (int)in.size(), in.data(), out.data());
+            const std::string err = rc == MFV_OK ? std::string() : std::string(mfv_last_error(ctx));
+            {
+                std::lock_guard<std::mutex> g(mu);
+                ++launches;
+                solves += (long long)batch.size();
+                for (size_t k = 0; k < batch.size(); ++k)
+                {
+                    batch[k]->perf = out[k];
+                    batch[k]->status = rc;
+                    batch[k]->error = err;
+                    batch[k]->done = true;
+                }
+            }
+            cv.notify_all();
+        }
+    }
+};
+
 // the request's steps on a built system: from p = 0, each step's right-hand side -div(U) of a seeded
-// velocity, the solve warm-started from the previous step's p, the solution's fingerprint
-void solveSteps(mfv_ctx ctx, System& S, const mfv_serve_request& q, int precision, mfv_serve_result& res)
+// velocity, the solve warm-started from the previous step's p, the solution's fingerprint. With a
+// batcher the solves go through it (the same cycle, one workgroup each), else through the per-grid
+// kernels.
+void solveSteps(mfv_ctx ctx, System& S, const mfv_serve_request& q, int precision, mfv_serve_result& res,
+                Batcher* batcher = nullptr)
 {
     const int nC = S.nC, nI = S.nI;
     hipStream_t st = static_cast<hipStream_t>(mfv_ctx_stream(ctx));
@@ -297,7 +365,17 @@ void solveSteps(mfv_ctx ctx, System& S, const mfv_serve_request& q, int precisio
         hipCheck(hipMemsetAsync(S.b.p, 0, sizeof(double) * nC, st), "rhs clear");
         check(mfv_div_source(ctx, S.mesh, S.phi.p, S.zeroB.p, S.V.p, -1.0, S.b.p), ctx, "div");
         mfv_perf perf{};
-        check(mfv_amg_pcg_solve(ctx, S.mesh, S.amg, &A, S.b.p, S.p.p, nullptr, &pc, &perf), ctx, "solve");
+        if (batcher)
+        {
+            hipCheck(hipStreamSynchronize(st), "rhs sync");
+            Batcher::Job j;
+            j.solve = mfv_batch_solve{S.mesh, S.amg, S.diag.p, S.b.p, S.p.p, pc};
+            batcher->solve(j);
+            if (j.status != MFV_OK) throw Error(j.status, "mfv: serve: batched solve: " + j.error);
+            perf = j.perf;
+        }
+        else
+            check(mfv_amg_pcg_solve(ctx, S.mesh, S.amg, &A, S.b.p, S.p.p, nullptr, &pc, &perf), ctx, "solve");
         res.totalIterations += perf.nIterations;
         res.finalResidual = perf.finalResidual;
         bitsSumK<<<nParts, kTpb, 0, st>>>(nC, S.p.p, S.parts.p);
@@ -322,6 +400,7 @@ struct mfv_server_s
     double budget = 0;
     double bytesPerCell = 600.0;
     double resident = 0;   // bytes of the budget that hold systems between requests (0: none kept)
+    int batchCells = 0;    // requests of at most this many cells solve in batches (0: none)
 };
 
 extern "C" {
@@ -345,6 +424,13 @@ int mfv_server_set_resident(mfv_server s, double residentBytes)
 {
     if (!s || !(residentBytes >= 0) || residentBytes >= s->budget) return MFV_ERR_INVALID;
     s->resident = residentBytes;
+    return MFV_OK;
+}
+
+int mfv_server_set_batch(mfv_server s, int maxCells)
+{
+    if (!s || maxCells < 0) return MFV_ERR_INVALID;
+    s->batchCells = maxCells;
     return MFV_OK;
 }
 
@@ -385,6 +471,14 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
     std::map<BoxKey, int> owner;
     std::vector<char> idle(s->nSlots, 0);
     const double perSlot = s->resident / s->nSlots;
+    // the batcher of the small requests' solves, on a context of its own
+    Batcher batcher;
+    std::thread batchThread;
+    if (s->batchCells > 0)
+    {
+        if (mfv_ctx_create(s->device, nullptr, &batcher.ctx) != MFV_OK) return MFV_ERR_HIP;
+        batchThread = std::thread([&] { batcher.run(); });
+    }
     auto worker = [&](int slot) {
         mfv_ctx ctx = nullptr;
         if (mfv_ctx_create(s->device, nullptr, &ctx) != MFV_OK)
@@ -453,7 +547,9 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
                     }
                 results[i].resident = sys ? 1 : 0;
                 if (!sys) sys = buildSystem(ctx, reqs[i], s->precision, bytes);
-                solveSteps(ctx, *sys, reqs[i], s->precision, results[i]);
+                const bool small = s->batchCells > 0 && (double)reqs[i].nx * reqs[i].ny * reqs[i].nz <= s->batchCells;
+                results[i].batched = small ? 1 : 0;
+                solveSteps(ctx, *sys, reqs[i], s->precision, results[i], small ? &batcher : nullptr);
                 if (sys->bytes <= perSlot)
                 {
                     // keep it, the least recently used going first when the slot's share is full
@@ -510,6 +606,16 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
     std::vector<std::thread> threads;
     for (int k = 0; k < s->nSlots; ++k) threads.emplace_back(worker, k);
     for (auto& t : threads) t.join();
+    if (batchThread.joinable())
+    {
+        {
+            std::lock_guard<std::mutex> g(batcher.mu);
+            batcher.quit = true;
+        }
+        batcher.cv.notify_all();
+        batchThread.join();
+        mfv_ctx_destroy(batcher.ctx);
+    }
     return firstError;
 }
 

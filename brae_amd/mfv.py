@@ -42,6 +42,10 @@ class PcgControls(ctypes.Structure):
                 ("nVcycles", I), ("scaleCorrection", I), ("captureVcycle", I)]
 
 
+class BatchSolve(ctypes.Structure):
+    _fields_ = [("mesh", P), ("amg", P), ("dDiag", P), ("dB", P), ("dPsi", P), ("controls", PcgControls)]
+
+
 AGG_BRAE, AGG_FACEAREAPAIR, AGG_PAIR = 0, 1, 2
 P_AMG_PCG, P_GAMG_REF = 0, 1
 LAYOUT_AUTO, LAYOUT_ROWS = 0, 1
@@ -86,7 +90,8 @@ class ServeRequest(ctypes.Structure):
 
 class ServeResult(ctypes.Structure):
     _fields_ = [("arrival", D), ("admitted", D), ("finished", D), ("status", I), ("nCells", I), ("nSteps", I),
-                ("totalIterations", I), ("finalResidual", D), ("fingerprint", ctypes.c_ulonglong), ("resident", I)]
+                ("totalIterations", I), ("finalResidual", D), ("fingerprint", ctypes.c_ulonglong), ("resident", I),
+                ("batched", I)]
 
 
 class Perf(ctypes.Structure):
@@ -182,6 +187,7 @@ _SIGS = {
     "mfv_amg_grid_size": [P, I, P, P],
     "mfv_amg_fine_stencil": [P, P, P],
     "mfv_amg_coarse_mode": [P, I, P, P, P],
+    "mfv_amg_pcg_solve_batch": [P, I, P, P],
     "mfv_fold_boundary_diag": [P, P, P, P, P, P, P],
     "mfv_reciprocal_v": [P, P, P, P, P],
     "mfv_boundary_diag_term": [P, P, P, P, P, I, P],
@@ -231,15 +237,25 @@ _SIGS = {
     "mfv_server_create": [I, I, D, I, P],
     "mfv_server_destroy": [P],
     "mfv_server_set_resident": [P, D],
+    "mfv_server_set_batch": [P, I],
     "mfv_server_request_bytes": [P, P],
     "mfv_server_run": [P, P, I, P],
 }
 _RESTYPES = {"mfv_last_error": ctypes.c_char_p, "mfv_ctx_stream": P, "mfv_server_request_bytes": D}
 
 
-def serve(requests, nSlots=8, hbmBudgetBytes=200e9, precision=32, device=0, residentBytes=0.0):
+def solve_batch(ctx, jobs):
+    """whole AMG-PCG solves batched into one launch (Amg.solve_batch_job entries); returns their Perf"""
+    arr = (BatchSolve * len(jobs))(*jobs)
+    perfs = (Perf * len(jobs))()
+    ctx.call("mfv_amg_pcg_solve_batch", len(jobs), arr, perfs)
+    return list(perfs)
+
+
+def serve(requests, nSlots=8, hbmBudgetBytes=200e9, precision=32, device=0, residentBytes=0.0, batchCells=0):
     """run a request stream (list of ServeRequest or dicts of its fields) through the serving loop, with
-    residentBytes of the budget keeping systems between requests; returns (status, [ServeResult])"""
+    residentBytes of the budget keeping systems between requests and requests of at most batchCells cells
+    solved in batches; returns (status, [ServeResult])"""
     reqs = (ServeRequest * len(requests))()
     for i, r in enumerate(requests):
         if isinstance(r, dict):
@@ -257,6 +273,10 @@ def serve(requests, nSlots=8, hbmBudgetBytes=200e9, precision=32, device=0, resi
             rc = lib().mfv_server_set_resident(srv, float(residentBytes))
             if rc != MFV_OK:
                 raise MfvError("server_set_resident failed (%d)" % rc)
+        if batchCells:
+            rc = lib().mfv_server_set_batch(srv, int(batchCells))
+            if rc != MFV_OK:
+                raise MfvError("server_set_batch failed (%d)" % rc)
         rc = lib().mfv_server_run(srv, reqs, len(requests), res)
     finally:
         lib().mfv_server_destroy(srv)
@@ -540,6 +560,14 @@ class Amg:
         self.ctx.call("mfv_amg_pcg_solve", self.mesh.h, self.h, ctypes.byref(A), ptr(b), ptr(psi),
                       ptr(normFactor), ctypes.byref(c), ctypes.byref(perf))
         return perf
+
+    def solve_batch_job(self, diag, b, psi, **kw):
+        """one entry of a batched solve (solve_batch) on this hierarchy"""
+        c = PcgControls()
+        lib().mfv_pcg_default_controls(ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return BatchSolve(self.mesh.h, self.h, ptr(diag), ptr(b), ptr(psi), c)
 
     def gamg_solve(self, diag, upper, b, psi, **kw):
         """OpenFOAM's GAMGSolver on the folded system (psi in place)."""

@@ -405,6 +405,25 @@ int mfv_pcg_default_controls(mfv_pcg_controls* controls);
 int mfv_amg_pcg_solve(mfv_ctx ctx, mfv_mesh mesh, mfv_amg amg, const mfv_ldu* A, const double* dB, double* dPsi,
                       const double* dNormFactor, const mfv_pcg_controls* controls, mfv_perf* perf);
 
+/* Small solves batched into one launch, no counterpart in the reference (a solve of tens of thousands
+ * of cells is latency-bound on the per-grid kernels): every solve runs whole in one workgroup -- the
+ * AMG-PCG of mfv_amg_pcg_solve with OpenFOAM's normFactor, the V-cycle in controls.precision with or
+ * without the scaled correction -- and all of them in the same launch, each to its own tolerance. A
+ * solve's result is the same bits in any batch. Each hierarchy appears once per call, its Galerkin done
+ * in that precision; checkEvery/captureVcycle are ignored. The meshes and
+ * hierarchies may belong to other contexts on ctx's device (the launch runs on ctx's stream; their own
+ * streams must have nothing in flight on them meanwhile). Synchronises. */
+typedef struct
+{
+    mfv_mesh mesh;
+    mfv_amg amg;
+    const double* dDiag;   /* the folded FP64 diagonal; the upper is the one the hierarchy's Galerkin saw */
+    const double* dB;
+    double* dPsi;          /* initial guess in, solution out */
+    mfv_pcg_controls controls;
+} mfv_batch_solve;
+int mfv_amg_pcg_solve_batch(mfv_ctx ctx, int n, const mfv_batch_solve* solves, mfv_perf* perfs);
+
 /* ---------------------------------------------------------------------------------------------
  * OpenFOAM's sequential sweeps, level-scheduled on the device and equal to the host loops to the bit.
  * ------------------------------------------------------------------------------------------- */
@@ -665,6 +684,7 @@ typedef struct
     double finalResidual;                 /* of the last step */
     unsigned long long fingerprint;       /* of every step's solution bits */
     int resident;                         /* 1: solved on a system a slot kept from an earlier request */
+    int batched;                          /* 1: its solves ran in batches (mfv_server_set_batch) */
 } mfv_serve_result;
 int mfv_server_create(int device, int nSlots, double hbmBudgetBytes, int precision, mfv_server* out);
 int mfv_server_destroy(mfv_server server);
@@ -672,6 +692,11 @@ int mfv_server_destroy(mfv_server server);
  * requests, an equal share per slot, least recently used dropped first; held back from admission for
  * the whole run. 0 (the default) keeps none. */
 int mfv_server_set_resident(mfv_server server, double residentBytes);
+/* requests of at most maxCells cells solve in batches: each slot hands its solve (right-hand side ready)
+ * to the server's batcher, which launches every solve waiting as one mfv_amg_pcg_solve_batch (one
+ * workgroup each, the same cycle) and the next arrivals in the next launch. 0 (the default): every
+ * request on the per-grid kernels. */
+int mfv_server_set_batch(mfv_server server, int maxCells);
 /* the HBM estimate admission charges a request */
 double mfv_server_request_bytes(mfv_server server, const mfv_serve_request* request);
 /* runs the stream to the end; returns MFV_OK or the first request error (each result has its status) */

@@ -87,3 +87,29 @@ def test_resident_systems_between_requests(precision):
     assert rc == mfv.MFV_OK and not any(r.resident for r in tiny)
     with pytest.raises(mfv.MfvError):
         mfv.serve(stream[:1], nSlots=1, hbmBudgetBytes=1e9, precision=precision, residentBytes=2e9)
+
+
+@pytest.mark.parametrize("precision", [32, 16])
+def test_batched_small_requests(precision):
+    """Requests under the batch size solve in batches of whole solves (one workgroup each): every one
+    converges, each is the same bits alone (a batch of its own) and in the mix (batched with others), in
+    about as many iterations as on the per-grid kernels."""
+    print("case: batched serving mix of", len(MIX), "precision", precision, flush=True)
+    rc, mixed = mfv.serve(MIX, nSlots=6, hbmBudgetBytes=50e9, precision=precision, batchCells=200000)
+    assert rc == mfv.MFV_OK
+    for r in mixed:
+        assert r.status == 0 and r.batched == 1 and r.finalResidual < 1e-6 and r.totalIterations > 0
+    rc, plain = mfv.serve(MIX, nSlots=4, hbmBudgetBytes=50e9, precision=precision)
+    assert rc == mfv.MFV_OK
+    for b, p in zip(mixed, plain):
+        assert p.batched == 0
+        # the same cycle, its sums reduced in another order (and, in FP16, rounded at other points): about
+        # as many iterations
+        slack = 2 * b.nSteps if precision == 32 else max(2 * b.nSteps, 0.25 * p.totalIterations)
+        assert abs(b.totalIterations - p.totalIterations) <= slack, (b.totalIterations, p.totalIterations)
+    for i in range(len(MIX)):
+        rc, alone = mfv.serve([dict(MIX[i], arrival=0.0)], nSlots=1, hbmBudgetBytes=50e9, precision=precision,
+                              batchCells=200000)
+        assert rc == mfv.MFV_OK and alone[0].batched == 1
+        assert alone[0].fingerprint == mixed[i].fingerprint, i
+        assert alone[0].totalIterations == mixed[i].totalIterations
