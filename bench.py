@@ -75,6 +75,8 @@ def parse():
     ap.add_argument("--turbulence", type=int, default=1, help="1: k-epsilon with wall functions; 0: laminar")
     ap.add_argument("--tol-u", type=float, default=1e-5, help="momentum solve tolerance")
     ap.add_argument("--reltol-u", type=float, default=0.1, help="momentum solve relative tolerance")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="also time this many small solves at once, batched and on the per-grid kernels")
     ap.add_argument("--c5", action="store_true",
                     help="also time the largest single-GPU case: the 800x400x375 channel (120 M cells), turbulent SIMPLE "
                          "iterations with the FP32 and the FP64 cycle, and the peak HBM in use (adds a few minutes)")
@@ -537,6 +539,71 @@ def predictor_run(ch, ctx, patches, reps=5):
                         % ch.shape}
 
 
+def batch_run(args):
+    """Many small solves at once: --batch hierarchies of each of three graded, jittered channels (8,192,
+    16,000 and 65,536 cells) on one context, the FP16 V(2,1) scaled cycle to 1e-6 from p = 0, solved in
+    one mfv_amg_pcg_solve_batch launch (one workgroup per solve) and one after another on the per-grid
+    kernels."""
+    import torch
+    from brae_amd import hexmesh, mfv
+    ctx = mfv.Context(0)
+    out = []
+    ctl = dict(tolerance=1e-6, flexible=1, nPreSweeps=2, nPostSweeps=1, scaleCorrection=1)
+    for nx, ny, nz in [(32, 16, 16), (40, 25, 16), (64, 32, 32)]:
+        m = hexmesh.box_mesh(nx, ny, nz, (4.0, 1.0, 0.5), (1.0, 20.0, 20.0), 0.1, seed=3)
+        nI, nC = m["nInternal"], m["nCells"]
+        own, nei, bfc = (np.ascontiguousarray(a, np.int32) for a in (m["owner"][:nI], m["neighbour"], m["owner"][nI:]))
+        # -laplacian(1) held positive on the mesh, the outlet fixed at 0, a seeded source
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to("cuda:0")
+        dc = dev(m["deltaCoeffs"])
+        magSf = dev(m["magSf"][:nI])
+        meshes, amgs = [], []
+        for _ in range(args.batch):
+            meshes.append(mfv.Mesh(ctx, nC, own, nei, bfc))
+        D, U = torch.zeros(nC, dtype=torch.float64, device="cuda:0"), torch.zeros(nI, dtype=torch.float64,
+                                                                                    device="cuda:0")
+        gam = -torch.ones(nI, dtype=torch.float64, device="cuda:0")
+        ctx.call("mfv_laplacian_coeffs", meshes[0].h, mfv.ptr(dc), mfv.ptr(gam), mfv.ptr(magSf), mfv.ptr(D),
+                 mfv.ptr(U), None)
+        D[0] += D[0]   # a reference cell: the walls leave the matrix singular
+        src = dev(np.random.default_rng(5).normal(size=nC) * m["V"])
+        src -= src.mean()
+        for mesh in meshes:
+            a = mfv.Amg(mesh, np.asarray(m["magSf"][:nI]), mfv.AGG_BRAE, 64)
+            a.galerkin(D, U, mfv.PREC_FP16)
+            amgs.append(a)
+        psis = [torch.zeros(nC, dtype=torch.float64, device="cuda:0") for _ in amgs]
+        jobs = [a.solve_batch_job(D, src, p, precision=mfv.PREC_FP16, **ctl) for a, p in zip(amgs, psis)]
+        mfv.solve_batch(ctx, jobs[:2])   # first use
+        for p in psis:
+            p.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        perfs = mfv.solve_batch(ctx, jobs)
+        tb = time.perf_counter() - t
+        amgs[0].solve(D, U, src, psis[0], precision=mfv.PREC_FP16, checkEvery=4, **ctl)   # first use
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        its = []
+        for a, p in zip(amgs, psis):
+            p.zero_()
+            its.append(a.solve(D, U, src, p, precision=mfv.PREC_FP16, checkEvery=4, **ctl).nIterations)
+        torch.cuda.synchronize()
+        tk = time.perf_counter() - t
+        out.append(dict(cells=nC, solves=len(amgs), batch_ms=round(tb * 1e3, 1), per_grid_ms=round(tk * 1e3, 1),
+                        iterations_batch=perfs[0].nIterations, iterations_per_grid=its[0],
+                        max_final_residual=max(q.finalResidual for q in perfs)))
+        log("batch: %d solves of %d cells: %.1f ms batched, %.1f ms one after another" % (len(amgs), nC, tb * 1e3,
+                                                                                          tk * 1e3))
+        for a in amgs:
+            a.close()
+        for mesh in meshes:
+            mesh.close()
+    ctx.close()
+    return dict(workload="small FP16 AMG-PCG solves at once, batched (one workgroup each) vs the per-grid "
+                         "kernels in turn", runs=out)
+
+
 def serving_run(args):
     """The serving stream: Poisson arrivals at --serve-rate of pressure-Poisson requests on boxes of
     log-uniform 32k-4M cells, x-aspect uniform 1-8, 30% with 10%-jittered points, 1-20 right-hand sides
@@ -773,6 +840,7 @@ def main():
         print(json.dumps(dict(r["profile_simple"], size=args.size, v_cycle=args.precision)), flush=True)
         return
     c5 = c5_run(args) if args.c5 else None
+    batch = batch_run(args) if args.batch > 0 else None
     serving = serving_run(args) if args.serving > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, nx * ny * nz)
     sim = r.get("simple")
@@ -839,6 +907,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "c5": c5,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 
