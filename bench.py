@@ -629,6 +629,9 @@ def serving_run(args):
     if rc != mfv.MFV_OK:
         raise RuntimeError("serving run failed (%d)" % rc)
     lat = np.array([r.finished - r.arrival for r in res])
+    wait = np.array([r.admitted - r.arrival for r in res])
+    setup = np.array([r.built - r.admitted for r in res])
+    solve = np.array([r.finished - r.built for r in res])
     solves = sum(r.nSteps for r in res)
     makespan = max(r.finished for r in res) - min(r.arrival for r in res)
     log("serving: %d requests, %d solves in %.1f s; latency p50 %.3f s p99 %.3f s" % (
@@ -639,6 +642,9 @@ def serving_run(args):
                 wall_s=round(wall, 2), arrival_rate_per_s=args.serve_rate, slots=args.serve_slots,
                 hbm_budget_gb=200, max_residual=max(r.finalResidual for r in res),
                 resident_gb=getattr(args, "serve_resident", 0.0), resident_hits=int(sum(r.resident for r in res)),
+                latency_parts_s={name: dict(p50=round(float(np.percentile(v, 50)), 4),
+                                            p99=round(float(np.percentile(v, 99)), 4))
+                                 for name, v in (("admission_wait", wait), ("system_build", setup), ("solves", solve))},
                 workload="pressure Poisson on hex boxes, log-uniform 32k-4M cells, x-aspect 1-8, 30%% jittered, "
                          "1-20 RHS per request, %s V-cycle AMG-PCG to 1e-6" % args.precision.upper())
 

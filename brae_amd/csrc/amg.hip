@@ -1534,7 +1534,8 @@ void buildSell(Ctx& c, Sell& S, int n, const std::vector<int>& own, const std::v
     std::vector<uint8_t> pat(n, 0);
     std::map<std::vector<int>, int> patterns;
     bool patterned = true;
-    std::vector<int> offs;
+    std::vector<int> offs, lastOffs;
+    int lastPat = -1;
     for (int r = 0; r < n; ++r)
     {
         const int s = r / kSlice, lane = r % kSlice;
@@ -1559,15 +1560,26 @@ void buildSell(Ctx& c, Sell& S, int n, const std::vector<int>& own, const std::v
         for (; k < w; ++k) cols[slicePtr[s] + k * kSlice + lane] = r;
         if (patterned)
         {
-            auto it = patterns.find(offs);
-            if (it == patterns.end())
+            // neighbouring rows mostly share a pattern: the previous row's is tried before the map
+            if (lastPat >= 0 && offs == lastOffs)
+                pat[r] = (uint8_t)lastPat;
+            else
             {
-                if (patterns.size() >= 255)
-                    patterned = false;
-                else
-                    it = patterns.emplace(offs, (int)patterns.size()).first;
+                auto it = patterns.find(offs);
+                if (it == patterns.end())
+                {
+                    if (patterns.size() >= 255)
+                        patterned = false;
+                    else
+                        it = patterns.emplace(offs, (int)patterns.size()).first;
+                }
+                if (patterned)
+                {
+                    pat[r] = (uint8_t)it->second;
+                    lastPat = it->second;
+                    lastOffs = offs;
+                }
             }
-            if (patterned) pat[r] = (uint8_t)it->second;
         }
     }
     // rows of the last slice beyond n: padding points at row 0 (never read: threads exit)

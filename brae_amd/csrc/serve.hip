@@ -102,7 +102,10 @@ uint64_t splitmix(uint64_t& s)
 }
 
 // uniform box: internal faces by owner (+x, +y, +z neighbours), then the six walls; quads with their
-// normal out of the owner; interior points jittered by +-jitter of the spacing
+// normal out of the owner; interior points jittered by +-jitter of the spacing. Built by threads over
+// plane ranges: an interior point's jitter draws are the splitmix calls 3q, 3q+1, 3q+2 of the sequence
+// (q its rank among the interior points, the state after call m being seed + (m+1)*gamma), and each
+// thread's faces are its cells' in cell order, so the box is the same whatever the thread count.
 Box makeBox(int nx, int ny, int nz, double lx, double ly, double lz, double jitter, uint64_t seed)
 {
     Box b;
@@ -110,62 +113,104 @@ Box makeBox(int nx, int ny, int nz, double lx, double ly, double lz, double jitt
     const int px = nx + 1, py = ny + 1, pz = nz + 1;
     auto P = [&](int i, int j, int k) { return i + px * (j + py * k); };
     b.points.resize(3 * (size_t)px * py * pz);
-    uint64_t s = seed;
-    for (int k = 0; k < pz; ++k)
-        for (int j = 0; j < py; ++j)
-            for (int i = 0; i < px; ++i)
-            {
-                double x = lx * i / nx, y = ly * j / ny, z = lz * k / nz;
-                if (jitter > 0 && i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz)
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nT = (int)std::min<long long>(hw, std::max<long long>(1, (long long)b.nC / 65536));
+    auto parallel = [&](int n, const auto& body) {   // body(lo, hi) over [0, n) in nT contiguous ranges
+        if (nT <= 1)
+        {
+            body(0, n);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (int t = 0; t < nT; ++t)
+        {
+            const int lo = (int)((long long)n * t / nT), hi = (int)((long long)n * (t + 1) / nT);
+            th.emplace_back([&, lo, hi] { body(lo, hi); });
+        }
+        for (auto& x : th) x.join();
+    };
+    parallel(pz, [&](int k0, int k1) {
+        for (int k = k0; k < k1; ++k)
+            for (int j = 0; j < py; ++j)
+                for (int i = 0; i < px; ++i)
                 {
-                    auto u = [&] { return 2.0 * ((double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0)) - 1.0; };
-                    x += jitter * lx / nx * u();
-                    y += jitter * ly / ny * u();
-                    z += jitter * lz / nz * u();
+                    double x = lx * i / nx, y = ly * j / ny, z = lz * k / nz;
+                    if (jitter > 0 && i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz)
+                    {
+                        const uint64_t q = ((uint64_t)(k - 1) * (ny - 1) + (uint64_t)(j - 1)) * (nx - 1) + (i - 1);
+                        uint64_t st = seed + 3 * q * 0x9e3779b97f4a7c15ull;
+                        auto u = [&] { return 2.0 * ((double)(splitmix(st) >> 11) * (1.0 / 9007199254740992.0)) - 1.0; };
+                        x += jitter * lx / nx * u();
+                        y += jitter * ly / ny * u();
+                        z += jitter * lz / nz * u();
+                    }
+                    const size_t p = (size_t)P(i, j, k);
+                    b.points[3 * p] = x;
+                    b.points[3 * p + 1] = y;
+                    b.points[3 * p + 2] = z;
                 }
-                const size_t p = (size_t)P(i, j, k);
-                b.points[3 * p] = x;
-                b.points[3 * p + 1] = y;
-                b.points[3 * p + 2] = z;
-            }
-    auto quad = [&](int a, int bb, int c, int d) { b.faceVerts.insert(b.faceVerts.end(), {a, bb, c, d}); };
+    });
+    // internal faces in cell order, each cell's +x, +y, +z face where it has one; a cell's first face
+    // index in closed form (the faces of the cells before it, direction by direction), so the threads
+    // write their cell ranges in place
     const size_t nI = (size_t)(nx - 1) * ny * nz + (size_t)nx * (ny - 1) * nz + (size_t)nx * ny * (nz - 1);
-    b.owner.reserve(nI + 2 * ((size_t)nx * ny + (size_t)ny * nz + (size_t)nx * nz));
-    b.neighbour.reserve(nI);
-    b.faceVerts.reserve(4 * b.owner.capacity());
-    for (int c = 0; c < b.nC; ++c)
-    {
-        const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
-        if (i < nx - 1)
-        {
-            b.owner.push_back(c);
-            b.neighbour.push_back(c + 1);
-            quad(P(i + 1, j, k), P(i + 1, j + 1, k), P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1));
-        }
-        if (j < ny - 1)
-        {
-            b.owner.push_back(c);
-            b.neighbour.push_back(c + nx);
-            quad(P(i, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k));
-        }
-        if (k < nz - 1)
-        {
-            b.owner.push_back(c);
-            b.neighbour.push_back(c + nx * ny);
-            quad(P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j + 1, k + 1), P(i, j + 1, k + 1));
-        }
-    }
-    for (int side = 0; side < 6; ++side)
-        for (int c = 0; c < b.nC; ++c)
+    const size_t nWall = 2 * ((size_t)nx * ny + (size_t)ny * nz + (size_t)nx * nz);
+    b.owner.resize(nI + nWall);
+    b.neighbour.resize(nI);
+    b.faceVerts.resize(4 * (nI + nWall));
+    auto put = [&](size_t f, int own, int a, int bb, int c, int d) {
+        b.owner[f] = own;
+        int* v = b.faceVerts.data() + 4 * f;
+        v[0] = a;
+        v[1] = bb;
+        v[2] = c;
+        v[3] = d;
+    };
+    parallel(b.nC, [&](int c0, int c1) {
+        for (int c = c0; c < c1; ++c)
         {
             const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
-            if (side == 0 && i == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i, j, k + 1), P(i, j + 1, k + 1), P(i, j + 1, k)); }
-            if (side == 1 && i == nx - 1) { b.owner.push_back(c); quad(P(i + 1, j, k), P(i + 1, j + 1, k), P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1)); }
-            if (side == 2 && j == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i + 1, j, k), P(i + 1, j, k + 1), P(i, j, k + 1)); }
-            if (side == 3 && j == ny - 1) { b.owner.push_back(c); quad(P(i, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k)); }
-            if (side == 4 && k == 0) { b.owner.push_back(c); quad(P(i, j, k), P(i, j + 1, k), P(i + 1, j + 1, k), P(i + 1, j, k)); }
-            if (side == 5 && k == nz - 1) { b.owner.push_back(c); quad(P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j + 1, k + 1), P(i, j + 1, k + 1)); }
+            size_t f = ((size_t)j + (size_t)ny * k) * (nx - 1) + i;                          // +x faces before
+            f += (size_t)k * nx * (ny - 1) + (size_t)j * nx + (j < ny - 1 ? i : 0);          // +y faces before
+            f += (size_t)k * nx * ny + (k < nz - 1 ? (size_t)j * nx + i : 0);                // +z faces before
+            if (i < nx - 1)
+            {
+                b.neighbour[f] = c + 1;
+                put(f++, c, P(i + 1, j, k), P(i + 1, j + 1, k), P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1));
+            }
+            if (j < ny - 1)
+            {
+                b.neighbour[f] = c + nx;
+                put(f++, c, P(i, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k));
+            }
+            if (k < nz - 1)
+            {
+                b.neighbour[f] = c + nx * ny;
+                put(f++, c, P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j + 1, k + 1), P(i, j + 1, k + 1));
+            }
         }
+    });
+    // the six walls, each in cell order
+    auto cell = [&](int i, int j, int k) { return i + nx * (j + ny * k); };
+    size_t f = nI;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            put(f++, cell(0, j, k), P(0, j, k), P(0, j, k + 1), P(0, j + 1, k + 1), P(0, j + 1, k));
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            put(f++, cell(nx - 1, j, k), P(nx, j, k), P(nx, j + 1, k), P(nx, j + 1, k + 1), P(nx, j, k + 1));
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nx; ++i)
+            put(f++, cell(i, 0, k), P(i, 0, k), P(i + 1, 0, k), P(i + 1, 0, k + 1), P(i, 0, k + 1));
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nx; ++i)
+            put(f++, cell(i, ny - 1, k), P(i, ny, k), P(i, ny, k + 1), P(i + 1, ny, k + 1), P(i + 1, ny, k));
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            put(f++, cell(i, j, 0), P(i, j, 0), P(i, j + 1, 0), P(i + 1, j + 1, 0), P(i + 1, j, 0));
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            put(f++, cell(i, j, nz - 1), P(i, j, nz), P(i + 1, j, nz), P(i + 1, j + 1, nz), P(i, j + 1, nz));
     b.faceOffsets.resize(b.owner.size() + 1);
     for (size_t f = 0; f <= b.owner.size(); ++f) b.faceOffsets[f] = (int)(4 * f);
     return b;
@@ -547,6 +592,7 @@ int mfv_server_run(mfv_server s, const mfv_serve_request* reqs, int nReq, mfv_se
                     }
                 results[i].resident = sys ? 1 : 0;
                 if (!sys) sys = buildSystem(ctx, reqs[i], s->precision, bytes);
+                results[i].built = now(t0);
                 const bool small = s->batchCells > 0 && (double)reqs[i].nx * reqs[i].ny * reqs[i].nz <= s->batchCells;
                 results[i].batched = small ? 1 : 0;
                 solveSteps(ctx, *sys, reqs[i], s->precision, results[i], small ? &batcher : nullptr);

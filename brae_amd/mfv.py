@@ -91,7 +91,7 @@ class ServeRequest(ctypes.Structure):
 class ServeResult(ctypes.Structure):
     _fields_ = [("arrival", D), ("admitted", D), ("finished", D), ("status", I), ("nCells", I), ("nSteps", I),
                 ("totalIterations", I), ("finalResidual", D), ("fingerprint", ctypes.c_ulonglong), ("resident", I),
-                ("batched", I)]
+                ("batched", I), ("built", D)]
 
 
 class Perf(ctypes.Structure):

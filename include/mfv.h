@@ -685,6 +685,7 @@ typedef struct
     unsigned long long fingerprint;       /* of every step's solution bits */
     int resident;                         /* 1: solved on a system a slot kept from an earlier request */
     int batched;                          /* 1: its solves ran in batches (mfv_server_set_batch) */
+    double built;                         /* seconds after the run started when its system was ready */
 } mfv_serve_result;
 int mfv_server_create(int device, int nSlots, double hbmBudgetBytes, int precision, mfv_server* out);
 int mfv_server_destroy(mfv_server server);
