@@ -15,6 +15,11 @@ mixed/FP64 field differences are reported beside it.
 Inputs are synthetic and seeded: perturbed plug flow U = (10,0,0) + uniform(-1,1) per component (inlet
 fixedValue (10,0,0), k 0.375, epsilon 14.855; outlet zeroGradient with p fixedValue 0; walls no-slip with
 wall functions), p = 0, phi = interpolate(U).Sf; the timed iterations start from the seeded fields.
+
+Opt-in legs, each a key of the same JSON line: --c5 (the same iteration on the 800x400x375 channel,
+120 M cells, FP32 against FP64, with peak HBM), --batch N (N small solves at once, batched into one
+launch against the per-grid kernels), --gamg-step 1 (OpenFOAM's GAMG-PCG on the device), --profile-simple
+(only the mixed SIMPLE iterations, for a profiler).
 """
 import argparse
 import collections
