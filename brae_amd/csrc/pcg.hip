@@ -412,6 +412,36 @@ __global__ __launch_bounds__(kReduceBlock) void pcgUpdateK(int n, const double* 
     storePartial(blockSum(s, lds), partials);
 }
 
+// Level one of a two-level finish, for the outer products' tens of thousands of partials: workgroup b sums
+// (and with Max takes the maximum of) the b-th of gridDim.x contiguous ranges of them, in a fixed order, so
+// the one-workgroup finish then reads gridDim.x values instead of reading them all on one CU.
+constexpr int kLevelOneBlocks = 64;
+constexpr int kLevelOneFrom = 8192;   // partials at and above which the finish takes two levels
+template <bool Max>
+__global__ __launch_bounds__(kFinishBlock) void finishLevelOneK(int nParts, const double* __restrict__ pa,
+                                                                const double* __restrict__ pm,
+                                                                double* __restrict__ outA, double* __restrict__ outM,
+                                                                const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    __shared__ double lds[kFinishBlock / 64];
+    const int per = (nParts + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int lo = (int)blockIdx.x * per, hi = min(nParts, lo + per);
+    double sa = 0.0, m = 0.0;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += kFinishBlock)
+    {
+        sa += pa[i];
+        if (Max) m = fmax(m, pm[i]);
+    }
+    sa = blockSum<kFinishBlock>(sa, lds);
+    if (Max) m = blockMax<kFinishBlock>(m, lds);
+    if (threadIdx.x == 0)
+    {
+        outA[blockIdx.x] = sa;
+        if (Max) outM[blockIdx.x] = m;
+    }
+}
+
 // the update's finish with the FP16 cycle's max|r|
 __global__ __launch_bounds__(kFinishBlock) void updFinishK(int nParts, const double* __restrict__ pa,
                                                            const double* __restrict__ pMax, UpdPost post,
@@ -483,6 +513,10 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
     const int gRu = reduceGrid(n);
     double* pa = c.partials.p + 5 * (size_t)kMaxReduceBlocks;
     double* pb = c.partials.p + 6 * (size_t)kMaxReduceBlocks;
+    // a two-level finish's level-one sums and maxima: the top of pb's region, beyond any partition's
+    // partials (pb holds the z.r partials, at most reduceGridRows(n, 8) of them, when they are read)
+    double* l1a = c.partials.p + 7 * (size_t)kMaxReduceBlocks - 2 * kLevelOneBlocks;
+    double* l1m = l1a + kLevelOneBlocks;
     const bool seq = c.sequentialReduce;
     if (seq && w.seqA.n != (size_t)n)
     {
@@ -572,6 +606,12 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         }
         if (seq)
             seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, PqPost{d}, stopF);
+        else if (gRd >= kLevelOneFrom)
+        {
+            finishLevelOneK<false><<<kLevelOneBlocks, kFinishBlock, 0, c.stream>>>(gRd, pa, nullptr, l1a, nullptr,
+                                                                                  stopF);
+            finishK<<<1, kFinishBlock, 0, c.stream>>>(kLevelOneBlocks, l1a, PqPost{d}, stopF);
+        }
         else
             finishK<<<1, kFinishBlock, 0, c.stream>>>(gRd, pa, PqPost{d}, stopF);
         {
@@ -583,10 +623,21 @@ PcgPerf solveT(Ctx& c, const Mesh& m, Amg& a, const double* diag, const double* 
         const UpdPost upd{d, sig};
         if (half && seq)
             updSeqFinishK<<<1, 64, 0, c.stream>>>(n, sA, gRu, pMax, upd, stopF);
+        else if (half && gRu >= kLevelOneFrom)
+        {
+            finishLevelOneK<true><<<kLevelOneBlocks, kFinishBlock, 0, c.stream>>>(gRu, pa, pMax, l1a, l1m, stopF);
+            updFinishK<<<1, kFinishBlock, 0, c.stream>>>(kLevelOneBlocks, l1a, l1m, upd, stopF);
+        }
         else if (half)
             updFinishK<<<1, kFinishBlock, 0, c.stream>>>(gRu, pa, pMax, upd, stopF);
         else if (seq)
             seqFinishK<<<1, 64, 0, c.stream>>>(n, sA, upd, stopF);
+        else if (gRu >= kLevelOneFrom)
+        {
+            finishLevelOneK<false><<<kLevelOneBlocks, kFinishBlock, 0, c.stream>>>(gRu, pa, nullptr, l1a, nullptr,
+                                                                                  stopF);
+            finishK<<<1, kFinishBlock, 0, c.stream>>>(kLevelOneBlocks, l1a, upd, stopF);
+        }
         else
             finishK<<<1, kFinishBlock, 0, c.stream>>>(gRu, pa, upd, stopF);
         hipCheck(hipGetLastError(), "pcg iteration");
